@@ -1,0 +1,134 @@
+/*
+ * rfdtd -- MI355X (gfx950) FDTD engine for rfnetwork's field solver: the C ABI.
+ *
+ * Plain C types only: pointers, sizes and status codes.  Every function returns 0 on success and a
+ * non-zero RFDTD_E* code on failure; rfdtd_last_error() then holds a message for the calling
+ * thread.  No C++ exception crosses this boundary and no global solver state exists: everything
+ * lives in an rfdtd_session, and one session per stream may be used from any thread.
+ *
+ * Layout ("GPU packing" of the reference, rfnetwork/solver.py:1463-1523 with gpu=True): every one of
+ * the 30 coefficient arrays and every field array is float[Nx][Ny][Nz], C order, flat index
+ * (x*Ny + y)*Nz + z.  Cell (x,y,z) holds Ex(x,y+1,z+1), Ey(x+1,y,z+1), Ez(x+1,y+1,z), Hx(x+1,y,z),
+ * Hy(x,y+1,z) and Hz(x,y,z+1) of the global Yee grid (rfnetwork/core/src/solver.cu:134).
+ */
+#ifndef RFDTD_H
+#define RFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RFDTD_NUM_COEFFS 30
+#define RFDTD_NUM_FIELDS 18
+#define RFDTD_MAX_MONITORS 50  /* rfnetwork/core/inc/solver.h:15 */
+#define RFDTD_MAX_PROBES 5000  /* rfnetwork/core/inc/solver.h:16 */
+
+/* status codes */
+#define RFDTD_OK 0
+#define RFDTD_EINVAL 1   /* bad argument (shape, index, count, dtype) */
+#define RFDTD_EDEVICE 2  /* HIP runtime error */
+#define RFDTD_ESTATE 3   /* call out of order (e.g. run past the probe series) */
+
+/* flags for rfdtd_create / rfdtd_solver_run */
+#define RFDTD_F64 0x1u            /* fields, coefficients, probe series and monitors in double */
+#define RFDTD_COEF_RAW 0x2u       /* keep the 30 coefficient arrays as given (no deduplication) */
+#define RFDTD_COEF_TABLE 0x4u     /* force the deduplicated coefficient table */
+#define RFDTD_NO_GRAPHS 0x8u      /* launch every step eagerly instead of replaying a captured chunk */
+#define RFDTD_QUIET 0x10u         /* no progress output */
+#define RFDTD_KERNEL_TIMING 0x20u /* launch eagerly and time the E/H update kernels with HIP events */
+
+/*
+ * Coefficient order: the keys of the reference's coefficient dictionary (solver.py:1473-1518):
+ *   0 Ca_ex_y  1 Ca_ex_z  2 Cb_ex_y  3 Cb_ex_z  4 Ca_ey_z  5 Ca_ey_x  6 Cb_ey_z  7 Cb_ey_x
+ *   8 Ca_ez_x  9 Ca_ez_y 10 Cb_ez_x 11 Cb_ez_y
+ *  12 Da_hx_y 13 Da_hx_z 14 Db_hx_y1 15 Db_hx_y2 16 Db_hx_z1 17 Db_hx_z2
+ *  18 Da_hy_z 19 Da_hy_x 20 Db_hy_z1 21 Db_hy_z2 22 Db_hy_x1 23 Db_hy_x2
+ *  24 Da_hz_x 25 Da_hz_y 26 Db_hz_x1 27 Db_hz_x2 28 Db_hz_y1 29 Db_hz_y2
+ * Field order (rfdtd_read_fields): ex_y ex_z ey_z ey_x ez_x ez_y hx_y hx_z hy_z hy_x hz_x hz_y
+ *                                  ex ey ez hx hy hz
+ */
+
+/* One probe or soft source (solver.py:1541-1575; solver.cpp:465-532).  idx holds GLOBAL Yee
+ * indices; the callee applies the component shifts.  values: float (or double with RFDTD_F64)
+ * [>= Nt]; source samples in, sampled total field out, written back by rfdtd_read_probes. */
+typedef struct rfdtd_probe {
+    int32_t field;    /* 0 ex, 1 ey, 2 ez, 3 hx, 4 hy, 5 hz */
+    int32_t idx[3];
+    int32_t is_source;
+    int32_t reserved;
+    void* values;
+} rfdtd_probe;
+
+/* One plane monitor (solver.py:1597-1620; solver.cpp:351-463 with gpu=1).  position is the global
+ * field index along axis.  n_frequencies == 0: values is float[Nt/n_step+1][N1][N2] (time samples);
+ * otherwise values is complex64[n_frequencies][N1][N2] (running DFT) and dtft_phase is
+ * complex64[Nt/n_step+1][n_frequencies].  (N1,N2) = (Ny,Nz), (Nx,Nz), (Nx,Ny) for axis 0, 1, 2. */
+typedef struct rfdtd_monitor {
+    int32_t axis;
+    int32_t position;
+    int32_t field;
+    int32_t n_step;
+    int32_t n_frequencies;
+    int32_t reserved;
+    void* values;
+    const void* dtft_phase;
+} rfdtd_monitor;
+
+typedef struct rfdtd_session rfdtd_session;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rfdtd_last_error(void);
+
+/* Library/ABI version, e.g. 1. */
+int32_t rfdtd_version(void);
+
+/*
+ * One-shot solve with the semantics of the reference's GPU entry point
+ * cuda_func.solver_run_cu(coefficients, probes, monitors, mem, Nx, Ny, Nz, Nt, n_threads,
+ * update_interval) (rfnetwork/core/src/cuda_binding.cpp:20-68 -> solver.cu:546-1143): copies the 30
+ * host coefficient arrays to HBM (treating the x = Nx-1 planes of Cy, Cz and Dx as zero, the PEC
+ * wall of solver.cpp:294-328), runs Nt steps, and writes every probe series and monitor back into the
+ * caller's host arrays.  Monitors sample after both half steps, as the CPU solver does
+ * (solver.cpp:1130).  update_interval > 0 prints the reference's progress bar.
+ */
+int rfdtd_solver_run(const void* const coefficients[RFDTD_NUM_COEFFS], const rfdtd_probe* probes,
+                     int32_t n_probes, const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nx,
+                     int32_t Ny, int32_t Nz, int32_t Nt, int32_t update_interval, uint32_t flags);
+
+/* Session API: the same solve split into resident steps (fields and coefficients stay in HBM). */
+int rfdtd_create(rfdtd_session** out, int32_t Nx, int32_t Ny, int32_t Nz,
+                 const void* const coefficients[RFDTD_NUM_COEFFS], uint32_t flags);
+int rfdtd_set_probes(rfdtd_session* s, const rfdtd_probe* probes, int32_t n_probes, int32_t Nt);
+int rfdtd_set_monitors(rfdtd_session* s, const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nt);
+/* Advance n_steps time steps on stream (NULL = the session's own stream); asynchronous. */
+int rfdtd_run(rfdtd_session* s, int32_t n_steps, void* stream);
+int rfdtd_synchronize(rfdtd_session* s);
+/* Copy probe series / monitor values back into the host arrays given to set_probes / set_monitors. */
+int rfdtd_read_probes(rfdtd_session* s);
+int rfdtd_read_monitors(rfdtd_session* s);
+/* Copy the 18 fields into global Yee-shaped host arrays: ex* (Nx,Ny+1,Nz+1), ey* (Nx+1,Ny,Nz+1),
+ * ez* (Nx+1,Ny+1,Nz), hx* (Nx+1,Ny,Nz), hy* (Nx,Ny+1,Nz), hz* (Nx,Ny,Nz+1); NULL entries skipped. */
+int rfdtd_read_fields(rfdtd_session* s, void* const out[RFDTD_NUM_FIELDS]);
+/* Zero fields, probe series state and monitor accumulators; the step counter returns to 0. */
+int rfdtd_reset(rfdtd_session* s);
+/* Time steps taken since create/reset. */
+int64_t rfdtd_step(const rfdtd_session* s);
+/* Timing of the last rfdtd_run, read after rfdtd_synchronize (HIP events on the launch stream),
+ * milliseconds: the whole run, and with RFDTD_KERNEL_TIMING the average duration of one E and one
+ * H update kernel over up to 64 evenly spread steps of that run. */
+int rfdtd_last_timing(const rfdtd_session* s, double* total_ms, double* e_kernel_ms, double* h_kernel_ms);
+/* Device bytes held by the session, and coefficient storage kind (0 raw arrays, 1 table). */
+int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* coef_kind,
+                      int64_t* n_e_entries, int64_t* n_h_entries);
+/* Launch shape: threads per block (64..1024, multiple of 64) and x planes per block (>= 1);
+ * 0 keeps the current value. */
+int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
+void rfdtd_destroy(rfdtd_session* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RFDTD_H */

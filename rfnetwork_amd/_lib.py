@@ -1,0 +1,151 @@
+"""ctypes binding of librfdtd.so (the HIP engine's C ABI, include/rfdtd.h).
+
+The library is built in-tree by ``rfnetwork_amd.build`` (``python -m rfnetwork_amd.build``).  There
+is no fallback: if the library is missing or fails to load, ``lib()`` raises, so no solve can
+silently run on anything but the HIP kernels.
+"""
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "librfdtd.so"
+
+NUM_COEFFS = 30
+NUM_FIELDS = 18
+MAX_MONITORS = 50
+MAX_PROBES = 5000
+
+F64 = 0x1
+COEF_RAW = 0x2
+COEF_TABLE = 0x4
+NO_GRAPHS = 0x8
+QUIET = 0x10
+KERNEL_TIMING = 0x20
+
+# Coefficient keys in the order of the C ABI (rfnetwork/solver.py:1473-1518).
+COEFF_NAMES = (
+    "Ca_ex_y", "Ca_ex_z", "Cb_ex_y", "Cb_ex_z",
+    "Ca_ey_z", "Ca_ey_x", "Cb_ey_z", "Cb_ey_x",
+    "Ca_ez_x", "Ca_ez_y", "Cb_ez_x", "Cb_ez_y",
+    "Da_hx_y", "Da_hx_z", "Db_hx_y1", "Db_hx_y2", "Db_hx_z1", "Db_hx_z2",
+    "Da_hy_z", "Da_hy_x", "Db_hy_z1", "Db_hy_z2", "Db_hy_x1", "Db_hy_x2",
+    "Da_hz_x", "Da_hz_y", "Db_hz_x1", "Db_hz_x2", "Db_hz_y1", "Db_hz_y2",
+)
+FIELD_NAMES = (
+    "ex_y", "ex_z", "ey_z", "ey_x", "ez_x", "ez_y",
+    "hx_y", "hx_z", "hy_z", "hy_x", "hz_x", "hz_y",
+    "ex", "ey", "ez", "hx", "hy", "hz",
+)
+# component (0 ex .. 5 hz) of each field array
+FIELD_COMPONENT = (0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 0, 1, 2, 3, 4, 5)
+
+# the C entry points declared in include/rfdtd.h
+EXPORTS = (
+    "rfdtd_last_error", "rfdtd_version", "rfdtd_solver_run", "rfdtd_create", "rfdtd_set_probes",
+    "rfdtd_set_monitors", "rfdtd_run", "rfdtd_synchronize", "rfdtd_read_probes", "rfdtd_read_monitors",
+    "rfdtd_read_fields", "rfdtd_reset", "rfdtd_step", "rfdtd_last_timing", "rfdtd_memory_info",
+    "rfdtd_set_launch", "rfdtd_destroy",
+)
+
+
+class Probe(ctypes.Structure):
+    _fields_ = [
+        ("field", ctypes.c_int32),
+        ("idx", ctypes.c_int32 * 3),
+        ("is_source", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("values", ctypes.c_void_p),
+    ]
+
+
+class Monitor(ctypes.Structure):
+    _fields_ = [
+        ("axis", ctypes.c_int32),
+        ("position", ctypes.c_int32),
+        ("field", ctypes.c_int32),
+        ("n_step", ctypes.c_int32),
+        ("n_frequencies", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("values", ctypes.c_void_p),
+        ("dtft_phase", ctypes.c_void_p),
+    ]
+
+
+class RfdtdError(RuntimeError):
+    """A failed call into librfdtd (device error or invalid state)."""
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load librfdtd.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m rfnetwork_amd.build` (hipcc --offload-arch=gfx950)"
+        )
+    L = ctypes.CDLL(str(LIB_PATH), mode=ctypes.RTLD_GLOBAL)
+    vp, i32, u32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64
+    L.rfdtd_last_error.restype = ctypes.c_char_p
+    L.rfdtd_version.restype = i32
+    L.rfdtd_solver_run.argtypes = [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, u32]
+    L.rfdtd_create.argtypes = [ctypes.POINTER(vp), i32, i32, i32, vp, u32]
+    L.rfdtd_set_probes.argtypes = [vp, vp, i32, i32]
+    L.rfdtd_set_monitors.argtypes = [vp, vp, i32, i32]
+    L.rfdtd_run.argtypes = [vp, i32, vp]
+    L.rfdtd_synchronize.argtypes = [vp]
+    L.rfdtd_read_probes.argtypes = [vp]
+    L.rfdtd_read_monitors.argtypes = [vp]
+    L.rfdtd_read_fields.argtypes = [vp, vp]
+    L.rfdtd_reset.argtypes = [vp]
+    L.rfdtd_step.argtypes = [vp]
+    L.rfdtd_step.restype = i64
+    L.rfdtd_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]
+    L.rfdtd_memory_info.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(i32),
+                                    ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.rfdtd_set_launch.argtypes = [vp, i32, i32, i32]
+    L.rfdtd_destroy.argtypes = [vp]
+    L.rfdtd_destroy.restype = None
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    """Raise on a non-zero status: ValueError for invalid arguments, RfdtdError otherwise."""
+    if rc == 0:
+        return
+    msg = lib().rfdtd_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == 1:
+        raise ValueError(text)
+    raise RfdtdError(text)
+
+
+def pointer_array(arrays) -> "ctypes.Array":
+    """void*[n] holding the data pointers of numpy arrays (None -> NULL)."""
+    arr = (ctypes.c_void_p * len(arrays))()
+    for k, a in enumerate(arrays):
+        arr[k] = None if a is None else a.ctypes.data
+    return arr
+
+
+def gpu_visible() -> bool:
+    """True when HIP reports at least one device (does not initialise torch)."""
+    if os.environ.get("HIP_VISIBLE_DEVICES", None) == "":
+        return False
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        n = ctypes.c_int(0)
+        return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False
+
+
+def as_c(a: np.ndarray, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)

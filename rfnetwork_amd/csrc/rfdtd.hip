@@ -1,0 +1,1122 @@
+// rfdtd: FDTD time stepping for rfnetwork's field solver on MI355X (gfx950), behind the C ABI of
+// include/rfdtd.h.
+//
+// Replaces the reference's GPU entry point (rfnetwork/core/src/cuda_binding.cpp:20-68 ->
+// rfnetwork/core/src/solver.cu:546-1143) and follows the update semantics of the reference's CPU
+// solver (rfnetwork/core/src/solver.cpp:848-1163):
+//   * E half step: six split updates s = Ca*s + Cb*(h+ - h) with the upper-edge guards of the
+//     GPU packing, then totals (solver.cu:152-190, solver.cpp:851-960);
+//   * E soft sources / probes, in list order per cell (solver.cpp:963-976);
+//   * H half step: s = Da*s + (Db2*e - Db1*e-) -- the CPU association -- then totals
+//     (solver.cpp:992-1101);
+//   * H probes (solver.cpp:1104-1116);
+//   * monitors sampled after both half steps (solver.cpp:1130-1161): plane copies, or running DFT
+//     values[f] += field * phase[m, f] in complex arithmetic (solver.cpp:1194-1225).
+// All device code is compiled with -ffp-contract=off, so each product and sum is rounded on its
+// own, exactly as in the reference's x86 build: for the same inputs the float results equal the CPU
+// solver's bit for bit.
+//
+// Design for gfx950 (not a translation of the CUDA kernels):
+//   * one thread owns a (y,z) column of a block of x planes and marches up in x; the x+1 (E update)
+//     and x-1 (H update) neighbours are carried in registers, the y/z neighbours come from the
+//     lanes of the same or the next wavefront (L1/L2 hits), so each array is read from HBM once;
+//   * lanes run along z, the contiguous axis: every load and store is a coalesced 256-byte row;
+//   * workgroups are renumbered so that consecutive (y,z) tiles land on the same XCD and share its
+//     L2 for the y-neighbour rows;
+//   * the 30 coefficient arrays are deduplicated on the host, bit-exactly, into per-cell 32-bit
+//     entry indices plus small tables of distinct E (12 values) and H (18 values) tuples; interior
+//     vacuum, dielectric and PML cells collapse to a few thousand entries, which stay in L2, so a
+//     step moves 8 B of coefficient index per cell instead of 120 B;
+//   * a chunk of steps is captured once into a hipGraph and replayed; the kernels read the step
+//     number from a device word set before each replay.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rfdtd.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct RfdtdError {
+    int code;
+    std::string msg;
+};
+
+#define RF_THROW(code, msg) throw RfdtdError{code, (msg)}
+#define HIP_OK(expr)                                                                                         \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            throw RfdtdError{RFDTD_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)};              \
+    } while (0)
+
+enum { EX = 0, EY = 1, EZ = 2, HX = 3, HY = 4, HZ = 5 };
+
+// ------------------------------------------------------------------------------------------------
+// device side
+// ------------------------------------------------------------------------------------------------
+
+template <typename T>
+struct Fields {
+    T *ex_y, *ex_z, *ey_z, *ey_x, *ez_x, *ez_y;
+    T *hx_y, *hx_z, *hy_z, *hy_x, *hz_x, *hz_y;
+    T *ex, *ey, *ez, *hx, *hy, *hz;
+};
+
+template <typename T>
+struct alignas(4 * sizeof(T)) Vec4 {
+    T v[4];
+};
+
+// E coefficients of one cell, order of the reference dictionary entries 0..11.
+template <typename T>
+struct ECoef {
+    T ca_xy, ca_xz, cb_xy, cb_xz, ca_yz, ca_yx, cb_yz, cb_yx, ca_zx, ca_zy, cb_zx, cb_zy;
+};
+// H coefficients of one cell, entries 12..29.
+template <typename T>
+struct HCoef {
+    T da_xy, da_xz, db_xy1, db_xy2, db_xz1, db_xz2;
+    T da_yz, da_yx, db_yz1, db_yz2, db_yx1, db_yx2;
+    T da_zx, da_zy, db_zx1, db_zx2, db_zy1, db_zy2;
+};
+
+// The 30 arrays as given (one coalesced load per coefficient).
+template <typename T>
+struct CoefRaw {
+    const T* c[30];
+    __device__ __forceinline__ ECoef<T> e(int64_t i) const {
+        ECoef<T> r;
+        r.ca_xy = c[0][i]; r.ca_xz = c[1][i]; r.cb_xy = c[2][i]; r.cb_xz = c[3][i];
+        r.ca_yz = c[4][i]; r.ca_yx = c[5][i]; r.cb_yz = c[6][i]; r.cb_yx = c[7][i];
+        r.ca_zx = c[8][i]; r.ca_zy = c[9][i]; r.cb_zx = c[10][i]; r.cb_zy = c[11][i];
+        return r;
+    }
+    __device__ __forceinline__ HCoef<T> h(int64_t i) const {
+        HCoef<T> r;
+        r.da_xy = c[12][i]; r.da_xz = c[13][i]; r.db_xy1 = c[14][i]; r.db_xy2 = c[15][i];
+        r.db_xz1 = c[16][i]; r.db_xz2 = c[17][i];
+        r.da_yz = c[18][i]; r.da_yx = c[19][i]; r.db_yz1 = c[20][i]; r.db_yz2 = c[21][i];
+        r.db_yx1 = c[22][i]; r.db_yx2 = c[23][i];
+        r.da_zx = c[24][i]; r.da_zy = c[25][i]; r.db_zx1 = c[26][i]; r.db_zx2 = c[27][i];
+        r.db_zy1 = c[28][i]; r.db_zy2 = c[29][i];
+        return r;
+    }
+};
+
+// Deduplicated coefficients: per-cell entry index into tables of distinct tuples.  E entries are
+// 12 values (3 x Vec4), H entries 18 values padded to 20 (5 x Vec4).
+template <typename T>
+struct CoefTable {
+    const uint32_t* eidx;
+    const uint32_t* hidx;
+    const Vec4<T>* etab;
+    const Vec4<T>* htab;
+    __device__ __forceinline__ ECoef<T> e(int64_t i) const {
+        const Vec4<T>* t = etab + (size_t)eidx[i] * 3;
+        const Vec4<T> a = t[0], b = t[1], c = t[2];
+        ECoef<T> r;
+        r.ca_xy = a.v[0]; r.ca_xz = a.v[1]; r.cb_xy = a.v[2]; r.cb_xz = a.v[3];
+        r.ca_yz = b.v[0]; r.ca_yx = b.v[1]; r.cb_yz = b.v[2]; r.cb_yx = b.v[3];
+        r.ca_zx = c.v[0]; r.ca_zy = c.v[1]; r.cb_zx = c.v[2]; r.cb_zy = c.v[3];
+        return r;
+    }
+    __device__ __forceinline__ HCoef<T> h(int64_t i) const {
+        const Vec4<T>* t = htab + (size_t)hidx[i] * 5;
+        const Vec4<T> a = t[0], b = t[1], c = t[2], d = t[3], e = t[4];
+        HCoef<T> r;
+        r.da_xy = a.v[0]; r.da_xz = a.v[1]; r.db_xy1 = a.v[2]; r.db_xy2 = a.v[3];
+        r.db_xz1 = b.v[0]; r.db_xz2 = b.v[1];
+        r.da_yz = b.v[2]; r.da_yx = b.v[3]; r.db_yz1 = c.v[0]; r.db_yz2 = c.v[1];
+        r.db_yx1 = c.v[2]; r.db_yx2 = c.v[3];
+        r.da_zx = d.v[0]; r.da_zy = d.v[1]; r.db_zx1 = d.v[2]; r.db_zx2 = d.v[3];
+        r.db_zy1 = e.v[0]; r.db_zy2 = e.v[1];
+        return r;
+    }
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (id -> XCD id % 8).  Renumber them so that
+// each XCD works through one contiguous range of logical tiles; the launcher pads the grid to a
+// multiple of 8 and surplus ids exit.
+__device__ __forceinline__ int xcd_logical(int bid, int nblocks) {
+    const int per = nblocks >> 3;
+    return (bid & 7) * per + (bid >> 3);
+}
+
+struct Tile {
+    int64_t p;  // in-plane offset y*Nz + z
+    int y, z, x0, x1;
+};
+
+__device__ __forceinline__ bool tile_of(Tile& t, int Nx, int Ny, int Nz, int xplanes, int npb, int nlog) {
+    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    if (L >= nlog) return false;
+    const int pb = L % npb, xc = L / npb;
+    const int64_t NyNz = (int64_t)Ny * Nz;
+    t.p = (int64_t)pb * blockDim.x + threadIdx.x;
+    if (t.p >= NyNz) return false;
+    t.y = (int)(t.p / Nz);
+    t.z = (int)(t.p - (int64_t)t.y * Nz);
+    t.x0 = xc * xplanes;
+    t.x1 = min(Nx, t.x0 + xplanes);
+    return true;
+}
+
+// E half step over one column of x planes.
+template <typename T, class CA>
+__global__ void __launch_bounds__(1024) k_update_e(Fields<T> F, CA C, int Nx, int Ny, int Nz, int xplanes, int npb,
+                                                   int nlog) {
+    Tile t;
+    if (!tile_of(t, Nx, Ny, Nz, xplanes, npb, nlog)) return;
+    const int64_t NyNz = (int64_t)Ny * Nz;
+    const bool yin = t.y < Ny - 1, zin = t.z < Nz - 1;
+    int64_t i = (int64_t)t.x0 * NyNz + t.p;
+    T hy_c = F.hy[i], hz_c = F.hz[i];
+    for (int x = t.x0; x < t.x1; ++x, i += NyNz) {
+        const bool xin = x < Nx - 1;
+        const T hx_c = F.hx[i];
+        const T hy_xn = xin ? F.hy[i + NyNz] : T(0);
+        const T hz_xn = xin ? F.hz[i + NyNz] : T(0);
+        const T hz_yn = yin ? F.hz[i + Nz] : T(0);
+        const T hx_yn = yin ? F.hx[i + Nz] : T(0);
+        const T hy_zn = zin ? F.hy[i + 1] : T(0);
+        const T hx_zn = zin ? F.hx[i + 1] : T(0);
+        const ECoef<T> c = C.e(i);
+        T exy = F.ex_y[i], exz = F.ex_z[i], eyz = F.ey_z[i], eyx = F.ey_x[i], ezx = F.ez_x[i], ezy = F.ez_y[i];
+        if (yin) exy = c.ca_xy * exy + c.cb_xy * (hz_yn - hz_c);
+        if (zin) exz = c.ca_xz * exz + c.cb_xz * (hy_zn - hy_c);
+        if (zin) eyz = c.ca_yz * eyz + c.cb_yz * (hx_zn - hx_c);
+        if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (hz_xn - hz_c);
+        if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (hy_xn - hy_c);
+        if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (hx_yn - hx_c);
+        F.ex_y[i] = exy; F.ex_z[i] = exz; F.ey_z[i] = eyz; F.ey_x[i] = eyx; F.ez_x[i] = ezx; F.ez_y[i] = ezy;
+        F.ex[i] = exy + exz;
+        F.ey[i] = eyz + eyx;
+        F.ez[i] = ezx + ezy;
+        hy_c = hy_xn;
+        hz_c = hz_xn;
+    }
+}
+
+// H half step over one column of x planes; neighbours below index 0 are zero.
+template <typename T, class CA>
+__global__ void __launch_bounds__(1024) k_update_h(Fields<T> F, CA C, int Nx, int Ny, int Nz, int xplanes, int npb,
+                                                   int nlog) {
+    Tile t;
+    if (!tile_of(t, Nx, Ny, Nz, xplanes, npb, nlog)) return;
+    const int64_t NyNz = (int64_t)Ny * Nz;
+    const bool ylo = t.y > 0, zlo = t.z > 0;
+    int64_t i = (int64_t)t.x0 * NyNz + t.p;
+    T ey_xm = t.x0 > 0 ? F.ey[i - NyNz] : T(0);
+    T ez_xm = t.x0 > 0 ? F.ez[i - NyNz] : T(0);
+    for (int x = t.x0; x < t.x1; ++x, i += NyNz) {
+        const T ex_c = F.ex[i], ey_c = F.ey[i], ez_c = F.ez[i];
+        const T ez_ym = ylo ? F.ez[i - Nz] : T(0);
+        const T ex_ym = ylo ? F.ex[i - Nz] : T(0);
+        const T ey_zm = zlo ? F.ey[i - 1] : T(0);
+        const T ex_zm = zlo ? F.ex[i - 1] : T(0);
+        const HCoef<T> c = C.h(i);
+        const T hxy = c.da_xy * F.hx_y[i] + (c.db_xy2 * ez_c - c.db_xy1 * ez_ym);
+        const T hxz = c.da_xz * F.hx_z[i] + (c.db_xz2 * ey_c - c.db_xz1 * ey_zm);
+        const T hyz = c.da_yz * F.hy_z[i] + (c.db_yz2 * ex_c - c.db_yz1 * ex_zm);
+        const T hyx = c.da_yx * F.hy_x[i] + (c.db_yx2 * ez_c - c.db_yx1 * ez_xm);
+        const T hzx = c.da_zx * F.hz_x[i] + (c.db_zx2 * ey_c - c.db_zx1 * ey_xm);
+        const T hzy = c.da_zy * F.hz_y[i] + (c.db_zy2 * ex_c - c.db_zy1 * ex_ym);
+        F.hx_y[i] = hxy; F.hx_z[i] = hxz; F.hy_z[i] = hyz; F.hy_x[i] = hyx; F.hz_x[i] = hzx; F.hz_y[i] = hzy;
+        F.hx[i] = hxy + hxz;
+        F.hy[i] = hyz + hyx;
+        F.hz[i] = hzx + hzy;
+        ey_xm = ey_c;
+        ez_xm = ez_c;
+    }
+}
+
+// Probes sharing one (component, cell) form a group processed by one thread in list order.
+struct ProbeGroup {
+    int64_t cell;
+    int32_t field;
+    int32_t start;
+    int32_t count;
+    int32_t pad;
+};
+
+__device__ __forceinline__ int step_of(const int* base, int local) { return base ? (*base + local) : local; }
+
+template <typename T>
+__global__ void k_probes(Fields<T> F, const ProbeGroup* groups, int ngroups, const int32_t* order,
+                         const int32_t* is_source, T* series, int nprobes, const int* base, int local) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const int n = step_of(base, local);
+    const ProbeGroup G = groups[g];
+    T *tot, *s1, *s2;
+    switch (G.field) {
+        case EX: tot = F.ex; s1 = F.ex_y; s2 = F.ex_z; break;
+        case EY: tot = F.ey; s1 = F.ey_z; s2 = F.ey_x; break;
+        case EZ: tot = F.ez; s1 = F.ez_x; s2 = F.ez_y; break;
+        case HX: tot = F.hx; s1 = F.hx_y; s2 = F.hx_z; break;
+        case HY: tot = F.hy; s1 = F.hy_z; s2 = F.hy_x; break;
+        default: tot = F.hz; s1 = F.hz_x; s2 = F.hz_y; break;
+    }
+    T a = s1[G.cell], b = s2[G.cell], s = tot[G.cell];
+    T* row = series + (size_t)n * nprobes;
+    for (int k = 0; k < G.count; ++k) {
+        const int pid = order[G.start + k];
+        const T v = row[pid];
+        if (is_source[pid]) {
+            a = a + v;
+            b = b + v;
+            s = a + b;
+        }
+        row[pid] = s;
+    }
+    s1[G.cell] = a;
+    s2[G.cell] = b;
+    tot[G.cell] = s;
+}
+
+struct MonitorDev {
+    int32_t axis, pos, field, n_step, nf, N1, N2, blk0;
+    int64_t N1N2;
+    void* values;
+    const void* phase;
+};
+
+template <typename T>
+__global__ void k_monitors(Fields<T> F, const MonitorDev* mons, int nmon, int Nx, int Ny, int Nz, const int* base,
+                           int local) {
+    __shared__ int sm;
+    if (threadIdx.x == 0) {
+        int m = 0;
+        while (m + 1 < nmon && mons[m + 1].blk0 <= (int)blockIdx.x) ++m;
+        sm = m;
+    }
+    __syncthreads();
+    const MonitorDev M = mons[sm];
+    const int n = step_of(base, local);
+    if (n > 0 && (n % M.n_step) != 0) return;
+    const int64_t q = (int64_t)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
+    if (q >= M.N1N2) return;
+    const int a = (int)(q / M.N2), b = (int)(q - (int64_t)a * M.N2);
+    int64_t idx;
+    if (M.axis == 0)
+        idx = ((int64_t)M.pos * Ny + a) * Nz + b;
+    else if (M.axis == 1)
+        idx = ((int64_t)a * Ny + M.pos) * Nz + b;
+    else
+        idx = ((int64_t)a * Ny + b) * Nz + M.pos;
+    const T* src;
+    switch (M.field) {
+        case EX: src = F.ex; break;
+        case EY: src = F.ey; break;
+        case EZ: src = F.ez; break;
+        case HX: src = F.hx; break;
+        case HY: src = F.hy; break;
+        default: src = F.hz; break;
+    }
+    const T v = src[idx];
+    const int mn = n / M.n_step;
+    if (M.nf == 0) {
+        ((T*)M.values)[(int64_t)mn * M.N1N2 + q] = v;
+    } else {
+        // values[f] += v * phase[mn, f]: a real field times a complex phase, one product and one sum
+        // per component.
+        const T* ph = (const T*)M.phase + (size_t)mn * M.nf * 2;
+        T* acc = (T*)M.values;
+        for (int f = 0; f < M.nf; ++f) {
+            T* c = acc + 2 * ((int64_t)f * M.N1N2 + q);
+            const T re = c[0] + v * ph[2 * f];
+            const T im = c[1] + v * ph[2 * f + 1];
+            c[0] = re;
+            c[1] = im;
+        }
+    }
+}
+
+__global__ void k_set_step(int* p, int v) { *p = v; }
+
+// ------------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------------
+
+struct MonitorHost {
+    rfdtd_monitor desc;
+    bool active;
+    int N1, N2, Nm, shifted;
+    void* d_values;
+    void* d_phase;
+};
+
+struct Session {
+    int Nx = 0, Ny = 0, Nz = 0;
+    int64_t cells = 0;
+    bool f64 = false;
+    size_t esz = 4;
+    uint32_t flags = 0;
+
+    hipStream_t own_stream = nullptr;
+    void* d_fields = nullptr;  // 18 arrays of cells
+    void* d_coef = nullptr;    // raw arrays or index + tables
+    size_t fields_bytes = 0, coef_bytes = 0;
+    int coef_kind = 0;  // 0 raw, 1 table
+    int64_t nE = 0, nH = 0;
+    const void* raw[30] = {};
+    const uint32_t* eidx = nullptr;
+    const uint32_t* hidx = nullptr;
+    const void* etab = nullptr;
+    const void* htab = nullptr;
+
+    // probes
+    int P = 0, Nt_probes = 0;
+    void* d_series = nullptr;
+    int32_t* d_order = nullptr;
+    int32_t* d_is_src = nullptr;
+    ProbeGroup* d_groups = nullptr;
+    int ngE = 0, ngH = 0;  // groups [0, ngE) are E, [ngE, ngE+ngH) are H
+    std::vector<void*> probe_host_values;
+
+    // monitors
+    std::vector<MonitorHost> mons;
+    MonitorDev* d_mons = nullptr;
+    int n_active = 0, mon_blocks = 0;
+    int Nt_monitors = 0;
+
+    int64_t step = 0;
+    int block = 256, xplanes = 0, graph_steps = 100;
+
+    int* d_step_base = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    bool graph_dirty = true;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_total_ms = 0, last_e_ms = 0, last_h_ms = 0;
+    static constexpr int kTimedSteps = 64;
+    std::vector<hipEvent_t> kev;  // 4 per timed step: E start/end, H start/end
+    int n_timed = 0;
+
+    ~Session();
+    void release_probes();
+    void release_monitors();
+    void drop_graph();
+    size_t field_pitch() const { return ((size_t)cells * esz + 255) & ~(size_t)255; }
+    template <typename T>
+    Fields<T> fields() const {
+        Fields<T> f;
+        T** a = (T**)&f;
+        for (int k = 0; k < 18; ++k) a[k] = (T*)((char*)d_fields + (size_t)k * field_pitch());
+        return f;
+    }
+};
+
+void Session::drop_graph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_def) (void)hipGraphDestroy(graph_def);
+    graph = nullptr;
+    graph_def = nullptr;
+    graph_dirty = true;
+}
+
+void Session::release_probes() {
+    if (d_series) (void)hipFree(d_series);
+    if (d_order) (void)hipFree(d_order);
+    if (d_is_src) (void)hipFree(d_is_src);
+    if (d_groups) (void)hipFree(d_groups);
+    d_series = nullptr; d_order = nullptr; d_is_src = nullptr; d_groups = nullptr;
+    P = 0; ngE = ngH = 0; Nt_probes = 0;
+    probe_host_values.clear();
+}
+
+void Session::release_monitors() {
+    for (auto& m : mons) {
+        if (m.d_values) (void)hipFree(m.d_values);
+        if (m.d_phase) (void)hipFree(m.d_phase);
+    }
+    mons.clear();
+    if (d_mons) (void)hipFree(d_mons);
+    d_mons = nullptr;
+    n_active = mon_blocks = 0;
+}
+
+Session::~Session() {
+    drop_graph();
+    release_probes();
+    release_monitors();
+    if (d_fields) (void)hipFree(d_fields);
+    if (d_coef) (void)hipFree(d_coef);
+    if (d_step_base) (void)hipFree(d_step_base);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    for (auto e : kev) (void)hipEventDestroy(e);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+// ---- bit-exact deduplication of coefficient tuples ----
+
+inline uint64_t mix64(uint64_t h) {
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdULL;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ULL;
+    h ^= h >> 33;
+    return h;
+}
+
+// Collect the distinct K-tuples (compared bit for bit) of arrays[0..K) over all cells.  zero_last_x
+// marks arrays whose x = Nx-1 plane reads as zero.  Gives up (returns false) once more than
+// max_entries distinct tuples exist.
+template <typename U, int K>
+bool dedup(const U* const* arrays, const bool* zero_last_x, int64_t cells, int64_t plane, int64_t max_entries,
+           std::vector<uint32_t>& idx, std::vector<U>& table) {
+    idx.resize(cells);
+    table.clear();
+    size_t cap = 1 << 12;
+    std::vector<int32_t> slots(cap, -1);
+    int64_t count = 0;
+    const int64_t last0 = cells - plane;
+    U key[K];
+    for (int64_t i = 0; i < cells; ++i) {
+        const bool last = i >= last0;
+        uint64_t h = 0x9e3779b97f4a7c15ULL;
+        for (int k = 0; k < K; ++k) {
+            key[k] = (last && zero_last_x[k]) ? U(0) : arrays[k][i];
+            h = mix64(h ^ (uint64_t)key[k]) + (uint64_t)k;
+        }
+        size_t s = h & (cap - 1);
+        int32_t found = -1;
+        while (slots[s] >= 0) {
+            const U* e = &table[(size_t)slots[s] * K];
+            bool eq = true;
+            for (int k = 0; k < K; ++k)
+                if (e[k] != key[k]) { eq = false; break; }
+            if (eq) { found = slots[s]; break; }
+            s = (s + 1) & (cap - 1);
+        }
+        if (found < 0) {
+            if (count >= max_entries) return false;
+            found = (int32_t)count++;
+            table.insert(table.end(), key, key + K);
+            slots[s] = found;
+            if ((size_t)count * 2 > cap) {  // grow and rehash
+                cap <<= 1;
+                std::vector<int32_t> ns(cap, -1);
+                for (int64_t e = 0; e < count; ++e) {
+                    uint64_t hh = 0x9e3779b97f4a7c15ULL;
+                    for (int k = 0; k < K; ++k) hh = mix64(hh ^ (uint64_t)table[(size_t)e * K + k]) + (uint64_t)k;
+                    size_t t = hh & (cap - 1);
+                    while (ns[t] >= 0) t = (t + 1) & (cap - 1);
+                    ns[t] = (int32_t)e;
+                }
+                slots.swap(ns);
+            }
+        }
+        idx[i] = (uint32_t)found;
+    }
+    return true;
+}
+
+// which of the 30 arrays read as zero on the x = Nx-1 plane (solver.cpp:294-328: Cy, Cz, Dx)
+bool zero_last_plane(int k) { return (k >= 4 && k <= 11) || (k >= 12 && k <= 17); }
+
+template <typename T>
+void upload_coefficients(Session& s, const void* const coeff[30]) {
+    const int64_t cells = s.cells, plane = (int64_t)s.Ny * s.Nz;
+    using U = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+    const U* arr[30];
+    bool zl[30];
+    for (int k = 0; k < 30; ++k) {
+        if (!coeff[k]) RF_THROW(RFDTD_EINVAL, "missing coefficient array " + std::to_string(k));
+        arr[k] = (const U*)coeff[k];
+        zl[k] = zero_last_plane(k);
+    }
+    bool use_table = false;
+    std::vector<uint32_t> eidx, hidx;
+    std::vector<U> etab, htab;
+    if (!(s.flags & RFDTD_COEF_RAW)) {
+        // A table is worth it while its entries cost well under the raw arrays.
+        const int64_t limit = (s.flags & RFDTD_COEF_TABLE) ? cells : std::max<int64_t>(4096, cells / 16);
+        use_table = dedup<U, 12>(arr, zl, cells, plane, limit, eidx, etab) &&
+                    dedup<U, 18>(arr + 12, zl + 12, cells, plane, limit, hidx, htab);
+        if (!use_table && (s.flags & RFDTD_COEF_TABLE))
+            RF_THROW(RFDTD_EINVAL, "coefficient table exceeds its entry limit");
+    }
+    if (use_table) {
+        s.coef_kind = 1;
+        s.nE = (int64_t)etab.size() / 12;
+        s.nH = (int64_t)htab.size() / 18;
+        const size_t idx_bytes = (size_t)cells * 4;
+        const size_t etab_bytes = (size_t)s.nE * 12 * sizeof(T);
+        const size_t htab_bytes = (size_t)s.nH * 20 * sizeof(T);
+        auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        s.coef_bytes = 2 * al(idx_bytes) + al(etab_bytes) + al(htab_bytes);
+        HIP_OK(hipMalloc(&s.d_coef, s.coef_bytes));
+        char* b = (char*)s.d_coef;
+        s.eidx = (const uint32_t*)b;
+        s.hidx = (const uint32_t*)(b + al(idx_bytes));
+        s.etab = b + 2 * al(idx_bytes);
+        s.htab = b + 2 * al(idx_bytes) + al(etab_bytes);
+        HIP_OK(hipMemcpy((void*)s.eidx, eidx.data(), idx_bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy((void*)s.hidx, hidx.data(), idx_bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy((void*)s.etab, etab.data(), etab_bytes, hipMemcpyHostToDevice));
+        std::vector<U> hpad((size_t)s.nH * 20, U(0));
+        for (int64_t e = 0; e < s.nH; ++e)
+            std::memcpy(&hpad[(size_t)e * 20], &htab[(size_t)e * 18], 18 * sizeof(U));
+        HIP_OK(hipMemcpy((void*)s.htab, hpad.data(), htab_bytes, hipMemcpyHostToDevice));
+    } else {
+        s.coef_kind = 0;
+        const size_t arr_bytes = ((size_t)cells * sizeof(T) + 255) & ~(size_t)255;
+        s.coef_bytes = 30 * arr_bytes;
+        HIP_OK(hipMalloc(&s.d_coef, s.coef_bytes));
+        for (int k = 0; k < 30; ++k) {
+            char* dst = (char*)s.d_coef + (size_t)k * arr_bytes;
+            s.raw[k] = dst;
+            HIP_OK(hipMemcpy(dst, coeff[k], (size_t)cells * sizeof(T), hipMemcpyHostToDevice));
+            if (zl[k])
+                HIP_OK(hipMemset(dst + (size_t)(cells - plane) * sizeof(T), 0, (size_t)plane * sizeof(T)));
+        }
+    }
+}
+
+void choose_launch(Session& s) {
+    // x planes per workgroup: enough workgroups to give every CU several waves (>= 2048 groups of
+    // 256 threads on 256 CUs), as many planes as possible for register reuse along x.
+    if (s.xplanes > 0) return;
+    const int64_t NyNz = (int64_t)s.Ny * s.Nz;
+    const int64_t npb = (NyNz + s.block - 1) / s.block;
+    const int64_t want = 2048;
+    int64_t chunks = std::max<int64_t>(1, (want + npb - 1) / npb);
+    chunks = std::min<int64_t>(chunks, s.Nx);
+    s.xplanes = (int)((s.Nx + chunks - 1) / chunks);
+}
+
+struct Launch {
+    int grid, npb, nlog;
+};
+
+Launch launch_shape(const Session& s) {
+    const int64_t NyNz = (int64_t)s.Ny * s.Nz;
+    Launch L;
+    L.npb = (int)((NyNz + s.block - 1) / s.block);
+    const int chunks = (s.Nx + s.xplanes - 1) / s.xplanes;
+    L.nlog = L.npb * chunks;
+    L.grid = (L.nlog + 7) & ~7;
+    return L;
+}
+
+template <typename T>
+void launch_step(Session& s, hipStream_t st, const int* base, int local, bool monitors_now,
+                 hipEvent_t* ev = nullptr) {
+    const Launch L = launch_shape(s);
+    const Fields<T> F = s.fields<T>();
+    if (ev) HIP_OK(hipEventRecord(ev[0], st));
+    if (s.coef_kind == 1) {
+        CoefTable<T> C{s.eidx, s.hidx, (const Vec4<T>*)s.etab, (const Vec4<T>*)s.htab};
+        k_update_e<T, CoefTable<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
+    } else {
+        CoefRaw<T> C;
+        for (int k = 0; k < 30; ++k) C.c[k] = (const T*)s.raw[k];
+        k_update_e<T, CoefRaw<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
+    }
+    if (ev) HIP_OK(hipEventRecord(ev[1], st));
+    if (s.ngE)
+        k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(F, s.d_groups, s.ngE, s.d_order, s.d_is_src, (T*)s.d_series,
+                                                      s.P, base, local);
+    if (ev) HIP_OK(hipEventRecord(ev[2], st));
+    if (s.coef_kind == 1) {
+        CoefTable<T> C{s.eidx, s.hidx, (const Vec4<T>*)s.etab, (const Vec4<T>*)s.htab};
+        k_update_h<T, CoefTable<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
+    } else {
+        CoefRaw<T> C;
+        for (int k = 0; k < 30; ++k) C.c[k] = (const T*)s.raw[k];
+        k_update_h<T, CoefRaw<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
+    }
+    if (ev) HIP_OK(hipEventRecord(ev[3], st));
+    if (s.ngH)
+        k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(F, s.d_groups + s.ngE, s.ngH, s.d_order, s.d_is_src,
+                                                      (T*)s.d_series, s.P, base, local);
+    if (s.n_active && monitors_now)
+        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(F, s.d_mons, s.n_active, s.Nx, s.Ny, s.Nz, base, local);
+}
+
+bool any_monitor_samples(const Session& s, int64_t n) {
+    for (const auto& m : s.mons)
+        if (m.active && (n == 0 || n % m.desc.n_step == 0)) return true;
+    return false;
+}
+
+template <typename T>
+void run_steps(Session& s, int n_steps, hipStream_t st) {
+    if (n_steps < 0) RF_THROW(RFDTD_EINVAL, "negative step count");
+    if (s.P && s.step + n_steps > s.Nt_probes)
+        RF_THROW(RFDTD_ESTATE, "run past the end of the probe series (Nt = " + std::to_string(s.Nt_probes) + ")");
+    if (s.n_active && s.step + n_steps > s.Nt_monitors)
+        RF_THROW(RFDTD_ESTATE, "run past the monitor length (Nt = " + std::to_string(s.Nt_monitors) + ")");
+    choose_launch(s);
+    HIP_OK(hipEventRecord(s.ev0, st));
+    const bool timing = (s.flags & RFDTD_KERNEL_TIMING) != 0;
+    const bool graphs = !timing && !(s.flags & RFDTD_NO_GRAPHS) && s.graph_steps > 1;
+    // kernel timing: events around the E and H updates of up to kTimedSteps evenly spread steps
+    s.n_timed = 0;
+    const int stride = timing ? std::max(1, n_steps / Session::kTimedSteps) : 0;
+    int done = 0;
+    if (graphs && n_steps >= s.graph_steps) {
+        if (s.graph_dirty || !s.graph) {
+            if (s.graph) { (void)hipGraphExecDestroy(s.graph); s.graph = nullptr; }
+            if (s.graph_def) { (void)hipGraphDestroy(s.graph_def); s.graph_def = nullptr; }
+            hipStream_t cs;
+            HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < s.graph_steps; ++k) launch_step<T>(s, cs, s.d_step_base, k, true);
+            hipError_t ce = hipStreamEndCapture(cs, &s.graph_def);
+            (void)hipStreamDestroy(cs);
+            HIP_OK(ce);
+            HIP_OK(hipGraphInstantiate(&s.graph, s.graph_def, nullptr, nullptr, 0));
+            s.graph_dirty = false;
+        }
+        while (n_steps - done >= s.graph_steps) {
+            k_set_step<<<1, 1, 0, st>>>(s.d_step_base, (int)(s.step + done));
+            HIP_OK(hipGraphLaunch(s.graph, st));
+            done += s.graph_steps;
+        }
+    }
+    for (; done < n_steps; ++done) {
+        const int64_t n = s.step + done;
+        hipEvent_t* ev = nullptr;
+        if (timing && done % stride == 0 && s.n_timed < Session::kTimedSteps) {
+            if (s.kev.empty()) {
+                s.kev.resize(4 * Session::kTimedSteps);
+                for (auto& e : s.kev) HIP_OK(hipEventCreate(&e));
+            }
+            ev = &s.kev[4 * s.n_timed++];
+        }
+        launch_step<T>(s, st, nullptr, (int)n, any_monitor_samples(s, n), ev);
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(s.ev1, st));
+    s.step += n_steps;
+}
+
+Session* make_session(int Nx, int Ny, int Nz, const void* const coeff[30], uint32_t flags) {
+    if (Nx < 2 || Ny < 2 || Nz < 2) RF_THROW(RFDTD_EINVAL, "grid must have at least 2 cells on every axis");
+    if ((int64_t)Nx * Ny * Nz > ((int64_t)1 << 32) - 1) RF_THROW(RFDTD_EINVAL, "grid too large");
+    if (!coeff) RF_THROW(RFDTD_EINVAL, "coefficients missing");
+    auto s = std::make_unique<Session>();
+    s->Nx = Nx; s->Ny = Ny; s->Nz = Nz;
+    s->cells = (int64_t)Nx * Ny * Nz;
+    s->f64 = (flags & RFDTD_F64) != 0;
+    s->esz = s->f64 ? 8 : 4;
+    s->flags = flags;
+    HIP_OK(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&s->ev0));
+    HIP_OK(hipEventCreate(&s->ev1));
+    HIP_OK(hipMalloc(&s->d_step_base, 256));
+    s->fields_bytes = 18 * (((size_t)s->cells * s->esz + 255) & ~(size_t)255);
+    HIP_OK(hipMalloc(&s->d_fields, s->fields_bytes));
+    HIP_OK(hipMemset(s->d_fields, 0, s->fields_bytes));
+    if (s->f64)
+        upload_coefficients<double>(*s, coeff);
+    else
+        upload_coefficients<float>(*s, coeff);
+    return s.release();
+}
+
+void set_probes(Session& s, const rfdtd_probe* probes, int n, int Nt) {
+    if (n < 0 || n > RFDTD_MAX_PROBES) RF_THROW(RFDTD_EINVAL, "Exceeded maximum number of probes.");
+    if (n > 0 && (!probes || Nt < 1)) RF_THROW(RFDTD_EINVAL, "invalid probe arguments");
+    (void)hipDeviceSynchronize();
+    s.drop_graph();
+    s.release_probes();
+    if (n == 0) return;
+    // component shifts of solver.cpp:503-520
+    std::vector<int64_t> cell(n);
+    std::vector<int32_t> field(n), is_src(n);
+    for (int p = 0; p < n; ++p) {
+        const rfdtd_probe& q = probes[p];
+        if (q.field < 0 || q.field > 5) RF_THROW(RFDTD_EINVAL, "probe field must be 0..5");
+        if (!q.values) RF_THROW(RFDTD_EINVAL, "probe values missing");
+        int x = q.idx[0], y = q.idx[1], z = q.idx[2];
+        const int f = q.field;
+        if (f == EY || f == EZ || f == HX) x -= 1;
+        if (f == EX || f == EZ || f == HY) y -= 1;
+        if (f == EX || f == EY || f == HZ) z -= 1;
+        if (x < 0 || y < 0 || z < 0) RF_THROW(RFDTD_EINVAL, "Source cannot be placed on grid edge");
+        if (x >= s.Nx || y >= s.Ny || z >= s.Nz) RF_THROW(RFDTD_EINVAL, "probe index outside the grid");
+        cell[p] = ((int64_t)x * s.Ny + y) * s.Nz + z;
+        field[p] = f;
+        is_src[p] = q.is_source ? 1 : 0;
+    }
+    // groups by (component, cell) in order of first appearance; E groups first
+    std::vector<ProbeGroup> groups;
+    std::vector<int32_t> order;
+    for (int half = 0; half < 2; ++half) {
+        std::unordered_map<int64_t, int> gid;
+        std::vector<std::vector<int32_t>> members;
+        std::vector<ProbeGroup> gs;
+        for (int p = 0; p < n; ++p) {
+            if ((field[p] >= 3) != (half == 1)) continue;
+            const int64_t key = cell[p] * 8 + field[p];
+            auto it = gid.find(key);
+            if (it == gid.end()) {
+                gid.emplace(key, (int)gs.size());
+                gs.push_back(ProbeGroup{cell[p], field[p], 0, 0, 0});
+                members.emplace_back();
+                it = gid.find(key);
+            }
+            members[it->second].push_back(p);
+        }
+        for (size_t g = 0; g < gs.size(); ++g) {
+            gs[g].start = (int32_t)order.size();
+            gs[g].count = (int32_t)members[g].size();
+            order.insert(order.end(), members[g].begin(), members[g].end());
+        }
+        if (half == 0) s.ngE = (int)gs.size(); else s.ngH = (int)gs.size();
+        groups.insert(groups.end(), gs.begin(), gs.end());
+    }
+    s.P = n;
+    s.Nt_probes = Nt;
+    // time-major series [Nt][P]
+    std::vector<char> series((size_t)Nt * n * s.esz);
+    for (int p = 0; p < n; ++p) {
+        const char* src = (const char*)probes[p].values;
+        for (int t = 0; t < Nt; ++t)
+            std::memcpy(&series[((size_t)t * n + p) * s.esz], src + (size_t)t * s.esz, s.esz);
+        s.probe_host_values.push_back(probes[p].values);
+    }
+    HIP_OK(hipMalloc(&s.d_series, series.size()));
+    HIP_OK(hipMemcpy(s.d_series, series.data(), series.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&s.d_order, order.size() * 4));
+    HIP_OK(hipMemcpy(s.d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&s.d_is_src, (size_t)n * 4));
+    HIP_OK(hipMemcpy(s.d_is_src, is_src.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&s.d_groups, groups.size() * sizeof(ProbeGroup)));
+    HIP_OK(hipMemcpy(s.d_groups, groups.data(), groups.size() * sizeof(ProbeGroup), hipMemcpyHostToDevice));
+}
+
+void set_monitors(Session& s, const rfdtd_monitor* mons, int n, int Nt) {
+    if (n < 0 || n > RFDTD_MAX_MONITORS) RF_THROW(RFDTD_EINVAL, "Exceeded maximum number of monitors.");
+    if (n > 0 && (!mons || Nt < 1)) RF_THROW(RFDTD_EINVAL, "invalid monitor arguments");
+    (void)hipDeviceSynchronize();
+    s.drop_graph();
+    s.release_monitors();
+    s.Nt_monitors = Nt;
+    std::vector<MonitorDev> dev;
+    int blocks = 0;
+    for (int m = 0; m < n; ++m) {
+        const rfdtd_monitor& d = mons[m];
+        if (d.axis < 0 || d.axis > 2 || d.field < 0 || d.field > 5 || d.n_step < 1 || d.n_frequencies < 0)
+            RF_THROW(RFDTD_EINVAL, "invalid monitor description");
+        if (!d.values || (d.n_frequencies > 0 && !d.dtft_phase)) RF_THROW(RFDTD_EINVAL, "monitor arrays missing");
+        MonitorHost h{};
+        h.desc = d;
+        const int f = d.field;
+        int pos = d.position;
+        // GPU-packing shifts of solver.cpp:394-433 (gpu = 1)
+        if (d.axis == 0 && (f == HX || f == EY || f == EZ)) pos -= 1;
+        if (d.axis == 1 && (f == HY || f == EX || f == EZ)) pos -= 1;
+        if (d.axis == 2 && (f == HZ || f == EX || f == EY)) pos -= 1;
+        h.shifted = pos;
+        const int along = d.axis == 0 ? s.Nx : d.axis == 1 ? s.Ny : s.Nz;
+        h.N1 = d.axis == 0 ? s.Ny : s.Nx;
+        h.N2 = d.axis == 2 ? s.Ny : s.Nz;
+        h.Nm = Nt / d.n_step + 1;
+        h.active = pos >= 0 && pos < along;
+        const size_t plane = (size_t)h.N1 * h.N2;
+        const size_t vbytes = d.n_frequencies ? (size_t)d.n_frequencies * plane * 2 * s.esz : (size_t)h.Nm * plane * s.esz;
+        HIP_OK(hipMalloc(&h.d_values, vbytes));
+        HIP_OK(hipMemcpy(h.d_values, d.values, vbytes, hipMemcpyHostToDevice));
+        if (d.n_frequencies) {
+            const size_t pbytes = (size_t)h.Nm * d.n_frequencies * 2 * s.esz;
+            HIP_OK(hipMalloc(&h.d_phase, pbytes));
+            HIP_OK(hipMemcpy(h.d_phase, d.dtft_phase, pbytes, hipMemcpyHostToDevice));
+        }
+        if (h.active) {
+            MonitorDev md{};
+            md.axis = d.axis; md.pos = pos; md.field = f; md.n_step = d.n_step; md.nf = d.n_frequencies;
+            md.N1 = h.N1; md.N2 = h.N2; md.N1N2 = (int64_t)plane; md.blk0 = blocks;
+            md.values = h.d_values; md.phase = h.d_phase;
+            blocks += (int)((plane + 255) / 256);
+            dev.push_back(md);
+        }
+        s.mons.push_back(h);
+    }
+    s.n_active = (int)dev.size();
+    s.mon_blocks = blocks;
+    if (!dev.empty()) {
+        HIP_OK(hipMalloc(&s.d_mons, dev.size() * sizeof(MonitorDev)));
+        HIP_OK(hipMemcpy(s.d_mons, dev.data(), dev.size() * sizeof(MonitorDev), hipMemcpyHostToDevice));
+    }
+}
+
+void read_probes(Session& s) {
+    if (!s.P) return;
+    HIP_OK(hipDeviceSynchronize());
+    std::vector<char> series((size_t)s.Nt_probes * s.P * s.esz);
+    HIP_OK(hipMemcpy(series.data(), s.d_series, series.size(), hipMemcpyDeviceToHost));
+    for (int p = 0; p < s.P; ++p) {
+        char* dst = (char*)s.probe_host_values[p];
+        for (int t = 0; t < s.Nt_probes; ++t)
+            std::memcpy(dst + (size_t)t * s.esz, &series[((size_t)t * s.P + p) * s.esz], s.esz);
+    }
+}
+
+void read_monitors(Session& s) {
+    HIP_OK(hipDeviceSynchronize());
+    for (auto& h : s.mons) {
+        const size_t plane = (size_t)h.N1 * h.N2;
+        const size_t vbytes = h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * s.esz
+                                                   : (size_t)h.Nm * plane * s.esz;
+        HIP_OK(hipMemcpy(h.desc.values, h.d_values, vbytes, hipMemcpyDeviceToHost));
+    }
+}
+
+void read_fields(Session& s, void* const out[18]) {
+    HIP_OK(hipDeviceSynchronize());
+    const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
+    const size_t pitch = ((size_t)s.cells * s.esz + 255) & ~(size_t)255;
+    std::vector<char> buf((size_t)s.cells * s.esz);
+    // component of each of the 18 arrays and the offset of GPU cell (0,0,0) in the global array
+    const int comp[18] = {EX, EX, EY, EY, EZ, EZ, HX, HX, HY, HY, HZ, HZ, EX, EY, EZ, HX, HY, HZ};
+    for (int a = 0; a < 18; ++a) {
+        if (!out[a]) continue;
+        HIP_OK(hipMemcpy(buf.data(), (char*)s.d_fields + (size_t)a * pitch, buf.size(), hipMemcpyDeviceToHost));
+        const int c = comp[a];
+        const int gx = Nx + ((c == EY || c == EZ || c == HX) ? 1 : 0);
+        const int gy = Ny + ((c == EX || c == EZ || c == HY) ? 1 : 0);
+        const int gz = Nz + ((c == EX || c == EY || c == HZ) ? 1 : 0);
+        const int ox = gx - Nx, oy = gy - Ny, oz = gz - Nz;
+        char* dst = (char*)out[a];
+        std::memset(dst, 0, (size_t)gx * gy * gz * s.esz);
+        for (int x = 0; x < Nx; ++x)
+            for (int y = 0; y < Ny; ++y)
+                std::memcpy(dst + (((size_t)(x + ox) * gy + (y + oy)) * gz + oz) * s.esz,
+                            &buf[(((size_t)x * Ny + y) * Nz) * s.esz], (size_t)Nz * s.esz);
+    }
+}
+
+void progress_bar(int64_t n, int64_t Nt) {
+    const int pct = (int)((n + 1) * 100 / Nt);
+    const int dots = pct / 5;
+    std::string str = "\r|";
+    str.append(dots, '.');
+    str.append(20 - dots, ' ');
+    str += "| " + std::to_string(pct) + "%  ";
+    std::fputs(str.c_str(), stdout);
+    std::fflush(stdout);
+}
+
+template <class Fn>
+int guarded(Fn&& fn) {
+    try {
+        fn();
+        g_last_error.clear();
+        return RFDTD_OK;
+    } catch (RfdtdError& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return RFDTD_EINVAL;
+    } catch (std::exception& e) {
+        g_last_error = e.what();
+        return RFDTD_EINVAL;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------------
+
+extern "C" {
+
+const char* rfdtd_last_error(void) { return g_last_error.c_str(); }
+
+int32_t rfdtd_version(void) { return 1; }
+
+int rfdtd_create(rfdtd_session** out, int32_t Nx, int32_t Ny, int32_t Nz, const void* const coefficients[30],
+                 uint32_t flags) {
+    return guarded([&] {
+        if (!out) RF_THROW(RFDTD_EINVAL, "null session pointer");
+        *out = nullptr;
+        *out = (rfdtd_session*)make_session(Nx, Ny, Nz, coefficients, flags);
+    });
+}
+
+int rfdtd_set_probes(rfdtd_session* s, const rfdtd_probe* probes, int32_t n_probes, int32_t Nt) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        set_probes(*(Session*)s, probes, n_probes, Nt);
+    });
+}
+
+int rfdtd_set_monitors(rfdtd_session* s, const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nt) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        set_monitors(*(Session*)s, monitors, n_monitors, Nt);
+    });
+}
+
+int rfdtd_run(rfdtd_session* s, int32_t n_steps, void* stream) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        Session& S = *(Session*)s;
+        hipStream_t st = stream ? (hipStream_t)stream : S.own_stream;
+        if (S.f64) run_steps<double>(S, n_steps, st); else run_steps<float>(S, n_steps, st);
+    });
+}
+
+int rfdtd_synchronize(rfdtd_session* s) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        HIP_OK(hipDeviceSynchronize());
+        Session& S = *(Session*)s;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, S.ev0, S.ev1) == hipSuccess) S.last_total_ms = ms;
+        double se = 0, sh = 0;
+        for (int k = 0; k < S.n_timed; ++k) {
+            float a = 0, b = 0;
+            HIP_OK(hipEventElapsedTime(&a, S.kev[4 * k], S.kev[4 * k + 1]));
+            HIP_OK(hipEventElapsedTime(&b, S.kev[4 * k + 2], S.kev[4 * k + 3]));
+            se += a;
+            sh += b;
+        }
+        S.last_e_ms = S.n_timed ? se / S.n_timed : 0;
+        S.last_h_ms = S.n_timed ? sh / S.n_timed : 0;
+    });
+}
+
+int rfdtd_read_probes(rfdtd_session* s) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        read_probes(*(Session*)s);
+    });
+}
+
+int rfdtd_read_monitors(rfdtd_session* s) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        read_monitors(*(Session*)s);
+    });
+}
+
+int rfdtd_read_fields(rfdtd_session* s, void* const out[18]) {
+    return guarded([&] {
+        if (!s || !out) RF_THROW(RFDTD_EINVAL, "null argument");
+        read_fields(*(Session*)s, out);
+    });
+}
+
+int rfdtd_reset(rfdtd_session* s) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        Session& S = *(Session*)s;
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipMemset(S.d_fields, 0, S.fields_bytes));
+        for (auto& h : S.mons) {
+            const size_t plane = (size_t)h.N1 * h.N2;
+            const size_t vbytes = h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * S.esz
+                                                       : (size_t)h.Nm * plane * S.esz;
+            HIP_OK(hipMemset(h.d_values, 0, vbytes));
+        }
+        S.step = 0;
+    });
+}
+
+int64_t rfdtd_step(const rfdtd_session* s) { return s ? ((const Session*)s)->step : -1; }
+
+int rfdtd_last_timing(const rfdtd_session* s, double* total_ms, double* e_kernel_ms, double* h_kernel_ms) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        const Session& S = *(const Session*)s;
+        if (total_ms) *total_ms = S.last_total_ms;
+        if (e_kernel_ms) *e_kernel_ms = S.last_e_ms;
+        if (h_kernel_ms) *h_kernel_ms = S.last_h_ms;
+    });
+}
+
+int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* coef_kind, int64_t* n_e_entries,
+                      int64_t* n_h_entries) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        const Session& S = *(const Session*)s;
+        if (device_bytes) *device_bytes = S.fields_bytes + S.coef_bytes;
+        if (coef_kind) *coef_kind = S.coef_kind;
+        if (n_e_entries) *n_e_entries = S.nE;
+        if (n_h_entries) *n_h_entries = S.nH;
+    });
+}
+
+int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps) {
+    return guarded([&] {
+        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        Session& S = *(Session*)s;
+        if (block_threads) {
+            if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
+                RF_THROW(RFDTD_EINVAL, "block_threads must be a multiple of 64 in [64, 1024]");
+            S.block = block_threads;
+            S.xplanes = 0;
+        }
+        if (x_planes) {
+            if (x_planes < 0) RF_THROW(RFDTD_EINVAL, "x_planes must be positive");
+            S.xplanes = std::min<int>(x_planes, S.Nx);
+        }
+        if (graph_steps) {
+            if (graph_steps < 0) RF_THROW(RFDTD_EINVAL, "graph_steps must be positive");
+            S.graph_steps = graph_steps;
+        }
+        S.graph_dirty = true;
+    });
+}
+
+void rfdtd_destroy(rfdtd_session* s) {
+    if (!s) return;
+    (void)hipDeviceSynchronize();
+    delete (Session*)s;
+}
+
+int rfdtd_solver_run(const void* const coefficients[30], const rfdtd_probe* probes, int32_t n_probes,
+                     const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nx, int32_t Ny, int32_t Nz,
+                     int32_t Nt, int32_t update_interval, uint32_t flags) {
+    Session* s = nullptr;
+    const int rc = guarded([&] {
+        if (Nt < 1) RF_THROW(RFDTD_EINVAL, "Nt must be positive");
+        s = make_session(Nx, Ny, Nz, coefficients, flags);
+        set_probes(*s, probes, n_probes, Nt);
+        set_monitors(*s, monitors, n_monitors, Nt);
+        const bool show = update_interval > 0 && !(flags & RFDTD_QUIET);
+        if (show) progress_bar(0, Nt);
+        // progress is reported per chunk of update_interval steps, as the reference's controller does
+        const int chunk = show ? std::max<int>(update_interval, s->graph_steps) : Nt;
+        for (int done = 0; done < Nt;) {
+            const int k = std::min(chunk, Nt - done);
+            if (s->f64) run_steps<double>(*s, k, s->own_stream); else run_steps<float>(*s, k, s->own_stream);
+            done += k;
+            if (show) {
+                HIP_OK(hipStreamSynchronize(s->own_stream));
+                progress_bar(done - 1, Nt);
+            }
+        }
+        HIP_OK(hipStreamSynchronize(s->own_stream));
+        if (show) progress_bar(Nt, Nt);
+        read_probes(*s);
+        read_monitors(*s);
+    });
+    delete s;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""Resident GPU sessions over the session half of the C ABI (include/rfdtd.h).
+
+A ``Session`` keeps the fields and coefficients of one grid in HBM between calls: set the probes and
+monitors once, advance any number of steps, read probes, monitors or the full 18-array field state.
+Arguments use the reference's calling convention (the GPU packing of rfnetwork/solver.py:1463-1523,
+probe and monitor dictionaries of :1541-1622), validated by ``hip_func.pack_call``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD_COMPONENT, FIELD_NAMES, check
+
+
+def yee_shape(component, Nx, Ny, Nz):
+    """Global Yee shape of field component 0..5 (ex ey ez hx hy hz)."""
+    return [(Nx, Ny + 1, Nz + 1), (Nx + 1, Ny, Nz + 1), (Nx + 1, Ny + 1, Nz),
+            (Nx + 1, Ny, Nz), (Nx, Ny + 1, Nz), (Nx, Ny, Nz + 1)][component]
+
+
+class Session:
+    """One grid resident on the GPU."""
+
+    def __init__(self, coefficients, Nx, Ny, Nz, dtype=np.float32, flags=0):
+        from .hip_func import _check_array, _real_dtype  # local: hip_func imports this module's peers
+
+        self.dtype = _real_dtype(dtype)
+        self.Nx, self.Ny, self.Nz = int(Nx), int(Ny), int(Nz)
+        arrays = []
+        for name in _lib.COEFF_NAMES:
+            a = coefficients[name]
+            _check_array(a, name, (self.Nx, self.Ny, self.Nz), self.dtype)
+            arrays.append(a)
+        self._L = _lib.lib()
+        self._h = ctypes.c_void_p()
+        f = int(flags) | (_lib.F64 if self.dtype == np.float64 else 0)
+        ptrs = _lib.pointer_array(arrays)
+        check(self._L.rfdtd_create(ctypes.byref(self._h), self.Nx, self.Ny, self.Nz,
+                                   ctypes.cast(ptrs, ctypes.c_void_p), f), "rfdtd_create")
+        self._keep = []
+        self.Nt = None
+
+    def set_sources(self, probes, monitors, Nt):
+        """Install probe/source records and monitors (dictionaries as in the reference's solve())."""
+        from .hip_func import pack_monitors, pack_probes
+
+        prb, keep_p = pack_probes(probes, int(Nt), self.dtype)
+        mons, keep_m = pack_monitors(monitors, self.Nx, self.Ny, self.Nz, int(Nt), self.dtype)
+        keep = keep_p + keep_m
+        check(self._L.rfdtd_set_probes(self._h, ctypes.cast(prb, ctypes.c_void_p), len(probes), int(Nt)),
+              "rfdtd_set_probes")
+        check(self._L.rfdtd_set_monitors(self._h, ctypes.cast(mons, ctypes.c_void_p), len(monitors), int(Nt)),
+              "rfdtd_set_monitors")
+        self._keep = [prb, mons, keep]
+        self.probes, self.monitors, self.Nt = probes, monitors, int(Nt)
+
+    def run(self, n_steps, stream=None):
+        """Advance n_steps (asynchronous on the session stream or a given HIP stream handle)."""
+        s = None if stream is None else ctypes.c_void_p(int(stream))
+        check(self._L.rfdtd_run(self._h, int(n_steps), s), "rfdtd_run")
+
+    def synchronize(self):
+        check(self._L.rfdtd_synchronize(self._h), "rfdtd_synchronize")
+
+    def read(self):
+        """Copy probe series and monitor values back into the dictionaries given to set_sources."""
+        check(self._L.rfdtd_read_probes(self._h), "rfdtd_read_probes")
+        check(self._L.rfdtd_read_monitors(self._h), "rfdtd_read_monitors")
+
+    def fields(self, names=FIELD_NAMES):
+        """Dict of Yee-shaped arrays for the requested field names."""
+        out = {}
+        ptrs = (ctypes.c_void_p * 18)()
+        for a, name in enumerate(FIELD_NAMES):
+            if name in names:
+                out[name] = np.zeros(yee_shape(FIELD_COMPONENT[a], self.Nx, self.Ny, self.Nz), self.dtype)
+                ptrs[a] = out[name].ctypes.data
+        check(self._L.rfdtd_read_fields(self._h, ctypes.cast(ptrs, ctypes.c_void_p)), "rfdtd_read_fields")
+        return out
+
+    def reset(self):
+        check(self._L.rfdtd_reset(self._h), "rfdtd_reset")
+
+    @property
+    def step(self):
+        return int(self._L.rfdtd_step(self._h))
+
+    def timing(self):
+        """(total_ms, e_kernel_ms, h_kernel_ms) of the last run (after synchronize)."""
+        t, e, h = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self._L.rfdtd_last_timing(self._h, ctypes.byref(t), ctypes.byref(e), ctypes.byref(h)))
+        return t.value, e.value, h.value
+
+    def memory_info(self):
+        b, k = ctypes.c_uint64(), ctypes.c_int32()
+        ne, nh = ctypes.c_int64(), ctypes.c_int64()
+        check(self._L.rfdtd_memory_info(self._h, ctypes.byref(b), ctypes.byref(k), ctypes.byref(ne), ctypes.byref(nh)))
+        return dict(device_bytes=b.value, coef_kind=("raw", "table")[k.value], n_e_entries=ne.value,
+                    n_h_entries=nh.value)
+
+    def set_launch(self, block_threads=0, x_planes=0, graph_steps=0):
+        check(self._L.rfdtd_set_launch(self._h, int(block_threads), int(x_planes), int(graph_steps)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.rfdtd_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,111 @@
+"""Model builders for the FDTD benchmark and test workloads.
+
+Each builder returns a meshed, excited ``FDTD_Solver`` on a uniform grid of ``cell_m`` metre cells
+(the grid is given in inches, the solver's unit: cell_in = cell_m * 39.3701).
+
+microstrip      120x48x32 cells of 0.1 mm: eps_r 3.66 substrate 4 cells thick on the z = 0 ground,
+                PEC strip 6 cells wide on top, 10-cell PML on x-, x+, y-, y+, z+, a 50 ohm port at
+                each end 1 cell in from the x PML, Gaussian-modulated sine (5.5 GHz, 430 ps) on
+                port 1, 4000 steps at 0.99 of the Courant limit, S-parameters at 101 frequencies
+                1-10 GHz.
+dipole_slab     400x400x200 cells of 0.1 mm, 12-cell PML on every face, eps_r ~ U[1, 4.5) per cell
+                (seed 2) in the centred 200x200x20 slab, two 40-cell PEC Ez rods fed by a 1-cell
+                50 ohm gap source with a 50 ps Gaussian pulse, 8000 steps, running DFTs at 51
+                frequencies 5-30 GHz on 6 planes 2 cells inside the PML.
+tiny_model      small grid with PML, a dielectric block, a conductor, a port, a probe and monitors,
+                for parity tests.
+"""
+import numpy as np
+
+from .geometry import Box, Line, Rectangle
+from .solver import FDTD_Solver
+
+IN_PER_M = 39.3701
+
+
+def _grid_solver(nx, ny, nz, cell_m):
+    d = cell_m * IN_PER_M
+    s = FDTD_Solver(Box((0, 0, 0), (nx * d, ny * d, nz * d)))
+    s.set_grid(np.arange(nx + 1) * d, np.arange(ny + 1) * d, np.arange(nz + 1) * d)
+    return s, d
+
+
+def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=10, n_freq=101):
+    """Two-port microstrip line (the 120x48x32 plain-FDTD workload)."""
+    s, d = _grid_solver(nx, ny, nz, cell_m)
+    h = 4  # substrate thickness, cells
+    y0, y1 = ny // 2 - 3, ny // 2 + 3  # 6-cell strip centred in y
+    xa, xb = n_pml + 1, nx - n_pml - 1  # ports 1 cell in from the x PML
+    s.add_dielectric(Box((0, 0, 0), (nx * d, ny * d, h * d)), er=3.66)
+    s.add_conductor(Rectangle([(xa * d, y0 * d, h * d), (xa * d, y1 * d, h * d), (xb * d, y1 * d, h * d)]))
+    s.add_lumped_port(1, Rectangle([(xa * d, y0 * d, 0), (xa * d, y1 * d, 0), (xa * d, y1 * d, h * d)]), "z+")
+    s.add_lumped_port(2, Rectangle([(xb * d, y0 * d, 0), (xb * d, y1 * d, 0), (xb * d, y1 * d, h * d)]), "z+")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=n_pml)
+    s.generate_mesh(courant=courant)
+    src = s.gaussian_modulated_source(f0=5.5e9, width=430e-12, t0=320e-12, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(1e-2 * np.asarray(src), 1)
+    s.frequency = np.linspace(1e9, 1e10, n_freq)
+    return s
+
+
+def dipole_slab(Nt=8000, nx=400, ny=400, nz=200, cell_m=1e-4, courant=0.99, n_pml=12, n_freq=51, seed=2,
+                monitors=True):
+    """Dipole over a random dielectric slab (the 400x400x200 bandwidth workload)."""
+    s, d = _grid_solver(nx, ny, nz, cell_m)
+    cx, cy, cz = nx // 2, ny // 2, nz // 2
+    rng = np.random.default_rng(seed)
+    sx, sy, sz = nx // 2, ny // 2, 20
+    er = rng.uniform(1.0, 4.5, size=(sx, sy, sz))
+    s.add_permittivity_volume(er, (cx - sx // 2, cy - sy // 2, cz - sz // 2))
+    # rods: Ez components on the (cx, cy) grid line, 40 cells each, 1-cell gap at z = cz
+    s.add_conductor(Line((cx * d, cy * d, (cz - 40) * d), (cx * d, cy * d, cz * d)))
+    s.add_conductor(Line((cx * d, cy * d, (cz + 1) * d), (cx * d, cy * d, (cz + 41) * d)))
+    # gap source: a port face 2 cells wide in y (one interior Ez component) and 1 cell tall
+    s.add_lumped_port(1, Rectangle([(cx * d, (cy - 1) * d, cz * d), (cx * d, (cy + 1) * d, cz * d),
+                                    (cx * d, (cy + 1) * d, (cz + 1) * d)]), "z+")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z-", "z+", n_pml=n_pml)
+    s.generate_mesh(courant=courant)
+    src = s.gaussian_source(width=50e-12, t0=40e-12, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(np.asarray(src), 1)
+    freqs = np.linspace(5e9, 3e10, n_freq)
+    s.frequency = freqs
+    if monitors:
+        k = n_pml + 2
+        s.add_field_monitor("ez_x0", "ez", "x", index=k, frequency=freqs)
+        s.add_field_monitor("ez_x1", "ez", "x", index=nx - k - 1, frequency=freqs)
+        s.add_field_monitor("ez_y0", "ez", "y", index=k, frequency=freqs)
+        s.add_field_monitor("ez_y1", "ez", "y", index=ny - k - 1, frequency=freqs)
+        s.add_field_monitor("ex_z0", "ex", "z", index=k, frequency=freqs)
+        s.add_field_monitor("ex_z1", "ex", "z", index=nz - k - 1, frequency=freqs)
+    return s
+
+
+def tiny_model(Nt=60, nx=14, ny=11, nz=9, cell_m=1e-4, courant=0.99, n_pml=3, seed=7, all_sides=True,
+               time_monitors=True, phasor_monitors=True):
+    """Small model that touches every code path: PML, random dielectric, conductor, port, probes,
+    time-domain and phasor monitors on every axis."""
+    s, d = _grid_solver(nx, ny, nz, cell_m)
+    rng = np.random.default_rng(seed)
+    s.add_permittivity_volume(rng.uniform(1.0, 4.0, size=(nx - 6, ny - 6, 3)), (3, 3, 1))
+    s.add_dielectric(Box((0, 0, 0), (nx * d, ny * d, 1 * d)), er=2.2)
+    s.add_conductor(Rectangle([(4 * d, 4 * d, 4 * d), (4 * d, 7 * d, 4 * d), (10 * d, 7 * d, 4 * d)]))
+    s.add_lumped_port(1, Rectangle([(4 * d, 4 * d, 0), (4 * d, 7 * d, 0), (4 * d, 7 * d, 4 * d)]), "z+")
+    s.add_lumped_port(2, Rectangle([(10 * d, 4 * d, 0), (10 * d, 7 * d, 0), (10 * d, 7 * d, 4 * d)]), "z+")
+    sides = ("x-", "x+", "y-", "y+", "z-", "z+") if all_sides else ("x+", "z+")
+    s.assign_PML_boundaries(*sides, n_pml=n_pml)
+    s.generate_mesh(courant=courant)
+    src = s.gaussian_modulated_source(f0=20e9, width=60e-12, t0=30e-12, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(1e-2 * np.asarray(src), 1)
+    s.add_voltage_probe("v1", Line((7 * d, 5 * d, 0), (7 * d, 5 * d, 4 * d)))
+    if time_monitors:
+        s.add_field_monitor("t_ez_x", "ez", "x", index=6, n_step=3)
+        s.add_field_monitor("t_hy_y", "hy", "y", index=5, n_step=2)
+        s.add_field_monitor("t_ex_z", "ex", "z", index=2, n_step=1)
+        s.add_field_monitor("t_hz_z", "hz", "z", index=4, n_step=5)
+    if phasor_monitors:
+        f = np.array([10e9, 20e9, 35e9])
+        s.add_field_monitor("p_ey_x", "ey", "x", index=8, frequency=f)
+        s.add_field_monitor("p_hx_y", "hx", "y", index=6, frequency=f, n_step=2)
+        s.add_field_monitor("p_ez_z", "ez", "z", index=3, frequency=f)
+    s.frequency = np.linspace(5e9, 40e9, 8)
+    return s
