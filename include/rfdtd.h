@@ -127,6 +127,11 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
 void rfdtd_destroy(rfdtd_session* s);
 
+/* Bandwidth probe (no counterpart in the reference): copy `bytes` between two device buffers `iters`
+ * times with a plain streaming kernel and report read+write GB/s and ms per copy -- the measured HBM
+ * ceiling, and the known-size kernel that calibrates the profiler's byte counters. */
+int rfdtd_copy_bandwidth(uint64_t bytes, int32_t iters, double* gbytes_per_s, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

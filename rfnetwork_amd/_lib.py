@@ -46,7 +46,7 @@ EXPORTS = (
     "rfdtd_last_error", "rfdtd_version", "rfdtd_solver_run", "rfdtd_create", "rfdtd_set_probes",
     "rfdtd_set_monitors", "rfdtd_run", "rfdtd_synchronize", "rfdtd_read_probes", "rfdtd_read_monitors",
     "rfdtd_read_fields", "rfdtd_reset", "rfdtd_step", "rfdtd_last_timing", "rfdtd_memory_info",
-    "rfdtd_set_launch", "rfdtd_destroy",
+    "rfdtd_set_launch", "rfdtd_destroy", "rfdtd_copy_bandwidth",
 )
 
 
@@ -112,6 +112,8 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_set_launch.argtypes = [vp, i32, i32, i32]
     L.rfdtd_destroy.argtypes = [vp]
     L.rfdtd_destroy.restype = None
+    L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -133,6 +135,14 @@ def pointer_array(arrays) -> "ctypes.Array":
     for k, a in enumerate(arrays):
         arr[k] = None if a is None else a.ctypes.data
     return arr
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20):
+    """(GB/s counting read + write, ms per copy) of a plain device-to-device streaming copy."""
+    g, ms = ctypes.c_double(), ctypes.c_double()
+    check(lib().rfdtd_copy_bandwidth(int(nbytes), int(iters), ctypes.byref(g), ctypes.byref(ms)),
+          "rfdtd_copy_bandwidth")
+    return g.value, ms.value
 
 
 def gpu_visible() -> bool:

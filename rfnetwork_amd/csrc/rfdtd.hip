@@ -348,6 +348,14 @@ __global__ void k_monitors(Fields<T> F, const MonitorDev* mons, int nmon, int Nx
 
 __global__ void k_set_step(int* p, int v) { *p = v; }
 
+// Plain streaming copy (16 B per lane per iteration): the measured bandwidth ceiling, and the
+// reference point that scales the profiler's HBM byte counters.
+__global__ void __launch_bounds__(256) k_stream_copy(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                     int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------------
@@ -1086,6 +1094,39 @@ void rfdtd_destroy(rfdtd_session* s) {
     if (!s) return;
     (void)hipDeviceSynchronize();
     delete (Session*)s;
+}
+
+int rfdtd_copy_bandwidth(uint64_t bytes, int32_t iters, double* gbytes_per_s, double* ms_per_copy) {
+    void* a = nullptr;
+    void* b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int rc = guarded([&] {
+        if (bytes < 4096 || iters < 1) RF_THROW(RFDTD_EINVAL, "bytes >= 4096 and iters >= 1 required");
+        bytes &= ~(uint64_t)15;
+        HIP_OK(hipMalloc(&a, bytes));
+        HIP_OK(hipMalloc(&b, bytes));
+        HIP_OK(hipMemset(a, 0, bytes));
+        HIP_OK(hipMemset(b, 0, bytes));
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        const int64_t n = (int64_t)(bytes / 16);
+        const int grid = 256 * 32;
+        k_stream_copy<<<grid, 256>>>((float4*)b, (const float4*)a, n);  // warm-up
+        HIP_OK(hipEventRecord(e0, nullptr));
+        for (int k = 0; k < iters; ++k) k_stream_copy<<<grid, 256>>>((float4*)b, (const float4*)a, n);
+        HIP_OK(hipEventRecord(e1, nullptr));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        const double per = (double)ms / iters;
+        if (ms_per_copy) *ms_per_copy = per;
+        if (gbytes_per_s) *gbytes_per_s = 2.0 * (double)bytes / (per * 1e6);
+    });
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int rfdtd_solver_run(const void* const coefficients[30], const rfdtd_probe* probes, int32_t n_probes,

@@ -129,6 +129,21 @@ def pack_probes(probes, Nt, dtype=np.float32):
     return prb, keep
 
 
+def check_probe_cells(probes, Nx, Ny, Nz):
+    """Component shifts of solver.cpp:503-520 from global Yee indices to grid cells, with the
+    reference's grid-edge error and an upper bound the reference does not check."""
+    for d in probes:
+        f = int(d["field"])
+        x, y, z = (int(i) for i in d["idx"])
+        x -= f in (1, 2, 3)
+        y -= f in (0, 2, 4)
+        z -= f in (0, 1, 5)
+        if x < 0 or y < 0 or z < 0:
+            raise ValueError("Source cannot be placed on grid edge")
+        if x >= Nx or y >= Ny or z >= Nz:
+            raise ValueError("probe index outside the grid")
+
+
 def pack_call(coefficients, probes, monitors, mem, Nx, Ny, Nz, Nt, dtype=np.float32):
     """Validate the reference-format arguments and build the C ABI records.
 
@@ -161,6 +176,7 @@ def pack_call(coefficients, probes, monitors, mem, Nx, Ny, Nz, Nt, dtype=np.floa
         coefficients[name][Nx - 1] = 0
     mons, keep_m = pack_monitors(monitors, Nx, Ny, Nz, Nt, dtype)
     prb, keep_p = pack_probes(probes, Nt, dtype)
+    check_probe_cells(probes, Nx, Ny, Nz)
     return _lib.pointer_array(coeff), prb, mons, keep_m + keep_p + coeff
 
 

@@ -657,18 +657,10 @@ class FDTD_Solver:
                 p["src"] = None
 
     # ------------------------------------------------------------------ solve
-    def pack(self, gpu: bool = True, dtype=np.float32, n_threads: int = 4):
-        """Solver inputs in the reference's calling convention (solver.py:1439-1622).
-
-        Returns (coefficients, probes, monitors, mem, Nx, Ny, Nz, Nt).  gpu=True gives the GPU packing
-        (all arrays (Nx, Ny, Nz)); gpu=False the CPU packing of the reference's threaded solver."""
+    def coefficient_arrays(self, gpu: bool = True, dtype=np.float32):
+        """The 30 solver coefficient arrays with cell spacings and curl signs folded in
+        (solver.py:1439-1523): all (Nx, Ny, Nz) for gpu=True, the CPU solver's shapes otherwise."""
         self.check_mesh()
-        srcs = [p["src"] for p in self.ports if p is not None and p["src"] is not None]
-        if not srcs:
-            raise ValueError("No port excitations found.")
-        Nt = len(srcs[0])
-        if not all(Nt == len(s) for s in srcs):
-            raise ValueError("Excitations must have identical lengths.")
         dtype_ = np.dtype(dtype)
         dx, dy, dz = [conv.m_in(d).astype(dtype_) for d in self.d_cells]
         dx_h, dy_h, dz_h = [conv.m_in(d).astype(dtype_) for d in self.dh_cells]
@@ -699,7 +691,23 @@ class FDTD_Solver:
             Db_hz_x1=-Db["hz_x1"][:, :, hs:] * dx_inv, Db_hz_x2=-Db["hz_x2"][:, :, hs:] * dx_inv,
             Db_hz_y1=Db["hz_y1"][:, :, hs:] * dy_inv, Db_hz_y2=Db["hz_y2"][:, :, hs:] * dy_inv,
         )
-        coefficients = {k: np.array(v, order="C", dtype=dtype_) for k, v in c.items()}
+        return {k: np.array(v, order="C", dtype=dtype_) for k, v in c.items()}
+
+    def pack(self, gpu: bool = True, dtype=np.float32, n_threads: int = 4):
+        """Solver inputs in the reference's calling convention (solver.py:1439-1622).
+
+        Returns (coefficients, probes, monitors, mem, Nx, Ny, Nz, Nt).  gpu=True gives the GPU packing
+        (all arrays (Nx, Ny, Nz)); gpu=False the CPU packing of the reference's threaded solver."""
+        self.check_mesh()
+        srcs = [p["src"] for p in self.ports if p is not None and p["src"] is not None]
+        if not srcs:
+            raise ValueError("No port excitations found.")
+        Nt = len(srcs[0])
+        if not all(Nt == len(s) for s in srcs):
+            raise ValueError("Excitations must have identical lengths.")
+        dtype_ = np.dtype(dtype)
+        coefficients = self.coefficient_arrays(gpu=gpu, dtype=dtype_)
+        Nx, Ny, Nz = self.n_cells
         mem_len = sum(3 * int(np.prod((Nx,) + s[1:])) for s in self.fshape.values())
         mem_len += (4 * (Ny + 1) * (Nz + 1)) * n_threads * 8
         mem = np.zeros(mem_len, dtype=dtype_) if not gpu else np.zeros(1, dtype=dtype_)
