@@ -179,3 +179,21 @@ def test_large_grid_parity_and_linearity(hip_ready):
     for k in ("ez", "hx"):
         np.testing.assert_array_equal(res[1][k], 2 * res[0][k])
     assert np.abs(res[0]["ez"]).max() > 0
+
+
+@pytest.mark.parametrize("dtype,name", [(np.float32, "tiny_fp32.npz"), (np.float64, "tiny_fp64.npz")])
+def test_hip_reproduces_golden(hip_ready, dtype, name):
+    """The committed oracle fixtures, reproduced by the HIP engine bit for bit."""
+    from pathlib import Path
+
+    print(f"[case] golden {name}", flush=True)
+    want = np.load(Path(__file__).resolve().parent / "golden" / name)
+    model = workloads.tiny_model(Nt=60)
+    p_g, m_g, f_g, _ = _hip_session(model, dtype)
+    model.unpack(p_g, m_g, gpu=True)
+    for i, p in enumerate(p_g):
+        np.testing.assert_array_equal(p["values"], want[f"probe_{i}"])
+    for k, m in model.monitors.items():
+        np.testing.assert_array_equal(m["values"], want[f"monitor_{k}"], err_msg=k)
+    for k, v in f_g.items():
+        np.testing.assert_array_equal(v, want[f"field_{k}"], err_msg=k)
