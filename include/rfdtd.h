@@ -97,15 +97,34 @@ int rfdtd_solver_run(const void* const coefficients[RFDTD_NUM_COEFFS], const rfd
                      int32_t n_probes, const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nx,
                      int32_t Ny, int32_t Nz, int32_t Nt, int32_t update_interval, uint32_t flags);
 
-/* Session API: the same solve split into resident steps (fields and coefficients stay in HBM). */
+/* Session API: the same solve split into resident steps (fields and coefficients stay in HBM).
+ * rfdtd_set_probes / rfdtd_set_monitors / rfdtd_read_fields address instance 0. */
 int rfdtd_create(rfdtd_session** out, int32_t Nx, int32_t Ny, int32_t Nz,
                  const void* const coefficients[RFDTD_NUM_COEFFS], uint32_t flags);
 int rfdtd_set_probes(rfdtd_session* s, const rfdtd_probe* probes, int32_t n_probes, int32_t Nt);
 int rfdtd_set_monitors(rfdtd_session* s, const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nt);
-/* Advance n_steps time steps on stream (NULL = the session's own stream); asynchronous. */
+/*
+ * Batched session (no counterpart in the reference, which solves one grid and one excitation per
+ * call): n_grids grids of their own sizes, grid g advanced for excitations[g] excitations that share
+ * its 30 coefficient arrays.  dims is int32[3*n_grids] (Nx, Ny, Nz per grid) and coefficients holds
+ * 30 pointers per grid, in the order above.  Instances (grid, excitation) are numbered grid by grid,
+ * excitation fastest; every phase of a time step is one launch over all of them.
+ */
+int rfdtd_create_batch(rfdtd_session** out, int32_t n_grids, const int32_t* dims, const int32_t* excitations,
+                       const void* const* coefficients, uint32_t flags);
+int32_t rfdtd_instance_count(const rfdtd_session* s);
+/* Probes/sources and monitors of one instance (all instances share Nt). */
+int rfdtd_set_instance_sources(rfdtd_session* s, int32_t instance, const rfdtd_probe* probes, int32_t n_probes,
+                               const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nt);
+/* Fields of one instance, in the layout of rfdtd_read_fields. */
+int rfdtd_read_instance_fields(rfdtd_session* s, int32_t instance, void* const out[RFDTD_NUM_FIELDS]);
+
+/* Advance n_steps time steps on stream (NULL = the session's own stream); asynchronous.  For a
+ * batched session every instance advances. */
 int rfdtd_run(rfdtd_session* s, int32_t n_steps, void* stream);
 int rfdtd_synchronize(rfdtd_session* s);
-/* Copy probe series / monitor values back into the host arrays given to set_probes / set_monitors. */
+/* Copy probe series / monitor values back into the host arrays given to set_probes / set_monitors
+ * (all instances of a batched session). */
 int rfdtd_read_probes(rfdtd_session* s);
 int rfdtd_read_monitors(rfdtd_session* s);
 /* Copy the 18 fields into global Yee-shaped host arrays: ex* (Nx,Ny+1,Nz+1), ey* (Nx+1,Ny,Nz+1),
@@ -125,6 +144,10 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
 /* Launch shape: threads per block (64..1024, multiple of 64) and x planes per block (>= 1);
  * 0 keeps the current value. */
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
+/* Kernel variant (no counterpart in the reference): pipelined != 0 (default) issues the loads of the
+ * next x plane before the arithmetic and stores of the current one; 0 loads one plane at a time.
+ * Results are identical; the choice only changes speed (the autotuner picks it). */
+int rfdtd_set_variant(rfdtd_session* s, int32_t pipelined);
 void rfdtd_destroy(rfdtd_session* s);
 
 /* Bandwidth probe (no counterpart in the reference): copy `bytes` between two device buffers `iters`

@@ -46,7 +46,8 @@ EXPORTS = (
     "rfdtd_last_error", "rfdtd_version", "rfdtd_solver_run", "rfdtd_create", "rfdtd_set_probes",
     "rfdtd_set_monitors", "rfdtd_run", "rfdtd_synchronize", "rfdtd_read_probes", "rfdtd_read_monitors",
     "rfdtd_read_fields", "rfdtd_reset", "rfdtd_step", "rfdtd_last_timing", "rfdtd_memory_info",
-    "rfdtd_set_launch", "rfdtd_destroy", "rfdtd_copy_bandwidth",
+    "rfdtd_set_launch", "rfdtd_destroy", "rfdtd_copy_bandwidth", "rfdtd_create_batch",
+    "rfdtd_instance_count", "rfdtd_set_instance_sources", "rfdtd_read_instance_fields", "rfdtd_set_variant",
 )
 
 
@@ -112,6 +113,12 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_set_launch.argtypes = [vp, i32, i32, i32]
     L.rfdtd_destroy.argtypes = [vp]
     L.rfdtd_destroy.restype = None
+    L.rfdtd_create_batch.argtypes = [ctypes.POINTER(vp), i32, vp, vp, vp, u32]
+    L.rfdtd_instance_count.argtypes = [vp]
+    L.rfdtd_instance_count.restype = i32
+    L.rfdtd_set_instance_sources.argtypes = [vp, i32, vp, i32, vp, i32, i32]
+    L.rfdtd_read_instance_fields.argtypes = [vp, i32, vp]
+    L.rfdtd_set_variant.argtypes = [vp, i32]
     L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]
     _lib = L

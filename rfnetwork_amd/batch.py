@@ -1,0 +1,188 @@
+"""Batched solves: many excitations and uneven grids advanced together on one GPU.
+
+A ``Batch`` session holds several grids; each grid is advanced for several excitations that share
+its coefficients.  Every phase of a time step (E update, E sources, H update, H probes, DFT planes)
+is one kernel launch over all instances, captured in hipGraph chunks, so a batch of small grids
+costs the launches of one grid and the excitations of a grid share its coefficient reads.
+
+``solve_ports(jobs, frequency)`` is the model-level entry: for each (model, ports) job every listed
+port is driven in its own excitation (the others terminated in their resistance), exactly as
+``model.assign_excitation(waveform, port); model.solve()`` would do one at a time, and the S-matrix
+columns are returned.  The reference runs one excitation per ``solve()`` call
+(rfnetwork/solver.py:1410; tests/test_solver/test_coupled_stripline.py:96-109 re-solves per mode).
+"""
+import copy
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD_COMPONENT, FIELD_NAMES, check
+from .session import yee_shape
+
+
+class Batch:
+    """Several grids x excitations resident on the GPU."""
+
+    def __init__(self, grids, dtype=np.float32, flags=0):
+        """grids: list of (coefficients dict, Nx, Ny, Nz, n_excitations) in the GPU packing."""
+        from .hip_func import _check_array, _real_dtype
+
+        self.dtype = _real_dtype(dtype)
+        self.grids = []
+        dims, exc, ptrs = [], [], []
+        for coeff, Nx, Ny, Nz, K in grids:
+            for name in _lib.COEFF_NAMES:
+                _check_array(coeff[name], name, (Nx, Ny, Nz), self.dtype)
+                ptrs.append(coeff[name])
+            dims += [int(Nx), int(Ny), int(Nz)]
+            exc.append(int(K))
+            self.grids.append((int(Nx), int(Ny), int(Nz), int(K)))
+        self._L = _lib.lib()
+        self._h = ctypes.c_void_p()
+        d = (ctypes.c_int32 * len(dims))(*dims)
+        e = (ctypes.c_int32 * len(exc))(*exc)
+        parr = _lib.pointer_array(ptrs)
+        f = int(flags) | (_lib.F64 if self.dtype == np.float64 else 0)
+        check(self._L.rfdtd_create_batch(ctypes.byref(self._h), len(grids), ctypes.cast(d, ctypes.c_void_p),
+                                         ctypes.cast(e, ctypes.c_void_p), ctypes.cast(parr, ctypes.c_void_p), f),
+              "rfdtd_create_batch")
+        self.instances = []  # (grid index, excitation index)
+        for g, (_, _, _, K) in enumerate(self.grids):
+            self.instances += [(g, k) for k in range(K)]
+        self._keep = {}
+
+    def set_sources(self, instance, probes, monitors, Nt):
+        from .hip_func import check_probe_cells, pack_monitors, pack_probes
+
+        Nx, Ny, Nz, _ = self.grids[self.instances[instance][0]]
+        check_probe_cells(probes, Nx, Ny, Nz)
+        prb, kp = pack_probes(probes, int(Nt), self.dtype)
+        mons, km = pack_monitors(monitors, Nx, Ny, Nz, int(Nt), self.dtype)
+        check(self._L.rfdtd_set_instance_sources(self._h, int(instance), ctypes.cast(prb, ctypes.c_void_p),
+                                                 len(probes), ctypes.cast(mons, ctypes.c_void_p), len(monitors),
+                                                 int(Nt)), "rfdtd_set_instance_sources")
+        self._keep[instance] = (prb, mons, kp, km)
+
+    def run(self, n_steps, stream=None):
+        s = None if stream is None else ctypes.c_void_p(int(stream))
+        check(self._L.rfdtd_run(self._h, int(n_steps), s), "rfdtd_run")
+
+    def synchronize(self):
+        check(self._L.rfdtd_synchronize(self._h), "rfdtd_synchronize")
+
+    def read(self):
+        check(self._L.rfdtd_read_probes(self._h), "rfdtd_read_probes")
+        check(self._L.rfdtd_read_monitors(self._h), "rfdtd_read_monitors")
+
+    def fields(self, instance, names=FIELD_NAMES):
+        Nx, Ny, Nz, _ = self.grids[self.instances[instance][0]]
+        out, ptrs = {}, (ctypes.c_void_p * 18)()
+        for a, name in enumerate(FIELD_NAMES):
+            if name in names:
+                out[name] = np.zeros(yee_shape(FIELD_COMPONENT[a], Nx, Ny, Nz), self.dtype)
+                ptrs[a] = out[name].ctypes.data
+        check(self._L.rfdtd_read_instance_fields(self._h, int(instance), ctypes.cast(ptrs, ctypes.c_void_p)))
+        return out
+
+    def timing(self):
+        t, e, h = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self._L.rfdtd_last_timing(self._h, ctypes.byref(t), ctypes.byref(e), ctypes.byref(h)))
+        return t.value, e.value, h.value
+
+    def set_launch(self, block_threads=0, x_planes=0, graph_steps=0):
+        check(self._L.rfdtd_set_launch(self._h, int(block_threads), int(x_planes), int(graph_steps)))
+
+    def set_variant(self, pipelined=1):
+        check(self._L.rfdtd_set_variant(self._h, int(pipelined)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.rfdtd_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _waveform(model):
+    for p in model.ports:
+        if p is not None and p["src"] is not None:
+            return np.asarray(p["src"])
+    raise ValueError("assign an excitation waveform to the model before batching")
+
+
+def pack_excitations(model, ports, dtype=np.float32):
+    """Per-port excitation inputs of one model: a list of (port, probes, monitors), Nt."""
+    wave = _waveform(model)
+    saved = [None if p is None else p["src"] for p in model.ports]
+    saved_inv = [None if p is None else p["src"] for p in model.ports_inv]
+    out = []
+    try:
+        for port in ports:
+            for p in model.ports + model.ports_inv:
+                if p is not None:
+                    p["src"] = None
+            model.ports[port - 1]["src"] = wave.astype(np.float32)
+            if model.ports_inv[port - 1] is not None:
+                model.ports_inv[port - 1]["src"] = wave.astype(np.float32)
+            model._time = np.arange(len(wave)) * model.dt
+            _, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype, coefficients=False)
+            out.append((port, probes, monitors))
+    finally:
+        for p, s in zip(model.ports, saved):
+            if p is not None:
+                p["src"] = s
+        for p, s in zip(model.ports_inv, saved_inv):
+            if p is not None:
+                p["src"] = s
+    return out, Nt
+
+
+def solve_ports(jobs, frequency, dtype=np.float32, flags=0, return_batch=False):
+    """Drive every (model, ports) job's ports one excitation each, all in one batched session.
+
+    Returns, per job, an array S[f, b, a] whose columns a are the driven ports (source_port order of
+    ``ports``), computed with the model's get_sparameters for each excitation, and the models'
+    monitor values per excitation (list of dicts)."""
+    grids, inputs = [], []
+    Nt_all = None
+    for model, ports in jobs:
+        exc, Nt = pack_excitations(model, ports, dtype)
+        if Nt_all is None:
+            Nt_all = Nt
+        elif Nt != Nt_all:
+            raise ValueError("all batched models must run the same number of steps")
+        coeff = model.coefficient_arrays(gpu=True, dtype=dtype)
+        grids.append((coeff, model.Nx, model.Ny, model.Nz, len(ports)))
+        inputs.append(exc)
+    batch = Batch(grids, dtype=dtype, flags=flags)
+    k = 0
+    for exc in inputs:
+        for _, probes, monitors in exc:
+            batch.set_sources(k, probes, monitors, Nt_all)
+            k += 1
+    batch.run(Nt_all)
+    batch.synchronize()
+    batch.read()
+    results = []
+    for (model, ports), exc in zip(jobs, inputs):
+        cols, mons = [], []
+        for port, probes, monitors in exc:
+            model.unpack(probes, monitors, gpu=True)
+            cols.append(np.asarray(model.get_sparameters(frequency, source_port=port))[..., 0])
+            mons.append({n: np.array(m["values"]) for n, m in model.monitors.items()})
+        results.append((np.stack(cols, axis=-1), mons))
+    if return_batch:
+        return results, batch
+    batch.close()
+    return results

@@ -17,23 +17,24 @@
 // solver's bit for bit.
 //
 // Design for gfx950 (not a translation of the CUDA kernels):
+//   * a session holds one or more grids ("members"), each advanced for one or more excitations
+//     ("instances" sharing the member's coefficients); one launch per phase covers every instance
+//     of every member, so a batch of uneven grids costs the launches of a single grid;
 //   * one thread owns a (y,z) column of a block of x planes and marches up in x; the x+1 (E update)
 //     and x-1 (H update) neighbours are carried in registers, the y/z neighbours come from the
 //     lanes of the same or the next wavefront (L1/L2 hits), so each array is read from HBM once;
 //   * lanes run along z, the contiguous axis: every load and store is a coalesced 256-byte row;
-//   * workgroups are renumbered so that consecutive (y,z) tiles land on the same XCD and share its
-//     L2 for the y-neighbour rows;
+//   * workgroups are renumbered so that consecutive tiles land on the same XCD and share its L2:
+//     neighbouring (y,z) tiles share y-neighbour rows, and the instances of one member -- ordered
+//     innermost -- share the coefficient reads of the same cells;
 //   * the 30 coefficient arrays are deduplicated on the host, bit-exactly, into per-cell 32-bit
-//     entry indices plus small tables of distinct E (12 values) and H (18 values) tuples; interior
-//     vacuum, dielectric and PML cells collapse to a few thousand entries, which stay in L2, so a
-//     step moves 8 B of coefficient index per cell instead of 120 B;
+//     entry indices plus tables of distinct E (12 values) and H (18 values) tuples: a step moves
+//     8 B of coefficient index per cell instead of 120 B wherever materials repeat;
 //   * a chunk of steps is captured once into a hipGraph and replayed; the kernels read the step
 //     number from a device word set before each replay.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
-#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -41,6 +42,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -65,6 +67,13 @@ struct RfdtdError {
 
 enum { EX = 0, EY = 1, EZ = 2, HX = 3, HY = 4, HZ = 5 };
 
+// Global address space qualifier for device code (the host pass parses device functions too).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define GLOBAL_AS
+#endif
+
 // ------------------------------------------------------------------------------------------------
 // device side
 // ------------------------------------------------------------------------------------------------
@@ -81,6 +90,17 @@ struct alignas(4 * sizeof(T)) Vec4 {
     T v[4];
 };
 
+// Load / store at a 32-bit byte offset from a wave-uniform base: lowers to the scalar-base +
+// 32-bit-vector-offset form of global_load / global_store (no 64-bit address per lane).
+template <class T>
+__device__ __forceinline__ T ldg(const GLOBAL_AS T* base, uint32_t boff) {
+    return *(const GLOBAL_AS T*)((const GLOBAL_AS char*)base + boff);
+}
+template <class T>
+__device__ __forceinline__ void stg(GLOBAL_AS T* base, uint32_t boff, T v) {
+    *(GLOBAL_AS T*)((GLOBAL_AS char*)base + boff) = v;
+}
+
 // E coefficients of one cell, order of the reference dictionary entries 0..11.
 template <typename T>
 struct ECoef {
@@ -94,49 +114,73 @@ struct HCoef {
     T da_zx, da_zy, db_zx1, db_zx2, db_zy1, db_zy2;
 };
 
+// One instance (a grid and one excitation) as the kernels see it.
+template <typename T>
+struct InstDev {
+    Fields<T> F;
+    const T* raw[30];        // coefficient arrays (coef_kind 0)
+    const uint32_t* eidx;    // per-cell E entry (coef_kind 1)
+    const uint32_t* hidx;    // per-cell H entry
+    const Vec4<T>* etab;     // 3 Vec4 per E entry
+    const Vec4<T>* htab;     // 5 Vec4 per H entry (18 values + 2 pad)
+    int32_t coef_kind, Nx, Ny, Nz, xplanes, npb, pad0, pad1;
+};
+
+// A member (grid) owns tiles * K logical workgroups starting at tile0; its K instances are
+// consecutive from inst0 and vary fastest in the logical numbering.
+struct MemberDev {
+    int32_t tile0, tiles, K, inst0;
+};
+
 // The 30 arrays as given (one coalesced load per coefficient).
 template <typename T>
 struct CoefRaw {
-    const T* c[30];
-    __device__ __forceinline__ ECoef<T> e(int64_t i) const {
+    const T* const* cp;
+    // b: byte offset of the cell in every array
+    __device__ __forceinline__ ECoef<T> e(uint32_t b) const {
+        auto c = [&](int k) { return ldg((const GLOBAL_AS T*)cp[k], b); };
         ECoef<T> r;
-        r.ca_xy = c[0][i]; r.ca_xz = c[1][i]; r.cb_xy = c[2][i]; r.cb_xz = c[3][i];
-        r.ca_yz = c[4][i]; r.ca_yx = c[5][i]; r.cb_yz = c[6][i]; r.cb_yx = c[7][i];
-        r.ca_zx = c[8][i]; r.ca_zy = c[9][i]; r.cb_zx = c[10][i]; r.cb_zy = c[11][i];
+        r.ca_xy = c(0); r.ca_xz = c(1); r.cb_xy = c(2); r.cb_xz = c(3);
+        r.ca_yz = c(4); r.ca_yx = c(5); r.cb_yz = c(6); r.cb_yx = c(7);
+        r.ca_zx = c(8); r.ca_zy = c(9); r.cb_zx = c(10); r.cb_zy = c(11);
         return r;
     }
-    __device__ __forceinline__ HCoef<T> h(int64_t i) const {
+    __device__ __forceinline__ HCoef<T> h(uint32_t b) const {
+        auto c = [&](int k) { return ldg((const GLOBAL_AS T*)cp[k], b); };
         HCoef<T> r;
-        r.da_xy = c[12][i]; r.da_xz = c[13][i]; r.db_xy1 = c[14][i]; r.db_xy2 = c[15][i];
-        r.db_xz1 = c[16][i]; r.db_xz2 = c[17][i];
-        r.da_yz = c[18][i]; r.da_yx = c[19][i]; r.db_yz1 = c[20][i]; r.db_yz2 = c[21][i];
-        r.db_yx1 = c[22][i]; r.db_yx2 = c[23][i];
-        r.da_zx = c[24][i]; r.da_zy = c[25][i]; r.db_zx1 = c[26][i]; r.db_zx2 = c[27][i];
-        r.db_zy1 = c[28][i]; r.db_zy2 = c[29][i];
+        r.da_xy = c(12); r.da_xz = c(13); r.db_xy1 = c(14); r.db_xy2 = c(15); r.db_xz1 = c(16); r.db_xz2 = c(17);
+        r.da_yz = c(18); r.da_yx = c(19); r.db_yz1 = c(20); r.db_yz2 = c(21); r.db_yx1 = c(22); r.db_yx2 = c(23);
+        r.da_zx = c(24); r.da_zy = c(25); r.db_zx1 = c(26); r.db_zx2 = c(27); r.db_zy1 = c(28); r.db_zy2 = c(29);
         return r;
     }
 };
 
-// Deduplicated coefficients: per-cell entry index into tables of distinct tuples.  E entries are
-// 12 values (3 x Vec4), H entries 18 values padded to 20 (5 x Vec4).
+// Deduplicated coefficients: per-cell entry index into tables of distinct tuples.
 template <typename T>
 struct CoefTable {
     const uint32_t* eidx;
     const uint32_t* hidx;
     const Vec4<T>* etab;
     const Vec4<T>* htab;
-    __device__ __forceinline__ ECoef<T> e(int64_t i) const {
-        const Vec4<T>* t = etab + (size_t)eidx[i] * 3;
-        const Vec4<T> a = t[0], b = t[1], c = t[2];
+    // bc: byte offset of the cell in a T array; the index arrays are uint32
+    __device__ __forceinline__ ECoef<T> e(uint32_t bc) const {
+        const uint32_t k = ldg((const GLOBAL_AS uint32_t*)eidx, bc / (uint32_t)sizeof(T) * 4u);
+        const GLOBAL_AS Vec4<T>* tb = (const GLOBAL_AS Vec4<T>*)etab;
+        const uint32_t o = k * (uint32_t)(3 * sizeof(Vec4<T>));
+        const Vec4<T> a = ldg(tb, o), b = ldg(tb, o + (uint32_t)sizeof(Vec4<T>)),
+                      c = ldg(tb, o + 2u * (uint32_t)sizeof(Vec4<T>));
         ECoef<T> r;
         r.ca_xy = a.v[0]; r.ca_xz = a.v[1]; r.cb_xy = a.v[2]; r.cb_xz = a.v[3];
         r.ca_yz = b.v[0]; r.ca_yx = b.v[1]; r.cb_yz = b.v[2]; r.cb_yx = b.v[3];
         r.ca_zx = c.v[0]; r.ca_zy = c.v[1]; r.cb_zx = c.v[2]; r.cb_zy = c.v[3];
         return r;
     }
-    __device__ __forceinline__ HCoef<T> h(int64_t i) const {
-        const Vec4<T>* t = htab + (size_t)hidx[i] * 5;
-        const Vec4<T> a = t[0], b = t[1], c = t[2], d = t[3], e = t[4];
+    __device__ __forceinline__ HCoef<T> h(uint32_t bc) const {
+        const uint32_t k = ldg((const GLOBAL_AS uint32_t*)hidx, bc / (uint32_t)sizeof(T) * 4u);
+        const GLOBAL_AS Vec4<T>* tb = (const GLOBAL_AS Vec4<T>*)htab;
+        const uint32_t o = k * (uint32_t)(5 * sizeof(Vec4<T>)), v = (uint32_t)sizeof(Vec4<T>);
+        const Vec4<T> a = ldg(tb, o), b = ldg(tb, o + v), c = ldg(tb, o + 2u * v), d = ldg(tb, o + 3u * v),
+                      e = ldg(tb, o + 4u * v);
         HCoef<T> r;
         r.da_xy = a.v[0]; r.da_xz = a.v[1]; r.db_xy1 = a.v[2]; r.db_xy2 = a.v[3];
         r.db_xz1 = b.v[0]; r.db_xz2 = b.v[1];
@@ -161,107 +205,217 @@ struct Tile {
     int y, z, x0, x1;
 };
 
-__device__ __forceinline__ bool tile_of(Tile& t, int Nx, int Ny, int Nz, int xplanes, int npb, int nlog) {
+// Map this workgroup to (instance, tile) and this thread to its (y,z) column and x planes.
+// Returns nullptr for threads with no cell.  The member lookup is uniform per workgroup.
+template <typename T>
+__device__ __forceinline__ const InstDev<T>* locate(const InstDev<T>* __restrict__ insts,
+                                                     const MemberDev* __restrict__ mem, int nmem, int total,
+                                                     int* s_member, Tile& t) {
     const int L = xcd_logical(blockIdx.x, gridDim.x);
-    if (L >= nlog) return false;
-    const int pb = L % npb, xc = L / npb;
-    const int64_t NyNz = (int64_t)Ny * Nz;
+    if (threadIdx.x == 0) {
+        int m = 0;
+        if (L < total)
+            while (m + 1 < nmem && mem[m + 1].tile0 <= L) ++m;
+        *s_member = m;
+    }
+    __syncthreads();
+    if (L >= total) return nullptr;
+    // the member index comes through LDS: tell the compiler it is wave-uniform, so the member and
+    // instance descriptors are fetched with scalar loads into SGPRs
+    const int m = __builtin_amdgcn_readfirstlane(*s_member);
+    const MemberDev M = mem[m];
+    const int local = L - M.tile0;
+    const int e = local % M.K, tile = local / M.K;
+    const InstDev<T>* I = insts + __builtin_amdgcn_readfirstlane(M.inst0 + e);
+    const int Ny = I->Ny, Nz = I->Nz, npb = I->npb;
+    const int pb = tile % npb, xc = tile / npb;
     t.p = (int64_t)pb * blockDim.x + threadIdx.x;
-    if (t.p >= NyNz) return false;
     t.y = (int)(t.p / Nz);
     t.z = (int)(t.p - (int64_t)t.y * Nz);
-    t.x0 = xc * xplanes;
-    t.x1 = min(Nx, t.x0 + xplanes);
-    return true;
+    t.x0 = xc * I->xplanes;
+    // lanes past the last (y,z) column get an empty x range: the pointer stays wave-uniform
+    t.x1 = t.p < (int64_t)Ny * Nz ? min(I->Nx, t.x0 + I->xplanes) : t.x0;
+    return I;
 }
 
-// E half step over one column of x planes.
-template <typename T, class CA>
-__global__ void __launch_bounds__(1024) k_update_e(Fields<T> F, CA C, int Nx, int Ny, int Nz, int xplanes, int npb,
-                                                   int nlog) {
-    Tile t;
-    if (!tile_of(t, Nx, Ny, Nz, xplanes, npb, nlog)) return;
-    const int64_t NyNz = (int64_t)Ny * Nz;
+// Field and coefficient pointers reach the kernels through descriptors in memory, so the compiler
+// sees generic pointers; casting them to the global address space gives global (not flat) loads
+// with a scalar base and a 32-bit vector offset.
+#define GLOBAL_T(T) GLOBAL_AS T
+
+// Everything one x plane of the E update reads, so that the next plane's loads can be issued
+// before this plane's arithmetic and stores (PF = true: two planes of loads in flight per lane).
+template <typename T>
+struct EIn {
+    T hx_c, hy_xn, hz_xn, hz_yn, hx_yn, hy_zn, hx_zn;
+    T exy, exz, eyz, eyx, ezx, ezy;
+    ECoef<T> c;
+};
+
+template <typename T>
+struct HIn {
+    T ex_c, ey_c, ez_c, ez_ym, ex_ym, ey_zm, ex_zm;
+    T hxy, hxz, hyz, hyx, hzx, hzy;
+    HCoef<T> c;
+};
+
+// E half step over one column of x planes.  Offsets are 32-bit byte offsets (grids up to 2^29
+// cells).
+template <typename T, class CA, bool PF>
+__device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+    using G = GLOBAL_T(T);
+    G* const ex_y = (G*)F.ex_y; G* const ex_z = (G*)F.ex_z;
+    G* const ey_z = (G*)F.ey_z; G* const ey_x = (G*)F.ey_x;
+    G* const ez_x = (G*)F.ez_x; G* const ez_y = (G*)F.ez_y;
+    G* const ex = (G*)F.ex; G* const ey = (G*)F.ey; G* const ez = (G*)F.ez;
+    const G* const hx = (const G*)F.hx; const G* const hy = (const G*)F.hy; const G* const hz = (const G*)F.hz;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
     const bool yin = t.y < Ny - 1, zin = t.z < Nz - 1;
-    int64_t i = (int64_t)t.x0 * NyNz + t.p;
-    T hy_c = F.hy[i], hz_c = F.hz[i];
-    for (int x = t.x0; x < t.x1; ++x, i += NyNz) {
-        const bool xin = x < Nx - 1;
-        const T hx_c = F.hx[i];
-        const T hy_xn = xin ? F.hy[i + NyNz] : T(0);
-        const T hz_xn = xin ? F.hz[i + NyNz] : T(0);
-        const T hz_yn = yin ? F.hz[i + Nz] : T(0);
-        const T hx_yn = yin ? F.hx[i + Nz] : T(0);
-        const T hy_zn = zin ? F.hy[i + 1] : T(0);
-        const T hx_zn = zin ? F.hx[i + 1] : T(0);
-        const ECoef<T> c = C.e(i);
-        T exy = F.ex_y[i], exz = F.ex_z[i], eyz = F.ey_z[i], eyx = F.ey_x[i], ezx = F.ez_x[i], ezy = F.ez_y[i];
-        if (yin) exy = c.ca_xy * exy + c.cb_xy * (hz_yn - hz_c);
-        if (zin) exz = c.ca_xz * exz + c.cb_xz * (hy_zn - hy_c);
-        if (zin) eyz = c.ca_yz * eyz + c.cb_yz * (hx_zn - hx_c);
-        if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (hz_xn - hz_c);
-        if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (hy_xn - hy_c);
-        if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (hx_yn - hx_c);
-        F.ex_y[i] = exy; F.ex_z[i] = exz; F.ey_z[i] = eyz; F.ey_x[i] = eyx; F.ez_x[i] = ezx; F.ez_y[i] = ezy;
-        F.ex[i] = exy + exz;
-        F.ey[i] = eyz + eyx;
-        F.ez[i] = ezx + ezy;
-        hy_c = hy_xn;
-        hz_c = hz_xn;
+    auto load = [&](EIn<T>& v, uint32_t b, bool xin) {
+        v.hx_c = ldg(hx, b);
+        v.hy_xn = xin ? ldg(hy, b + sx) : T(0);
+        v.hz_xn = xin ? ldg(hz, b + sx) : T(0);
+        v.hz_yn = yin ? ldg(hz, b + sy) : T(0);
+        v.hx_yn = yin ? ldg(hx, b + sy) : T(0);
+        v.hy_zn = zin ? ldg(hy, b + S) : T(0);
+        v.hx_zn = zin ? ldg(hx, b + S) : T(0);
+        v.c = C.e(b);
+        v.exy = ldg(ex_y, b); v.exz = ldg(ex_z, b); v.eyz = ldg(ey_z, b);
+        v.eyx = ldg(ey_x, b); v.ezx = ldg(ez_x, b); v.ezy = ldg(ez_y, b);
+    };
+    if (t.x0 >= t.x1) return;
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    T hy_c = ldg(hy, b), hz_c = ldg(hz, b);
+    EIn<T> cur;
+    load(cur, b, t.x0 < Nx - 1);
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool xin = x < Nx - 1, more = x + 1 < t.x1;
+        EIn<T> nxt;
+        if (PF && more) load(nxt, b + sx, x + 1 < Nx - 1);
+        const ECoef<T>& c = cur.c;
+        T exy = cur.exy, exz = cur.exz, eyz = cur.eyz, eyx = cur.eyx, ezx = cur.ezx, ezy = cur.ezy;
+        if (yin) exy = c.ca_xy * exy + c.cb_xy * (cur.hz_yn - hz_c);
+        if (zin) exz = c.ca_xz * exz + c.cb_xz * (cur.hy_zn - hy_c);
+        if (zin) eyz = c.ca_yz * eyz + c.cb_yz * (cur.hx_zn - cur.hx_c);
+        if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (cur.hz_xn - hz_c);
+        if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (cur.hy_xn - hy_c);
+        if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (cur.hx_yn - cur.hx_c);
+        stg(ex_y, b, exy); stg(ex_z, b, exz); stg(ey_z, b, eyz);
+        stg(ey_x, b, eyx); stg(ez_x, b, ezx); stg(ez_y, b, ezy);
+        stg(ex, b, exy + exz);
+        stg(ey, b, eyz + eyx);
+        stg(ez, b, ezx + ezy);
+        hy_c = cur.hy_xn;
+        hz_c = cur.hz_xn;
+        if (PF)
+            cur = nxt;
+        else if (more)
+            load(cur, b + sx, x + 1 < Nx - 1);
     }
 }
 
 // H half step over one column of x planes; neighbours below index 0 are zero.
-template <typename T, class CA>
-__global__ void __launch_bounds__(1024) k_update_h(Fields<T> F, CA C, int Nx, int Ny, int Nz, int xplanes, int npb,
-                                                   int nlog) {
-    Tile t;
-    if (!tile_of(t, Nx, Ny, Nz, xplanes, npb, nlog)) return;
-    const int64_t NyNz = (int64_t)Ny * Nz;
+template <typename T, class CA, bool PF>
+__device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+    using G = GLOBAL_T(T);
+    G* const hx_y = (G*)F.hx_y; G* const hx_z = (G*)F.hx_z;
+    G* const hy_z = (G*)F.hy_z; G* const hy_x = (G*)F.hy_x;
+    G* const hz_x = (G*)F.hz_x; G* const hz_y = (G*)F.hz_y;
+    G* const hx = (G*)F.hx; G* const hy = (G*)F.hy; G* const hz = (G*)F.hz;
+    const G* const ex = (const G*)F.ex; const G* const ey = (const G*)F.ey; const G* const ez = (const G*)F.ez;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
     const bool ylo = t.y > 0, zlo = t.z > 0;
-    int64_t i = (int64_t)t.x0 * NyNz + t.p;
-    T ey_xm = t.x0 > 0 ? F.ey[i - NyNz] : T(0);
-    T ez_xm = t.x0 > 0 ? F.ez[i - NyNz] : T(0);
-    for (int x = t.x0; x < t.x1; ++x, i += NyNz) {
-        const T ex_c = F.ex[i], ey_c = F.ey[i], ez_c = F.ez[i];
-        const T ez_ym = ylo ? F.ez[i - Nz] : T(0);
-        const T ex_ym = ylo ? F.ex[i - Nz] : T(0);
-        const T ey_zm = zlo ? F.ey[i - 1] : T(0);
-        const T ex_zm = zlo ? F.ex[i - 1] : T(0);
-        const HCoef<T> c = C.h(i);
-        const T hxy = c.da_xy * F.hx_y[i] + (c.db_xy2 * ez_c - c.db_xy1 * ez_ym);
-        const T hxz = c.da_xz * F.hx_z[i] + (c.db_xz2 * ey_c - c.db_xz1 * ey_zm);
-        const T hyz = c.da_yz * F.hy_z[i] + (c.db_yz2 * ex_c - c.db_yz1 * ex_zm);
-        const T hyx = c.da_yx * F.hy_x[i] + (c.db_yx2 * ez_c - c.db_yx1 * ez_xm);
-        const T hzx = c.da_zx * F.hz_x[i] + (c.db_zx2 * ey_c - c.db_zx1 * ey_xm);
-        const T hzy = c.da_zy * F.hz_y[i] + (c.db_zy2 * ex_c - c.db_zy1 * ex_ym);
-        F.hx_y[i] = hxy; F.hx_z[i] = hxz; F.hy_z[i] = hyz; F.hy_x[i] = hyx; F.hz_x[i] = hzx; F.hz_y[i] = hzy;
-        F.hx[i] = hxy + hxz;
-        F.hy[i] = hyz + hyx;
-        F.hz[i] = hzx + hzy;
-        ey_xm = ey_c;
-        ez_xm = ez_c;
+    (void)Nx;
+    auto load = [&](HIn<T>& v, uint32_t b) {
+        v.ex_c = ldg(ex, b); v.ey_c = ldg(ey, b); v.ez_c = ldg(ez, b);
+        v.ez_ym = ylo ? ldg(ez, b - sy) : T(0);
+        v.ex_ym = ylo ? ldg(ex, b - sy) : T(0);
+        v.ey_zm = zlo ? ldg(ey, b - S) : T(0);
+        v.ex_zm = zlo ? ldg(ex, b - S) : T(0);
+        v.c = C.h(b);
+        v.hxy = ldg(hx_y, b); v.hxz = ldg(hx_z, b); v.hyz = ldg(hy_z, b);
+        v.hyx = ldg(hy_x, b); v.hzx = ldg(hz_x, b); v.hzy = ldg(hz_y, b);
+    };
+    if (t.x0 >= t.x1) return;
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    T ey_xm = t.x0 > 0 ? ldg(ey, b - sx) : T(0);
+    T ez_xm = t.x0 > 0 ? ldg(ez, b - sx) : T(0);
+    HIn<T> cur;
+    load(cur, b);
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool more = x + 1 < t.x1;
+        HIn<T> nxt;
+        if (PF && more) load(nxt, b + sx);
+        const HCoef<T>& c = cur.c;
+        const T hxy = c.da_xy * cur.hxy + (c.db_xy2 * cur.ez_c - c.db_xy1 * cur.ez_ym);
+        const T hxz = c.da_xz * cur.hxz + (c.db_xz2 * cur.ey_c - c.db_xz1 * cur.ey_zm);
+        const T hyz = c.da_yz * cur.hyz + (c.db_yz2 * cur.ex_c - c.db_yz1 * cur.ex_zm);
+        const T hyx = c.da_yx * cur.hyx + (c.db_yx2 * cur.ez_c - c.db_yx1 * ez_xm);
+        const T hzx = c.da_zx * cur.hzx + (c.db_zx2 * cur.ey_c - c.db_zx1 * ey_xm);
+        const T hzy = c.da_zy * cur.hzy + (c.db_zy2 * cur.ex_c - c.db_zy1 * cur.ex_ym);
+        stg(hx_y, b, hxy); stg(hx_z, b, hxz); stg(hy_z, b, hyz);
+        stg(hy_x, b, hyx); stg(hz_x, b, hzx); stg(hz_y, b, hzy);
+        stg(hx, b, hxy + hxz);
+        stg(hy, b, hyz + hyx);
+        stg(hz, b, hzx + hzy);
+        ey_xm = cur.ey_c;
+        ez_xm = cur.ez_c;
+        if (PF)
+            cur = nxt;
+        else if (more)
+            load(cur, b + sx);
     }
 }
 
-// Probes sharing one (component, cell) form a group processed by one thread in list order.
+template <typename T, bool PF>
+__global__ void __launch_bounds__(1024) k_update_e(const InstDev<T>* __restrict__ insts,
+                                                   const MemberDev* __restrict__ mem, int nmem, int total) {
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
+    if (!I) return;
+    if (I->coef_kind == 1)
+        e_column<T, CoefTable<T>, PF>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+    else
+        e_column<T, CoefRaw<T>, PF>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+}
+
+template <typename T, bool PF>
+__global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict__ insts,
+                                                   const MemberDev* __restrict__ mem, int nmem, int total) {
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
+    if (!I) return;
+    if (I->coef_kind == 1)
+        h_column<T, CoefTable<T>, PF>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+    else
+        h_column<T, CoefRaw<T>, PF>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+}
+
+// Probes sharing one (component, cell) of one instance form a group processed by one thread in
+// list order.  series is the instance's time-major [Nt][nprobes] array.
+template <typename T>
 struct ProbeGroup {
     int64_t cell;
-    int32_t field;
-    int32_t start;
-    int32_t count;
-    int32_t pad;
+    T* series;
+    const int32_t* order;   // instance-local probe ids of the group, in list order
+    const int32_t* is_src;  // per instance-local probe id
+    int32_t inst, field, count, nprobes;
 };
 
 __device__ __forceinline__ int step_of(const int* base, int local) { return base ? (*base + local) : local; }
 
 template <typename T>
-__global__ void k_probes(Fields<T> F, const ProbeGroup* groups, int ngroups, const int32_t* order,
-                         const int32_t* is_source, T* series, int nprobes, const int* base, int local) {
+__global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<T>* __restrict__ groups,
+                         int ngroups, const int* base, int local) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
     const int n = step_of(base, local);
-    const ProbeGroup G = groups[g];
+    const ProbeGroup<T> G = groups[g];
+    const Fields<T>& F = insts[G.inst].F;
     T *tot, *s1, *s2;
     switch (G.field) {
         case EX: tot = F.ex; s1 = F.ex_y; s2 = F.ex_z; break;
@@ -272,11 +426,11 @@ __global__ void k_probes(Fields<T> F, const ProbeGroup* groups, int ngroups, con
         default: tot = F.hz; s1 = F.hz_x; s2 = F.hz_y; break;
     }
     T a = s1[G.cell], b = s2[G.cell], s = tot[G.cell];
-    T* row = series + (size_t)n * nprobes;
+    T* row = G.series + (size_t)n * G.nprobes;
     for (int k = 0; k < G.count; ++k) {
-        const int pid = order[G.start + k];
+        const int pid = G.order[k];
         const T v = row[pid];
-        if (is_source[pid]) {
+        if (G.is_src[pid]) {
             a = a + v;
             b = b + v;
             s = a + b;
@@ -289,15 +443,15 @@ __global__ void k_probes(Fields<T> F, const ProbeGroup* groups, int ngroups, con
 }
 
 struct MonitorDev {
-    int32_t axis, pos, field, n_step, nf, N1, N2, blk0;
+    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, pad;
     int64_t N1N2;
     void* values;
     const void* phase;
 };
 
 template <typename T>
-__global__ void k_monitors(Fields<T> F, const MonitorDev* mons, int nmon, int Nx, int Ny, int Nz, const int* base,
-                           int local) {
+__global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDev* __restrict__ mons, int nmon,
+                           const int* base, int local) {
     __shared__ int sm;
     if (threadIdx.x == 0) {
         int m = 0;
@@ -305,11 +459,13 @@ __global__ void k_monitors(Fields<T> F, const MonitorDev* mons, int nmon, int Nx
         sm = m;
     }
     __syncthreads();
-    const MonitorDev M = mons[sm];
+    const MonitorDev M = mons[__builtin_amdgcn_readfirstlane(sm)];
     const int n = step_of(base, local);
     if (n > 0 && (n % M.n_step) != 0) return;
     const int64_t q = (int64_t)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
     if (q >= M.N1N2) return;
+    const InstDev<T>& I = insts[M.inst];
+    const int Ny = I.Ny, Nz = I.Nz;
     const int a = (int)(q / M.N2), b = (int)(q - (int64_t)a * M.N2);
     int64_t idx;
     if (M.axis == 0)
@@ -320,12 +476,12 @@ __global__ void k_monitors(Fields<T> F, const MonitorDev* mons, int nmon, int Nx
         idx = ((int64_t)a * Ny + b) * Nz + M.pos;
     const T* src;
     switch (M.field) {
-        case EX: src = F.ex; break;
-        case EY: src = F.ey; break;
-        case EZ: src = F.ez; break;
-        case HX: src = F.hx; break;
-        case HY: src = F.hy; break;
-        default: src = F.hz; break;
+        case EX: src = I.F.ex; break;
+        case EY: src = I.F.ey; break;
+        case EZ: src = I.F.ez; break;
+        case HX: src = I.F.hx; break;
+        case HY: src = I.F.hy; break;
+        default: src = I.F.hz; break;
     }
     const T v = src[idx];
     const int mn = n / M.n_step;
@@ -360,25 +516,16 @@ __global__ void __launch_bounds__(256) k_stream_copy(float4* __restrict__ dst, c
 // host side
 // ------------------------------------------------------------------------------------------------
 
-struct MonitorHost {
-    rfdtd_monitor desc;
-    bool active;
-    int N1, N2, Nm, shifted;
-    void* d_values;
-    void* d_phase;
-};
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-struct Session {
+// A grid and its coefficients, shared by the member's instances.
+struct Member {
     int Nx = 0, Ny = 0, Nz = 0;
     int64_t cells = 0;
-    bool f64 = false;
-    size_t esz = 4;
-    uint32_t flags = 0;
-
-    hipStream_t own_stream = nullptr;
-    void* d_fields = nullptr;  // 18 arrays of cells
-    void* d_coef = nullptr;    // raw arrays or index + tables
-    size_t fields_bytes = 0, coef_bytes = 0;
+    int K = 1, inst0 = 0;
+    int xplanes_req = 0, xplanes = 0;  // requested (0 = automatic) and effective planes per workgroup
+    void* d_coef = nullptr;
+    size_t coef_bytes = 0;
     int coef_kind = 0;  // 0 raw, 1 table
     int64_t nE = 0, nH = 0;
     const void* raw[30] = {};
@@ -386,24 +533,59 @@ struct Session {
     const uint32_t* hidx = nullptr;
     const void* etab = nullptr;
     const void* htab = nullptr;
+};
 
+struct MonitorHost {
+    rfdtd_monitor desc;
+    bool active = false;
+    int N1 = 0, N2 = 0, Nm = 0, pos = 0;
+    void* d_values = nullptr;
+    void* d_phase = nullptr;
+};
+
+// One excitation of one member: its fields, probe series and monitors.
+struct Instance {
+    int member = 0;
+    void* d_fields = nullptr;
+    size_t fields_bytes = 0, pitch = 0;
     // probes
-    int P = 0, Nt_probes = 0;
+    int P = 0;
     void* d_series = nullptr;
     int32_t* d_order = nullptr;
     int32_t* d_is_src = nullptr;
-    ProbeGroup* d_groups = nullptr;
-    int ngE = 0, ngH = 0;  // groups [0, ngE) are E, [ngE, ngE+ngH) are H
     std::vector<void*> probe_host_values;
-
+    struct HostGroup {
+        int64_t cell;
+        int field, start, count;
+    };
+    std::vector<HostGroup> groups;  // E groups first, then H
+    int ngE = 0;
     // monitors
     std::vector<MonitorHost> mons;
+};
+
+struct Session {
+    bool f64 = false;
+    size_t esz = 4;
+    uint32_t flags = 0;
+    hipStream_t own_stream = nullptr;
+    std::vector<Member> members;
+    std::vector<Instance> insts;
+    int Nt_probes = 0, Nt_monitors = 0;
+
+    // device tables rebuilt by refresh()
+    void* d_insts = nullptr;
+    MemberDev* d_members = nullptr;
+    void* d_groups = nullptr;  // E groups of all instances, then H groups
+    int ngE = 0, ngH = 0;
     MonitorDev* d_mons = nullptr;
     int n_active = 0, mon_blocks = 0;
-    int Nt_monitors = 0;
+    int total_tiles = 0;
+    bool dirty = true;
 
     int64_t step = 0;
-    int block = 256, xplanes = 0, graph_steps = 100;
+    int block = 256, graph_steps = 100;
+    bool pipelined = true;  // issue the next x plane's loads before this plane's stores
 
     int* d_step_base = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -417,15 +599,14 @@ struct Session {
     int n_timed = 0;
 
     ~Session();
-    void release_probes();
-    void release_monitors();
     void drop_graph();
-    size_t field_pitch() const { return ((size_t)cells * esz + 255) & ~(size_t)255; }
+    void release_probes(Instance& I);
+    void release_monitors(Instance& I);
     template <typename T>
-    Fields<T> fields() const {
+    Fields<T> fields(const Instance& I) const {
         Fields<T> f;
         T** a = (T**)&f;
-        for (int k = 0; k < 18; ++k) a[k] = (T*)((char*)d_fields + (size_t)k * field_pitch());
+        for (int k = 0; k < 18; ++k) a[k] = (T*)((char*)I.d_fields + (size_t)k * I.pitch);
         return f;
     }
 };
@@ -438,34 +619,36 @@ void Session::drop_graph() {
     graph_dirty = true;
 }
 
-void Session::release_probes() {
-    if (d_series) (void)hipFree(d_series);
-    if (d_order) (void)hipFree(d_order);
-    if (d_is_src) (void)hipFree(d_is_src);
-    if (d_groups) (void)hipFree(d_groups);
-    d_series = nullptr; d_order = nullptr; d_is_src = nullptr; d_groups = nullptr;
-    P = 0; ngE = ngH = 0; Nt_probes = 0;
-    probe_host_values.clear();
+void Session::release_probes(Instance& I) {
+    if (I.d_series) (void)hipFree(I.d_series);
+    if (I.d_order) (void)hipFree(I.d_order);
+    if (I.d_is_src) (void)hipFree(I.d_is_src);
+    I.d_series = nullptr; I.d_order = nullptr; I.d_is_src = nullptr;
+    I.P = 0; I.ngE = 0;
+    I.groups.clear();
+    I.probe_host_values.clear();
 }
 
-void Session::release_monitors() {
-    for (auto& m : mons) {
+void Session::release_monitors(Instance& I) {
+    for (auto& m : I.mons) {
         if (m.d_values) (void)hipFree(m.d_values);
         if (m.d_phase) (void)hipFree(m.d_phase);
     }
-    mons.clear();
-    if (d_mons) (void)hipFree(d_mons);
-    d_mons = nullptr;
-    n_active = mon_blocks = 0;
+    I.mons.clear();
 }
 
 Session::~Session() {
+    (void)hipDeviceSynchronize();
     drop_graph();
-    release_probes();
-    release_monitors();
-    if (d_fields) (void)hipFree(d_fields);
-    if (d_coef) (void)hipFree(d_coef);
-    if (d_step_base) (void)hipFree(d_step_base);
+    for (auto& I : insts) {
+        release_probes(I);
+        release_monitors(I);
+        if (I.d_fields) (void)hipFree(I.d_fields);
+    }
+    for (auto& M : members)
+        if (M.d_coef) (void)hipFree(M.d_coef);
+    for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base})
+        if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (auto e : kev) (void)hipEventDestroy(e);
@@ -496,14 +679,15 @@ bool dedup(const U* const* arrays, const bool* zero_last_x, int64_t cells, int64
     int64_t count = 0;
     const int64_t last0 = cells - plane;
     U key[K];
+    auto hash = [](const U* k) {
+        uint64_t h = 0x9e3779b97f4a7c15ULL;
+        for (int j = 0; j < K; ++j) h = mix64(h ^ (uint64_t)k[j]) + (uint64_t)j;
+        return h;
+    };
     for (int64_t i = 0; i < cells; ++i) {
         const bool last = i >= last0;
-        uint64_t h = 0x9e3779b97f4a7c15ULL;
-        for (int k = 0; k < K; ++k) {
-            key[k] = (last && zero_last_x[k]) ? U(0) : arrays[k][i];
-            h = mix64(h ^ (uint64_t)key[k]) + (uint64_t)k;
-        }
-        size_t s = h & (cap - 1);
+        for (int k = 0; k < K; ++k) key[k] = (last && zero_last_x[k]) ? U(0) : arrays[k][i];
+        size_t s = hash(key) & (cap - 1);
         int32_t found = -1;
         while (slots[s] >= 0) {
             const U* e = &table[(size_t)slots[s] * K];
@@ -522,9 +706,7 @@ bool dedup(const U* const* arrays, const bool* zero_last_x, int64_t cells, int64
                 cap <<= 1;
                 std::vector<int32_t> ns(cap, -1);
                 for (int64_t e = 0; e < count; ++e) {
-                    uint64_t hh = 0x9e3779b97f4a7c15ULL;
-                    for (int k = 0; k < K; ++k) hh = mix64(hh ^ (uint64_t)table[(size_t)e * K + k]) + (uint64_t)k;
-                    size_t t = hh & (cap - 1);
+                    size_t t = hash(&table[(size_t)e * K]) & (cap - 1);
                     while (ns[t] >= 0) t = (t + 1) & (cap - 1);
                     ns[t] = (int32_t)e;
                 }
@@ -540,8 +722,8 @@ bool dedup(const U* const* arrays, const bool* zero_last_x, int64_t cells, int64
 bool zero_last_plane(int k) { return (k >= 4 && k <= 11) || (k >= 12 && k <= 17); }
 
 template <typename T>
-void upload_coefficients(Session& s, const void* const coeff[30]) {
-    const int64_t cells = s.cells, plane = (int64_t)s.Ny * s.Nz;
+void upload_coefficients(Member& M, uint32_t flags, const void* const coeff[30]) {
+    const int64_t cells = M.cells, plane = (int64_t)M.Ny * M.Nz;
     using U = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
     const U* arr[30];
     bool zl[30];
@@ -553,44 +735,43 @@ void upload_coefficients(Session& s, const void* const coeff[30]) {
     bool use_table = false;
     std::vector<uint32_t> eidx, hidx;
     std::vector<U> etab, htab;
-    if (!(s.flags & RFDTD_COEF_RAW)) {
+    if (!(flags & RFDTD_COEF_RAW)) {
         // A table is worth it while its entries cost well under the raw arrays.
-        const int64_t limit = (s.flags & RFDTD_COEF_TABLE) ? cells : std::max<int64_t>(4096, cells / 16);
+        const int64_t limit = (flags & RFDTD_COEF_TABLE) ? cells : std::max<int64_t>(4096, cells / 8);
         use_table = dedup<U, 12>(arr, zl, cells, plane, limit, eidx, etab) &&
                     dedup<U, 18>(arr + 12, zl + 12, cells, plane, limit, hidx, htab);
-        if (!use_table && (s.flags & RFDTD_COEF_TABLE))
+        if (!use_table && (flags & RFDTD_COEF_TABLE))
             RF_THROW(RFDTD_EINVAL, "coefficient table exceeds its entry limit");
     }
     if (use_table) {
-        s.coef_kind = 1;
-        s.nE = (int64_t)etab.size() / 12;
-        s.nH = (int64_t)htab.size() / 18;
+        M.coef_kind = 1;
+        M.nE = (int64_t)etab.size() / 12;
+        M.nH = (int64_t)htab.size() / 18;
         const size_t idx_bytes = (size_t)cells * 4;
-        const size_t etab_bytes = (size_t)s.nE * 12 * sizeof(T);
-        const size_t htab_bytes = (size_t)s.nH * 20 * sizeof(T);
-        auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-        s.coef_bytes = 2 * al(idx_bytes) + al(etab_bytes) + al(htab_bytes);
-        HIP_OK(hipMalloc(&s.d_coef, s.coef_bytes));
-        char* b = (char*)s.d_coef;
-        s.eidx = (const uint32_t*)b;
-        s.hidx = (const uint32_t*)(b + al(idx_bytes));
-        s.etab = b + 2 * al(idx_bytes);
-        s.htab = b + 2 * al(idx_bytes) + al(etab_bytes);
-        HIP_OK(hipMemcpy((void*)s.eidx, eidx.data(), idx_bytes, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy((void*)s.hidx, hidx.data(), idx_bytes, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy((void*)s.etab, etab.data(), etab_bytes, hipMemcpyHostToDevice));
-        std::vector<U> hpad((size_t)s.nH * 20, U(0));
-        for (int64_t e = 0; e < s.nH; ++e)
+        const size_t etab_bytes = (size_t)M.nE * 12 * sizeof(T);
+        const size_t htab_bytes = (size_t)M.nH * 20 * sizeof(T);
+        M.coef_bytes = 2 * align256(idx_bytes) + align256(etab_bytes) + align256(htab_bytes);
+        HIP_OK(hipMalloc(&M.d_coef, M.coef_bytes));
+        char* b = (char*)M.d_coef;
+        M.eidx = (const uint32_t*)b;
+        M.hidx = (const uint32_t*)(b + align256(idx_bytes));
+        M.etab = b + 2 * align256(idx_bytes);
+        M.htab = b + 2 * align256(idx_bytes) + align256(etab_bytes);
+        HIP_OK(hipMemcpy((void*)M.eidx, eidx.data(), idx_bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy((void*)M.hidx, hidx.data(), idx_bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy((void*)M.etab, etab.data(), etab_bytes, hipMemcpyHostToDevice));
+        std::vector<U> hpad((size_t)M.nH * 20, U(0));
+        for (int64_t e = 0; e < M.nH; ++e)
             std::memcpy(&hpad[(size_t)e * 20], &htab[(size_t)e * 18], 18 * sizeof(U));
-        HIP_OK(hipMemcpy((void*)s.htab, hpad.data(), htab_bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy((void*)M.htab, hpad.data(), htab_bytes, hipMemcpyHostToDevice));
     } else {
-        s.coef_kind = 0;
-        const size_t arr_bytes = ((size_t)cells * sizeof(T) + 255) & ~(size_t)255;
-        s.coef_bytes = 30 * arr_bytes;
-        HIP_OK(hipMalloc(&s.d_coef, s.coef_bytes));
+        M.coef_kind = 0;
+        const size_t arr_bytes = align256((size_t)cells * sizeof(T));
+        M.coef_bytes = 30 * arr_bytes;
+        HIP_OK(hipMalloc(&M.d_coef, M.coef_bytes));
         for (int k = 0; k < 30; ++k) {
-            char* dst = (char*)s.d_coef + (size_t)k * arr_bytes;
-            s.raw[k] = dst;
+            char* dst = (char*)M.d_coef + (size_t)k * arr_bytes;
+            M.raw[k] = dst;
             HIP_OK(hipMemcpy(dst, coeff[k], (size_t)cells * sizeof(T), hipMemcpyHostToDevice));
             if (zl[k])
                 HIP_OK(hipMemset(dst + (size_t)(cells - plane) * sizeof(T), 0, (size_t)plane * sizeof(T)));
@@ -598,81 +779,164 @@ void upload_coefficients(Session& s, const void* const coeff[30]) {
     }
 }
 
-void choose_launch(Session& s) {
-    // x planes per workgroup: enough workgroups to give every CU several waves (>= 2048 groups of
-    // 256 threads on 256 CUs), as many planes as possible for register reuse along x.
-    if (s.xplanes > 0) return;
-    const int64_t NyNz = (int64_t)s.Ny * s.Nz;
-    const int64_t npb = (NyNz + s.block - 1) / s.block;
+// x planes per workgroup: enough workgroups to give every CU several waves (>= ~2048 groups in a
+// launch on 256 CUs), as many planes as possible for register reuse along x.
+void plan_launch(Session& s) {
+    int64_t blocks_per_plane = 0;
+    for (auto& M : s.members) blocks_per_plane += M.K * (((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
     const int64_t want = 2048;
-    int64_t chunks = std::max<int64_t>(1, (want + npb - 1) / npb);
-    chunks = std::min<int64_t>(chunks, s.Nx);
-    s.xplanes = (int)((s.Nx + chunks - 1) / chunks);
+    for (auto& M : s.members) {
+        if (M.xplanes_req > 0) {
+            M.xplanes = std::min(M.xplanes_req, M.Nx);
+            continue;
+        }
+        int64_t chunks = std::max<int64_t>(1, (want + blocks_per_plane - 1) / blocks_per_plane);
+        chunks = std::min<int64_t>(chunks, M.Nx);
+        M.xplanes = (int)((M.Nx + chunks - 1) / chunks);
+    }
 }
 
-struct Launch {
-    int grid, npb, nlog;
-};
+template <typename T>
+void refresh(Session& s) {
+    if (!s.dirty) return;
+    plan_launch(s);
+    // instance descriptors
+    std::vector<InstDev<T>> di(s.insts.size());
+    std::vector<MemberDev> dm(s.members.size());
+    int tile0 = 0;
+    for (size_t m = 0; m < s.members.size(); ++m) {
+        Member& M = s.members[m];
+        const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
+        const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
+        dm[m] = MemberDev{tile0, npb * chunks, M.K, M.inst0};
+        tile0 += npb * chunks * M.K;
+        for (int e = 0; e < M.K; ++e) {
+            InstDev<T>& d = di[M.inst0 + e];
+            std::memset(&d, 0, sizeof(d));
+            d.F = s.fields<T>(s.insts[M.inst0 + e]);
+            for (int k = 0; k < 30; ++k) d.raw[k] = (const T*)M.raw[k];
+            d.eidx = M.eidx;
+            d.hidx = M.hidx;
+            d.etab = (const Vec4<T>*)M.etab;
+            d.htab = (const Vec4<T>*)M.htab;
+            d.coef_kind = M.coef_kind;
+            d.Nx = M.Nx; d.Ny = M.Ny; d.Nz = M.Nz;
+            d.xplanes = M.xplanes;
+            d.npb = npb;
+        }
+    }
+    s.total_tiles = tile0;
+    if (s.d_insts) (void)hipFree(s.d_insts);
+    if (s.d_members) (void)hipFree(s.d_members);
+    s.d_insts = nullptr;
+    s.d_members = nullptr;
+    HIP_OK(hipMalloc(&s.d_insts, di.size() * sizeof(InstDev<T>)));
+    HIP_OK(hipMemcpy(s.d_insts, di.data(), di.size() * sizeof(InstDev<T>), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&s.d_members, dm.size() * sizeof(MemberDev)));
+    HIP_OK(hipMemcpy(s.d_members, dm.data(), dm.size() * sizeof(MemberDev), hipMemcpyHostToDevice));
 
-Launch launch_shape(const Session& s) {
-    const int64_t NyNz = (int64_t)s.Ny * s.Nz;
-    Launch L;
-    L.npb = (int)((NyNz + s.block - 1) / s.block);
-    const int chunks = (s.Nx + s.xplanes - 1) / s.xplanes;
-    L.nlog = L.npb * chunks;
-    L.grid = (L.nlog + 7) & ~7;
-    return L;
+    // probe groups of all instances: E groups, then H groups
+    std::vector<ProbeGroup<T>> ge, gh;
+    for (size_t k = 0; k < s.insts.size(); ++k) {
+        const Instance& I = s.insts[k];
+        for (size_t g = 0; g < I.groups.size(); ++g) {
+            const auto& hg = I.groups[g];
+            ProbeGroup<T> pg{};
+            pg.cell = hg.cell;
+            pg.series = (T*)I.d_series;
+            pg.order = I.d_order + hg.start;
+            pg.is_src = I.d_is_src;
+            pg.inst = (int32_t)k;
+            pg.field = hg.field;
+            pg.count = hg.count;
+            pg.nprobes = I.P;
+            ((int)g < I.ngE ? ge : gh).push_back(pg);
+        }
+    }
+    s.ngE = (int)ge.size();
+    s.ngH = (int)gh.size();
+    ge.insert(ge.end(), gh.begin(), gh.end());
+    if (s.d_groups) (void)hipFree(s.d_groups);
+    s.d_groups = nullptr;
+    if (!ge.empty()) {
+        HIP_OK(hipMalloc(&s.d_groups, ge.size() * sizeof(ProbeGroup<T>)));
+        HIP_OK(hipMemcpy(s.d_groups, ge.data(), ge.size() * sizeof(ProbeGroup<T>), hipMemcpyHostToDevice));
+    }
+
+    // monitors of all instances
+    std::vector<MonitorDev> md;
+    int blocks = 0;
+    for (size_t k = 0; k < s.insts.size(); ++k)
+        for (const MonitorHost& h : s.insts[k].mons) {
+            if (!h.active) continue;
+            MonitorDev d{};
+            d.inst = (int32_t)k;
+            d.axis = h.desc.axis; d.pos = h.pos; d.field = h.desc.field; d.n_step = h.desc.n_step;
+            d.nf = h.desc.n_frequencies; d.N1 = h.N1; d.N2 = h.N2;
+            d.N1N2 = (int64_t)h.N1 * h.N2;
+            d.blk0 = blocks;
+            d.values = h.d_values;
+            d.phase = h.d_phase;
+            blocks += (int)((d.N1N2 + 255) / 256);
+            md.push_back(d);
+        }
+    s.n_active = (int)md.size();
+    s.mon_blocks = blocks;
+    if (s.d_mons) (void)hipFree(s.d_mons);
+    s.d_mons = nullptr;
+    if (!md.empty()) {
+        HIP_OK(hipMalloc(&s.d_mons, md.size() * sizeof(MonitorDev)));
+        HIP_OK(hipMemcpy(s.d_mons, md.data(), md.size() * sizeof(MonitorDev), hipMemcpyHostToDevice));
+    }
+    s.dirty = false;
+    s.drop_graph();
 }
 
 template <typename T>
 void launch_step(Session& s, hipStream_t st, const int* base, int local, bool monitors_now,
                  hipEvent_t* ev = nullptr) {
-    const Launch L = launch_shape(s);
-    const Fields<T> F = s.fields<T>();
+    const int grid = (s.total_tiles + 7) & ~7;
+    const InstDev<T>* di = (const InstDev<T>*)s.d_insts;
+    const int nmem = (int)s.members.size();
+    const ProbeGroup<T>* groups = (const ProbeGroup<T>*)s.d_groups;
     if (ev) HIP_OK(hipEventRecord(ev[0], st));
-    if (s.coef_kind == 1) {
-        CoefTable<T> C{s.eidx, s.hidx, (const Vec4<T>*)s.etab, (const Vec4<T>*)s.htab};
-        k_update_e<T, CoefTable<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
-    } else {
-        CoefRaw<T> C;
-        for (int k = 0; k < 30; ++k) C.c[k] = (const T*)s.raw[k];
-        k_update_e<T, CoefRaw<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
-    }
+    if (s.pipelined)
+        k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+    else
+        k_update_e<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
     if (ev) HIP_OK(hipEventRecord(ev[1], st));
-    if (s.ngE)
-        k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(F, s.d_groups, s.ngE, s.d_order, s.d_is_src, (T*)s.d_series,
-                                                      s.P, base, local);
+    if (s.ngE) k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(di, groups, s.ngE, base, local);
     if (ev) HIP_OK(hipEventRecord(ev[2], st));
-    if (s.coef_kind == 1) {
-        CoefTable<T> C{s.eidx, s.hidx, (const Vec4<T>*)s.etab, (const Vec4<T>*)s.htab};
-        k_update_h<T, CoefTable<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
-    } else {
-        CoefRaw<T> C;
-        for (int k = 0; k < 30; ++k) C.c[k] = (const T*)s.raw[k];
-        k_update_h<T, CoefRaw<T>><<<L.grid, s.block, 0, st>>>(F, C, s.Nx, s.Ny, s.Nz, s.xplanes, L.npb, L.nlog);
-    }
+    if (s.pipelined)
+        k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+    else
+        k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
     if (ev) HIP_OK(hipEventRecord(ev[3], st));
-    if (s.ngH)
-        k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(F, s.d_groups + s.ngE, s.ngH, s.d_order, s.d_is_src,
-                                                      (T*)s.d_series, s.P, base, local);
+    if (s.ngH) k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local);
     if (s.n_active && monitors_now)
-        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(F, s.d_mons, s.n_active, s.Nx, s.Ny, s.Nz, base, local);
+        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local);
 }
 
 bool any_monitor_samples(const Session& s, int64_t n) {
-    for (const auto& m : s.mons)
-        if (m.active && (n == 0 || n % m.desc.n_step == 0)) return true;
+    for (const auto& I : s.insts)
+        for (const auto& m : I.mons)
+            if (m.active && (n == 0 || n % m.desc.n_step == 0)) return true;
     return false;
 }
 
 template <typename T>
 void run_steps(Session& s, int n_steps, hipStream_t st) {
     if (n_steps < 0) RF_THROW(RFDTD_EINVAL, "negative step count");
-    if (s.P && s.step + n_steps > s.Nt_probes)
+    bool has_probes = false, has_mons = false;
+    for (const auto& I : s.insts) {
+        has_probes |= I.P > 0;
+        for (const auto& m : I.mons) has_mons |= m.active;
+    }
+    if (has_probes && s.step + n_steps > s.Nt_probes)
         RF_THROW(RFDTD_ESTATE, "run past the end of the probe series (Nt = " + std::to_string(s.Nt_probes) + ")");
-    if (s.n_active && s.step + n_steps > s.Nt_monitors)
+    if (has_mons && s.step + n_steps > s.Nt_monitors)
         RF_THROW(RFDTD_ESTATE, "run past the monitor length (Nt = " + std::to_string(s.Nt_monitors) + ")");
-    choose_launch(s);
+    refresh<T>(s);
     HIP_OK(hipEventRecord(s.ev0, st));
     const bool timing = (s.flags & RFDTD_KERNEL_TIMING) != 0;
     const bool graphs = !timing && !(s.flags & RFDTD_NO_GRAPHS) && s.graph_steps > 1;
@@ -682,8 +946,7 @@ void run_steps(Session& s, int n_steps, hipStream_t st) {
     int done = 0;
     if (graphs && n_steps >= s.graph_steps) {
         if (s.graph_dirty || !s.graph) {
-            if (s.graph) { (void)hipGraphExecDestroy(s.graph); s.graph = nullptr; }
-            if (s.graph_def) { (void)hipGraphDestroy(s.graph_def); s.graph_def = nullptr; }
+            s.drop_graph();
             hipStream_t cs;
             HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
             HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
@@ -717,13 +980,14 @@ void run_steps(Session& s, int n_steps, hipStream_t st) {
     s.step += n_steps;
 }
 
-Session* make_session(int Nx, int Ny, int Nz, const void* const coeff[30], uint32_t flags) {
-    if (Nx < 2 || Ny < 2 || Nz < 2) RF_THROW(RFDTD_EINVAL, "grid must have at least 2 cells on every axis");
-    if ((int64_t)Nx * Ny * Nz > ((int64_t)1 << 32) - 1) RF_THROW(RFDTD_EINVAL, "grid too large");
-    if (!coeff) RF_THROW(RFDTD_EINVAL, "coefficients missing");
+struct MemberSpec {
+    int Nx, Ny, Nz, K;
+    const void* const* coeff;
+};
+
+Session* make_session(const std::vector<MemberSpec>& specs, uint32_t flags) {
+    if (specs.empty()) RF_THROW(RFDTD_EINVAL, "a session needs at least one grid");
     auto s = std::make_unique<Session>();
-    s->Nx = Nx; s->Ny = Ny; s->Nz = Nz;
-    s->cells = (int64_t)Nx * Ny * Nz;
     s->f64 = (flags & RFDTD_F64) != 0;
     s->esz = s->f64 ? 8 : 4;
     s->flags = flags;
@@ -731,23 +995,52 @@ Session* make_session(int Nx, int Ny, int Nz, const void* const coeff[30], uint3
     HIP_OK(hipEventCreate(&s->ev0));
     HIP_OK(hipEventCreate(&s->ev1));
     HIP_OK(hipMalloc(&s->d_step_base, 256));
-    s->fields_bytes = 18 * (((size_t)s->cells * s->esz + 255) & ~(size_t)255);
-    HIP_OK(hipMalloc(&s->d_fields, s->fields_bytes));
-    HIP_OK(hipMemset(s->d_fields, 0, s->fields_bytes));
-    if (s->f64)
-        upload_coefficients<double>(*s, coeff);
-    else
-        upload_coefficients<float>(*s, coeff);
+    int inst0 = 0;
+    for (const MemberSpec& sp : specs) {
+        if (sp.Nx < 2 || sp.Ny < 2 || sp.Nz < 2) RF_THROW(RFDTD_EINVAL, "grid must have at least 2 cells on every axis");
+        // 32-bit element offsets in the kernels; byte offsets of double arrays must fit too
+        if ((int64_t)sp.Nx * sp.Ny * sp.Nz > ((int64_t)1 << 29)) RF_THROW(RFDTD_EINVAL, "grid too large (max 2^29 cells)");
+        if (sp.K < 1 || sp.K > 64) RF_THROW(RFDTD_EINVAL, "excitations per grid must be in [1, 64]");
+        if (!sp.coeff) RF_THROW(RFDTD_EINVAL, "coefficients missing");
+        Member M;
+        M.Nx = sp.Nx; M.Ny = sp.Ny; M.Nz = sp.Nz;
+        M.cells = (int64_t)sp.Nx * sp.Ny * sp.Nz;
+        M.K = sp.K;
+        M.inst0 = inst0;
+        s->members.push_back(M);
+        Member& Mr = s->members.back();
+        if (s->f64) upload_coefficients<double>(Mr, flags, sp.coeff); else upload_coefficients<float>(Mr, flags, sp.coeff);
+        for (int e = 0; e < sp.K; ++e) {
+            Instance I;
+            I.member = (int)s->members.size() - 1;
+            I.pitch = align256((size_t)M.cells * s->esz);
+            I.fields_bytes = 18 * I.pitch;
+            HIP_OK(hipMalloc(&I.d_fields, I.fields_bytes));
+            HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
+            s->insts.push_back(std::move(I));
+        }
+        inst0 += sp.K;
+    }
     return s.release();
 }
 
-void set_probes(Session& s, const rfdtd_probe* probes, int n, int Nt) {
+Instance& instance_at(Session& s, int k) {
+    if (k < 0 || k >= (int)s.insts.size()) RF_THROW(RFDTD_EINVAL, "instance index out of range");
+    return s.insts[k];
+}
+
+void set_probes(Session& s, int inst, const rfdtd_probe* probes, int n, int Nt) {
     if (n < 0 || n > RFDTD_MAX_PROBES) RF_THROW(RFDTD_EINVAL, "Exceeded maximum number of probes.");
     if (n > 0 && (!probes || Nt < 1)) RF_THROW(RFDTD_EINVAL, "invalid probe arguments");
+    Instance& I = instance_at(s, inst);
+    const Member& M = s.members[I.member];
     (void)hipDeviceSynchronize();
-    s.drop_graph();
-    s.release_probes();
+    s.release_probes(I);
+    s.dirty = true;
     if (n == 0) return;
+    bool other = false;
+    for (const auto& J : s.insts) other |= (&J != &I) && J.P > 0;
+    if (other && Nt != s.Nt_probes) RF_THROW(RFDTD_EINVAL, "all probe series of a session must have the same Nt");
     // component shifts of solver.cpp:503-520
     std::vector<int64_t> cell(n);
     std::vector<int32_t> field(n), is_src(n);
@@ -761,27 +1054,25 @@ void set_probes(Session& s, const rfdtd_probe* probes, int n, int Nt) {
         if (f == EX || f == EZ || f == HY) y -= 1;
         if (f == EX || f == EY || f == HZ) z -= 1;
         if (x < 0 || y < 0 || z < 0) RF_THROW(RFDTD_EINVAL, "Source cannot be placed on grid edge");
-        if (x >= s.Nx || y >= s.Ny || z >= s.Nz) RF_THROW(RFDTD_EINVAL, "probe index outside the grid");
-        cell[p] = ((int64_t)x * s.Ny + y) * s.Nz + z;
+        if (x >= M.Nx || y >= M.Ny || z >= M.Nz) RF_THROW(RFDTD_EINVAL, "probe index outside the grid");
+        cell[p] = ((int64_t)x * M.Ny + y) * M.Nz + z;
         field[p] = f;
         is_src[p] = q.is_source ? 1 : 0;
     }
     // groups by (component, cell) in order of first appearance; E groups first
-    std::vector<ProbeGroup> groups;
     std::vector<int32_t> order;
     for (int half = 0; half < 2; ++half) {
         std::unordered_map<int64_t, int> gid;
         std::vector<std::vector<int32_t>> members;
-        std::vector<ProbeGroup> gs;
+        std::vector<Instance::HostGroup> gs;
         for (int p = 0; p < n; ++p) {
             if ((field[p] >= 3) != (half == 1)) continue;
             const int64_t key = cell[p] * 8 + field[p];
             auto it = gid.find(key);
             if (it == gid.end()) {
-                gid.emplace(key, (int)gs.size());
-                gs.push_back(ProbeGroup{cell[p], field[p], 0, 0, 0});
+                it = gid.emplace(key, (int)gs.size()).first;
+                gs.push_back(Instance::HostGroup{cell[p], field[p], 0, 0});
                 members.emplace_back();
-                it = gid.find(key);
             }
             members[it->second].push_back(p);
         }
@@ -790,10 +1081,10 @@ void set_probes(Session& s, const rfdtd_probe* probes, int n, int Nt) {
             gs[g].count = (int32_t)members[g].size();
             order.insert(order.end(), members[g].begin(), members[g].end());
         }
-        if (half == 0) s.ngE = (int)gs.size(); else s.ngH = (int)gs.size();
-        groups.insert(groups.end(), gs.begin(), gs.end());
+        if (half == 0) I.ngE = (int)gs.size();
+        I.groups.insert(I.groups.end(), gs.begin(), gs.end());
     }
-    s.P = n;
+    I.P = n;
     s.Nt_probes = Nt;
     // time-major series [Nt][P]
     std::vector<char> series((size_t)Nt * n * s.esz);
@@ -801,33 +1092,35 @@ void set_probes(Session& s, const rfdtd_probe* probes, int n, int Nt) {
         const char* src = (const char*)probes[p].values;
         for (int t = 0; t < Nt; ++t)
             std::memcpy(&series[((size_t)t * n + p) * s.esz], src + (size_t)t * s.esz, s.esz);
-        s.probe_host_values.push_back(probes[p].values);
+        I.probe_host_values.push_back(probes[p].values);
     }
-    HIP_OK(hipMalloc(&s.d_series, series.size()));
-    HIP_OK(hipMemcpy(s.d_series, series.data(), series.size(), hipMemcpyHostToDevice));
-    HIP_OK(hipMalloc(&s.d_order, order.size() * 4));
-    HIP_OK(hipMemcpy(s.d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
-    HIP_OK(hipMalloc(&s.d_is_src, (size_t)n * 4));
-    HIP_OK(hipMemcpy(s.d_is_src, is_src.data(), (size_t)n * 4, hipMemcpyHostToDevice));
-    HIP_OK(hipMalloc(&s.d_groups, groups.size() * sizeof(ProbeGroup)));
-    HIP_OK(hipMemcpy(s.d_groups, groups.data(), groups.size() * sizeof(ProbeGroup), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&I.d_series, series.size()));
+    HIP_OK(hipMemcpy(I.d_series, series.data(), series.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&I.d_order, order.size() * 4));
+    HIP_OK(hipMemcpy(I.d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&I.d_is_src, (size_t)n * 4));
+    HIP_OK(hipMemcpy(I.d_is_src, is_src.data(), (size_t)n * 4, hipMemcpyHostToDevice));
 }
 
-void set_monitors(Session& s, const rfdtd_monitor* mons, int n, int Nt) {
+void set_monitors(Session& s, int inst, const rfdtd_monitor* mons, int n, int Nt) {
     if (n < 0 || n > RFDTD_MAX_MONITORS) RF_THROW(RFDTD_EINVAL, "Exceeded maximum number of monitors.");
     if (n > 0 && (!mons || Nt < 1)) RF_THROW(RFDTD_EINVAL, "invalid monitor arguments");
+    Instance& I = instance_at(s, inst);
+    const Member& M = s.members[I.member];
     (void)hipDeviceSynchronize();
-    s.drop_graph();
-    s.release_monitors();
+    s.release_monitors(I);
+    s.dirty = true;
+    if (n == 0) return;
+    bool other = false;
+    for (const auto& J : s.insts) other |= (&J != &I) && !J.mons.empty();
+    if (other && Nt != s.Nt_monitors) RF_THROW(RFDTD_EINVAL, "all monitors of a session must have the same Nt");
     s.Nt_monitors = Nt;
-    std::vector<MonitorDev> dev;
-    int blocks = 0;
     for (int m = 0; m < n; ++m) {
         const rfdtd_monitor& d = mons[m];
         if (d.axis < 0 || d.axis > 2 || d.field < 0 || d.field > 5 || d.n_step < 1 || d.n_frequencies < 0)
             RF_THROW(RFDTD_EINVAL, "invalid monitor description");
         if (!d.values || (d.n_frequencies > 0 && !d.dtft_phase)) RF_THROW(RFDTD_EINVAL, "monitor arrays missing");
-        MonitorHost h{};
+        MonitorHost h;
         h.desc = d;
         const int f = d.field;
         int pos = d.position;
@@ -835,10 +1128,10 @@ void set_monitors(Session& s, const rfdtd_monitor* mons, int n, int Nt) {
         if (d.axis == 0 && (f == HX || f == EY || f == EZ)) pos -= 1;
         if (d.axis == 1 && (f == HY || f == EX || f == EZ)) pos -= 1;
         if (d.axis == 2 && (f == HZ || f == EX || f == EY)) pos -= 1;
-        h.shifted = pos;
-        const int along = d.axis == 0 ? s.Nx : d.axis == 1 ? s.Ny : s.Nz;
-        h.N1 = d.axis == 0 ? s.Ny : s.Nx;
-        h.N2 = d.axis == 2 ? s.Ny : s.Nz;
+        h.pos = pos;
+        const int along = d.axis == 0 ? M.Nx : d.axis == 1 ? M.Ny : M.Nz;
+        h.N1 = d.axis == 0 ? M.Ny : M.Nx;
+        h.N2 = d.axis == 2 ? M.Ny : M.Nz;
         h.Nm = Nt / d.n_step + 1;
         h.active = pos >= 0 && pos < along;
         const size_t plane = (size_t)h.N1 * h.N2;
@@ -850,56 +1143,46 @@ void set_monitors(Session& s, const rfdtd_monitor* mons, int n, int Nt) {
             HIP_OK(hipMalloc(&h.d_phase, pbytes));
             HIP_OK(hipMemcpy(h.d_phase, d.dtft_phase, pbytes, hipMemcpyHostToDevice));
         }
-        if (h.active) {
-            MonitorDev md{};
-            md.axis = d.axis; md.pos = pos; md.field = f; md.n_step = d.n_step; md.nf = d.n_frequencies;
-            md.N1 = h.N1; md.N2 = h.N2; md.N1N2 = (int64_t)plane; md.blk0 = blocks;
-            md.values = h.d_values; md.phase = h.d_phase;
-            blocks += (int)((plane + 255) / 256);
-            dev.push_back(md);
-        }
-        s.mons.push_back(h);
-    }
-    s.n_active = (int)dev.size();
-    s.mon_blocks = blocks;
-    if (!dev.empty()) {
-        HIP_OK(hipMalloc(&s.d_mons, dev.size() * sizeof(MonitorDev)));
-        HIP_OK(hipMemcpy(s.d_mons, dev.data(), dev.size() * sizeof(MonitorDev), hipMemcpyHostToDevice));
+        I.mons.push_back(h);
     }
 }
 
 void read_probes(Session& s) {
-    if (!s.P) return;
     HIP_OK(hipDeviceSynchronize());
-    std::vector<char> series((size_t)s.Nt_probes * s.P * s.esz);
-    HIP_OK(hipMemcpy(series.data(), s.d_series, series.size(), hipMemcpyDeviceToHost));
-    for (int p = 0; p < s.P; ++p) {
-        char* dst = (char*)s.probe_host_values[p];
-        for (int t = 0; t < s.Nt_probes; ++t)
-            std::memcpy(dst + (size_t)t * s.esz, &series[((size_t)t * s.P + p) * s.esz], s.esz);
+    for (Instance& I : s.insts) {
+        if (!I.P) continue;
+        std::vector<char> series((size_t)s.Nt_probes * I.P * s.esz);
+        HIP_OK(hipMemcpy(series.data(), I.d_series, series.size(), hipMemcpyDeviceToHost));
+        for (int p = 0; p < I.P; ++p) {
+            char* dst = (char*)I.probe_host_values[p];
+            for (int t = 0; t < s.Nt_probes; ++t)
+                std::memcpy(dst + (size_t)t * s.esz, &series[((size_t)t * I.P + p) * s.esz], s.esz);
+        }
     }
+}
+
+size_t monitor_bytes(const Session& s, const MonitorHost& h) {
+    const size_t plane = (size_t)h.N1 * h.N2;
+    return h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * s.esz : (size_t)h.Nm * plane * s.esz;
 }
 
 void read_monitors(Session& s) {
     HIP_OK(hipDeviceSynchronize());
-    for (auto& h : s.mons) {
-        const size_t plane = (size_t)h.N1 * h.N2;
-        const size_t vbytes = h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * s.esz
-                                                   : (size_t)h.Nm * plane * s.esz;
-        HIP_OK(hipMemcpy(h.desc.values, h.d_values, vbytes, hipMemcpyDeviceToHost));
-    }
+    for (Instance& I : s.insts)
+        for (auto& h : I.mons) HIP_OK(hipMemcpy(h.desc.values, h.d_values, monitor_bytes(s, h), hipMemcpyDeviceToHost));
 }
 
-void read_fields(Session& s, void* const out[18]) {
+void read_fields(Session& s, int inst, void* const out[18]) {
     HIP_OK(hipDeviceSynchronize());
-    const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
-    const size_t pitch = ((size_t)s.cells * s.esz + 255) & ~(size_t)255;
-    std::vector<char> buf((size_t)s.cells * s.esz);
+    Instance& I = instance_at(s, inst);
+    const Member& M = s.members[I.member];
+    const int Nx = M.Nx, Ny = M.Ny, Nz = M.Nz;
+    std::vector<char> buf((size_t)M.cells * s.esz);
     // component of each of the 18 arrays and the offset of GPU cell (0,0,0) in the global array
     const int comp[18] = {EX, EX, EY, EY, EZ, EZ, HX, HX, HY, HY, HZ, HZ, EX, EY, EZ, HX, HY, HZ};
     for (int a = 0; a < 18; ++a) {
         if (!out[a]) continue;
-        HIP_OK(hipMemcpy(buf.data(), (char*)s.d_fields + (size_t)a * pitch, buf.size(), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(buf.data(), (char*)I.d_fields + (size_t)a * I.pitch, buf.size(), hipMemcpyDeviceToHost));
         const int c = comp[a];
         const int gx = Nx + ((c == EY || c == EZ || c == HX) ? 1 : 0);
         const int gy = Ny + ((c == EX || c == EZ || c == HY) ? 1 : 0);
@@ -943,6 +1226,11 @@ int guarded(Fn&& fn) {
     }
 }
 
+Session& S_of(rfdtd_session* s) {
+    if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+    return *(Session*)s;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -960,28 +1248,45 @@ int rfdtd_create(rfdtd_session** out, int32_t Nx, int32_t Ny, int32_t Nz, const 
     return guarded([&] {
         if (!out) RF_THROW(RFDTD_EINVAL, "null session pointer");
         *out = nullptr;
-        *out = (rfdtd_session*)make_session(Nx, Ny, Nz, coefficients, flags);
+        *out = (rfdtd_session*)make_session({MemberSpec{Nx, Ny, Nz, 1, coefficients}}, flags);
     });
 }
 
-int rfdtd_set_probes(rfdtd_session* s, const rfdtd_probe* probes, int32_t n_probes, int32_t Nt) {
+int rfdtd_create_batch(rfdtd_session** out, int32_t n_grids, const int32_t* dims, const int32_t* excitations,
+                       const void* const* coefficients, uint32_t flags) {
     return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
-        set_probes(*(Session*)s, probes, n_probes, Nt);
+        if (!out || !dims || !excitations || !coefficients || n_grids < 1)
+            RF_THROW(RFDTD_EINVAL, "invalid batch arguments");
+        *out = nullptr;
+        std::vector<MemberSpec> specs;
+        for (int g = 0; g < n_grids; ++g)
+            specs.push_back(MemberSpec{dims[3 * g], dims[3 * g + 1], dims[3 * g + 2], excitations[g],
+                                       coefficients + (size_t)30 * g});
+        *out = (rfdtd_session*)make_session(specs, flags);
     });
+}
+
+int32_t rfdtd_instance_count(const rfdtd_session* s) { return s ? (int32_t)((const Session*)s)->insts.size() : 0; }
+
+int rfdtd_set_probes(rfdtd_session* s, const rfdtd_probe* probes, int32_t n_probes, int32_t Nt) {
+    return guarded([&] { set_probes(S_of(s), 0, probes, n_probes, Nt); });
 }
 
 int rfdtd_set_monitors(rfdtd_session* s, const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nt) {
+    return guarded([&] { set_monitors(S_of(s), 0, monitors, n_monitors, Nt); });
+}
+
+int rfdtd_set_instance_sources(rfdtd_session* s, int32_t instance, const rfdtd_probe* probes, int32_t n_probes,
+                               const rfdtd_monitor* monitors, int32_t n_monitors, int32_t Nt) {
     return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
-        set_monitors(*(Session*)s, monitors, n_monitors, Nt);
+        set_probes(S_of(s), instance, probes, n_probes, Nt);
+        set_monitors(S_of(s), instance, monitors, n_monitors, Nt);
     });
 }
 
 int rfdtd_run(rfdtd_session* s, int32_t n_steps, void* stream) {
     return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
-        Session& S = *(Session*)s;
+        Session& S = S_of(s);
         hipStream_t st = stream ? (hipStream_t)stream : S.own_stream;
         if (S.f64) run_steps<double>(S, n_steps, st); else run_steps<float>(S, n_steps, st);
     });
@@ -989,9 +1294,8 @@ int rfdtd_run(rfdtd_session* s, int32_t n_steps, void* stream) {
 
 int rfdtd_synchronize(rfdtd_session* s) {
     return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
+        Session& S = S_of(s);
         HIP_OK(hipDeviceSynchronize());
-        Session& S = *(Session*)s;
         float ms = 0;
         if (hipEventElapsedTime(&ms, S.ev0, S.ev1) == hipSuccess) S.last_total_ms = ms;
         double se = 0, sh = 0;
@@ -1008,37 +1312,34 @@ int rfdtd_synchronize(rfdtd_session* s) {
 }
 
 int rfdtd_read_probes(rfdtd_session* s) {
-    return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
-        read_probes(*(Session*)s);
-    });
+    return guarded([&] { read_probes(S_of(s)); });
 }
 
 int rfdtd_read_monitors(rfdtd_session* s) {
-    return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
-        read_monitors(*(Session*)s);
-    });
+    return guarded([&] { read_monitors(S_of(s)); });
 }
 
 int rfdtd_read_fields(rfdtd_session* s, void* const out[18]) {
     return guarded([&] {
-        if (!s || !out) RF_THROW(RFDTD_EINVAL, "null argument");
-        read_fields(*(Session*)s, out);
+        if (!out) RF_THROW(RFDTD_EINVAL, "null argument");
+        read_fields(S_of(s), 0, out);
+    });
+}
+
+int rfdtd_read_instance_fields(rfdtd_session* s, int32_t instance, void* const out[18]) {
+    return guarded([&] {
+        if (!out) RF_THROW(RFDTD_EINVAL, "null argument");
+        read_fields(S_of(s), instance, out);
     });
 }
 
 int rfdtd_reset(rfdtd_session* s) {
     return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
-        Session& S = *(Session*)s;
+        Session& S = S_of(s);
         HIP_OK(hipDeviceSynchronize());
-        HIP_OK(hipMemset(S.d_fields, 0, S.fields_bytes));
-        for (auto& h : S.mons) {
-            const size_t plane = (size_t)h.N1 * h.N2;
-            const size_t vbytes = h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * S.esz
-                                                       : (size_t)h.Nm * plane * S.esz;
-            HIP_OK(hipMemset(h.d_values, 0, vbytes));
+        for (Instance& I : S.insts) {
+            HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
+            for (auto& h : I.mons) HIP_OK(hipMemset(h.d_values, 0, monitor_bytes(S, h)));
         }
         S.step = 0;
     });
@@ -1061,38 +1362,52 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
     return guarded([&] {
         if (!s) RF_THROW(RFDTD_EINVAL, "null session");
         const Session& S = *(const Session*)s;
-        if (device_bytes) *device_bytes = S.fields_bytes + S.coef_bytes;
-        if (coef_kind) *coef_kind = S.coef_kind;
-        if (n_e_entries) *n_e_entries = S.nE;
-        if (n_h_entries) *n_h_entries = S.nH;
+        uint64_t b = 0;
+        int64_t ne = 0, nh = 0;
+        for (const auto& M : S.members) {
+            b += M.coef_bytes;
+            ne += M.nE;
+            nh += M.nH;
+        }
+        for (const auto& I : S.insts) b += I.fields_bytes;
+        if (device_bytes) *device_bytes = b;
+        if (coef_kind) *coef_kind = S.members[0].coef_kind;
+        if (n_e_entries) *n_e_entries = ne;
+        if (n_h_entries) *n_h_entries = nh;
     });
 }
 
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps) {
     return guarded([&] {
-        if (!s) RF_THROW(RFDTD_EINVAL, "null session");
-        Session& S = *(Session*)s;
+        Session& S = S_of(s);
         if (block_threads) {
             if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
                 RF_THROW(RFDTD_EINVAL, "block_threads must be a multiple of 64 in [64, 1024]");
             S.block = block_threads;
-            S.xplanes = 0;
         }
         if (x_planes) {
             if (x_planes < 0) RF_THROW(RFDTD_EINVAL, "x_planes must be positive");
-            S.xplanes = std::min<int>(x_planes, S.Nx);
+            for (auto& M : S.members) M.xplanes_req = x_planes;
         }
         if (graph_steps) {
             if (graph_steps < 0) RF_THROW(RFDTD_EINVAL, "graph_steps must be positive");
             S.graph_steps = graph_steps;
         }
+        S.dirty = true;
+        S.graph_dirty = true;
+    });
+}
+
+int rfdtd_set_variant(rfdtd_session* s, int32_t pipelined) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        S.pipelined = pipelined != 0;
         S.graph_dirty = true;
     });
 }
 
 void rfdtd_destroy(rfdtd_session* s) {
     if (!s) return;
-    (void)hipDeviceSynchronize();
     delete (Session*)s;
 }
 
@@ -1135,9 +1450,9 @@ int rfdtd_solver_run(const void* const coefficients[30], const rfdtd_probe* prob
     Session* s = nullptr;
     const int rc = guarded([&] {
         if (Nt < 1) RF_THROW(RFDTD_EINVAL, "Nt must be positive");
-        s = make_session(Nx, Ny, Nz, coefficients, flags);
-        set_probes(*s, probes, n_probes, Nt);
-        set_monitors(*s, monitors, n_monitors, Nt);
+        s = make_session({MemberSpec{Nx, Ny, Nz, 1, coefficients}}, flags);
+        set_probes(*s, 0, probes, n_probes, Nt);
+        set_monitors(*s, 0, monitors, n_monitors, Nt);
         const bool show = update_interval > 0 && !(flags & RFDTD_QUIET);
         if (show) progress_bar(0, Nt);
         // progress is reported per chunk of update_interval steps, as the reference's controller does

@@ -102,6 +102,10 @@ class Session:
     def set_launch(self, block_threads=0, x_planes=0, graph_steps=0):
         check(self._L.rfdtd_set_launch(self._h, int(block_threads), int(x_planes), int(graph_steps)))
 
+    def set_variant(self, pipelined=1):
+        """Kernel variant: 1 issues the next x plane's loads before the current plane's stores."""
+        check(self._L.rfdtd_set_variant(self._h, int(pipelined)))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             self._L.rfdtd_destroy(self._h)

@@ -693,11 +693,12 @@ class FDTD_Solver:
         )
         return {k: np.array(v, order="C", dtype=dtype_) for k, v in c.items()}
 
-    def pack(self, gpu: bool = True, dtype=np.float32, n_threads: int = 4):
+    def pack(self, gpu: bool = True, dtype=np.float32, n_threads: int = 4, coefficients: bool = True):
         """Solver inputs in the reference's calling convention (solver.py:1439-1622).
 
         Returns (coefficients, probes, monitors, mem, Nx, Ny, Nz, Nt).  gpu=True gives the GPU packing
-        (all arrays (Nx, Ny, Nz)); gpu=False the CPU packing of the reference's threaded solver."""
+        (all arrays (Nx, Ny, Nz)); gpu=False the CPU packing of the reference's threaded solver.
+        coefficients=False skips building the coefficient arrays (returned as None)."""
         self.check_mesh()
         srcs = [p["src"] for p in self.ports if p is not None and p["src"] is not None]
         if not srcs:
@@ -706,7 +707,7 @@ class FDTD_Solver:
         if not all(Nt == len(s) for s in srcs):
             raise ValueError("Excitations must have identical lengths.")
         dtype_ = np.dtype(dtype)
-        coefficients = self.coefficient_arrays(gpu=gpu, dtype=dtype_)
+        coefficients = self.coefficient_arrays(gpu=gpu, dtype=dtype_) if coefficients else None
         Nx, Ny, Nz = self.n_cells
         mem_len = sum(3 * int(np.prod((Nx,) + s[1:])) for s in self.fshape.values())
         mem_len += (4 * (Ny + 1) * (Nz + 1)) * n_threads * 8
