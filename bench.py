@@ -85,10 +85,102 @@ def cpu_baseline(model, budget_s):
                        f"per step, -O2 -ffp-contract=off), {dt:.1f} s wall")
 
 
+def batched_leg(n_steps):
+    """The 4 uneven port variants x every port driven (12 excitations) advanced in one batched
+    session, against the same 12 excitations run one after another through single-grid sessions.
+    Both paths are prepared (coefficients, sources resident) before their timed runs; results must
+    agree bit for bit."""
+    from rfnetwork_amd import batch as rb, workloads
+    from rfnetwork_amd.session import Session
+
+    t0 = time.perf_counter()
+    models = [workloads.port_variant(v, Nt=n_steps) for v in range(len(workloads.PORT_VARIANTS))]
+    grids, inputs = [], []
+    for m in models:
+        ports = list(range(1, len(m.ports) + 1))
+        exc, Nt = rb.pack_excitations(m, ports)
+        coeff = m.coefficient_arrays(gpu=True)
+        grids.append((coeff, m.Nx, m.Ny, m.Nz, len(ports)))
+        inputs.append((m, coeff, exc, Nt))
+    exc_cells = sum(m.Nx * m.Ny * m.Nz * len(e) for m, _, e, _ in inputs)
+    log(f"batched: {len(models)} grids, {sum(len(e) for _, _, e, _ in inputs)} excitations, "
+        f"{exc_cells} excitation-cells, built in {time.perf_counter() - t0:.1f}s")
+
+    b = rb.Batch(grids)
+    log(f"batched: autotune {b.autotune()}")
+    k = 0
+    for _, _, exc, Nt in inputs:
+        for _, probes, monitors in exc:
+            b.set_sources(k, probes, monitors, Nt)
+            k += 1
+    t0 = time.perf_counter()
+    b.run(n_steps)
+    b.synchronize()
+    wall_b = time.perf_counter() - t0
+    b.read()
+    b.close()
+    batched_series = [[np.array(p["values"]) for p in probes] for _, _, exc, _ in inputs for _, probes, _ in exc]
+    log(f"batched: {n_steps} steps in {wall_b:.3f}s")
+
+    sessions = []
+    for m, coeff, exc, Nt in inputs:
+        for port in range(len(exc)):
+            exc2, _ = rb.pack_excitations(m, [port + 1])
+            _, probes, monitors = exc2[0]
+            s = Session(coeff, m.Nx, m.Ny, m.Nz)
+            s.autotune()
+            s.set_sources(probes, monitors, Nt)
+            sessions.append((s, probes))
+    t0 = time.perf_counter()
+    for s, _ in sessions:
+        s.run(n_steps)
+        s.synchronize()
+    wall_s = time.perf_counter() - t0
+    identical = True
+    for (s, probes), ref in zip(sessions, batched_series):
+        s.read()
+        identical &= all(np.array_equal(p["values"], r) for p, r in zip(probes, ref))
+        s.close()
+    log(f"sequential: {wall_s:.3f}s, identical {identical}")
+    b_rate = exc_cells * n_steps / wall_b / 1e6
+    s_rate = exc_cells * n_steps / wall_s / 1e6
+    return dict(workload="4 port variants 64^3..192x128x96, 12 excitations", steps=n_steps,
+                excitation_cells=exc_cells, batched_mcells=round(b_rate, 1), sequential_mcells=round(s_rate, 1),
+                speedup=round(wall_s / wall_b, 3), batched_s=round(wall_b, 3), sequential_s=round(wall_s, 3),
+                identical=bool(identical))
+
+
+def adjoint_leg(n_steps, every, repeats=2):
+    """dJ/deps_r of the 64x64x32 design box of the 200x160x80 two-port model, J = sum |S21|^2 at
+    2..6 GHz: one gradient = checkpointed forward + reverse pass with recomputation; the best of
+    `repeats` end-to-end evaluations (model assembly excluded, host objective included)."""
+    from rfnetwork_amd import adjoint, workloads
+
+    model = workloads.design_strip(Nt=n_steps)
+    cbox = workloads.design_box()
+    best = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        J, grad, S, t = adjoint.eps_r_gradient(model, cbox, model.frequency, checkpoint_every=every,
+                                               return_session_timing=True)
+        wall = time.perf_counter() - t0
+        if best is None or wall < best[0]:
+            best = (wall, t, J, grad)
+    wall, t, J, grad = best
+    cells = model.Nx * model.Ny * model.Nz
+    log(f"adjoint: {wall:.3f}s per gradient (forward {t['forward_ms']:.1f} ms, reverse {t['adjoint_ms']:.1f} ms)")
+    return dict(workload=f"design_strip {model.Nx}x{model.Ny}x{model.Nz}, design box 64x64x32, {n_steps} steps, "
+                         f"checkpoints every {every}", gradients_per_s=round(1.0 / wall, 4),
+                s_per_gradient=round(wall, 3), forward_ms=round(t["forward_ms"], 1),
+                reverse_ms=round(t["adjoint_ms"], 1), design_cells=int(grad.size), J=J,
+                forward_mcells=round(cells * n_steps / (t["forward_ms"] * 1e-3) / 1e6, 1),
+                finite=bool(np.isfinite(grad).all()))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dipole_slab")
     ap.add_argument("--coef", choices=("auto", "raw", "table"), default="auto")
@@ -96,7 +188,13 @@ def main():
     ap.add_argument("--xplanes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
     ap.add_argument("--no-copy-probe", action="store_true")
+    ap.add_argument("--legs", default="plain,batched,adjoint",
+                    help="comma list: plain (the headline), batched (12 excitations), adjoint (gradients/s)")
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--adjoint-steps", type=int, default=6000)
+    ap.add_argument("--checkpoint-every", type=int, default=200)
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
     if args.gpus != 1:
         print(json.dumps(dict(error="this engine runs on one GPU; --gpus above 1 is not supported",
                               n_gpus=args.gpus)))
@@ -116,8 +214,13 @@ def main():
     t0 = time.perf_counter()
     s = Session(c, Nx, Ny, Nz, flags=flags)
     del c
+    tuned = None
     if args.block or args.xplanes:
         s.set_launch(args.block, args.xplanes, 0)
+    else:
+        t1 = time.perf_counter()
+        tuned = s.autotune()
+        log(f"autotune {tuned} in {time.perf_counter() - t1:.1f}s")
     s.set_sources(probes, monitors, Nt)
     info = s.memory_info()
     log(f"session ready in {time.perf_counter() - t0:.1f}s: {info}")
@@ -155,20 +258,24 @@ def main():
         base = cpu_baseline(model, args.cpu_budget)
         log(f"cpu baseline {base['value']} Mcell/s on {base['cores']} threads")
 
+    batched = batched_leg(args.batch_steps) if "batched" in legs else None
+    adj = adjoint_leg(args.adjoint_steps, args.checkpoint_every) if "adjoint" in legs else None
+
     out = dict(
         metric=METRIC, value=round(mcells, 1), unit="Mcell-updates/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=round(ms_step, 4), higher_is_better=True, scaling="weak",
         vs_baseline=None, dtype="f32", data="synthetic",
         config=dict(workload=f"{args.workload} {model.Nx}x{model.Ny}x{model.Nz}", cells=cells,
                     dft_planes=len(model.monitors), dft_frequencies=51 if model.monitors else 0,
-                    coefficients=info["coef_kind"], coef_entries=[info["n_e_entries"], info["n_h_entries"]]),
+                    coefficients=info["coef_kind"], coef_entries=[info["n_e_entries"], info["n_h_entries"]],
+                    launch=tuned),
         roofline=dict(bound="hbm", kernel="k_update_h", achieved=round(h_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(h_gbs / HBM_PEAK_GBS, 4), traffic=None,
                       algorithmic_bytes_per_launch=cells * H_BYTES, launch_ms=round(h_ms, 4)),
         step=dict(hbm_frac_algorithmic=round(step_frac, 4), algorithmic_bytes=step_bytes,
                   e_kernel_ms=round(e_ms, 4), e_kernel_gbs=round(e_gbs, 1), h_kernel_ms=round(h_ms, 4),
                   dft_bytes=dft_b),
-        copy_probe=copy, cpu_baseline=base, finite=finite,
+        copy_probe=copy, cpu_baseline=base, finite=finite, batched=batched, adjoint=adj,
     )
     print(json.dumps(out), flush=True)
 

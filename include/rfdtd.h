@@ -141,9 +141,31 @@ int rfdtd_last_timing(const rfdtd_session* s, double* total_ms, double* e_kernel
 /* Device bytes held by the session, and coefficient storage kind (0 raw arrays, 1 table). */
 int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* coef_kind,
                       int64_t* n_e_entries, int64_t* n_h_entries);
-/* Launch shape: threads per block (64..1024, multiple of 64) and x planes per block (>= 1);
- * 0 keeps the current value. */
+/* Launch shape: threads per block (64..1024, multiple of 64) and x planes per block (>= 1, or -1 for
+ * the automatic choice that gives enough workgroups to fill the chip); 0 keeps the current value. */
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
+/* Launch autotuner (no counterpart in the reference, whose CUDA path uses one fixed launch shape):
+ * times every candidate {pipelined 1/0} x {block 128, 256, 512} x {x planes: automatic, Nx/2 .. Nx/32}
+ * over trial_steps field-only steps of this session's grids, keeps the fastest and returns it as
+ * config_out = {pipelined, block_threads, x_planes (0 = automatic)} with its ms per step.  Only on a
+ * session that has not run yet (step 0); fields stay zero, probes and monitors are untouched. */
+int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3], double* ms_per_step);
+/*
+ * Adjoint gradients (no counterpart in the reference, which has no differentiation).
+ * rfdtd_set_checkpointing(s, K): subsequent rfdtd_run calls save the 12 split fields of every
+ * instance before each step that is a multiple of K (one device copy per checkpoint).
+ * rfdtd_gradient: after a checkpointed forward run of Nt steps, runs the discrete adjoint of those
+ * steps backwards, recomputing each K-step segment from its checkpoint, and returns dJ/dCb for the
+ * six E splits of every cell of box = {x0, x1, y0, y1, z0, z1} (grid cells, upper bounds
+ * exclusive) as double[6][x1-x0][y1-y0][z1-z0] in split order ex_y ex_z ey_z ey_x ez_x ez_y, where
+ * Cb is the packed coefficient (reference dictionary entries 2, 3, 6, 7, 10, 11).
+ * adjoint_series[p] points to dJ/dy_p[n], n < Nt, for probe p of the instance (same order and dtype
+ * as its probes).  The forward fields are left untouched.
+ */
+int rfdtd_set_checkpointing(rfdtd_session* s, int32_t every);
+int rfdtd_gradient(rfdtd_session* s, int32_t instance, const int32_t box[6], const void* const* adjoint_series,
+                   int32_t n_series, double* grad_out);
+
 /* Kernel variant (no counterpart in the reference): pipelined != 0 (default) issues the loads of the
  * next x plane before the arithmetic and stores of the current one; 0 loads one plane at a time.
  * Results are identical; the choice only changes speed (the autotuner picks it). */

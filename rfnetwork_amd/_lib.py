@@ -48,6 +48,7 @@ EXPORTS = (
     "rfdtd_read_fields", "rfdtd_reset", "rfdtd_step", "rfdtd_last_timing", "rfdtd_memory_info",
     "rfdtd_set_launch", "rfdtd_destroy", "rfdtd_copy_bandwidth", "rfdtd_create_batch",
     "rfdtd_instance_count", "rfdtd_set_instance_sources", "rfdtd_read_instance_fields", "rfdtd_set_variant",
+    "rfdtd_set_checkpointing", "rfdtd_gradient", "rfdtd_autotune",
 )
 
 
@@ -79,6 +80,11 @@ class RfdtdError(RuntimeError):
 
 
 _lib = None
+
+
+def version() -> int:
+    """Version number of the loaded engine library."""
+    return int(lib().rfdtd_version())
 
 
 def lib() -> ctypes.CDLL:
@@ -119,6 +125,9 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_set_instance_sources.argtypes = [vp, i32, vp, i32, vp, i32, i32]
     L.rfdtd_read_instance_fields.argtypes = [vp, i32, vp]
     L.rfdtd_set_variant.argtypes = [vp, i32]
+    L.rfdtd_set_checkpointing.argtypes = [vp, i32]
+    L.rfdtd_gradient.argtypes = [vp, i32, vp, vp, i32, vp]
+    L.rfdtd_autotune.argtypes = [vp, i32, vp, ctypes.POINTER(ctypes.c_double)]
     L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]
     _lib = L

@@ -1,0 +1,83 @@
+"""Launch-shape autotuning with a per-shape cache.
+
+The engine's tuner (rfdtd_autotune) times every candidate kernel variant, block size and number of
+x planes per workgroup on the session's own grids and keeps the fastest.  This module keys the
+result by what determines it -- the grids (Nx, Ny, Nz, excitations), the element type and the
+coefficient storage -- and keeps it in memory and in a JSON file, so a shape is tuned once per
+machine.  The file is ``$RFNETWORK_AMD_TUNE_CACHE`` or ``~/.cache/rfnetwork_amd/launch.json``;
+failing to read or write it only costs a re-tune.
+"""
+import ctypes
+import json
+import os
+import pathlib
+
+from . import _lib
+from ._lib import check
+
+_memory = {}
+
+
+def cache_path():
+    p = os.environ.get("RFNETWORK_AMD_TUNE_CACHE")
+    return pathlib.Path(p) if p else pathlib.Path.home() / ".cache" / "rfnetwork_amd" / "launch.json"
+
+
+def shape_key(grids, dtype, coef_kind):
+    """grids: [(Nx, Ny, Nz, K)]; the key also holds the library version (kernels change with it)."""
+    g = ";".join(f"{nx}x{ny}x{nz}x{k}" for nx, ny, nz, k in grids)
+    return f"{_lib.version()}|{dtype}|coef{coef_kind}|{g}"
+
+
+def _load():
+    try:
+        return json.loads(cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key, cfg):
+    try:
+        path = cache_path()
+        path.parent.mkdir(parents=True, exist_ok=True)
+        data = _load()
+        data[key] = cfg
+        tmp = path.with_suffix(".tmp")
+        tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+        tmp.replace(path)
+    except OSError:
+        pass
+
+
+def run_tuner(session, trial_steps=20):
+    """Run the engine's tuner on a Session or Batch that has not stepped yet; the best shape is
+    left applied.  Returns {pipelined, block, x_planes (0 = automatic), ms_per_step}."""
+    cfg = (ctypes.c_int32 * 3)()
+    ms = ctypes.c_double()
+    check(session._L.rfdtd_autotune(session._h, int(trial_steps), ctypes.cast(cfg, ctypes.c_void_p),
+                                    ctypes.byref(ms)), "rfdtd_autotune")
+    return dict(pipelined=int(cfg[0]), block=int(cfg[1]), x_planes=int(cfg[2]), ms_per_step=float(ms.value))
+
+
+def apply(session, cfg):
+    session.set_variant(cfg["pipelined"])
+    session.set_launch(cfg["block"], cfg["x_planes"] if cfg["x_planes"] > 0 else -1, 0)
+
+
+def tune(session, grids, dtype, coef_kind, trial_steps=20, use_cache=True):
+    """Tuned launch shape of the session's grids: from the cache when known, else measured (and
+    cached).  The shape is applied to the session either way.  Returns the config with
+    cached=True/False."""
+    key = shape_key(grids, dtype, coef_kind)
+    cfg = _memory.get(key)
+    if cfg is None and use_cache:
+        cfg = _load().get(key)
+    if cfg is not None and use_cache:
+        apply(session, cfg)
+        _memory[key] = cfg
+        return dict(cfg, cached=True)
+    cfg = run_tuner(session, trial_steps)
+    _memory[key] = cfg
+    if use_cache:
+        _store(key, cfg)
+    return dict(cfg, cached=False)

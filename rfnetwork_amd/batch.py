@@ -93,6 +93,15 @@ class Batch:
     def set_launch(self, block_threads=0, x_planes=0, graph_steps=0):
         check(self._L.rfdtd_set_launch(self._h, int(block_threads), int(x_planes), int(graph_steps)))
 
+    def autotune(self, trial_steps=20, use_cache=True):
+        """Pick the fastest launch shape for this set of grids (before the first run); cached per
+        set of shapes.  Returns {pipelined, block, x_planes, ms_per_step, cached}."""
+        from . import autotune
+
+        kind = ctypes.c_int32()
+        check(self._L.rfdtd_memory_info(self._h, None, ctypes.byref(kind), None, None))
+        return autotune.tune(self, self.grids, np.dtype(self.dtype).name, kind.value, trial_steps, use_cache)
+
     def set_variant(self, pipelined=1):
         check(self._L.rfdtd_set_variant(self._h, int(pipelined)))
 

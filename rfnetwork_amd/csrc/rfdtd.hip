@@ -400,7 +400,8 @@ __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict_
 template <typename T>
 struct ProbeGroup {
     int64_t cell;
-    T* series;
+    T* series;              // in/out: source samples in, sampled totals out
+    const T* series_in;     // pristine copy of the source samples (recomputation)
     const int32_t* order;   // instance-local probe ids of the group, in list order
     const int32_t* is_src;  // per instance-local probe id
     int32_t inst, field, count, nprobes;
@@ -408,9 +409,10 @@ struct ProbeGroup {
 
 __device__ __forceinline__ int step_of(const int* base, int local) { return base ? (*base + local) : local; }
 
+// recompute = true replays a step from the pristine source samples and records nothing.
 template <typename T>
 __global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<T>* __restrict__ groups,
-                         int ngroups, const int* base, int local) {
+                         int ngroups, const int* base, int local, int recompute) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
     const int n = step_of(base, local);
@@ -427,19 +429,231 @@ __global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<
     }
     T a = s1[G.cell], b = s2[G.cell], s = tot[G.cell];
     T* row = G.series + (size_t)n * G.nprobes;
+    const T* row_in = recompute ? G.series_in + (size_t)n * G.nprobes : row;
     for (int k = 0; k < G.count; ++k) {
         const int pid = G.order[k];
-        const T v = row[pid];
+        const T v = row_in[pid];
         if (G.is_src[pid]) {
             a = a + v;
             b = b + v;
             s = a + b;
         }
-        row[pid] = s;
+        if (!recompute) row[pid] = s;
     }
     s1[G.cell] = a;
     s2[G.cell] = b;
     tot[G.cell] = s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// adjoint (reverse-time) kernels
+//
+// Forward step n (state e = 6 E splits, h = 6 H splits; totals are sums of splits):
+//   e' = Ca e + Cb D(H^n)          e'' = e' + sources          y_E = E''(probe cells)
+//   h' = Da h + G(E'')             h'' = h' + sources          y_H = H''(probe cells)
+// For an objective J(y) with g = dJ/dy, the reverse step n, given the split adjoints of the state
+// after the step (lambda_h in the H arrays; in the E arrays nu of step n+1, whose Ca scaling is
+// applied on reading), is:
+//   mu  = lambda_h + g_H (both splits of each probed H component)           k_adj_inject (H)
+//   nu  = Ca nu_old + S^T G^T mu                                             k_adj_e
+//   nu += g_E (both splits)                                                  k_adj_inject (E)
+//   dJ/dCb += nu * D(H^n)   (design box, forward H^n recomputed)             k_adj_grad
+//   lambda_h = Da mu + S^T D^T (Cb nu)                                       k_adj_h
+// where S^T copies a total's adjoint into both of its splits.  Split updates that the forward step
+// skips (upper edges of the GPU packing) pass their adjoint through unchanged.
+// ------------------------------------------------------------------------------------------------
+
+// Single coefficient k (0..29) of the cell at byte offset b.
+template <typename T>
+__device__ __forceinline__ T coef_at(const InstDev<T>& I, int k, uint32_t b) {
+    if (I.coef_kind == 1) {
+        const uint32_t S = (uint32_t)sizeof(T);
+        if (k < 12) {
+            const uint32_t e = ldg((const GLOBAL_AS uint32_t*)I.eidx, b / S * 4u);
+            return ldg((const GLOBAL_AS T*)I.etab, (e * 12u + (uint32_t)k) * S);
+        }
+        const uint32_t e = ldg((const GLOBAL_AS uint32_t*)I.hidx, b / S * 4u);
+        return ldg((const GLOBAL_AS T*)I.htab, (e * 20u + (uint32_t)(k - 12)) * S);
+    }
+    return ldg((const GLOBAL_AS T*)I.raw[k], b);
+}
+
+enum {  // coefficient positions
+    C_CA_XY = 0, C_CA_XZ, C_CB_XY, C_CB_XZ, C_CA_YZ, C_CA_YX, C_CB_YZ, C_CB_YX, C_CA_ZX, C_CA_ZY, C_CB_ZX, C_CB_ZY,
+    C_DA_XY, C_DA_XZ, C_DB_XY1, C_DB_XY2, C_DB_XZ1, C_DB_XZ2, C_DA_YZ, C_DA_YX, C_DB_YZ1, C_DB_YZ2, C_DB_YX1,
+    C_DB_YX2, C_DA_ZX, C_DA_ZY, C_DB_ZX1, C_DB_ZX2, C_DB_ZY1, C_DB_ZY2
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_e(const InstDev<T>* __restrict__ pA) {
+    const InstDev<T>& A = *pA;
+    const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz;
+    const uint32_t cells = (uint32_t)Nx * Ny * Nz, S = (uint32_t)sizeof(T);
+    const uint32_t sy = (uint32_t)Nz * S, sx = (uint32_t)Ny * Nz * S;
+    using G = GLOBAL_T(T);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
+        const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
+        const uint32_t b = i * S;
+        const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
+        const G* mxy = (const G*)A.F.hx_y; const G* mxz = (const G*)A.F.hx_z;
+        const G* myz = (const G*)A.F.hy_z; const G* myx = (const G*)A.F.hy_x;
+        const G* mzx = (const G*)A.F.hz_x; const G* mzy = (const G*)A.F.hz_y;
+        // E totals' adjoints: every H split that reads E at this cell (as e or as e- of a neighbour)
+        T ez = coef_at(A, C_DB_XY2, b) * ldg(mxy, b) + coef_at(A, C_DB_YX2, b) * ldg(myx, b);
+        if (yin) ez = ez - coef_at(A, C_DB_XY1, b + sy) * ldg(mxy, b + sy);
+        if (xin) ez = ez - coef_at(A, C_DB_YX1, b + sx) * ldg(myx, b + sx);
+        T ey = coef_at(A, C_DB_XZ2, b) * ldg(mxz, b) + coef_at(A, C_DB_ZX2, b) * ldg(mzx, b);
+        if (zin) ey = ey - coef_at(A, C_DB_XZ1, b + S) * ldg(mxz, b + S);
+        if (xin) ey = ey - coef_at(A, C_DB_ZX1, b + sx) * ldg(mzx, b + sx);
+        T ex = coef_at(A, C_DB_YZ2, b) * ldg(myz, b) + coef_at(A, C_DB_ZY2, b) * ldg(mzy, b);
+        if (zin) ex = ex - coef_at(A, C_DB_YZ1, b + S) * ldg(myz, b + S);
+        if (yin) ex = ex - coef_at(A, C_DB_ZY1, b + sy) * ldg(mzy, b + sy);
+        G* nxy = (G*)A.F.ex_y; G* nxz = (G*)A.F.ex_z; G* nyz = (G*)A.F.ey_z;
+        G* nyx = (G*)A.F.ey_x; G* nzx = (G*)A.F.ez_x; G* nzy = (G*)A.F.ez_y;
+        stg(nxy, b, (yin ? coef_at(A, C_CA_XY, b) : T(1)) * ldg(nxy, b) + ex);
+        stg(nxz, b, (zin ? coef_at(A, C_CA_XZ, b) : T(1)) * ldg(nxz, b) + ex);
+        stg(nyz, b, (zin ? coef_at(A, C_CA_YZ, b) : T(1)) * ldg(nyz, b) + ey);
+        stg(nyx, b, (xin ? coef_at(A, C_CA_YX, b) : T(1)) * ldg(nyx, b) + ey);
+        stg(nzx, b, (xin ? coef_at(A, C_CA_ZX, b) : T(1)) * ldg(nzx, b) + ez);
+        stg(nzy, b, (yin ? coef_at(A, C_CA_ZY, b) : T(1)) * ldg(nzy, b) + ez);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_h(const InstDev<T>* __restrict__ pA) {
+    const InstDev<T>& A = *pA;
+    const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz;
+    const uint32_t cells = (uint32_t)Nx * Ny * Nz, S = (uint32_t)sizeof(T);
+    const uint32_t sy = (uint32_t)Nz * S, sx = (uint32_t)Ny * Nz * S;
+    using G = GLOBAL_T(T);
+    const G* nxy = (const G*)A.F.ex_y; const G* nxz = (const G*)A.F.ex_z; const G* nyz = (const G*)A.F.ey_z;
+    const G* nyx = (const G*)A.F.ey_x; const G* nzx = (const G*)A.F.ez_x; const G* nzy = (const G*)A.F.ez_y;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
+        const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
+        const uint32_t b = i * S;
+        const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
+        // H totals' adjoints: every E split whose curl difference reads H at this cell
+        T hz = T(0), hy = T(0), hx = T(0);
+        if (yin) hz = hz - coef_at(A, C_CB_XY, b) * ldg(nxy, b);
+        if (y > 0) hz = hz + coef_at(A, C_CB_XY, b - sy) * ldg(nxy, b - sy);
+        if (xin) hz = hz - coef_at(A, C_CB_YX, b) * ldg(nyx, b);
+        if (x > 0) hz = hz + coef_at(A, C_CB_YX, b - sx) * ldg(nyx, b - sx);
+        if (zin) hy = hy - coef_at(A, C_CB_XZ, b) * ldg(nxz, b);
+        if (z > 0) hy = hy + coef_at(A, C_CB_XZ, b - S) * ldg(nxz, b - S);
+        if (xin) hy = hy - coef_at(A, C_CB_ZX, b) * ldg(nzx, b);
+        if (x > 0) hy = hy + coef_at(A, C_CB_ZX, b - sx) * ldg(nzx, b - sx);
+        if (zin) hx = hx - coef_at(A, C_CB_YZ, b) * ldg(nyz, b);
+        if (z > 0) hx = hx + coef_at(A, C_CB_YZ, b - S) * ldg(nyz, b - S);
+        if (yin) hx = hx - coef_at(A, C_CB_ZY, b) * ldg(nzy, b);
+        if (y > 0) hx = hx + coef_at(A, C_CB_ZY, b - sy) * ldg(nzy, b - sy);
+        G* mxy = (G*)A.F.hx_y; G* mxz = (G*)A.F.hx_z; G* myz = (G*)A.F.hy_z;
+        G* myx = (G*)A.F.hy_x; G* mzx = (G*)A.F.hz_x; G* mzy = (G*)A.F.hz_y;
+        stg(mxy, b, coef_at(A, C_DA_XY, b) * ldg(mxy, b) + hx);
+        stg(mxz, b, coef_at(A, C_DA_XZ, b) * ldg(mxz, b) + hx);
+        stg(myz, b, coef_at(A, C_DA_YZ, b) * ldg(myz, b) + hy);
+        stg(myx, b, coef_at(A, C_DA_YX, b) * ldg(myx, b) + hy);
+        stg(mzx, b, coef_at(A, C_DA_ZX, b) * ldg(mzx, b) + hz);
+        stg(mzy, b, coef_at(A, C_DA_ZY, b) * ldg(mzy, b) + hz);
+    }
+}
+
+// Adjoint sources: g[n] of every probe of a group added to both splits of its component.
+template <typename T>
+__global__ void k_adj_inject(const InstDev<T>* __restrict__ pA, const ProbeGroup<T>* __restrict__ groups,
+                             int ngroups, const T* __restrict__ g, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ngroups) return;
+    const ProbeGroup<T> Gp = groups[k];
+    const Fields<T>& F = pA->F;
+    T *s1, *s2;
+    switch (Gp.field) {
+        case EX: s1 = F.ex_y; s2 = F.ex_z; break;
+        case EY: s1 = F.ey_z; s2 = F.ey_x; break;
+        case EZ: s1 = F.ez_x; s2 = F.ez_y; break;
+        case HX: s1 = F.hx_y; s2 = F.hx_z; break;
+        case HY: s1 = F.hy_z; s2 = F.hy_x; break;
+        default: s1 = F.hz_x; s2 = F.hz_y; break;
+    }
+    const T* row = g + (size_t)n * Gp.nprobes;
+    T sum = T(0);
+    for (int j = 0; j < Gp.count; ++j) sum = sum + row[Gp.order[j]];
+    s1[Gp.cell] = s1[Gp.cell] + sum;
+    s2[Gp.cell] = s2[Gp.cell] + sum;
+}
+
+// Design box [b0, b0 + bn) in grid cells; the H slot holds (bn + 1) cells per axis.
+struct Box {
+    int32_t x0, y0, z0, nx, ny, nz;
+};
+
+// Save the forward H totals of the box (plus one cell on each upper side) into a slot.
+template <typename T>
+__global__ void k_save_h(const InstDev<T>* __restrict__ pF, Box B, T* __restrict__ slot) {
+    const InstDev<T>& I = *pF;
+    const int hx_n = B.nx + 1, hy_n = B.ny + 1, hz_n = B.nz + 1;
+    const int total = hx_n * hy_n * hz_n;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        const int lz = q % hz_n, ly = q / hz_n % hy_n, lx = q / (hz_n * hy_n);
+        const int x = B.x0 + lx, y = B.y0 + ly, z = B.z0 + lz;
+        T a = T(0), b = T(0), c = T(0);
+        if (x < I.Nx && y < I.Ny && z < I.Nz) {
+            const size_t i = ((size_t)x * I.Ny + y) * I.Nz + z;
+            a = I.F.hx[i];
+            b = I.F.hy[i];
+            c = I.F.hz[i];
+        }
+        slot[q] = a;
+        slot[total + q] = b;
+        slot[2 * total + q] = c;
+    }
+}
+
+// dJ/dCb of the six E splits in the box: += nu * (the curl difference the forward update used).
+template <typename T>
+__global__ void k_adj_grad(const InstDev<T>* __restrict__ pA, Box B, const T* __restrict__ slot,
+                           double* __restrict__ grad) {
+    const InstDev<T>& A = *pA;
+    const int hy_n = B.ny + 1, hz_n = B.nz + 1;
+    const int total_h = (B.nx + 1) * hy_n * hz_n;
+    const int nbox = B.nx * B.ny * B.nz;
+    const T* hx = slot;
+    const T* hy = slot + total_h;
+    const T* hz = slot + 2 * total_h;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += gridDim.x * blockDim.x) {
+        const int lz = q % B.nz, ly = q / B.nz % B.ny, lx = q / (B.nz * B.ny);
+        const int x = B.x0 + lx, y = B.y0 + ly, z = B.z0 + lz;
+        const size_t i = ((size_t)x * A.Ny + y) * A.Nz + z;
+        const int h = (lx * hy_n + ly) * hz_n + lz, hyp = h + hz_n, hxp = h + hy_n * hz_n;
+        const bool xin = x < A.Nx - 1, yin = y < A.Ny - 1, zin = z < A.Nz - 1;
+        const T d_xy = yin ? hz[hyp] - hz[h] : T(0);
+        const T d_xz = zin ? hy[h + 1] - hy[h] : T(0);
+        const T d_yz = zin ? hx[h + 1] - hx[h] : T(0);
+        const T d_yx = xin ? hz[hxp] - hz[h] : T(0);
+        const T d_zx = xin ? hy[hxp] - hy[h] : T(0);
+        const T d_zy = yin ? hx[hyp] - hx[h] : T(0);
+        grad[0 * (size_t)nbox + q] += (double)A.F.ex_y[i] * (double)d_xy;
+        grad[1 * (size_t)nbox + q] += (double)A.F.ex_z[i] * (double)d_xz;
+        grad[2 * (size_t)nbox + q] += (double)A.F.ey_z[i] * (double)d_yz;
+        grad[3 * (size_t)nbox + q] += (double)A.F.ey_x[i] * (double)d_yx;
+        grad[4 * (size_t)nbox + q] += (double)A.F.ez_x[i] * (double)d_zx;
+        grad[5 * (size_t)nbox + q] += (double)A.F.ez_y[i] * (double)d_zy;
+    }
+}
+
+// Totals from splits after a checkpoint restore (the forward computes them the same way).
+template <typename T>
+__global__ void k_totals(const InstDev<T>* __restrict__ pF) {
+    const InstDev<T>& I = *pF;
+    const size_t cells = (size_t)I.Nx * I.Ny * I.Nz;
+    const Fields<T>& F = I.F;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += (size_t)gridDim.x * blockDim.x) {
+        F.ex[i] = F.ex_y[i] + F.ex_z[i];
+        F.ey[i] = F.ey_z[i] + F.ey_x[i];
+        F.ez[i] = F.ez_x[i] + F.ez_y[i];
+        F.hx[i] = F.hx_y[i] + F.hx_z[i];
+        F.hy[i] = F.hy_z[i] + F.hy_x[i];
+        F.hz[i] = F.hz_x[i] + F.hz_y[i];
+    }
 }
 
 struct MonitorDev {
@@ -551,6 +765,7 @@ struct Instance {
     // probes
     int P = 0;
     void* d_series = nullptr;
+    void* d_series_in = nullptr;  // pristine source samples, for recomputation
     int32_t* d_order = nullptr;
     int32_t* d_is_src = nullptr;
     std::vector<void*> probe_host_values;
@@ -562,6 +777,8 @@ struct Instance {
     int ngE = 0;
     // monitors
     std::vector<MonitorHost> mons;
+    // checkpoints: the 12 split arrays at steps 0, every, 2*every, ...
+    std::vector<void*> ckpt;
 };
 
 struct Session {
@@ -586,6 +803,8 @@ struct Session {
     int64_t step = 0;
     int block = 256, graph_steps = 100;
     bool pipelined = true;  // issue the next x plane's loads before this plane's stores
+    int ckpt_every = 0;     // save the split fields every this many steps (0: off)
+    bool tuning = false;    // autotune trials: field updates only, no probes or monitors
 
     int* d_step_base = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -621,6 +840,8 @@ void Session::drop_graph() {
 
 void Session::release_probes(Instance& I) {
     if (I.d_series) (void)hipFree(I.d_series);
+    if (I.d_series_in) (void)hipFree(I.d_series_in);
+    I.d_series_in = nullptr;
     if (I.d_order) (void)hipFree(I.d_order);
     if (I.d_is_src) (void)hipFree(I.d_is_src);
     I.d_series = nullptr; I.d_order = nullptr; I.d_is_src = nullptr;
@@ -644,6 +865,8 @@ Session::~Session() {
         release_probes(I);
         release_monitors(I);
         if (I.d_fields) (void)hipFree(I.d_fields);
+        for (void* c : I.ckpt)
+            if (c) (void)hipFree(c);
     }
     for (auto& M : members)
         if (M.d_coef) (void)hipFree(M.d_coef);
@@ -796,6 +1019,25 @@ void plan_launch(Session& s) {
     }
 }
 
+// Kernel descriptor of a field buffer (18 arrays at the given pitch) on member M's grid.
+template <typename T>
+InstDev<T> make_instdev(const Member& M, void* fields, size_t pitch, int npb) {
+    InstDev<T> d;
+    std::memset(&d, 0, sizeof(d));
+    T** a = (T**)&d.F;
+    for (int k = 0; k < 18; ++k) a[k] = (T*)((char*)fields + (size_t)k * pitch);
+    for (int k = 0; k < 30; ++k) d.raw[k] = (const T*)M.raw[k];
+    d.eidx = M.eidx;
+    d.hidx = M.hidx;
+    d.etab = (const Vec4<T>*)M.etab;
+    d.htab = (const Vec4<T>*)M.htab;
+    d.coef_kind = M.coef_kind;
+    d.Nx = M.Nx; d.Ny = M.Ny; d.Nz = M.Nz;
+    d.xplanes = M.xplanes;
+    d.npb = npb;
+    return d;
+}
+
 template <typename T>
 void refresh(Session& s) {
     if (!s.dirty) return;
@@ -811,18 +1053,8 @@ void refresh(Session& s) {
         dm[m] = MemberDev{tile0, npb * chunks, M.K, M.inst0};
         tile0 += npb * chunks * M.K;
         for (int e = 0; e < M.K; ++e) {
-            InstDev<T>& d = di[M.inst0 + e];
-            std::memset(&d, 0, sizeof(d));
-            d.F = s.fields<T>(s.insts[M.inst0 + e]);
-            for (int k = 0; k < 30; ++k) d.raw[k] = (const T*)M.raw[k];
-            d.eidx = M.eidx;
-            d.hidx = M.hidx;
-            d.etab = (const Vec4<T>*)M.etab;
-            d.htab = (const Vec4<T>*)M.htab;
-            d.coef_kind = M.coef_kind;
-            d.Nx = M.Nx; d.Ny = M.Ny; d.Nz = M.Nz;
-            d.xplanes = M.xplanes;
-            d.npb = npb;
+            const Instance& I = s.insts[M.inst0 + e];
+            di[M.inst0 + e] = make_instdev<T>(M, I.d_fields, I.pitch, npb);
         }
     }
     s.total_tiles = tile0;
@@ -844,6 +1076,7 @@ void refresh(Session& s) {
             ProbeGroup<T> pg{};
             pg.cell = hg.cell;
             pg.series = (T*)I.d_series;
+            pg.series_in = (const T*)I.d_series_in;
             pg.order = I.d_order + hg.start;
             pg.is_src = I.d_is_src;
             pg.inst = (int32_t)k;
@@ -905,15 +1138,16 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, bool mo
     else
         k_update_e<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
     if (ev) HIP_OK(hipEventRecord(ev[1], st));
-    if (s.ngE) k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(di, groups, s.ngE, base, local);
+    const bool io = !s.tuning;
+    if (io && s.ngE) k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(di, groups, s.ngE, base, local, 0);
     if (ev) HIP_OK(hipEventRecord(ev[2], st));
     if (s.pipelined)
         k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
     else
         k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
     if (ev) HIP_OK(hipEventRecord(ev[3], st));
-    if (s.ngH) k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local);
-    if (s.n_active && monitors_now)
+    if (io && s.ngH) k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local, 0);
+    if (io && s.n_active && monitors_now)
         k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local);
 }
 
@@ -922,6 +1156,54 @@ bool any_monitor_samples(const Session& s, int64_t n) {
         for (const auto& m : I.mons)
             if (m.active && (n == 0 || n % m.desc.n_step == 0)) return true;
     return false;
+}
+
+void save_checkpoints(Session& s, int slot, hipStream_t st) {
+    for (Instance& I : s.insts) {
+        if ((int)I.ckpt.size() <= slot) I.ckpt.resize(slot + 1, nullptr);
+        if (!I.ckpt[slot]) HIP_OK(hipMalloc(&I.ckpt[slot], 12 * I.pitch));
+        HIP_OK(hipMemcpyAsync(I.ckpt[slot], I.d_fields, 12 * I.pitch, hipMemcpyDeviceToDevice, st));
+    }
+}
+
+template <typename T>
+void capture_graph(Session& s) {
+    s.drop_graph();
+    hipStream_t cs;
+    HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < s.graph_steps; ++k) launch_step<T>(s, cs, s.d_step_base, k, true);
+    hipError_t ce = hipStreamEndCapture(cs, &s.graph_def);
+    (void)hipStreamDestroy(cs);
+    HIP_OK(ce);
+    HIP_OK(hipGraphInstantiate(&s.graph, s.graph_def, nullptr, nullptr, 0));
+    s.graph_dirty = false;
+}
+
+// len steps starting at s.step + offset (s.step itself is advanced by the caller)
+template <typename T>
+void run_chunk(Session& s, int len, int offset, hipStream_t st, bool graphs, bool timing, int stride) {
+    int done = 0;
+    if (graphs && len >= s.graph_steps) {
+        if (s.graph_dirty || !s.graph) capture_graph<T>(s);
+        while (len - done >= s.graph_steps) {
+            k_set_step<<<1, 1, 0, st>>>(s.d_step_base, (int)(s.step + offset + done));
+            HIP_OK(hipGraphLaunch(s.graph, st));
+            done += s.graph_steps;
+        }
+    }
+    for (; done < len; ++done) {
+        const int64_t n = s.step + offset + done;
+        hipEvent_t* ev = nullptr;
+        if (timing && (offset + done) % stride == 0 && s.n_timed < Session::kTimedSteps) {
+            if (s.kev.empty()) {
+                s.kev.resize(4 * Session::kTimedSteps);
+                for (auto& e : s.kev) HIP_OK(hipEventCreate(&e));
+            }
+            ev = &s.kev[4 * s.n_timed++];
+        }
+        launch_step<T>(s, st, nullptr, (int)n, any_monitor_samples(s, n), ev);
+    }
 }
 
 template <typename T>
@@ -944,40 +1226,92 @@ void run_steps(Session& s, int n_steps, hipStream_t st) {
     s.n_timed = 0;
     const int stride = timing ? std::max(1, n_steps / Session::kTimedSteps) : 0;
     int done = 0;
-    if (graphs && n_steps >= s.graph_steps) {
-        if (s.graph_dirty || !s.graph) {
-            s.drop_graph();
-            hipStream_t cs;
-            HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < s.graph_steps; ++k) launch_step<T>(s, cs, s.d_step_base, k, true);
-            hipError_t ce = hipStreamEndCapture(cs, &s.graph_def);
-            (void)hipStreamDestroy(cs);
-            HIP_OK(ce);
-            HIP_OK(hipGraphInstantiate(&s.graph, s.graph_def, nullptr, nullptr, 0));
-            s.graph_dirty = false;
+    if (s.ckpt_every > 0) {
+        // checkpointed run: save the split fields before every multiple of ckpt_every, and replay
+        // graph chunks only between checkpoints
+        while (done < n_steps) {
+            const int64_t n = s.step + done;
+            if (n % s.ckpt_every == 0) save_checkpoints(s, (int)(n / s.ckpt_every), st);
+            const int64_t next = (n / s.ckpt_every + 1) * s.ckpt_every;
+            const int len = (int)std::min<int64_t>(n_steps - done, next - n);
+            run_chunk<T>(s, len, done, st, graphs, timing, stride);
+            done += len;
         }
-        while (n_steps - done >= s.graph_steps) {
-            k_set_step<<<1, 1, 0, st>>>(s.d_step_base, (int)(s.step + done));
-            HIP_OK(hipGraphLaunch(s.graph, st));
-            done += s.graph_steps;
-        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(s.ev1, st));
+        s.step += n_steps;
+        return;
     }
-    for (; done < n_steps; ++done) {
-        const int64_t n = s.step + done;
-        hipEvent_t* ev = nullptr;
-        if (timing && done % stride == 0 && s.n_timed < Session::kTimedSteps) {
-            if (s.kev.empty()) {
-                s.kev.resize(4 * Session::kTimedSteps);
-                for (auto& e : s.kev) HIP_OK(hipEventCreate(&e));
-            }
-            ev = &s.kev[4 * s.n_timed++];
-        }
-        launch_step<T>(s, st, nullptr, (int)n, any_monitor_samples(s, n), ev);
-    }
+    run_chunk<T>(s, n_steps, 0, st, graphs, timing, stride);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(s.ev1, st));
     s.step += n_steps;
+}
+
+// Launch autotuner: times every (variant, block size, x planes per workgroup) candidate over
+// trial_steps field-only steps of this session's own grids and keeps the fastest.  The session must
+// not have run yet (step 0); its fields are zero before and after.
+template <typename T>
+void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
+    if (s.step != 0) RF_THROW(RFDTD_ESTATE, "autotune a session before its first run (or after rfdtd_reset)");
+    if (trial_steps < 1 || trial_steps > 10000) RF_THROW(RFDTD_EINVAL, "trial_steps must be in [1, 10000]");
+    int maxNx = 1;
+    for (const auto& M : s.members) maxNx = std::max(maxNx, M.Nx);
+    std::vector<int> xps{0};  // 0: automatic (enough workgroups to fill the chip)
+    for (int chunks : {2, 4, 8, 16, 32}) {
+        const int xp = (maxNx + chunks - 1) / chunks;
+        if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
+    }
+    const bool pf0 = s.pipelined;
+    const int blk0 = s.block;
+    std::vector<int> xr0;
+    for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
+    auto apply = [&](bool pf, int blk, const std::vector<int>& xr) {
+        s.pipelined = pf;
+        s.block = blk;
+        for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
+        s.dirty = true;
+        s.graph_dirty = true;
+    };
+    hipStream_t st = s.own_stream;
+    double best = 1e300;
+    bool bpf = pf0;
+    int bblk = blk0, bxp = 0;
+    s.tuning = true;
+    try {
+        for (int pf : {1, 0})
+            for (int blk : {128, 256, 512})
+                for (int xp : xps) {
+                    apply(pf != 0, blk, std::vector<int>(s.members.size(), xp));
+                    refresh<T>(s);
+                    run_chunk<T>(s, std::min(trial_steps, 3), 0, st, false, false, 0);  // warm-up
+                    HIP_OK(hipEventRecord(s.ev0, st));
+                    run_chunk<T>(s, trial_steps, 0, st, false, false, 0);
+                    HIP_OK(hipEventRecord(s.ev1, st));
+                    HIP_OK(hipEventSynchronize(s.ev1));
+                    float ms = 0;
+                    HIP_OK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+                    const double per = (double)ms / trial_steps;
+                    if (per < best) {
+                        best = per;
+                        bpf = pf != 0;
+                        bblk = blk;
+                        bxp = xp;
+                    }
+                }
+    } catch (...) {
+        s.tuning = false;
+        apply(pf0, blk0, xr0);
+        throw;
+    }
+    s.tuning = false;
+    apply(bpf, bblk, std::vector<int>(s.members.size(), bxp));
+    HIP_OK(hipDeviceSynchronize());
+    for (Instance& I : s.insts) HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
+    cfg[0] = bpf ? 1 : 0;
+    cfg[1] = bblk;
+    cfg[2] = bxp;
+    if (ms_out) *ms_out = best;
 }
 
 struct MemberSpec {
@@ -1096,6 +1430,8 @@ void set_probes(Session& s, int inst, const rfdtd_probe* probes, int n, int Nt) 
     }
     HIP_OK(hipMalloc(&I.d_series, series.size()));
     HIP_OK(hipMemcpy(I.d_series, series.data(), series.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&I.d_series_in, series.size()));
+    HIP_OK(hipMemcpy(I.d_series_in, series.data(), series.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMalloc(&I.d_order, order.size() * 4));
     HIP_OK(hipMemcpy(I.d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMalloc(&I.d_is_src, (size_t)n * 4));
@@ -1195,6 +1531,129 @@ void read_fields(Session& s, int inst, void* const out[18]) {
                 std::memcpy(dst + (((size_t)(x + ox) * gy + (y + oy)) * gz + oz) * s.esz,
                             &buf[(((size_t)x * Ny + y) * Nz) * s.esz], (size_t)Nz * s.esz);
     }
+}
+
+// Device buffers of one gradient evaluation, freed on every exit path.
+struct GradBuffers {
+    std::vector<void*> ptrs;
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        HIP_OK(hipMalloc(&p, bytes));
+        ptrs.push_back(p);
+        return p;
+    }
+    ~GradBuffers() {
+        (void)hipDeviceSynchronize();
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+// Reverse-time pass over a checkpointed forward run of one instance: recomputes each checkpoint
+// segment forward (saving H in the design box every step), then steps the adjoint back through the
+// segment and accumulates dJ/dCb of the six E splits in the box.  adj[p] is dJ/dy of probe p,
+// T[Nt] in the instance's probe order.  grad_out: double[6][nx][ny][nz] (ex_y ex_z ey_z ey_x ez_x
+// ez_y), box cells in grid coordinates.
+template <typename T>
+void gradient(Session& s, int inst, const int32_t box[6], const void* const* adj, int n_adj, double* grad_out) {
+    Instance& I = instance_at(s, inst);
+    const Member& M = s.members[I.member];
+    if (s.ckpt_every <= 0) RF_THROW(RFDTD_ESTATE, "enable checkpointing before the forward run");
+    const int64_t Nt = s.step;
+    if (Nt < 1) RF_THROW(RFDTD_ESTATE, "run the forward solve first");
+    const int K = s.ckpt_every;
+    const int nseg = (int)((Nt + K - 1) / K);
+    if ((int)I.ckpt.size() < nseg) RF_THROW(RFDTD_ESTATE, "missing checkpoints of the forward run");
+    if (n_adj != I.P || !adj) RF_THROW(RFDTD_EINVAL, "one adjoint series per probe is required");
+    const Box B{box[0], box[2], box[4], box[1] - box[0], box[3] - box[2], box[5] - box[4]};
+    if (B.nx < 1 || B.ny < 1 || B.nz < 1 || B.x0 < 0 || B.y0 < 0 || B.z0 < 0 || box[1] > M.Nx || box[3] > M.Ny ||
+        box[5] > M.Nz)
+        RF_THROW(RFDTD_EINVAL, "design box outside the grid");
+    if (!grad_out) RF_THROW(RFDTD_EINVAL, "gradient output missing");
+    refresh<T>(s);
+    HIP_OK(hipDeviceSynchronize());
+    hipStream_t st = s.own_stream;
+    HIP_OK(hipEventRecord(s.ev0, st));
+    GradBuffers buf;
+    void* d_re = buf.alloc(18 * I.pitch);
+    void* d_adj = buf.alloc(18 * I.pitch);
+    HIP_OK(hipMemsetAsync(d_adj, 0, 18 * I.pitch, st));
+    const size_t esz = sizeof(T);
+    // adjoint sources, time-major [Nt][P]
+    std::vector<T> gser((size_t)Nt * I.P);
+    for (int p = 0; p < I.P; ++p) {
+        if (!adj[p]) RF_THROW(RFDTD_EINVAL, "adjoint series missing");
+        const T* src = (const T*)adj[p];
+        for (int64_t t = 0; t < Nt; ++t) gser[(size_t)t * I.P + p] = src[t];
+    }
+    T* d_g = nullptr;
+    if (I.P) {
+        d_g = (T*)buf.alloc(gser.size() * esz);
+        HIP_OK(hipMemcpyAsync(d_g, gser.data(), gser.size() * esz, hipMemcpyHostToDevice, st));
+    }
+    const size_t slot_elems = (size_t)3 * (B.nx + 1) * (B.ny + 1) * (B.nz + 1);
+    T* d_slots = (T*)buf.alloc((size_t)K * slot_elems * esz);
+    const size_t nbox = (size_t)B.nx * B.ny * B.nz;
+    double* d_grad = (double*)buf.alloc(6 * nbox * sizeof(double));
+    HIP_OK(hipMemsetAsync(d_grad, 0, 6 * nbox * sizeof(double), st));
+    // descriptors: [0] recomputed forward fields, [1] adjoint fields; one member over [0]
+    const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
+    const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
+    InstDev<T> di[2] = {make_instdev<T>(M, d_re, I.pitch, npb), make_instdev<T>(M, d_adj, I.pitch, npb)};
+    InstDev<T>* d_di = (InstDev<T>*)buf.alloc(sizeof(di));
+    HIP_OK(hipMemcpyAsync(d_di, di, sizeof(di), hipMemcpyHostToDevice, st));
+    const MemberDev md{0, npb * chunks, 1, 0};
+    MemberDev* d_md = (MemberDev*)buf.alloc(sizeof(md));
+    HIP_OK(hipMemcpyAsync(d_md, &md, sizeof(md), hipMemcpyHostToDevice, st));
+    std::vector<ProbeGroup<T>> gr;
+    for (const auto& hg : I.groups) {
+        ProbeGroup<T> pg{};
+        pg.cell = hg.cell;
+        pg.series = (T*)I.d_series;
+        pg.series_in = (const T*)I.d_series_in;
+        pg.order = I.d_order + hg.start;
+        pg.is_src = I.d_is_src;
+        pg.inst = 0;
+        pg.field = hg.field;
+        pg.count = hg.count;
+        pg.nprobes = I.P;
+        gr.push_back(pg);
+    }
+    const int ngE = I.ngE, ngH = (int)gr.size() - I.ngE;
+    ProbeGroup<T>* d_gr = nullptr;
+    if (!gr.empty()) {
+        d_gr = (ProbeGroup<T>*)buf.alloc(gr.size() * sizeof(ProbeGroup<T>));
+        HIP_OK(hipMemcpyAsync(d_gr, gr.data(), gr.size() * sizeof(ProbeGroup<T>), hipMemcpyHostToDevice, st));
+    }
+    const int grid = (npb * chunks + 7) & ~7;
+    const int flat = (int)std::min<int64_t>((M.cells + 255) / 256, 16384);
+    const int gbox = (int)std::min<size_t>((nbox + 255) / 256 + 1, 4096);
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int64_t n0 = (int64_t)seg * K, n1 = std::min<int64_t>(Nt, n0 + K);
+        HIP_OK(hipMemcpyAsync(d_re, I.ckpt[seg], 12 * I.pitch, hipMemcpyDeviceToDevice, st));
+        k_totals<T><<<flat, 256, 0, st>>>(d_di);
+        for (int64_t n = n0; n < n1; ++n) {
+            k_save_h<T><<<gbox, 256, 0, st>>>(d_di, B, d_slots + (size_t)(n - n0) * slot_elems);
+            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
+            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
+            if (ngE) k_probes<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di, d_gr, ngE, nullptr, (int)n, 1);
+            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
+            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
+            if (ngH) k_probes<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di, d_gr + ngE, ngH, nullptr, (int)n, 1);
+        }
+        for (int64_t n = n1 - 1; n >= n0; --n) {
+            if (ngH) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + 1, d_gr + ngE, ngH, d_g, (int)n);
+            k_adj_e<T><<<flat, 256, 0, st>>>(d_di + 1);
+            if (ngE) k_adj_inject<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di + 1, d_gr, ngE, d_g, (int)n);
+            k_adj_grad<T><<<gbox, 256, 0, st>>>(d_di + 1, B, d_slots + (size_t)(n - n0) * slot_elems, d_grad);
+            k_adj_h<T><<<flat, 256, 0, st>>>(d_di + 1);
+        }
+        HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipEventRecord(s.ev1, st));
+    HIP_OK(hipMemcpyAsync(grad_out, d_grad, 6 * nbox * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, s.ev0, s.ev1) == hipSuccess) s.last_total_ms = ms;
 }
 
 void progress_bar(int64_t n, int64_t Nt) {
@@ -1386,8 +1845,8 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
             S.block = block_threads;
         }
         if (x_planes) {
-            if (x_planes < 0) RF_THROW(RFDTD_EINVAL, "x_planes must be positive");
-            for (auto& M : S.members) M.xplanes_req = x_planes;
+            if (x_planes < -1) RF_THROW(RFDTD_EINVAL, "x_planes must be positive, or -1 for automatic");
+            for (auto& M : S.members) M.xplanes_req = x_planes < 0 ? 0 : x_planes;
         }
         if (graph_steps) {
             if (graph_steps < 0) RF_THROW(RFDTD_EINVAL, "graph_steps must be positive");
@@ -1403,6 +1862,33 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t pipelined) {
         Session& S = S_of(s);
         S.pipelined = pipelined != 0;
         S.graph_dirty = true;
+    });
+}
+
+int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3], double* ms_per_step) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (!config_out) RF_THROW(RFDTD_EINVAL, "config_out missing");
+        if (S.f64) autotune<double>(S, trial_steps, config_out, ms_per_step);
+        else autotune<float>(S, trial_steps, config_out, ms_per_step);
+    });
+}
+
+int rfdtd_set_checkpointing(rfdtd_session* s, int32_t every) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (every < 0) RF_THROW(RFDTD_EINVAL, "checkpoint interval must be >= 0");
+        S.ckpt_every = every;
+    });
+}
+
+int rfdtd_gradient(rfdtd_session* s, int32_t instance, const int32_t box[6], const void* const* adjoint_series,
+                   int32_t n_series, double* grad_out) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (!box) RF_THROW(RFDTD_EINVAL, "design box missing");
+        if (S.f64) gradient<double>(S, instance, box, adjoint_series, n_series, grad_out);
+        else gradient<float>(S, instance, box, adjoint_series, n_series, grad_out);
     });
 }
 

@@ -102,6 +102,34 @@ class Session:
     def set_launch(self, block_threads=0, x_planes=0, graph_steps=0):
         check(self._L.rfdtd_set_launch(self._h, int(block_threads), int(x_planes), int(graph_steps)))
 
+    def autotune(self, trial_steps=20, use_cache=True):
+        """Pick the fastest launch shape for this grid (before the first run); cached per shape.
+        Returns {pipelined, block, x_planes, ms_per_step, cached}."""
+        from . import autotune
+
+        kind = self.memory_info()["coef_kind"]
+        return autotune.tune(self, [(self.Nx, self.Ny, self.Nz, 1)], np.dtype(self.dtype).name, kind,
+                             trial_steps, use_cache)
+
+    def set_checkpointing(self, every):
+        """Save the split fields every `every` steps of later runs (needed by gradient())."""
+        check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
+
+    def gradient(self, box, adjoint_series, instance=0):
+        """dJ/dCb of the 6 E splits in box (x0, x1, y0, y1, z0, z1 grid cells) after a checkpointed
+        forward run; adjoint_series: per probe, dJ/dy[n] (n < steps run).  Returns float64
+        [6, nx, ny, nz] in split order ex_y ex_z ey_z ey_x ez_x ez_y."""
+        box = [int(v) for v in box]
+        nx, ny, nz = box[1] - box[0], box[3] - box[2], box[5] - box[4]
+        out = np.zeros((6, nx, ny, nz), np.float64)
+        series = [np.ascontiguousarray(a, dtype=self.dtype) for a in adjoint_series]
+        ptrs = _lib.pointer_array(series)
+        b = (ctypes.c_int32 * 6)(*box)
+        check(self._L.rfdtd_gradient(self._h, int(instance), ctypes.cast(b, ctypes.c_void_p),
+                                     ctypes.cast(ptrs, ctypes.c_void_p), len(series), out.ctypes.data_as(ctypes.c_void_p)),
+              "rfdtd_gradient")
+        return out
+
     def set_variant(self, pipelined=1):
         """Kernel variant: 1 issues the next x plane's loads before the current plane's stores."""
         check(self._L.rfdtd_set_variant(self._h, int(pipelined)))

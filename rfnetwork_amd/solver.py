@@ -73,6 +73,9 @@ class FDTD_Solver:
         self._time = None
         self._edges = None
         self.courant = 0.95
+        # element type of the coefficient assembly: float32 as in the reference; float64 gives exact
+        # coefficients for finite-difference checks of gradients
+        self.coef_dtype = np.float32
 
     # ------------------------------------------------------------------ state
     @property
@@ -302,7 +305,7 @@ class FDTD_Solver:
                 self.sigma[blk] = vol["sigma"]
 
     def _init_coefficients(self):
-        dtype_ = np.float32
+        dtype_ = self.coef_dtype
         dx, dy, dz = [conv.m_in(d) for d in self.d_cells]
         Nx, Ny, Nz = self.n_cells
         # time step from the smallest cell on each axis (free-space Courant limit)

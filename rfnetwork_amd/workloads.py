@@ -14,6 +14,18 @@ dipole_slab     400x400x200 cells of 0.1 mm, 12-cell PML on every face, eps_r ~ 
                 frequencies 5-30 GHz on 6 planes 2 cells inside the PML.
 tiny_model      small grid with PML, a dielectric block, a conductor, a port, a probe and monitors,
                 for parity tests.
+design_strip    200x160x80 cells of 0.1 mm: a 64x64x32 box of per-cell eps_r (U[1, 4), seed 4) on
+                the z = 0 ground, centred in x and y, in the gap between two microstrip feed lines
+                (6 cells wide, 8-cell eps_r 3.66 substrate) that run from a 50 ohm port 1 cell in
+                from each x PML to one cell short of the box; 10-cell PML on x-, x+, y-, y+, z+;
+                Gaussian-modulated sine at 4 GHz; objective frequencies 2, 3, 4, 5, 6 GHz; 6000
+                steps.  design_box() gives the box as (x0, x1, y0, y1, z0, z1) cells.
+port_variant    variant v of the batched workload: V0 64^3, V1 128x96x64, V2 160x128x96, V3
+                192x128x96 cells of 0.1 mm with an eps_r ~ U[1, 6) block (seed 300 + v) centred in x
+                and y over half the interior extent, from the ground to half height; 6-cell strips on
+                a 4-cell eps_r 3.66 substrate from 2/3/3/4 ports (x-, x+, y-, y+ sides, 1 cell in from
+                the PML) to the centre; 10-cell PML on every face but the z = 0 ground; the C1 pulse;
+                101 frequencies 1-10 GHz; 5000 steps.
 """
 import numpy as np
 
@@ -108,4 +120,82 @@ def tiny_model(Nt=60, nx=14, ny=11, nz=9, cell_m=1e-4, courant=0.99, n_pml=3, se
         s.add_field_monitor("p_hx_y", "hx", "y", index=6, frequency=f, n_step=2)
         s.add_field_monitor("p_ez_z", "ez", "z", index=3, frequency=f)
     s.frequency = np.linspace(5e9, 40e9, 8)
+    return s
+
+
+def design_box(nx=200, ny=160, design=(64, 64, 32)):
+    """Cell box (x0, x1, y0, y1, z0, z1) of the design region of design_strip."""
+    dx, dy, dz = design
+    x0, y0 = (nx - dx) // 2, (ny - dy) // 2
+    return (x0, x0 + dx, y0, y0 + dy, 0, dz)
+
+
+def design_strip(eps_design=None, Nt=6000, nx=200, ny=160, nz=80, design=(64, 64, 32), cell_m=1e-4,
+                 courant=0.99, n_pml=10, h=8, seed=4, coef_dtype=np.float32, pulse=(4e9, 600e-12, 420e-12),
+                 frequency=(2e9, 3e9, 4e9, 5e9, 6e9)):
+    """Two microstrip feeds coupled through a box of per-cell permittivity (the adjoint workload).
+
+    eps_design: eps_r of the box cells, shape ``design``; default U[1, 4) from ``seed``.
+    pulse: (f0, width, t0) of the Gaussian-modulated sine on port 1; frequency: objective points."""
+    if eps_design is None:
+        eps_design = np.random.default_rng(seed).uniform(1.0, 4.0, size=design)
+    eps_design = np.asarray(eps_design, np.float64)
+    if eps_design.shape != tuple(design):
+        raise ValueError(f"eps_design must have shape {tuple(design)}")
+    s, d = _grid_solver(nx, ny, nz, cell_m)
+    s.coef_dtype = coef_dtype
+    x0, x1, y0, y1, z0, z1 = design_box(nx, ny, design)
+    w0, w1 = ny // 2 - 3, ny // 2 + 3  # 6-cell strip centred in y
+    xa, xb = n_pml + 1, nx - n_pml - 1
+    s.add_dielectric(Box((0, 0, 0), (nx * d, ny * d, h * d)), er=3.66)
+    s.add_permittivity_volume(eps_design, (x0, y0, z0))
+    # feeds end one node short of the box so that no conductor edge borders a design cell
+    s.add_conductor(Rectangle([(xa * d, w0 * d, h * d), (xa * d, w1 * d, h * d), ((x0 - 1) * d, w1 * d, h * d)]))
+    s.add_conductor(Rectangle([((x1 + 1) * d, w0 * d, h * d), ((x1 + 1) * d, w1 * d, h * d), (xb * d, w1 * d, h * d)]))
+    s.add_lumped_port(1, Rectangle([(xa * d, w0 * d, 0), (xa * d, w1 * d, 0), (xa * d, w1 * d, h * d)]), "z+")
+    s.add_lumped_port(2, Rectangle([(xb * d, w0 * d, 0), (xb * d, w1 * d, 0), (xb * d, w1 * d, h * d)]), "z+")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=n_pml)
+    s.generate_mesh(courant=courant)
+    f0, width, t0 = pulse
+    src = s.gaussian_modulated_source(f0=f0, width=width, t0=t0, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(1e-2 * np.asarray(src), 1)
+    s.frequency = np.asarray(frequency, dtype=np.float64)
+    return s
+
+
+PORT_VARIANTS = ((64, 64, 64, 2), (128, 96, 64, 3), (160, 128, 96, 3), (192, 128, 96, 4))
+
+
+def port_variant(v, Nt=5000, cell_m=1e-4, courant=0.99, n_pml=10, h=4, n_freq=101, coef_dtype=np.float32):
+    """Variant v (0..3) of the batched multi-port workload; port 1 carries the excitation."""
+    nx, ny, nz, n_ports = PORT_VARIANTS[v]
+    s, d = _grid_solver(nx, ny, nz, cell_m)
+    s.coef_dtype = coef_dtype
+    rng = np.random.default_rng(300 + v)
+    ix, iy = nx - 2 * n_pml, ny - 2 * n_pml  # interior extent
+    bx, by, bz = ix // 2, iy // 2, nz // 2
+    bx0, by0 = (nx - bx) // 2, (ny - by) // 2
+    s.add_dielectric(Box((0, 0, 0), (nx * d, ny * d, h * d)), er=3.66)
+    s.add_permittivity_volume(rng.uniform(1.0, 6.0, size=(bx, by, bz)), (bx0, by0, 0))
+    cx, cy = nx // 2, ny // 2
+    # (side, port position): strips along x from the x ports, along y from the y ports, to the centre
+    ends = (("x", n_pml + 1), ("x", nx - n_pml - 1), ("y", n_pml + 1), ("y", ny - n_pml - 1))[:n_ports]
+    for k, (axis, p) in enumerate(ends):
+        if axis == "x":
+            a, b = sorted((p, cx + 3 if p < cx else cx - 3))
+            s.add_conductor(Rectangle([(a * d, (cy - 3) * d, h * d), (a * d, (cy + 3) * d, h * d),
+                                       (b * d, (cy + 3) * d, h * d)]))
+            s.add_lumped_port(k + 1, Rectangle([(p * d, (cy - 3) * d, 0), (p * d, (cy + 3) * d, 0),
+                                                (p * d, (cy + 3) * d, h * d)]), "z+")
+        else:
+            a, b = sorted((p, cy + 3 if p < cy else cy - 3))
+            s.add_conductor(Rectangle([((cx - 3) * d, a * d, h * d), ((cx + 3) * d, a * d, h * d),
+                                       ((cx + 3) * d, b * d, h * d)]))
+            s.add_lumped_port(k + 1, Rectangle([((cx - 3) * d, p * d, 0), ((cx + 3) * d, p * d, 0),
+                                                ((cx + 3) * d, p * d, h * d)]), "z+")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=n_pml)
+    s.generate_mesh(courant=courant)
+    src = s.gaussian_modulated_source(f0=5.5e9, width=430e-12, t0=320e-12, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(1e-2 * np.asarray(src), 1)
+    s.frequency = np.linspace(1e9, 1e10, n_freq)
     return s

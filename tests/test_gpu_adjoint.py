@@ -1,0 +1,142 @@
+"""The engine's adjoint (rfdtd_gradient) against an unrolled torch autograd of the same time loop,
+and the eps_r gradient against central finite differences of the float64 oracle."""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu_solver
+from rfnetwork_amd import adjoint, workloads
+from rfnetwork_amd.session import Session
+
+import yee_torch
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SPLITS = ("Cb_ex_y", "Cb_ex_z", "Cb_ey_z", "Cb_ey_x", "Cb_ez_x", "Cb_ez_y")
+
+
+def _autograd_reference(model, g, Nt):
+    c, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=False, dtype=np.float64, n_threads=1)
+    series, leaf = yee_torch.run(c, probes, Nx, Ny, Nz, Nt)
+    torch.sum(series * torch.tensor(g)).backward()
+    return {k: leaf[k].grad.numpy() for k in SPLITS}
+
+
+def _engine_gradient(model, g, Nt, dtype, every):
+    coeff, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True, dtype=dtype)
+    with Session(coeff, Nx, Ny, Nz, dtype=dtype) as s:
+        s.set_checkpointing(every)
+        s.set_sources(probes, [], Nt)
+        s.run(Nt)
+        s.synchronize()
+        return s.gradient((0, Nx, 0, Ny, 0, Nz), [row.astype(dtype) for row in g])
+
+
+def _cpu_view(grad, k, Nx, Ny, Nz):
+    """The part of the packed-layout gradient that the CPU-layout coefficient k covers."""
+    comp = SPLITS[k][3:5]
+    ny = Ny - 1 if comp in ("ex", "ez") else Ny
+    nz = Nz - 1 if comp in ("ex", "ey") else Nz
+    return grad[k][:, :ny, :nz]
+
+
+@pytest.mark.parametrize("every", [7, 40])
+def test_adjoint_fp64_matches_unrolled_autograd(hip_ready, every):
+    """float64: the hand-written reverse pass (checkpoints every `every` steps, recomputation between
+    them) against torch autograd through the unrolled loop, for a random dJ/dy on every probe."""
+    print(f"[case] adjoint vs autograd fp64 every={every}", flush=True)
+    Nt = 40
+    model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
+    Nx, Ny, Nz = model.n_cells
+    n_probes = len(model.pack(gpu=True, coefficients=False)[1])
+    g = np.random.default_rng(17).standard_normal((n_probes, Nt))
+    ref = _autograd_reference(model, g, Nt)
+    grad = _engine_gradient(model, g, Nt, np.float64, every)
+    for k, name in enumerate(SPLITS):
+        got = _cpu_view(grad, k, Nx, Ny, Nz)
+        r = ref[name]
+        if name[3:5] in ("ey", "ez"):  # the x = Nx-1 plane is the zeroed PEC wall
+            got, r = got[:-1], r[:-1]
+        scale = np.abs(r).max()
+        assert scale > 0, name
+        np.testing.assert_allclose(got, r, rtol=1e-9, atol=1e-12 * scale, err_msg=name)
+    # entries of the packed layout that hold no updated component carry no gradient
+    assert not np.any(grad[0][:, Ny - 1, :]) and not np.any(grad[0][:, :, Nz - 1])
+    assert not np.any(grad[4][:, Ny - 1, :])
+
+
+def test_adjoint_fp32_error_against_fp64(hip_ready):
+    """float32 reverse pass: its gradient against the float64 autograd reference (stated)."""
+    print("[case] adjoint fp32 vs fp64", flush=True)
+    Nt = 40
+    model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
+    Nx, Ny, Nz = model.n_cells
+    n_probes = len(model.pack(gpu=True, coefficients=False)[1])
+    g = np.random.default_rng(18).standard_normal((n_probes, Nt))
+    ref = _autograd_reference(model, g, Nt)
+    grad = _engine_gradient(model, g, Nt, np.float32, 10)
+    err = max(rel_l2(_cpu_view(grad, k, Nx, Ny, Nz)[:-1], ref[n][:-1]) for k, n in enumerate(SPLITS))
+    print(f"[adjoint] fp32 vs fp64 gradient rel L2 {err:.3e}", flush=True)
+    assert err < 1e-4
+
+
+SMALL_F = (16e9, 20e9, 24e9)
+
+
+def _small_design(eps, coef_dtype=np.float64, Nt=900):
+    """The adjoint workload shrunk to 48x32x20 cells with an 8x8x6 design box and a 20 GHz pulse."""
+    return workloads.design_strip(eps, Nt=Nt, nx=48, ny=32, nz=20, design=(8, 8, 6), n_pml=5, h=3,
+                                  coef_dtype=coef_dtype, pulse=(20e9, 80e-12, 60e-12), frequency=SMALL_F)
+
+
+def _fd_oracle(eps, cell, h, frequency):
+    Js = []
+    for sgn in (1, -1):
+        e = eps.copy()
+        e[cell] += sgn * h
+        m = _small_design(e)
+        Js.append(adjoint.objective_only(m, frequency, dtype=np.float64, backend=cpu_solver.solver_run)[0])
+    return (Js[0] - Js[1]) / (2 * h)
+
+
+def test_eps_gradient_matches_oracle_finite_differences(hip_ready):
+    """dJ/deps_r, J = sum_f |S21|^2 at 16, 20, 24 GHz, from the float64 engine adjoint against central
+    differences (Delta eps_r = 1e-3) of the float64 oracle, on cells across the design box."""
+    print("[case] eps gradient vs oracle FD", flush=True)
+    rng = np.random.default_rng(4)
+    eps = rng.uniform(1.0, 4.0, size=(8, 8, 6))
+    cbox = workloads.design_box(48, 32, (8, 8, 6))
+    model = _small_design(eps)
+    J, grad, _ = adjoint.eps_r_gradient(model, cbox, model.frequency, checkpoint_every=64, dtype=np.float64)
+    J_o = adjoint.objective_only(_small_design(eps), model.frequency, dtype=np.float64,
+                                 backend=cpu_solver.solver_run)[0]
+    assert J == pytest.approx(J_o, rel=1e-12)
+    cells = [(0, 0, 0), (4, 4, 2), (7, 3, 5), (2, 6, 1), (5, 1, 3)]
+    fd = np.array([_fd_oracle(eps, c, 1e-3, model.frequency) for c in cells])
+    ad = np.array([grad[c] for c in cells])
+    err = np.abs(ad - fd) / np.abs(fd)
+    print(f"[adjoint] J {J:.6e} grad {ad} fd {fd} rel err {err}", flush=True)
+    assert np.all(err <= 1e-3)
+    # float32 fields and coefficients: the same gradient, error stated
+    m32 = _small_design(eps, coef_dtype=np.float32)
+    J32, g32, _ = adjoint.eps_r_gradient(m32, cbox, m32.frequency, checkpoint_every=64, dtype=np.float32)
+    e32 = rel_l2(g32, grad)
+    print(f"[adjoint] fp32 gradient vs fp64 rel L2 {e32:.3e}, J {J32:.6e}", flush=True)
+    assert e32 < 1e-2
+
+
+def test_autograd_function_backward(hip_ready):
+    """EpsRObjective.backward hands torch the adjoint gradient of the design cells."""
+    print("[case] autograd function", flush=True)
+    eps = torch.tensor(np.random.default_rng(4).uniform(1.0, 4.0, size=(8, 8, 6)), requires_grad=True)
+    cbox = workloads.design_box(48, 32, (8, 8, 6))
+    f = np.array(SMALL_F)
+    J = adjoint.EpsRObjective.apply(eps, lambda e: _small_design(e, coef_dtype=np.float32), cbox, f, 100)
+    (2.0 * J).backward()
+    J_ref, g_ref, _ = adjoint.eps_r_gradient(_small_design(eps.detach().numpy(), coef_dtype=np.float32), cbox, f,
+                                             checkpoint_every=100)
+    assert float(J) == pytest.approx(J_ref, rel=1e-6)
+    np.testing.assert_allclose(eps.grad.numpy(), 2.0 * g_ref, rtol=1e-6, atol=1e-9 * np.abs(g_ref).max())

@@ -162,7 +162,12 @@ def test_microstrip_4000_steps(hip_ready):
 
 def test_large_grid_parity_and_linearity(hip_ready):
     """Full 400x400x200 grid: 12 steps against the oracle, and linearity (source x2 gives fields x2
-    exactly, since scaling by 2 commutes with rounding) after 60 steps."""
+    exactly, since scaling by 2 commutes with rounding) after 60 steps.  Rounding is relative only
+    above FLT_MIN: the wavefront's leading edge decays into subnormals, where the grid is absolute
+    and x2 is no longer exact.  Those differences (measured: at most 4.7e-38 absolute) then travel
+    with the field and flip the last bit of a few values up to ~1e-31 (measured: 199 values, 2.9e-7
+    relative); far above that a flip needs a 1e-12-probability coincidence, so values of magnitude
+    >= 1e-25 must match exactly."""
     print("[case] 400x400x200 parity", flush=True)
     model = workloads.dipole_slab(Nt=12, monitors=True)
     p_c, m_c, f_c = run_oracle(model, np.float32, n_threads=16)
@@ -182,7 +187,10 @@ def test_large_grid_parity_and_linearity(hip_ready):
             s.synchronize()
             res.append(s.fields(("ez", "hx")))
     for k in ("ez", "hx"):
-        np.testing.assert_array_equal(res[1][k], 2 * res[0][k])
+        big = np.abs(res[0][k]) >= 1e-25
+        np.testing.assert_array_equal(res[1][k][big], 2 * res[0][k][big])
+        np.testing.assert_allclose(res[1][k][~big], 2 * res[0][k][~big], rtol=1e-6, atol=1e-36)
+        assert rel_l2(res[1][k], 2 * res[0][k]) < 1e-7
     assert np.abs(res[0]["ez"]).max() > 0
 
 
