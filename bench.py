@@ -1,15 +1,21 @@
 #!/usr/bin/env python3
-"""FDTD throughput on one MI355X: Mcell-updates/s and % of HBM peak.
+"""FDTD throughput on one MI355X: Mcell-updates/s and % of HBM peak, batched vs plain, adjoint gradients/s.
 
-Workload (default): the 400x400x200 dipole-over-random-slab model (32 M cells, 12-cell PML on all
-faces, a 1-cell 50 ohm gap source, running DFTs at 51 frequencies on 6 planes), float32.  One bench
-step is one full FDTD time step: E update, E sources/probes, H update, H probes, DFT accumulation.
-Inputs (fields, coefficients, source series, DFT tables) are resident in HBM before timing starts.
+Headline (the "plain" leg): the 400x400x200 dipole-over-random-slab model (32 M cells, 12-cell PML on
+all faces, a 1-cell 50 ohm gap source, running DFTs at 51 frequencies on 6 planes), float32, 8000
+timed steps.  One bench step is one full FDTD time step: E update, E sources/probes, H update, H
+probes, DFT accumulation.  Inputs (fields, coefficients, source series, DFT tables) are resident in
+HBM before timing starts.  The JSON line carries the roofline of the H update kernel (144 algorithmic
+bytes per cell: 6 split H read and written, 18 coefficients, 3 E totals read, 3 H totals written;
+the E update is 120 B, a step 264 B), its profiler-measured HBM bytes, a copy-kernel bandwidth, and
+the CPU oracle (a restatement of the reference's threaded C++ solver) timed on this host over a
+bounded window of the same workload.
 
-Prints one JSON line with the throughput, the roofline of the dominant kernel (H update: 144
-algorithmic bytes per cell: 6 split H read and written, 18 coefficients, 3 E totals read, 3 H totals
-written), a measured copy-kernel bandwidth, and the CPU oracle (a restatement of the reference's
-threaded C++ solver) timed on this host over a bounded window of the same workload.
+Further legs in the same line: "batched" -- 4 uneven multi-port grids x every port driven (12
+excitations, 5000 steps) in one batched session vs the same excitations one after another;
+"adjoint" -- gradients/s of dJ/deps_r over a 64x64x32 design box of a 200x160x80 model (6000 steps,
+checkpoints every 200).  "adjoint_fd" (on request) validates that gradient against central
+differences on 32 cells.
 """
 import argparse
 import json
@@ -26,6 +32,12 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 E_BYTES, H_BYTES = 120, 144  # algorithmic bytes per cell of the E and H half steps (sum 264)
 METRIC = "Mcell-updates/s + % HBM peak, batched vs plain; adjoint gradients/s, 1 GPU"
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X (KiB per
+# dispatch x 1024, FETCH_SIZE x 2 and WRITE_SIZE x 1: the factors that make the counters of a plain
+# 1 GiB copy kernel read 1 GiB each way); profiles/k_update_eh_dipole_slab_pmc.txt
+MEASURED_TRAFFIC = {
+    ("dipole_slab", "table"): dict(k_update_e=2646090000, k_update_h=2522127000),
+}
 
 
 def log(msg):
@@ -85,7 +97,7 @@ def cpu_baseline(model, budget_s):
                        f"per step, -O2 -ffp-contract=off), {dt:.1f} s wall")
 
 
-def batched_leg(n_steps):
+def batched_leg(n_steps, flags=0):
     """The 4 uneven port variants x every port driven (12 excitations) advanced in one batched
     session, against the same 12 excitations run one after another through single-grid sessions.
     Both paths are prepared (coefficients, sources resident) before their timed runs; results must
@@ -106,7 +118,7 @@ def batched_leg(n_steps):
     log(f"batched: {len(models)} grids, {sum(len(e) for _, _, e, _ in inputs)} excitations, "
         f"{exc_cells} excitation-cells, built in {time.perf_counter() - t0:.1f}s")
 
-    b = rb.Batch(grids)
+    b = rb.Batch(grids, flags=flags)
     log(f"batched: autotune {b.autotune()}")
     k = 0
     for _, _, exc, Nt in inputs:
@@ -127,7 +139,7 @@ def batched_leg(n_steps):
         for port in range(len(exc)):
             exc2, _ = rb.pack_excitations(m, [port + 1])
             _, probes, monitors = exc2[0]
-            s = Session(coeff, m.Nx, m.Ny, m.Nz)
+            s = Session(coeff, m.Nx, m.Ny, m.Nz, flags=flags)
             s.autotune()
             s.set_sources(probes, monitors, Nt)
             sessions.append((s, probes))
@@ -142,15 +154,64 @@ def batched_leg(n_steps):
         identical &= all(np.array_equal(p["values"], r) for p, r in zip(probes, ref))
         s.close()
     log(f"sequential: {wall_s:.3f}s, identical {identical}")
+
+    # the same sequence through the reference formulation of the plain kernel: 30 raw coefficient
+    # arrays per cell and one launch per kernel per step (no table, no graph replay)
+    from rfnetwork_amd import _lib
+
+    plain = []
+    for m, coeff, exc, Nt in inputs:
+        for port in range(len(exc)):
+            exc2, _ = rb.pack_excitations(m, [port + 1])
+            _, probes, monitors = exc2[0]
+            s = Session(coeff, m.Nx, m.Ny, m.Nz, flags=flags | _lib.COEF_RAW | _lib.NO_GRAPHS)
+            s.set_sources(probes, monitors, Nt)
+            plain.append((s, probes))
+    t0 = time.perf_counter()
+    for s, _ in plain:
+        s.run(n_steps)
+        s.synchronize()
+    wall_p = time.perf_counter() - t0
+    for (s, probes), ref in zip(plain, batched_series):
+        s.read()
+        identical &= all(np.array_equal(p["values"], r) for p, r in zip(probes, ref))
+        s.close()
+    log(f"sequential, raw coefficients and per-step launches: {wall_p:.3f}s, identical {identical}")
+    del sessions, plain, batched_series
+
+    # end to end, as a user runs it: S-parameter columns of every port of every variant, either one
+    # model.solve() per driven port (packing, coefficient upload, run, read back each time; the
+    # reference's workflow) or solve_ports() (one batched session for all)
+    freq = models[0].frequency
+    jobs = [(m, list(range(1, len(m.ports) + 1))) for m in models]
+    waves = [np.asarray(rb._waveform(m)) for m in models]
+    t0 = time.perf_counter()
+    res = rb.solve_ports(jobs, freq, flags=flags)
+    e2e_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cols = []
+    for (m, ports), wave in zip(jobs, waves):
+        for port in ports:
+            m.reset_excitations()
+            m.assign_excitation(wave, port)
+            m.solve(gpu=True, show_progress=False, flags=flags)
+            cols.append(np.asarray(m.get_sparameters(freq, source_port=port))[..., 0])
+    e2e_s = time.perf_counter() - t0
+    ref = [S[..., c] for S, _ in res for c in range(S.shape[-1])]
+    same = all(np.array_equal(a, b) for a, b in zip(ref, cols))
+    log(f"end to end: solve_ports {e2e_b:.3f}s, sequential solve() {e2e_s:.3f}s, identical S {same}")
     b_rate = exc_cells * n_steps / wall_b / 1e6
     s_rate = exc_cells * n_steps / wall_s / 1e6
     return dict(workload="4 port variants 64^3..192x128x96, 12 excitations", steps=n_steps,
                 excitation_cells=exc_cells, batched_mcells=round(b_rate, 1), sequential_mcells=round(s_rate, 1),
                 speedup=round(wall_s / wall_b, 3), batched_s=round(wall_b, 3), sequential_s=round(wall_s, 3),
-                identical=bool(identical))
+                sequential_plain_formulation_s=round(wall_p, 3), speedup_vs_plain_formulation=round(wall_p / wall_b, 3),
+                identical=bool(identical),
+                end_to_end=dict(solve_ports_s=round(e2e_b, 3), sequential_solve_s=round(e2e_s, 3),
+                                speedup=round(e2e_s / e2e_b, 3), identical=bool(same)))
 
 
-def adjoint_leg(n_steps, every, repeats=2):
+def adjoint_leg(n_steps, every, repeats=2, flags=0):
     """dJ/deps_r of the 64x64x32 design box of the 200x160x80 two-port model, J = sum |S21|^2 at
     2..6 GHz: one gradient = checkpointed forward + reverse pass with recomputation; the best of
     `repeats` end-to-end evaluations (model assembly excluded, host objective included)."""
@@ -162,7 +223,7 @@ def adjoint_leg(n_steps, every, repeats=2):
     for _ in range(repeats):
         t0 = time.perf_counter()
         J, grad, S, t = adjoint.eps_r_gradient(model, cbox, model.frequency, checkpoint_every=every,
-                                               return_session_timing=True)
+                                               return_session_timing=True, flags=flags)
         wall = time.perf_counter() - t0
         if best is None or wall < best[0]:
             best = (wall, t, J, grad)
@@ -177,6 +238,56 @@ def adjoint_leg(n_steps, every, repeats=2):
                 finite=bool(np.isfinite(grad).all()))
 
 
+def adjoint_fd_leg(n_steps, every, n_cells=32, seed=41, h=1e-3, oracle_cells=1):
+    """Validation (not a throughput figure): the adjoint dJ/deps_r of the 200x160x80 design model in
+    float64 and float32 against central differences (Delta eps_r = h) of float64 forward solves on
+    n_cells random design cells.  The differences use the float64 HIP engine, which reproduces the
+    float64 oracle bit for bit; for the first `oracle_cells` cells the oracle itself is run too and
+    its J must equal the engine's exactly."""
+    from oracle import cpu_solver
+    from rfnetwork_amd import adjoint, workloads
+
+    eps0 = np.random.default_rng(4).uniform(1.0, 4.0, size=(64, 64, 32))
+    cbox = workloads.design_box()
+    build = lambda e, dt=np.float64: workloads.design_strip(e, Nt=n_steps, coef_dtype=dt)
+    m64 = build(eps0)
+    t0 = time.perf_counter()
+    J64, g64, _ = adjoint.eps_r_gradient(m64, cbox, m64.frequency, checkpoint_every=every, dtype=np.float64)
+    log(f"adjoint_fd: float64 gradient in {time.perf_counter() - t0:.1f}s, J {J64:.9e}")
+    m32 = build(eps0, np.float32)
+    J32, g32, _ = adjoint.eps_r_gradient(m32, cbox, m32.frequency, checkpoint_every=every, dtype=np.float32)
+    rng = np.random.default_rng(seed)
+    cells = [tuple(int(rng.integers(0, n)) for n in (64, 64, 32)) for _ in range(n_cells)]
+    fd, oracle_equal = [], []
+    threads = max(1, min(16, os.cpu_count() or 1))
+    for k, c in enumerate(cells):
+        Js = []
+        for sgn in (1, -1):
+            e = eps0.copy()
+            e[c] += sgn * h
+            m = build(e)
+            J = adjoint.objective_only(m, m.frequency, dtype=np.float64)[0]
+            if k < oracle_cells:
+                Jo = adjoint.objective_only(build(e), m.frequency, dtype=np.float64, backend=cpu_solver.solver_run,
+                                            n_threads=threads)[0]
+                oracle_equal.append(Jo == J)
+            Js.append(J)
+        fd.append((Js[0] - Js[1]) / (2 * h))
+        log(f"adjoint_fd: cell {k} {c}: fd {fd[-1]:.9e} adjoint64 {g64[c]:.9e} adjoint32 {g32[c]:.9e}")
+    fd = np.array(fd)
+    a64 = np.array([g64[c] for c in cells])
+    a32 = np.array([g32[c] for c in cells])
+    r64 = np.abs(a64 - fd) / np.abs(fd)
+    r32 = np.abs(a32 - fd) / np.abs(fd)
+    return dict(workload=f"design_strip 200x160x80, design box 64x64x32, {n_steps} steps, checkpoints every {every}",
+                cells=n_cells, seed=seed, delta_eps_r=h, J64=J64, J32=J32,
+                max_rel_err_fp64=float(r64.max()), median_rel_err_fp64=float(np.median(r64)),
+                max_rel_err_fp32=float(r32.max()), median_rel_err_fp32=float(np.median(r32)),
+                normwise_rel_err_fp32=float(np.linalg.norm(a32 - fd) / np.linalg.norm(fd)),
+                fp32_vs_fp64_gradient_rel_l2=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+                oracle_J_equal=oracle_equal, pass_1e3=bool(r64.max() <= 1e-3))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,10 +300,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--legs", default="plain,batched,adjoint",
-                    help="comma list: plain (the headline), batched (12 excitations), adjoint (gradients/s)")
+                    help="comma list: plain (the headline), batched (12 excitations), adjoint (gradients/s), "
+                         "adjoint_fd (finite-difference validation, minutes)")
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--adjoint-steps", type=int, default=6000)
     ap.add_argument("--checkpoint-every", type=int, default=200)
+    ap.add_argument("--no-graphs", action="store_true", help="launch every step directly (no hipGraph replay)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -201,9 +314,28 @@ def main():
         sys.exit(2)
 
     from rfnetwork_amd import _lib
-    from rfnetwork_amd.session import Session
 
     _lib.lib()  # fails loudly if the HIP library is not built
+    out = dict(metric=METRIC, value=None, unit="Mcell-updates/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+               data="synthetic")
+    if "plain" in legs:
+        out.update(plain_leg(args))
+    gflags = _lib.NO_GRAPHS if args.no_graphs else 0
+    if "batched" in legs:
+        out["batched"] = batched_leg(args.batch_steps, gflags)
+    if "adjoint" in legs:
+        out["adjoint"] = adjoint_leg(args.adjoint_steps, args.checkpoint_every, flags=gflags)
+    if "adjoint_fd" in legs:
+        out["adjoint_fd"] = adjoint_fd_leg(args.adjoint_steps, args.checkpoint_every)
+    print(json.dumps(out), flush=True)
+
+
+def plain_leg(args):
+    """The headline: K timed steps of the single-grid workload after W warm-up steps."""
+    from rfnetwork_amd import _lib
+    from rfnetwork_amd.session import Session
+
     Nt = args.warmup + args.steps
     t0 = time.perf_counter()
     model = build_model(args.workload, Nt)
@@ -245,6 +377,7 @@ def main():
     step_frac = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     h_gbs = cells * H_BYTES / (h_ms * 1e-3) / 1e9 if h_ms > 0 else 0.0
     e_gbs = cells * E_BYTES / (e_ms * 1e-3) / 1e9 if e_ms > 0 else 0.0
+    traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"]))
 
     copy = None
     if not args.no_copy_probe:
@@ -258,26 +391,21 @@ def main():
         base = cpu_baseline(model, args.cpu_budget)
         log(f"cpu baseline {base['value']} Mcell/s on {base['cores']} threads")
 
-    batched = batched_leg(args.batch_steps) if "batched" in legs else None
-    adj = adjoint_leg(args.adjoint_steps, args.checkpoint_every) if "adjoint" in legs else None
-
-    out = dict(
-        metric=METRIC, value=round(mcells, 1), unit="Mcell-updates/s", n_gpus=1, steps=args.steps,
-        warmup=args.warmup, ms_per_step=round(ms_step, 4), higher_is_better=True, scaling="weak",
-        vs_baseline=None, dtype="f32", data="synthetic",
+    return dict(
+        value=round(mcells, 1), ms_per_step=round(ms_step, 4),
         config=dict(workload=f"{args.workload} {model.Nx}x{model.Ny}x{model.Nz}", cells=cells,
                     dft_planes=len(model.monitors), dft_frequencies=51 if model.monitors else 0,
                     coefficients=info["coef_kind"], coef_entries=[info["n_e_entries"], info["n_h_entries"]],
                     launch=tuned),
         roofline=dict(bound="hbm", kernel="k_update_h", achieved=round(h_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(h_gbs / HBM_PEAK_GBS, 4), traffic=None,
-                      algorithmic_bytes_per_launch=cells * H_BYTES, launch_ms=round(h_ms, 4)),
+                      frac=round(h_gbs / HBM_PEAK_GBS, 4), traffic=traffic and traffic["k_update_h"],
+                      algorithmic_bytes_per_launch=cells * H_BYTES, launch_ms=round(h_ms, 4),
+                      measured_hbm_gbs=round(traffic["k_update_h"] / (h_ms * 1e-3) / 1e9, 1) if traffic and h_ms else None),
         step=dict(hbm_frac_algorithmic=round(step_frac, 4), algorithmic_bytes=step_bytes,
                   e_kernel_ms=round(e_ms, 4), e_kernel_gbs=round(e_gbs, 1), h_kernel_ms=round(h_ms, 4),
-                  dft_bytes=dft_b),
-        copy_probe=copy, cpu_baseline=base, finite=finite, batched=batched, adjoint=adj,
+                  e_traffic=traffic and traffic["k_update_e"], dft_bytes=dft_b),
+        copy_probe=copy, cpu_baseline=base, finite=finite,
     )
-    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -30,6 +30,7 @@ extern "C" {
 #define RFDTD_EINVAL 1   /* bad argument (shape, index, count, dtype) */
 #define RFDTD_EDEVICE 2  /* HIP runtime error */
 #define RFDTD_ESTATE 3   /* call out of order (e.g. run past the probe series) */
+#define RFDTD_ENOMEM 4   /* the request does not fit in device memory */
 
 /* flags for rfdtd_create / rfdtd_solver_run */
 #define RFDTD_F64 0x1u            /* fields, coefficients, probe series and monitors in double */
@@ -165,6 +166,17 @@ int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3],
 int rfdtd_set_checkpointing(rfdtd_session* s, int32_t every);
 int rfdtd_gradient(rfdtd_session* s, int32_t instance, const int32_t box[6], const void* const* adjoint_series,
                    int32_t n_series, double* grad_out);
+/* The same for n instances of a batched session at once (their reverse passes share every launch):
+ * instances[k], boxes[6k .. 6k+6), adjoint_series[k][p], n_series[k], grad_out[k] as above.  An
+ * instance appears at most once. */
+int rfdtd_gradient_batch(rfdtd_session* s, int32_t n, const int32_t* instances, const int32_t* boxes,
+                         const void* const* const* adjoint_series, const int32_t* n_series, double* const* grad_out);
+/* Checkpoint interval for such a gradient after an Nt-step forward run: the K that minimises the
+ * memory of ceil(Nt/K) checkpoint sets (12 split arrays of every instance) plus K per-step H slots
+ * of the design boxes, with the adjoint buffers added; fails with RFDTD_ENOMEM when even that
+ * exceeds budget_bytes (0: the device's free memory less 1 GiB).  Returns K and the bytes needed. */
+int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_t* instances, const int32_t* boxes,
+                           uint64_t budget_bytes, int32_t* every_out, uint64_t* bytes_out);
 
 /* Kernel variant (no counterpart in the reference): pipelined != 0 (default) issues the loads of the
  * next x plane before the arithmetic and stores of the current one; 0 loads one plane at a time.

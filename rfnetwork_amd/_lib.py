@@ -48,7 +48,8 @@ EXPORTS = (
     "rfdtd_read_fields", "rfdtd_reset", "rfdtd_step", "rfdtd_last_timing", "rfdtd_memory_info",
     "rfdtd_set_launch", "rfdtd_destroy", "rfdtd_copy_bandwidth", "rfdtd_create_batch",
     "rfdtd_instance_count", "rfdtd_set_instance_sources", "rfdtd_read_instance_fields", "rfdtd_set_variant",
-    "rfdtd_set_checkpointing", "rfdtd_gradient", "rfdtd_autotune",
+    "rfdtd_set_checkpointing", "rfdtd_gradient", "rfdtd_autotune", "rfdtd_gradient_batch",
+    "rfdtd_plan_checkpoints",
 )
 
 
@@ -128,6 +129,9 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_set_checkpointing.argtypes = [vp, i32]
     L.rfdtd_gradient.argtypes = [vp, i32, vp, vp, i32, vp]
     L.rfdtd_autotune.argtypes = [vp, i32, vp, ctypes.POINTER(ctypes.c_double)]
+    L.rfdtd_gradient_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.rfdtd_plan_checkpoints.argtypes = [vp, i32, i32, vp, vp, ctypes.c_uint64, ctypes.POINTER(i32),
+                                         ctypes.POINTER(ctypes.c_uint64)]
     L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]
     _lib = L

@@ -198,15 +198,65 @@ def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=
     return J, grad, S
 
 
-def objective_only(model, frequency, b_port=2, a_port=1, dtype=np.float32, backend=None):
-    """J of the model from a plain solve (GPU by default; backend: a solver_run-style callable)."""
+def eps_r_gradients_batch(jobs, b_port=2, a_port=1, checkpoint_every=0, dtype=np.float32, flags=0, tune=True,
+                          budget_bytes=0, return_info=False):
+    """Gradients of several models at once: one batched session holds every (meshed, excited)
+    model; their forward runs and their reverse passes share every launch.
+
+    jobs: [(model, cbox, frequency)], all models with the same number of steps.  checkpoint_every
+    0 picks the interval that fits device memory (budget_bytes, 0: what is free).  Returns
+    [(J, dJ/deps_r over cbox, S_ba)] per job (and a dict of timings and the interval)."""
+    from .batch import Batch
+
+    grids, packed, Nt_all = [], [], None
+    for model, cbox, _ in jobs:
+        coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype)
+        if Nt_all is None:
+            Nt_all = Nt
+        elif Nt != Nt_all:
+            raise ValueError("all models of a batched gradient must run the same number of steps")
+        grids.append((coeff, Nx, Ny, Nz, 1))
+        packed.append((coeff, probes, gpu_box_for_cells(model, cbox)))
+    info = {}
+    with Batch(grids, dtype=dtype, flags=flags) as b:
+        if tune:
+            info["launch"] = b.autotune()
+        every = int(checkpoint_every)
+        if every <= 0:
+            every, info["planned_bytes"] = b.plan_checkpoints(Nt_all, [(k, p[2]) for k, p in enumerate(packed)],
+                                                              budget_bytes)
+        info["checkpoint_every"] = every
+        b.set_checkpointing(every)
+        for k, (_, probes, _) in enumerate(packed):
+            b.set_sources(k, probes, [], Nt_all)
+        b.run(Nt_all)
+        b.synchronize()
+        info["forward_ms"] = b.timing()[0]
+        b.read()
+        requests, objectives = [], []
+        for k, ((model, cbox, frequency), (_, probes, gbox)) in enumerate(zip(jobs, packed)):
+            series = np.array([p["values"][:Nt_all] for p in probes])
+            J, S, g = sparam_objective(model, series, frequency, b_port, a_port)
+            objectives.append((J, S))
+            requests.append((k, gbox, [row.astype(dtype) for row in g]))
+        grads = b.gradients(requests)
+        info["adjoint_ms"] = b.timing()[0]
+    out = []
+    for (model, cbox, _), (coeff, _, gbox), (J, S), g in zip(jobs, packed, objectives, grads):
+        out.append((J, cb_to_eps_r_gradient(model, coeff, g, gbox, cbox), S))
+    return (out, info) if return_info else out
+
+
+def objective_only(model, frequency, b_port=2, a_port=1, dtype=np.float32, backend=None, n_threads=4):
+    """J of the model from a plain solve (GPU by default; backend: a solver_run-style callable run
+    with n_threads)."""
     gpu = backend is None
-    coeff, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=gpu, dtype=dtype)
+    coeff, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=gpu, dtype=dtype, n_threads=n_threads)
     if backend is None:
         from .hip_func import solver_run_hip
         solver_run_hip(coeff, probes, [], mem, Nx, Ny, Nz, Nt, 0, 0, dtype=dtype)
     else:
-        backend(coeff, probes, [], mem, Nx, Ny, Nz, Nt, 4, 0, dtype=dtype)
+        backend(coeff, probes, [], mem, Nx, Ny, Nz, Nt, n_threads, 0, dtype=dtype)
     series = np.array([p["values"][:Nt] for p in probes])
     J, S, _ = sparam_objective(model, series, frequency, b_port, a_port)
     return J, S

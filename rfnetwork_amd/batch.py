@@ -102,6 +102,51 @@ class Batch:
         check(self._L.rfdtd_memory_info(self._h, None, ctypes.byref(kind), None, None))
         return autotune.tune(self, self.grids, np.dtype(self.dtype).name, kind.value, trial_steps, use_cache)
 
+    def set_checkpointing(self, every):
+        """Save the split fields of every instance every `every` steps of later runs."""
+        check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
+
+    def plan_checkpoints(self, Nt, requests, budget_bytes=0):
+        """Checkpoint interval for gradients of requests [(instance, box), ...] after Nt steps that
+        fits device memory (budget_bytes, 0: what is free).  Returns (every, bytes needed)."""
+        n = len(requests)
+        inst = (ctypes.c_int32 * n)(*[int(r[0]) for r in requests])
+        boxes = (ctypes.c_int32 * (6 * n))(*[int(v) for r in requests for v in r[1]])
+        every, need = ctypes.c_int32(), ctypes.c_uint64()
+        check(self._L.rfdtd_plan_checkpoints(self._h, int(Nt), n, ctypes.cast(inst, ctypes.c_void_p),
+                                             ctypes.cast(boxes, ctypes.c_void_p), int(budget_bytes),
+                                             ctypes.byref(every), ctypes.byref(need)), "rfdtd_plan_checkpoints")
+        return every.value, need.value
+
+    def gradients(self, requests):
+        """dJ/dCb of the six E splits in each request's box after a checkpointed forward run, for
+        requests [(instance, box (x0, x1, y0, y1, z0, z1), adjoint_series per probe)], all reverse
+        passes in the same launches.  Returns a float64 [6, nx, ny, nz] array per request."""
+        n = len(requests)
+        outs, keep = [], []
+        inst = (ctypes.c_int32 * n)()
+        boxes = (ctypes.c_int32 * (6 * n))()
+        series_ptrs = (ctypes.c_void_p * n)()
+        counts = (ctypes.c_int32 * n)()
+        out_ptrs = (ctypes.c_void_p * n)()
+        for k, (instance, box, adj) in enumerate(requests):
+            box = [int(v) for v in box]
+            inst[k] = int(instance)
+            boxes[6 * k:6 * k + 6] = box
+            arrs = [np.ascontiguousarray(a, dtype=self.dtype) for a in adj]
+            ptrs = _lib.pointer_array(arrs)
+            keep += [arrs, ptrs]
+            series_ptrs[k] = ctypes.cast(ptrs, ctypes.c_void_p).value
+            counts[k] = len(arrs)
+            out = np.zeros((6, box[1] - box[0], box[3] - box[2], box[5] - box[4]), np.float64)
+            outs.append(out)
+            out_ptrs[k] = out.ctypes.data
+        check(self._L.rfdtd_gradient_batch(self._h, n, ctypes.cast(inst, ctypes.c_void_p),
+                                           ctypes.cast(boxes, ctypes.c_void_p), ctypes.cast(series_ptrs, ctypes.c_void_p),
+                                           ctypes.cast(counts, ctypes.c_void_p), ctypes.cast(out_ptrs, ctypes.c_void_p)),
+              "rfdtd_gradient_batch")
+        return outs
+
     def set_variant(self, pipelined=1):
         check(self._L.rfdtd_set_variant(self._h, int(pipelined)))
 

@@ -484,87 +484,125 @@ enum {  // coefficient positions
     C_DB_YX2, C_DA_ZX, C_DA_ZY, C_DB_ZX1, C_DB_ZX2, C_DB_ZY1, C_DB_ZY2
 };
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_e(const InstDev<T>* __restrict__ pA) {
-    const InstDev<T>& A = *pA;
-    const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz;
-    const uint32_t cells = (uint32_t)Nx * Ny * Nz, S = (uint32_t)sizeof(T);
-    const uint32_t sy = (uint32_t)Nz * S, sx = (uint32_t)Ny * Nz * S;
-    using G = GLOBAL_T(T);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
-        const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
-        const uint32_t b = i * S;
-        const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
-        const G* mxy = (const G*)A.F.hx_y; const G* mxz = (const G*)A.F.hx_z;
-        const G* myz = (const G*)A.F.hy_z; const G* myx = (const G*)A.F.hy_x;
-        const G* mzx = (const G*)A.F.hz_x; const G* mzy = (const G*)A.F.hz_y;
-        // E totals' adjoints: every H split that reads E at this cell (as e or as e- of a neighbour)
-        T ez = coef_at(A, C_DB_XY2, b) * ldg(mxy, b) + coef_at(A, C_DB_YX2, b) * ldg(myx, b);
-        if (yin) ez = ez - coef_at(A, C_DB_XY1, b + sy) * ldg(mxy, b + sy);
-        if (xin) ez = ez - coef_at(A, C_DB_YX1, b + sx) * ldg(myx, b + sx);
-        T ey = coef_at(A, C_DB_XZ2, b) * ldg(mxz, b) + coef_at(A, C_DB_ZX2, b) * ldg(mzx, b);
-        if (zin) ey = ey - coef_at(A, C_DB_XZ1, b + S) * ldg(mxz, b + S);
-        if (xin) ey = ey - coef_at(A, C_DB_ZX1, b + sx) * ldg(mzx, b + sx);
-        T ex = coef_at(A, C_DB_YZ2, b) * ldg(myz, b) + coef_at(A, C_DB_ZY2, b) * ldg(mzy, b);
-        if (zin) ex = ex - coef_at(A, C_DB_YZ1, b + S) * ldg(myz, b + S);
-        if (yin) ex = ex - coef_at(A, C_DB_ZY1, b + sy) * ldg(mzy, b + sy);
-        G* nxy = (G*)A.F.ex_y; G* nxz = (G*)A.F.ex_z; G* nyz = (G*)A.F.ey_z;
-        G* nyx = (G*)A.F.ey_x; G* nzx = (G*)A.F.ez_x; G* nzy = (G*)A.F.ez_y;
-        stg(nxy, b, (yin ? coef_at(A, C_CA_XY, b) : T(1)) * ldg(nxy, b) + ex);
-        stg(nxz, b, (zin ? coef_at(A, C_CA_XZ, b) : T(1)) * ldg(nxz, b) + ex);
-        stg(nyz, b, (zin ? coef_at(A, C_CA_YZ, b) : T(1)) * ldg(nyz, b) + ey);
-        stg(nyx, b, (xin ? coef_at(A, C_CA_YX, b) : T(1)) * ldg(nyx, b) + ey);
-        stg(nzx, b, (xin ? coef_at(A, C_CA_ZX, b) : T(1)) * ldg(nzx, b) + ez);
-        stg(nzy, b, (yin ? coef_at(A, C_CA_ZY, b) : T(1)) * ldg(nzy, b) + ez);
-    }
+// Batched flat launches: item i (an instance, a design box) owns blocks [blk0[i], blk0[i+1]).
+__device__ __forceinline__ int item_of(const int64_t* __restrict__ blk0, int n, int64_t b) {
+    int i = 0;
+    while (i + 1 < n && blk0[i + 1] <= b) ++i;
+    return i;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_h(const InstDev<T>* __restrict__ pA) {
-    const InstDev<T>& A = *pA;
+__device__ __forceinline__ void adj_e_cell(const InstDev<T>& A, uint32_t i) {
     const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz;
-    const uint32_t cells = (uint32_t)Nx * Ny * Nz, S = (uint32_t)sizeof(T);
+    const uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sy = (uint32_t)Nz * S, sx = (uint32_t)Ny * Nz * S;
+    using G = GLOBAL_T(T);
+    const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
+    const uint32_t b = i * S;
+    const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
+    const G* mxy = (const G*)A.F.hx_y; const G* mxz = (const G*)A.F.hx_z;
+    const G* myz = (const G*)A.F.hy_z; const G* myx = (const G*)A.F.hy_x;
+    const G* mzx = (const G*)A.F.hz_x; const G* mzy = (const G*)A.F.hz_y;
+    // E totals' adjoints: every H split that reads E at this cell (as e or as e- of a neighbour)
+    T ez = coef_at(A, C_DB_XY2, b) * ldg(mxy, b) + coef_at(A, C_DB_YX2, b) * ldg(myx, b);
+    if (yin) ez = ez - coef_at(A, C_DB_XY1, b + sy) * ldg(mxy, b + sy);
+    if (xin) ez = ez - coef_at(A, C_DB_YX1, b + sx) * ldg(myx, b + sx);
+    T ey = coef_at(A, C_DB_XZ2, b) * ldg(mxz, b) + coef_at(A, C_DB_ZX2, b) * ldg(mzx, b);
+    if (zin) ey = ey - coef_at(A, C_DB_XZ1, b + S) * ldg(mxz, b + S);
+    if (xin) ey = ey - coef_at(A, C_DB_ZX1, b + sx) * ldg(mzx, b + sx);
+    T ex = coef_at(A, C_DB_YZ2, b) * ldg(myz, b) + coef_at(A, C_DB_ZY2, b) * ldg(mzy, b);
+    if (zin) ex = ex - coef_at(A, C_DB_YZ1, b + S) * ldg(myz, b + S);
+    if (yin) ex = ex - coef_at(A, C_DB_ZY1, b + sy) * ldg(mzy, b + sy);
+    G* nxy = (G*)A.F.ex_y; G* nxz = (G*)A.F.ex_z; G* nyz = (G*)A.F.ey_z;
+    G* nyx = (G*)A.F.ey_x; G* nzx = (G*)A.F.ez_x; G* nzy = (G*)A.F.ez_y;
+    stg(nxy, b, (yin ? coef_at(A, C_CA_XY, b) : T(1)) * ldg(nxy, b) + ex);
+    stg(nxz, b, (zin ? coef_at(A, C_CA_XZ, b) : T(1)) * ldg(nxz, b) + ex);
+    stg(nyz, b, (zin ? coef_at(A, C_CA_YZ, b) : T(1)) * ldg(nyz, b) + ey);
+    stg(nyx, b, (xin ? coef_at(A, C_CA_YX, b) : T(1)) * ldg(nyx, b) + ey);
+    stg(nzx, b, (xin ? coef_at(A, C_CA_ZX, b) : T(1)) * ldg(nzx, b) + ez);
+    stg(nzy, b, (yin ? coef_at(A, C_CA_ZY, b) : T(1)) * ldg(nzy, b) + ez);
+}
+
+template <typename T>
+__device__ __forceinline__ void adj_h_cell(const InstDev<T>& A, uint32_t i) {
+    const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz;
+    const uint32_t S = (uint32_t)sizeof(T);
     const uint32_t sy = (uint32_t)Nz * S, sx = (uint32_t)Ny * Nz * S;
     using G = GLOBAL_T(T);
     const G* nxy = (const G*)A.F.ex_y; const G* nxz = (const G*)A.F.ex_z; const G* nyz = (const G*)A.F.ey_z;
     const G* nyx = (const G*)A.F.ey_x; const G* nzx = (const G*)A.F.ez_x; const G* nzy = (const G*)A.F.ez_y;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
-        const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
-        const uint32_t b = i * S;
-        const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
-        // H totals' adjoints: every E split whose curl difference reads H at this cell
-        T hz = T(0), hy = T(0), hx = T(0);
-        if (yin) hz = hz - coef_at(A, C_CB_XY, b) * ldg(nxy, b);
-        if (y > 0) hz = hz + coef_at(A, C_CB_XY, b - sy) * ldg(nxy, b - sy);
-        if (xin) hz = hz - coef_at(A, C_CB_YX, b) * ldg(nyx, b);
-        if (x > 0) hz = hz + coef_at(A, C_CB_YX, b - sx) * ldg(nyx, b - sx);
-        if (zin) hy = hy - coef_at(A, C_CB_XZ, b) * ldg(nxz, b);
-        if (z > 0) hy = hy + coef_at(A, C_CB_XZ, b - S) * ldg(nxz, b - S);
-        if (xin) hy = hy - coef_at(A, C_CB_ZX, b) * ldg(nzx, b);
-        if (x > 0) hy = hy + coef_at(A, C_CB_ZX, b - sx) * ldg(nzx, b - sx);
-        if (zin) hx = hx - coef_at(A, C_CB_YZ, b) * ldg(nyz, b);
-        if (z > 0) hx = hx + coef_at(A, C_CB_YZ, b - S) * ldg(nyz, b - S);
-        if (yin) hx = hx - coef_at(A, C_CB_ZY, b) * ldg(nzy, b);
-        if (y > 0) hx = hx + coef_at(A, C_CB_ZY, b - sy) * ldg(nzy, b - sy);
-        G* mxy = (G*)A.F.hx_y; G* mxz = (G*)A.F.hx_z; G* myz = (G*)A.F.hy_z;
-        G* myx = (G*)A.F.hy_x; G* mzx = (G*)A.F.hz_x; G* mzy = (G*)A.F.hz_y;
-        stg(mxy, b, coef_at(A, C_DA_XY, b) * ldg(mxy, b) + hx);
-        stg(mxz, b, coef_at(A, C_DA_XZ, b) * ldg(mxz, b) + hx);
-        stg(myz, b, coef_at(A, C_DA_YZ, b) * ldg(myz, b) + hy);
-        stg(myx, b, coef_at(A, C_DA_YX, b) * ldg(myx, b) + hy);
-        stg(mzx, b, coef_at(A, C_DA_ZX, b) * ldg(mzx, b) + hz);
-        stg(mzy, b, coef_at(A, C_DA_ZY, b) * ldg(mzy, b) + hz);
-    }
+    const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
+    const uint32_t b = i * S;
+    const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
+    // H totals' adjoints: every E split whose curl difference reads H at this cell
+    T hz = T(0), hy = T(0), hx = T(0);
+    if (yin) hz = hz - coef_at(A, C_CB_XY, b) * ldg(nxy, b);
+    if (y > 0) hz = hz + coef_at(A, C_CB_XY, b - sy) * ldg(nxy, b - sy);
+    if (xin) hz = hz - coef_at(A, C_CB_YX, b) * ldg(nyx, b);
+    if (x > 0) hz = hz + coef_at(A, C_CB_YX, b - sx) * ldg(nyx, b - sx);
+    if (zin) hy = hy - coef_at(A, C_CB_XZ, b) * ldg(nxz, b);
+    if (z > 0) hy = hy + coef_at(A, C_CB_XZ, b - S) * ldg(nxz, b - S);
+    if (xin) hy = hy - coef_at(A, C_CB_ZX, b) * ldg(nzx, b);
+    if (x > 0) hy = hy + coef_at(A, C_CB_ZX, b - sx) * ldg(nzx, b - sx);
+    if (zin) hx = hx - coef_at(A, C_CB_YZ, b) * ldg(nyz, b);
+    if (z > 0) hx = hx + coef_at(A, C_CB_YZ, b - S) * ldg(nyz, b - S);
+    if (yin) hx = hx - coef_at(A, C_CB_ZY, b) * ldg(nzy, b);
+    if (y > 0) hx = hx + coef_at(A, C_CB_ZY, b - sy) * ldg(nzy, b - sy);
+    G* mxy = (G*)A.F.hx_y; G* mxz = (G*)A.F.hx_z; G* myz = (G*)A.F.hy_z;
+    G* myx = (G*)A.F.hy_x; G* mzx = (G*)A.F.hz_x; G* mzy = (G*)A.F.hz_y;
+    stg(mxy, b, coef_at(A, C_DA_XY, b) * ldg(mxy, b) + hx);
+    stg(mxz, b, coef_at(A, C_DA_XZ, b) * ldg(mxz, b) + hx);
+    stg(myz, b, coef_at(A, C_DA_YZ, b) * ldg(myz, b) + hy);
+    stg(myx, b, coef_at(A, C_DA_YX, b) * ldg(myx, b) + hy);
+    stg(mzx, b, coef_at(A, C_DA_ZX, b) * ldg(mzx, b) + hz);
+    stg(mzy, b, coef_at(A, C_DA_ZY, b) * ldg(mzy, b) + hz);
 }
 
-// Adjoint sources: g[n] of every probe of a group added to both splits of its component.
+// One cell per thread over the cells of n instances; blk0: n + 1 block offsets.
 template <typename T>
-__global__ void k_adj_inject(const InstDev<T>* __restrict__ pA, const ProbeGroup<T>* __restrict__ groups,
-                             int ngroups, const T* __restrict__ g, int n) {
+__global__ void __launch_bounds__(256) k_adj_e(const InstDev<T>* __restrict__ insts, const int64_t* __restrict__ blk0,
+                                               int n) {
+    const int k = item_of(blk0, n, blockIdx.x);
+    const InstDev<T>& A = insts[k];
+    const int64_t i = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    if (i < (int64_t)A.Nx * A.Ny * A.Nz) adj_e_cell(A, (uint32_t)i);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_h(const InstDev<T>* __restrict__ insts, const int64_t* __restrict__ blk0,
+                                               int n) {
+    const int k = item_of(blk0, n, blockIdx.x);
+    const InstDev<T>& A = insts[k];
+    const int64_t i = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    if (i < (int64_t)A.Nx * A.Ny * A.Nz) adj_h_cell(A, (uint32_t)i);
+}
+
+// Totals from splits after a checkpoint restore (the forward computes them the same way).
+template <typename T>
+__global__ void __launch_bounds__(256) k_totals(const InstDev<T>* __restrict__ insts, const int64_t* __restrict__ blk0,
+                                                int n) {
+    const int k = item_of(blk0, n, blockIdx.x);
+    const InstDev<T>& I = insts[k];
+    const int64_t i = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)I.Nx * I.Ny * I.Nz) return;
+    const Fields<T>& F = I.F;
+    F.ex[i] = F.ex_y[i] + F.ex_z[i];
+    F.ey[i] = F.ey_z[i] + F.ey_x[i];
+    F.ez[i] = F.ez_x[i] + F.ez_y[i];
+    F.hx[i] = F.hx_y[i] + F.hx_z[i];
+    F.hy[i] = F.hy_z[i] + F.hy_x[i];
+    F.hz[i] = F.hz_x[i] + F.hz_y[i];
+}
+
+// Adjoint sources: g[n] of every probe of a group added to both splits of its component.  For
+// these groups `series` holds the instance's dJ/dy, time-major [Nt][nprobes].
+template <typename T>
+__global__ void k_adj_inject(const InstDev<T>* __restrict__ insts, const ProbeGroup<T>* __restrict__ groups,
+                             int ngroups, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ngroups) return;
     const ProbeGroup<T> Gp = groups[k];
-    const Fields<T>& F = pA->F;
+    const Fields<T>& F = insts[Gp.inst].F;
     T *s1, *s2;
     switch (Gp.field) {
         case EX: s1 = F.ex_y; s2 = F.ex_z; break;
@@ -574,86 +612,91 @@ __global__ void k_adj_inject(const InstDev<T>* __restrict__ pA, const ProbeGroup
         case HY: s1 = F.hy_z; s2 = F.hy_x; break;
         default: s1 = F.hz_x; s2 = F.hz_y; break;
     }
-    const T* row = g + (size_t)n * Gp.nprobes;
+    const T* row = Gp.series + (size_t)n * Gp.nprobes;
     T sum = T(0);
     for (int j = 0; j < Gp.count; ++j) sum = sum + row[Gp.order[j]];
     s1[Gp.cell] = s1[Gp.cell] + sum;
     s2[Gp.cell] = s2[Gp.cell] + sum;
 }
 
-// Design box [b0, b0 + bn) in grid cells; the H slot holds (bn + 1) cells per axis.
+// Design box [b0, b0 + bn) in grid cells; its H slot holds (bn + 1) cells per axis.
 struct Box {
     int32_t x0, y0, z0, nx, ny, nz;
 };
 
-// Save the forward H totals of the box (plus one cell on each upper side) into a slot.
+// A design box of one instance: the per-step slots of forward H totals (K of them, slot_elems
+// each), its dJ/dCb accumulator, and the block offsets of the save (hblk0) and gradient (gblk0)
+// launches.
 template <typename T>
-__global__ void k_save_h(const InstDev<T>* __restrict__ pF, Box B, T* __restrict__ slot) {
-    const InstDev<T>& I = *pF;
-    const int hx_n = B.nx + 1, hy_n = B.ny + 1, hz_n = B.nz + 1;
-    const int total = hx_n * hy_n * hz_n;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
-        const int lz = q % hz_n, ly = q / hz_n % hy_n, lx = q / (hz_n * hy_n);
-        const int x = B.x0 + lx, y = B.y0 + ly, z = B.z0 + lz;
-        T a = T(0), b = T(0), c = T(0);
-        if (x < I.Nx && y < I.Ny && z < I.Nz) {
-            const size_t i = ((size_t)x * I.Ny + y) * I.Nz + z;
-            a = I.F.hx[i];
-            b = I.F.hy[i];
-            c = I.F.hz[i];
-        }
-        slot[q] = a;
-        slot[total + q] = b;
-        slot[2 * total + q] = c;
+struct BoxDev {
+    Box B;
+    int32_t inst, pad;
+    T* slots;
+    double* grad;
+    int64_t slot_elems;
+};
+
+// Save the forward H totals of every box (plus one cell on each upper side) into slot `j`.
+template <typename T>
+__global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
+                                                const int64_t* __restrict__ blk0, int n, int j) {
+    const int k = item_of(blk0, n, blockIdx.x);
+    const BoxDev<T> D = boxes[k];
+    const InstDev<T>& I = insts[D.inst];
+    const int hy_n = D.B.ny + 1, hz_n = D.B.nz + 1;
+    const int64_t total = (int64_t)(D.B.nx + 1) * hy_n * hz_n;
+    const int64_t q = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const int lz = (int)(q % hz_n), ly = (int)(q / hz_n % hy_n), lx = (int)(q / ((int64_t)hz_n * hy_n));
+    const int x = D.B.x0 + lx, y = D.B.y0 + ly, z = D.B.z0 + lz;
+    T a = T(0), b = T(0), c = T(0);
+    if (x < I.Nx && y < I.Ny && z < I.Nz) {
+        const size_t i = ((size_t)x * I.Ny + y) * I.Nz + z;
+        a = I.F.hx[i];
+        b = I.F.hy[i];
+        c = I.F.hz[i];
     }
+    T* slot = D.slots + (size_t)j * D.slot_elems;
+    slot[q] = a;
+    slot[total + q] = b;
+    slot[2 * total + q] = c;
 }
 
-// dJ/dCb of the six E splits in the box: += nu * (the curl difference the forward update used).
+// dJ/dCb of the six E splits in every box: += nu * (the curl difference the forward update used),
+// with the forward H of this step from slot `j`.
 template <typename T>
-__global__ void k_adj_grad(const InstDev<T>* __restrict__ pA, Box B, const T* __restrict__ slot,
-                           double* __restrict__ grad) {
-    const InstDev<T>& A = *pA;
+__global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
+                                                  const int64_t* __restrict__ blk0, int n, int j) {
+    const int k = item_of(blk0, n, blockIdx.x);
+    const BoxDev<T> D = boxes[k];
+    const InstDev<T>& A = insts[D.inst];
+    const Box B = D.B;
     const int hy_n = B.ny + 1, hz_n = B.nz + 1;
-    const int total_h = (B.nx + 1) * hy_n * hz_n;
-    const int nbox = B.nx * B.ny * B.nz;
-    const T* hx = slot;
-    const T* hy = slot + total_h;
-    const T* hz = slot + 2 * total_h;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += gridDim.x * blockDim.x) {
-        const int lz = q % B.nz, ly = q / B.nz % B.ny, lx = q / (B.nz * B.ny);
-        const int x = B.x0 + lx, y = B.y0 + ly, z = B.z0 + lz;
-        const size_t i = ((size_t)x * A.Ny + y) * A.Nz + z;
-        const int h = (lx * hy_n + ly) * hz_n + lz, hyp = h + hz_n, hxp = h + hy_n * hz_n;
-        const bool xin = x < A.Nx - 1, yin = y < A.Ny - 1, zin = z < A.Nz - 1;
-        const T d_xy = yin ? hz[hyp] - hz[h] : T(0);
-        const T d_xz = zin ? hy[h + 1] - hy[h] : T(0);
-        const T d_yz = zin ? hx[h + 1] - hx[h] : T(0);
-        const T d_yx = xin ? hz[hxp] - hz[h] : T(0);
-        const T d_zx = xin ? hy[hxp] - hy[h] : T(0);
-        const T d_zy = yin ? hx[hyp] - hx[h] : T(0);
-        grad[0 * (size_t)nbox + q] += (double)A.F.ex_y[i] * (double)d_xy;
-        grad[1 * (size_t)nbox + q] += (double)A.F.ex_z[i] * (double)d_xz;
-        grad[2 * (size_t)nbox + q] += (double)A.F.ey_z[i] * (double)d_yz;
-        grad[3 * (size_t)nbox + q] += (double)A.F.ey_x[i] * (double)d_yx;
-        grad[4 * (size_t)nbox + q] += (double)A.F.ez_x[i] * (double)d_zx;
-        grad[5 * (size_t)nbox + q] += (double)A.F.ez_y[i] * (double)d_zy;
-    }
-}
-
-// Totals from splits after a checkpoint restore (the forward computes them the same way).
-template <typename T>
-__global__ void k_totals(const InstDev<T>* __restrict__ pF) {
-    const InstDev<T>& I = *pF;
-    const size_t cells = (size_t)I.Nx * I.Ny * I.Nz;
-    const Fields<T>& F = I.F;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += (size_t)gridDim.x * blockDim.x) {
-        F.ex[i] = F.ex_y[i] + F.ex_z[i];
-        F.ey[i] = F.ey_z[i] + F.ey_x[i];
-        F.ez[i] = F.ez_x[i] + F.ez_y[i];
-        F.hx[i] = F.hx_y[i] + F.hx_z[i];
-        F.hy[i] = F.hy_z[i] + F.hy_x[i];
-        F.hz[i] = F.hz_x[i] + F.hz_y[i];
-    }
+    const int64_t total_h = (int64_t)(B.nx + 1) * hy_n * hz_n;
+    const int64_t nbox = (int64_t)B.nx * B.ny * B.nz;
+    const int64_t q = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    if (q >= nbox) return;
+    const T* hx = D.slots + (size_t)j * D.slot_elems;
+    const T* hy = hx + total_h;
+    const T* hz = hx + 2 * total_h;
+    const int lz = (int)(q % B.nz), ly = (int)(q / B.nz % B.ny), lx = (int)(q / ((int64_t)B.nz * B.ny));
+    const int x = B.x0 + lx, y = B.y0 + ly, z = B.z0 + lz;
+    const size_t i = ((size_t)x * A.Ny + y) * A.Nz + z;
+    const int64_t h = ((int64_t)lx * hy_n + ly) * hz_n + lz, hyp = h + hz_n, hxp = h + (int64_t)hy_n * hz_n;
+    const bool xin = x < A.Nx - 1, yin = y < A.Ny - 1, zin = z < A.Nz - 1;
+    const T d_xy = yin ? hz[hyp] - hz[h] : T(0);
+    const T d_xz = zin ? hy[h + 1] - hy[h] : T(0);
+    const T d_yz = zin ? hx[h + 1] - hx[h] : T(0);
+    const T d_yx = xin ? hz[hxp] - hz[h] : T(0);
+    const T d_zx = xin ? hy[hxp] - hy[h] : T(0);
+    const T d_zy = yin ? hx[hyp] - hx[h] : T(0);
+    double* g = D.grad;
+    g[0 * nbox + q] += (double)A.F.ex_y[i] * (double)d_xy;
+    g[1 * nbox + q] += (double)A.F.ex_z[i] * (double)d_xz;
+    g[2 * nbox + q] += (double)A.F.ey_z[i] * (double)d_yz;
+    g[3 * nbox + q] += (double)A.F.ey_x[i] * (double)d_yx;
+    g[4 * nbox + q] += (double)A.F.ez_x[i] * (double)d_zx;
+    g[5 * nbox + q] += (double)A.F.ez_y[i] * (double)d_zy;
 }
 
 struct MonitorDev {
@@ -1548,112 +1591,213 @@ struct GradBuffers {
     }
 };
 
-// Reverse-time pass over a checkpointed forward run of one instance: recomputes each checkpoint
-// segment forward (saving H in the design box every step), then steps the adjoint back through the
-// segment and accumulates dJ/dCb of the six E splits in the box.  adj[p] is dJ/dy of probe p,
-// T[Nt] in the instance's probe order.  grad_out: double[6][nx][ny][nz] (ex_y ex_z ey_z ey_x ez_x
-// ez_y), box cells in grid coordinates.
+// Block offsets of a batched flat launch over items of the given sizes (256 threads per block).
+std::vector<int64_t> block_offsets(const std::vector<int64_t>& sizes) {
+    std::vector<int64_t> off(sizes.size() + 1, 0);
+    for (size_t i = 0; i < sizes.size(); ++i) off[i + 1] = off[i] + (sizes[i] + 255) / 256;
+    if (off.back() > INT32_MAX) RF_THROW(RFDTD_EINVAL, "batched launch too large");
+    return off;
+}
+
+// One gradient request: instance, design box, dJ/dy of its probes, output.
+struct GradItem {
+    int inst;
+    Box B;
+    const void* const* adj;
+    int n_adj;
+    double* out;
+};
+
+// Reverse-time pass over a checkpointed forward run, for several instances in the same launches:
+// each checkpoint segment is recomputed forward from its checkpoint (saving H in every design box
+// each step), then the adjoint steps back through the segment and accumulates dJ/dCb of the six E
+// splits in every box.  adj[p] of an item is dJ/dy of probe p, T[Nt] in the instance's probe
+// order; out: double[6][nx][ny][nz] (ex_y ex_z ey_z ey_x ez_x ez_y), box cells in grid coordinates.
 template <typename T>
-void gradient(Session& s, int inst, const int32_t box[6], const void* const* adj, int n_adj, double* grad_out) {
-    Instance& I = instance_at(s, inst);
-    const Member& M = s.members[I.member];
+void gradients(Session& s, const std::vector<GradItem>& items) {
     if (s.ckpt_every <= 0) RF_THROW(RFDTD_ESTATE, "enable checkpointing before the forward run");
     const int64_t Nt = s.step;
     if (Nt < 1) RF_THROW(RFDTD_ESTATE, "run the forward solve first");
     const int K = s.ckpt_every;
     const int nseg = (int)((Nt + K - 1) / K);
-    if ((int)I.ckpt.size() < nseg) RF_THROW(RFDTD_ESTATE, "missing checkpoints of the forward run");
-    if (n_adj != I.P || !adj) RF_THROW(RFDTD_EINVAL, "one adjoint series per probe is required");
-    const Box B{box[0], box[2], box[4], box[1] - box[0], box[3] - box[2], box[5] - box[4]};
-    if (B.nx < 1 || B.ny < 1 || B.nz < 1 || B.x0 < 0 || B.y0 < 0 || B.z0 < 0 || box[1] > M.Nx || box[3] > M.Ny ||
-        box[5] > M.Nz)
-        RF_THROW(RFDTD_EINVAL, "design box outside the grid");
-    if (!grad_out) RF_THROW(RFDTD_EINVAL, "gradient output missing");
+    const int n = (int)items.size();
+    if (n < 1) RF_THROW(RFDTD_EINVAL, "no gradient requested");
+    std::vector<int> seen(s.insts.size(), 0);
+    for (const GradItem& it : items) {
+        Instance& I = instance_at(s, it.inst);
+        const Member& M = s.members[I.member];
+        if (seen[it.inst]++) RF_THROW(RFDTD_EINVAL, "an instance may appear once per gradient call");
+        if ((int)I.ckpt.size() < nseg) RF_THROW(RFDTD_ESTATE, "missing checkpoints of the forward run");
+        if (it.n_adj != I.P || (I.P && !it.adj)) RF_THROW(RFDTD_EINVAL, "one adjoint series per probe is required");
+        const Box& B = it.B;
+        if (B.nx < 1 || B.ny < 1 || B.nz < 1 || B.x0 < 0 || B.y0 < 0 || B.z0 < 0 || B.x0 + B.nx > M.Nx ||
+            B.y0 + B.ny > M.Ny || B.z0 + B.nz > M.Nz)
+            RF_THROW(RFDTD_EINVAL, "design box outside the grid");
+        if (!it.out) RF_THROW(RFDTD_EINVAL, "gradient output missing");
+    }
     refresh<T>(s);
     HIP_OK(hipDeviceSynchronize());
     hipStream_t st = s.own_stream;
     HIP_OK(hipEventRecord(s.ev0, st));
-    GradBuffers buf;
-    void* d_re = buf.alloc(18 * I.pitch);
-    void* d_adj = buf.alloc(18 * I.pitch);
-    HIP_OK(hipMemsetAsync(d_adj, 0, 18 * I.pitch, st));
     const size_t esz = sizeof(T);
-    // adjoint sources, time-major [Nt][P]
-    std::vector<T> gser((size_t)Nt * I.P);
-    for (int p = 0; p < I.P; ++p) {
-        if (!adj[p]) RF_THROW(RFDTD_EINVAL, "adjoint series missing");
-        const T* src = (const T*)adj[p];
-        for (int64_t t = 0; t < Nt; ++t) gser[(size_t)t * I.P + p] = src[t];
+    GradBuffers buf;
+    // descriptors: [0, n) recomputed forward fields, [n, 2n) adjoint fields
+    std::vector<InstDev<T>> di(2 * n);
+    std::vector<MemberDev> md(n);
+    std::vector<void*> d_re(n);
+    std::vector<BoxDev<T>> bd(n);
+    std::vector<int64_t> cells(n), hcells(n), gcells(n);
+    std::vector<ProbeGroup<T>> gE, gH, aE, aH;
+    int tile0 = 0;
+    for (int k = 0; k < n; ++k) {
+        const GradItem& it = items[k];
+        Instance& I = s.insts[it.inst];
+        const Member& M = s.members[I.member];
+        d_re[k] = buf.alloc(18 * I.pitch);
+        void* d_adj = buf.alloc(18 * I.pitch);
+        HIP_OK(hipMemsetAsync(d_adj, 0, 18 * I.pitch, st));
+        const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
+        const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
+        di[k] = make_instdev<T>(M, d_re[k], I.pitch, npb);
+        di[n + k] = make_instdev<T>(M, d_adj, I.pitch, npb);
+        md[k] = MemberDev{tile0, npb * chunks, 1, k};
+        tile0 += npb * chunks;
+        cells[k] = M.cells;
+        // adjoint sources, time-major [Nt][P]
+        T* d_g = nullptr;
+        if (I.P) {
+            std::vector<T> gser((size_t)Nt * I.P);
+            for (int p = 0; p < I.P; ++p) {
+                if (!it.adj[p]) RF_THROW(RFDTD_EINVAL, "adjoint series missing");
+                const T* src = (const T*)it.adj[p];
+                for (int64_t t = 0; t < Nt; ++t) gser[(size_t)t * I.P + p] = src[t];
+            }
+            d_g = (T*)buf.alloc(gser.size() * esz);
+            HIP_OK(hipMemcpyAsync(d_g, gser.data(), gser.size() * esz, hipMemcpyHostToDevice, st));
+            // the copy reads host memory that goes out of scope: finish it now
+            HIP_OK(hipStreamSynchronize(st));
+        }
+        for (size_t g = 0; g < I.groups.size(); ++g) {
+            const auto& hg = I.groups[g];
+            ProbeGroup<T> pg{};
+            pg.cell = hg.cell;
+            pg.series = (T*)I.d_series;
+            pg.series_in = (const T*)I.d_series_in;
+            pg.order = I.d_order + hg.start;
+            pg.is_src = I.d_is_src;
+            pg.inst = k;
+            pg.field = hg.field;
+            pg.count = hg.count;
+            pg.nprobes = I.P;
+            ProbeGroup<T> ag = pg;
+            ag.series = d_g;
+            ((int)g < I.ngE ? gE : gH).push_back(pg);
+            ((int)g < I.ngE ? aE : aH).push_back(ag);
+        }
+        const Box& B = it.B;
+        const int64_t slot_elems = (int64_t)3 * (B.nx + 1) * (B.ny + 1) * (B.nz + 1);
+        bd[k].B = B;
+        bd[k].inst = k;
+        bd[k].slot_elems = slot_elems;
+        bd[k].slots = (T*)buf.alloc((size_t)K * slot_elems * esz);
+        const size_t nbox = (size_t)B.nx * B.ny * B.nz;
+        bd[k].grad = (double*)buf.alloc(6 * nbox * sizeof(double));
+        HIP_OK(hipMemsetAsync(bd[k].grad, 0, 6 * nbox * sizeof(double), st));
+        hcells[k] = slot_elems / 3;
+        gcells[k] = (int64_t)nbox;
     }
-    T* d_g = nullptr;
-    if (I.P) {
-        d_g = (T*)buf.alloc(gser.size() * esz);
-        HIP_OK(hipMemcpyAsync(d_g, gser.data(), gser.size() * esz, hipMemcpyHostToDevice, st));
-    }
-    const size_t slot_elems = (size_t)3 * (B.nx + 1) * (B.ny + 1) * (B.nz + 1);
-    T* d_slots = (T*)buf.alloc((size_t)K * slot_elems * esz);
-    const size_t nbox = (size_t)B.nx * B.ny * B.nz;
-    double* d_grad = (double*)buf.alloc(6 * nbox * sizeof(double));
-    HIP_OK(hipMemsetAsync(d_grad, 0, 6 * nbox * sizeof(double), st));
-    // descriptors: [0] recomputed forward fields, [1] adjoint fields; one member over [0]
-    const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
-    const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
-    InstDev<T> di[2] = {make_instdev<T>(M, d_re, I.pitch, npb), make_instdev<T>(M, d_adj, I.pitch, npb)};
-    InstDev<T>* d_di = (InstDev<T>*)buf.alloc(sizeof(di));
-    HIP_OK(hipMemcpyAsync(d_di, di, sizeof(di), hipMemcpyHostToDevice, st));
-    const MemberDev md{0, npb * chunks, 1, 0};
-    MemberDev* d_md = (MemberDev*)buf.alloc(sizeof(md));
-    HIP_OK(hipMemcpyAsync(d_md, &md, sizeof(md), hipMemcpyHostToDevice, st));
-    std::vector<ProbeGroup<T>> gr;
-    for (const auto& hg : I.groups) {
-        ProbeGroup<T> pg{};
-        pg.cell = hg.cell;
-        pg.series = (T*)I.d_series;
-        pg.series_in = (const T*)I.d_series_in;
-        pg.order = I.d_order + hg.start;
-        pg.is_src = I.d_is_src;
-        pg.inst = 0;
-        pg.field = hg.field;
-        pg.count = hg.count;
-        pg.nprobes = I.P;
-        gr.push_back(pg);
-    }
-    const int ngE = I.ngE, ngH = (int)gr.size() - I.ngE;
-    ProbeGroup<T>* d_gr = nullptr;
-    if (!gr.empty()) {
-        d_gr = (ProbeGroup<T>*)buf.alloc(gr.size() * sizeof(ProbeGroup<T>));
-        HIP_OK(hipMemcpyAsync(d_gr, gr.data(), gr.size() * sizeof(ProbeGroup<T>), hipMemcpyHostToDevice, st));
-    }
-    const int grid = (npb * chunks + 7) & ~7;
-    const int flat = (int)std::min<int64_t>((M.cells + 255) / 256, 16384);
-    const int gbox = (int)std::min<size_t>((nbox + 255) / 256 + 1, 4096);
+    const std::vector<int64_t> flat0 = block_offsets(cells), h0 = block_offsets(hcells), g0 = block_offsets(gcells);
+    auto upload = [&](const void* src, size_t bytes) {
+        void* d = buf.alloc(bytes);
+        HIP_OK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st));
+        return d;
+    };
+    const InstDev<T>* d_di = (const InstDev<T>*)upload(di.data(), di.size() * sizeof(InstDev<T>));
+    const MemberDev* d_md = (const MemberDev*)upload(md.data(), md.size() * sizeof(MemberDev));
+    const BoxDev<T>* d_bd = (const BoxDev<T>*)upload(bd.data(), bd.size() * sizeof(BoxDev<T>));
+    const int64_t* d_flat0 = (const int64_t*)upload(flat0.data(), flat0.size() * 8);
+    const int64_t* d_h0 = (const int64_t*)upload(h0.data(), h0.size() * 8);
+    const int64_t* d_g0 = (const int64_t*)upload(g0.data(), g0.size() * 8);
+    const int ngE = (int)gE.size(), ngH = (int)gH.size();
+    gE.insert(gE.end(), gH.begin(), gH.end());
+    aE.insert(aE.end(), aH.begin(), aH.end());
+    const ProbeGroup<T>* d_gr = gE.empty() ? nullptr : (const ProbeGroup<T>*)upload(gE.data(), gE.size() * sizeof(ProbeGroup<T>));
+    const ProbeGroup<T>* d_ag = aE.empty() ? nullptr : (const ProbeGroup<T>*)upload(aE.data(), aE.size() * sizeof(ProbeGroup<T>));
+    HIP_OK(hipStreamSynchronize(st));  // host-side descriptor vectors stay alive; uploads done
+    const int grid = (tile0 + 7) & ~7;
+    const int flat = (int)flat0.back(), hblocks = (int)h0.back(), gblocks = (int)g0.back();
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int64_t n0 = (int64_t)seg * K, n1 = std::min<int64_t>(Nt, n0 + K);
-        HIP_OK(hipMemcpyAsync(d_re, I.ckpt[seg], 12 * I.pitch, hipMemcpyDeviceToDevice, st));
-        k_totals<T><<<flat, 256, 0, st>>>(d_di);
-        for (int64_t n = n0; n < n1; ++n) {
-            k_save_h<T><<<gbox, 256, 0, st>>>(d_di, B, d_slots + (size_t)(n - n0) * slot_elems);
-            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
-            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
-            if (ngE) k_probes<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di, d_gr, ngE, nullptr, (int)n, 1);
-            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
-            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, 1, md.tiles);
-            if (ngH) k_probes<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di, d_gr + ngE, ngH, nullptr, (int)n, 1);
+        for (int k = 0; k < n; ++k) {
+            const Instance& I = s.insts[items[k].inst];
+            HIP_OK(hipMemcpyAsync(d_re[k], I.ckpt[seg], 12 * I.pitch, hipMemcpyDeviceToDevice, st));
         }
-        for (int64_t n = n1 - 1; n >= n0; --n) {
-            if (ngH) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + 1, d_gr + ngE, ngH, d_g, (int)n);
-            k_adj_e<T><<<flat, 256, 0, st>>>(d_di + 1);
-            if (ngE) k_adj_inject<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di + 1, d_gr, ngE, d_g, (int)n);
-            k_adj_grad<T><<<gbox, 256, 0, st>>>(d_di + 1, B, d_slots + (size_t)(n - n0) * slot_elems, d_grad);
-            k_adj_h<T><<<flat, 256, 0, st>>>(d_di + 1);
+        k_totals<T><<<flat, 256, 0, st>>>(d_di, d_flat0, n);
+        for (int64_t t = n0; t < n1; ++t) {
+            const int j = (int)(t - n0);
+            k_save_h<T><<<hblocks, 256, 0, st>>>(d_di, d_bd, d_h0, n, j);
+            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
+            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
+            if (ngE) k_probes<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di, d_gr, ngE, nullptr, (int)t, 1);
+            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
+            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
+            if (ngH) k_probes<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di, d_gr + ngE, ngH, nullptr, (int)t, 1);
+        }
+        for (int64_t t = n1 - 1; t >= n0; --t) {
+            const int j = (int)(t - n0);
+            if (ngH) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + n, d_ag + ngE, ngH, (int)t);
+            k_adj_e<T><<<flat, 256, 0, st>>>(d_di + n, d_flat0, n);
+            if (ngE) k_adj_inject<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di + n, d_ag, ngE, (int)t);
+            k_adj_grad<T><<<gblocks, 256, 0, st>>>(d_di + n, d_bd, d_g0, n, j);
+            k_adj_h<T><<<flat, 256, 0, st>>>(d_di + n, d_flat0, n);
         }
         HIP_OK(hipGetLastError());
     }
     HIP_OK(hipEventRecord(s.ev1, st));
-    HIP_OK(hipMemcpyAsync(grad_out, d_grad, 6 * nbox * sizeof(double), hipMemcpyDeviceToHost, st));
+    for (int k = 0; k < n; ++k)
+        HIP_OK(hipMemcpyAsync(items[k].out, bd[k].grad, 6 * (size_t)gcells[k] * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     float ms = 0;
     if (hipEventElapsedTime(&ms, s.ev0, s.ev1) == hipSuccess) s.last_total_ms = ms;
+}
+
+// Checkpoint interval for a gradient over `items` after an Nt-step forward run: the interval K that
+// minimises checkpoint memory (ceil(Nt/K) sets of every instance's 12 split arrays) plus the design
+// boxes' per-step H slots (K of them), checked against `budget` bytes (0: the device's free memory
+// less 1 GiB).
+template <typename T>
+void plan_checkpoints(Session& s, int Nt, const std::vector<GradItem>& items, uint64_t budget, int32_t* every_out,
+                      uint64_t* bytes_out) {
+    if (Nt < 1) RF_THROW(RFDTD_EINVAL, "Nt must be >= 1");
+    const size_t esz = sizeof(T);
+    double per_ckpt = 0, per_slot = 0, fixed = 0;
+    for (const Instance& I : s.insts) per_ckpt += 12.0 * I.pitch;
+    for (const GradItem& it : items) {
+        const Instance& I = instance_at(s, it.inst);
+        const Box& B = it.B;
+        if (B.nx < 1 || B.ny < 1 || B.nz < 1) RF_THROW(RFDTD_EINVAL, "empty design box");
+        per_slot += 3.0 * (B.nx + 1) * (B.ny + 1) * (B.nz + 1) * esz;
+        fixed += 36.0 * I.pitch + (double)Nt * I.P * esz + 48.0 * B.nx * B.ny * B.nz;
+    }
+    if (budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        budget = free_b > (1ull << 30) ? free_b - (1ull << 30) : 0;
+    }
+    int best = 1;
+    double best_bytes = 1e300;
+    for (int K = 1; K <= Nt; ++K) {
+        const double b = std::ceil((double)Nt / K) * per_ckpt + (double)K * per_slot + fixed;
+        if (b < best_bytes) {
+            best_bytes = b;
+            best = K;
+        }
+    }
+    if (best_bytes > (double)budget)
+        RF_THROW(RFDTD_ENOMEM, "checkpoints and adjoint buffers need " + std::to_string((uint64_t)best_bytes) +
+                                   " bytes, more than the " + std::to_string(budget) + " available");
+    *every_out = best;
+    if (bytes_out) *bytes_out = (uint64_t)best_bytes;
 }
 
 void progress_bar(int64_t n, int64_t Nt) {
@@ -1874,6 +2018,22 @@ int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3],
     });
 }
 
+int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_t* instances, const int32_t* boxes,
+                           uint64_t budget_bytes, int32_t* every_out, uint64_t* bytes_out) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (n < 1 || !instances || !boxes || !every_out) RF_THROW(RFDTD_EINVAL, "plan request arrays missing");
+        std::vector<GradItem> items(n);
+        for (int k = 0; k < n; ++k) {
+            const int32_t* b = boxes + 6 * k;
+            items[k] = GradItem{instances[k], Box{b[0], b[2], b[4], b[1] - b[0], b[3] - b[2], b[5] - b[4]}, nullptr, 0,
+                                nullptr};
+        }
+        if (S.f64) plan_checkpoints<double>(S, Nt, items, budget_bytes, every_out, bytes_out);
+        else plan_checkpoints<float>(S, Nt, items, budget_bytes, every_out, bytes_out);
+    });
+}
+
 int rfdtd_set_checkpointing(rfdtd_session* s, int32_t every) {
     return guarded([&] {
         Session& S = S_of(s);
@@ -1884,11 +2044,23 @@ int rfdtd_set_checkpointing(rfdtd_session* s, int32_t every) {
 
 int rfdtd_gradient(rfdtd_session* s, int32_t instance, const int32_t box[6], const void* const* adjoint_series,
                    int32_t n_series, double* grad_out) {
+    return rfdtd_gradient_batch(s, 1, &instance, box, &adjoint_series, &n_series, &grad_out);
+}
+
+int rfdtd_gradient_batch(rfdtd_session* s, int32_t n, const int32_t* instances, const int32_t* boxes,
+                         const void* const* const* adjoint_series, const int32_t* n_series, double* const* grad_out) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (!box) RF_THROW(RFDTD_EINVAL, "design box missing");
-        if (S.f64) gradient<double>(S, instance, box, adjoint_series, n_series, grad_out);
-        else gradient<float>(S, instance, box, adjoint_series, n_series, grad_out);
+        if (n < 1 || !instances || !boxes || !adjoint_series || !n_series || !grad_out)
+            RF_THROW(RFDTD_EINVAL, "gradient request arrays missing");
+        std::vector<GradItem> items(n);
+        for (int k = 0; k < n; ++k) {
+            const int32_t* b = boxes + 6 * k;
+            items[k] = GradItem{instances[k], Box{b[0], b[2], b[4], b[1] - b[0], b[3] - b[2], b[5] - b[4]},
+                                adjoint_series[k], n_series[k], grad_out[k]};
+        }
+        if (S.f64) gradients<double>(S, items);
+        else gradients<float>(S, items);
     });
 }
 

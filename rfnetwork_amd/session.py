@@ -115,6 +115,17 @@ class Session:
         """Save the split fields every `every` steps of later runs (needed by gradient())."""
         check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
 
+    def plan_checkpoints(self, Nt, box, budget_bytes=0):
+        """Checkpoint interval for a gradient over box after an Nt-step forward run that fits device
+        memory (budget_bytes, 0: what is free).  Returns (every, bytes needed)."""
+        inst = (ctypes.c_int32 * 1)(0)
+        b = (ctypes.c_int32 * 6)(*[int(v) for v in box])
+        every, need = ctypes.c_int32(), ctypes.c_uint64()
+        check(self._L.rfdtd_plan_checkpoints(self._h, int(Nt), 1, ctypes.cast(inst, ctypes.c_void_p),
+                                             ctypes.cast(b, ctypes.c_void_p), int(budget_bytes), ctypes.byref(every),
+                                             ctypes.byref(need)), "rfdtd_plan_checkpoints")
+        return every.value, need.value
+
     def gradient(self, box, adjoint_series, instance=0):
         """dJ/dCb of the 6 E splits in box (x0, x1, y0, y1, z0, z1 grid cells) after a checkpointed
         forward run; adjoint_series: per probe, dJ/dy[n] (n < steps run).  Returns float64

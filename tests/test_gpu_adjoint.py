@@ -140,3 +140,60 @@ def test_autograd_function_backward(hip_ready):
                                              checkpoint_every=100)
     assert float(J) == pytest.approx(J_ref, rel=1e-6)
     np.testing.assert_allclose(eps.grad.numpy(), 2.0 * g_ref, rtol=1e-6, atol=1e-9 * np.abs(g_ref).max())
+
+
+def test_batched_gradients_equal_single_sessions(hip_ready):
+    """Reverse passes of uneven grids sharing launches give each instance exactly what a single-grid
+    session gives; the checkpoint planner's interval fits and is used."""
+    print("[case] batched gradients", flush=True)
+    from rfnetwork_amd.batch import Batch
+
+    Nt = 40
+    models = [workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False, seed=7),
+              workloads.tiny_model(Nt=Nt, nx=19, ny=12, nz=10, time_monitors=False, phasor_monitors=False, seed=8)]
+    rng = np.random.default_rng(9)
+    inputs = []
+    for m in models:
+        coeff, probes, _, _, Nx, Ny, Nz, _ = m.pack(gpu=True)
+        inputs.append((coeff, probes, (Nx, Ny, Nz), rng.standard_normal((len(probes), Nt)).astype(np.float32)))
+    boxes = [(0, 14, 0, 11, 0, 9), (3, 15, 2, 9, 1, 8)]
+    with Batch([(c, *dims, 1) for c, _, dims, _ in inputs]) as b:
+        every, need = b.plan_checkpoints(Nt, [(0, boxes[0]), (1, boxes[1])])
+        assert 1 <= every <= Nt and need > 0
+        with pytest.raises(Exception, match="available"):
+            b.plan_checkpoints(Nt, [(0, boxes[0])], budget_bytes=1024)
+        b.set_checkpointing(every)
+        for k, (_, probes, _, _) in enumerate(inputs):
+            b.set_sources(k, probes, [], Nt)
+        b.run(Nt)
+        b.synchronize()
+        got = b.gradients([(k, boxes[k], inputs[k][3]) for k in range(2)])
+    for k, (coeff, _, dims, g) in enumerate(inputs):
+        _, probes, _, _, *_ = models[k].pack(gpu=True, coefficients=False)
+        with Session(coeff, *dims) as s:
+            s.set_checkpointing(every)
+            s.set_sources(probes, [], Nt)
+            s.run(Nt)
+            s.synchronize()
+            ref = s.gradient(boxes[k], list(g))
+        np.testing.assert_array_equal(got[k], ref)
+        assert np.abs(ref).max() > 0
+
+
+def test_eps_gradients_batch_equal_single(hip_ready):
+    """Two design models of different size through eps_r_gradients_batch: J and dJ/deps_r equal the
+    single-model eps_r_gradient results."""
+    print("[case] eps gradients batch", flush=True)
+    eps_a = np.random.default_rng(4).uniform(1.0, 4.0, size=(8, 8, 6))
+    eps_b = np.random.default_rng(5).uniform(1.0, 4.0, size=(6, 10, 5))
+    ma = _small_design(eps_a, coef_dtype=np.float32, Nt=300)
+    mb = workloads.design_strip(eps_b, Nt=300, nx=52, ny=36, nz=18, design=(6, 10, 5), n_pml=5, h=3,
+                                pulse=(20e9, 80e-12, 60e-12), frequency=SMALL_F)
+    ca, cb = workloads.design_box(48, 32, (8, 8, 6)), workloads.design_box(52, 36, (6, 10, 5))
+    res, info = adjoint.eps_r_gradients_batch([(ma, ca, ma.frequency), (mb, cb, mb.frequency)], checkpoint_every=70,
+                                              return_info=True)
+    for (J, g, _), (m, cbox) in zip(res, ((ma, ca), (mb, cb))):
+        J1, g1, _ = adjoint.eps_r_gradient(m, cbox, m.frequency, checkpoint_every=70)
+        assert J == J1
+        np.testing.assert_array_equal(g, g1)
+    assert info["checkpoint_every"] == 70
