@@ -238,6 +238,33 @@ def adjoint_leg(n_steps, every, repeats=2, flags=0):
                 finite=bool(np.isfinite(grad).all()))
 
 
+def c5_leg(n_steps, flags=0):
+    """8 design variants from 96^3 to 512x512x256 (142 M cells), port 1 driven in each: the forward
+    runs and the adjoint passes of all variants in batched launches, the checkpoint interval planned
+    to fit device memory.  Reported: wall time of the whole gradient of all 8 (model assembly
+    excluded), the planned interval and bytes."""
+    from rfnetwork_amd import _lib, adjoint, workloads
+
+    jobs, cells = [], 0
+    for v in range(len(workloads.DESIGN_VARIANTS)):
+        t0 = time.perf_counter()
+        m = workloads.design_variant(v, Nt=n_steps)
+        jobs.append((m, workloads.design_variant_box(v), m.frequency))
+        cells += m.Nx * m.Ny * m.Nz
+        log(f"c5: variant {v} {m.Nx}x{m.Ny}x{m.Nz} built in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    res, info = adjoint.eps_r_gradients_batch(jobs, flags=flags | _lib.VERBOSE, return_info=True)
+    wall = time.perf_counter() - t0
+    log(f"c5: {wall:.1f}s for all gradients, {info}")
+    return dict(workload=f"8 design variants 96^3..512x512x256, design boxes 48^3..128^3, {n_steps} steps",
+                cells=cells, wall_s=round(wall, 2), gradients_per_s=round(len(jobs) / wall, 4),
+                forward_ms=round(info["forward_ms"], 1), reverse_ms=round(info["adjoint_ms"], 1),
+                forward_mcells=round(cells * n_steps / (info["forward_ms"] * 1e-3) / 1e6, 1),
+                checkpoint_every=info["checkpoint_every"], planned_bytes=info.get("planned_bytes"),
+                launch=info.get("launch"), J=[r[0] for r in res],
+                finite=bool(all(np.isfinite(r[1]).all() for r in res)))
+
+
 def adjoint_fd_leg(n_steps, every, n_cells=32, seed=41, h=1e-3, oracle_cells=1):
     """Validation (not a throughput figure): the adjoint dJ/deps_r of the 200x160x80 design model in
     float64 and float32 against central differences (Delta eps_r = h) of float64 forward solves on
@@ -301,10 +328,12 @@ def main():
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--legs", default="plain,batched,adjoint",
                     help="comma list: plain (the headline), batched (12 excitations), adjoint (gradients/s), "
+                         "c5 (8 design variants, batched forward+adjoint, minutes), "
                          "adjoint_fd (finite-difference validation, minutes)")
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--adjoint-steps", type=int, default=6000)
     ap.add_argument("--checkpoint-every", type=int, default=200)
+    ap.add_argument("--c5-steps", type=int, default=8000)
     ap.add_argument("--no-graphs", action="store_true", help="launch every step directly (no hipGraph replay)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -326,6 +355,8 @@ def main():
         out["batched"] = batched_leg(args.batch_steps, gflags)
     if "adjoint" in legs:
         out["adjoint"] = adjoint_leg(args.adjoint_steps, args.checkpoint_every, flags=gflags)
+    if "c5" in legs:
+        out["c5"] = c5_leg(args.c5_steps, gflags)
     if "adjoint_fd" in legs:
         out["adjoint_fd"] = adjoint_fd_leg(args.adjoint_steps, args.checkpoint_every)
     print(json.dumps(out), flush=True)

@@ -39,6 +39,7 @@ extern "C" {
 #define RFDTD_NO_GRAPHS 0x8u      /* launch every step eagerly instead of replaying a captured chunk */
 #define RFDTD_QUIET 0x10u         /* no progress output */
 #define RFDTD_KERNEL_TIMING 0x20u /* launch eagerly and time the E/H update kernels with HIP events */
+#define RFDTD_VERBOSE 0x40u       /* progress lines on stderr during long gradient passes */
 
 /*
  * Coefficient order: the keys of the reference's coefficient dictionary (solver.py:1473-1518):

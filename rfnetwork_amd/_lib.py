@@ -23,6 +23,7 @@ COEF_TABLE = 0x4
 NO_GRAPHS = 0x8
 QUIET = 0x10
 KERNEL_TIMING = 0x20
+VERBOSE = 0x40
 
 # Coefficient keys in the order of the C ABI (rfnetwork/solver.py:1473-1518).
 COEFF_NAMES = (

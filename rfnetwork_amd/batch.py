@@ -240,3 +240,14 @@ def solve_ports(jobs, frequency, dtype=np.float32, flags=0, return_batch=False):
         return results, batch
     batch.close()
     return results
+
+
+def sparameter_matrix(model, frequency, dtype=np.float32, flags=0):
+    """The model's full S-matrix [frequency, b, a]: every port driven once (the others terminated),
+    all excitations in one batched session."""
+    from .utils import LabeledArray
+
+    ports = [i + 1 for i, p in enumerate(model.ports) if p is not None]
+    (S, _), = solve_ports([(model, ports)], frequency, dtype, flags)
+    return LabeledArray(S, coords=dict(frequency=np.atleast_1d(np.asarray(frequency, np.float64)),
+                                       b=np.arange(1, S.shape[1] + 1), a=np.array(ports)))

@@ -700,27 +700,53 @@ __global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__
 }
 
 struct MonitorDev {
-    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, pad;
+    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id;
     int64_t N1N2;
     void* values;
     const void* phase;
+    void* stage;  // phasor monitors: kStageSlots plane samples awaiting accumulation
 };
 
+// Running DFTs are accumulated every kStageSlots steps instead of every step: each step copies the
+// monitor plane into a staging slot, and a flush adds the staged samples to the accumulators in time
+// order, so the sums are the same float operations in the same order while the accumulators (Nf
+// complex values per plane cell) are read and written once per kStageSlots steps.
+constexpr int kStageSlots = 10;
+
 template <typename T>
-__global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDev* __restrict__ mons, int nmon,
-                           const int* base, int local) {
-    __shared__ int sm;
+__device__ __forceinline__ const T* monitor_source(const InstDev<T>& I, int field) {
+    switch (field) {
+        case EX: return I.F.ex;
+        case EY: return I.F.ey;
+        case EZ: return I.F.ez;
+        case HX: return I.F.hx;
+        case HY: return I.F.hy;
+        default: return I.F.hz;
+    }
+}
+
+__device__ __forceinline__ int monitor_of(const MonitorDev* __restrict__ mons, int nmon, int* sm) {
     if (threadIdx.x == 0) {
         int m = 0;
         while (m + 1 < nmon && mons[m + 1].blk0 <= (int)blockIdx.x) ++m;
-        sm = m;
+        *sm = m;
     }
     __syncthreads();
-    const MonitorDev M = mons[__builtin_amdgcn_readfirstlane(sm)];
+    return __builtin_amdgcn_readfirstlane(*sm);
+}
+
+// Sample step n: time-domain monitors write their plane row; phasor monitors copy the plane into
+// staging slot `slot` and record the phase row of the sample (-1: not sampled at this step).
+template <typename T>
+__global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDev* __restrict__ mons, int nmon,
+                           const int* base, int local, int slot, int* __restrict__ rows) {
+    __shared__ int sm;
+    const MonitorDev M = mons[monitor_of(mons, nmon, &sm)];
     const int n = step_of(base, local);
-    if (n > 0 && (n % M.n_step) != 0) return;
+    const bool sampled = n == 0 || (n % M.n_step) == 0;
     const int64_t q = (int64_t)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
-    if (q >= M.N1N2) return;
+    if (M.nf && q == 0) rows[M.id * kStageSlots + slot] = sampled ? n / M.n_step : -1;
+    if (!sampled || q >= M.N1N2) return;
     const InstDev<T>& I = insts[M.inst];
     const int Ny = I.Ny, Nz = I.Nz;
     const int a = (int)(q / M.N2), b = (int)(q - (int64_t)a * M.N2);
@@ -731,31 +757,43 @@ __global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDe
         idx = ((int64_t)a * Ny + M.pos) * Nz + b;
     else
         idx = ((int64_t)a * Ny + b) * Nz + M.pos;
-    const T* src;
-    switch (M.field) {
-        case EX: src = I.F.ex; break;
-        case EY: src = I.F.ey; break;
-        case EZ: src = I.F.ez; break;
-        case HX: src = I.F.hx; break;
-        case HY: src = I.F.hy; break;
-        default: src = I.F.hz; break;
+    const T v = monitor_source(I, M.field)[idx];
+    if (M.nf == 0)
+        ((T*)M.values)[(int64_t)(n / M.n_step) * M.N1N2 + q] = v;
+    else
+        ((T*)M.stage)[(int64_t)slot * M.N1N2 + q] = v;
+}
+
+// Accumulate staged slots [0, nslots): values[f] += v_s * phase[row_s, f] for s in time order (a real
+// field times a complex phase: one product and one sum per component, as each step would).
+template <typename T>
+__global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, int nslots, const int* __restrict__ rows) {
+    __shared__ int sm;
+    const MonitorDev M = mons[monitor_of(mons, nmon, &sm)];
+    if (M.nf == 0) return;
+    const int64_t q = (int64_t)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
+    if (q >= M.N1N2) return;
+    T v[kStageSlots];
+    int r[kStageSlots];
+#pragma unroll
+    for (int j = 0; j < kStageSlots; ++j) {
+        r[j] = j < nslots ? rows[M.id * kStageSlots + j] : -1;
+        v[j] = r[j] >= 0 ? ((const T*)M.stage)[(int64_t)j * M.N1N2 + q] : T(0);
     }
-    const T v = src[idx];
-    const int mn = n / M.n_step;
-    if (M.nf == 0) {
-        ((T*)M.values)[(int64_t)mn * M.N1N2 + q] = v;
-    } else {
-        // values[f] += v * phase[mn, f]: a real field times a complex phase, one product and one sum
-        // per component.
-        const T* ph = (const T*)M.phase + (size_t)mn * M.nf * 2;
-        T* acc = (T*)M.values;
-        for (int f = 0; f < M.nf; ++f) {
-            T* c = acc + 2 * ((int64_t)f * M.N1N2 + q);
-            const T re = c[0] + v * ph[2 * f];
-            const T im = c[1] + v * ph[2 * f + 1];
-            c[0] = re;
-            c[1] = im;
+    const T* phase = (const T*)M.phase;
+    T* acc = (T*)M.values;
+    for (int f = 0; f < M.nf; ++f) {
+        T* c = acc + 2 * ((int64_t)f * M.N1N2 + q);
+        T re = c[0], im = c[1];
+#pragma unroll
+        for (int j = 0; j < kStageSlots; ++j) {
+            if (r[j] < 0) continue;
+            const T* ph = phase + ((size_t)r[j] * M.nf + f) * 2;
+            re = re + v[j] * ph[0];
+            im = im + v[j] * ph[1];
         }
+        c[0] = re;
+        c[1] = im;
     }
 }
 
@@ -840,6 +878,8 @@ struct Session {
     int ngE = 0, ngH = 0;
     MonitorDev* d_mons = nullptr;
     int n_active = 0, mon_blocks = 0;
+    void* d_stage = nullptr;  // staged phasor-monitor samples
+    int* d_rows = nullptr;    // [n_active][kStageSlots] phase row of each staged sample
     int total_tiles = 0;
     bool dirty = true;
 
@@ -913,7 +953,7 @@ Session::~Session() {
     }
     for (auto& M : members)
         if (M.d_coef) (void)hipFree(M.d_coef);
-    for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base})
+    for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base, d_stage, (void*)d_rows})
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1142,25 +1182,37 @@ void refresh(Session& s) {
     // monitors of all instances
     std::vector<MonitorDev> md;
     int blocks = 0;
+    size_t stage_elems = 0;
     for (size_t k = 0; k < s.insts.size(); ++k)
         for (const MonitorHost& h : s.insts[k].mons) {
             if (!h.active) continue;
             MonitorDev d{};
             d.inst = (int32_t)k;
+            d.id = (int32_t)md.size();
             d.axis = h.desc.axis; d.pos = h.pos; d.field = h.desc.field; d.n_step = h.desc.n_step;
             d.nf = h.desc.n_frequencies; d.N1 = h.N1; d.N2 = h.N2;
             d.N1N2 = (int64_t)h.N1 * h.N2;
             d.blk0 = blocks;
             d.values = h.d_values;
             d.phase = h.d_phase;
+            d.stage = (void*)stage_elems;  // offset for now
+            if (d.nf) stage_elems += (size_t)kStageSlots * d.N1N2;
             blocks += (int)((d.N1N2 + 255) / 256);
             md.push_back(d);
         }
     s.n_active = (int)md.size();
     s.mon_blocks = blocks;
     if (s.d_mons) (void)hipFree(s.d_mons);
+    if (s.d_stage) (void)hipFree(s.d_stage);
+    if (s.d_rows) (void)hipFree(s.d_rows);
     s.d_mons = nullptr;
+    s.d_stage = nullptr;
+    s.d_rows = nullptr;
+    if (stage_elems) HIP_OK(hipMalloc(&s.d_stage, stage_elems * sizeof(T)));
+    for (auto& d : md) d.stage = d.nf ? (void*)((T*)s.d_stage + (size_t)d.stage) : nullptr;
     if (!md.empty()) {
+        HIP_OK(hipMalloc(&s.d_rows, md.size() * kStageSlots * sizeof(int)));
+        HIP_OK(hipMemset(s.d_rows, 0xff, md.size() * kStageSlots * sizeof(int)));
         HIP_OK(hipMalloc(&s.d_mons, md.size() * sizeof(MonitorDev)));
         HIP_OK(hipMemcpy(s.d_mons, md.data(), md.size() * sizeof(MonitorDev), hipMemcpyHostToDevice));
     }
@@ -1168,8 +1220,10 @@ void refresh(Session& s) {
     s.drop_graph();
 }
 
+// One time step.  slot: the step's staging slot for running DFTs; flush > 0 accumulates staged
+// slots [0, flush) after this step's sample.
 template <typename T>
-void launch_step(Session& s, hipStream_t st, const int* base, int local, bool monitors_now,
+void launch_step(Session& s, hipStream_t st, const int* base, int local, int slot, int flush,
                  hipEvent_t* ev = nullptr) {
     const int grid = (s.total_tiles + 7) & ~7;
     const InstDev<T>* di = (const InstDev<T>*)s.d_insts;
@@ -1190,16 +1244,12 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, bool mo
         k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
     if (ev) HIP_OK(hipEventRecord(ev[3], st));
     if (io && s.ngH) k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local, 0);
-    if (io && s.n_active && monitors_now)
-        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local);
+    if (io && s.n_active) {
+        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local, slot, s.d_rows);
+        if (flush) k_monitor_flush<T><<<s.mon_blocks, 256, 0, st>>>(s.d_mons, s.n_active, flush, s.d_rows);
+    }
 }
 
-bool any_monitor_samples(const Session& s, int64_t n) {
-    for (const auto& I : s.insts)
-        for (const auto& m : I.mons)
-            if (m.active && (n == 0 || n % m.desc.n_step == 0)) return true;
-    return false;
-}
 
 void save_checkpoints(Session& s, int slot, hipStream_t st) {
     for (Instance& I : s.insts) {
@@ -1215,7 +1265,11 @@ void capture_graph(Session& s) {
     hipStream_t cs;
     HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < s.graph_steps; ++k) launch_step<T>(s, cs, s.d_step_base, k, true);
+    for (int k = 0; k < s.graph_steps; ++k) {
+        const int slot = k % kStageSlots;
+        const bool last = slot == kStageSlots - 1 || k == s.graph_steps - 1;
+        launch_step<T>(s, cs, s.d_step_base, k, slot, last ? slot + 1 : 0);
+    }
     hipError_t ce = hipStreamEndCapture(cs, &s.graph_def);
     (void)hipStreamDestroy(cs);
     HIP_OK(ce);
@@ -1235,8 +1289,10 @@ void run_chunk(Session& s, int len, int offset, hipStream_t st, bool graphs, boo
             done += s.graph_steps;
         }
     }
-    for (; done < len; ++done) {
+    for (int d = 0; done < len; ++done, ++d) {
         const int64_t n = s.step + offset + done;
+        const int slot = d % kStageSlots;
+        const bool last = slot == kStageSlots - 1 || done == len - 1;
         hipEvent_t* ev = nullptr;
         if (timing && (offset + done) % stride == 0 && s.n_timed < Session::kTimedSteps) {
             if (s.kev.empty()) {
@@ -1245,7 +1301,7 @@ void run_chunk(Session& s, int len, int offset, hipStream_t st, bool graphs, boo
             }
             ev = &s.kev[4 * s.n_timed++];
         }
-        launch_step<T>(s, st, nullptr, (int)n, any_monitor_samples(s, n), ev);
+        launch_step<T>(s, st, nullptr, (int)n, slot, last ? slot + 1 : 0, ev);
     }
 }
 
@@ -1752,6 +1808,11 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
             k_adj_h<T><<<flat, 256, 0, st>>>(d_di + n, d_flat0, n);
         }
         HIP_OK(hipGetLastError());
+        if ((s.flags & RFDTD_VERBOSE) && (seg % std::max(1, nseg / 10) == 0 || seg == 0)) {
+            HIP_OK(hipStreamSynchronize(st));
+            std::fprintf(stderr, "[rfdtd] adjoint: %d of %d checkpoint segments left\n", seg, nseg);
+            std::fflush(stderr);
+        }
     }
     HIP_OK(hipEventRecord(s.ev1, st));
     for (int k = 0; k < n; ++k)

@@ -854,6 +854,19 @@ class FDTD_Solver:
         return LabeledArray(S.T[..., None], coords=dict(frequency=frequency, b=np.arange(1, nports + 1),
                                                        a=[source_port]))
 
+    def write_touchstone(self, filepath, frequency, fmt: str = "ri", freq_unit: str = "hz", comments=()):
+        """Solve every port of the model (one batched GPU session) and write the full S-matrix to an
+        .sNp file (the circuit side reads it with rfnetwork/touchstone.py:7).  Returns the matrix."""
+        from . import batch, touchstone
+
+        S = batch.sparameter_matrix(self, frequency)
+        r0 = {p["r0"] for p in self.ports if p is not None}
+        if len(r0) != 1:
+            raise ValueError("Touchstone files hold one reference impedance; the ports differ")
+        touchstone.write_snp(filepath, S, frequency, fmt=fmt, ref_z=float(r0.pop()), freq_unit=freq_unit,
+                             comments=comments)
+        return S
+
     def get_monitor_data(self, name: str):
         """Monitor values with time (or frequency) and the two in-plane coordinates (inches)."""
         self.check_solution()

@@ -26,6 +26,12 @@ port_variant    variant v of the batched workload: V0 64^3, V1 128x96x64, V2 160
                 a 4-cell eps_r 3.66 substrate from 2/3/3/4 ports (x-, x+, y-, y+ sides, 1 cell in from
                 the PML) to the centre; 10-cell PML on every face but the z = 0 ground; the C1 pulse;
                 101 frequencies 1-10 GHz; 5000 steps.
+design_variant  variant v of the batched forward+adjoint workload: grids from 96^3 to 512x512x256
+                cells of 0.1 mm, a cubic eps_r ~ U[1, 4) design box (seed 500 + v) from 48^3 to 128^3
+                on the ground, centred in x and y, between 2/2/3/3/3/4/4/4 microstrip feeds (x-, x+,
+                y-, y+ sides) that end one cell short of the box; 10-cell PML except on the ground;
+                port 1 driven (4 GHz Gaussian-modulated sine); objective |S21|^2 at 2..6 GHz; 8000
+                steps.
 """
 import numpy as np
 
@@ -198,4 +204,51 @@ def port_variant(v, Nt=5000, cell_m=1e-4, courant=0.99, n_pml=10, h=4, n_freq=10
     src = s.gaussian_modulated_source(f0=5.5e9, width=430e-12, t0=320e-12, t_len=(Nt + 0.5) * s.dt)
     s.assign_excitation(1e-2 * np.asarray(src), 1)
     s.frequency = np.linspace(1e9, 1e10, n_freq)
+    return s
+
+
+DESIGN_VARIANTS = ((96, 96, 96, 2), (128, 128, 128, 2), (160, 160, 128, 3), (192, 192, 160, 3),
+                   (256, 256, 160, 3), (320, 320, 192, 4), (384, 384, 224, 4), (512, 512, 256, 4))
+
+
+def design_variant_box(v):
+    """(x0, x1, y0, y1, z0, z1) of the design box of design_variant(v): a cube 48 + 80 v / 7 cells."""
+    nx, ny, nz, _ = DESIGN_VARIANTS[v]
+    b = int(round(48 + 80 * v / (len(DESIGN_VARIANTS) - 1)))
+    x0, y0 = (nx - b) // 2, (ny - b) // 2
+    return (x0, x0 + b, y0, y0 + b, 0, b)
+
+
+def design_variant(v, eps_design=None, Nt=8000, cell_m=1e-4, courant=0.99, n_pml=10, h=8, coef_dtype=np.float32):
+    """Variant v (0..7) of the batched forward+adjoint workload."""
+    nx, ny, nz, n_ports = DESIGN_VARIANTS[v]
+    x0, x1, y0, y1, z0, z1 = design_variant_box(v)
+    b = x1 - x0
+    if eps_design is None:
+        eps_design = np.random.default_rng(500 + v).uniform(1.0, 4.0, size=(b, b, b))
+    eps_design = np.asarray(eps_design, np.float64)
+    s, d = _grid_solver(nx, ny, nz, cell_m)
+    s.coef_dtype = coef_dtype
+    cx, cy = nx // 2, ny // 2
+    s.add_dielectric(Box((0, 0, 0), (nx * d, ny * d, h * d)), er=3.66)
+    s.add_permittivity_volume(eps_design, (x0, y0, z0))
+    ends = (("x", n_pml + 1, x0 - 1), ("x", nx - n_pml - 1, x1 + 1), ("y", n_pml + 1, y0 - 1),
+            ("y", ny - n_pml - 1, y1 + 1))[:n_ports]
+    for k, (axis, p, stop) in enumerate(ends):
+        a, e = sorted((p, stop))
+        if axis == "x":
+            s.add_conductor(Rectangle([(a * d, (cy - 3) * d, h * d), (a * d, (cy + 3) * d, h * d),
+                                       (e * d, (cy + 3) * d, h * d)]))
+            s.add_lumped_port(k + 1, Rectangle([(p * d, (cy - 3) * d, 0), (p * d, (cy + 3) * d, 0),
+                                                (p * d, (cy + 3) * d, h * d)]), "z+")
+        else:
+            s.add_conductor(Rectangle([((cx - 3) * d, a * d, h * d), ((cx + 3) * d, a * d, h * d),
+                                       ((cx + 3) * d, e * d, h * d)]))
+            s.add_lumped_port(k + 1, Rectangle([((cx - 3) * d, p * d, 0), ((cx + 3) * d, p * d, 0),
+                                                ((cx + 3) * d, p * d, h * d)]), "z+")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=n_pml)
+    s.generate_mesh(courant=courant)
+    src = s.gaussian_modulated_source(f0=4e9, width=600e-12, t0=420e-12, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(1e-2 * np.asarray(src), 1)
+    s.frequency = np.array([2e9, 3e9, 4e9, 5e9, 6e9])
     return s

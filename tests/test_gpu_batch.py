@@ -100,3 +100,19 @@ def test_solve_ports_equals_per_port_solves(hip_ready):
             ref.solve(gpu=True, show_progress=False)
             S_ref = np.asarray(ref.get_sparameters(f, source_port=port))[..., 0]
             np.testing.assert_array_equal(S[..., col], S_ref)
+
+
+def test_write_touchstone_full_matrix(hip_ready, tmp_path):
+    """FDTD_Solver.write_touchstone: the file read back holds the S-matrix whose columns are the
+    per-port solve() results."""
+    print("[case] touchstone", flush=True)
+    from rfnetwork_amd import touchstone
+
+    f = np.linspace(4e9, 12e9, 5)
+    model = workloads.microstrip(Nt=400, nx=40, ny=20, nz=10, n_pml=4)
+    S = model.write_touchstone(tmp_path / "line.s2p", f)
+    S2, _, _ = touchstone.read_snp(tmp_path / "line.s2p")
+    np.testing.assert_allclose(np.asarray(S2), np.asarray(S), rtol=1e-10, atol=1e-13)
+    ref = workloads.microstrip(Nt=400, nx=40, ny=20, nz=10, n_pml=4)
+    ref.solve(gpu=True, show_progress=False)
+    np.testing.assert_array_equal(S[..., 0], np.asarray(ref.get_sparameters(f, source_port=1))[..., 0])
