@@ -145,6 +145,8 @@ struct CoefRaw {
         r.ca_zx = c(8); r.ca_zy = c(9); r.cb_zx = c(10); r.cb_zy = c(11);
         return r;
     }
+    // coefficient k (0..29) alone
+    __device__ __forceinline__ T at(uint32_t b, int k) const { return ldg((const GLOBAL_AS T*)cp[k], b); }
     __device__ __forceinline__ HCoef<T> h(uint32_t b) const {
         auto c = [&](int k) { return ldg((const GLOBAL_AS T*)cp[k], b); };
         HCoef<T> r;
@@ -162,6 +164,16 @@ struct CoefTable {
     const uint32_t* hidx;
     const Vec4<T>* etab;
     const Vec4<T>* htab;
+    // coefficient k (0..29) alone: E entries hold 12 values, H entries 20 (18 and padding)
+    __device__ __forceinline__ T at(uint32_t bc, int k) const {
+        const uint32_t S = (uint32_t)sizeof(T);
+        if (k < 12) {
+            const uint32_t e = ldg((const GLOBAL_AS uint32_t*)eidx, bc / S * 4u);
+            return ldg((const GLOBAL_AS T*)etab, (e * 12u + (uint32_t)k) * S);
+        }
+        const uint32_t e = ldg((const GLOBAL_AS uint32_t*)hidx, bc / S * 4u);
+        return ldg((const GLOBAL_AS T*)htab, (e * 20u + (uint32_t)(k - 12)) * S);
+    }
     // bc: byte offset of the cell in a T array; the index arrays are uint32
     __device__ __forceinline__ ECoef<T> e(uint32_t bc) const {
         const uint32_t k = ldg((const GLOBAL_AS uint32_t*)eidx, bc / (uint32_t)sizeof(T) * 4u);
@@ -463,21 +475,6 @@ __global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<
 // skips (upper edges of the GPU packing) pass their adjoint through unchanged.
 // ------------------------------------------------------------------------------------------------
 
-// Single coefficient k (0..29) of the cell at byte offset b.
-template <typename T>
-__device__ __forceinline__ T coef_at(const InstDev<T>& I, int k, uint32_t b) {
-    if (I.coef_kind == 1) {
-        const uint32_t S = (uint32_t)sizeof(T);
-        if (k < 12) {
-            const uint32_t e = ldg((const GLOBAL_AS uint32_t*)I.eidx, b / S * 4u);
-            return ldg((const GLOBAL_AS T*)I.etab, (e * 12u + (uint32_t)k) * S);
-        }
-        const uint32_t e = ldg((const GLOBAL_AS uint32_t*)I.hidx, b / S * 4u);
-        return ldg((const GLOBAL_AS T*)I.htab, (e * 20u + (uint32_t)(k - 12)) * S);
-    }
-    return ldg((const GLOBAL_AS T*)I.raw[k], b);
-}
-
 enum {  // coefficient positions
     C_CA_XY = 0, C_CA_XZ, C_CB_XY, C_CB_XZ, C_CA_YZ, C_CA_YX, C_CB_YZ, C_CB_YX, C_CA_ZX, C_CA_ZY, C_CB_ZX, C_CB_ZY,
     C_DA_XY, C_DA_XZ, C_DB_XY1, C_DB_XY2, C_DB_XZ1, C_DB_XZ2, C_DA_YZ, C_DA_YX, C_DB_YZ1, C_DB_YZ2, C_DB_YX1,
@@ -491,90 +488,131 @@ __device__ __forceinline__ int item_of(const int64_t* __restrict__ blk0, int n, 
     return i;
 }
 
-template <typename T>
-__device__ __forceinline__ void adj_e_cell(const InstDev<T>& A, uint32_t i) {
-    const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz;
-    const uint32_t S = (uint32_t)sizeof(T);
-    const uint32_t sy = (uint32_t)Nz * S, sx = (uint32_t)Ny * Nz * S;
+// Reverse E half step over one column of x planes (the transpose of h_column followed by the
+// Ca scaling of the next step's e_column).  Marches x upward and carries the x+1 plane's H
+// coefficients and the two mu splits that read it.
+template <typename T, class CA>
+__device__ __forceinline__ void adj_e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
     using G = GLOBAL_T(T);
-    const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
-    const uint32_t b = i * S;
-    const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
-    const G* mxy = (const G*)A.F.hx_y; const G* mxz = (const G*)A.F.hx_z;
-    const G* myz = (const G*)A.F.hy_z; const G* myx = (const G*)A.F.hy_x;
-    const G* mzx = (const G*)A.F.hz_x; const G* mzy = (const G*)A.F.hz_y;
-    // E totals' adjoints: every H split that reads E at this cell (as e or as e- of a neighbour)
-    T ez = coef_at(A, C_DB_XY2, b) * ldg(mxy, b) + coef_at(A, C_DB_YX2, b) * ldg(myx, b);
-    if (yin) ez = ez - coef_at(A, C_DB_XY1, b + sy) * ldg(mxy, b + sy);
-    if (xin) ez = ez - coef_at(A, C_DB_YX1, b + sx) * ldg(myx, b + sx);
-    T ey = coef_at(A, C_DB_XZ2, b) * ldg(mxz, b) + coef_at(A, C_DB_ZX2, b) * ldg(mzx, b);
-    if (zin) ey = ey - coef_at(A, C_DB_XZ1, b + S) * ldg(mxz, b + S);
-    if (xin) ey = ey - coef_at(A, C_DB_ZX1, b + sx) * ldg(mzx, b + sx);
-    T ex = coef_at(A, C_DB_YZ2, b) * ldg(myz, b) + coef_at(A, C_DB_ZY2, b) * ldg(mzy, b);
-    if (zin) ex = ex - coef_at(A, C_DB_YZ1, b + S) * ldg(myz, b + S);
-    if (yin) ex = ex - coef_at(A, C_DB_ZY1, b + sy) * ldg(mzy, b + sy);
-    G* nxy = (G*)A.F.ex_y; G* nxz = (G*)A.F.ex_z; G* nyz = (G*)A.F.ey_z;
-    G* nyx = (G*)A.F.ey_x; G* nzx = (G*)A.F.ez_x; G* nzy = (G*)A.F.ez_y;
-    stg(nxy, b, (yin ? coef_at(A, C_CA_XY, b) : T(1)) * ldg(nxy, b) + ex);
-    stg(nxz, b, (zin ? coef_at(A, C_CA_XZ, b) : T(1)) * ldg(nxz, b) + ex);
-    stg(nyz, b, (zin ? coef_at(A, C_CA_YZ, b) : T(1)) * ldg(nyz, b) + ey);
-    stg(nyx, b, (xin ? coef_at(A, C_CA_YX, b) : T(1)) * ldg(nyx, b) + ey);
-    stg(nzx, b, (xin ? coef_at(A, C_CA_ZX, b) : T(1)) * ldg(nzx, b) + ez);
-    stg(nzy, b, (yin ? coef_at(A, C_CA_ZY, b) : T(1)) * ldg(nzy, b) + ez);
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
+    const G* mxy = (const G*)F.hx_y; const G* mxz = (const G*)F.hx_z;
+    const G* myz = (const G*)F.hy_z; const G* myx = (const G*)F.hy_x;
+    const G* mzx = (const G*)F.hz_x; const G* mzy = (const G*)F.hz_y;
+    G* nxy = (G*)F.ex_y; G* nxz = (G*)F.ex_z; G* nyz = (G*)F.ey_z;
+    G* nyx = (G*)F.ey_x; G* nzx = (G*)F.ez_x; G* nzy = (G*)F.ez_y;
+    const bool yin = t.y < Ny - 1, zin = t.z < Nz - 1;
+    if (t.x0 >= t.x1) return;
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    HCoef<T> hc = C.h(b);
+    T m_yx = ldg(myx, b), m_zx = ldg(mzx, b);
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool xin = x < Nx - 1;
+        HCoef<T> hn{};
+        T n_yx = T(0), n_zx = T(0);
+        if (xin) {
+            hn = C.h(b + sx);
+            n_yx = ldg(myx, b + sx);
+            n_zx = ldg(mzx, b + sx);
+        }
+        const T m_xy = ldg(mxy, b), m_xz = ldg(mxz, b), m_yz = ldg(myz, b), m_zy = ldg(mzy, b);
+        // E totals' adjoints: every H split that reads E at this cell (as e or as e- of a neighbour)
+        T ez = hc.db_xy2 * m_xy + hc.db_yx2 * m_yx;
+        if (yin) ez = ez - C.at(b + sy, C_DB_XY1) * ldg(mxy, b + sy);
+        if (xin) ez = ez - hn.db_yx1 * n_yx;
+        T ey = hc.db_xz2 * m_xz + hc.db_zx2 * m_zx;
+        if (zin) ey = ey - C.at(b + S, C_DB_XZ1) * ldg(mxz, b + S);
+        if (xin) ey = ey - hn.db_zx1 * n_zx;
+        T ex = hc.db_yz2 * m_yz + hc.db_zy2 * m_zy;
+        if (zin) ex = ex - C.at(b + S, C_DB_YZ1) * ldg(myz, b + S);
+        if (yin) ex = ex - C.at(b + sy, C_DB_ZY1) * ldg(mzy, b + sy);
+        const ECoef<T> ec = C.e(b);
+        stg(nxy, b, (yin ? ec.ca_xy : T(1)) * ldg(nxy, b) + ex);
+        stg(nxz, b, (zin ? ec.ca_xz : T(1)) * ldg(nxz, b) + ex);
+        stg(nyz, b, (zin ? ec.ca_yz : T(1)) * ldg(nyz, b) + ey);
+        stg(nyx, b, (xin ? ec.ca_yx : T(1)) * ldg(nyx, b) + ey);
+        stg(nzx, b, (xin ? ec.ca_zx : T(1)) * ldg(nzx, b) + ez);
+        stg(nzy, b, (yin ? ec.ca_zy : T(1)) * ldg(nzy, b) + ez);
+        hc = hn;
+        m_yx = n_yx;
+        m_zx = n_zx;
+    }
 }
 
-template <typename T>
-__device__ __forceinline__ void adj_h_cell(const InstDev<T>& A, uint32_t i) {
-    const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz;
-    const uint32_t S = (uint32_t)sizeof(T);
-    const uint32_t sy = (uint32_t)Nz * S, sx = (uint32_t)Ny * Nz * S;
+// Reverse H half step over one column (the transpose of e_column followed by the Da scaling of the
+// next step's h_column).  Marches x upward and carries Cb * nu of the x-1 plane's two x splits.
+template <typename T, class CA>
+__device__ __forceinline__ void adj_h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
     using G = GLOBAL_T(T);
-    const G* nxy = (const G*)A.F.ex_y; const G* nxz = (const G*)A.F.ex_z; const G* nyz = (const G*)A.F.ey_z;
-    const G* nyx = (const G*)A.F.ey_x; const G* nzx = (const G*)A.F.ez_x; const G* nzy = (const G*)A.F.ez_y;
-    const int z = (int)(i % (uint32_t)Nz), y = (int)(i / (uint32_t)Nz % (uint32_t)Ny), x = (int)(i / ((uint32_t)Ny * Nz));
-    const uint32_t b = i * S;
-    const bool xin = x < Nx - 1, yin = y < Ny - 1, zin = z < Nz - 1;
-    // H totals' adjoints: every E split whose curl difference reads H at this cell
-    T hz = T(0), hy = T(0), hx = T(0);
-    if (yin) hz = hz - coef_at(A, C_CB_XY, b) * ldg(nxy, b);
-    if (y > 0) hz = hz + coef_at(A, C_CB_XY, b - sy) * ldg(nxy, b - sy);
-    if (xin) hz = hz - coef_at(A, C_CB_YX, b) * ldg(nyx, b);
-    if (x > 0) hz = hz + coef_at(A, C_CB_YX, b - sx) * ldg(nyx, b - sx);
-    if (zin) hy = hy - coef_at(A, C_CB_XZ, b) * ldg(nxz, b);
-    if (z > 0) hy = hy + coef_at(A, C_CB_XZ, b - S) * ldg(nxz, b - S);
-    if (xin) hy = hy - coef_at(A, C_CB_ZX, b) * ldg(nzx, b);
-    if (x > 0) hy = hy + coef_at(A, C_CB_ZX, b - sx) * ldg(nzx, b - sx);
-    if (zin) hx = hx - coef_at(A, C_CB_YZ, b) * ldg(nyz, b);
-    if (z > 0) hx = hx + coef_at(A, C_CB_YZ, b - S) * ldg(nyz, b - S);
-    if (yin) hx = hx - coef_at(A, C_CB_ZY, b) * ldg(nzy, b);
-    if (y > 0) hx = hx + coef_at(A, C_CB_ZY, b - sy) * ldg(nzy, b - sy);
-    G* mxy = (G*)A.F.hx_y; G* mxz = (G*)A.F.hx_z; G* myz = (G*)A.F.hy_z;
-    G* myx = (G*)A.F.hy_x; G* mzx = (G*)A.F.hz_x; G* mzy = (G*)A.F.hz_y;
-    stg(mxy, b, coef_at(A, C_DA_XY, b) * ldg(mxy, b) + hx);
-    stg(mxz, b, coef_at(A, C_DA_XZ, b) * ldg(mxz, b) + hx);
-    stg(myz, b, coef_at(A, C_DA_YZ, b) * ldg(myz, b) + hy);
-    stg(myx, b, coef_at(A, C_DA_YX, b) * ldg(myx, b) + hy);
-    stg(mzx, b, coef_at(A, C_DA_ZX, b) * ldg(mzx, b) + hz);
-    stg(mzy, b, coef_at(A, C_DA_ZY, b) * ldg(mzy, b) + hz);
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
+    const G* nxy = (const G*)F.ex_y; const G* nxz = (const G*)F.ex_z; const G* nyz = (const G*)F.ey_z;
+    const G* nyx = (const G*)F.ey_x; const G* nzx = (const G*)F.ez_x; const G* nzy = (const G*)F.ez_y;
+    G* mxy = (G*)F.hx_y; G* mxz = (G*)F.hx_z; G* myz = (G*)F.hy_z;
+    G* myx = (G*)F.hy_x; G* mzx = (G*)F.hz_x; G* mzy = (G*)F.hz_y;
+    const bool yin = t.y < Ny - 1, zin = t.z < Nz - 1, ylo = t.y > 0, zlo = t.z > 0;
+    if (t.x0 >= t.x1) return;
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    T p_yx = T(0), p_zx = T(0);
+    if (t.x0 > 0) {
+        p_yx = C.at(b - sx, C_CB_YX) * ldg(nyx, b - sx);
+        p_zx = C.at(b - sx, C_CB_ZX) * ldg(nzx, b - sx);
+    }
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool xin = x < Nx - 1, xlo = x > 0;
+        const ECoef<T> ec = C.e(b);
+        const T v_xy = ldg(nxy, b), v_xz = ldg(nxz, b), v_yz = ldg(nyz, b);
+        const T v_yx = ldg(nyx, b), v_zx = ldg(nzx, b), v_zy = ldg(nzy, b);
+        // H totals' adjoints: every E split whose curl difference reads H at this cell
+        T hz = T(0), hy = T(0), hx = T(0);
+        if (yin) hz = hz - ec.cb_xy * v_xy;
+        if (ylo) hz = hz + C.at(b - sy, C_CB_XY) * ldg(nxy, b - sy);
+        if (xin) hz = hz - ec.cb_yx * v_yx;
+        if (xlo) hz = hz + p_yx;
+        if (zin) hy = hy - ec.cb_xz * v_xz;
+        if (zlo) hy = hy + C.at(b - S, C_CB_XZ) * ldg(nxz, b - S);
+        if (xin) hy = hy - ec.cb_zx * v_zx;
+        if (xlo) hy = hy + p_zx;
+        if (zin) hx = hx - ec.cb_yz * v_yz;
+        if (zlo) hx = hx + C.at(b - S, C_CB_YZ) * ldg(nyz, b - S);
+        if (yin) hx = hx - ec.cb_zy * v_zy;
+        if (ylo) hx = hx + C.at(b - sy, C_CB_ZY) * ldg(nzy, b - sy);
+        const HCoef<T> hc = C.h(b);
+        stg(mxy, b, hc.da_xy * ldg(mxy, b) + hx);
+        stg(mxz, b, hc.da_xz * ldg(mxz, b) + hx);
+        stg(myz, b, hc.da_yz * ldg(myz, b) + hy);
+        stg(myx, b, hc.da_yx * ldg(myx, b) + hy);
+        stg(mzx, b, hc.da_zx * ldg(mzx, b) + hz);
+        stg(mzy, b, hc.da_zy * ldg(mzy, b) + hz);
+        p_yx = ec.cb_yx * v_yx;
+        p_zx = ec.cb_zx * v_zx;
+    }
 }
 
-// One cell per thread over the cells of n instances; blk0: n + 1 block offsets.
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_e(const InstDev<T>* __restrict__ insts, const int64_t* __restrict__ blk0,
-                                               int n) {
-    const int k = item_of(blk0, n, blockIdx.x);
-    const InstDev<T>& A = insts[k];
-    const int64_t i = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
-    if (i < (int64_t)A.Nx * A.Ny * A.Nz) adj_e_cell(A, (uint32_t)i);
+__global__ void __launch_bounds__(1024) k_adj_e(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
+                                                int nmem, int total) {
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
+    if (!I) return;
+    if (I->coef_kind == 1)
+        adj_e_column<T, CoefTable<T>>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+    else
+        adj_e_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_h(const InstDev<T>* __restrict__ insts, const int64_t* __restrict__ blk0,
-                                               int n) {
-    const int k = item_of(blk0, n, blockIdx.x);
-    const InstDev<T>& A = insts[k];
-    const int64_t i = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
-    if (i < (int64_t)A.Nx * A.Ny * A.Nz) adj_h_cell(A, (uint32_t)i);
+__global__ void __launch_bounds__(1024) k_adj_h(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
+                                                int nmem, int total) {
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
+    if (!I) return;
+    if (I->coef_kind == 1)
+        adj_h_column<T, CoefTable<T>>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+    else
+        adj_h_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
 }
 
 // Totals from splits after a checkpoint restore (the forward computes them the same way).
@@ -700,7 +738,7 @@ __global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__
 }
 
 struct MonitorDev {
-    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id;
+    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id, fblk0, pad;
     int64_t N1N2;
     void* values;
     const void* phase;
@@ -712,6 +750,7 @@ struct MonitorDev {
 // order, so the sums are the same float operations in the same order while the accumulators (Nf
 // complex values per plane cell) are read and written once per kStageSlots steps.
 constexpr int kStageSlots = 10;
+constexpr int kFreqGroup = 4;  // frequencies per thread in the flush
 
 template <typename T>
 __device__ __forceinline__ const T* monitor_source(const InstDev<T>& I, int field) {
@@ -725,10 +764,11 @@ __device__ __forceinline__ const T* monitor_source(const InstDev<T>& I, int fiel
     }
 }
 
-__device__ __forceinline__ int monitor_of(const MonitorDev* __restrict__ mons, int nmon, int* sm) {
+// The monitor whose block range holds this workgroup (blk0: sampling launch, fblk0: flush launch).
+__device__ __forceinline__ int monitor_of(const MonitorDev* __restrict__ mons, int nmon, int* sm, bool flush = false) {
     if (threadIdx.x == 0) {
         int m = 0;
-        while (m + 1 < nmon && mons[m + 1].blk0 <= (int)blockIdx.x) ++m;
+        while (m + 1 < nmon && (flush ? mons[m + 1].fblk0 : mons[m + 1].blk0) <= (int)blockIdx.x) ++m;
         *sm = m;
     }
     __syncthreads();
@@ -765,14 +805,20 @@ __global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDe
 }
 
 // Accumulate staged slots [0, nslots): values[f] += v_s * phase[row_s, f] for s in time order (a real
-// field times a complex phase: one product and one sum per component, as each step would).
+// field times a complex phase: one product and one sum per component, as each step would).  A thread
+// owns one plane cell and kFreqGroup frequencies; consecutive workgroups cover consecutive cells of
+// one frequency group.
 template <typename T>
 __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, int nslots, const int* __restrict__ rows) {
     __shared__ int sm;
-    const MonitorDev M = mons[monitor_of(mons, nmon, &sm)];
+    const MonitorDev M = mons[monitor_of(mons, nmon, &sm, true)];
     if (M.nf == 0) return;
-    const int64_t q = (int64_t)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
+    const int64_t qblocks = (M.N1N2 + blockDim.x - 1) / blockDim.x;
+    const int64_t local = (int64_t)blockIdx.x - M.fblk0;
+    const int fg = (int)(local / qblocks);
+    const int64_t q = (local - fg * qblocks) * blockDim.x + threadIdx.x;
     if (q >= M.N1N2) return;
+    const int f0 = fg * kFreqGroup, f1 = min(M.nf, f0 + kFreqGroup);
     T v[kStageSlots];
     int r[kStageSlots];
 #pragma unroll
@@ -782,7 +828,7 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
     }
     const T* phase = (const T*)M.phase;
     T* acc = (T*)M.values;
-    for (int f = 0; f < M.nf; ++f) {
+    for (int f = f0; f < f1; ++f) {
         T* c = acc + 2 * ((int64_t)f * M.N1N2 + q);
         T re = c[0], im = c[1];
 #pragma unroll
@@ -877,7 +923,7 @@ struct Session {
     void* d_groups = nullptr;  // E groups of all instances, then H groups
     int ngE = 0, ngH = 0;
     MonitorDev* d_mons = nullptr;
-    int n_active = 0, mon_blocks = 0;
+    int n_active = 0, mon_blocks = 0, flush_blocks = 0;
     void* d_stage = nullptr;  // staged phasor-monitor samples
     int* d_rows = nullptr;    // [n_active][kStageSlots] phase row of each staged sample
     int total_tiles = 0;
@@ -1181,7 +1227,7 @@ void refresh(Session& s) {
 
     // monitors of all instances
     std::vector<MonitorDev> md;
-    int blocks = 0;
+    int blocks = 0, fblocks = 0;
     size_t stage_elems = 0;
     for (size_t k = 0; k < s.insts.size(); ++k)
         for (const MonitorHost& h : s.insts[k].mons) {
@@ -1197,11 +1243,14 @@ void refresh(Session& s) {
             d.phase = h.d_phase;
             d.stage = (void*)stage_elems;  // offset for now
             if (d.nf) stage_elems += (size_t)kStageSlots * d.N1N2;
+            d.fblk0 = fblocks;
+            fblocks += (int)((d.N1N2 + 255) / 256) * ((d.nf + kFreqGroup - 1) / kFreqGroup);
             blocks += (int)((d.N1N2 + 255) / 256);
             md.push_back(d);
         }
     s.n_active = (int)md.size();
     s.mon_blocks = blocks;
+    s.flush_blocks = fblocks;
     if (s.d_mons) (void)hipFree(s.d_mons);
     if (s.d_stage) (void)hipFree(s.d_stage);
     if (s.d_rows) (void)hipFree(s.d_rows);
@@ -1246,7 +1295,8 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     if (io && s.ngH) k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local, 0);
     if (io && s.n_active) {
         k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local, slot, s.d_rows);
-        if (flush) k_monitor_flush<T><<<s.mon_blocks, 256, 0, st>>>(s.d_mons, s.n_active, flush, s.d_rows);
+        if (flush && s.flush_blocks)
+            k_monitor_flush<T><<<s.flush_blocks, 256, 0, st>>>(s.d_mons, s.n_active, flush, s.d_rows);
     }
 }
 
@@ -1802,10 +1852,10 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         for (int64_t t = n1 - 1; t >= n0; --t) {
             const int j = (int)(t - n0);
             if (ngH) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + n, d_ag + ngE, ngH, (int)t);
-            k_adj_e<T><<<flat, 256, 0, st>>>(d_di + n, d_flat0, n);
+            k_adj_e<T><<<grid, s.block, 0, st>>>(d_di + n, d_md, n, tile0);
             if (ngE) k_adj_inject<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di + n, d_ag, ngE, (int)t);
             k_adj_grad<T><<<gblocks, 256, 0, st>>>(d_di + n, d_bd, d_g0, n, j);
-            k_adj_h<T><<<flat, 256, 0, st>>>(d_di + n, d_flat0, n);
+            k_adj_h<T><<<grid, s.block, 0, st>>>(d_di + n, d_md, n, tile0);
         }
         HIP_OK(hipGetLastError());
         if ((s.flags & RFDTD_VERBOSE) && (seg % std::max(1, nseg / 10) == 0 || seg == 0)) {
