@@ -34,9 +34,9 @@ E_BYTES, H_BYTES = 120, 144  # algorithmic bytes per cell of the E and H half st
 METRIC = "Mcell-updates/s + % HBM peak, batched vs plain; adjoint gradients/s, 1 GPU"
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X (KiB per
 # dispatch x 1024, FETCH_SIZE x 2 and WRITE_SIZE x 1: the factors that make the counters of a plain
-# 1 GiB copy kernel read 1 GiB each way); profiles/k_update_eh_dipole_slab_pmc.txt
+# 1 GiB copy kernel read 1 GiB each way); profiles/k_update_eh_dipole_slab_pmc.md
 MEASURED_TRAFFIC = {
-    ("dipole_slab", "table"): dict(k_update_e=2646090000, k_update_h=2522127000),
+    ("dipole_slab", "table"): dict(k_update_e=2504854000, k_update_h=2454030000),
 }
 
 
