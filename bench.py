@@ -36,7 +36,8 @@ METRIC = "Mcell-updates/s + % HBM peak, batched vs plain; adjoint gradients/s, 1
 # dispatch x 1024, FETCH_SIZE x 2 and WRITE_SIZE x 1: the factors that make the counters of a plain
 # 1 GiB copy kernel read 1 GiB each way); profiles/k_update_eh_dipole_slab_pmc.md
 MEASURED_TRAFFIC = {
-    ("dipole_slab", "table"): dict(k_update_e=2504854000, k_update_h=2454030000),
+    ("dipole_slab", "table", "k_update_e"): 2504854000,
+    ("dipole_slab", "table", "k_update_h"): 2454030000,
 }
 
 
@@ -396,7 +397,8 @@ def plain_leg(args):
     s.synchronize()
     wall = time.perf_counter() - t0
     total_ms, e_ms, h_ms = s.timing()
-    log(f"timed {args.steps} steps: wall {wall:.3f}s, events {total_ms / 1e3:.3f}s, E {e_ms:.4f} ms, H {h_ms:.4f} ms")
+    log(f"timed {args.steps} steps: wall {wall:.3f}s, events {total_ms / 1e3:.3f}s, E (or fused) {e_ms:.4f} ms, "
+        f"H {h_ms:.4f} ms")
     s.read()
     finite = all(np.isfinite(p["values"][:Nt]).all() for p in probes)
     s.close()
@@ -406,9 +408,12 @@ def plain_leg(args):
     dft_b = dft_bytes_per_step(model)
     step_bytes = cells * (E_BYTES + H_BYTES) + dft_b
     step_frac = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
-    h_gbs = cells * H_BYTES / (h_ms * 1e-3) / 1e9 if h_ms > 0 else 0.0
-    e_gbs = cells * E_BYTES / (e_ms * 1e-3) / 1e9 if e_ms > 0 else 0.0
-    traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"]))
+    fused = bool(tuned and tuned.get("variant", 1) >= 2)
+    # the dominant kernel: the fused E+H pass (264 B/cell) or the H update (144 B/cell)
+    kname, kbytes, kms = ("k_step_fused", E_BYTES + H_BYTES, e_ms) if fused else ("k_update_h", H_BYTES, h_ms)
+    k_gbs = cells * kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    e_gbs = cells * E_BYTES / (e_ms * 1e-3) / 1e9 if e_ms > 0 and not fused else None
+    traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname))
 
     copy = None
     if not args.no_copy_probe:
@@ -428,13 +433,13 @@ def plain_leg(args):
                     dft_planes=len(model.monitors), dft_frequencies=51 if model.monitors else 0,
                     coefficients=info["coef_kind"], coef_entries=[info["n_e_entries"], info["n_h_entries"]],
                     launch=tuned),
-        roofline=dict(bound="hbm", kernel="k_update_h", achieved=round(h_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(h_gbs / HBM_PEAK_GBS, 4), traffic=traffic and traffic["k_update_h"],
-                      algorithmic_bytes_per_launch=cells * H_BYTES, launch_ms=round(h_ms, 4),
-                      measured_hbm_gbs=round(traffic["k_update_h"] / (h_ms * 1e-3) / 1e9, 1) if traffic and h_ms else None),
+        roofline=dict(bound="hbm", kernel=kname, achieved=round(k_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(k_gbs / HBM_PEAK_GBS, 4), traffic=traffic,
+                      algorithmic_bytes_per_launch=cells * kbytes, launch_ms=round(kms, 4),
+                      measured_hbm_gbs=round(traffic / (kms * 1e-3) / 1e9, 1) if traffic and kms else None),
         step=dict(hbm_frac_algorithmic=round(step_frac, 4), algorithmic_bytes=step_bytes,
-                  e_kernel_ms=round(e_ms, 4), e_kernel_gbs=round(e_gbs, 1), h_kernel_ms=round(h_ms, 4),
-                  e_traffic=traffic and traffic["k_update_e"], dft_bytes=dft_b),
+                  e_kernel_ms=None if fused else round(e_ms, 4), e_kernel_gbs=e_gbs and round(e_gbs, 1),
+                  h_kernel_ms=None if fused else round(h_ms, 4), dft_bytes=dft_b),
         copy_probe=copy, cpu_baseline=base, finite=finite,
     )
 

@@ -147,9 +147,10 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
  * the automatic choice that gives enough workgroups to fill the chip); 0 keeps the current value. */
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
 /* Launch autotuner (no counterpart in the reference, whose CUDA path uses one fixed launch shape):
- * times every candidate {pipelined 1/0} x {block 128, 256, 512} x {x planes: automatic, Nx/2 .. Nx/32}
- * over trial_steps field-only steps of this session's grids, keeps the fastest and returns it as
- * config_out = {pipelined, block_threads, x_planes (0 = automatic)} with its ms per step.  Only on a
+ * times every candidate {variant 1, 0} x {block 128, 256, 512} and {variant 2, 3} x {x planes:
+ * automatic, Nx/2 .. Nx/32} over trial_steps field-only steps of this session's grids, keeps the
+ * fastest and returns it as config_out = {variant, block_threads, x_planes (0 = automatic)} with its
+ * ms per step.  Only on a
  * session that has not run yet (step 0); fields stay zero, probes and monitors are untouched. */
 int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3], double* ms_per_step);
 /*
@@ -179,10 +180,13 @@ int rfdtd_gradient_batch(rfdtd_session* s, int32_t n, const int32_t* instances, 
 int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_t* instances, const int32_t* boxes,
                            uint64_t budget_bytes, int32_t* every_out, uint64_t* bytes_out);
 
-/* Kernel variant (no counterpart in the reference): pipelined != 0 (default) issues the loads of the
- * next x plane before the arithmetic and stores of the current one; 0 loads one plane at a time.
- * Results are identical; the choice only changes speed (the autotuner picks it). */
-int rfdtd_set_variant(rfdtd_session* s, int32_t pipelined);
+/* Kernel variant (no counterpart in the reference): 1 (default) separate E and H kernels that issue
+ * the loads of the next x plane before the arithmetic and stores of the current one; 0 the same
+ * kernels loading one plane at a time; 2 and 3 one fused pass per time step (E and H of a tile of
+ * 6 or 14 rows x 62 cells, neighbours through LDS, halos recomputed) that reads one buffer of a
+ * per-instance pair and writes the other (the session then holds two field buffers per
+ * instance).  Results are identical; the choice only changes speed (the autotuner picks it). */
+int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 void rfdtd_destroy(rfdtd_session* s);
 
 /* Bandwidth probe (no counterpart in the reference): copy `bytes` between two device buffers `iters`

@@ -51,16 +51,16 @@ def _store(key, cfg):
 
 def run_tuner(session, trial_steps=20):
     """Run the engine's tuner on a Session or Batch that has not stepped yet; the best shape is
-    left applied.  Returns {pipelined, block, x_planes (0 = automatic), ms_per_step}."""
+    left applied.  Returns {variant, block, x_planes (0 = automatic), ms_per_step}."""
     cfg = (ctypes.c_int32 * 3)()
     ms = ctypes.c_double()
     check(session._L.rfdtd_autotune(session._h, int(trial_steps), ctypes.cast(cfg, ctypes.c_void_p),
                                     ctypes.byref(ms)), "rfdtd_autotune")
-    return dict(pipelined=int(cfg[0]), block=int(cfg[1]), x_planes=int(cfg[2]), ms_per_step=float(ms.value))
+    return dict(variant=int(cfg[0]), block=int(cfg[1]), x_planes=int(cfg[2]), ms_per_step=float(ms.value))
 
 
 def apply(session, cfg):
-    session.set_variant(cfg["pipelined"])
+    session.set_variant(cfg["variant"])
     session.set_launch(cfg["block"], cfg["x_planes"] if cfg["x_planes"] > 0 else -1, 0)
 
 

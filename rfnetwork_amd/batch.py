@@ -95,7 +95,7 @@ class Batch:
 
     def autotune(self, trial_steps=20, use_cache=True):
         """Pick the fastest launch shape for this set of grids (before the first run); cached per
-        set of shapes.  Returns {pipelined, block, x_planes, ms_per_step, cached}."""
+        set of shapes.  Returns {variant, block, x_planes, ms_per_step, cached}."""
         from . import autotune
 
         kind = ctypes.c_int32()
@@ -147,8 +147,8 @@ class Batch:
               "rfdtd_gradient_batch")
         return outs
 
-    def set_variant(self, pipelined=1):
-        check(self._L.rfdtd_set_variant(self._h, int(pipelined)))
+    def set_variant(self, variant=1):
+        check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -117,13 +117,16 @@ struct HCoef {
 // One instance (a grid and one excitation) as the kernels see it.
 template <typename T>
 struct InstDev {
-    Fields<T> F;
+    Fields<T> F;             // the fields the step reads (and the separate kernels update in place)
+    Fields<T> G;             // fused step: the fields it writes (the other buffer of the pair)
     const T* raw[30];        // coefficient arrays (coef_kind 0)
     const uint32_t* eidx;    // per-cell E entry (coef_kind 1)
     const uint32_t* hidx;    // per-cell H entry
     const Vec4<T>* etab;     // 3 Vec4 per E entry
     const Vec4<T>* htab;     // 5 Vec4 per H entry (18 values + 2 pad)
-    int32_t coef_kind, Nx, Ny, Nz, xplanes, npb, pad0, pad1;
+    const uint32_t* ebits;   // fused step: one bit per cell holding an E probe
+    const void* egroups;     // fused step: the E probe groups, sorted by cell
+    int32_t coef_kind, Nx, Ny, Nz, xplanes, npb, tz_n, negroups;
 };
 
 // A member (grid) owns tiles * K logical workgroups starting at tile0; its K instances are
@@ -439,7 +442,7 @@ __global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<
         case HY: tot = F.hy; s1 = F.hy_z; s2 = F.hy_x; break;
         default: tot = F.hz; s1 = F.hz_x; s2 = F.hz_y; break;
     }
-    T a = s1[G.cell], b = s2[G.cell], s = tot[G.cell];
+    T a = s1[G.cell], b = s2[G.cell], s = a + b;  // the total, as the update kernels form it
     T* row = G.series + (size_t)n * G.nprobes;
     const T* row_in = recompute ? G.series_in + (size_t)n * G.nprobes : row;
     for (int k = 0; k < G.count; ++k) {
@@ -455,6 +458,178 @@ __global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<
     s1[G.cell] = a;
     s2[G.cell] = b;
     tot[G.cell] = s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// fused step: E and H half steps of one time step in one pass, from one field buffer into the other
+//
+// A workgroup owns a tile of TY rows (y) x kFz cells (z) and a range of x planes; it has TY + 2
+// wavefronts of 64 lanes.  Lane l of wave w sits at y = y0 - 1 + w, z = z0 - 1 + l:
+//   interior   1 <= w <= TY, 1 <= l <= kFz: E and H updated and stored;
+//   lower halo w = 0 or l = 0: E computed (not stored) for the interior's H update (y-1, z-1);
+//   upper halo w = TY + 1 or l = kFz + 1: H totals read for the interior's E update (y+1, z+1).
+// The tile marches in x: the x+1 H splits are prefetched and carried, the x-1 E totals carried, and
+// the y/z neighbours pass through LDS.  A chunk that does not start at x = 0 first computes E on
+// plane x0 - 1 without storing it.  E sources are applied where they are computed (the bitmask
+// marks probe cells; halo copies apply them too, only the owner records), reading the source
+// samples from the pristine series; H sources and monitors follow in their own launches on the
+// written buffer.  Every value is formed by the same operations in the same order as in the
+// separate E and H kernels, so results are identical.
+// ------------------------------------------------------------------------------------------------
+
+constexpr int kFz = 62;  // interior z cells per tile (64 lanes minus the two z halos)
+
+template <typename T>
+__device__ __forceinline__ void inject_e(const InstDev<T>& I, int64_t cell, int n, bool record, T& exy, T& exz,
+                                         T& eyz, T& eyx, T& ezx, T& ezy) {
+    const ProbeGroup<T>* g = (const ProbeGroup<T>*)I.egroups;
+    int lo = 0, hi = I.negroups;
+    while (lo < hi) {  // first group at this cell
+        const int mid = (lo + hi) >> 1;
+        if (g[mid].cell < cell) lo = mid + 1;
+        else hi = mid;
+    }
+    for (int k = lo; k < I.negroups && g[k].cell == cell; ++k) {
+        const ProbeGroup<T> G = g[k];
+        T &a = G.field == EX ? exy : G.field == EY ? eyz : ezx;
+        T &b = G.field == EX ? exz : G.field == EY ? eyx : ezy;
+        T s = a + b;
+        T* row = G.series + (size_t)n * G.nprobes;
+        const T* row_in = G.series_in + (size_t)n * G.nprobes;
+        for (int j = 0; j < G.count; ++j) {
+            const int pid = G.order[j];
+            if (G.is_src[pid]) {
+                const T v = row_in[pid];
+                a = a + v;
+                b = b + v;
+                s = a + b;
+            }
+            if (record) row[pid] = s;
+        }
+    }
+}
+
+template <typename T, int TY, class CA>
+__device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int x0, int x1, int y0, int z0, int n,
+                                           bool io, T* sm) {
+    using G = GLOBAL_T(T);
+    constexpr int NT = (TY + 2) * 64;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const int Nx = I.Nx, Ny = I.Ny, Nz = I.Nz;
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int y = y0 - 1 + w, z = z0 - 1 + l;
+    const bool valid = y >= 0 && y < Ny && z >= 0 && z < Nz;
+    const bool ecomp = valid && w <= TY && l <= kFz;             // E computed here
+    const bool inner = ecomp && w >= 1 && l >= 1;                // E and H stored here
+    const bool yin = y < Ny - 1, zin = z < Nz - 1;
+    T* sHx = sm; T* sHy = sm + NT; T* sHz = sm + 2 * NT;
+    T* sEx = sm + 3 * NT; T* sEy = sm + 4 * NT; T* sEz = sm + 5 * NT;
+    const Fields<T>& A = I.F;
+    const Fields<T>& B = I.G;
+    const int xs = x0 > 0 ? x0 - 1 : x0;
+    const uint32_t p = valid ? ((uint32_t)y * (uint32_t)Nz + (uint32_t)z) : 0u;
+    uint32_t b = ((uint32_t)xs * (uint32_t)Ny * (uint32_t)Nz + p) * S;
+    // H splits of the current plane (carried), E totals of the previous plane (carried)
+    T hxy = T(0), hxz = T(0), hyz = T(0), hyx = T(0), hzx = T(0), hzy = T(0);
+    if (valid) {
+        hxy = ldg((const G*)A.hx_y, b); hxz = ldg((const G*)A.hx_z, b); hyz = ldg((const G*)A.hy_z, b);
+        hyx = ldg((const G*)A.hy_x, b); hzx = ldg((const G*)A.hz_x, b); hzy = ldg((const G*)A.hz_y, b);
+    }
+    T ey_xm = T(0), ez_xm = T(0);
+    for (int x = xs; x < x1; ++x, b += sx) {
+        const bool pre = x < x0, xin = x < Nx - 1;
+        // next plane's H splits (E needs its Hy, Hz; the H update of the next plane needs all six)
+        T nxy = T(0), nxz = T(0), nyz = T(0), nyx = T(0), nzx = T(0), nzy = T(0);
+        if (valid && xin) {
+            nxy = ldg((const G*)A.hx_y, b + sx); nxz = ldg((const G*)A.hx_z, b + sx);
+            nyz = ldg((const G*)A.hy_z, b + sx); nyx = ldg((const G*)A.hy_x, b + sx);
+            nzx = ldg((const G*)A.hz_x, b + sx); nzy = ldg((const G*)A.hz_y, b + sx);
+        }
+        const T hx_c = hxy + hxz, hy_c = hyz + hyx, hz_c = hzx + hzy;
+        sHx[tid] = hx_c;
+        sHy[tid] = hy_c;
+        sHz[tid] = hz_c;
+        __syncthreads();
+        T ex = T(0), ey = T(0), ez = T(0);
+        if (ecomp) {
+            T exy = ldg((const G*)A.ex_y, b), exz = ldg((const G*)A.ex_z, b), eyz = ldg((const G*)A.ey_z, b);
+            T eyx = ldg((const G*)A.ey_x, b), ezx = ldg((const G*)A.ez_x, b), ezy = ldg((const G*)A.ez_y, b);
+            const ECoef<T> c = C.e(b);
+            const T hz_yn = yin ? sHz[tid + 64] : T(0), hx_yn = yin ? sHx[tid + 64] : T(0);
+            const T hy_zn = zin ? sHy[tid + 1] : T(0), hx_zn = zin ? sHx[tid + 1] : T(0);
+            const T hy_xn = xin ? nyz + nyx : T(0), hz_xn = xin ? nzx + nzy : T(0);
+            if (yin) exy = c.ca_xy * exy + c.cb_xy * (hz_yn - hz_c);
+            if (zin) exz = c.ca_xz * exz + c.cb_xz * (hy_zn - hy_c);
+            if (zin) eyz = c.ca_yz * eyz + c.cb_yz * (hx_zn - hx_c);
+            if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (hz_xn - hz_c);
+            if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (hy_xn - hy_c);
+            if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (hx_yn - hx_c);
+            if (io && I.negroups) {
+                const int64_t cell = ((int64_t)x * Ny + y) * Nz + z;
+                if ((I.ebits[cell >> 5] >> (cell & 31)) & 1u)
+                    inject_e(I, cell, n, inner && !pre, exy, exz, eyz, eyx, ezx, ezy);
+            }
+            ex = exy + exz;
+            ey = eyz + eyx;
+            ez = ezx + ezy;
+            if (inner && !pre) {
+                stg((G*)B.ex_y, b, exy); stg((G*)B.ex_z, b, exz); stg((G*)B.ey_z, b, eyz);
+                stg((G*)B.ey_x, b, eyx); stg((G*)B.ez_x, b, ezx); stg((G*)B.ez_y, b, ezy);
+            }
+        }
+        sEx[tid] = ex;
+        sEy[tid] = ey;
+        sEz[tid] = ez;
+        __syncthreads();
+        if (inner && !pre) {
+            const HCoef<T> c = C.h(b);
+            const T ez_ym = y > 0 ? sEz[tid - 64] : T(0), ex_ym = y > 0 ? sEx[tid - 64] : T(0);
+            const T ey_zm = z > 0 ? sEy[tid - 1] : T(0), ex_zm = z > 0 ? sEx[tid - 1] : T(0);
+            const T xy = c.da_xy * hxy + (c.db_xy2 * ez - c.db_xy1 * ez_ym);
+            const T xz = c.da_xz * hxz + (c.db_xz2 * ey - c.db_xz1 * ey_zm);
+            const T yz = c.da_yz * hyz + (c.db_yz2 * ex - c.db_yz1 * ex_zm);
+            const T yx = c.da_yx * hyx + (c.db_yx2 * ez - c.db_yx1 * ez_xm);
+            const T zx = c.da_zx * hzx + (c.db_zx2 * ey - c.db_zx1 * ey_xm);
+            const T zy = c.da_zy * hzy + (c.db_zy2 * ex - c.db_zy1 * ex_ym);
+            stg((G*)B.hx_y, b, xy); stg((G*)B.hx_z, b, xz); stg((G*)B.hy_z, b, yz);
+            stg((G*)B.hy_x, b, yx); stg((G*)B.hz_x, b, zx); stg((G*)B.hz_y, b, zy);
+        }
+        ey_xm = ey;
+        ez_xm = ez;
+        hxy = nxy; hxz = nxz; hyz = nyz; hyx = nyx; hzx = nzx; hzy = nzy;
+    }
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__((TY + 2) * 64) k_step_fused(const InstDev<T>* __restrict__ insts,
+                                                              const MemberDev* __restrict__ mem, int nmem, int total,
+                                                              const int* base, int local, int io) {
+    __shared__ int s_member;
+    __shared__ T sm[6 * (TY + 2) * 64];
+    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    if (threadIdx.x == 0) {
+        int m = 0;
+        if (L < total)
+            while (m + 1 < nmem && mem[m + 1].tile0 <= L) ++m;
+        s_member = m;
+    }
+    __syncthreads();
+    if (L >= total) return;  // uniform over the workgroup: no barrier is skipped by part of it
+    const int m = __builtin_amdgcn_readfirstlane(s_member);
+    const MemberDev M = mem[m];
+    const int local_t = L - M.tile0;
+    const int e = local_t % M.K, tile = local_t / M.K;
+    const InstDev<T>* I = insts + __builtin_amdgcn_readfirstlane(M.inst0 + e);
+    const int tyz = I->npb, tz_n = I->tz_n;
+    const int pb = tile % tyz, xc = tile / tyz;
+    const int y0 = (pb / tz_n) * TY, z0 = (pb % tz_n) * kFz;
+    const int x0 = xc * I->xplanes, x1 = min(I->Nx, x0 + I->xplanes);
+    const int n = step_of(base, local);
+    if (I->coef_kind == 1)
+        fused_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0, z0, n, io != 0, sm);
+    else
+        fused_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, sm);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -752,15 +927,16 @@ struct MonitorDev {
 constexpr int kStageSlots = 10;
 constexpr int kFreqGroup = 4;  // frequencies per thread in the flush
 
+// The total of a component at one cell, formed from its splits as the update kernels form it.
 template <typename T>
-__device__ __forceinline__ const T* monitor_source(const InstDev<T>& I, int field) {
+__device__ __forceinline__ T total_at(const InstDev<T>& I, int field, int64_t i) {
     switch (field) {
-        case EX: return I.F.ex;
-        case EY: return I.F.ey;
-        case EZ: return I.F.ez;
-        case HX: return I.F.hx;
-        case HY: return I.F.hy;
-        default: return I.F.hz;
+        case EX: return I.F.ex_y[i] + I.F.ex_z[i];
+        case EY: return I.F.ey_z[i] + I.F.ey_x[i];
+        case EZ: return I.F.ez_x[i] + I.F.ez_y[i];
+        case HX: return I.F.hx_y[i] + I.F.hx_z[i];
+        case HY: return I.F.hy_z[i] + I.F.hy_x[i];
+        default: return I.F.hz_x[i] + I.F.hz_y[i];
     }
 }
 
@@ -797,7 +973,7 @@ __global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDe
         idx = ((int64_t)a * Ny + M.pos) * Nz + b;
     else
         idx = ((int64_t)a * Ny + b) * Nz + M.pos;
-    const T v = monitor_source(I, M.field)[idx];
+    const T v = total_at(I, M.field, idx);
     if (M.nf == 0)
         ((T*)M.values)[(int64_t)(n / M.n_step) * M.N1N2 + q] = v;
     else
@@ -888,7 +1064,11 @@ struct MonitorHost {
 struct Instance {
     int member = 0;
     void* d_fields = nullptr;
+    void* d_fields2 = nullptr;  // the second buffer of the fused step's pair
     size_t fields_bytes = 0, pitch = 0;
+    uint32_t* d_ebits = nullptr;  // fused step: E-probe cells
+    void* d_egroups = nullptr;    // fused step: E probe groups sorted by cell
+    int negroups = 0;
     // probes
     int P = 0;
     void* d_series = nullptr;
@@ -934,10 +1114,16 @@ struct Session {
     bool pipelined = true;  // issue the next x plane's loads before this plane's stores
     int ckpt_every = 0;     // save the split fields every this many steps (0: off)
     bool tuning = false;    // autotune trials: field updates only, no probes or monitors
+    int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
+    int parity = 0;         // which buffer of each instance's pair holds the current fields
+    bool totals_stale = false;  // the fused step leaves the total arrays unwritten
+    void* d_insts2 = nullptr;   // descriptors with the buffers of each pair exchanged
+    std::vector<int64_t> flat0; // block offsets of a flat launch over all instances' cells
+    int64_t* d_flat0 = nullptr;
 
     int* d_step_base = nullptr;
-    hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_def = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // captured chunk, per starting parity
+    hipGraph_t graph_def[2] = {nullptr, nullptr};
     bool graph_dirty = true;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -950,20 +1136,17 @@ struct Session {
     void drop_graph();
     void release_probes(Instance& I);
     void release_monitors(Instance& I);
-    template <typename T>
-    Fields<T> fields(const Instance& I) const {
-        Fields<T> f;
-        T** a = (T**)&f;
-        for (int k = 0; k < 18; ++k) a[k] = (T*)((char*)I.d_fields + (size_t)k * I.pitch);
-        return f;
-    }
+    // the buffer that holds the instance's current fields
+    void* cur(const Instance& I) const { return parity ? I.d_fields2 : I.d_fields; }
 };
 
 void Session::drop_graph() {
-    if (graph) (void)hipGraphExecDestroy(graph);
-    if (graph_def) (void)hipGraphDestroy(graph_def);
-    graph = nullptr;
-    graph_def = nullptr;
+    for (int p = 0; p < 2; ++p) {
+        if (graph[p]) (void)hipGraphExecDestroy(graph[p]);
+        if (graph_def[p]) (void)hipGraphDestroy(graph_def[p]);
+        graph[p] = nullptr;
+        graph_def[p] = nullptr;
+    }
     graph_dirty = true;
 }
 
@@ -974,6 +1157,9 @@ void Session::release_probes(Instance& I) {
     if (I.d_order) (void)hipFree(I.d_order);
     if (I.d_is_src) (void)hipFree(I.d_is_src);
     I.d_series = nullptr; I.d_order = nullptr; I.d_is_src = nullptr;
+    if (I.d_ebits) (void)hipFree(I.d_ebits);
+    if (I.d_egroups) (void)hipFree(I.d_egroups);
+    I.d_ebits = nullptr; I.d_egroups = nullptr; I.negroups = 0;
     I.P = 0; I.ngE = 0;
     I.groups.clear();
     I.probe_host_values.clear();
@@ -994,12 +1180,14 @@ Session::~Session() {
         release_probes(I);
         release_monitors(I);
         if (I.d_fields) (void)hipFree(I.d_fields);
+        if (I.d_fields2) (void)hipFree(I.d_fields2);
         for (void* c : I.ckpt)
             if (c) (void)hipFree(c);
     }
     for (auto& M : members)
         if (M.d_coef) (void)hipFree(M.d_coef);
-    for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base, d_stage, (void*)d_rows})
+    for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base, d_stage, (void*)d_rows,
+                    d_insts2, (void*)d_flat0})
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1133,9 +1321,15 @@ void upload_coefficients(Member& M, uint32_t flags, const void* const coeff[30])
 
 // x planes per workgroup: enough workgroups to give every CU several waves (>= ~2048 groups in a
 // launch on 256 CUs), as many planes as possible for register reuse along x.
+// Workgroups per x plane of a member: (y,z) columns per workgroup, or fused tiles.
+int64_t tiles_per_plane(const Session& s, const Member& M) {
+    if (s.fused) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kFz - 1) / kFz);
+    return ((int64_t)M.Ny * M.Nz + s.block - 1) / s.block;
+}
+
 void plan_launch(Session& s) {
     int64_t blocks_per_plane = 0;
-    for (auto& M : s.members) blocks_per_plane += M.K * (((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
+    for (auto& M : s.members) blocks_per_plane += M.K * tiles_per_plane(s, M);
     const int64_t want = 2048;
     for (auto& M : s.members) {
         if (M.xplanes_req > 0) {
@@ -1148,13 +1342,53 @@ void plan_launch(Session& s) {
     }
 }
 
+// Block offsets of a batched flat launch over items of the given sizes (256 threads per block).
+std::vector<int64_t> block_offsets(const std::vector<int64_t>& sizes) {
+    std::vector<int64_t> off(sizes.size() + 1, 0);
+    for (size_t i = 0; i < sizes.size(); ++i) off[i + 1] = off[i] + (sizes[i] + 255) / 256;
+    if (off.back() > INT32_MAX) RF_THROW(RFDTD_EINVAL, "batched launch too large");
+    return off;
+}
+
+// Fused step: the instance's E probe groups sorted by cell, and a bitmask of their cells.
+template <typename T>
+void build_e_lookup(Session& s, Instance& I, const Member& M) {
+    if (!s.fused || I.d_ebits || I.ngE == 0) return;
+    std::vector<ProbeGroup<T>> g;
+    for (int k = 0; k < I.ngE; ++k) {
+        const auto& hg = I.groups[k];
+        ProbeGroup<T> pg{};
+        pg.cell = hg.cell;
+        pg.series = (T*)I.d_series;
+        pg.series_in = (const T*)I.d_series_in;
+        pg.order = I.d_order + hg.start;
+        pg.is_src = I.d_is_src;
+        pg.field = hg.field;
+        pg.count = hg.count;
+        pg.nprobes = I.P;
+        g.push_back(pg);
+    }
+    std::stable_sort(g.begin(), g.end(), [](const ProbeGroup<T>& a, const ProbeGroup<T>& b) { return a.cell < b.cell; });
+    std::vector<uint32_t> bits((size_t)(M.cells + 31) / 32, 0u);
+    for (const auto& pg : g) bits[(size_t)(pg.cell >> 5)] |= 1u << (pg.cell & 31);
+    HIP_OK(hipMalloc(&I.d_ebits, bits.size() * 4));
+    HIP_OK(hipMemcpy(I.d_ebits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&I.d_egroups, g.size() * sizeof(ProbeGroup<T>)));
+    HIP_OK(hipMemcpy(I.d_egroups, g.data(), g.size() * sizeof(ProbeGroup<T>), hipMemcpyHostToDevice));
+    I.negroups = (int)g.size();
+}
+
 // Kernel descriptor of a field buffer (18 arrays at the given pitch) on member M's grid.
 template <typename T>
-InstDev<T> make_instdev(const Member& M, void* fields, size_t pitch, int npb) {
+InstDev<T> make_instdev(const Member& M, void* fields, size_t pitch, int npb, void* out = nullptr) {
     InstDev<T> d;
     std::memset(&d, 0, sizeof(d));
     T** a = (T**)&d.F;
     for (int k = 0; k < 18; ++k) a[k] = (T*)((char*)fields + (size_t)k * pitch);
+    if (out) {
+        T** g = (T**)&d.G;
+        for (int k = 0; k < 18; ++k) g[k] = (T*)((char*)out + (size_t)k * pitch);
+    }
     for (int k = 0; k < 30; ++k) d.raw[k] = (const T*)M.raw[k];
     d.eidx = M.eidx;
     d.hidx = M.hidx;
@@ -1164,6 +1398,7 @@ InstDev<T> make_instdev(const Member& M, void* fields, size_t pitch, int npb) {
     d.Nx = M.Nx; d.Ny = M.Ny; d.Nz = M.Nz;
     d.xplanes = M.xplanes;
     d.npb = npb;
+    d.tz_n = (M.Nz + kFz - 1) / kFz;
     return d;
 }
 
@@ -1171,30 +1406,51 @@ template <typename T>
 void refresh(Session& s) {
     if (!s.dirty) return;
     plan_launch(s);
-    // instance descriptors
-    std::vector<InstDev<T>> di(s.insts.size());
+    if (s.fused)
+        for (Instance& I : s.insts)
+            if (!I.d_fields2) {
+                HIP_OK(hipMalloc(&I.d_fields2, I.fields_bytes));
+                HIP_OK(hipMemset(I.d_fields2, 0, I.fields_bytes));
+            }
+    // instance descriptors: [parity 0] reads buffer 1 of each pair and writes buffer 2, [parity 1]
+    // the other way round (the separate kernels only use F, the current buffer)
+    std::vector<InstDev<T>> di(s.insts.size()), di2(s.insts.size());
     std::vector<MemberDev> dm(s.members.size());
     int tile0 = 0;
     for (size_t m = 0; m < s.members.size(); ++m) {
         Member& M = s.members[m];
-        const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
+        const int npb = (int)tiles_per_plane(s, M);
         const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
         dm[m] = MemberDev{tile0, npb * chunks, M.K, M.inst0};
         tile0 += npb * chunks * M.K;
         for (int e = 0; e < M.K; ++e) {
-            const Instance& I = s.insts[M.inst0 + e];
-            di[M.inst0 + e] = make_instdev<T>(M, I.d_fields, I.pitch, npb);
+            Instance& I = s.insts[M.inst0 + e];
+            build_e_lookup<T>(s, I, M);
+            di[M.inst0 + e] = make_instdev<T>(M, I.d_fields, I.pitch, npb, I.d_fields2);
+            di2[M.inst0 + e] = I.d_fields2 ? make_instdev<T>(M, I.d_fields2, I.pitch, npb, I.d_fields) : di[M.inst0 + e];
+            for (InstDev<T>* d : {&di[M.inst0 + e], &di2[M.inst0 + e]}) {
+                d->ebits = I.d_ebits;
+                d->egroups = I.d_egroups;
+                d->negroups = I.negroups;
+            }
         }
     }
     s.total_tiles = tile0;
-    if (s.d_insts) (void)hipFree(s.d_insts);
-    if (s.d_members) (void)hipFree(s.d_members);
-    s.d_insts = nullptr;
-    s.d_members = nullptr;
+    for (void** p : {&s.d_insts, &s.d_insts2, (void**)&s.d_members, (void**)&s.d_flat0}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
     HIP_OK(hipMalloc(&s.d_insts, di.size() * sizeof(InstDev<T>)));
     HIP_OK(hipMemcpy(s.d_insts, di.data(), di.size() * sizeof(InstDev<T>), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&s.d_insts2, di2.size() * sizeof(InstDev<T>)));
+    HIP_OK(hipMemcpy(s.d_insts2, di2.data(), di2.size() * sizeof(InstDev<T>), hipMemcpyHostToDevice));
     HIP_OK(hipMalloc(&s.d_members, dm.size() * sizeof(MemberDev)));
     HIP_OK(hipMemcpy(s.d_members, dm.data(), dm.size() * sizeof(MemberDev), hipMemcpyHostToDevice));
+    std::vector<int64_t> cells;
+    for (const Instance& I : s.insts) cells.push_back(s.members[I.member].cells);
+    s.flat0 = block_offsets(cells);
+    HIP_OK(hipMalloc(&s.d_flat0, s.flat0.size() * sizeof(int64_t)));
+    HIP_OK(hipMemcpy(s.d_flat0, s.flat0.data(), s.flat0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
 
     // probe groups of all instances: E groups, then H groups
     std::vector<ProbeGroup<T>> ge, gh;
@@ -1275,23 +1531,41 @@ template <typename T>
 void launch_step(Session& s, hipStream_t st, const int* base, int local, int slot, int flush,
                  hipEvent_t* ev = nullptr) {
     const int grid = (s.total_tiles + 7) & ~7;
-    const InstDev<T>* di = (const InstDev<T>*)s.d_insts;
+    const InstDev<T>* desc[2] = {(const InstDev<T>*)s.d_insts, (const InstDev<T>*)s.d_insts2};
+    const InstDev<T>* di = desc[s.parity];
     const int nmem = (int)s.members.size();
     const ProbeGroup<T>* groups = (const ProbeGroup<T>*)s.d_groups;
-    if (ev) HIP_OK(hipEventRecord(ev[0], st));
-    if (s.pipelined)
-        k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
-    else
-        k_update_e<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
-    if (ev) HIP_OK(hipEventRecord(ev[1], st));
     const bool io = !s.tuning;
-    if (io && s.ngE) k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(di, groups, s.ngE, base, local, 0);
-    if (ev) HIP_OK(hipEventRecord(ev[2], st));
-    if (s.pipelined)
-        k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
-    else
-        k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
-    if (ev) HIP_OK(hipEventRecord(ev[3], st));
+    if (s.fused) {
+        // one pass writes the other buffer of each pair; H sources and monitors then work on it
+        if (ev) HIP_OK(hipEventRecord(ev[0], st));
+        if (s.fused == 14)
+            k_step_fused<T, 14><<<grid, 16 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
+        else
+            k_step_fused<T, 6><<<grid, 8 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
+        if (ev) {
+            HIP_OK(hipEventRecord(ev[1], st));
+            HIP_OK(hipEventRecord(ev[2], st));
+            HIP_OK(hipEventRecord(ev[3], st));
+        }
+        s.parity ^= 1;
+        s.totals_stale = true;
+        di = desc[s.parity];
+    } else {
+        if (ev) HIP_OK(hipEventRecord(ev[0], st));
+        if (s.pipelined)
+            k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+        else
+            k_update_e<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+        if (ev) HIP_OK(hipEventRecord(ev[1], st));
+        if (io && s.ngE) k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(di, groups, s.ngE, base, local, 0);
+        if (ev) HIP_OK(hipEventRecord(ev[2], st));
+        if (s.pipelined)
+            k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+        else
+            k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+        if (ev) HIP_OK(hipEventRecord(ev[3], st));
+    }
     if (io && s.ngH) k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local, 0);
     if (io && s.n_active) {
         k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local, slot, s.d_rows);
@@ -1300,44 +1574,62 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     }
 }
 
+// After fused steps the separate kernels need the totals of the current buffer again.
+template <typename T>
+void rebuild_totals(Session& s, hipStream_t st) {
+    if (!s.totals_stale) return;
+    const InstDev<T>* di = s.parity ? (const InstDev<T>*)s.d_insts2 : (const InstDev<T>*)s.d_insts;
+    k_totals<T><<<(int)s.flat0.back(), 256, 0, st>>>(di, s.d_flat0, (int)s.insts.size());
+    HIP_OK(hipGetLastError());
+    s.totals_stale = false;
+}
 
 void save_checkpoints(Session& s, int slot, hipStream_t st) {
     for (Instance& I : s.insts) {
         if ((int)I.ckpt.size() <= slot) I.ckpt.resize(slot + 1, nullptr);
         if (!I.ckpt[slot]) HIP_OK(hipMalloc(&I.ckpt[slot], 12 * I.pitch));
-        HIP_OK(hipMemcpyAsync(I.ckpt[slot], I.d_fields, 12 * I.pitch, hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemcpyAsync(I.ckpt[slot], s.cur(I), 12 * I.pitch, hipMemcpyDeviceToDevice, st));
     }
 }
 
+// Steps per captured chunk: even for the fused step, so that a replay ends on the buffer it began on.
+int chunk_steps(const Session& s) { return s.fused ? std::max(2, s.graph_steps & ~1) : s.graph_steps; }
+
+// Capture one chunk of steps starting from the current parity (the graph for that parity).
 template <typename T>
 void capture_graph(Session& s) {
-    s.drop_graph();
+    if (s.graph_dirty) s.drop_graph();
+    s.graph_dirty = false;
+    const int p = s.parity, n = chunk_steps(s);
     hipStream_t cs;
     HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < s.graph_steps; ++k) {
+    for (int k = 0; k < n; ++k) {
         const int slot = k % kStageSlots;
-        const bool last = slot == kStageSlots - 1 || k == s.graph_steps - 1;
+        const bool last = slot == kStageSlots - 1 || k == n - 1;
         launch_step<T>(s, cs, s.d_step_base, k, slot, last ? slot + 1 : 0);
     }
-    hipError_t ce = hipStreamEndCapture(cs, &s.graph_def);
+    hipError_t ce = hipStreamEndCapture(cs, &s.graph_def[p]);
     (void)hipStreamDestroy(cs);
+    s.parity = p;  // an even number of fused steps: the chunk ends where it began
     HIP_OK(ce);
-    HIP_OK(hipGraphInstantiate(&s.graph, s.graph_def, nullptr, nullptr, 0));
-    s.graph_dirty = false;
+    HIP_OK(hipGraphInstantiate(&s.graph[p], s.graph_def[p], nullptr, nullptr, 0));
 }
 
 // len steps starting at s.step + offset (s.step itself is advanced by the caller)
 template <typename T>
 void run_chunk(Session& s, int len, int offset, hipStream_t st, bool graphs, bool timing, int stride) {
     int done = 0;
-    if (graphs && len >= s.graph_steps) {
-        if (s.graph_dirty || !s.graph) capture_graph<T>(s);
-        while (len - done >= s.graph_steps) {
+    const int gs = chunk_steps(s);
+    if (!s.fused) rebuild_totals<T>(s, st);
+    if (graphs && len >= gs) {
+        if (s.graph_dirty || !s.graph[s.parity]) capture_graph<T>(s);
+        while (len - done >= gs) {
             k_set_step<<<1, 1, 0, st>>>(s.d_step_base, (int)(s.step + offset + done));
-            HIP_OK(hipGraphLaunch(s.graph, st));
-            done += s.graph_steps;
+            HIP_OK(hipGraphLaunch(s.graph[s.parity], st));
+            done += gs;
         }
+        if (s.fused) s.totals_stale = true;
     }
     for (int d = 0; done < len; ++done, ++d) {
         const int64_t n = s.step + offset + done;
@@ -1411,12 +1703,14 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const bool pf0 = s.pipelined;
+    const int v0 = s.fused == 14 ? 3 : s.fused == 6 ? 2 : s.pipelined ? 1 : 0;
     const int blk0 = s.block;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
-    auto apply = [&](bool pf, int blk, const std::vector<int>& xr) {
-        s.pipelined = pf;
+    // variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row tiles, 3 fused 14-row tiles
+    auto apply = [&](int v, int blk, const std::vector<int>& xr) {
+        s.pipelined = v != 0;
+        s.fused = v == 2 ? 6 : v == 3 ? 14 : 0;
         s.block = blk;
         for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
         s.dirty = true;
@@ -1424,14 +1718,14 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     };
     hipStream_t st = s.own_stream;
     double best = 1e300;
-    bool bpf = pf0;
-    int bblk = blk0, bxp = 0;
+    int bv = v0, bblk = blk0, bxp = 0;
     s.tuning = true;
     try {
-        for (int pf : {1, 0})
+        for (int v : {1, 0, 2, 3})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
-                    apply(pf != 0, blk, std::vector<int>(s.members.size(), xp));
+                    if (v >= 2 && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
+                    apply(v, blk, std::vector<int>(s.members.size(), xp));
                     refresh<T>(s);
                     run_chunk<T>(s, std::min(trial_steps, 3), 0, st, false, false, 0);  // warm-up
                     HIP_OK(hipEventRecord(s.ev0, st));
@@ -1443,21 +1737,26 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
                     const double per = (double)ms / trial_steps;
                     if (per < best) {
                         best = per;
-                        bpf = pf != 0;
+                        bv = v;
                         bblk = blk;
                         bxp = xp;
                     }
                 }
     } catch (...) {
         s.tuning = false;
-        apply(pf0, blk0, xr0);
+        apply(v0, blk0, xr0);
         throw;
     }
     s.tuning = false;
-    apply(bpf, bblk, std::vector<int>(s.members.size(), bxp));
+    apply(bv, bblk, std::vector<int>(s.members.size(), bxp));
     HIP_OK(hipDeviceSynchronize());
-    for (Instance& I : s.insts) HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
-    cfg[0] = bpf ? 1 : 0;
+    for (Instance& I : s.insts) {
+        HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
+        if (I.d_fields2) HIP_OK(hipMemset(I.d_fields2, 0, I.fields_bytes));
+    }
+    s.parity = 0;
+    s.totals_stale = false;
+    cfg[0] = bv;
     cfg[1] = bblk;
     cfg[2] = bxp;
     if (ms_out) *ms_out = best;
@@ -1662,12 +1961,25 @@ void read_fields(Session& s, int inst, void* const out[18]) {
     Instance& I = instance_at(s, inst);
     const Member& M = s.members[I.member];
     const int Nx = M.Nx, Ny = M.Ny, Nz = M.Nz;
-    std::vector<char> buf((size_t)M.cells * s.esz);
+    std::vector<char> buf((size_t)M.cells * s.esz), b2;
     // component of each of the 18 arrays and the offset of GPU cell (0,0,0) in the global array
     const int comp[18] = {EX, EX, EY, EY, EZ, EZ, HX, HX, HY, HY, HZ, HZ, EX, EY, EZ, HX, HY, HZ};
+    const char* cur = (const char*)s.cur(I);
     for (int a = 0; a < 18; ++a) {
         if (!out[a]) continue;
-        HIP_OK(hipMemcpy(buf.data(), (char*)I.d_fields + (size_t)a * I.pitch, buf.size(), hipMemcpyDeviceToHost));
+        if (a < 12) {
+            HIP_OK(hipMemcpy(buf.data(), cur + (size_t)a * I.pitch, buf.size(), hipMemcpyDeviceToHost));
+        } else {
+            // a total is the sum of its two splits, formed as the update kernels form it
+            const int k = 2 * (a - 12);
+            b2.resize(buf.size());
+            HIP_OK(hipMemcpy(buf.data(), cur + (size_t)k * I.pitch, buf.size(), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(b2.data(), cur + (size_t)(k + 1) * I.pitch, b2.size(), hipMemcpyDeviceToHost));
+            if (s.f64)
+                for (int64_t i = 0; i < M.cells; ++i) ((double*)buf.data())[i] += ((const double*)b2.data())[i];
+            else
+                for (int64_t i = 0; i < M.cells; ++i) ((float*)buf.data())[i] += ((const float*)b2.data())[i];
+        }
         const int c = comp[a];
         const int gx = Nx + ((c == EY || c == EZ || c == HX) ? 1 : 0);
         const int gy = Ny + ((c == EX || c == EZ || c == HY) ? 1 : 0);
@@ -1697,13 +2009,6 @@ struct GradBuffers {
     }
 };
 
-// Block offsets of a batched flat launch over items of the given sizes (256 threads per block).
-std::vector<int64_t> block_offsets(const std::vector<int64_t>& sizes) {
-    std::vector<int64_t> off(sizes.size() + 1, 0);
-    for (size_t i = 0; i < sizes.size(); ++i) off[i + 1] = off[i] + (sizes[i] + 255) / 256;
-    if (off.back() > INT32_MAX) RF_THROW(RFDTD_EINVAL, "batched launch too large");
-    return off;
-}
 
 // One gradient request: instance, design box, dJ/dy of its probes, output.
 struct GradItem {
@@ -1955,7 +2260,7 @@ extern "C" {
 
 const char* rfdtd_last_error(void) { return g_last_error.c_str(); }
 
-int32_t rfdtd_version(void) { return 1; }
+int32_t rfdtd_version(void) { return 2; }
 
 int rfdtd_create(rfdtd_session** out, int32_t Nx, int32_t Ny, int32_t Nz, const void* const coefficients[30],
                  uint32_t flags) {
@@ -2053,9 +2358,12 @@ int rfdtd_reset(rfdtd_session* s) {
         HIP_OK(hipDeviceSynchronize());
         for (Instance& I : S.insts) {
             HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
+            if (I.d_fields2) HIP_OK(hipMemset(I.d_fields2, 0, I.fields_bytes));
             for (auto& h : I.mons) HIP_OK(hipMemset(h.d_values, 0, monitor_bytes(S, h)));
         }
         S.step = 0;
+        S.parity = 0;
+        S.totals_stale = false;
     });
 }
 
@@ -2083,7 +2391,7 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
             ne += M.nE;
             nh += M.nH;
         }
-        for (const auto& I : S.insts) b += I.fields_bytes;
+        for (const auto& I : S.insts) b += I.fields_bytes * (I.d_fields2 ? 2 : 1);
         if (device_bytes) *device_bytes = b;
         if (coef_kind) *coef_kind = S.members[0].coef_kind;
         if (n_e_entries) *n_e_entries = ne;
@@ -2112,10 +2420,13 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
     });
 }
 
-int rfdtd_set_variant(rfdtd_session* s, int32_t pipelined) {
+int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        S.pipelined = pipelined != 0;
+        if (variant < 0 || variant > 3) RF_THROW(RFDTD_EINVAL, "variant must be 0, 1, 2 or 3");
+        S.pipelined = variant != 0;
+        S.fused = variant == 2 ? 6 : variant == 3 ? 14 : 0;
+        S.dirty = true;
         S.graph_dirty = true;
     });
 }

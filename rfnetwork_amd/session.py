@@ -104,7 +104,7 @@ class Session:
 
     def autotune(self, trial_steps=20, use_cache=True):
         """Pick the fastest launch shape for this grid (before the first run); cached per shape.
-        Returns {pipelined, block, x_planes, ms_per_step, cached}."""
+        Returns {variant, block, x_planes, ms_per_step, cached}."""
         from . import autotune
 
         kind = self.memory_info()["coef_kind"]
@@ -141,9 +141,11 @@ class Session:
               "rfdtd_gradient")
         return out
 
-    def set_variant(self, pipelined=1):
-        """Kernel variant: 1 issues the next x plane's loads before the current plane's stores."""
-        check(self._L.rfdtd_set_variant(self._h, int(pipelined)))
+    def set_variant(self, variant=1):
+        """Kernel variant: 1 (default) separate E and H kernels issuing the next x plane's loads early,
+        0 the same one plane at a time, 2 and 3 the fused E+H pass with 6- or 14-row tiles (two field
+        buffers per instance).  Results are identical."""
+        check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -22,7 +22,7 @@ def test_autotuned_session_matches_oracle(hip_ready, tmp_path, monkeypatch):
         s.set_sources(probes, monitors, Nt)
         cfg = s.autotune(trial_steps=5)
         assert cfg["cached"] is False
-        assert cfg["pipelined"] in (0, 1) and cfg["block"] in (128, 256, 512) and cfg["x_planes"] >= 0
+        assert cfg["variant"] in (0, 1, 2, 3) and cfg["block"] in (128, 256, 512) and cfg["x_planes"] >= 0
         assert cfg["ms_per_step"] > 0
         s.run(Nt)
         s.synchronize()
@@ -38,7 +38,7 @@ def test_autotuned_session_matches_oracle(hip_ready, tmp_path, monkeypatch):
     with Session(c, Nx, Ny, Nz) as s:
         cfg2 = s.autotune(trial_steps=5)
     assert cfg2["cached"] is True
-    assert {k: cfg2[k] for k in ("pipelined", "block", "x_planes")} == {k: cfg[k] for k in ("pipelined", "block", "x_planes")}
+    assert {k: cfg2[k] for k in ("variant", "block", "x_planes")} == {k: cfg[k] for k in ("variant", "block", "x_planes")}
 
 
 def test_autotune_refuses_a_session_that_ran(hip_ready):
