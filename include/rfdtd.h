@@ -147,7 +147,7 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
  * the automatic choice that gives enough workgroups to fill the chip); 0 keeps the current value. */
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
 /* Launch autotuner (no counterpart in the reference, whose CUDA path uses one fixed launch shape):
- * times every candidate {variant 1, 0} x {block 128, 256, 512} and {variant 2, 3} x {x planes:
+ * times every candidate {variant 1, 4, 0} x {block 128, 256, 512} and {variant 2, 3} x {x planes:
  * automatic, Nx/2 .. Nx/32} over trial_steps field-only steps of this session's grids, keeps the
  * fastest and returns it as config_out = {variant, block_threads, x_planes (0 = automatic)} with its
  * ms per step.  Only on a
@@ -185,9 +185,31 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * kernels loading one plane at a time; 2 and 3 one fused pass per time step (E and H of a tile of
  * 6 or 14 rows x 62 cells, neighbours through LDS, halos recomputed) that reads one buffer of a
  * per-instance pair and writes the other (the session then holds two field buffers per
- * instance).  Results are identical; the choice only changes speed (the autotuner picks it). */
+ * instance); 4 variant 1 with nontemporal stores of the fields.  Results are identical; the choice
+ * only changes speed (the autotuner picks it). */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 void rfdtd_destroy(rfdtd_session* s);
+
+/*
+ * Far field (replaces core_func.nf2ff, rfnetwork/core/src/postprocess.cpp:130-505, called from
+ * FDTD_Solver.get_farfield_rE, rfnetwork/solver.py:2870-3020): the radiation integrals of the
+ * equivalent surface currents J = n x H and M = -n x E on the faces of a closed box (Balanis,
+ * Advanced Engineering Electromagnetics, 2nd ed., 6.8.2), on the GPU.  out: complex64
+ * [2][n_frequencies][n_theta][n_phi], polarisation theta then phi, r E with exp(-j beta r)/r removed.
+ */
+typedef struct rfdtd_ff_face {
+    int32_t axis;       /* normal axis 0..2 */
+    int32_t n0, n1;     /* cells along the two in-face axes (lower axis number first) */
+    int32_t reserved;
+    double position;    /* face position along the normal, metres */
+    const float* r0;    /* [n0][n1] cell-centre coordinate along the first in-face axis, metres */
+    const float* r1;    /* [n0][n1] along the second */
+    const float* ds;    /* [n0][n1] cell areas, square metres */
+    const void* J;      /* complex64 [3][n_frequencies][n0][n1] */
+    const void* M;      /* complex64 [3][n_frequencies][n0][n1] */
+} rfdtd_ff_face;
+int rfdtd_nf2ff(int32_t n_faces, const rfdtd_ff_face* faces, int32_t n_frequencies, const float* beta,
+                int32_t n_theta, const float* theta, int32_t n_phi, const float* phi, float eta0, void* out);
 
 /* Bandwidth probe (no counterpart in the reference): copy `bytes` between two device buffers `iters`
  * times with a plain streaming kernel and report read+write GB/s and ms per copy -- the measured HBM

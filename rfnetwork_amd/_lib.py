@@ -50,7 +50,7 @@ EXPORTS = (
     "rfdtd_set_launch", "rfdtd_destroy", "rfdtd_copy_bandwidth", "rfdtd_create_batch",
     "rfdtd_instance_count", "rfdtd_set_instance_sources", "rfdtd_read_instance_fields", "rfdtd_set_variant",
     "rfdtd_set_checkpointing", "rfdtd_gradient", "rfdtd_autotune", "rfdtd_gradient_batch",
-    "rfdtd_plan_checkpoints",
+    "rfdtd_plan_checkpoints", "rfdtd_nf2ff",
 )
 
 
@@ -131,6 +131,7 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_gradient.argtypes = [vp, i32, vp, vp, i32, vp]
     L.rfdtd_autotune.argtypes = [vp, i32, vp, ctypes.POINTER(ctypes.c_double)]
     L.rfdtd_gradient_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.rfdtd_nf2ff.argtypes = [i32, vp, i32, vp, i32, vp, i32, vp, ctypes.c_float, vp]
     L.rfdtd_plan_checkpoints.argtypes = [vp, i32, i32, vp, vp, ctypes.c_uint64, ctypes.POINTER(i32),
                                          ctypes.POINTER(ctypes.c_uint64)]
     L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
@@ -156,6 +157,53 @@ def pointer_array(arrays) -> "ctypes.Array":
     for k, a in enumerate(arrays):
         arr[k] = None if a is None else a.ctypes.data
     return arr
+
+
+class FFFace(ctypes.Structure):
+    _fields_ = [
+        ("axis", ctypes.c_int32),
+        ("n0", ctypes.c_int32),
+        ("n1", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("position", ctypes.c_double),
+        ("r0", ctypes.c_void_p),
+        ("r1", ctypes.c_void_p),
+        ("ds", ctypes.c_void_p),
+        ("J", ctypes.c_void_p),
+        ("M", ctypes.c_void_p),
+    ]
+
+
+def nf2ff(faces, beta, theta, phi, eta0):
+    """Radiation integrals of far-field box faces on the GPU.  faces: dicts with axis, position,
+    r0, r1, ds (float32 [n0, n1]) and J, M (complex64 [3, nf, n0, n1]); beta, theta, phi float32
+    (radians).  Returns complex64 [2, nf, ntheta, nphi]."""
+    import numpy as np
+
+    arr = (FFFace * len(faces))()
+    keep = []
+    for k, f in enumerate(faces):
+        r0 = np.ascontiguousarray(f["r0"], dtype=np.float32)
+        r1 = np.ascontiguousarray(f["r1"], dtype=np.float32)
+        ds = np.ascontiguousarray(f["ds"], dtype=np.float32)
+        J = np.ascontiguousarray(f["J"], dtype=np.complex64)
+        M = np.ascontiguousarray(f["M"], dtype=np.complex64)
+        if J.shape != M.shape or J.shape[0] != 3 or J.shape[2:] != r0.shape or r1.shape != r0.shape or ds.shape != r0.shape:
+            raise ValueError("inconsistent far-field face arrays")
+        keep += [r0, r1, ds, J, M]
+        arr[k].axis, arr[k].n0, arr[k].n1 = int(f["axis"]), r0.shape[0], r0.shape[1]
+        arr[k].position = float(f["position"])
+        arr[k].r0, arr[k].r1, arr[k].ds = r0.ctypes.data, r1.ctypes.data, ds.ctypes.data
+        arr[k].J, arr[k].M = J.ctypes.data, M.ctypes.data
+    beta = np.ascontiguousarray(beta, dtype=np.float32)
+    theta = np.ascontiguousarray(theta, dtype=np.float32)
+    phi = np.ascontiguousarray(phi, dtype=np.float32)
+    if faces and keep[3].shape[1] != len(beta):
+        raise ValueError("one beta per frequency of the currents is required")
+    out = np.zeros((2, len(beta), len(theta), len(phi)), dtype=np.complex64)
+    check(lib().rfdtd_nf2ff(len(faces), ctypes.cast(arr, ctypes.c_void_p), len(beta), beta.ctypes.data, len(theta),
+                            theta.ctypes.data, len(phi), phi.ctypes.data, float(eta0), out.ctypes.data), "rfdtd_nf2ff")
+    return out
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20):

@@ -100,6 +100,15 @@ template <class T>
 __device__ __forceinline__ void stg(GLOBAL_AS T* base, uint32_t boff, T v) {
     *(GLOBAL_AS T*)((GLOBAL_AS char*)base + boff) = v;
 }
+// The same store with the nontemporal hint (NT = true): streamed fields that will not be reread
+// before they leave the caches.
+template <bool NT, class T>
+__device__ __forceinline__ void stf(GLOBAL_AS T* base, uint32_t boff, T v) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, (GLOBAL_AS T*)((GLOBAL_AS char*)base + boff));
+    else
+        stg(base, boff, v);
+}
 
 // E coefficients of one cell, order of the reference dictionary entries 0..11.
 template <typename T>
@@ -276,7 +285,7 @@ struct HIn {
 
 // E half step over one column of x planes.  Offsets are 32-bit byte offsets (grids up to 2^29
 // cells).
-template <typename T, class CA, bool PF>
+template <typename T, class CA, bool PF, bool NT>
 __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
     using G = GLOBAL_T(T);
     G* const ex_y = (G*)F.ex_y; G* const ex_z = (G*)F.ex_z;
@@ -316,11 +325,11 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
         if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (cur.hz_xn - hz_c);
         if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (cur.hy_xn - hy_c);
         if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (cur.hx_yn - cur.hx_c);
-        stg(ex_y, b, exy); stg(ex_z, b, exz); stg(ey_z, b, eyz);
-        stg(ey_x, b, eyx); stg(ez_x, b, ezx); stg(ez_y, b, ezy);
-        stg(ex, b, exy + exz);
-        stg(ey, b, eyz + eyx);
-        stg(ez, b, ezx + ezy);
+        stf<NT>(ex_y, b, exy); stf<NT>(ex_z, b, exz); stf<NT>(ey_z, b, eyz);
+        stf<NT>(ey_x, b, eyx); stf<NT>(ez_x, b, ezx); stf<NT>(ez_y, b, ezy);
+        stf<NT>(ex, b, exy + exz);
+        stf<NT>(ey, b, eyz + eyx);
+        stf<NT>(ez, b, ezx + ezy);
         hy_c = cur.hy_xn;
         hz_c = cur.hz_xn;
         if (PF)
@@ -331,7 +340,7 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
 }
 
 // H half step over one column of x planes; neighbours below index 0 are zero.
-template <typename T, class CA, bool PF>
+template <typename T, class CA, bool PF, bool NT>
 __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
     using G = GLOBAL_T(T);
     G* const hx_y = (G*)F.hx_y; G* const hx_z = (G*)F.hx_z;
@@ -370,11 +379,11 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
         const T hyx = c.da_yx * cur.hyx + (c.db_yx2 * cur.ez_c - c.db_yx1 * ez_xm);
         const T hzx = c.da_zx * cur.hzx + (c.db_zx2 * cur.ey_c - c.db_zx1 * ey_xm);
         const T hzy = c.da_zy * cur.hzy + (c.db_zy2 * cur.ex_c - c.db_zy1 * cur.ex_ym);
-        stg(hx_y, b, hxy); stg(hx_z, b, hxz); stg(hy_z, b, hyz);
-        stg(hy_x, b, hyx); stg(hz_x, b, hzx); stg(hz_y, b, hzy);
-        stg(hx, b, hxy + hxz);
-        stg(hy, b, hyz + hyx);
-        stg(hz, b, hzx + hzy);
+        stf<NT>(hx_y, b, hxy); stf<NT>(hx_z, b, hxz); stf<NT>(hy_z, b, hyz);
+        stf<NT>(hy_x, b, hyx); stf<NT>(hz_x, b, hzx); stf<NT>(hz_y, b, hzy);
+        stf<NT>(hx, b, hxy + hxz);
+        stf<NT>(hy, b, hyz + hyx);
+        stf<NT>(hz, b, hzx + hzy);
         ey_xm = cur.ey_c;
         ez_xm = cur.ez_c;
         if (PF)
@@ -384,7 +393,7 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
     }
 }
 
-template <typename T, bool PF>
+template <typename T, bool PF, bool NT = false>
 __global__ void __launch_bounds__(1024) k_update_e(const InstDev<T>* __restrict__ insts,
                                                    const MemberDev* __restrict__ mem, int nmem, int total) {
     __shared__ int s_member;
@@ -392,12 +401,12 @@ __global__ void __launch_bounds__(1024) k_update_e(const InstDev<T>* __restrict_
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
     if (I->coef_kind == 1)
-        e_column<T, CoefTable<T>, PF>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+        e_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
     else
-        e_column<T, CoefRaw<T>, PF>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        e_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
 }
 
-template <typename T, bool PF>
+template <typename T, bool PF, bool NT = false>
 __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict__ insts,
                                                    const MemberDev* __restrict__ mem, int nmem, int total) {
     __shared__ int s_member;
@@ -405,9 +414,9 @@ __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict_
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
     if (I->coef_kind == 1)
-        h_column<T, CoefTable<T>, PF>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+        h_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
     else
-        h_column<T, CoefRaw<T>, PF>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        h_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
 }
 
 // Probes sharing one (component, cell) of one instance form a group processed by one thread in
@@ -546,6 +555,17 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
             nyz = ldg((const G*)A.hy_z, b + sx); nyx = ldg((const G*)A.hy_x, b + sx);
             nzx = ldg((const G*)A.hz_x, b + sx); nzy = ldg((const G*)A.hz_y, b + sx);
         }
+        // this plane's E splits and both coefficient sets, issued before the barriers so that they
+        // arrive while the tile exchanges neighbours
+        T exy = T(0), exz = T(0), eyz = T(0), eyx = T(0), ezx = T(0), ezy = T(0);
+        ECoef<T> c{};
+        HCoef<T> ch{};
+        if (ecomp) {
+            exy = ldg((const G*)A.ex_y, b); exz = ldg((const G*)A.ex_z, b); eyz = ldg((const G*)A.ey_z, b);
+            eyx = ldg((const G*)A.ey_x, b); ezx = ldg((const G*)A.ez_x, b); ezy = ldg((const G*)A.ez_y, b);
+            c = C.e(b);
+        }
+        if (inner && !pre) ch = C.h(b);
         const T hx_c = hxy + hxz, hy_c = hyz + hyx, hz_c = hzx + hzy;
         sHx[tid] = hx_c;
         sHy[tid] = hy_c;
@@ -553,9 +573,6 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
         __syncthreads();
         T ex = T(0), ey = T(0), ez = T(0);
         if (ecomp) {
-            T exy = ldg((const G*)A.ex_y, b), exz = ldg((const G*)A.ex_z, b), eyz = ldg((const G*)A.ey_z, b);
-            T eyx = ldg((const G*)A.ey_x, b), ezx = ldg((const G*)A.ez_x, b), ezy = ldg((const G*)A.ez_y, b);
-            const ECoef<T> c = C.e(b);
             const T hz_yn = yin ? sHz[tid + 64] : T(0), hx_yn = yin ? sHx[tid + 64] : T(0);
             const T hy_zn = zin ? sHy[tid + 1] : T(0), hx_zn = zin ? sHx[tid + 1] : T(0);
             const T hy_xn = xin ? nyz + nyx : T(0), hz_xn = xin ? nzx + nzy : T(0);
@@ -583,7 +600,7 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
         sEz[tid] = ez;
         __syncthreads();
         if (inner && !pre) {
-            const HCoef<T> c = C.h(b);
+            const HCoef<T>& c = ch;
             const T ez_ym = y > 0 ? sEz[tid - 64] : T(0), ex_ym = y > 0 ? sEx[tid - 64] : T(0);
             const T ey_zm = z > 0 ? sEy[tid - 1] : T(0), ex_zm = z > 0 ? sEx[tid - 1] : T(0);
             const T xy = c.da_xy * hxy + (c.db_xy2 * ez - c.db_xy1 * ez_ym);
@@ -1019,6 +1036,101 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// far field (NF2FF): radiation integrals of the equivalent surface currents of a closed box
+// ------------------------------------------------------------------------------------------------
+
+struct FFFace {
+    int32_t axis, n0, n1, pad;
+    float pos;               // face position along its normal, metres
+    int32_t cell0;           // first cell of this face in the concatenated cell list
+    const float* r0;         // [n0][n1] cell centres along the first in-face axis
+    const float* r1;         // [n0][n1] along the second
+    const float* ds;         // [n0][n1] cell areas
+    const float2* J;         // [3][nf][n0][n1] n x H
+    const float2* M;         // [3][nf][n0][n1] -n x E
+};
+
+constexpr int kFFChunk = 256;
+
+// One thread per direction, a workgroup per (frequency, block of directions); the face cells pass
+// through LDS in chunks.  For every cell: I = exp(j beta r'.r^) dS, and
+//   N_theta += (Jx ct cp + Jy ct sp - Jz st) I,  N_phi += (-Jx sp + Jy cp) I,  L likewise with M;
+// then E_theta = (-j beta / 4 pi)(L_phi + eta0 N_theta), E_phi = (j beta / 4 pi)(L_theta - eta0 N_phi).
+__global__ void __launch_bounds__(kFFChunk) k_nf2ff(const FFFace* __restrict__ faces, int nfaces, int ncells, int nf,
+                                                   const float* __restrict__ beta, const float* __restrict__ theta,
+                                                   int ntheta, const float* __restrict__ phi, int nphi, float eta0,
+                                                   float2* __restrict__ out) {
+    __shared__ float s_r0[kFFChunk], s_r1[kFFChunk], s_ds[kFFChunk];
+    __shared__ float2 s_j[3][kFFChunk], s_m[3][kFFChunk];
+    __shared__ int s_axis[kFFChunk];
+    __shared__ float s_pos[kFFChunk];
+    const int ndir = ntheta * nphi;
+    const int dblocks = (ndir + kFFChunk - 1) / kFFChunk;
+    const int f = blockIdx.x / dblocks;
+    const int dir = (blockIdx.x % dblocks) * kFFChunk + threadIdx.x;
+    const bool live = dir < ndir;
+    const int th = live ? dir / nphi : 0, ph = live ? dir % nphi : 0;
+    float st, ct, sp, cp;
+    sincosf(theta[th], &st, &ct);
+    sincosf(phi[ph], &sp, &cp);
+    const float u = st * cp, v = st * sp, w = ct;
+    const float b = beta[f];
+    float2 Nt = make_float2(0.f, 0.f), Np = Nt, Lt = Nt, Lp = Nt;
+    for (int c0 = 0; c0 < ncells; c0 += kFFChunk) {
+        __syncthreads();
+        const int c = c0 + threadIdx.x;
+        if (c < ncells) {
+            int k = 0;
+            while (k + 1 < nfaces && faces[k + 1].cell0 <= c) ++k;
+            const FFFace& F = faces[k];
+            const int i = c - F.cell0;
+            const int64_t plane = (int64_t)F.n0 * F.n1;
+            s_r0[threadIdx.x] = F.r0[i];
+            s_r1[threadIdx.x] = F.r1[i];
+            s_ds[threadIdx.x] = F.ds[i];
+            s_axis[threadIdx.x] = F.axis;
+            s_pos[threadIdx.x] = F.pos;
+            for (int a = 0; a < 3; ++a) {
+                s_j[a][threadIdx.x] = F.J[((int64_t)a * nf + f) * plane + i];
+                s_m[a][threadIdx.x] = F.M[((int64_t)a * nf + f) * plane + i];
+            }
+        }
+        __syncthreads();
+        const int n = min(kFFChunk, ncells - c0);
+        for (int q = 0; q < n; ++q) {
+            const int ax = s_axis[q];
+            const float r0 = s_r0[q], r1 = s_r1[q];
+            const float rdot = ax == 0 ? s_pos[q] * u + (r0 * v + r1 * w)
+                             : ax == 1 ? s_pos[q] * v + (r0 * u + r1 * w)
+                                       : s_pos[q] * w + (r0 * u + r1 * v);
+            float si, co;
+            sincosf(b * rdot, &si, &co);
+            const float ire = co * s_ds[q], iim = si * s_ds[q];
+            const float2 jx = s_j[0][q], jy = s_j[1][q], jz = s_j[2][q];
+            const float2 mx = s_m[0][q], my = s_m[1][q], mz = s_m[2][q];
+            // theta and phi projections of the currents (real factors of the direction)
+            const float2 jt = make_float2(jx.x * (ct * cp) + jy.x * (ct * sp) - jz.x * st,
+                                          jx.y * (ct * cp) + jy.y * (ct * sp) - jz.y * st);
+            const float2 jp = make_float2(-jx.x * sp + jy.x * cp, -jx.y * sp + jy.y * cp);
+            const float2 mt = make_float2(mx.x * (ct * cp) + my.x * (ct * sp) - mz.x * st,
+                                          mx.y * (ct * cp) + my.y * (ct * sp) - mz.y * st);
+            const float2 mp = make_float2(-mx.x * sp + my.x * cp, -mx.y * sp + my.y * cp);
+            Nt.x += jt.x * ire - jt.y * iim; Nt.y += jt.x * iim + jt.y * ire;
+            Np.x += jp.x * ire - jp.y * iim; Np.y += jp.x * iim + jp.y * ire;
+            Lt.x += mt.x * ire - mt.y * iim; Lt.y += mt.x * iim + mt.y * ire;
+            Lp.x += mp.x * ire - mp.y * iim; Lp.y += mp.x * iim + mp.y * ire;
+        }
+    }
+    if (!live) return;
+    const float k4 = b / (4.0f * 3.14159265358979f);
+    // -j k4 (Lp + eta0 Nt) and j k4 (Lt - eta0 Np)
+    const float2 a1 = make_float2(Lp.x + eta0 * Nt.x, Lp.y + eta0 * Nt.y);
+    const float2 a2 = make_float2(Lt.x - eta0 * Np.x, Lt.y - eta0 * Np.y);
+    out[((int64_t)0 * nf + f) * ndir + dir] = make_float2(k4 * a1.y, -k4 * a1.x);
+    out[((int64_t)1 * nf + f) * ndir + dir] = make_float2(-k4 * a2.y, k4 * a2.x);
+}
+
 __global__ void k_set_step(int* p, int v) { *p = v; }
 
 // Plain streaming copy (16 B per lane per iteration): the measured bandwidth ceiling, and the
@@ -1115,6 +1227,7 @@ struct Session {
     int ckpt_every = 0;     // save the split fields every this many steps (0: off)
     bool tuning = false;    // autotune trials: field updates only, no probes or monitors
     int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
+    bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
     int parity = 0;         // which buffer of each instance's pair holds the current fields
     bool totals_stale = false;  // the fused step leaves the total arrays unwritten
     void* d_insts2 = nullptr;   // descriptors with the buffers of each pair exchanged
@@ -1553,14 +1666,18 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
         di = desc[s.parity];
     } else {
         if (ev) HIP_OK(hipEventRecord(ev[0], st));
-        if (s.pipelined)
+        if (s.nontemporal)
+            k_update_e<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+        else if (s.pipelined)
             k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
         else
             k_update_e<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
         if (ev) HIP_OK(hipEventRecord(ev[1], st));
         if (io && s.ngE) k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(di, groups, s.ngE, base, local, 0);
         if (ev) HIP_OK(hipEventRecord(ev[2], st));
-        if (s.pipelined)
+        if (s.nontemporal)
+            k_update_h<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+        else if (s.pipelined)
             k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
         else
             k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
@@ -1703,7 +1820,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const int v0 = s.fused == 14 ? 3 : s.fused == 6 ? 2 : s.pipelined ? 1 : 0;
+    const int v0 = s.fused == 14 ? 3 : s.fused == 6 ? 2 : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
     const int blk0 = s.block;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
@@ -1711,6 +1828,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     auto apply = [&](int v, int blk, const std::vector<int>& xr) {
         s.pipelined = v != 0;
         s.fused = v == 2 ? 6 : v == 3 ? 14 : 0;
+        s.nontemporal = v == 4;
         s.block = blk;
         for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
         s.dirty = true;
@@ -1721,7 +1839,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     int bv = v0, bblk = blk0, bxp = 0;
     s.tuning = true;
     try {
-        for (int v : {1, 0, 2, 3})
+        for (int v : {1, 4, 0, 2, 3})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
                     if (v >= 2 && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
@@ -2423,9 +2541,10 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 3) RF_THROW(RFDTD_EINVAL, "variant must be 0, 1, 2 or 3");
+        if (variant < 0 || variant > 4) RF_THROW(RFDTD_EINVAL, "variant must be 0, 1, 2, 3 or 4");
         S.pipelined = variant != 0;
         S.fused = variant == 2 ? 6 : variant == 3 ? 14 : 0;
+        S.nontemporal = variant == 4;
         S.dirty = true;
         S.graph_dirty = true;
     });
@@ -2489,6 +2608,59 @@ int rfdtd_gradient_batch(rfdtd_session* s, int32_t n, const int32_t* instances, 
 void rfdtd_destroy(rfdtd_session* s) {
     if (!s) return;
     delete (Session*)s;
+}
+
+int rfdtd_nf2ff(int32_t n_faces, const rfdtd_ff_face* faces, int32_t n_frequencies, const float* beta,
+                int32_t n_theta, const float* theta, int32_t n_phi, const float* phi, float eta0, void* out) {
+    std::vector<void*> bufs;
+    const int rc = guarded([&] {
+        if (n_faces < 1 || !faces || n_frequencies < 1 || !beta || n_theta < 1 || !theta || n_phi < 1 || !phi || !out)
+            RF_THROW(RFDTD_EINVAL, "nf2ff arguments missing");
+        auto up = [&](const void* src, size_t bytes) {
+            void* d = nullptr;
+            HIP_OK(hipMalloc(&d, std::max<size_t>(bytes, 4)));
+            bufs.push_back(d);
+            HIP_OK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+            return d;
+        };
+        std::vector<FFFace> ff(n_faces);
+        int64_t ncells = 0;
+        for (int k = 0; k < n_faces; ++k) {
+            const rfdtd_ff_face& F = faces[k];
+            if (F.axis < 0 || F.axis > 2 || F.n0 < 1 || F.n1 < 1 || !F.r0 || !F.r1 || !F.ds || !F.J || !F.M)
+                RF_THROW(RFDTD_EINVAL, "invalid far-field face");
+            const size_t plane = (size_t)F.n0 * F.n1;
+            ff[k].axis = F.axis;
+            ff[k].n0 = F.n0;
+            ff[k].n1 = F.n1;
+            ff[k].pos = (float)F.position;
+            ff[k].cell0 = (int32_t)ncells;
+            ff[k].r0 = (const float*)up(F.r0, plane * 4);
+            ff[k].r1 = (const float*)up(F.r1, plane * 4);
+            ff[k].ds = (const float*)up(F.ds, plane * 4);
+            ff[k].J = (const float2*)up(F.J, 3 * (size_t)n_frequencies * plane * 8);
+            ff[k].M = (const float2*)up(F.M, 3 * (size_t)n_frequencies * plane * 8);
+            ncells += (int64_t)plane;
+        }
+        if (ncells > INT32_MAX) RF_THROW(RFDTD_EINVAL, "too many far-field cells");
+        const FFFace* d_faces = (const FFFace*)up(ff.data(), ff.size() * sizeof(FFFace));
+        const float* d_beta = (const float*)up(beta, (size_t)n_frequencies * 4);
+        const float* d_theta = (const float*)up(theta, (size_t)n_theta * 4);
+        const float* d_phi = (const float*)up(phi, (size_t)n_phi * 4);
+        const int64_t ndir = (int64_t)n_theta * n_phi;
+        const size_t out_bytes = 2 * (size_t)n_frequencies * ndir * 8;
+        float2* d_out = nullptr;
+        HIP_OK(hipMalloc(&d_out, out_bytes));
+        bufs.push_back(d_out);
+        const int64_t blocks = (int64_t)n_frequencies * ((ndir + kFFChunk - 1) / kFFChunk);
+        if (blocks > INT32_MAX) RF_THROW(RFDTD_EINVAL, "too many far-field directions");
+        k_nf2ff<<<(int)blocks, kFFChunk>>>(d_faces, n_faces, (int)ncells, n_frequencies, d_beta, d_theta, n_theta, d_phi,
+                                          n_phi, eta0, d_out);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(out, d_out, out_bytes, hipMemcpyDeviceToHost));
+    });
+    for (void* p : bufs) (void)hipFree(p);
+    return rc;
 }
 
 int rfdtd_copy_bandwidth(uint64_t bytes, int32_t iters, double* gbytes_per_s, double* ms_per_copy) {

@@ -76,6 +76,7 @@ class FDTD_Solver:
         # element type of the coefficient assembly: float32 as in the reference; float64 gives exact
         # coefficients for finite-difference checks of gradients
         self.coef_dtype = np.float32
+        self.farfield = dict()
 
     # ------------------------------------------------------------------ state
     @property
@@ -536,6 +537,142 @@ class FDTD_Solver:
                 raise ValueError("Field position out of bounds")
             self.monitors[nm] = dict(field=f, axis=ai, position=pos, index=idx, n_step=n_step, shape=tuple(shape),
                                      frequency=np.atleast_1d(frequency) if frequency is not None else None)
+
+    def add_farfield_monitor(self, frequency, padding: int = 2):
+        """Phasor monitors on the faces of a box ``padding`` cells inside the PML, for the far field
+        (rfnetwork/solver.py:1777-1891): on every face with PML outside it, the two tangential E
+        components on the face and the two tangential H components on the planes either side of it.
+        Faces without PML are PEC walls and get no monitors (they radiate nothing)."""
+        self.check_mesh()
+        if not len(self.pml_boundaries):
+            raise ValueError("No radiation or PML boundaries found. Unable to create farfield monitor.")
+        frequency = np.atleast_1d(np.asarray(frequency, dtype=np.float64))
+        ff_idx = np.zeros((3, 2), dtype=np.int64)
+        self.farfield["surf_pos"] = np.zeros((3, 2), dtype=np.float64)
+        self.farfield["cell_pos"] = [None] * 3
+        self.farfield["cell_w"] = [None] * 3
+        for axis in range(3):
+            a_s = _AXES[axis]
+            sf0, sf1 = [i for i in (0, 1, 2) if i != axis]
+            for j, side in enumerate(("n", "p")):
+                pml = a_s + ("-" if side == "n" else "+")
+                skipped = pml not in self.pml_boundaries
+                if skipped:
+                    ff_idx[axis, j] = 0 if j == 0 else len(self.g_edges[axis]) - 1
+                elif side == "n":
+                    ff_idx[axis, j] = self._n_pml + padding
+                else:
+                    ff_idx[axis, j] = len(self.g_edges[axis]) - self._n_pml - 1 - padding
+                surf = int(ff_idx[axis, j])
+                self.farfield["surf_pos"][axis, j] = conv.m_in(self.g_edges[axis][surf])
+                if skipped:
+                    continue
+                for f in (sf0, sf1):
+                    f_s = _AXES[f]
+                    self.add_field_monitor(f"ff_e{f_s}_{side}{a_s}", f"e{f_s}", a_s, index=surf, frequency=frequency)
+                    self.add_field_monitor(f"ff_h{f_s}1_{side}{a_s}", f"h{f_s}", a_s, index=surf, frequency=frequency)
+                    other = surf - 1 if j == 0 else surf + 1
+                    self.add_field_monitor(f"ff_h{f_s}2_{side}{a_s}", f"h{f_s}", a_s, index=other, frequency=frequency)
+        for axis in range(3):
+            sf0, sf1 = [i for i in (0, 1, 2) if i != axis]
+            c0s = slice(ff_idx[sf0, 0], ff_idx[sf0, 1])
+            c1s = slice(ff_idx[sf1, 0], ff_idx[sf1, 1])
+            self.farfield["cell_pos"][axis] = np.meshgrid(conv.m_in(self.g_cells[sf0][c0s]),
+                                                          conv.m_in(self.g_cells[sf1][c1s]), indexing="ij")
+            self.farfield["cell_w"][axis] = np.meshgrid(conv.m_in(self.d_cells[sf0][c0s]),
+                                                        conv.m_in(self.d_cells[sf1][c1s]), indexing="ij")
+        self.farfield["frequency"] = frequency
+        self.farfield["idx"] = ff_idx
+
+    def _farfield_faces(self):
+        """Equivalent surface currents on the far-field box faces (rfnetwork/solver.py:2907-3002): E
+        and H averaged to the cell centres of each face, J = n x H, M = -n x E, complex64."""
+        ff_idx = self.farfield["idx"]
+        nf = len(self.farfield["frequency"])
+        faces = []
+        for axis in range(3):
+            a_s = _AXES[axis]
+            sf0, sf1 = [i for i in (0, 1, 2) if i != axis]
+            shape = self.farfield["cell_pos"][axis][0].shape
+            r0 = np.ascontiguousarray(self.farfield["cell_pos"][axis][0], dtype=np.float32)
+            r1 = np.ascontiguousarray(self.farfield["cell_pos"][axis][1], dtype=np.float32)
+            ds = np.ascontiguousarray(self.farfield["cell_w"][axis][0] * self.farfield["cell_w"][axis][1],
+                                      dtype=np.float32)
+            for j, side in enumerate(("n", "p")):
+                e_xyz = np.zeros((3, nf) + shape, dtype=np.complex64)
+                h_xyz = np.zeros((3, nf) + shape, dtype=np.complex64)
+                present = False
+                for f in (sf0, sf1):
+                    f_s = _AXES[f]
+                    emon, hm1, hm2 = f"ff_e{f_s}_{side}{a_s}", f"ff_h{f_s}1_{side}{a_s}", f"ff_h{f_s}2_{side}{a_s}"
+                    if emon not in self.monitors:
+                        continue
+                    present = True
+                    edata = np.asarray(self.get_monitor_data(emon))
+                    h1, h2 = np.asarray(self.get_monitor_data(hm1)), np.asarray(self.get_monitor_data(hm2))
+                    w1 = self.d_cells[axis][self.monitors[hm1]["index"]]
+                    w2 = self.d_cells[axis][self.monitors[hm2]["index"]]
+                    hdata = (h1 * (w1 / 2) + h2 * (w2 / 2)) / ((w1 / 2) + (w2 / 2))
+                    # E averaged along the other in-face axis, H along its own axis, to the cell centres
+                    ax_e = 1 if f == sf1 else 2
+                    ax_h = 1 if f == sf0 else 2
+                    lo, hi = [slice(None)] * 3, [slice(None)] * 3
+                    lo[ax_e], hi[ax_e] = slice(1, None), slice(None, -1)
+                    ecell = (edata[tuple(lo)] + edata[tuple(hi)]) / 2
+                    lo, hi = [slice(None)] * 3, [slice(None)] * 3
+                    lo[ax_h], hi[ax_h] = slice(1, None), slice(None, -1)
+                    hcell = (hdata[tuple(lo)] + hdata[tuple(hi)]) / 2
+                    win = (slice(None), slice(ff_idx[sf0, 0], ff_idx[sf0, 1]), slice(ff_idx[sf1, 0], ff_idx[sf1, 1]))
+                    e_xyz[f] = ecell[win]
+                    h_xyz[f] = hcell[win]
+                if not present:
+                    continue
+                n = np.zeros(3)
+                n[axis] = -1 if j == 0 else 1
+                J = np.ascontiguousarray(np.cross(n, h_xyz, axis=0), dtype=np.complex64)
+                M = np.ascontiguousarray(np.cross(-n, e_xyz, axis=0), dtype=np.complex64)
+                faces.append(dict(axis=axis, position=float(self.farfield["surf_pos"][axis, j]), r0=r0, r1=r1, ds=ds,
+                                  J=J, M=M))
+        return faces
+
+    def get_farfield_rE(self, theta, phi):
+        """Far-field r E (exp(-j beta r)/r removed) for theta and phi in degrees: labelled array
+        [polarization (thetapol, phipol), frequency, theta, phi] (rfnetwork/solver.py:2870-3020).
+        The radiation integrals run on the GPU (rfdtd_nf2ff)."""
+        from . import _lib
+
+        self.check_solution()
+        if not len(self.farfield.keys()):
+            raise RuntimeError("Far-field monitor not found.")
+        theta = np.atleast_1d(theta)
+        phi = np.atleast_1d(phi)
+        frequency = self.farfield["frequency"]
+        faces = self._farfield_faces()
+        data = _lib.nf2ff(faces, np.asarray(2 * np.pi * frequency / const.c0, dtype=np.float32),
+                          np.asarray(np.deg2rad(theta), dtype=np.float32), np.asarray(np.deg2rad(phi), dtype=np.float32),
+                          float(const.eta0))
+        return LabeledArray(data, coords=dict(polarization=["thetapol", "phipol"], frequency=frequency, theta=theta,
+                                              phi=phi))
+
+    def get_farfield_gain(self, theta, phi, polarization=None):
+        """Realised gain [polarization, frequency, theta, phi] against the power the sources deliver into
+        50 ohm (rfnetwork/solver.py:2804-2868); polarization: thetapol, phipol, rhcp, lhcp."""
+        rE = np.asarray(self.get_farfield_rE(theta, phi))
+        if polarization is None:
+            polarization = ["thetapol", "phipol"]
+        polarization = list(np.atleast_1d(polarization))
+        vectors = dict(thetapol=[1, 0], phipol=[0, 1], rhcp=[1 / np.sqrt(2), 1j / np.sqrt(2)],
+                       lhcp=[1 / np.sqrt(2), -1j / np.sqrt(2)])
+        proj = np.array([vectors[k] for k in polarization])
+        rE_pol = np.einsum("ij,j...->i...", proj, rE)
+        frequency = self.farfield["frequency"]
+        sources = [p["src"] for p in self.ports if p is not None and p["src"] is not None]
+        Vs = np.array([utils.dtft(v, frequency, 1 / self.dt, downsample=False) for v in sources])
+        Pin = np.sum((1 / 2) * (np.abs(Vs) ** 2 / 50), axis=0)
+        U = (1 / (2 * const.eta0)) * np.abs(rE_pol) ** 2
+        gain = (4 * np.pi / Pin[None, :, None, None]) * U
+        return LabeledArray(gain, coords=dict(polarization=polarization, frequency=frequency,
+                                              theta=np.atleast_1d(theta), phi=np.atleast_1d(phi)))
 
     def add_current_probe(self, name: str, face):
         """Ampere-loop probes of H around a rectangular face; vi_probe_values(name) gives the current

@@ -32,6 +32,10 @@ design_variant  variant v of the batched forward+adjoint workload: grids from 96
                 y-, y+ sides) that end one cell short of the box; 10-cell PML except on the ground;
                 port 1 driven (4 GHz Gaussian-modulated sine); objective |S21|^2 at 2..6 GHz; 8000
                 steps.
+half_wave_dipole  rfnetwork's own 10 GHz dipole test case (tests/test_solver/test_dipole.py) on a
+                uniform grid of 0.0075 in cells: two 4-cell-wide PEC strips of 37 cells each side of
+                a 2-cell gap port, 5-cell PML on every face, far-field monitor at 10 and 20 GHz,
+                edge-singularity correction on the strip edges, 50 ps Gaussian pulse.
 """
 import numpy as np
 
@@ -251,4 +255,30 @@ def design_variant(v, eps_design=None, Nt=8000, cell_m=1e-4, courant=0.99, n_pml
     src = s.gaussian_modulated_source(f0=4e9, width=600e-12, t0=420e-12, t_len=(Nt + 0.5) * s.dt)
     s.assign_excitation(1e-2 * np.asarray(src), 1)
     s.frequency = np.array([2e9, 3e9, 4e9, 5e9, 6e9])
+    return s
+
+
+def half_wave_dipole(d=0.0075):
+    """The strip dipole of rfnetwork's dipole test on a uniform grid of d inches (the strip width, gap
+    and arm length snapped to it: 4, 2 and 37 cells)."""
+    nx, ny, nz = 200, 148, 148
+    xs = (np.arange(nx + 1) - nx // 2) * d
+    ys = (np.arange(ny + 1) - ny // 2) * d
+    zs = (np.arange(nz + 1) - nz // 2) * d
+    s = FDTD_Solver(Box((xs[0], ys[0], zs[0]), (xs[-1], ys[-1], zs[-1])))
+    s.set_grid(xs, ys, zs)
+    ms_w, gap, half = 4 * d, 2 * d, 37 * d
+    ms_y = (-ms_w / 2, ms_w / 2)
+    z1, z2 = (-half, -gap / 2), (gap / 2, half)
+    s.add_conductor(Rectangle([(0, ms_y[0], z1[0]), (0, ms_y[1], z1[0]), (0, ms_y[1], z1[1])]))
+    s.add_conductor(Rectangle([(0, ms_y[0], z2[0]), (0, ms_y[1], z2[0]), (0, ms_y[1], z2[1])]))
+    s.add_lumped_port(1, Rectangle([(0, ms_y[0], gap / 2), (0, ms_y[1], gap / 2), (0, ms_y[1], -gap / 2)]), "z-")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", "z-", n_pml=5)
+    s.generate_mesh()
+    s.add_farfield_monitor(frequency=[10e9, 20e9])
+    for i, mz in enumerate((z1, z2)):
+        s.edge_correction((0, ms_y[0], mz[0]), (0, ms_y[0], mz[1]), integration_line="y-")
+        s.edge_correction((0, ms_y[1], mz[0]), (0, ms_y[1], mz[1]), integration_line="y+")
+        s.edge_correction((0, ms_y[0], mz[i]), (0, ms_y[1], mz[i]), integration_line=("z-" if i == 0 else "z+"))
+    s.assign_excitation(s.gaussian_source(width=50e-12, t0=40e-12, t_len=600e-12), 1)
     return s

@@ -79,7 +79,7 @@ def test_launch_shapes_agree(hip_ready):
     print("[case] launch shapes", flush=True)
     ref = None
     for launch in [(256, 0, 100), (64, 1, 7), (1024, 5, 40), (128, 21, 1)]:
-        for variant in (1, 0):
+        for variant in (1, 0, 4):
             _, _, f, _ = _hip_session(model, launch=launch, variant=variant)
             if ref is None:
                 ref = f
