@@ -159,6 +159,11 @@ struct CoefRaw {
     }
     // coefficient k (0..29) alone
     __device__ __forceinline__ T at(uint32_t b, int k) const { return ldg((const GLOBAL_AS T*)cp[k], b); }
+    // two-stage access (key first, values later) for software pipelining: here the key is the offset
+    __device__ __forceinline__ uint32_t ekey(uint32_t b) const { return b; }
+    __device__ __forceinline__ uint32_t hkey(uint32_t b) const { return b; }
+    __device__ __forceinline__ ECoef<T> e_at(uint32_t key) const { return e(key); }
+    __device__ __forceinline__ HCoef<T> h_at(uint32_t key) const { return h(key); }
     __device__ __forceinline__ HCoef<T> h(uint32_t b) const {
         auto c = [&](int k) { return ldg((const GLOBAL_AS T*)cp[k], b); };
         HCoef<T> r;
@@ -186,9 +191,16 @@ struct CoefTable {
         const uint32_t e = ldg((const GLOBAL_AS uint32_t*)hidx, bc / S * 4u);
         return ldg((const GLOBAL_AS T*)htab, (e * 20u + (uint32_t)(k - 12)) * S);
     }
-    // bc: byte offset of the cell in a T array; the index arrays are uint32
-    __device__ __forceinline__ ECoef<T> e(uint32_t bc) const {
-        const uint32_t k = ldg((const GLOBAL_AS uint32_t*)eidx, bc / (uint32_t)sizeof(T) * 4u);
+    // two-stage access (entry index first, values later) for software pipelining
+    __device__ __forceinline__ uint32_t ekey(uint32_t bc) const {
+        return ldg((const GLOBAL_AS uint32_t*)eidx, bc / (uint32_t)sizeof(T) * 4u);
+    }
+    __device__ __forceinline__ uint32_t hkey(uint32_t bc) const {
+        return ldg((const GLOBAL_AS uint32_t*)hidx, bc / (uint32_t)sizeof(T) * 4u);
+    }
+    __device__ __forceinline__ ECoef<T> e(uint32_t bc) const { return e_at(ekey(bc)); }
+    __device__ __forceinline__ HCoef<T> h(uint32_t bc) const { return h_at(hkey(bc)); }
+    __device__ __forceinline__ ECoef<T> e_at(uint32_t k) const {
         const GLOBAL_AS Vec4<T>* tb = (const GLOBAL_AS Vec4<T>*)etab;
         const uint32_t o = k * (uint32_t)(3 * sizeof(Vec4<T>));
         const Vec4<T> a = ldg(tb, o), b = ldg(tb, o + (uint32_t)sizeof(Vec4<T>)),
@@ -199,8 +211,7 @@ struct CoefTable {
         r.ca_zx = c.v[0]; r.ca_zy = c.v[1]; r.cb_zx = c.v[2]; r.cb_zy = c.v[3];
         return r;
     }
-    __device__ __forceinline__ HCoef<T> h(uint32_t bc) const {
-        const uint32_t k = ldg((const GLOBAL_AS uint32_t*)hidx, bc / (uint32_t)sizeof(T) * 4u);
+    __device__ __forceinline__ HCoef<T> h_at(uint32_t k) const {
         const GLOBAL_AS Vec4<T>* tb = (const GLOBAL_AS Vec4<T>*)htab;
         const uint32_t o = k * (uint32_t)(5 * sizeof(Vec4<T>)), v = (uint32_t)sizeof(Vec4<T>);
         const Vec4<T> a = ldg(tb, o), b = ldg(tb, o + v), c = ldg(tb, o + 2u * v), d = ldg(tb, o + 3u * v),
@@ -546,8 +557,17 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
         hyx = ldg((const G*)A.hy_x, b); hzx = ldg((const G*)A.hz_x, b); hzy = ldg((const G*)A.hz_y, b);
     }
     T ey_xm = T(0), ez_xm = T(0);
+    // this plane's E splits and coefficient keys, loaded one plane ahead (software pipelining)
+    T exy = T(0), exz = T(0), eyz = T(0), eyx = T(0), ezx = T(0), ezy = T(0);
+    uint32_t ek = 0, hk = 0;
+    if (ecomp) {
+        exy = ldg((const G*)A.ex_y, b); exz = ldg((const G*)A.ex_z, b); eyz = ldg((const G*)A.ey_z, b);
+        eyx = ldg((const G*)A.ey_x, b); ezx = ldg((const G*)A.ez_x, b); ezy = ldg((const G*)A.ez_y, b);
+        ek = C.ekey(b);
+        if (inner) hk = C.hkey(b);
+    }
     for (int x = xs; x < x1; ++x, b += sx) {
-        const bool pre = x < x0, xin = x < Nx - 1;
+        const bool pre = x < x0, xin = x < Nx - 1, more = x + 1 < x1;
         // next plane's H splits (E needs its Hy, Hz; the H update of the next plane needs all six)
         T nxy = T(0), nxz = T(0), nyz = T(0), nyx = T(0), nzx = T(0), nzy = T(0);
         if (valid && xin) {
@@ -555,17 +575,22 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
             nyz = ldg((const G*)A.hy_z, b + sx); nyx = ldg((const G*)A.hy_x, b + sx);
             nzx = ldg((const G*)A.hz_x, b + sx); nzy = ldg((const G*)A.hz_y, b + sx);
         }
-        // this plane's E splits and both coefficient sets, issued before the barriers so that they
-        // arrive while the tile exchanges neighbours
-        T exy = T(0), exz = T(0), eyz = T(0), eyx = T(0), ezx = T(0), ezy = T(0);
+        // this plane's coefficient entries, issued before the barriers so that they arrive while the
+        // tile exchanges neighbours
         ECoef<T> c{};
         HCoef<T> ch{};
-        if (ecomp) {
-            exy = ldg((const G*)A.ex_y, b); exz = ldg((const G*)A.ex_z, b); eyz = ldg((const G*)A.ey_z, b);
-            eyx = ldg((const G*)A.ey_x, b); ezx = ldg((const G*)A.ez_x, b); ezy = ldg((const G*)A.ez_y, b);
-            c = C.e(b);
+        if (ecomp) c = C.e_at(ek);
+        if (inner && !pre) ch = C.h_at(hk);
+        // and the next plane's E splits and keys
+        T pxy = T(0), pxz = T(0), pyz = T(0), pyx = T(0), pzx = T(0), pzy = T(0);
+        uint32_t pek = 0, phk = 0;
+        if (ecomp && more) {
+            pxy = ldg((const G*)A.ex_y, b + sx); pxz = ldg((const G*)A.ex_z, b + sx);
+            pyz = ldg((const G*)A.ey_z, b + sx); pyx = ldg((const G*)A.ey_x, b + sx);
+            pzx = ldg((const G*)A.ez_x, b + sx); pzy = ldg((const G*)A.ez_y, b + sx);
+            pek = C.ekey(b + sx);
+            if (inner) phk = C.hkey(b + sx);
         }
-        if (inner && !pre) ch = C.h(b);
         const T hx_c = hxy + hxz, hy_c = hyz + hyx, hz_c = hzx + hzy;
         sHx[tid] = hx_c;
         sHy[tid] = hy_c;
@@ -615,6 +640,9 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
         ey_xm = ey;
         ez_xm = ez;
         hxy = nxy; hxz = nxz; hyz = nyz; hyx = nyx; hzx = nzx; hzy = nzy;
+        exy = pxy; exz = pxz; eyz = pyz; eyx = pyx; ezx = pzx; ezy = pzy;
+        ek = pek;
+        hk = phk;
     }
 }
 
