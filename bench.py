@@ -11,11 +11,12 @@ the E update is 120 B, a step 264 B), its profiler-measured HBM bytes, a copy-ke
 the CPU oracle (a restatement of the reference's threaded C++ solver) timed on this host over a
 bounded window of the same workload.
 
-Further legs in the same line: "batched" -- 4 uneven multi-port grids x every port driven (12
+Further legs in the same line: "c1" -- the 120x48x32 microstrip (4000 steps) on the GPU and, in
+full, on the CPU oracle; "batched" -- 4 uneven multi-port grids x every port driven (12
 excitations, 5000 steps) in one batched session vs the same excitations one after another;
 "adjoint" -- gradients/s of dJ/deps_r over a 64x64x32 design box of a 200x160x80 model (6000 steps,
-checkpoints every 200).  "adjoint_fd" (on request) validates that gradient against central
-differences on 32 cells.
+checkpoints every 200).  On request: "c5" (8 design variants up to 512x512x256, batched forward
+and adjoint) and "adjoint_fd" (that gradient against central differences on 32 cells).
 """
 import argparse
 import json
@@ -96,6 +97,43 @@ def cpu_baseline(model, budget_s):
                 sample=f"{steps} steps of the full {model.Nx}x{model.Ny}x{model.Nz} workload from zero fields "
                        f"(oracle restating rfnetwork/core/src/solver.cpp, {threads} threads, x slabs, 2 barriers "
                        f"per step, -O2 -ffp-contract=off), {dt:.1f} s wall")
+
+
+def c1_leg(flags=0):
+    """The 120x48x32 microstrip (4000 steps, 2-port S at 101 frequencies): a launch-bound grid that
+    fits the Infinity Cache.  GPU: resident session, tuned, all 4000 steps timed; CPU: the oracle
+    over the same full run on this host's cores (the reference's CPU path restated)."""
+    from oracle import cpu_solver
+    from rfnetwork_amd import workloads
+    from rfnetwork_amd.session import Session
+
+    model = workloads.microstrip()
+    c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    cells = Nx * Ny * Nz
+    with Session(c, Nx, Ny, Nz, flags=flags) as s:
+        tuned = s.autotune()
+        s.set_sources(probes, monitors, Nt)
+        t0 = time.perf_counter()
+        s.run(Nt)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        s.read()
+    model.unpack(probes, monitors, gpu=True)
+    S_gpu = np.asarray(model.get_sparameters(model.frequency))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    m2 = workloads.microstrip()
+    c2, p2, mo2, mem2, *_ = m2.pack(gpu=False, n_threads=threads)
+    t0 = time.perf_counter()
+    cpu_solver.solver_run(c2, p2, mo2, mem2, Nx, Ny, Nz, Nt, threads, 0)
+    wall_cpu = time.perf_counter() - t0
+    m2.unpack(p2, mo2, gpu=False)
+    S_cpu = np.asarray(m2.get_sparameters(m2.frequency))
+    log(f"c1: GPU {wall * 1e3:.1f} ms, CPU oracle {wall_cpu:.2f} s on {threads} threads, "
+        f"max |dS| {np.abs(S_gpu - S_cpu).max():.2e}")
+    return dict(workload=f"microstrip {Nx}x{Ny}x{Nz}, {Nt} steps, 101 frequencies", gpu_ms=round(wall * 1e3, 2),
+                gpu_mcells=round(cells * Nt / wall / 1e6, 1), launch=tuned,
+                cpu_oracle_s=round(wall_cpu, 3), cpu_mcells=round(cells * Nt / wall_cpu / 1e6, 1), cpu_threads=threads,
+                speedup=round(wall_cpu / wall, 1), s_params_identical=bool(np.array_equal(S_gpu, S_cpu)))
 
 
 def batched_leg(n_steps, flags=0):
@@ -327,8 +365,9 @@ def main():
     ap.add_argument("--xplanes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
     ap.add_argument("--no-copy-probe", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,adjoint",
-                    help="comma list: plain (the headline), batched (12 excitations), adjoint (gradients/s), "
+    ap.add_argument("--legs", default="plain,c1,batched,adjoint",
+                    help="comma list: plain (the headline), c1 (the 120x48x32 microstrip, GPU and CPU oracle), "
+                         "batched (12 excitations), adjoint (gradients/s), "
                          "c5 (8 design variants, batched forward+adjoint, minutes), "
                          "adjoint_fd (finite-difference validation, minutes)")
     ap.add_argument("--batch-steps", type=int, default=5000)
@@ -352,6 +391,8 @@ def main():
     if "plain" in legs:
         out.update(plain_leg(args))
     gflags = _lib.NO_GRAPHS if args.no_graphs else 0
+    if "c1" in legs:
+        out["c1"] = c1_leg(gflags)
     if "batched" in legs:
         out["batched"] = batched_leg(args.batch_steps, gflags)
     if "adjoint" in legs:

@@ -1834,6 +1834,24 @@ void run_steps(Session& s, int n_steps, hipStream_t st) {
     s.step += n_steps;
 }
 
+// The separate kernels use one field buffer per instance: after a switch away from the fused pass,
+// bring the current fields into the first buffer and free the second.
+void release_second_buffers(Session& s) {
+    bool any = false;
+    for (const Instance& I : s.insts) any |= I.d_fields2 != nullptr;
+    if (!any) return;
+    HIP_OK(hipDeviceSynchronize());
+    for (Instance& I : s.insts) {
+        if (!I.d_fields2) continue;
+        if (s.parity) std::swap(I.d_fields, I.d_fields2);
+        (void)hipFree(I.d_fields2);
+        I.d_fields2 = nullptr;
+    }
+    s.parity = 0;
+    s.dirty = true;
+    s.graph_dirty = true;
+}
+
 // Launch autotuner: times every (variant, block size, x planes per workgroup) candidate over
 // trial_steps field-only steps of this session's own grids and keeps the fastest.  The session must
 // not have run yet (step 0); its fields are zero before and after.
@@ -1902,6 +1920,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     }
     s.parity = 0;
     s.totals_stale = false;
+    if (!s.fused) release_second_buffers(s);
     cfg[0] = bv;
     cfg[1] = bblk;
     cfg[2] = bxp;
@@ -2575,6 +2594,7 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
         S.nontemporal = variant == 4;
         S.dirty = true;
         S.graph_dirty = true;
+        if (!S.fused) release_second_buffers(S);
     });
 }
 
