@@ -239,6 +239,34 @@ def batched_leg(n_steps, flags=0):
     ref = [S[..., c] for S, _ in res for c in range(S.shape[-1])]
     same = all(np.array_equal(a, b) for a, b in zip(ref, cols))
     log(f"end to end: solve_ports {e2e_b:.3f}s, sequential solve() {e2e_s:.3f}s, identical S {same}")
+    # where batching pays: 12 excitations (ports 1 and 2, alternately) of the 120x48x32 microstrip,
+    # a grid small enough that a single run is bound by launch latency, not bandwidth
+    sm = workloads.microstrip(Nt=4000)
+    sexc, sNt = rb.pack_excitations(sm, [1, 2] * 6)
+    scoef = sm.coefficient_arrays(gpu=True)
+    with rb.Batch([(scoef, sm.Nx, sm.Ny, sm.Nz, len(sexc))], flags=flags) as sb:
+        sb.autotune()
+        for k, (_, probes, monitors) in enumerate(sexc):
+            sb.set_sources(k, probes, monitors, sNt)
+        t0 = time.perf_counter()
+        sb.run(sNt)
+        sb.synchronize()
+        small_b = time.perf_counter() - t0
+    singles = []
+    for port in [1, 2] * 6:
+        (_, probes, monitors), = rb.pack_excitations(sm, [port])[0]
+        ss = Session(scoef, sm.Nx, sm.Ny, sm.Nz, flags=flags)
+        ss.autotune()
+        ss.set_sources(probes, monitors, sNt)
+        singles.append(ss)
+    t0 = time.perf_counter()
+    for ss in singles:
+        ss.run(sNt)
+        ss.synchronize()
+    small_s = time.perf_counter() - t0
+    for ss in singles:
+        ss.close()
+    log(f"small grids: batched {small_b * 1e3:.1f} ms, sequential {small_s * 1e3:.1f} ms")
     b_rate = exc_cells * n_steps / wall_b / 1e6
     s_rate = exc_cells * n_steps / wall_s / 1e6
     return dict(workload="4 port variants 64^3..192x128x96, 12 excitations", steps=n_steps,
@@ -247,7 +275,10 @@ def batched_leg(n_steps, flags=0):
                 sequential_plain_formulation_s=round(wall_p, 3), speedup_vs_plain_formulation=round(wall_p / wall_b, 3),
                 identical=bool(identical),
                 end_to_end=dict(solve_ports_s=round(e2e_b, 3), sequential_solve_s=round(e2e_s, 3),
-                                speedup=round(e2e_s / e2e_b, 3), identical=bool(same)))
+                                speedup=round(e2e_s / e2e_b, 3), identical=bool(same)),
+                small_grids=dict(workload="12 excitations of the 120x48x32 microstrip, 4000 steps",
+                                 batched_ms=round(small_b * 1e3, 2), sequential_ms=round(small_s * 1e3, 2),
+                                 speedup=round(small_s / small_b, 3)))
 
 
 def adjoint_leg(n_steps, every, repeats=2, flags=0):
