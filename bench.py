@@ -283,28 +283,39 @@ def batched_leg(n_steps, flags=0):
 
 def adjoint_leg(n_steps, every, repeats=2, flags=0):
     """dJ/deps_r of the 64x64x32 design box of the 200x160x80 two-port model, J = sum |S21|^2 at
-    2..6 GHz: one gradient = checkpointed forward + reverse pass with recomputation; the best of
-    `repeats` end-to-end evaluations (model assembly excluded, host objective included)."""
+    2..6 GHz: one gradient = forward run recording the design box's H every step + reverse pass (no
+    recomputation); the checkpointed form (checkpoints every `every` steps, segments recomputed) is
+    timed beside it.  Best of `repeats` end-to-end evaluations each (model assembly excluded, host
+    objective included); both gradients must agree bit for bit."""
     from rfnetwork_amd import adjoint, workloads
 
     model = workloads.design_strip(Nt=n_steps)
     cbox = workloads.design_box()
-    best = None
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        J, grad, S, t = adjoint.eps_r_gradient(model, cbox, model.frequency, checkpoint_every=every,
-                                               return_session_timing=True, flags=flags)
-        wall = time.perf_counter() - t0
-        if best is None or wall < best[0]:
-            best = (wall, t, J, grad)
-    wall, t, J, grad = best
+    res = {}
+    for mode in ("history", "checkpoint"):
+        best = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            J, grad, S, t = adjoint.eps_r_gradient(model, cbox, model.frequency, checkpoint_every=every,
+                                                   return_session_timing=True, flags=flags, mode=mode)
+            wall = time.perf_counter() - t0
+            if best is None or wall < best[0]:
+                best = (wall, t, J, grad)
+        res[mode] = best
+        log(f"adjoint ({mode}): {best[0]:.3f}s per gradient (forward {best[1]['forward_ms']:.1f} ms, "
+            f"reverse {best[1]['adjoint_ms']:.1f} ms)")
+    wall, t, J, grad = res["history"]
+    wc, tc, Jc, gc = res["checkpoint"]
     cells = model.Nx * model.Ny * model.Nz
-    log(f"adjoint: {wall:.3f}s per gradient (forward {t['forward_ms']:.1f} ms, reverse {t['adjoint_ms']:.1f} ms)")
-    return dict(workload=f"design_strip {model.Nx}x{model.Ny}x{model.Nz}, design box 64x64x32, {n_steps} steps, "
-                         f"checkpoints every {every}", gradients_per_s=round(1.0 / wall, 4),
+    return dict(workload=f"design_strip {model.Nx}x{model.Ny}x{model.Nz}, design box 64x64x32, "
+                         f"{n_steps} steps, design-box H history", gradients_per_s=round(1.0 / wall, 4),
                 s_per_gradient=round(wall, 3), forward_ms=round(t["forward_ms"], 1),
                 reverse_ms=round(t["adjoint_ms"], 1), design_cells=int(grad.size), J=J,
+                history_bytes=int(adjoint.history_bytes([adjoint.gpu_box_for_cells(model, cbox)], n_steps)),
                 forward_mcells=round(cells * n_steps / (t["forward_ms"] * 1e-3) / 1e6, 1),
+                checkpointed=dict(every=every, gradients_per_s=round(1.0 / wc, 4), forward_ms=round(tc["forward_ms"], 1),
+                                  reverse_ms=round(tc["adjoint_ms"], 1)),
+                modes_equal=bool(J == Jc and np.array_equal(grad, gc)),
                 finite=bool(np.isfinite(grad).all()))
 
 
@@ -330,7 +341,7 @@ def c5_leg(n_steps, flags=0):
                 cells=cells, wall_s=round(wall, 2), gradients_per_s=round(len(jobs) / wall, 4),
                 forward_ms=round(info["forward_ms"], 1), reverse_ms=round(info["adjoint_ms"], 1),
                 forward_mcells=round(cells * n_steps / (info["forward_ms"] * 1e-3) / 1e6, 1),
-                checkpoint_every=info["checkpoint_every"], planned_bytes=info.get("planned_bytes"),
+                mode=info["mode"], checkpoint_every=info.get("checkpoint_every"), planned_bytes=info.get("planned_bytes"),
                 launch=info.get("launch"), J=[r[0] for r in res],
                 finite=bool(all(np.isfinite(r[1]).all() for r in res)))
 

@@ -157,8 +157,9 @@ int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3],
  * Adjoint gradients (no counterpart in the reference, which has no differentiation).
  * rfdtd_set_checkpointing(s, K): subsequent rfdtd_run calls save the 12 split fields of every
  * instance before each step that is a multiple of K (one device copy per checkpoint).
- * rfdtd_gradient: after a checkpointed forward run of Nt steps, runs the discrete adjoint of those
- * steps backwards, recomputing each K-step segment from its checkpoint, and returns dJ/dCb for the
+ * rfdtd_gradient: after a checkpointed forward run of Nt steps (or one that recorded the box's
+ * history, below), runs the discrete adjoint of those steps backwards, recomputing each K-step
+ * segment from its checkpoint when there is no history, and returns dJ/dCb for the
  * six E splits of every cell of box = {x0, x1, y0, y1, z0, z1} (grid cells, upper bounds
  * exclusive) as double[6][x1-x0][y1-y0][z1-z0] in split order ex_y ex_z ey_z ey_x ez_x ez_y, where
  * Cb is the packed coefficient (reference dictionary entries 2, 3, 6, 7, 10, 11).
@@ -166,6 +167,13 @@ int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3],
  * as its probes).  The forward fields are left untouched.
  */
 int rfdtd_set_checkpointing(rfdtd_session* s, int32_t every);
+/* Design-box history (no counterpart in the reference): record the H totals of box k of instance k
+ * (boxes[6k .. 6k+6), as for rfdtd_gradient) at every step of later runs, for up to n_steps steps;
+ * a gradient over exactly these boxes then needs neither checkpoints nor recomputation.  Only before
+ * the first step (or after rfdtd_reset); n = 0 drops the history.  RFDTD_ENOMEM if it does not fit. */
+int rfdtd_set_history(rfdtd_session* s, int32_t n, const int32_t* instances, const int32_t* boxes, int32_t n_steps);
+/* Free and total device memory in bytes (no counterpart in the reference). */
+int rfdtd_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
 int rfdtd_gradient(rfdtd_session* s, int32_t instance, const int32_t box[6], const void* const* adjoint_series,
                    int32_t n_series, double* grad_out);
 /* The same for n instances of a batched session at once (their reverse passes share every launch):

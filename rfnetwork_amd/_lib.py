@@ -50,7 +50,7 @@ EXPORTS = (
     "rfdtd_set_launch", "rfdtd_destroy", "rfdtd_copy_bandwidth", "rfdtd_create_batch",
     "rfdtd_instance_count", "rfdtd_set_instance_sources", "rfdtd_read_instance_fields", "rfdtd_set_variant",
     "rfdtd_set_checkpointing", "rfdtd_gradient", "rfdtd_autotune", "rfdtd_gradient_batch",
-    "rfdtd_plan_checkpoints", "rfdtd_nf2ff",
+    "rfdtd_plan_checkpoints", "rfdtd_nf2ff", "rfdtd_set_history", "rfdtd_device_memory",
 )
 
 
@@ -82,6 +82,13 @@ class RfdtdError(RuntimeError):
 
 
 _lib = None
+
+
+def device_memory():
+    """(free, total) device memory in bytes."""
+    f, t = ctypes.c_uint64(), ctypes.c_uint64()
+    check(lib().rfdtd_device_memory(ctypes.byref(f), ctypes.byref(t)), "rfdtd_device_memory")
+    return f.value, t.value
 
 
 def version() -> int:
@@ -131,6 +138,8 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_gradient.argtypes = [vp, i32, vp, vp, i32, vp]
     L.rfdtd_autotune.argtypes = [vp, i32, vp, ctypes.POINTER(ctypes.c_double)]
     L.rfdtd_gradient_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.rfdtd_set_history.argtypes = [vp, i32, vp, vp, i32]
+    L.rfdtd_device_memory.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
     L.rfdtd_nf2ff.argtypes = [i32, vp, i32, vp, i32, vp, i32, vp, ctypes.c_float, vp]
     L.rfdtd_plan_checkpoints.argtypes = [vp, i32, i32, vp, vp, ctypes.c_uint64, ctypes.POINTER(i32),
                                          ctypes.POINTER(ctypes.c_uint64)]

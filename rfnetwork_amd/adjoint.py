@@ -172,17 +172,40 @@ def gpu_box_for_cells(model, cbox):
     return (max(0, x0 - 1), min(Nx, x1), max(0, y0 - 1), min(Ny, y1), max(0, z0 - 1), min(Nz, z1))
 
 
-def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=200, dtype=np.float32,
-                   flags=0, return_session_timing=False, tune=True):
-    """Forward solve of the model on the GPU with checkpoints, then the adjoint pass.
+def history_bytes(boxes, Nt, dtype=np.float32):
+    """Device bytes of the design-box H history of boxes (grid-cell boxes) over Nt steps."""
+    esz = np.dtype(dtype).itemsize
+    return sum(3 * (b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1) * esz * Nt for b in boxes)
 
+
+def _use_history(mode, boxes, Nt, dtype):
+    if mode not in ("auto", "history", "checkpoint"):
+        raise ValueError("mode must be auto, history or checkpoint")
+    if mode != "auto":
+        return mode == "history"
+    from ._lib import device_memory
+
+    return history_bytes(boxes, Nt, dtype) < 0.5 * device_memory()[0]
+
+
+def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=200, dtype=np.float32,
+                   flags=0, return_session_timing=False, tune=True, mode="auto"):
+    """Forward solve of the model on the GPU, then the adjoint pass.
+
+    mode "history" records the design box's H at every forward step (no recomputation in the reverse
+    pass), "checkpoint" saves the fields every checkpoint_every steps and recomputes between them,
+    "auto" records the history when it takes less than half the free device memory.
     Returns (J, dJ/deps_r over cbox, S_ba).  The model must be meshed and excited."""
     coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype)
     gbox = gpu_box_for_cells(model, cbox)
+    history = _use_history(mode, [gbox], Nt, dtype)
     with Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags) as s:
         if tune:
             s.autotune()
-        s.set_checkpointing(checkpoint_every)
+        if history:
+            s.set_history(gbox, Nt)
+        else:
+            s.set_checkpointing(checkpoint_every)
         s.set_sources(probes, [], Nt)
         s.run(Nt)
         s.synchronize()
@@ -194,12 +217,12 @@ def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=
         t_adj = s.timing()[0]
     grad = cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox)
     if return_session_timing:
-        return J, grad, S, dict(forward_ms=t_fwd, adjoint_ms=t_adj)
+        return J, grad, S, dict(forward_ms=t_fwd, adjoint_ms=t_adj, mode="history" if history else "checkpoint")
     return J, grad, S
 
 
 def eps_r_gradients_batch(jobs, b_port=2, a_port=1, checkpoint_every=0, dtype=np.float32, flags=0, tune=True,
-                          budget_bytes=0, return_info=False):
+                          budget_bytes=0, return_info=False, mode="auto"):
     """Gradients of several models at once: one batched session holds every (meshed, excited)
     model; their forward runs and their reverse passes share every launch.
 
@@ -218,15 +241,20 @@ def eps_r_gradients_batch(jobs, b_port=2, a_port=1, checkpoint_every=0, dtype=np
         grids.append((coeff, Nx, Ny, Nz, 1))
         packed.append((coeff, probes, gpu_box_for_cells(model, cbox)))
     info = {}
+    history = _use_history(mode, [p[2] for p in packed], Nt_all, dtype)
+    info["mode"] = "history" if history else "checkpoint"
     with Batch(grids, dtype=dtype, flags=flags) as b:
         if tune:
             info["launch"] = b.autotune()
-        every = int(checkpoint_every)
-        if every <= 0:
-            every, info["planned_bytes"] = b.plan_checkpoints(Nt_all, [(k, p[2]) for k, p in enumerate(packed)],
-                                                              budget_bytes)
-        info["checkpoint_every"] = every
-        b.set_checkpointing(every)
+        if history:
+            b.set_history([(k, p[2]) for k, p in enumerate(packed)], Nt_all)
+        else:
+            every = int(checkpoint_every)
+            if every <= 0:
+                every, info["planned_bytes"] = b.plan_checkpoints(Nt_all, [(k, p[2]) for k, p in enumerate(packed)],
+                                                                  budget_bytes)
+            info["checkpoint_every"] = every
+            b.set_checkpointing(every)
         for k, (_, probes, _) in enumerate(packed):
             b.set_sources(k, probes, [], Nt_all)
         b.run(Nt_all)

@@ -106,6 +106,15 @@ class Batch:
         """Save the split fields of every instance every `every` steps of later runs."""
         check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
 
+    def set_history(self, requests, Nt):
+        """Record the H totals of each request's box, requests [(instance, box), ...], at every step of
+        later runs (up to Nt steps), for gradients without recomputation.  Before the first step."""
+        n = len(requests)
+        inst = (ctypes.c_int32 * max(n, 1))(*[int(r[0]) for r in requests])
+        boxes = (ctypes.c_int32 * max(6 * n, 1))(*[int(v) for r in requests for v in r[1]])
+        check(self._L.rfdtd_set_history(self._h, n, ctypes.cast(inst, ctypes.c_void_p), ctypes.cast(boxes, ctypes.c_void_p),
+                                        int(Nt)), "rfdtd_set_history")
+
     def plan_checkpoints(self, Nt, requests, budget_bytes=0):
         """Checkpoint interval for gradients of requests [(instance, box), ...] after Nt steps that
         fits device memory (budget_bytes, 0: what is free).  Returns (every, bytes needed)."""

@@ -701,6 +701,19 @@ enum {  // coefficient positions
     C_DB_YX2, C_DA_ZX, C_DA_ZY, C_DB_ZX1, C_DB_ZX2, C_DB_ZY1, C_DB_ZY2
 };
 
+// The total of a component at one cell, formed from its splits as the update kernels form it.
+template <typename T>
+__device__ __forceinline__ T total_at(const InstDev<T>& I, int field, int64_t i) {
+    switch (field) {
+        case EX: return I.F.ex_y[i] + I.F.ex_z[i];
+        case EY: return I.F.ey_z[i] + I.F.ey_x[i];
+        case EZ: return I.F.ez_x[i] + I.F.ez_y[i];
+        case HX: return I.F.hx_y[i] + I.F.hx_z[i];
+        case HY: return I.F.hy_z[i] + I.F.hy_x[i];
+        default: return I.F.hz_x[i] + I.F.hz_y[i];
+    }
+}
+
 // Batched flat launches: item i (an instance, a design box) owns blocks [blk0[i], blk0[i+1]).
 __device__ __forceinline__ int item_of(const int64_t* __restrict__ blk0, int n, int64_t b) {
     int i = 0;
@@ -897,7 +910,8 @@ struct BoxDev {
 // Save the forward H totals of every box (plus one cell on each upper side) into slot `j`.
 template <typename T>
 __global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
-                                                const int64_t* __restrict__ blk0, int n, int j) {
+                                                const int64_t* __restrict__ blk0, int n, int j, const int* base = nullptr) {
+    if (base) j += *base;  // history recording inside a captured chunk: slot = step number
     const int k = item_of(blk0, n, blockIdx.x);
     const BoxDev<T> D = boxes[k];
     const InstDev<T>& I = insts[D.inst];
@@ -909,10 +923,10 @@ __global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ i
     const int x = D.B.x0 + lx, y = D.B.y0 + ly, z = D.B.z0 + lz;
     T a = T(0), b = T(0), c = T(0);
     if (x < I.Nx && y < I.Ny && z < I.Nz) {
-        const size_t i = ((size_t)x * I.Ny + y) * I.Nz + z;
-        a = I.F.hx[i];
-        b = I.F.hy[i];
-        c = I.F.hz[i];
+        const int64_t i = ((int64_t)x * I.Ny + y) * I.Nz + z;
+        a = total_at(I, HX, i);
+        b = total_at(I, HY, i);
+        c = total_at(I, HZ, i);
     }
     T* slot = D.slots + (size_t)j * D.slot_elems;
     slot[q] = a;
@@ -972,18 +986,6 @@ struct MonitorDev {
 constexpr int kStageSlots = 10;
 constexpr int kFreqGroup = 4;  // frequencies per thread in the flush
 
-// The total of a component at one cell, formed from its splits as the update kernels form it.
-template <typename T>
-__device__ __forceinline__ T total_at(const InstDev<T>& I, int field, int64_t i) {
-    switch (field) {
-        case EX: return I.F.ex_y[i] + I.F.ex_z[i];
-        case EY: return I.F.ey_z[i] + I.F.ey_x[i];
-        case EZ: return I.F.ez_x[i] + I.F.ez_y[i];
-        case HX: return I.F.hx_y[i] + I.F.hx_z[i];
-        case HY: return I.F.hy_z[i] + I.F.hy_x[i];
-        default: return I.F.hz_x[i] + I.F.hz_y[i];
-    }
-}
 
 // The monitor whose block range holds this workgroup (blk0: sampling launch, fblk0: flush launch).
 __device__ __forceinline__ int monitor_of(const MonitorDev* __restrict__ mons, int nmon, int* sm, bool flush = false) {
@@ -1259,6 +1261,18 @@ struct Session {
     int parity = 0;         // which buffer of each instance's pair holds the current fields
     bool totals_stale = false;  // the fused step leaves the total arrays unwritten
     void* d_insts2 = nullptr;   // descriptors with the buffers of each pair exchanged
+    // design-box H history: recorded every step of the forward run so that a gradient needs no
+    // recomputation (per item: instance, box, buffer of capacity steps x slot_elems values)
+    struct History {
+        int inst;
+        int32_t box[6];
+        void* d;
+        int64_t slot_elems, capacity;
+    };
+    std::vector<History> hist;
+    void* d_hist_boxes = nullptr;
+    int64_t* d_hist_h0 = nullptr;
+    int hist_blocks = 0;
     std::vector<int64_t> flat0; // block offsets of a flat launch over all instances' cells
     int64_t* d_flat0 = nullptr;
 
@@ -1327,8 +1341,10 @@ Session::~Session() {
     }
     for (auto& M : members)
         if (M.d_coef) (void)hipFree(M.d_coef);
+    for (auto& h : hist)
+        if (h.d) (void)hipFree(h.d);
     for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base, d_stage, (void*)d_rows,
-                    d_insts2, (void*)d_flat0})
+                    d_insts2, (void*)d_flat0, d_hist_boxes, (void*)d_hist_h0})
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1592,6 +1608,30 @@ void refresh(Session& s) {
     s.flat0 = block_offsets(cells);
     HIP_OK(hipMalloc(&s.d_flat0, s.flat0.size() * sizeof(int64_t)));
     HIP_OK(hipMemcpy(s.d_flat0, s.flat0.data(), s.flat0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    for (void** p : {&s.d_hist_boxes, (void**)&s.d_hist_h0}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    s.hist_blocks = 0;
+    if (!s.hist.empty()) {
+        std::vector<BoxDev<T>> hb(s.hist.size());
+        std::vector<int64_t> hc;
+        for (size_t k = 0; k < s.hist.size(); ++k) {
+            const auto& h = s.hist[k];
+            hb[k].B = Box{h.box[0], h.box[2], h.box[4], h.box[1] - h.box[0], h.box[3] - h.box[2], h.box[5] - h.box[4]};
+            hb[k].inst = h.inst;
+            hb[k].slots = (T*)h.d;
+            hb[k].grad = nullptr;
+            hb[k].slot_elems = h.slot_elems;
+            hc.push_back(h.slot_elems / 3);
+        }
+        const std::vector<int64_t> h0 = block_offsets(hc);
+        s.hist_blocks = (int)h0.back();
+        HIP_OK(hipMalloc(&s.d_hist_boxes, hb.size() * sizeof(BoxDev<T>)));
+        HIP_OK(hipMemcpy(s.d_hist_boxes, hb.data(), hb.size() * sizeof(BoxDev<T>), hipMemcpyHostToDevice));
+        HIP_OK(hipMalloc(&s.d_hist_h0, h0.size() * sizeof(int64_t)));
+        HIP_OK(hipMemcpy(s.d_hist_h0, h0.data(), h0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
 
     // probe groups of all instances: E groups, then H groups
     std::vector<ProbeGroup<T>> ge, gh;
@@ -1677,6 +1717,9 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     const int nmem = (int)s.members.size();
     const ProbeGroup<T>* groups = (const ProbeGroup<T>*)s.d_groups;
     const bool io = !s.tuning;
+    if (io && s.hist_blocks)  // H^n of the design boxes, before this step's E half step
+        k_save_h<T><<<s.hist_blocks, 256, 0, st>>>(di, (const BoxDev<T>*)s.d_hist_boxes, s.d_hist_h0, (int)s.hist.size(),
+                                                  local, base);
     if (s.fused) {
         // one pass writes the other buffer of each pair; H sources and monitors then work on it
         if (ev) HIP_OK(hipEventRecord(ev[0], st));
@@ -1804,6 +1847,9 @@ void run_steps(Session& s, int n_steps, hipStream_t st) {
         RF_THROW(RFDTD_ESTATE, "run past the end of the probe series (Nt = " + std::to_string(s.Nt_probes) + ")");
     if (has_mons && s.step + n_steps > s.Nt_monitors)
         RF_THROW(RFDTD_ESTATE, "run past the monitor length (Nt = " + std::to_string(s.Nt_monitors) + ")");
+    for (const auto& h : s.hist)
+        if (s.step + n_steps > h.capacity)
+            RF_THROW(RFDTD_ESTATE, "run past the design-box history (" + std::to_string(h.capacity) + " steps)");
     refresh<T>(s);
     HIP_OK(hipEventRecord(s.ev0, st));
     const bool timing = (s.flags & RFDTD_KERNEL_TIMING) != 0;
@@ -2191,19 +2237,33 @@ struct GradItem {
 // order; out: double[6][nx][ny][nz] (ex_y ex_z ey_z ey_x ez_x ez_y), box cells in grid coordinates.
 template <typename T>
 void gradients(Session& s, const std::vector<GradItem>& items) {
-    if (s.ckpt_every <= 0) RF_THROW(RFDTD_ESTATE, "enable checkpointing before the forward run");
     const int64_t Nt = s.step;
     if (Nt < 1) RF_THROW(RFDTD_ESTATE, "run the forward solve first");
-    const int K = s.ckpt_every;
-    const int nseg = (int)((Nt + K - 1) / K);
     const int n = (int)items.size();
     if (n < 1) RF_THROW(RFDTD_EINVAL, "no gradient requested");
+    // history mode: the forward run recorded every item's box H at every step, so the reverse pass
+    // needs no checkpoints and no recomputation
+    std::vector<const Session::History*> hs(n, nullptr);
+    bool hist_mode = true;
+    for (int k = 0; k < n; ++k) {
+        const GradItem& it = items[k];
+        for (const auto& h : s.hist)
+            if (h.inst == it.inst && h.capacity >= Nt && h.box[0] == it.B.x0 && h.box[2] == it.B.y0 &&
+                h.box[4] == it.B.z0 && h.box[1] - h.box[0] == it.B.nx && h.box[3] - h.box[2] == it.B.ny &&
+                h.box[5] - h.box[4] == it.B.nz)
+                hs[k] = &h;
+        hist_mode &= hs[k] != nullptr;
+    }
+    if (!hist_mode && s.ckpt_every <= 0)
+        RF_THROW(RFDTD_ESTATE, "record the design-box history or enable checkpointing before the forward run");
+    const int K = hist_mode ? (int)std::min<int64_t>(Nt, INT32_MAX) : s.ckpt_every;
+    const int nseg = hist_mode ? 1 : (int)((Nt + K - 1) / K);
     std::vector<int> seen(s.insts.size(), 0);
     for (const GradItem& it : items) {
         Instance& I = instance_at(s, it.inst);
         const Member& M = s.members[I.member];
         if (seen[it.inst]++) RF_THROW(RFDTD_EINVAL, "an instance may appear once per gradient call");
-        if ((int)I.ckpt.size() < nseg) RF_THROW(RFDTD_ESTATE, "missing checkpoints of the forward run");
+        if (!hist_mode && (int)I.ckpt.size() < nseg) RF_THROW(RFDTD_ESTATE, "missing checkpoints of the forward run");
         if (it.n_adj != I.P || (I.P && !it.adj)) RF_THROW(RFDTD_EINVAL, "one adjoint series per probe is required");
         const Box& B = it.B;
         if (B.nx < 1 || B.ny < 1 || B.nz < 1 || B.x0 < 0 || B.y0 < 0 || B.z0 < 0 || B.x0 + B.nx > M.Nx ||
@@ -2229,8 +2289,8 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         const GradItem& it = items[k];
         Instance& I = s.insts[it.inst];
         const Member& M = s.members[I.member];
-        d_re[k] = buf.alloc(18 * I.pitch);
         void* d_adj = buf.alloc(18 * I.pitch);
+        d_re[k] = hist_mode ? d_adj : buf.alloc(18 * I.pitch);  // no recomputation in history mode
         HIP_OK(hipMemsetAsync(d_adj, 0, 18 * I.pitch, st));
         const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
         const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
@@ -2275,7 +2335,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         bd[k].B = B;
         bd[k].inst = k;
         bd[k].slot_elems = slot_elems;
-        bd[k].slots = (T*)buf.alloc((size_t)K * slot_elems * esz);
+        bd[k].slots = hist_mode ? (T*)hs[k]->d : (T*)buf.alloc((size_t)K * slot_elems * esz);
         const size_t nbox = (size_t)B.nx * B.ny * B.nz;
         bd[k].grad = (double*)buf.alloc(6 * nbox * sizeof(double));
         HIP_OK(hipMemsetAsync(bd[k].grad, 0, 6 * nbox * sizeof(double), st));
@@ -2303,13 +2363,14 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     const int grid = (tile0 + 7) & ~7;
     const int flat = (int)flat0.back(), hblocks = (int)h0.back(), gblocks = (int)g0.back();
     for (int seg = nseg - 1; seg >= 0; --seg) {
-        const int64_t n0 = (int64_t)seg * K, n1 = std::min<int64_t>(Nt, n0 + K);
-        for (int k = 0; k < n; ++k) {
-            const Instance& I = s.insts[items[k].inst];
-            HIP_OK(hipMemcpyAsync(d_re[k], I.ckpt[seg], 12 * I.pitch, hipMemcpyDeviceToDevice, st));
-        }
-        k_totals<T><<<flat, 256, 0, st>>>(d_di, d_flat0, n);
-        for (int64_t t = n0; t < n1; ++t) {
+        const int64_t n0 = hist_mode ? 0 : (int64_t)seg * K, n1 = hist_mode ? Nt : std::min<int64_t>(Nt, n0 + K);
+        if (!hist_mode)
+            for (int k = 0; k < n; ++k) {
+                const Instance& I = s.insts[items[k].inst];
+                HIP_OK(hipMemcpyAsync(d_re[k], I.ckpt[seg], 12 * I.pitch, hipMemcpyDeviceToDevice, st));
+            }
+        if (!hist_mode) k_totals<T><<<flat, 256, 0, st>>>(d_di, d_flat0, n);
+        for (int64_t t = n0; t < n1 && !hist_mode; ++t) {
             const int j = (int)(t - n0);
             k_save_h<T><<<hblocks, 256, 0, st>>>(d_di, d_bd, d_h0, n, j);
             if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
@@ -2620,6 +2681,47 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
         }
         if (S.f64) plan_checkpoints<double>(S, Nt, items, budget_bytes, every_out, bytes_out);
         else plan_checkpoints<float>(S, Nt, items, budget_bytes, every_out, bytes_out);
+    });
+}
+
+int rfdtd_set_history(rfdtd_session* s, int32_t n, const int32_t* instances, const int32_t* boxes, int32_t n_steps) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (n < 0 || (n > 0 && (!instances || !boxes || n_steps < 1)))
+            RF_THROW(RFDTD_EINVAL, "history request arrays missing");
+        if (n > 0 && S.step != 0) RF_THROW(RFDTD_ESTATE, "record the history from the first step (run after rfdtd_reset)");
+        HIP_OK(hipDeviceSynchronize());
+        for (auto& h : S.hist)
+            if (h.d) (void)hipFree(h.d);
+        S.hist.clear();
+        for (int k = 0; k < n; ++k) {
+            const Instance& I = instance_at(S, instances[k]);
+            const Member& M = S.members[I.member];
+            const int32_t* b = boxes + 6 * k;
+            if (b[0] < 0 || b[2] < 0 || b[4] < 0 || b[1] <= b[0] || b[3] <= b[2] || b[5] <= b[4] || b[1] > M.Nx ||
+                b[3] > M.Ny || b[5] > M.Nz)
+                RF_THROW(RFDTD_EINVAL, "design box outside the grid");
+            Session::History h{};
+            h.inst = instances[k];
+            std::memcpy(h.box, b, sizeof(h.box));
+            h.slot_elems = (int64_t)3 * (b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
+            h.capacity = n_steps;
+            const size_t bytes = (size_t)h.slot_elems * n_steps * S.esz;
+            if (hipMalloc(&h.d, bytes) != hipSuccess)
+                RF_THROW(RFDTD_ENOMEM, "design-box history of " + std::to_string(bytes) + " bytes does not fit");
+            S.hist.push_back(h);
+        }
+        S.dirty = true;
+        S.graph_dirty = true;
+    });
+}
+
+int rfdtd_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+    return guarded([&] {
+        size_t f = 0, t = 0;
+        HIP_OK(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
     });
 }
 

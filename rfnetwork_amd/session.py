@@ -115,6 +115,14 @@ class Session:
         """Save the split fields every `every` steps of later runs (needed by gradient())."""
         check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
 
+    def set_history(self, box, Nt):
+        """Record the H totals of box at every step of later runs (up to Nt steps): a gradient over
+        that box then needs no checkpoints and no recomputation.  Before the first step only."""
+        inst = (ctypes.c_int32 * 1)(0)
+        b = (ctypes.c_int32 * 6)(*[int(v) for v in box])
+        check(self._L.rfdtd_set_history(self._h, 1, ctypes.cast(inst, ctypes.c_void_p), ctypes.cast(b, ctypes.c_void_p),
+                                        int(Nt)), "rfdtd_set_history")
+
     def plan_checkpoints(self, Nt, box, budget_bytes=0):
         """Checkpoint interval for a gradient over box after an Nt-step forward run that fits device
         memory (budget_bytes, 0: what is free).  Returns (every, bytes needed)."""

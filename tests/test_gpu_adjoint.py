@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from oracle import cpu_solver
-from rfnetwork_amd import adjoint, workloads
+from rfnetwork_amd import _lib, adjoint, workloads
 from rfnetwork_amd.session import Session
 
 import yee_torch
@@ -190,10 +190,96 @@ def test_eps_gradients_batch_equal_single(hip_ready):
     mb = workloads.design_strip(eps_b, Nt=300, nx=52, ny=36, nz=18, design=(6, 10, 5), n_pml=5, h=3,
                                 pulse=(20e9, 80e-12, 60e-12), frequency=SMALL_F)
     ca, cb = workloads.design_box(48, 32, (8, 8, 6)), workloads.design_box(52, 36, (6, 10, 5))
-    res, info = adjoint.eps_r_gradients_batch([(ma, ca, ma.frequency), (mb, cb, mb.frequency)], checkpoint_every=70,
-                                              return_info=True)
-    for (J, g, _), (m, cbox) in zip(res, ((ma, ca), (mb, cb))):
-        J1, g1, _ = adjoint.eps_r_gradient(m, cbox, m.frequency, checkpoint_every=70)
-        assert J == J1
-        np.testing.assert_array_equal(g, g1)
-    assert info["checkpoint_every"] == 70
+    for mode in ("checkpoint", "history"):
+        res, info = adjoint.eps_r_gradients_batch([(ma, ca, ma.frequency), (mb, cb, mb.frequency)],
+                                                  checkpoint_every=70, return_info=True, mode=mode)
+        for (J, g, _), (m, cbox) in zip(res, ((ma, ca), (mb, cb))):
+            J1, g1, _ = adjoint.eps_r_gradient(m, cbox, m.frequency, checkpoint_every=70, mode="checkpoint")
+            assert J == J1
+            np.testing.assert_array_equal(g, g1)
+        assert info["mode"] == mode
+        if mode == "checkpoint":
+            assert info["checkpoint_every"] == 70
+
+
+def _session_gradient(model, g, box, variant, history, chunks, flags=0):
+    coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    with Session(coeff, Nx, Ny, Nz, flags=flags) as s:
+        s.set_variant(variant)
+        if history:
+            s.set_history(box, Nt)
+        else:
+            s.set_checkpointing(9)
+        s.set_sources(probes, [], Nt)
+        for n in chunks:
+            s.run(n)
+        s.synchronize()
+        return s.gradient(box, list(g))
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("flags", [0, _lib.NO_GRAPHS])
+def test_history_gradient_equals_checkpointed(hip_ready, variant, flags):
+    """Recording the design box's H at every forward step (inside captured chunks or step by step,
+    with the separate kernels or the fused pass) gives the checkpointed gradient bit for bit."""
+    print(f"[case] history gradient variant={variant} flags={flags}", flush=True)
+    Nt = 60
+    model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
+    _, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True, coefficients=False)
+    g = np.random.default_rng(3).standard_normal((len(probes), Nt)).astype(np.float32)
+    box = (2, Nx - 3, 1, Ny - 2, 3, Nz - 1)
+    ref = _session_gradient(model, g, box, 1, False, [Nt], flags)
+    got = _session_gradient(model, g, box, variant, True, [37, 23], flags)
+    np.testing.assert_array_equal(got, ref)
+    assert np.abs(ref).max() > 0
+
+
+def test_history_requests_checked(hip_ready):
+    """The history is set before the first step, a gradient over another box needs checkpoints, and
+    a run past the recorded length is refused."""
+    print("[case] history errors", flush=True)
+    Nt = 20
+    model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
+    coeff, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True)
+    with Session(coeff, Nx, Ny, Nz) as s:
+        with pytest.raises(Exception, match="outside"):
+            s.set_history((0, Nx + 1, 0, Ny, 0, Nz), Nt)
+        s.set_history((0, 4, 0, 4, 0, 4), Nt - 5)
+        s.set_sources(probes, [], Nt)
+        s.run(Nt - 5)
+        with pytest.raises(Exception, match="history"):
+            s.run(1)
+        with pytest.raises(Exception, match="first step"):
+            s.set_history((0, 4, 0, 4, 0, 4), Nt)
+        s.synchronize()
+        zero = [np.zeros(Nt, np.float32) for _ in probes]
+        with pytest.raises(Exception, match="checkpointing"):
+            s.gradient((0, 5, 0, 4, 0, 4), zero)
+        assert s.gradient((0, 4, 0, 4, 0, 4), zero).shape == (6, 4, 4, 4)
+
+
+def test_history_batch_gradients(hip_ready):
+    """Histories of two uneven grids in one batch: each gradient equals its checkpointed single-grid
+    gradient."""
+    print("[case] history batch", flush=True)
+    from rfnetwork_amd.batch import Batch
+
+    Nt = 40
+    models = [workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False, seed=7),
+              workloads.tiny_model(Nt=Nt, nx=19, ny=12, nz=10, time_monitors=False, phasor_monitors=False, seed=8)]
+    rng = np.random.default_rng(9)
+    inputs = []
+    for m in models:
+        coeff, probes, _, _, Nx, Ny, Nz, _ = m.pack(gpu=True)
+        inputs.append((coeff, probes, (Nx, Ny, Nz), rng.standard_normal((len(probes), Nt)).astype(np.float32)))
+    boxes = [(0, 14, 0, 11, 0, 9), (3, 15, 2, 9, 1, 8)]
+    with Batch([(c, *dims, 1) for c, _, dims, _ in inputs]) as b:
+        b.set_history([(0, boxes[0]), (1, boxes[1])], Nt)
+        for k, (_, probes, _, _) in enumerate(inputs):
+            b.set_sources(k, probes, [], Nt)
+        b.run(Nt)
+        b.synchronize()
+        got = b.gradients([(k, boxes[k], inputs[k][3]) for k in range(2)])
+    for k, (_, _, _, g) in enumerate(inputs):
+        ref = _session_gradient(models[k], g, boxes[k], 1, False, [Nt])
+        np.testing.assert_array_equal(got[k], ref)
