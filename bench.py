@@ -311,7 +311,7 @@ def adjoint_leg(n_steps, every, repeats=2, flags=0):
                          f"{n_steps} steps, design-box H history", gradients_per_s=round(1.0 / wall, 4),
                 s_per_gradient=round(wall, 3), forward_ms=round(t["forward_ms"], 1),
                 reverse_ms=round(t["adjoint_ms"], 1), design_cells=int(grad.size), J=J,
-                history_bytes=int(adjoint.history_bytes([adjoint.gpu_box_for_cells(model, cbox)], n_steps)),
+                adjoint_launch=t["adjoint_launch"], history_bytes=int(adjoint.history_bytes([adjoint.gpu_box_for_cells(model, cbox)], n_steps)),
                 forward_mcells=round(cells * n_steps / (t["forward_ms"] * 1e-3) / 1e6, 1),
                 checkpointed=dict(every=every, gradients_per_s=round(1.0 / wc, 4), forward_ms=round(tc["forward_ms"], 1),
                                   reverse_ms=round(tc["adjoint_ms"], 1)),

@@ -196,6 +196,16 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * instance); 4 variant 1 with nontemporal stores of the fields.  Results are identical; the choice
  * only changes speed (the autotuner picks it). */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
+
+/* Launch shape of the reverse pass of rfdtd_gradient(_batch) (no counterpart in the reference):
+ * variant 1 x-marching adjoint columns that issue the next plane's loads (and the coefficient keys
+ * of the plane after) before the current plane's arithmetic, 0 one plane at a time; block_threads
+ * per workgroup; x_chunks workgroups along x per (y,z) tile (0: the forward's planes per
+ * workgroup).  block_threads 0 (the default) times the candidates on the zeroed adjoint state at
+ * the next gradient and keeps the fastest for the session.  Results are identical either way.
+ * rfdtd_get_adjoint_launch returns {variant, block_threads, x_chunks} in force. */
+int rfdtd_set_adjoint_launch(rfdtd_session* s, int32_t variant, int32_t block_threads, int32_t x_chunks);
+int rfdtd_get_adjoint_launch(rfdtd_session* s, int32_t config_out[3]);
 void rfdtd_destroy(rfdtd_session* s);
 
 /*

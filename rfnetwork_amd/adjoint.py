@@ -215,9 +215,11 @@ def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=
         J, S, g = sparam_objective(model, series, frequency, b_port, a_port)
         grad_cb = s.gradient(gbox, [row.astype(dtype) for row in g])
         t_adj = s.timing()[0]
+        adj_launch = s.adjoint_launch()
     grad = cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox)
     if return_session_timing:
-        return J, grad, S, dict(forward_ms=t_fwd, adjoint_ms=t_adj, mode="history" if history else "checkpoint")
+        return J, grad, S, dict(forward_ms=t_fwd, adjoint_ms=t_adj, mode="history" if history else "checkpoint",
+                                adjoint_launch=adj_launch)
     return J, grad, S
 
 

@@ -64,11 +64,37 @@ def apply(session, cfg):
     session.set_launch(cfg["block"], cfg["x_planes"] if cfg["x_planes"] > 0 else -1, 0)
 
 
+def _adjoint_from_cache(session, key, use_cache):
+    """Apply the reverse-pass shape cached for key (timed by an earlier gradient), if any; later
+    gradients of the session record theirs under key (remember_adjoint)."""
+    session._tune_key = key if use_cache else None
+    if not use_cache:
+        return
+    akey = key + "|adjoint"
+    acfg = _memory.get(akey) or _load().get(akey)
+    if acfg:
+        _memory[akey] = acfg
+        session.set_adjoint_launch(acfg["variant"], acfg["block"], acfg["x_chunks"])
+
+
+def remember_adjoint(session):
+    """Cache the reverse-pass shape that a gradient of an autotuned session timed."""
+    key = getattr(session, "_tune_key", None)
+    if key is None:
+        return
+    cfg = session.adjoint_launch()
+    akey = key + "|adjoint"
+    if cfg["block"] > 0 and _memory.get(akey) != cfg:
+        _memory[akey] = cfg
+        _store(akey, cfg)
+
+
 def tune(session, grids, dtype, coef_kind, trial_steps=20, use_cache=True):
     """Tuned launch shape of the session's grids: from the cache when known, else measured (and
-    cached).  The shape is applied to the session either way.  Returns the config with
-    cached=True/False."""
+    cached).  The shape is applied to the session either way, with the reverse-pass shape when one
+    is cached.  Returns the config with cached=True/False."""
     key = shape_key(grids, dtype, coef_kind)
+    _adjoint_from_cache(session, key, use_cache)
     cfg = _memory.get(key)
     if cfg is None and use_cache:
         cfg = _load().get(key)

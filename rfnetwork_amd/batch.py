@@ -106,6 +106,18 @@ class Batch:
         """Save the split fields of every instance every `every` steps of later runs."""
         check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
 
+    def set_adjoint_launch(self, variant=1, block=0, x_chunks=0):
+        """Reverse-pass launch shape: variant 1 pipelined adjoint columns, 0 plain; block threads per
+        workgroup (0: timed at the next gradient); x_chunks workgroups along x (0: the forward's)."""
+        check(self._L.rfdtd_set_adjoint_launch(self._h, int(variant), int(block), int(x_chunks)),
+              "rfdtd_set_adjoint_launch")
+
+    def adjoint_launch(self):
+        """{variant, block, x_chunks} of the reverse pass (block 0 until the first gradient)."""
+        cfg = (ctypes.c_int32 * 3)()
+        check(self._L.rfdtd_get_adjoint_launch(self._h, ctypes.cast(cfg, ctypes.c_void_p)), "rfdtd_get_adjoint_launch")
+        return dict(variant=int(cfg[0]), block=int(cfg[1]), x_chunks=int(cfg[2]))
+
     def set_history(self, requests, Nt):
         """Record the H totals of each request's box, requests [(instance, box), ...], at every step of
         later runs (up to Nt steps), for gradients without recomputation.  Before the first step."""
@@ -154,6 +166,9 @@ class Batch:
                                            ctypes.cast(boxes, ctypes.c_void_p), ctypes.cast(series_ptrs, ctypes.c_void_p),
                                            ctypes.cast(counts, ctypes.c_void_p), ctypes.cast(out_ptrs, ctypes.c_void_p)),
               "rfdtd_gradient_batch")
+        from . import autotune
+
+        autotune.remember_adjoint(self)
         return outs
 
     def set_variant(self, variant=1):

@@ -162,6 +162,7 @@ struct CoefRaw {
     // two-stage access (key first, values later) for software pipelining: here the key is the offset
     __device__ __forceinline__ uint32_t ekey(uint32_t b) const { return b; }
     __device__ __forceinline__ uint32_t hkey(uint32_t b) const { return b; }
+    __device__ __forceinline__ T at_key(uint32_t key, int k) const { return at(key, k); }
     __device__ __forceinline__ ECoef<T> e_at(uint32_t key) const { return e(key); }
     __device__ __forceinline__ HCoef<T> h_at(uint32_t key) const { return h(key); }
     __device__ __forceinline__ HCoef<T> h(uint32_t b) const {
@@ -197,6 +198,11 @@ struct CoefTable {
     }
     __device__ __forceinline__ uint32_t hkey(uint32_t bc) const {
         return ldg((const GLOBAL_AS uint32_t*)hidx, bc / (uint32_t)sizeof(T) * 4u);
+    }
+    // coefficient k alone from an entry index (an E entry for k < 12, else an H entry)
+    __device__ __forceinline__ T at_key(uint32_t key, int k) const {
+        if (k < 12) return ldg((const GLOBAL_AS T*)etab, (key * 12u + (uint32_t)k) * (uint32_t)sizeof(T));
+        return ldg((const GLOBAL_AS T*)htab, (key * 20u + (uint32_t)(k - 12)) * (uint32_t)sizeof(T));
     }
     __device__ __forceinline__ ECoef<T> e(uint32_t bc) const { return e_at(ekey(bc)); }
     __device__ __forceinline__ HCoef<T> h(uint32_t bc) const { return h_at(hkey(bc)); }
@@ -822,30 +828,229 @@ __device__ __forceinline__ void adj_h_column(const Fields<T>& F, const CA& C, in
     }
 }
 
+// The same two reverse half steps software-pipelined: the coefficient entry keys of plane x + 2 and
+// every other load of plane x + 1 are issued before plane x's arithmetic and stores, so a lane keeps
+// two planes of loads in flight and never waits on a key just loaded.  Arithmetic (and so every
+// result bit) is that of adj_e_column / adj_h_column.
 template <typename T>
+struct AEIn {
+    T m_xy, m_xz, m_yz, m_zy;              // mu at the cell
+    T n_yx, n_zx;                          // mu x splits at x + 1
+    T m_xy_y, m_zy_y, m_xz_z, m_yz_z;      // mu at y + 1 and z + 1
+    T d_xy1_y, d_zy1_y, d_xz1_z, d_yz1_z;  // their Db coefficients
+    HCoef<T> hn;                           // H coefficients at x + 1
+    T ca_xy, ca_xz, ca_yz, ca_yx, ca_zx, ca_zy;
+    T v_xy, v_xz, v_yz, v_yx, v_zx, v_zy;  // nu at the cell
+};
+struct AEKeys {
+    uint32_t hx, hy, hz, e;  // H entry at x + 1, y + 1, z + 1; E entry at the cell
+};
+
+template <typename T, class CA>
+__device__ __forceinline__ void adj_e_column_pf(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+    using G = GLOBAL_T(T);
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
+    const G* mxy = (const G*)F.hx_y; const G* mxz = (const G*)F.hx_z;
+    const G* myz = (const G*)F.hy_z; const G* myx = (const G*)F.hy_x;
+    const G* mzx = (const G*)F.hz_x; const G* mzy = (const G*)F.hz_y;
+    G* nxy = (G*)F.ex_y; G* nxz = (G*)F.ex_z; G* nyz = (G*)F.ey_z;
+    G* nyx = (G*)F.ey_x; G* nzx = (G*)F.ez_x; G* nzy = (G*)F.ez_y;
+    const bool yin = t.y < Ny - 1, zin = t.z < Nz - 1;
+    if (t.x0 >= t.x1) return;
+    auto keys = [&](uint32_t b, bool xin) {
+        AEKeys k;
+        k.hx = xin ? C.hkey(b + sx) : 0u;
+        k.hy = yin ? C.hkey(b + sy) : 0u;
+        k.hz = zin ? C.hkey(b + S) : 0u;
+        k.e = C.ekey(b);
+        return k;
+    };
+    auto load = [&](AEIn<T>& v, uint32_t b, bool xin, const AEKeys& k) {
+        v.m_xy = ldg(mxy, b); v.m_xz = ldg(mxz, b); v.m_yz = ldg(myz, b); v.m_zy = ldg(mzy, b);
+        v.n_yx = xin ? ldg(myx, b + sx) : T(0);
+        v.n_zx = xin ? ldg(mzx, b + sx) : T(0);
+        v.m_xy_y = yin ? ldg(mxy, b + sy) : T(0);
+        v.m_zy_y = yin ? ldg(mzy, b + sy) : T(0);
+        v.m_xz_z = zin ? ldg(mxz, b + S) : T(0);
+        v.m_yz_z = zin ? ldg(myz, b + S) : T(0);
+        v.v_xy = ldg((const G*)nxy, b); v.v_xz = ldg((const G*)nxz, b); v.v_yz = ldg((const G*)nyz, b);
+        v.v_yx = ldg((const G*)nyx, b); v.v_zx = ldg((const G*)nzx, b); v.v_zy = ldg((const G*)nzy, b);
+        v.d_xy1_y = yin ? C.at_key(k.hy, C_DB_XY1) : T(0);
+        v.d_zy1_y = yin ? C.at_key(k.hy, C_DB_ZY1) : T(0);
+        v.d_xz1_z = zin ? C.at_key(k.hz, C_DB_XZ1) : T(0);
+        v.d_yz1_z = zin ? C.at_key(k.hz, C_DB_YZ1) : T(0);
+        if (xin) v.hn = C.h_at(k.hx);
+        else v.hn = HCoef<T>{};
+        const ECoef<T> ec = C.e_at(k.e);
+        v.ca_xy = ec.ca_xy; v.ca_xz = ec.ca_xz; v.ca_yz = ec.ca_yz;
+        v.ca_yx = ec.ca_yx; v.ca_zx = ec.ca_zx; v.ca_zy = ec.ca_zy;
+    };
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    HCoef<T> hc = C.h(b);
+    T m_yx = ldg(myx, b), m_zx = ldg(mzx, b);
+    AEIn<T> cur;
+    load(cur, b, t.x0 < Nx - 1, keys(b, t.x0 < Nx - 1));
+    AEKeys kn{};
+    if (t.x0 + 1 < t.x1) kn = keys(b + sx, t.x0 + 1 < Nx - 1);
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool xin = x < Nx - 1, more = x + 1 < t.x1;
+        AEIn<T> nxt;
+        if (more) {
+            load(nxt, b + sx, x + 1 < Nx - 1, kn);
+            if (x + 2 < t.x1) kn = keys(b + 2u * sx, x + 2 < Nx - 1);
+        }
+        T ez = hc.db_xy2 * cur.m_xy + hc.db_yx2 * m_yx;
+        if (yin) ez = ez - cur.d_xy1_y * cur.m_xy_y;
+        if (xin) ez = ez - cur.hn.db_yx1 * cur.n_yx;
+        T ey = hc.db_xz2 * cur.m_xz + hc.db_zx2 * m_zx;
+        if (zin) ey = ey - cur.d_xz1_z * cur.m_xz_z;
+        if (xin) ey = ey - cur.hn.db_zx1 * cur.n_zx;
+        T ex = hc.db_yz2 * cur.m_yz + hc.db_zy2 * cur.m_zy;
+        if (zin) ex = ex - cur.d_yz1_z * cur.m_yz_z;
+        if (yin) ex = ex - cur.d_zy1_y * cur.m_zy_y;
+        stg(nxy, b, (yin ? cur.ca_xy : T(1)) * cur.v_xy + ex);
+        stg(nxz, b, (zin ? cur.ca_xz : T(1)) * cur.v_xz + ex);
+        stg(nyz, b, (zin ? cur.ca_yz : T(1)) * cur.v_yz + ey);
+        stg(nyx, b, (xin ? cur.ca_yx : T(1)) * cur.v_yx + ey);
+        stg(nzx, b, (xin ? cur.ca_zx : T(1)) * cur.v_zx + ez);
+        stg(nzy, b, (yin ? cur.ca_zy : T(1)) * cur.v_zy + ez);
+        hc = cur.hn;
+        m_yx = cur.n_yx;
+        m_zx = cur.n_zx;
+        if (more) cur = nxt;
+    }
+}
+
+template <typename T>
+struct AHIn {
+    T cb_xy, cb_xz, cb_yz, cb_yx, cb_zx, cb_zy;  // E coefficients at the cell
+    T v_xy, v_xz, v_yz, v_yx, v_zx, v_zy;        // nu at the cell
+    T w_xy_y, w_zy_y, w_xz_z, w_yz_z;            // Cb * nu at y - 1 and z - 1
+    T da_xy, da_xz, da_yz, da_yx, da_zx, da_zy;  // H coefficients at the cell
+    T m_xy, m_xz, m_yz, m_yx, m_zx, m_zy;        // mu at the cell
+};
+struct AHKeys {
+    uint32_t e, ey, ez, h;  // E entry at the cell, y - 1, z - 1; H entry at the cell
+};
+
+template <typename T, class CA>
+__device__ __forceinline__ void adj_h_column_pf(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+    using G = GLOBAL_T(T);
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
+    const G* nxy = (const G*)F.ex_y; const G* nxz = (const G*)F.ex_z; const G* nyz = (const G*)F.ey_z;
+    const G* nyx = (const G*)F.ey_x; const G* nzx = (const G*)F.ez_x; const G* nzy = (const G*)F.ez_y;
+    G* mxy = (G*)F.hx_y; G* mxz = (G*)F.hx_z; G* myz = (G*)F.hy_z;
+    G* myx = (G*)F.hy_x; G* mzx = (G*)F.hz_x; G* mzy = (G*)F.hz_y;
+    const bool yin = t.y < Ny - 1, zin = t.z < Nz - 1, ylo = t.y > 0, zlo = t.z > 0;
+    if (t.x0 >= t.x1) return;
+    auto keys = [&](uint32_t b) {
+        AHKeys k;
+        k.e = C.ekey(b);
+        k.ey = ylo ? C.ekey(b - sy) : 0u;
+        k.ez = zlo ? C.ekey(b - S) : 0u;
+        k.h = C.hkey(b);
+        return k;
+    };
+    // the neighbour products are formed as the plain kernel forms them (coefficient * nu)
+    auto load = [&](AHIn<T>& v, uint32_t b, const AHKeys& k) {
+        v.v_xy = ldg(nxy, b); v.v_xz = ldg(nxz, b); v.v_yz = ldg(nyz, b);
+        v.v_yx = ldg(nyx, b); v.v_zx = ldg(nzx, b); v.v_zy = ldg(nzy, b);
+        v.m_xy = ldg((const G*)mxy, b); v.m_xz = ldg((const G*)mxz, b); v.m_yz = ldg((const G*)myz, b);
+        v.m_yx = ldg((const G*)myx, b); v.m_zx = ldg((const G*)mzx, b); v.m_zy = ldg((const G*)mzy, b);
+        T a_xy = T(0), a_zy = T(0), a_xz = T(0), a_yz = T(0), c_xy = T(0), c_zy = T(0), c_xz = T(0), c_yz = T(0);
+        if (ylo) {
+            a_xy = ldg(nxy, b - sy); a_zy = ldg(nzy, b - sy);
+            c_xy = C.at_key(k.ey, C_CB_XY); c_zy = C.at_key(k.ey, C_CB_ZY);
+        }
+        if (zlo) {
+            a_xz = ldg(nxz, b - S); a_yz = ldg(nyz, b - S);
+            c_xz = C.at_key(k.ez, C_CB_XZ); c_yz = C.at_key(k.ez, C_CB_YZ);
+        }
+        const ECoef<T> ec = C.e_at(k.e);
+        v.cb_xy = ec.cb_xy; v.cb_xz = ec.cb_xz; v.cb_yz = ec.cb_yz;
+        v.cb_yx = ec.cb_yx; v.cb_zx = ec.cb_zx; v.cb_zy = ec.cb_zy;
+        const HCoef<T> hc = C.h_at(k.h);
+        v.da_xy = hc.da_xy; v.da_xz = hc.da_xz; v.da_yz = hc.da_yz;
+        v.da_yx = hc.da_yx; v.da_zx = hc.da_zx; v.da_zy = hc.da_zy;
+        v.w_xy_y = c_xy * a_xy; v.w_zy_y = c_zy * a_zy;
+        v.w_xz_z = c_xz * a_xz; v.w_yz_z = c_yz * a_yz;
+    };
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    T p_yx = T(0), p_zx = T(0);
+    if (t.x0 > 0) {
+        p_yx = C.at(b - sx, C_CB_YX) * ldg(nyx, b - sx);
+        p_zx = C.at(b - sx, C_CB_ZX) * ldg(nzx, b - sx);
+    }
+    AHIn<T> cur;
+    load(cur, b, keys(b));
+    AHKeys kn{};
+    if (t.x0 + 1 < t.x1) kn = keys(b + sx);
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool xin = x < Nx - 1, xlo = x > 0, more = x + 1 < t.x1;
+        AHIn<T> nxt;
+        if (more) {
+            load(nxt, b + sx, kn);
+            if (x + 2 < t.x1) kn = keys(b + 2u * sx);
+        }
+        T hz = T(0), hy = T(0), hx = T(0);
+        if (yin) hz = hz - cur.cb_xy * cur.v_xy;
+        if (ylo) hz = hz + cur.w_xy_y;
+        if (xin) hz = hz - cur.cb_yx * cur.v_yx;
+        if (xlo) hz = hz + p_yx;
+        if (zin) hy = hy - cur.cb_xz * cur.v_xz;
+        if (zlo) hy = hy + cur.w_xz_z;
+        if (xin) hy = hy - cur.cb_zx * cur.v_zx;
+        if (xlo) hy = hy + p_zx;
+        if (zin) hx = hx - cur.cb_yz * cur.v_yz;
+        if (zlo) hx = hx + cur.w_yz_z;
+        if (yin) hx = hx - cur.cb_zy * cur.v_zy;
+        if (ylo) hx = hx + cur.w_zy_y;
+        stg(mxy, b, cur.da_xy * cur.m_xy + hx);
+        stg(mxz, b, cur.da_xz * cur.m_xz + hx);
+        stg(myz, b, cur.da_yz * cur.m_yz + hy);
+        stg(myx, b, cur.da_yx * cur.m_yx + hy);
+        stg(mzx, b, cur.da_zx * cur.m_zx + hz);
+        stg(mzy, b, cur.da_zy * cur.m_zy + hz);
+        p_yx = cur.cb_yx * cur.v_yx;
+        p_zx = cur.cb_zx * cur.v_zx;
+        if (more) cur = nxt;
+    }
+}
+
+template <typename T, bool PF>
 __global__ void __launch_bounds__(1024) k_adj_e(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
                                                 int nmem, int total) {
     __shared__ int s_member;
     Tile t;
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
-    if (I->coef_kind == 1)
-        adj_e_column<T, CoefTable<T>>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
-    else
-        adj_e_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+    if (I->coef_kind == 1) {
+        const CoefTable<T> C{I->eidx, I->hidx, I->etab, I->htab};
+        if (PF) adj_e_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
+        else adj_e_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
+    } else {
+        if (PF) adj_e_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        else adj_e_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+    }
 }
 
-template <typename T>
+template <typename T, bool PF>
 __global__ void __launch_bounds__(1024) k_adj_h(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
                                                 int nmem, int total) {
     __shared__ int s_member;
     Tile t;
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
-    if (I->coef_kind == 1)
-        adj_h_column<T, CoefTable<T>>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
-    else
-        adj_h_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+    if (I->coef_kind == 1) {
+        const CoefTable<T> C{I->eidx, I->hidx, I->etab, I->htab};
+        if (PF) adj_h_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
+        else adj_h_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
+    } else {
+        if (PF) adj_h_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        else adj_h_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+    }
 }
 
 // Totals from splits after a checkpoint restore (the forward computes them the same way).
@@ -1258,6 +1463,10 @@ struct Session {
     bool tuning = false;    // autotune trials: field updates only, no probes or monitors
     int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
+    // reverse-pass launch shape: pipelined (1) or plain (0) columns, threads per workgroup and x
+    // chunks per grid (0: the forward's planes per workgroup); adj_block 0 = timed at the first
+    // gradient and kept
+    int adj_pf = 1, adj_block = 0, adj_chunks = 0;
     int parity = 0;         // which buffer of each instance's pair holds the current fields
     bool totals_stale = false;  // the fused step leaves the total arrays unwritten
     void* d_insts2 = nullptr;   // descriptors with the buffers of each pair exchanged
@@ -1908,7 +2117,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     int maxNx = 1;
     for (const auto& M : s.members) maxNx = std::max(maxNx, M.Nx);
     std::vector<int> xps{0};  // 0: automatic (enough workgroups to fill the chip)
-    for (int chunks : {2, 4, 8, 16, 32}) {
+    for (int chunks : {2, 4, 8, 16, 32, 64, 128}) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
@@ -2348,8 +2557,92 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         HIP_OK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st));
         return d;
     };
+    // reverse-pass tiling: its own workgroup size and planes per workgroup in the adjoint
+    // descriptors [n, 2n) and member table d_mda
+    std::vector<MemberDev> mda(n);
+    auto adj_tiling = [&](int block, int chunks) {
+        int tiles = 0;
+        for (int k = 0; k < n; ++k) {
+            const Member& M = s.members[s.insts[items[k].inst].member];
+            const int xp = chunks > 0 ? (M.Nx + chunks - 1) / chunks : M.xplanes;
+            const int npb = (int)(((int64_t)M.Ny * M.Nz + block - 1) / block);
+            const int nch = (M.Nx + xp - 1) / xp;
+            di[n + k].npb = npb;
+            di[n + k].xplanes = xp;
+            mda[k] = MemberDev{tiles, npb * nch, 1, k};
+            tiles += npb * nch;
+        }
+        return tiles;
+    };
+    int ablock = s.adj_block > 0 ? s.adj_block : s.block;
+    int atiles = adj_tiling(ablock, s.adj_block > 0 ? s.adj_chunks : 0);
     const InstDev<T>* d_di = (const InstDev<T>*)upload(di.data(), di.size() * sizeof(InstDev<T>));
     const MemberDev* d_md = (const MemberDev*)upload(md.data(), md.size() * sizeof(MemberDev));
+    MemberDev* d_mda = (MemberDev*)upload(mda.data(), mda.size() * sizeof(MemberDev));
+    auto adj_e = [&](int pf, int block, int tiles) {
+        const int g = (tiles + 7) & ~7;
+        if (pf) k_adj_e<T, true><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
+        else k_adj_e<T, false><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
+    };
+    auto adj_h = [&](int pf, int block, int tiles) {
+        const int g = (tiles + 7) & ~7;
+        if (pf) k_adj_h<T, true><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
+        else k_adj_h<T, false><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
+    };
+    if (s.adj_block <= 0) {
+        HIP_OK(hipStreamSynchronize(st));  // the descriptor uploads above read di and mda
+        // time the candidate shapes on the zeroed adjoint state (a linear map keeps it zero; it is
+        // cleared again below) and keep the fastest for this session
+        std::vector<int> chunk_opts;
+        int maxNx = 1;
+        for (const GradItem& it : items) maxNx = std::max(maxNx, s.members[s.insts[it.inst].member].Nx);
+        for (int c : {1, 2, 4, 8, 16, 32, 64, 128, 256})
+            if (c < maxNx) chunk_opts.push_back(c);
+        chunk_opts.push_back(maxNx);  // one plane per workgroup
+        double best = 1e300;
+        int bpf = 1, bblk = ablock, bch = 1;
+        const int reps = 3;
+        for (int pf : {1, 0})
+            for (int blk : {128, 256, 512})
+                for (int ch : chunk_opts) {
+                    const int tiles = adj_tiling(blk, ch);
+                    HIP_OK(hipMemcpyAsync((void*)(d_di + n), di.data() + n, n * sizeof(InstDev<T>), hipMemcpyHostToDevice, st));
+                    HIP_OK(hipMemcpyAsync(d_mda, mda.data(), n * sizeof(MemberDev), hipMemcpyHostToDevice, st));
+                    adj_e(pf, blk, tiles);
+                    adj_h(pf, blk, tiles);
+                    HIP_OK(hipEventRecord(s.ev0, st));
+                    for (int r = 0; r < reps; ++r) {
+                        adj_e(pf, blk, tiles);
+                        adj_h(pf, blk, tiles);
+                    }
+                    HIP_OK(hipEventRecord(s.ev1, st));
+                    HIP_OK(hipEventSynchronize(s.ev1));
+                    HIP_OK(hipGetLastError());
+                    float ms = 0;
+                    HIP_OK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+                    if (ms < best) {
+                        best = ms;
+                        bpf = pf;
+                        bblk = blk;
+                        bch = ch;
+                    }
+                }
+        s.adj_pf = bpf;
+        s.adj_block = bblk;
+        s.adj_chunks = bch;
+        if (s.flags & RFDTD_VERBOSE) {
+            std::fprintf(stderr, "[rfdtd] adjoint launch: %s columns, %d threads, %d x chunks (%.3f ms per step)\n",
+                         bpf ? "pipelined" : "plain", bblk, bch, best / reps);
+            std::fflush(stderr);
+        }
+        ablock = bblk;
+        atiles = adj_tiling(ablock, bch);
+        HIP_OK(hipMemcpyAsync((void*)(d_di + n), di.data() + n, n * sizeof(InstDev<T>), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(d_mda, mda.data(), n * sizeof(MemberDev), hipMemcpyHostToDevice, st));
+        for (int k = 0; k < n; ++k) HIP_OK(hipMemsetAsync(di[n + k].F.ex_y, 0, 18 * s.insts[items[k].inst].pitch, st));
+        HIP_OK(hipEventRecord(s.ev0, st));
+    }
+    const int apf = s.adj_pf;
     const BoxDev<T>* d_bd = (const BoxDev<T>*)upload(bd.data(), bd.size() * sizeof(BoxDev<T>));
     const int64_t* d_flat0 = (const int64_t*)upload(flat0.data(), flat0.size() * 8);
     const int64_t* d_h0 = (const int64_t*)upload(h0.data(), h0.size() * 8);
@@ -2383,10 +2676,10 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         for (int64_t t = n1 - 1; t >= n0; --t) {
             const int j = (int)(t - n0);
             if (ngH) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + n, d_ag + ngE, ngH, (int)t);
-            k_adj_e<T><<<grid, s.block, 0, st>>>(d_di + n, d_md, n, tile0);
+            adj_e(apf, ablock, atiles);
             if (ngE) k_adj_inject<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di + n, d_ag, ngE, (int)t);
             k_adj_grad<T><<<gblocks, 256, 0, st>>>(d_di + n, d_bd, d_g0, n, j);
-            k_adj_h<T><<<grid, s.block, 0, st>>>(d_di + n, d_md, n, tile0);
+            adj_h(apf, ablock, atiles);
         }
         HIP_OK(hipGetLastError());
         if ((s.flags & RFDTD_VERBOSE) && (seg % std::max(1, nseg / 10) == 0 || seg == 0)) {
@@ -2656,6 +2949,29 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
         S.dirty = true;
         S.graph_dirty = true;
         if (!S.fused) release_second_buffers(S);
+    });
+}
+
+int rfdtd_set_adjoint_launch(rfdtd_session* s, int32_t variant, int32_t block_threads, int32_t x_chunks) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (variant < 0 || variant > 1) RF_THROW(RFDTD_EINVAL, "adjoint variant must be 0 or 1");
+        if (block_threads != 0 && (block_threads < 64 || block_threads > 1024 || block_threads % 64))
+            RF_THROW(RFDTD_EINVAL, "block_threads must be 0 or a multiple of 64 in [64, 1024]");
+        if (x_chunks < 0) RF_THROW(RFDTD_EINVAL, "x_chunks must be >= 0");
+        S.adj_pf = variant;
+        S.adj_block = block_threads;
+        S.adj_chunks = x_chunks;
+    });
+}
+
+int rfdtd_get_adjoint_launch(rfdtd_session* s, int32_t config_out[3]) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (!config_out) RF_THROW(RFDTD_EINVAL, "config_out missing");
+        config_out[0] = S.adj_pf;
+        config_out[1] = S.adj_block;
+        config_out[2] = S.adj_chunks;
     });
 }
 

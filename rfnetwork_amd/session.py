@@ -115,6 +115,18 @@ class Session:
         """Save the split fields every `every` steps of later runs (needed by gradient())."""
         check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
 
+    def set_adjoint_launch(self, variant=1, block=0, x_chunks=0):
+        """Reverse-pass launch shape: variant 1 pipelined adjoint columns, 0 plain; block threads per
+        workgroup (0: timed at the next gradient); x_chunks workgroups along x (0: the forward's)."""
+        check(self._L.rfdtd_set_adjoint_launch(self._h, int(variant), int(block), int(x_chunks)),
+              "rfdtd_set_adjoint_launch")
+
+    def adjoint_launch(self):
+        """{variant, block, x_chunks} of the reverse pass (block 0 until the first gradient)."""
+        cfg = (ctypes.c_int32 * 3)()
+        check(self._L.rfdtd_get_adjoint_launch(self._h, ctypes.cast(cfg, ctypes.c_void_p)), "rfdtd_get_adjoint_launch")
+        return dict(variant=int(cfg[0]), block=int(cfg[1]), x_chunks=int(cfg[2]))
+
     def set_history(self, box, Nt):
         """Record the H totals of box at every step of later runs (up to Nt steps): a gradient over
         that box then needs no checkpoints and no recomputation.  Before the first step only."""
@@ -147,6 +159,9 @@ class Session:
         check(self._L.rfdtd_gradient(self._h, int(instance), ctypes.cast(b, ctypes.c_void_p),
                                      ctypes.cast(ptrs, ctypes.c_void_p), len(series), out.ctypes.data_as(ctypes.c_void_p)),
               "rfdtd_gradient")
+        from . import autotune
+
+        autotune.remember_adjoint(self)
         return out
 
     def set_variant(self, variant=1):

@@ -283,3 +283,31 @@ def test_history_batch_gradients(hip_ready):
     for k, (_, _, _, g) in enumerate(inputs):
         ref = _session_gradient(models[k], g, boxes[k], 1, False, [Nt])
         np.testing.assert_array_equal(got[k], ref)
+
+
+def test_adjoint_launch_shapes_give_identical_gradients(hip_ready):
+    """The reverse pass's launch shapes (pipelined or plain columns, workgroup sizes, x chunks down
+    to one plane per workgroup) and the shape its timing picks all give the same gradient bits."""
+    print("[case] adjoint launch shapes", flush=True)
+    Nt = 40
+    model = workloads.tiny_model(Nt=Nt, nx=17, ny=13, nz=70, time_monitors=False, phasor_monitors=False)
+    coeff, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True)
+    g = np.random.default_rng(11).standard_normal((len(probes), Nt)).astype(np.float32)
+    box = (1, Nx - 1, 2, Ny, 0, Nz - 3)
+    with Session(coeff, Nx, Ny, Nz) as s:
+        s.set_history(box, Nt)
+        s.set_sources(probes, [], Nt)
+        s.run(Nt)
+        s.synchronize()
+        s.set_adjoint_launch(0, 256, 0)
+        ref = s.gradient(box, list(g))
+        for v, blk, ch in [(1, 256, 0), (1, 128, 1), (1, 512, 3), (0, 64, 7), (1, 64, Nx), (1, 1024, 2)]:
+            s.set_adjoint_launch(v, blk, ch)
+            assert s.adjoint_launch() == dict(variant=v, block=blk, x_chunks=ch)
+            np.testing.assert_array_equal(s.gradient(box, list(g)), ref, err_msg=f"{v} {blk} {ch}")
+        s.set_adjoint_launch(1, 0, 0)
+        np.testing.assert_array_equal(s.gradient(box, list(g)), ref)
+        assert s.adjoint_launch()["block"] in (128, 256, 512)
+        with pytest.raises(Exception, match="block_threads"):
+            s.set_adjoint_launch(1, 100, 0)
+    assert np.abs(ref).max() > 0

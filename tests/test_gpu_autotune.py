@@ -75,3 +75,34 @@ def test_autotuned_batch_matches_single_runs(hip_ready, tmp_path, monkeypatch):
         p_c, _, _ = run_oracle(m, np.float32, n_threads=2, fields=False)
         for a, c in zip(probes, p_c):
             np.testing.assert_array_equal(a["values"], c["values"])
+
+
+def test_adjoint_shape_timed_once_and_cached(hip_ready, tmp_path, monkeypatch):
+    """The first gradient of an autotuned session times the reverse-pass shapes; the chosen shape is
+    cached under the grid's key and applied to the next session of that shape, with the same
+    gradient."""
+    print("[case] adjoint shape cache", flush=True)
+    monkeypatch.setenv("RFNETWORK_AMD_TUNE_CACHE", str(tmp_path / "launch.json"))
+    autotune._memory.clear()
+    Nt = 30
+    model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
+    c, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True)
+    g = [np.random.default_rng(2).standard_normal(Nt).astype(np.float32) for _ in probes]
+    box = (0, Nx, 0, Ny, 0, Nz)
+    out, shapes = [], []
+    for _ in range(2):
+        autotune._memory.clear()
+        _, pr, _, _, *_ = model.pack(gpu=True, coefficients=False)
+        with Session(c, Nx, Ny, Nz) as s:
+            s.autotune(trial_steps=3)
+            shapes.append(s.adjoint_launch())
+            s.set_history(box, Nt)
+            s.set_sources(pr, [], Nt)
+            s.run(Nt)
+            s.synchronize()
+            out.append(s.gradient(box, g))
+            assert s.adjoint_launch()["block"] in (128, 256, 512)
+            shapes.append(s.adjoint_launch())
+    assert shapes[0]["block"] == 0  # nothing cached yet: timed at the gradient
+    assert shapes[2] == shapes[1] == shapes[3]  # cached, applied before the second gradient
+    np.testing.assert_array_equal(out[0], out[1])
