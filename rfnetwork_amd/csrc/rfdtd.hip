@@ -133,9 +133,11 @@ struct InstDev {
     const uint32_t* hidx;    // per-cell H entry
     const Vec4<T>* etab;     // 3 Vec4 per E entry
     const Vec4<T>* htab;     // 5 Vec4 per H entry (18 values + 2 pad)
-    const uint32_t* ebits;   // fused step: one bit per cell holding an E probe
-    const void* egroups;     // fused step: the E probe groups, sorted by cell
-    int32_t coef_kind, Nx, Ny, Nz, xplanes, npb, tz_n, negroups;
+    const uint32_t* ebits;   // one bit per cell holding an E probe
+    const void* egroups;     // the E probe groups, sorted by cell
+    const uint32_t* hbits;   // separate kernels: one bit per cell holding an H probe
+    const void* hgroups;     // the H probe groups, sorted by cell
+    int32_t coef_kind, Nx, Ny, Nz, xplanes, npb, tz_n, negroups, nhgroups, pad_;
 };
 
 // A member (grid) owns tiles * K logical workgroups starting at tile0; its K instances are
@@ -284,6 +286,99 @@ __device__ __forceinline__ const InstDev<T>* locate(const InstDev<T>* __restrict
 // with a scalar base and a 32-bit vector offset.
 #define GLOBAL_T(T) GLOBAL_AS T
 
+// Probes sharing one (component, cell) of one instance form a group processed by one thread in
+// list order.  series is the instance's time-major [Nt][nprobes] array.
+template <typename T>
+struct ProbeGroup {
+    int64_t cell;
+    T* series;              // in/out: source samples in, sampled totals out
+    const T* series_in;     // pristine copy of the source samples (recomputation)
+    const int32_t* order;   // instance-local probe ids of the group, in list order
+    const int32_t* is_src;  // per instance-local probe id
+    int32_t inst, field, count, nprobes;
+};
+
+__device__ __forceinline__ int step_of(const int* base, int local) { return base ? (*base + local) : local; }
+
+// Sources and records of the probe groups at one cell (groups sorted by cell, stably, so groups
+// keep list order) applied to the six split values of E, or of H, just computed there: the
+// operations k_probes performs, in its order, with the source samples from the pristine series.
+template <typename T>
+__device__ __forceinline__ void inject_cell(const ProbeGroup<T>* __restrict__ g, int ng, int64_t cell, int n,
+                                            bool record, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) {
+    int lo = 0, hi = ng;
+    while (lo < hi) {  // first group at this cell
+        const int mid = (lo + hi) >> 1;
+        if (g[mid].cell < cell) lo = mid + 1;
+        else hi = mid;
+    }
+    for (int k = lo; k < ng && g[k].cell == cell; ++k) {
+        const ProbeGroup<T> G = g[k];
+        const int c = G.field % 3;  // x, y or z component of E or H
+        T& a = c == 0 ? xy : c == 1 ? yz : zx;
+        T& b = c == 0 ? xz : c == 1 ? yx : zy;
+        T s = a + b;
+        T* row = G.series + (size_t)n * G.nprobes;
+        const T* row_in = G.series_in + (size_t)n * G.nprobes;
+        for (int j = 0; j < G.count; ++j) {
+            const int pid = G.order[j];
+            if (G.is_src[pid]) {
+                const T v = row_in[pid];
+                a = a + v;
+                b = b + v;
+                s = a + b;
+            }
+            if (record) row[pid] = s;
+        }
+    }
+}
+
+// The probes of one field (E or H) folded into its update kernel: a bit per cell marks probe
+// cells, whose groups (sorted by cell) are applied to the values before they are stored.
+// ngroups 0: nothing folded (tuning, recomputation, or no probes).
+template <typename T>
+struct ProbeFold {
+    const uint32_t* bits;
+    const ProbeGroup<T>* groups;
+    int ngroups, n;
+    __device__ __forceinline__ void apply(uint32_t cell, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) const {
+        if (!ngroups) return;
+        if ((ldg((const GLOBAL_AS uint32_t*)bits, (cell >> 5) * 4u) >> (cell & 31)) & 1u)
+            inject_cell(groups, ngroups, (int64_t)cell, n, true, xy, xz, yz, yx, zx, zy);
+    }
+};
+
+// Adjoint sources folded into the reverse kernels: for every group at a marked cell, g[n] summed
+// over the group's probes (series holds dJ/dy, time-major) and added to both splits of its
+// component, as k_adj_inject does.  ngroups 0: nothing folded.
+template <typename T>
+struct AdjFold {
+    const uint32_t* bits;
+    const ProbeGroup<T>* groups;
+    int ngroups, n;
+    __device__ __forceinline__ void apply(uint32_t cell, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) const {
+        if (!ngroups) return;
+        if (!((ldg((const GLOBAL_AS uint32_t*)bits, (cell >> 5) * 4u) >> (cell & 31)) & 1u)) return;
+        int lo = 0, hi = ngroups;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (groups[mid].cell < (int64_t)cell) lo = mid + 1;
+            else hi = mid;
+        }
+        for (int k = lo; k < ngroups && groups[k].cell == (int64_t)cell; ++k) {
+            const ProbeGroup<T> G = groups[k];
+            const T* row = G.series + (size_t)n * G.nprobes;
+            T sum = T(0);
+            for (int j = 0; j < G.count; ++j) sum = sum + row[G.order[j]];
+            const int c = G.field % 3;
+            T& a = c == 0 ? xy : c == 1 ? yz : zx;
+            T& b = c == 0 ? xz : c == 1 ? yx : zy;
+            a = a + sum;
+            b = b + sum;
+        }
+    }
+};
+
 // Everything one x plane of the E update reads, so that the next plane's loads can be issued
 // before this plane's arithmetic and stores (PF = true: two planes of loads in flight per lane).
 template <typename T>
@@ -303,7 +398,8 @@ struct HIn {
 // E half step over one column of x planes.  Offsets are 32-bit byte offsets (grids up to 2^29
 // cells).
 template <typename T, class CA, bool PF, bool NT>
-__device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+__device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                         const ProbeFold<T>& P) {
     using G = GLOBAL_T(T);
     G* const ex_y = (G*)F.ex_y; G* const ex_z = (G*)F.ex_z;
     G* const ey_z = (G*)F.ey_z; G* const ey_x = (G*)F.ey_x;
@@ -342,6 +438,7 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
         if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (cur.hz_xn - hz_c);
         if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (cur.hy_xn - hy_c);
         if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (cur.hx_yn - cur.hx_c);
+        P.apply(b / S, exy, exz, eyz, eyx, ezx, ezy);
         stf<NT>(ex_y, b, exy); stf<NT>(ex_z, b, exz); stf<NT>(ey_z, b, eyz);
         stf<NT>(ey_x, b, eyx); stf<NT>(ez_x, b, ezx); stf<NT>(ez_y, b, ezy);
         stf<NT>(ex, b, exy + exz);
@@ -358,7 +455,8 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
 
 // H half step over one column of x planes; neighbours below index 0 are zero.
 template <typename T, class CA, bool PF, bool NT>
-__device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+__device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                         const ProbeFold<T>& P) {
     using G = GLOBAL_T(T);
     G* const hx_y = (G*)F.hx_y; G* const hx_z = (G*)F.hx_z;
     G* const hy_z = (G*)F.hy_z; G* const hy_x = (G*)F.hy_x;
@@ -390,12 +488,13 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
         HIn<T> nxt;
         if (PF && more) load(nxt, b + sx);
         const HCoef<T>& c = cur.c;
-        const T hxy = c.da_xy * cur.hxy + (c.db_xy2 * cur.ez_c - c.db_xy1 * cur.ez_ym);
-        const T hxz = c.da_xz * cur.hxz + (c.db_xz2 * cur.ey_c - c.db_xz1 * cur.ey_zm);
-        const T hyz = c.da_yz * cur.hyz + (c.db_yz2 * cur.ex_c - c.db_yz1 * cur.ex_zm);
-        const T hyx = c.da_yx * cur.hyx + (c.db_yx2 * cur.ez_c - c.db_yx1 * ez_xm);
-        const T hzx = c.da_zx * cur.hzx + (c.db_zx2 * cur.ey_c - c.db_zx1 * ey_xm);
-        const T hzy = c.da_zy * cur.hzy + (c.db_zy2 * cur.ex_c - c.db_zy1 * cur.ex_ym);
+        T hxy = c.da_xy * cur.hxy + (c.db_xy2 * cur.ez_c - c.db_xy1 * cur.ez_ym);
+        T hxz = c.da_xz * cur.hxz + (c.db_xz2 * cur.ey_c - c.db_xz1 * cur.ey_zm);
+        T hyz = c.da_yz * cur.hyz + (c.db_yz2 * cur.ex_c - c.db_yz1 * cur.ex_zm);
+        T hyx = c.da_yx * cur.hyx + (c.db_yx2 * cur.ez_c - c.db_yx1 * ez_xm);
+        T hzx = c.da_zx * cur.hzx + (c.db_zx2 * cur.ey_c - c.db_zx1 * ey_xm);
+        T hzy = c.da_zy * cur.hzy + (c.db_zy2 * cur.ex_c - c.db_zy1 * cur.ex_ym);
+        P.apply(b / S, hxy, hxz, hyz, hyx, hzx, hzy);
         stf<NT>(hx_y, b, hxy); stf<NT>(hx_z, b, hxz); stf<NT>(hy_z, b, hyz);
         stf<NT>(hy_x, b, hyx); stf<NT>(hz_x, b, hzx); stf<NT>(hz_y, b, hzy);
         stf<NT>(hx, b, hxy + hxz);
@@ -410,45 +509,39 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
     }
 }
 
+// io != 0: the instance's E (H) probes are applied inside the kernel at step base/local.
 template <typename T, bool PF, bool NT = false>
 __global__ void __launch_bounds__(1024) k_update_e(const InstDev<T>* __restrict__ insts,
-                                                   const MemberDev* __restrict__ mem, int nmem, int total) {
+                                                   const MemberDev* __restrict__ mem, int nmem, int total,
+                                                   const int* base, int local, int io) {
     __shared__ int s_member;
     Tile t;
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
+    const int ng = io ? I->negroups : 0;
+    const ProbeFold<T> P{I->ebits, (const ProbeGroup<T>*)I->egroups, ng, ng ? step_of(base, local) : 0};
     if (I->coef_kind == 1)
-        e_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+        e_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t, P);
     else
-        e_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        e_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
 }
 
 template <typename T, bool PF, bool NT = false>
 __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict__ insts,
-                                                   const MemberDev* __restrict__ mem, int nmem, int total) {
+                                                   const MemberDev* __restrict__ mem, int nmem, int total,
+                                                   const int* base, int local, int io) {
     __shared__ int s_member;
     Tile t;
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
+    const int ng = io ? I->nhgroups : 0;
+    const ProbeFold<T> P{I->hbits, (const ProbeGroup<T>*)I->hgroups, ng, ng ? step_of(base, local) : 0};
     if (I->coef_kind == 1)
-        h_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t);
+        h_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t, P);
     else
-        h_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        h_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
 }
 
-// Probes sharing one (component, cell) of one instance form a group processed by one thread in
-// list order.  series is the instance's time-major [Nt][nprobes] array.
-template <typename T>
-struct ProbeGroup {
-    int64_t cell;
-    T* series;              // in/out: source samples in, sampled totals out
-    const T* series_in;     // pristine copy of the source samples (recomputation)
-    const int32_t* order;   // instance-local probe ids of the group, in list order
-    const int32_t* is_src;  // per instance-local probe id
-    int32_t inst, field, count, nprobes;
-};
-
-__device__ __forceinline__ int step_of(const int* base, int local) { return base ? (*base + local) : local; }
 
 // recompute = true replays a step from the pristine source samples and records nothing.
 template <typename T>
@@ -508,31 +601,7 @@ constexpr int kFz = 62;  // interior z cells per tile (64 lanes minus the two z 
 template <typename T>
 __device__ __forceinline__ void inject_e(const InstDev<T>& I, int64_t cell, int n, bool record, T& exy, T& exz,
                                          T& eyz, T& eyx, T& ezx, T& ezy) {
-    const ProbeGroup<T>* g = (const ProbeGroup<T>*)I.egroups;
-    int lo = 0, hi = I.negroups;
-    while (lo < hi) {  // first group at this cell
-        const int mid = (lo + hi) >> 1;
-        if (g[mid].cell < cell) lo = mid + 1;
-        else hi = mid;
-    }
-    for (int k = lo; k < I.negroups && g[k].cell == cell; ++k) {
-        const ProbeGroup<T> G = g[k];
-        T &a = G.field == EX ? exy : G.field == EY ? eyz : ezx;
-        T &b = G.field == EX ? exz : G.field == EY ? eyx : ezy;
-        T s = a + b;
-        T* row = G.series + (size_t)n * G.nprobes;
-        const T* row_in = G.series_in + (size_t)n * G.nprobes;
-        for (int j = 0; j < G.count; ++j) {
-            const int pid = G.order[j];
-            if (G.is_src[pid]) {
-                const T v = row_in[pid];
-                a = a + v;
-                b = b + v;
-                s = a + b;
-            }
-            if (record) row[pid] = s;
-        }
-    }
+    inject_cell((const ProbeGroup<T>*)I.egroups, I.negroups, cell, n, record, exy, exz, eyz, eyx, ezx, ezy);
 }
 
 template <typename T, int TY, class CA>
@@ -731,7 +800,8 @@ __device__ __forceinline__ int item_of(const int64_t* __restrict__ blk0, int n, 
 // Ca scaling of the next step's e_column).  Marches x upward and carries the x+1 plane's H
 // coefficients and the two mu splits that read it.
 template <typename T, class CA>
-__device__ __forceinline__ void adj_e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+__device__ __forceinline__ void adj_e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                             const AdjFold<T>& P) {
     using G = GLOBAL_T(T);
     constexpr uint32_t S = (uint32_t)sizeof(T);
     const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
@@ -766,12 +836,15 @@ __device__ __forceinline__ void adj_e_column(const Fields<T>& F, const CA& C, in
         if (zin) ex = ex - C.at(b + S, C_DB_YZ1) * ldg(myz, b + S);
         if (yin) ex = ex - C.at(b + sy, C_DB_ZY1) * ldg(mzy, b + sy);
         const ECoef<T> ec = C.e(b);
-        stg(nxy, b, (yin ? ec.ca_xy : T(1)) * ldg(nxy, b) + ex);
-        stg(nxz, b, (zin ? ec.ca_xz : T(1)) * ldg(nxz, b) + ex);
-        stg(nyz, b, (zin ? ec.ca_yz : T(1)) * ldg(nyz, b) + ey);
-        stg(nyx, b, (xin ? ec.ca_yx : T(1)) * ldg(nyx, b) + ey);
-        stg(nzx, b, (xin ? ec.ca_zx : T(1)) * ldg(nzx, b) + ez);
-        stg(nzy, b, (yin ? ec.ca_zy : T(1)) * ldg(nzy, b) + ez);
+        T o_xy = (yin ? ec.ca_xy : T(1)) * ldg(nxy, b) + ex;
+        T o_xz = (zin ? ec.ca_xz : T(1)) * ldg(nxz, b) + ex;
+        T o_yz = (zin ? ec.ca_yz : T(1)) * ldg(nyz, b) + ey;
+        T o_yx = (xin ? ec.ca_yx : T(1)) * ldg(nyx, b) + ey;
+        T o_zx = (xin ? ec.ca_zx : T(1)) * ldg(nzx, b) + ez;
+        T o_zy = (yin ? ec.ca_zy : T(1)) * ldg(nzy, b) + ez;
+        P.apply(b / S, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
+        stg(nxy, b, o_xy); stg(nxz, b, o_xz); stg(nyz, b, o_yz);
+        stg(nyx, b, o_yx); stg(nzx, b, o_zx); stg(nzy, b, o_zy);
         hc = hn;
         m_yx = n_yx;
         m_zx = n_zx;
@@ -781,7 +854,8 @@ __device__ __forceinline__ void adj_e_column(const Fields<T>& F, const CA& C, in
 // Reverse H half step over one column (the transpose of e_column followed by the Da scaling of the
 // next step's h_column).  Marches x upward and carries Cb * nu of the x-1 plane's two x splits.
 template <typename T, class CA>
-__device__ __forceinline__ void adj_h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+__device__ __forceinline__ void adj_h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                             const AdjFold<T>& P) {
     using G = GLOBAL_T(T);
     constexpr uint32_t S = (uint32_t)sizeof(T);
     const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
@@ -817,12 +891,15 @@ __device__ __forceinline__ void adj_h_column(const Fields<T>& F, const CA& C, in
         if (yin) hx = hx - ec.cb_zy * v_zy;
         if (ylo) hx = hx + C.at(b - sy, C_CB_ZY) * ldg(nzy, b - sy);
         const HCoef<T> hc = C.h(b);
-        stg(mxy, b, hc.da_xy * ldg(mxy, b) + hx);
-        stg(mxz, b, hc.da_xz * ldg(mxz, b) + hx);
-        stg(myz, b, hc.da_yz * ldg(myz, b) + hy);
-        stg(myx, b, hc.da_yx * ldg(myx, b) + hy);
-        stg(mzx, b, hc.da_zx * ldg(mzx, b) + hz);
-        stg(mzy, b, hc.da_zy * ldg(mzy, b) + hz);
+        T o_xy = hc.da_xy * ldg(mxy, b) + hx;
+        T o_xz = hc.da_xz * ldg(mxz, b) + hx;
+        T o_yz = hc.da_yz * ldg(myz, b) + hy;
+        T o_yx = hc.da_yx * ldg(myx, b) + hy;
+        T o_zx = hc.da_zx * ldg(mzx, b) + hz;
+        T o_zy = hc.da_zy * ldg(mzy, b) + hz;
+        P.apply(b / S, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
+        stg(mxy, b, o_xy); stg(mxz, b, o_xz); stg(myz, b, o_yz);
+        stg(myx, b, o_yx); stg(mzx, b, o_zx); stg(mzy, b, o_zy);
         p_yx = ec.cb_yx * v_yx;
         p_zx = ec.cb_zx * v_zx;
     }
@@ -847,7 +924,8 @@ struct AEKeys {
 };
 
 template <typename T, class CA>
-__device__ __forceinline__ void adj_e_column_pf(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+__device__ __forceinline__ void adj_e_column_pf(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                                const AdjFold<T>& P) {
     using G = GLOBAL_T(T);
     constexpr uint32_t S = (uint32_t)sizeof(T);
     const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
@@ -909,12 +987,15 @@ __device__ __forceinline__ void adj_e_column_pf(const Fields<T>& F, const CA& C,
         T ex = hc.db_yz2 * cur.m_yz + hc.db_zy2 * cur.m_zy;
         if (zin) ex = ex - cur.d_yz1_z * cur.m_yz_z;
         if (yin) ex = ex - cur.d_zy1_y * cur.m_zy_y;
-        stg(nxy, b, (yin ? cur.ca_xy : T(1)) * cur.v_xy + ex);
-        stg(nxz, b, (zin ? cur.ca_xz : T(1)) * cur.v_xz + ex);
-        stg(nyz, b, (zin ? cur.ca_yz : T(1)) * cur.v_yz + ey);
-        stg(nyx, b, (xin ? cur.ca_yx : T(1)) * cur.v_yx + ey);
-        stg(nzx, b, (xin ? cur.ca_zx : T(1)) * cur.v_zx + ez);
-        stg(nzy, b, (yin ? cur.ca_zy : T(1)) * cur.v_zy + ez);
+        T o_xy = (yin ? cur.ca_xy : T(1)) * cur.v_xy + ex;
+        T o_xz = (zin ? cur.ca_xz : T(1)) * cur.v_xz + ex;
+        T o_yz = (zin ? cur.ca_yz : T(1)) * cur.v_yz + ey;
+        T o_yx = (xin ? cur.ca_yx : T(1)) * cur.v_yx + ey;
+        T o_zx = (xin ? cur.ca_zx : T(1)) * cur.v_zx + ez;
+        T o_zy = (yin ? cur.ca_zy : T(1)) * cur.v_zy + ez;
+        P.apply(b / S, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
+        stg(nxy, b, o_xy); stg(nxz, b, o_xz); stg(nyz, b, o_yz);
+        stg(nyx, b, o_yx); stg(nzx, b, o_zx); stg(nzy, b, o_zy);
         hc = cur.hn;
         m_yx = cur.n_yx;
         m_zx = cur.n_zx;
@@ -935,7 +1016,8 @@ struct AHKeys {
 };
 
 template <typename T, class CA>
-__device__ __forceinline__ void adj_h_column_pf(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t) {
+__device__ __forceinline__ void adj_h_column_pf(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                                const AdjFold<T>& P) {
     using G = GLOBAL_T(T);
     constexpr uint32_t S = (uint32_t)sizeof(T);
     const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
@@ -1007,49 +1089,56 @@ __device__ __forceinline__ void adj_h_column_pf(const Fields<T>& F, const CA& C,
         if (zlo) hx = hx + cur.w_yz_z;
         if (yin) hx = hx - cur.cb_zy * cur.v_zy;
         if (ylo) hx = hx + cur.w_zy_y;
-        stg(mxy, b, cur.da_xy * cur.m_xy + hx);
-        stg(mxz, b, cur.da_xz * cur.m_xz + hx);
-        stg(myz, b, cur.da_yz * cur.m_yz + hy);
-        stg(myx, b, cur.da_yx * cur.m_yx + hy);
-        stg(mzx, b, cur.da_zx * cur.m_zx + hz);
-        stg(mzy, b, cur.da_zy * cur.m_zy + hz);
+        T o_xy = cur.da_xy * cur.m_xy + hx;
+        T o_xz = cur.da_xz * cur.m_xz + hx;
+        T o_yz = cur.da_yz * cur.m_yz + hy;
+        T o_yx = cur.da_yx * cur.m_yx + hy;
+        T o_zx = cur.da_zx * cur.m_zx + hz;
+        T o_zy = cur.da_zy * cur.m_zy + hz;
+        P.apply(b / S, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
+        stg(mxy, b, o_xy); stg(mxz, b, o_xz); stg(myz, b, o_yz);
+        stg(myx, b, o_yx); stg(mzx, b, o_zx); stg(mzy, b, o_zy);
         p_yx = cur.cb_yx * cur.v_yx;
         p_zx = cur.cb_zx * cur.v_zx;
         if (more) cur = nxt;
     }
 }
 
+// n >= 0: the adjoint sources of step n (E groups for k_adj_e, H groups for k_adj_h, from the
+// instance descriptor's sorted groups) are added where the values are computed.
 template <typename T, bool PF>
 __global__ void __launch_bounds__(1024) k_adj_e(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
-                                                int nmem, int total) {
+                                                int nmem, int total, int n) {
     __shared__ int s_member;
     Tile t;
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
+    const AdjFold<T> P{I->ebits, (const ProbeGroup<T>*)I->egroups, n >= 0 ? I->negroups : 0, n};
     if (I->coef_kind == 1) {
         const CoefTable<T> C{I->eidx, I->hidx, I->etab, I->htab};
-        if (PF) adj_e_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
-        else adj_e_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
+        if (PF) adj_e_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
+        else adj_e_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
     } else {
-        if (PF) adj_e_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
-        else adj_e_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        if (PF) adj_e_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+        else adj_e_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
     }
 }
 
 template <typename T, bool PF>
 __global__ void __launch_bounds__(1024) k_adj_h(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
-                                                int nmem, int total) {
+                                                int nmem, int total, int n) {
     __shared__ int s_member;
     Tile t;
     const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
     if (!I) return;
+    const AdjFold<T> P{I->hbits, (const ProbeGroup<T>*)I->hgroups, n >= 0 ? I->nhgroups : 0, n};
     if (I->coef_kind == 1) {
         const CoefTable<T> C{I->eidx, I->hidx, I->etab, I->htab};
-        if (PF) adj_h_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
-        else adj_h_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t);
+        if (PF) adj_h_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
+        else adj_h_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
     } else {
-        if (PF) adj_h_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
-        else adj_h_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t);
+        if (PF) adj_h_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+        else adj_h_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
     }
 }
 
@@ -1413,9 +1502,12 @@ struct Instance {
     void* d_fields = nullptr;
     void* d_fields2 = nullptr;  // the second buffer of the fused step's pair
     size_t fields_bytes = 0, pitch = 0;
-    uint32_t* d_ebits = nullptr;  // fused step: E-probe cells
-    void* d_egroups = nullptr;    // fused step: E probe groups sorted by cell
+    uint32_t* d_ebits = nullptr;  // E-probe cells (one bit per cell)
+    void* d_egroups = nullptr;    // E probe groups sorted by cell
     int negroups = 0;
+    uint32_t* d_hbits = nullptr;  // the same for the H probes
+    void* d_hgroups = nullptr;
+    int nhgroups = 0;
     // probes
     int P = 0;
     void* d_series = nullptr;
@@ -1521,9 +1613,10 @@ void Session::release_probes(Instance& I) {
     if (I.d_order) (void)hipFree(I.d_order);
     if (I.d_is_src) (void)hipFree(I.d_is_src);
     I.d_series = nullptr; I.d_order = nullptr; I.d_is_src = nullptr;
-    if (I.d_ebits) (void)hipFree(I.d_ebits);
-    if (I.d_egroups) (void)hipFree(I.d_egroups);
+    for (void* p : {(void*)I.d_ebits, I.d_egroups, (void*)I.d_hbits, I.d_hgroups})
+        if (p) (void)hipFree(p);
     I.d_ebits = nullptr; I.d_egroups = nullptr; I.negroups = 0;
+    I.d_hbits = nullptr; I.d_hgroups = nullptr; I.nhgroups = 0;
     I.P = 0; I.ngE = 0;
     I.groups.clear();
     I.probe_host_values.clear();
@@ -1716,32 +1809,41 @@ std::vector<int64_t> block_offsets(const std::vector<int64_t>& sizes) {
     return off;
 }
 
-// Fused step: the instance's E probe groups sorted by cell, and a bitmask of their cells.
+// Probe groups of the instance sorted by cell, E and H apart, with a bitmask of their cells: the
+// update kernels (and the fused pass, for E) apply them where the values are computed.
 template <typename T>
-void build_e_lookup(Session& s, Instance& I, const Member& M) {
-    if (!s.fused || I.d_ebits || I.ngE == 0) return;
-    std::vector<ProbeGroup<T>> g;
-    for (int k = 0; k < I.ngE; ++k) {
-        const auto& hg = I.groups[k];
-        ProbeGroup<T> pg{};
-        pg.cell = hg.cell;
-        pg.series = (T*)I.d_series;
-        pg.series_in = (const T*)I.d_series_in;
-        pg.order = I.d_order + hg.start;
-        pg.is_src = I.d_is_src;
-        pg.field = hg.field;
-        pg.count = hg.count;
-        pg.nprobes = I.P;
-        g.push_back(pg);
+void build_probe_lookup(Instance& I, const Member& M) {
+    const int nG = (int)I.groups.size();
+    for (int h = 0; h < 2; ++h) {
+        const int g0 = h ? I.ngE : 0, g1 = h ? nG : I.ngE;
+        uint32_t*& d_bits = h ? I.d_hbits : I.d_ebits;
+        void*& d_groups = h ? I.d_hgroups : I.d_egroups;
+        int& count = h ? I.nhgroups : I.negroups;
+        if (d_bits || g1 <= g0) continue;
+        std::vector<ProbeGroup<T>> g;
+        for (int k = g0; k < g1; ++k) {
+            const auto& hg = I.groups[k];
+            ProbeGroup<T> pg{};
+            pg.cell = hg.cell;
+            pg.series = (T*)I.d_series;
+            pg.series_in = (const T*)I.d_series_in;
+            pg.order = I.d_order + hg.start;
+            pg.is_src = I.d_is_src;
+            pg.field = hg.field;
+            pg.count = hg.count;
+            pg.nprobes = I.P;
+            g.push_back(pg);
+        }
+        std::stable_sort(g.begin(), g.end(),
+                         [](const ProbeGroup<T>& a, const ProbeGroup<T>& b) { return a.cell < b.cell; });
+        std::vector<uint32_t> bits((size_t)(M.cells + 31) / 32, 0u);
+        for (const auto& pg : g) bits[(size_t)(pg.cell >> 5)] |= 1u << (pg.cell & 31);
+        HIP_OK(hipMalloc(&d_bits, bits.size() * 4));
+        HIP_OK(hipMemcpy(d_bits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
+        HIP_OK(hipMalloc(&d_groups, g.size() * sizeof(ProbeGroup<T>)));
+        HIP_OK(hipMemcpy(d_groups, g.data(), g.size() * sizeof(ProbeGroup<T>), hipMemcpyHostToDevice));
+        count = (int)g.size();
     }
-    std::stable_sort(g.begin(), g.end(), [](const ProbeGroup<T>& a, const ProbeGroup<T>& b) { return a.cell < b.cell; });
-    std::vector<uint32_t> bits((size_t)(M.cells + 31) / 32, 0u);
-    for (const auto& pg : g) bits[(size_t)(pg.cell >> 5)] |= 1u << (pg.cell & 31);
-    HIP_OK(hipMalloc(&I.d_ebits, bits.size() * 4));
-    HIP_OK(hipMemcpy(I.d_ebits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
-    HIP_OK(hipMalloc(&I.d_egroups, g.size() * sizeof(ProbeGroup<T>)));
-    HIP_OK(hipMemcpy(I.d_egroups, g.data(), g.size() * sizeof(ProbeGroup<T>), hipMemcpyHostToDevice));
-    I.negroups = (int)g.size();
 }
 
 // Kernel descriptor of a field buffer (18 arrays at the given pitch) on member M's grid.
@@ -1791,13 +1893,16 @@ void refresh(Session& s) {
         tile0 += npb * chunks * M.K;
         for (int e = 0; e < M.K; ++e) {
             Instance& I = s.insts[M.inst0 + e];
-            build_e_lookup<T>(s, I, M);
+            build_probe_lookup<T>(I, M);
             di[M.inst0 + e] = make_instdev<T>(M, I.d_fields, I.pitch, npb, I.d_fields2);
             di2[M.inst0 + e] = I.d_fields2 ? make_instdev<T>(M, I.d_fields2, I.pitch, npb, I.d_fields) : di[M.inst0 + e];
             for (InstDev<T>* d : {&di[M.inst0 + e], &di2[M.inst0 + e]}) {
                 d->ebits = I.d_ebits;
                 d->egroups = I.d_egroups;
                 d->negroups = I.negroups;
+                d->hbits = I.d_hbits;
+                d->hgroups = I.d_hgroups;
+                d->nhgroups = I.nhgroups;
             }
         }
     }
@@ -1945,25 +2050,29 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
         s.totals_stale = true;
         di = desc[s.parity];
     } else {
+        // E and H probes (sources and records) are applied inside the update kernels
+        const int pio = io ? 1 : 0;
         if (ev) HIP_OK(hipEventRecord(ev[0], st));
         if (s.nontemporal)
-            k_update_e<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+            k_update_e<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         else if (s.pipelined)
-            k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+            k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         else
-            k_update_e<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
-        if (ev) HIP_OK(hipEventRecord(ev[1], st));
-        if (io && s.ngE) k_probes<T><<<(s.ngE + 63) / 64, 64, 0, st>>>(di, groups, s.ngE, base, local, 0);
-        if (ev) HIP_OK(hipEventRecord(ev[2], st));
+            k_update_e<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
+        if (ev) {
+            HIP_OK(hipEventRecord(ev[1], st));
+            HIP_OK(hipEventRecord(ev[2], st));
+        }
         if (s.nontemporal)
-            k_update_h<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+            k_update_h<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         else if (s.pipelined)
-            k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+            k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         else
-            k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles);
+            k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         if (ev) HIP_OK(hipEventRecord(ev[3], st));
     }
-    if (io && s.ngH) k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local, 0);
+    if (s.fused && io && s.ngH)
+        k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local, 0);
     if (io && s.n_active) {
         k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local, slot, s.d_rows);
         if (flush && s.flush_blocks)
@@ -2574,20 +2683,44 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         }
         return tiles;
     };
+    // each item's adjoint sources sorted by cell, with a bit per cell, for the reverse kernels
+    for (int k = 0; k < n; ++k) {
+        const Member& M = s.members[s.insts[items[k].inst].member];
+        for (int h = 0; h < 2; ++h) {
+            std::vector<ProbeGroup<T>> g;
+            for (const auto& pg : h ? aH : aE)
+                if (pg.inst == k) g.push_back(pg);
+            if (g.empty()) continue;
+            std::stable_sort(g.begin(), g.end(),
+                             [](const ProbeGroup<T>& a, const ProbeGroup<T>& b) { return a.cell < b.cell; });
+            std::vector<uint32_t> bits((size_t)(M.cells + 31) / 32, 0u);
+            for (const auto& pg : g) bits[(size_t)(pg.cell >> 5)] |= 1u << (pg.cell & 31);
+            const uint32_t* d_bits = (const uint32_t*)upload(bits.data(), bits.size() * 4);
+            const void* d_grp = upload(g.data(), g.size() * sizeof(ProbeGroup<T>));
+            HIP_OK(hipStreamSynchronize(st));  // the copies read bits and g
+            InstDev<T>& d = di[n + k];
+            if (h) {
+                d.hbits = d_bits; d.hgroups = d_grp; d.nhgroups = (int)g.size();
+            } else {
+                d.ebits = d_bits; d.egroups = d_grp; d.negroups = (int)g.size();
+            }
+        }
+    }
     int ablock = s.adj_block > 0 ? s.adj_block : s.block;
     int atiles = adj_tiling(ablock, s.adj_block > 0 ? s.adj_chunks : 0);
     const InstDev<T>* d_di = (const InstDev<T>*)upload(di.data(), di.size() * sizeof(InstDev<T>));
     const MemberDev* d_md = (const MemberDev*)upload(md.data(), md.size() * sizeof(MemberDev));
     MemberDev* d_mda = (MemberDev*)upload(mda.data(), mda.size() * sizeof(MemberDev));
-    auto adj_e = [&](int pf, int block, int tiles) {
+    // step >= 0: the kernel also adds that step's adjoint sources (E for adj_e, H for adj_h)
+    auto adj_e = [&](int pf, int block, int tiles, int step) {
         const int g = (tiles + 7) & ~7;
-        if (pf) k_adj_e<T, true><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
-        else k_adj_e<T, false><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
+        if (pf) k_adj_e<T, true><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles, step);
+        else k_adj_e<T, false><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles, step);
     };
-    auto adj_h = [&](int pf, int block, int tiles) {
+    auto adj_h = [&](int pf, int block, int tiles, int step) {
         const int g = (tiles + 7) & ~7;
-        if (pf) k_adj_h<T, true><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
-        else k_adj_h<T, false><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles);
+        if (pf) k_adj_h<T, true><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles, step);
+        else k_adj_h<T, false><<<g, block, 0, st>>>(d_di + n, d_mda, n, tiles, step);
     };
     if (s.adj_block <= 0) {
         HIP_OK(hipStreamSynchronize(st));  // the descriptor uploads above read di and mda
@@ -2608,12 +2741,12 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
                     const int tiles = adj_tiling(blk, ch);
                     HIP_OK(hipMemcpyAsync((void*)(d_di + n), di.data() + n, n * sizeof(InstDev<T>), hipMemcpyHostToDevice, st));
                     HIP_OK(hipMemcpyAsync(d_mda, mda.data(), n * sizeof(MemberDev), hipMemcpyHostToDevice, st));
-                    adj_e(pf, blk, tiles);
-                    adj_h(pf, blk, tiles);
+                    adj_e(pf, blk, tiles, -1);
+                    adj_h(pf, blk, tiles, -1);
                     HIP_OK(hipEventRecord(s.ev0, st));
                     for (int r = 0; r < reps; ++r) {
-                        adj_e(pf, blk, tiles);
-                        adj_h(pf, blk, tiles);
+                        adj_e(pf, blk, tiles, -1);
+                        adj_h(pf, blk, tiles, -1);
                     }
                     HIP_OK(hipEventRecord(s.ev1, st));
                     HIP_OK(hipEventSynchronize(s.ev1));
@@ -2666,20 +2799,21 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         for (int64_t t = n0; t < n1 && !hist_mode; ++t) {
             const int j = (int)(t - n0);
             k_save_h<T><<<hblocks, 256, 0, st>>>(d_di, d_bd, d_h0, n, j);
-            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
-            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
+            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
+            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
             if (ngE) k_probes<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di, d_gr, ngE, nullptr, (int)t, 1);
-            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
-            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0);
+            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
+            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
             if (ngH) k_probes<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di, d_gr + ngE, ngH, nullptr, (int)t, 1);
         }
         for (int64_t t = n1 - 1; t >= n0; --t) {
             const int j = (int)(t - n0);
-            if (ngH) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + n, d_ag + ngE, ngH, (int)t);
-            adj_e(apf, ablock, atiles);
-            if (ngE) k_adj_inject<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di + n, d_ag, ngE, (int)t);
+            // the H adjoint sources of step t were added by the previous reverse step's adj_h (the
+            // last step's here); those of E by adj_e itself
+            if (ngH && t == Nt - 1) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + n, d_ag + ngE, ngH, (int)t);
+            adj_e(apf, ablock, atiles, (int)t);
             k_adj_grad<T><<<gblocks, 256, 0, st>>>(d_di + n, d_bd, d_g0, n, j);
-            adj_h(apf, ablock, atiles);
+            adj_h(apf, ablock, atiles, (int)t - 1);
         }
         HIP_OK(hipGetLastError());
         if ((s.flags & RFDTD_VERBOSE) && (seg % std::max(1, nseg / 10) == 0 || seg == 0)) {
