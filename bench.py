@@ -36,9 +36,19 @@ METRIC = "Mcell-updates/s + % HBM peak, batched vs plain; adjoint gradients/s, 1
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X (KiB per
 # dispatch x 1024, FETCH_SIZE x 2 and WRITE_SIZE x 1: the factors that make the counters of a plain
 # 1 GiB copy kernel read 1 GiB each way); profiles/k_update_eh_dipole_slab_pmc.md
+# HBM bytes per launch of the E and H kernels on the 400x400x200 workload, per launch shape
+# (variant, block, x planes): rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE (KiB, calibrated on k_stream_copy).
 MEASURED_TRAFFIC = {
-    ("dipole_slab", "table", "k_update_e"): 2504854000,
-    ("dipole_slab", "table", "k_update_h"): 2454030000,
+    ("dipole_slab", "table", k, shape): v
+    for shape, (e, h) in {
+        (4, 256, 4): (2543200000, 2501400000),
+        (1, 256, 4): (2572700000, 2501900000),
+        (1, 128, 200): (2497900000, 2455000000),
+        (4, 128, 200): (2490300000, 2448600000),
+        (1, 256, 200): (2500000000, 2456700000),
+        (4, 256, 200): (2493700000, 2452600000),
+    }.items()
+    for k, v in (("k_update_e", e), ("k_update_h", h))
 }
 
 
@@ -403,6 +413,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dipole_slab")
     ap.add_argument("--coef", choices=("auto", "raw", "table"), default="auto")
+    ap.add_argument("--variant", type=int, default=-1, help="kernel variant 0..4 (default: autotuned)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--xplanes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
@@ -462,8 +473,11 @@ def plain_leg(args):
     s = Session(c, Nx, Ny, Nz, flags=flags)
     del c
     tuned = None
-    if args.block or args.xplanes:
+    if args.block or args.xplanes or args.variant >= 0:
         s.set_launch(args.block, args.xplanes, 0)
+        if args.variant >= 0:
+            s.set_variant(args.variant)
+        tuned = dict(variant=args.variant if args.variant >= 0 else 1, block=args.block or 256, x_planes=args.xplanes, cached=None)
     else:
         t1 = time.perf_counter()
         tuned = s.autotune()
@@ -496,7 +510,8 @@ def plain_leg(args):
     kname, kbytes, kms = ("k_step_fused", E_BYTES + H_BYTES, e_ms) if fused else ("k_update_h", H_BYTES, h_ms)
     k_gbs = cells * kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     e_gbs = cells * E_BYTES / (e_ms * 1e-3) / 1e9 if e_ms > 0 and not fused else None
-    traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname))
+    shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
+    traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname, shape))
 
     copy = None
     if not args.no_copy_probe:

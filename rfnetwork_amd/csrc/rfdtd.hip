@@ -42,6 +42,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
@@ -591,9 +592,10 @@ __global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<
 // the y/z neighbours pass through LDS.  A chunk that does not start at x = 0 first computes E on
 // plane x0 - 1 without storing it.  E sources are applied where they are computed (the bitmask
 // marks probe cells; halo copies apply them too, only the owner records), reading the source
-// samples from the pristine series; H sources and monitors follow in their own launches on the
-// written buffer.  Every value is formed by the same operations in the same order as in the
-// separate E and H kernels, so results are identical.
+// samples from the pristine series; H sources and records are applied by the owner of the cell
+// before its H store; monitors follow in their own launch on the written buffer.  Every value is
+// formed by the same operations in the same order as in the separate E and H kernels, so results
+// are identical.
 // ------------------------------------------------------------------------------------------------
 
 constexpr int kFz = 62;  // interior z cells per tile (64 lanes minus the two z halos)
@@ -703,12 +705,17 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
             const HCoef<T>& c = ch;
             const T ez_ym = y > 0 ? sEz[tid - 64] : T(0), ex_ym = y > 0 ? sEx[tid - 64] : T(0);
             const T ey_zm = z > 0 ? sEy[tid - 1] : T(0), ex_zm = z > 0 ? sEx[tid - 1] : T(0);
-            const T xy = c.da_xy * hxy + (c.db_xy2 * ez - c.db_xy1 * ez_ym);
-            const T xz = c.da_xz * hxz + (c.db_xz2 * ey - c.db_xz1 * ey_zm);
-            const T yz = c.da_yz * hyz + (c.db_yz2 * ex - c.db_yz1 * ex_zm);
-            const T yx = c.da_yx * hyx + (c.db_yx2 * ez - c.db_yx1 * ez_xm);
-            const T zx = c.da_zx * hzx + (c.db_zx2 * ey - c.db_zx1 * ey_xm);
-            const T zy = c.da_zy * hzy + (c.db_zy2 * ex - c.db_zy1 * ex_ym);
+            T xy = c.da_xy * hxy + (c.db_xy2 * ez - c.db_xy1 * ez_ym);
+            T xz = c.da_xz * hxz + (c.db_xz2 * ey - c.db_xz1 * ey_zm);
+            T yz = c.da_yz * hyz + (c.db_yz2 * ex - c.db_yz1 * ex_zm);
+            T yx = c.da_yx * hyx + (c.db_yx2 * ez - c.db_yx1 * ez_xm);
+            T zx = c.da_zx * hzx + (c.db_zx2 * ey - c.db_zx1 * ey_xm);
+            T zy = c.da_zy * hzy + (c.db_zy2 * ex - c.db_zy1 * ex_ym);
+            if (io && I.nhgroups) {  // H sources and records where H is computed
+                const int64_t cell = ((int64_t)x * Ny + y) * Nz + z;
+                if ((I.hbits[cell >> 5] >> (cell & 31)) & 1u)
+                    inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell, n, true, xy, xz, yz, yx, zx, zy);
+            }
             stg((G*)B.hx_y, b, xy); stg((G*)B.hx_z, b, xz); stg((G*)B.hy_z, b, yz);
             stg((G*)B.hy_x, b, yx); stg((G*)B.hz_x, b, zx); stg((G*)B.hz_y, b, zy);
         }
@@ -1665,55 +1672,104 @@ inline uint64_t mix64(uint64_t h) {
     return h;
 }
 
-// Collect the distinct K-tuples (compared bit for bit) of arrays[0..K) over all cells.  zero_last_x
-// marks arrays whose x = Nx-1 plane reads as zero.  Gives up (returns false) once more than
-// max_entries distinct tuples exist.
+// Open-addressing set of distinct K-tuples (compared bit for bit), numbered in insertion order.
 template <typename U, int K>
-bool dedup(const U* const* arrays, const bool* zero_last_x, int64_t cells, int64_t plane, int64_t max_entries,
-           std::vector<uint32_t>& idx, std::vector<U>& table) {
-    idx.resize(cells);
-    table.clear();
-    size_t cap = 1 << 12;
-    std::vector<int32_t> slots(cap, -1);
+struct TupleSet {
+    std::vector<U> table;  // K values per entry
+    std::vector<int32_t> slots = std::vector<int32_t>(1 << 12, -1);
     int64_t count = 0;
-    const int64_t last0 = cells - plane;
-    U key[K];
-    auto hash = [](const U* k) {
+    static uint64_t hash(const U* k) {
         uint64_t h = 0x9e3779b97f4a7c15ULL;
         for (int j = 0; j < K; ++j) h = mix64(h ^ (uint64_t)k[j]) + (uint64_t)j;
         return h;
-    };
-    for (int64_t i = 0; i < cells; ++i) {
-        const bool last = i >= last0;
-        for (int k = 0; k < K; ++k) key[k] = (last && zero_last_x[k]) ? U(0) : arrays[k][i];
+    }
+    // entry of key, inserted if new; -1 once more than max_entries would exist
+    int32_t find_or_insert(const U* key, int64_t max_entries) {
+        size_t cap = slots.size();
         size_t s = hash(key) & (cap - 1);
-        int32_t found = -1;
         while (slots[s] >= 0) {
             const U* e = &table[(size_t)slots[s] * K];
             bool eq = true;
             for (int k = 0; k < K; ++k)
                 if (e[k] != key[k]) { eq = false; break; }
-            if (eq) { found = slots[s]; break; }
+            if (eq) return slots[s];
             s = (s + 1) & (cap - 1);
         }
-        if (found < 0) {
-            if (count >= max_entries) return false;
-            found = (int32_t)count++;
-            table.insert(table.end(), key, key + K);
-            slots[s] = found;
-            if ((size_t)count * 2 > cap) {  // grow and rehash
-                cap <<= 1;
-                std::vector<int32_t> ns(cap, -1);
-                for (int64_t e = 0; e < count; ++e) {
-                    size_t t = hash(&table[(size_t)e * K]) & (cap - 1);
-                    while (ns[t] >= 0) t = (t + 1) & (cap - 1);
-                    ns[t] = (int32_t)e;
-                }
-                slots.swap(ns);
+        if (count >= max_entries) return -1;
+        const int32_t id = (int32_t)count++;
+        table.insert(table.end(), key, key + K);
+        slots[s] = id;
+        if ((size_t)count * 2 > cap) {  // grow and rehash
+            cap <<= 1;
+            std::vector<int32_t> ns(cap, -1);
+            for (int64_t e = 0; e < count; ++e) {
+                size_t t = hash(&table[(size_t)e * K]) & (cap - 1);
+                while (ns[t] >= 0) t = (t + 1) & (cap - 1);
+                ns[t] = (int32_t)e;
             }
+            slots.swap(ns);
         }
-        idx[i] = (uint32_t)found;
+        return id;
     }
+};
+
+// Collect the distinct K-tuples of arrays[0..K) over all cells into idx (per cell) and table.
+// zero_last_x marks arrays whose x = Nx-1 plane reads as zero.  Gives up (returns false) once more
+// than max_entries distinct tuples exist.  The cells are split into contiguous ranges deduplicated
+// by separate host threads; merging their sets in range order numbers every tuple by its first
+// cell, exactly as one pass over all cells would.
+template <typename U, int K>
+bool dedup(const U* const* arrays, const bool* zero_last_x, int64_t cells, int64_t plane, int64_t max_entries,
+           std::vector<uint32_t>& idx, std::vector<U>& table) {
+    idx.resize(cells);
+    table.clear();
+    const int64_t last0 = cells - plane;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)hw, 16, cells / (1 << 16)}));
+    std::vector<TupleSet<U, K>> local(nt);
+    std::vector<char> ok(nt, 1);
+    auto range = [&](int t) {
+        return std::make_pair(cells * t / nt, cells * (t + 1) / nt);
+    };
+    auto work = [&](int t) {
+        const auto r = range(t);
+        U key[K];
+        for (int64_t i = r.first; i < r.second; ++i) {
+            const bool last = i >= last0;
+            for (int k = 0; k < K; ++k) key[k] = (last && zero_last_x[k]) ? U(0) : arrays[k][i];
+            const int32_t id = local[t].find_or_insert(key, max_entries);
+            if (id < 0) {
+                ok[t] = 0;
+                return;
+            }
+            idx[i] = (uint32_t)id;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    for (int t = 0; t < nt; ++t)
+        if (!ok[t]) return false;
+    // merge in range order; remap each range's local numbers to the global ones
+    TupleSet<U, K>& all = local[0];
+    std::vector<std::vector<uint32_t>> remap(nt);
+    for (int t = 1; t < nt; ++t) {
+        remap[t].resize(local[t].count);
+        for (int64_t e = 0; e < local[t].count; ++e) {
+            const int32_t id = all.find_or_insert(&local[t].table[(size_t)e * K], max_entries);
+            if (id < 0) return false;
+            remap[t][e] = (uint32_t)id;
+        }
+    }
+    pool.clear();
+    for (int t = 1; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            const auto r = range(t);
+            for (int64_t i = r.first; i < r.second; ++i) idx[i] = remap[t][idx[i]];
+        });
+    for (auto& th : pool) th.join();
+    table.swap(all.table);
     return true;
 }
 
@@ -2029,13 +2085,12 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     const InstDev<T>* desc[2] = {(const InstDev<T>*)s.d_insts, (const InstDev<T>*)s.d_insts2};
     const InstDev<T>* di = desc[s.parity];
     const int nmem = (int)s.members.size();
-    const ProbeGroup<T>* groups = (const ProbeGroup<T>*)s.d_groups;
     const bool io = !s.tuning;
     if (io && s.hist_blocks)  // H^n of the design boxes, before this step's E half step
         k_save_h<T><<<s.hist_blocks, 256, 0, st>>>(di, (const BoxDev<T>*)s.d_hist_boxes, s.d_hist_h0, (int)s.hist.size(),
                                                   local, base);
     if (s.fused) {
-        // one pass writes the other buffer of each pair; H sources and monitors then work on it
+        // one pass writes the other buffer of each pair; monitors then sample it
         if (ev) HIP_OK(hipEventRecord(ev[0], st));
         if (s.fused == 14)
             k_step_fused<T, 14><<<grid, 16 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
@@ -2071,8 +2126,7 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
             k_update_h<T, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         if (ev) HIP_OK(hipEventRecord(ev[3], st));
     }
-    if (s.fused && io && s.ngH)
-        k_probes<T><<<(s.ngH + 63) / 64, 64, 0, st>>>(di, groups + s.ngE, s.ngH, base, local, 0);
+
     if (io && s.n_active) {
         k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local, slot, s.d_rows);
         if (flush && s.flush_blocks)
