@@ -505,7 +505,7 @@ def plain_leg(args):
     dft_b = dft_bytes_per_step(model)
     step_bytes = cells * (E_BYTES + H_BYTES) + dft_b
     step_frac = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
-    fused = bool(tuned and tuned.get("variant", 1) in (2, 3))
+    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6))
     # the dominant kernel: the fused E+H pass (264 B/cell) or the H update (144 B/cell)
     kname, kbytes, kms = ("k_step_fused", E_BYTES + H_BYTES, e_ms) if fused else ("k_update_h", H_BYTES, h_ms)
     k_gbs = cells * kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0

@@ -193,8 +193,9 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * kernels loading one plane at a time; 2 and 3 one fused pass per time step (E and H of a tile of
  * 6 or 14 rows x 62 cells, neighbours through LDS, halos recomputed) that reads one buffer of a
  * per-instance pair and writes the other (the session then holds two field buffers per
- * instance); 4 variant 1 with nontemporal stores of the fields.  Results are identical; the choice
- * only changes speed (the autotuner picks it). */
+ * instance); 4 variant 1 with nontemporal stores of the fields; 5 and 6 variants 2 and 3 with the
+ * next x plane's H splits loaded one plane earlier and the coefficient values issued first.  Results
+ * are identical; the choice only changes speed (the autotuner picks it). */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 
 /* Launch shape of the reverse pass of rfdtd_gradient(_batch) (no counterpart in the reference):

@@ -728,12 +728,169 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
     }
 }
 
-template <typename T, int TY>
+// The same pass with E one plane ahead of H, so that one barrier per plane suffices: iteration i
+// writes plane i's H totals to LDS, waits once, then updates H on plane i - 1 (from plane i - 1's E
+// totals, written to LDS in the previous iteration) and E on plane i (from the H totals just
+// exchanged).  The two LDS exchanges alternate between two buffers by plane parity, so a wave
+// writing plane i + 1's totals never overwrites what a slower wave still reads.  H splits are
+// loaded two planes ahead, E splits and coefficient keys one plane ahead, and the coefficient values
+// of the iteration are issued before its other loads.  Every value is formed as in fused_tile.
+template <typename T>
+struct Split6 {
+    T xy, xz, yz, yx, zx, zy;
+};
+
+template <typename T, int TY, class CA>
+__device__ __forceinline__ void fused_tile_skew(const InstDev<T>& I, const CA& C, int x0, int x1, int y0, int z0,
+                                                int n, bool io, T* sm) {
+    using G = GLOBAL_T(T);
+    constexpr int NT = (TY + 2) * 64;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const int Nx = I.Nx, Ny = I.Ny, Nz = I.Nz;
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int y = y0 - 1 + w, z = z0 - 1 + l;
+    const bool valid = y >= 0 && y < Ny && z >= 0 && z < Nz;
+    const bool ecomp = valid && w <= TY && l <= kFz;  // E computed here
+    const bool inner = ecomp && w >= 1 && l >= 1;     // E and H stored here
+    const bool yin = y < Ny - 1, zin = z < Nz - 1;
+    T* const sH = sm;           // [parity][component][NT] H totals
+    T* const sE = sm + 6 * NT;  // [parity][component][NT] E totals
+    const Fields<T>& A = I.F;
+    const Fields<T>& B = I.G;
+    const int xs = x0 > 0 ? x0 - 1 : x0;  // a chunk not starting at 0 computes E on x0 - 1 first
+    const int xh = min(x1, Nx - 1);       // last plane whose H splits are read
+    const uint32_t p = valid ? ((uint32_t)y * (uint32_t)Nz + (uint32_t)z) : 0u;
+    const uint32_t b0 = ((uint32_t)xs * (uint32_t)Ny * (uint32_t)Nz + p) * S;
+    auto off = [&](int x) { return b0 + (uint32_t)(x - xs) * sx; };
+    auto load_h = [&](int x) {
+        Split6<T> h{};
+        if (valid && x <= xh) {
+            const uint32_t b = off(x);
+            h.xy = ldg((const G*)A.hx_y, b); h.xz = ldg((const G*)A.hx_z, b); h.yz = ldg((const G*)A.hy_z, b);
+            h.yx = ldg((const G*)A.hy_x, b); h.zx = ldg((const G*)A.hz_x, b); h.zy = ldg((const G*)A.hz_y, b);
+        }
+        return h;
+    };
+    auto load_e = [&](int x) {
+        Split6<T> e{};
+        const uint32_t b = off(x);
+        e.xy = ldg((const G*)A.ex_y, b); e.xz = ldg((const G*)A.ex_z, b); e.yz = ldg((const G*)A.ey_z, b);
+        e.yx = ldg((const G*)A.ey_x, b); e.zx = ldg((const G*)A.ez_x, b); e.zy = ldg((const G*)A.ez_y, b);
+        return e;
+    };
+    Split6<T> hA{}, hB = load_h(xs), hC = load_h(xs + 1);  // H splits of planes i - 1, i, i + 1
+    Split6<T> e{};                                          // E splits of plane i
+    uint32_t ek = 0, hk = 0;                                // keys: E of plane i, H of plane i - 1
+    if (ecomp) {
+        e = load_e(xs);
+        ek = C.ekey(off(xs));
+    }
+    T ex1 = T(0), ey1 = T(0), ez1 = T(0);  // E totals of plane i - 1 at this cell
+    T ey2 = T(0), ez2 = T(0);              // E totals of plane i - 2
+    for (int i = xs; i <= x1; ++i) {
+        const bool eplane = i < x1;     // E updated on plane i
+        const bool hplane = i - 1 >= x0;  // H updated on plane i - 1 (not the extra plane x0 - 1)
+        const int par = i & 1;
+        ECoef<T> c{};
+        HCoef<T> ch{};
+        if (eplane && ecomp) c = C.e_at(ek);
+        if (hplane && inner) ch = C.h_at(hk);
+        // the next iteration's loads
+        const Split6<T> hN = load_h(i + 2);
+        Split6<T> eN{};
+        uint32_t ekN = 0, hkN = 0;
+        if (ecomp && i + 1 < x1) {
+            eN = load_e(i + 1);
+            ekN = C.ekey(off(i + 1));
+        }
+        if (inner && i >= x0 && i < x1) hkN = C.hkey(off(i));
+        const T hx_c = hB.xy + hB.xz, hy_c = hB.yz + hB.yx, hz_c = hB.zx + hB.zy;
+        if (eplane) {
+            sH[(par * 3 + 0) * NT + tid] = hx_c;
+            sH[(par * 3 + 1) * NT + tid] = hy_c;
+            sH[(par * 3 + 2) * NT + tid] = hz_c;
+        }
+        __syncthreads();
+        if (hplane && inner) {
+            const int pp = par ^ 1;
+            const T* sEx = sE + (pp * 3 + 0) * NT;
+            const T* sEy = sE + (pp * 3 + 1) * NT;
+            const T* sEz = sE + (pp * 3 + 2) * NT;
+            const T ex = ex1, ey = ey1, ez = ez1, ey_xm = ey2, ez_xm = ez2;
+            const T ez_ym = y > 0 ? sEz[tid - 64] : T(0), ex_ym = y > 0 ? sEx[tid - 64] : T(0);
+            const T ey_zm = z > 0 ? sEy[tid - 1] : T(0), ex_zm = z > 0 ? sEx[tid - 1] : T(0);
+            T xy = ch.da_xy * hA.xy + (ch.db_xy2 * ez - ch.db_xy1 * ez_ym);
+            T xz = ch.da_xz * hA.xz + (ch.db_xz2 * ey - ch.db_xz1 * ey_zm);
+            T yz = ch.da_yz * hA.yz + (ch.db_yz2 * ex - ch.db_yz1 * ex_zm);
+            T yx = ch.da_yx * hA.yx + (ch.db_yx2 * ez - ch.db_yx1 * ez_xm);
+            T zx = ch.da_zx * hA.zx + (ch.db_zx2 * ey - ch.db_zx1 * ey_xm);
+            T zy = ch.da_zy * hA.zy + (ch.db_zy2 * ex - ch.db_zy1 * ex_ym);
+            if (io && I.nhgroups) {  // H sources and records where H is computed
+                const int64_t cell = ((int64_t)(i - 1) * Ny + y) * Nz + z;
+                if ((I.hbits[cell >> 5] >> (cell & 31)) & 1u)
+                    inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell, n, true, xy, xz, yz, yx, zx, zy);
+            }
+            const uint32_t b = off(i - 1);
+            stg((G*)B.hx_y, b, xy); stg((G*)B.hx_z, b, xz); stg((G*)B.hy_z, b, yz);
+            stg((G*)B.hy_x, b, yx); stg((G*)B.hz_x, b, zx); stg((G*)B.hz_y, b, zy);
+        }
+        if (eplane) {
+            T ex = T(0), ey = T(0), ez = T(0);
+            if (ecomp) {
+                const T* sHx = sH + (par * 3 + 0) * NT;
+                const T* sHy = sH + (par * 3 + 1) * NT;
+                const T* sHz = sH + (par * 3 + 2) * NT;
+                const bool xin = i < Nx - 1;
+                const T hz_yn = yin ? sHz[tid + 64] : T(0), hx_yn = yin ? sHx[tid + 64] : T(0);
+                const T hy_zn = zin ? sHy[tid + 1] : T(0), hx_zn = zin ? sHx[tid + 1] : T(0);
+                const T hy_xn = xin ? hC.yz + hC.yx : T(0), hz_xn = xin ? hC.zx + hC.zy : T(0);
+                T exy = e.xy, exz = e.xz, eyz = e.yz, eyx = e.yx, ezx = e.zx, ezy = e.zy;
+                if (yin) exy = c.ca_xy * exy + c.cb_xy * (hz_yn - hz_c);
+                if (zin) exz = c.ca_xz * exz + c.cb_xz * (hy_zn - hy_c);
+                if (zin) eyz = c.ca_yz * eyz + c.cb_yz * (hx_zn - hx_c);
+                if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (hz_xn - hz_c);
+                if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (hy_xn - hy_c);
+                if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (hx_yn - hx_c);
+                const bool own = inner && i >= x0;
+                if (io && I.negroups) {
+                    const int64_t cell = ((int64_t)i * Ny + y) * Nz + z;
+                    if ((I.ebits[cell >> 5] >> (cell & 31)) & 1u)
+                        inject_e(I, cell, n, own, exy, exz, eyz, eyx, ezx, ezy);
+                }
+                ex = exy + exz;
+                ey = eyz + eyx;
+                ez = ezx + ezy;
+                if (own) {
+                    const uint32_t b = off(i);
+                    stg((G*)B.ex_y, b, exy); stg((G*)B.ex_z, b, exz); stg((G*)B.ey_z, b, eyz);
+                    stg((G*)B.ey_x, b, eyx); stg((G*)B.ez_x, b, ezx); stg((G*)B.ez_y, b, ezy);
+                }
+            }
+            sE[(par * 3 + 0) * NT + tid] = ex;
+            sE[(par * 3 + 1) * NT + tid] = ey;
+            sE[(par * 3 + 2) * NT + tid] = ez;
+            ey2 = ey1;
+            ez2 = ez1;
+            ex1 = ex;
+            ey1 = ey;
+            ez1 = ez;
+        }
+        hA = hB;
+        hB = hC;
+        hC = hN;
+        e = eN;
+        ek = ekN;
+        hk = hkN;
+    }
+}
+
+template <typename T, int TY, bool D>
 __global__ void __launch_bounds__((TY + 2) * 64) k_step_fused(const InstDev<T>* __restrict__ insts,
                                                               const MemberDev* __restrict__ mem, int nmem, int total,
                                                               const int* base, int local, int io) {
     __shared__ int s_member;
-    __shared__ T sm[6 * (TY + 2) * 64];
+    __shared__ T sm[(D ? 12 : 6) * (TY + 2) * 64];
     const int L = xcd_logical(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         int m = 0;
@@ -753,10 +910,19 @@ __global__ void __launch_bounds__((TY + 2) * 64) k_step_fused(const InstDev<T>* 
     const int y0 = (pb / tz_n) * TY, z0 = (pb % tz_n) * kFz;
     const int x0 = xc * I->xplanes, x1 = min(I->Nx, x0 + I->xplanes);
     const int n = step_of(base, local);
-    if (I->coef_kind == 1)
-        fused_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0, z0, n, io != 0, sm);
-    else
-        fused_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, sm);
+    if (D) {  // E one plane ahead of H, one barrier per plane
+        if (I->coef_kind == 1)
+            fused_tile_skew<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0, z0,
+                                                 n, io != 0, sm);
+        else
+            fused_tile_skew<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, sm);
+    } else {
+        if (I->coef_kind == 1)
+            fused_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0, z0, n,
+                                            io != 0, sm);
+        else
+            fused_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, sm);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1562,6 +1728,7 @@ struct Session {
     bool tuning = false;    // autotune trials: field updates only, no probes or monitors
     int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
+    bool fused_deep = false;   // fused step with E one plane ahead of H (one barrier per plane)
     // reverse-pass launch shape: pipelined (1) or plain (0) columns, threads per workgroup and x
     // chunks per grid (0: the forward's planes per workgroup); adj_block 0 = timed at the first
     // gradient and kept
@@ -2092,10 +2259,14 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     if (s.fused) {
         // one pass writes the other buffer of each pair; monitors then sample it
         if (ev) HIP_OK(hipEventRecord(ev[0], st));
-        if (s.fused == 14)
-            k_step_fused<T, 14><<<grid, 16 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
+        if (s.fused == 14 && s.fused_deep)
+            k_step_fused<T, 14, true><<<grid, 16 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
+        else if (s.fused == 14)
+            k_step_fused<T, 14, false><<<grid, 16 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
+        else if (s.fused_deep)
+            k_step_fused<T, 6, true><<<grid, 8 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
         else
-            k_step_fused<T, 6><<<grid, 8 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
+            k_step_fused<T, 6, false><<<grid, 8 * 64, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, io);
         if (ev) {
             HIP_OK(hipEventRecord(ev[1], st));
             HIP_OK(hipEventRecord(ev[2], st));
@@ -2284,14 +2455,17 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const int v0 = s.fused == 14 ? 3 : s.fused == 6 ? 2 : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
+    const int v0 = s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
+                 : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
     const int blk0 = s.block;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
-    // variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row tiles, 3 fused 14-row tiles
+    // variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row tiles, 3 fused 14-row tiles,
+    // 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead of H
     auto apply = [&](int v, int blk, const std::vector<int>& xr) {
         s.pipelined = v != 0;
-        s.fused = v == 2 ? 6 : v == 3 ? 14 : 0;
+        s.fused = (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
+        s.fused_deep = v >= 5;
         s.nontemporal = v == 4;
         s.block = blk;
         for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
@@ -2303,10 +2477,11 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     int bv = v0, bblk = blk0, bxp = 0;
     s.tuning = true;
     try {
-        for (int v : {1, 4, 0, 2, 3})
+        for (int v : {1, 4, 0, 2, 3, 5, 6})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
-                    if (v >= 2 && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
+                    const bool fused_v = v == 2 || v == 3 || v >= 5;
+                    if (fused_v && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
                     apply(v, blk, std::vector<int>(s.members.size(), xp));
                     refresh<T>(s);
                     run_chunk<T>(s, std::min(trial_steps, 3), 0, st, false, false, 0);  // warm-up
@@ -3130,9 +3305,10 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 4) RF_THROW(RFDTD_EINVAL, "variant must be 0, 1, 2, 3 or 4");
+        if (variant < 0 || variant > 6) RF_THROW(RFDTD_EINVAL, "variant must be in 0..6");
         S.pipelined = variant != 0;
-        S.fused = variant == 2 ? 6 : variant == 3 ? 14 : 0;
+        S.fused = (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
+        S.fused_deep = variant >= 5;
         S.nontemporal = variant == 4;
         S.dirty = true;
         S.graph_dirty = true;

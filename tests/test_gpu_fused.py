@@ -1,5 +1,6 @@
-"""The fused E+H pass (variants 2 and 3): one launch per step that reads one field buffer and writes
-the other, with y/z neighbours through LDS and halo cells recomputed.  Its results must equal the
+"""The fused E+H pass (variants 2 and 3, and 5 and 6 with deeper load pipelining): one launch per
+step that reads one field buffer and writes the other, with y/z neighbours through LDS and halo
+cells recomputed.  Its results must equal the
 oracle's and the separate kernels' bit for bit, whatever the tile, x planes, chunking and buffer
 parity."""
 import numpy as np
@@ -44,7 +45,7 @@ def _assert_equal_to_oracle(model, dtype, probes, monitors, f):
         np.testing.assert_array_equal(f[k], f_c[k], err_msg=k)
 
 
-@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("variant", [2, 3, 5, 6])
 @pytest.mark.parametrize("flags", [0, _lib.COEF_RAW, _lib.NO_GRAPHS])
 def test_fused_tiny_fp32_bit_exact(hip_ready, variant, flags):
     """Every code path of the small model (PML, dielectric, conductor, two ports, a voltage probe,
@@ -58,7 +59,7 @@ def test_fused_tiny_fp32_bit_exact(hip_ready, variant, flags):
 def test_fused_tiny_fp64_bit_exact(hip_ready):
     print("[case] fused tiny fp64", flush=True)
     model = workloads.tiny_model(Nt=50)
-    p, m, f = _run(model, np.float64, 0, 3, chunks=[21, 29], launch=(0, 4, 6))
+    p, m, f = _run(model, np.float64, 0, 6, chunks=[21, 29], launch=(0, 4, 6))
     _assert_equal_to_oracle(model, np.float64, p, m, f)
 
 
@@ -67,7 +68,7 @@ def test_fused_and_separate_kernels_interleave(hip_ready):
     arrays the fused pass leaves unwritten) changes nothing."""
     print("[case] fused/separate interleave", flush=True)
     model = workloads.tiny_model(Nt=60)
-    p, m, f = _run(model, np.float32, 0, 2, chunks=[13, 20, 7, 20], switch=[2, 1, 3, 0])
+    p, m, f = _run(model, np.float32, 0, 2, chunks=[13, 20, 7, 11, 9], switch=[2, 1, 3, 0, 5, 6])
     _assert_equal_to_oracle(model, np.float32, p, m, f)
 
 
@@ -77,8 +78,8 @@ def test_fused_many_tiles_and_planes(hip_ready):
     print("[case] fused tiles", flush=True)
     model = workloads.tiny_model(Nt=30, nx=23, ny=31, nz=131)
     _, _, ref = _run(model, variant=1)
-    for variant in (2, 3):
-        for xp in (1, 5, 23):
+    for variant in (2, 3, 5, 6):
+        for xp in (1, 2, 5, 23):
             _, _, f = _run(model, variant=variant, launch=(0, xp, 8))
             for k in ref:
                 np.testing.assert_array_equal(f[k], ref[k], err_msg=f"{k} variant {variant} x planes {xp}")
@@ -120,7 +121,7 @@ def test_fused_forward_gradient_equals_separate(hip_ready):
     coeff, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True)
     g = np.random.default_rng(3).standard_normal((len(probes), Nt)).astype(np.float32)
     out = []
-    for variant in (1, 2):
+    for variant in (1, 2, 6):
         _, pr, _, _, *_ = model.pack(gpu=True, coefficients=False)
         with Session(coeff, Nx, Ny, Nz) as s:
             s.set_variant(variant)
@@ -130,6 +131,7 @@ def test_fused_forward_gradient_equals_separate(hip_ready):
             s.synchronize()
             out.append(s.gradient((0, Nx, 0, Ny, 0, Nz), list(g)))
     np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
 
 
 def test_fused_large_grid_parity(hip_ready):
@@ -137,7 +139,7 @@ def test_fused_large_grid_parity(hip_ready):
     print("[case] fused 400x400x200", flush=True)
     model = workloads.dipole_slab(Nt=12, monitors=True)
     p_c, m_c, f_c = run_oracle(model, np.float32, n_threads=16)
-    p, m, f = _run(model, np.float32, 0, 3)
+    p, m, f = _run(model, np.float32, 0, 6)
     for a, b in zip(p, p_c):
         np.testing.assert_array_equal(a["values"], b["values"])
     for k in f_c:
