@@ -205,6 +205,22 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
  * workgroup).  block_threads 0 (the default) times the candidates on the zeroed adjoint state at
  * the next gradient and keeps the fastest for the session.  Results are identical either way.
  * rfdtd_get_adjoint_launch returns {variant, block_threads, x_chunks} in force. */
+/* Design-region coefficient assembly on the GPU (the on-device counterpart of the host assembly
+ * rfnetwork/solver.py:892-961 and the packing :1463-1523, for edges of plain dielectric): the 12 E
+ * coefficients of every packed cell of gbox (x0, x1, y0, y1, z0, z1, packed cells) of member
+ * `member` are recomputed from eps_new (F/m) and sigma (S/m) given per cell over gbox grown by one
+ * cell on the upper side of each axis (clipped to the grid), C order, float64.  dx, dy, dz: cell
+ * widths (m, float64, Nx / Ny / Nz); dx1, dy1, dz1: the packing's inverse dual spacings in the
+ * session's element type (Nx / Ny / Nz, last one 0); dt: the time step; e0: vacuum permittivity.
+ * eps_now is the permittivity the session's coefficients were built from (or last set): the tuples
+ * it gives must equal the session's, else RFDTD_EINVAL and nothing is changed (so a box touching
+ * PML, conductor, lumped-element or edge-corrected edges is refused).  Every instance of the member
+ * sees the new coefficients from its next step. */
+int rfdtd_set_design_permittivity(rfdtd_session* s, int32_t member, const int32_t gbox[6], const double* eps_now,
+                                  const double* eps_new, const double* sigma, const double* dx, const double* dy,
+                                  const double* dz, const void* dx1, const void* dy1, const void* dz1, double dt,
+                                  double e0);
+
 int rfdtd_set_adjoint_launch(rfdtd_session* s, int32_t variant, int32_t block_threads, int32_t x_chunks);
 int rfdtd_get_adjoint_launch(rfdtd_session* s, int32_t config_out[3]);
 void rfdtd_destroy(rfdtd_session* s);

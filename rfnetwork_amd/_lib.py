@@ -51,7 +51,7 @@ EXPORTS = (
     "rfdtd_instance_count", "rfdtd_set_instance_sources", "rfdtd_read_instance_fields", "rfdtd_set_variant",
     "rfdtd_set_checkpointing", "rfdtd_gradient", "rfdtd_autotune", "rfdtd_gradient_batch",
     "rfdtd_plan_checkpoints", "rfdtd_nf2ff", "rfdtd_set_history", "rfdtd_device_memory",
-    "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch",
+    "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
 )
 
 

@@ -118,6 +118,15 @@ class Batch:
         check(self._L.rfdtd_get_adjoint_launch(self._h, ctypes.cast(cfg, ctypes.c_void_p)), "rfdtd_get_adjoint_launch")
         return dict(variant=int(cfg[0]), block=int(cfg[1]), x_chunks=int(cfg[2]))
 
+    def set_design_permittivity(self, member, model, cbox, eps_r):
+        """Relative permittivity eps_r of the cells of cbox of member `member` (a grid; every excitation
+        of it sees the change), recomputed on the GPU as Session.set_design_permittivity does."""
+        from .session import set_design_permittivity
+
+        if not hasattr(self, "_design_state"):
+            self._design_state = {}
+        set_design_permittivity(self._h, self._L, int(member), model, cbox, eps_r, self._design_state, self.dtype)
+
     def set_history(self, requests, Nt):
         """Record the H totals of each request's box, requests [(instance, box), ...], at every step of
         later runs (up to Nt steps), for gradients without recomputation.  Before the first step."""

@@ -1438,6 +1438,129 @@ __global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__
     g[5 * nbox + q] += (double)A.F.ez_y[i] * (double)d_zy;
 }
 
+// ------------------------------------------------------------------------------------------------
+// design-region coefficient assembly
+//
+// The E coefficients of a box of packed cells from cell permittivities and conductivities, as the
+// host assembly forms them (rfnetwork/solver.py:892-961 averaging and Ca/Cb, :1463-1523 packing):
+// the edge value of each E component is the half-width-weighted average of its 4 adjacent cells in
+// float64 (first along the lower of the two transverse axes, then the other), rounded to the
+// element type; Ca = (2 eps - sig dt) / (2 eps + sig dt), Cb = 2 dt / (2 eps + sig dt) in that
+// type; edges on the grid boundary take eps0 and no loss; the packed Cb carries the inverse dual
+// spacing (zero past the last one) and the curl sign; the x = Nx-1 plane of the Ey and Ez entries
+// reads zero.  Valid for edges of plain dielectric only (no PML, conductor, lumped element or edge
+// correction); the check pass compares the tuple of the current permittivity with what the session
+// holds.
+// ------------------------------------------------------------------------------------------------
+
+struct DesignDev {
+    int32_t gx0, gy0, gz0, gnx, gny, gnz;  // packed-cell box
+    int32_t ex1, ey1, ez1;                  // extents of the cell volumes: the box + 1 cell, clipped
+    int32_t Nx, Ny, Nz;
+    const double *eps, *sig;                // cells [gx0, gx0 + ex1) x ..., C order
+    const double *dx, *dy, *dz;             // cell widths
+    const void *dx1, *dy1, *dz1;            // packing inverses, element type
+    double dt, e0;
+};
+
+template <typename T>
+__device__ __forceinline__ void design_tuple(const DesignDev& D, int i, int j, int k, T e[12]) {
+    const int li = i - D.gx0, lj = j - D.gy0, lk = k - D.gz0;
+    auto cell = [&](const double* v, int a, int b, int c) {
+        return v[((int64_t)(li + a) * D.ey1 + (lj + b)) * D.ez1 + (lk + c)];
+    };
+    auto avg = [](double v0, double v1, double d0, double d1) {
+        return (v0 * (d0 / 2) + v1 * (d1 / 2)) / (d0 / 2 + d1 / 2);
+    };
+    const T dt = (T)D.dt;
+    auto coef = [&](bool interior, double eps64, double sig64, T& ca, T& cb) {
+        const T eps = interior ? (T)eps64 : (T)D.e0;
+        const T sig = interior ? (T)sig64 : T(0);
+        ca = (T(2) * eps - (sig * dt)) / (T(2) * eps + (sig * dt));
+        cb = (T(2) * dt) / (T(2) * eps + (sig * dt));
+    };
+    const double dxi = D.dx[i], dyj = D.dy[j], dzk = D.dz[k];
+    const bool xi = i <= D.Nx - 2, yi = j <= D.Ny - 2, zi = k <= D.Nz - 2;
+    const double dxn = xi ? D.dx[i + 1] : 0.0, dyn = yi ? D.dy[j + 1] : 0.0, dzn = zi ? D.dz[k + 1] : 0.0;
+    T cax, cbx, cay, cby, caz, cbz;
+    {  // Ex edge: cells (i, j..j+1, k..k+1), y then z
+        double ve = 0, vs = 0;
+        if (yi && zi) {
+            for (int h = 0; h < 2; ++h) {
+                const double* v = h ? D.sig : D.eps;
+                const double u0 = avg(cell(v, 0, 0, 0), cell(v, 0, 1, 0), dyj, dyn);
+                const double u1 = avg(cell(v, 0, 0, 1), cell(v, 0, 1, 1), dyj, dyn);
+                (h ? vs : ve) = avg(u0, u1, dzk, dzn);
+            }
+        }
+        coef(yi && zi, ve, vs, cax, cbx);
+    }
+    {  // Ey edge: cells (i..i+1, j, k..k+1), x then z
+        double ve = 0, vs = 0;
+        if (xi && zi) {
+            for (int h = 0; h < 2; ++h) {
+                const double* v = h ? D.sig : D.eps;
+                const double u0 = avg(cell(v, 0, 0, 0), cell(v, 1, 0, 0), dxi, dxn);
+                const double u1 = avg(cell(v, 0, 0, 1), cell(v, 1, 0, 1), dxi, dxn);
+                (h ? vs : ve) = avg(u0, u1, dzk, dzn);
+            }
+        }
+        coef(xi && zi, ve, vs, cay, cby);
+    }
+    {  // Ez edge: cells (i..i+1, j..j+1, k), x then y
+        double ve = 0, vs = 0;
+        if (xi && yi) {
+            for (int h = 0; h < 2; ++h) {
+                const double* v = h ? D.sig : D.eps;
+                const double u0 = avg(cell(v, 0, 0, 0), cell(v, 1, 0, 0), dxi, dxn);
+                const double u1 = avg(cell(v, 0, 1, 0), cell(v, 1, 1, 0), dxi, dxn);
+                (h ? vs : ve) = avg(u0, u1, dyj, dyn);
+            }
+        }
+        coef(xi && yi, ve, vs, caz, cbz);
+    }
+    const T ix = ((const T*)D.dx1)[i], iy = ((const T*)D.dy1)[j], iz = ((const T*)D.dz1)[k];
+    e[0] = cax; e[1] = cax; e[2] = cbx * iy; e[3] = (-cbx) * iz;
+    e[4] = cay; e[5] = cay; e[6] = cby * iz; e[7] = (-cby) * ix;
+    e[8] = caz; e[9] = caz; e[10] = cbz * ix; e[11] = (-cbz) * iy;
+    if (i == D.Nx - 1)
+        for (int c = 4; c < 12; ++c) e[c] = T(0);
+}
+
+// check = 1: compare each packed cell's tuple with the session's (table entry or raw arrays) and
+// mark the workgroup in bad[] on any difference; check = 0: write the tuples (table entries
+// base + q and the cells' keys, or the raw arrays).
+template <typename T>
+__global__ void __launch_bounds__(256) k_design_coefficients(DesignDev D, int coef_kind, uint32_t* eidx,
+                                                             T* etab, T* const* raw, int64_t base, int check,
+                                                             int* bad) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)D.gnx * D.gny * D.gnz;
+    bool diff = false;
+    if (q < n) {
+        const int lk = (int)(q % D.gnz), lj = (int)(q / D.gnz % D.gny), li = (int)(q / ((int64_t)D.gnz * D.gny));
+        const int i = D.gx0 + li, j = D.gy0 + lj, k = D.gz0 + lk;
+        const int64_t c = ((int64_t)i * D.Ny + j) * D.Nz + k;
+        T e[12];
+        design_tuple<T>(D, i, j, k, e);
+        if (check) {
+            for (int m = 0; m < 12; ++m) {
+                const T have = coef_kind == 1 ? etab[(int64_t)eidx[c] * 12 + m] : raw[m][c];
+                diff |= have != e[m];
+            }
+        } else if (coef_kind == 1) {
+            for (int m = 0; m < 12; ++m) etab[(base + q) * 12 + m] = e[m];
+            eidx[c] = (uint32_t)(base + q);
+        } else {
+            for (int m = 0; m < 12; ++m) raw[m][c] = e[m];
+        }
+    }
+    if (check) {
+        const int any = __syncthreads_or(diff ? 1 : 0);
+        if (threadIdx.x == 0) bad[blockIdx.x] = any;
+    }
+}
+
 struct MonitorDev {
     int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id, fblk0, pad;
     int64_t N1N2;
@@ -1659,6 +1782,12 @@ struct Member {
     const uint32_t* hidx = nullptr;
     const void* etab = nullptr;
     const void* htab = nullptr;
+    // design-region coefficients assembled on the device (table form): the E table grows to hold
+    // one entry per cell of each design box after the nE deduplicated ones; updates of the same box
+    // rewrite its entries in place
+    void* d_etab_design = nullptr;
+    int64_t etab_cap = 0, design_used = 0, design_base = -1;
+    int32_t design_box[6] = {};
 };
 
 struct MonitorHost {
@@ -1815,8 +1944,10 @@ Session::~Session() {
         for (void* c : I.ckpt)
             if (c) (void)hipFree(c);
     }
-    for (auto& M : members)
+    for (auto& M : members) {
         if (M.d_coef) (void)hipFree(M.d_coef);
+        if (M.d_etab_design) (void)hipFree(M.d_etab_design);
+    }
     for (auto& h : hist)
         if (h.d) (void)hipFree(h.d);
     for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base, d_stage, (void*)d_rows,
@@ -3313,6 +3444,103 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
         S.dirty = true;
         S.graph_dirty = true;
         if (!S.fused) release_second_buffers(S);
+    });
+}
+
+int rfdtd_set_design_permittivity(rfdtd_session* s, int32_t member, const int32_t gbox[6], const double* eps_now,
+                                  const double* eps_new, const double* sigma, const double* dx, const double* dy,
+                                  const double* dz, const void* dx1, const void* dy1, const void* dz1, double dt,
+                                  double e0) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (member < 0 || member >= (int)S.members.size()) RF_THROW(RFDTD_EINVAL, "member out of range");
+        if (!gbox || !eps_now || !eps_new || !sigma || !dx || !dy || !dz || !dx1 || !dy1 || !dz1)
+            RF_THROW(RFDTD_EINVAL, "design permittivity arrays missing");
+        Member& M = S.members[member];
+        const int gx0 = gbox[0], gx1 = gbox[1], gy0 = gbox[2], gy1 = gbox[3], gz0 = gbox[4], gz1 = gbox[5];
+        if (gx0 < 0 || gy0 < 0 || gz0 < 0 || gx1 <= gx0 || gy1 <= gy0 || gz1 <= gz0 || gx1 > M.Nx || gy1 > M.Ny ||
+            gz1 > M.Nz)
+            RF_THROW(RFDTD_EINVAL, "design box outside the grid");
+        const int64_t n = (int64_t)(gx1 - gx0) * (gy1 - gy0) * (gz1 - gz0);
+        if (n > (int64_t)UINT32_MAX - M.nE - M.design_used) RF_THROW(RFDTD_EINVAL, "design box too large");
+        DesignDev D{};
+        D.gx0 = gx0; D.gy0 = gy0; D.gz0 = gz0;
+        D.gnx = gx1 - gx0; D.gny = gy1 - gy0; D.gnz = gz1 - gz0;
+        D.ex1 = std::min(gx1 + 1, M.Nx) - gx0;
+        D.ey1 = std::min(gy1 + 1, M.Ny) - gy0;
+        D.ez1 = std::min(gz1 + 1, M.Nz) - gz0;
+        D.Nx = M.Nx; D.Ny = M.Ny; D.Nz = M.Nz;
+        D.dt = dt;
+        D.e0 = e0;
+        const size_t esz = S.esz;
+        const size_t vol = (size_t)D.ex1 * D.ey1 * D.ez1 * sizeof(double);
+        HIP_OK(hipDeviceSynchronize());
+        GradBuffers buf;  // scratch freed on return
+        auto upload = [&](const void* src, size_t bytes) {
+            void* d = buf.alloc(bytes);
+            HIP_OK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+            return d;
+        };
+        const double* d_now = (const double*)upload(eps_now, vol);
+        const double* d_new = (const double*)upload(eps_new, vol);
+        D.sig = (const double*)upload(sigma, vol);
+        D.dx = (const double*)upload(dx, M.Nx * sizeof(double));
+        D.dy = (const double*)upload(dy, M.Ny * sizeof(double));
+        D.dz = (const double*)upload(dz, M.Nz * sizeof(double));
+        D.dx1 = upload(dx1, M.Nx * esz);
+        D.dy1 = upload(dy1, M.Ny * esz);
+        D.dz1 = upload(dz1, M.Nz * esz);
+        const int blocks = (int)((n + 255) / 256);
+        int* d_bad = (int*)buf.alloc((size_t)blocks * sizeof(int));
+        void** d_raw = (void**)upload(M.raw, sizeof(M.raw));
+        auto launch = [&](const double* eps, int check, int64_t base) {
+            D.eps = eps;
+            if (S.f64)
+                k_design_coefficients<double><<<blocks, 256>>>(D, M.coef_kind, (uint32_t*)M.eidx, (double*)M.etab,
+                                                               (double* const*)d_raw, base, check, d_bad);
+            else
+                k_design_coefficients<float><<<blocks, 256>>>(D, M.coef_kind, (uint32_t*)M.eidx, (float*)M.etab,
+                                                              (float* const*)d_raw, base, check, d_bad);
+            HIP_OK(hipGetLastError());
+        };
+        // the tuples of the current permittivity must be what the session holds
+        launch(d_now, 1, 0);
+        std::vector<int> bad(blocks);
+        HIP_OK(hipMemcpy(bad.data(), d_bad, blocks * sizeof(int), hipMemcpyDeviceToHost));
+        for (int b : bad)
+            if (b)
+                RF_THROW(RFDTD_EINVAL, "the design box touches edges that are not plain dielectric (PML, conductor, "
+                                       "lumped element or edge correction), or the current permittivity given does "
+                                       "not match the session's coefficients");
+        int64_t base = 0;
+        if (M.coef_kind == 1) {
+            const bool same = M.design_base >= 0 && std::equal(gbox, gbox + 6, M.design_box);
+            if (same) {
+                base = M.design_base;  // the box's cells already own entries [base, base + n)
+            } else {
+                // new entries after every entry in use (cells of earlier boxes keep theirs)
+                const int64_t used = M.nE + M.design_used;
+                if (M.etab_cap < M.nE) M.etab_cap = M.nE;
+                if (used + n > M.etab_cap) {
+                    const int64_t cap = used + n;
+                    void* t = nullptr;
+                    HIP_OK(hipMalloc(&t, (size_t)cap * 12 * esz));
+                    HIP_OK(hipMemcpy(t, M.etab, (size_t)used * 12 * esz, hipMemcpyDeviceToDevice));
+                    if (M.d_etab_design) (void)hipFree(M.d_etab_design);
+                    M.d_etab_design = t;
+                    M.etab = t;
+                    M.etab_cap = cap;
+                    S.dirty = true;
+                    S.graph_dirty = true;
+                }
+                base = used;
+                M.design_used += n;
+                M.design_base = base;
+                std::copy(gbox, gbox + 6, M.design_box);
+            }
+        }
+        launch(d_new, 0, base);
+        HIP_OK(hipDeviceSynchronize());
     });
 }
 

@@ -797,20 +797,27 @@ class FDTD_Solver:
                 p["src"] = None
 
     # ------------------------------------------------------------------ solve
+    def packing_inverses(self, dtype=np.float32):
+        """The inverse dual spacings the packing folds into Cb, one per cell along each axis (the
+        last one 0: that component's update coefficient vanishes), in dtype: as coefficient_arrays
+        forms them."""
+        dtype_ = np.dtype(dtype)
+        out = []
+        for d in self.dh_cells:
+            inv = 1 / conv.m_in(d).astype(dtype_)
+            out.append(np.concatenate([inv, np.zeros(1, dtype=dtype_)]).astype(dtype_))
+        return out
+
     def coefficient_arrays(self, gpu: bool = True, dtype=np.float32):
         """The 30 solver coefficient arrays with cell spacings and curl signs folded in
         (solver.py:1439-1523): all (Nx, Ny, Nz) for gpu=True, the CPU solver's shapes otherwise."""
         self.check_mesh()
         dtype_ = np.dtype(dtype)
         dx, dy, dz = [conv.m_in(d).astype(dtype_) for d in self.d_cells]
-        dx_h, dy_h, dz_h = [conv.m_in(d).astype(dtype_) for d in self.dh_cells]
-        Nx, Ny, Nz = len(dx), len(dy), len(dz)
         dx_inv, dy_inv, dz_inv = 1 / dx[:, None, None], 1 / dy[None, :, None], 1 / dz[None, None, :]
-        dx_h_inv, dy_h_inv, dz_h_inv = 1 / dx_h[:, None, None], 1 / dy_h[None, :, None], 1 / dz_h[None, None, :]
         # one more (zero) spacing for the last component: its update coefficient vanishes (PEC edge)
-        dx1 = np.concatenate([dx_h_inv, [[[0]]]], axis=0)
-        dy1 = np.concatenate([dy_h_inv, [[[0]]]], axis=1)
-        dz1 = np.concatenate([dz_h_inv, [[[0]]]], axis=2)
+        ix, iy, iz = self.packing_inverses(dtype_)
+        dx1, dy1, dz1 = ix[:, None, None], iy[None, :, None], iz[None, None, :]
         end = None if gpu else -1
         hs = 1 if gpu else 0
         Ca, Cb, Da, Db = self.Ca, self.Cb, self.Da, self.Db

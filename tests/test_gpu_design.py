@@ -1,0 +1,125 @@
+"""Design-region coefficient assembly on the GPU (rfdtd_set_design_permittivity): a session built
+from one permittivity of the design box and updated on the device to another runs, bit for bit,
+as a session built by the host assembly from the second permittivity (rfnetwork/solver.py:892-961
+averaging and Ca/Cb, :1463-1523 packing)."""
+import numpy as np
+import pytest
+
+from rfnetwork_amd import _lib, workloads
+from rfnetwork_amd.batch import Batch
+from rfnetwork_amd.session import Session
+
+pytestmark = pytest.mark.gpu
+
+DESIGN = (8, 8, 6)
+
+
+def _model(eps, coef_dtype=np.float32, Nt=120, **kw):
+    return workloads.design_strip(eps, Nt=Nt, nx=48, ny=32, nz=20, design=DESIGN, n_pml=5, h=3,
+                                  coef_dtype=coef_dtype, pulse=(20e9, 80e-12, 60e-12), **kw)
+
+
+def _eps(seed):
+    return np.random.default_rng(seed).uniform(1.0, 4.0, size=DESIGN)
+
+
+def _run(session, model, Nt):
+    _, probes, _, _, *_ = model.pack(gpu=True, dtype=session.dtype, coefficients=False)
+    session.set_sources(probes, [], Nt)
+    session.run(Nt)
+    session.synchronize()
+    session.read()
+    return [np.array(p["values"]) for p in probes], session.fields()
+
+
+def _reference(model, dtype, flags, Nt):
+    coeff, _, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True, dtype=dtype)
+    with Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags) as s:
+        return _run(s, model, Nt)
+
+
+def _assert_same(a, b):
+    for x, y in zip(a[0], b[0]):
+        np.testing.assert_array_equal(x, y)
+    for k in a[1]:
+        np.testing.assert_array_equal(a[1][k], b[1][k], err_msg=k)
+
+
+@pytest.mark.parametrize("dtype,flags", [(np.float32, 0), (np.float32, _lib.COEF_RAW), (np.float64, 0)])
+def test_design_update_equals_host_assembly(hip_ready, dtype, flags):
+    """One session updated twice (the second update rewrites the box's entries in place) against
+    sessions built from the host assembly of each permittivity."""
+    print(f"[case] design update dtype={np.dtype(dtype).name} flags={flags}", flush=True)
+    Nt = 120
+    m0, m1, m2 = (_model(_eps(s), coef_dtype=dtype, Nt=Nt) for s in (1, 2, 3))
+    cbox = workloads.design_box(48, 32, DESIGN)
+    coeff, _, _, _, Nx, Ny, Nz, _ = m0.pack(gpu=True, dtype=dtype)
+    for target, eps in ((m1, _eps(2)), (m2, _eps(3))):
+        with Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags) as s:
+            s.set_design_permittivity(m0, cbox, _eps(2))
+            if target is m2:
+                s.set_design_permittivity(m0, cbox, eps)
+            got = _run(s, m0, Nt)
+        _assert_same(got, _reference(target, dtype, flags, Nt))
+
+
+def test_design_update_two_boxes(hip_ready):
+    """Two different boxes one after the other: the cells of the first keep their entries."""
+    print("[case] design two boxes", flush=True)
+    Nt = 100
+    cbox = workloads.design_box(48, 32, DESIGN)
+    x0, x1, y0, y1, z0, z1 = cbox
+    e1, e2 = _eps(4), _eps(5)
+    m0 = _model(_eps(1), Nt=Nt)
+    # the expected model: the whole box from e1, then its lower half in x from e2
+    expect = e1.copy()
+    expect[:4] = e2[:4]
+    m_exp = _model(expect, Nt=Nt)
+    coeff, _, _, _, Nx, Ny, Nz, _ = m0.pack(gpu=True)
+    with Session(coeff, Nx, Ny, Nz) as s:
+        s.set_design_permittivity(m0, cbox, e1)
+        s.set_design_permittivity(m0, (x0, x0 + 4, y0, y1, z0, z1), e2[:4])
+        got = _run(s, m0, Nt)
+    _assert_same(got, _reference(m_exp, np.float32, 0, Nt))
+
+
+def test_design_update_in_a_batch(hip_ready):
+    """Updating one member of a batch changes that member only."""
+    print("[case] design batch", flush=True)
+    Nt = 100
+    cbox = workloads.design_box(48, 32, DESIGN)
+    ma, mb, mb2 = _model(_eps(6), Nt=Nt), _model(_eps(7), Nt=Nt), _model(_eps(8), Nt=Nt)
+    grids = []
+    for m in (ma, mb):
+        coeff, _, _, _, Nx, Ny, Nz, _ = m.pack(gpu=True)
+        grids.append((coeff, Nx, Ny, Nz, 1))
+    with Batch(grids) as b:
+        b.set_design_permittivity(1, mb, cbox, _eps(8))
+        for k, m in enumerate((ma, mb)):
+            _, probes, _, _, *_ = m.pack(gpu=True, coefficients=False)
+            b.set_sources(k, probes, [], Nt)
+        b.run(Nt)
+        b.synchronize()
+        b.read()
+        fields = [b.fields(k) for k in range(2)]
+    for m, f in ((ma, fields[0]), (mb2, fields[1])):
+        ref = _reference(m, np.float32, 0, Nt)[1]
+        for k in ref:
+            np.testing.assert_array_equal(f[k], ref[k], err_msg=k)
+
+
+def test_design_update_refuses_non_plain_edges(hip_ready):
+    """A box reaching a conductor, port or PML edge is refused and the session is left unchanged;
+    so is a stated current permittivity that the session does not hold."""
+    print("[case] design refusal", flush=True)
+    Nt = 60
+    m0 = _model(_eps(1), Nt=Nt)
+    coeff, _, _, _, Nx, Ny, Nz, _ = m0.pack(gpu=True)
+    with Session(coeff, Nx, Ny, Nz) as s:
+        with pytest.raises(Exception, match="plain dielectric"):
+            s.set_design_permittivity(m0, (1, 4, 1, 4, 0, 3), np.full((3, 3, 3), 2.0))  # in the PML
+        x0, x1, y0, y1, z0, z1 = workloads.design_box(48, 32, DESIGN)
+        with pytest.raises(Exception, match="plain dielectric"):  # reaches the feed conductor at x0 - 1
+            s.set_design_permittivity(m0, (x0 - 2, x0, y0, y1, z0, z1), np.full((2, 8, 6), 2.0))
+        got = _run(s, m0, Nt)
+    _assert_same(got, _reference(m0, np.float32, 0, Nt))
