@@ -316,6 +316,23 @@ def adjoint_leg(n_steps, every, repeats=2, flags=0):
             f"reverse {best[1]['adjoint_ms']:.1f} ms)")
     wall, t, J, grad = res["history"]
     wc, tc, Jc, gc = res["checkpoint"]
+    # a design loop on one resident session: the box's coefficients recomputed on the GPU per
+    # evaluation instead of rebuilding the model and the session
+    from rfnetwork_amd.design import DesignLoop
+
+    eps_model = np.random.default_rng(4).uniform(1.0, 4.0, size=(64, 64, 32))  # design_strip's default design
+    eps_next = np.random.default_rng(5).uniform(1.0, 4.0, size=eps_model.shape)
+    with DesignLoop(model, cbox, model.frequency, flags=flags, mode="history") as loop:
+        Jl, gl, _ = loop.evaluate(eps_model)
+        loop_equal = bool(Jl == J and np.array_equal(gl, grad))
+        walls = []
+        for e in (eps_next, eps_model):
+            t0 = time.perf_counter()
+            loop.evaluate(e)
+            walls.append(time.perf_counter() - t0)
+        lt = loop.timing
+    log(f"adjoint (design loop): {min(walls):.3f}s per gradient (update {lt['update_ms']:.2f} ms, forward "
+        f"{lt['forward_ms']:.1f} ms, reverse {lt['adjoint_ms']:.1f} ms), equal to the rebuilt model's: {loop_equal}")
     cells = model.Nx * model.Ny * model.Nz
     return dict(workload=f"design_strip {model.Nx}x{model.Ny}x{model.Nz}, design box 64x64x32, "
                          f"{n_steps} steps, design-box H history", gradients_per_s=round(1.0 / wall, 4),
@@ -326,6 +343,8 @@ def adjoint_leg(n_steps, every, repeats=2, flags=0):
                 checkpointed=dict(every=every, gradients_per_s=round(1.0 / wc, 4), forward_ms=round(tc["forward_ms"], 1),
                                   reverse_ms=round(tc["adjoint_ms"], 1)),
                 modes_equal=bool(J == Jc and np.array_equal(grad, gc)),
+                design_loop=dict(gradients_per_s=round(1.0 / min(walls), 4), s_per_gradient=round(min(walls), 3),
+                                 coefficient_update_ms=round(lt["update_ms"], 2), equal=loop_equal),
                 finite=bool(np.isfinite(grad).all()))
 
 

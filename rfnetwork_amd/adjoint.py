@@ -118,9 +118,11 @@ def _pair_weights(d):
     return w0, 1 - w0
 
 
-def cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox):
+def cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox, edges=None):
     """dJ/deps_r of the cells of cbox = (x0, x1, y0, y1, z0, z1) from dJ/dCb of the six E splits on
-    the grid-cell box gbox (as returned by Session.gradient, GPU packing)."""
+    the grid-cell box gbox (as returned by Session.gradient, GPU packing).  The edge permittivity
+    and packed coefficients come from the model and its packed arrays `coeff`, or, given
+    edges = design.edge_values(...), from those (values over gbox for the current design)."""
     gx0, gx1, gy0, gy1, gz0, gz1 = gbox
     x0, x1, y0, y1, z0, z1 = cbox
     Nx, Ny, Nz = model.n_cells
@@ -133,7 +135,7 @@ def cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox):
     comps = (("ex", ("Cb_ex_y", "Cb_ex_z"), ("Ca_ex_y", "Ca_ex_z"), (0, 1, 1), ("y", "z")),
              ("ey", ("Cb_ey_z", "Cb_ey_x"), ("Ca_ey_z", "Ca_ey_x"), (1, 0, 1), ("x", "z")),
              ("ez", ("Cb_ez_x", "Cb_ez_y"), ("Ca_ez_x", "Ca_ez_y"), (1, 1, 0), ("x", "y")))
-    eps_edge = dict(ex=model.eps_ex, ey=model.eps_ey, ez=model.eps_ez)
+    eps_edge = dict(ex=model.eps_ex, ey=model.eps_ey, ez=model.eps_ez) if edges is None else None
     for c, (comp, cbk, cak, off, axes) in enumerate(comps):
         X, Y, Z = gx + off[0], gy + off[1], gz + off[2]
         # interior edges only: boundary edges hold e0 and take no cell permittivity
@@ -141,11 +143,16 @@ def cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox):
         for ax_name, G, N in (("x", X, Nx), ("y", Y, Ny), ("z", Z, Nz)):
             if ax_name in axes:
                 valid &= (G >= 1) & (G <= N - 1)
-        Xc, Yc, Zc = np.clip(X, 0, Nx), np.clip(Y, 0, Ny), np.clip(Z, 0, Nz)
-        eps = eps_edge[comp][Xc, Yc, Zc].astype(np.float64)
-        cb1, cb2 = coeff[cbk[0]][sl].astype(np.float64), coeff[cbk[1]][sl].astype(np.float64)
+        if edges is None:
+            Xc, Yc, Zc = np.clip(X, 0, Nx), np.clip(Y, 0, Ny), np.clip(Z, 0, Nz)
+            eps = eps_edge[comp][Xc, Yc, Zc].astype(np.float64)
+            cget = lambda key: coeff[key][sl]  # noqa: E731
+        else:
+            eps = edges[0][comp[1]].astype(np.float64)
+            cget = lambda key: edges[1][key]  # noqa: E731
+        cb1, cb2 = cget(cbk[0]).astype(np.float64), cget(cbk[1]).astype(np.float64)
         dJ_deps = np.where(valid, -(grad_cb[2 * c] * cb1 + grad_cb[2 * c + 1] * cb2) / eps, 0.0)
-        plain = (coeff[cak[0]][sl] == 1.0) & (coeff[cak[1]][sl] == 1.0)
+        plain = (cget(cak[0]) == 1.0) & (cget(cak[1]) == 1.0)
         for a in (0, 1):
             for b2 in (0, 1):
                 # the cell: lower neighbour along each averaged axis, shifted by a / b

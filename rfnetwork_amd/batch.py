@@ -121,7 +121,7 @@ class Batch:
     def set_design_permittivity(self, member, model, cbox, eps_r):
         """Relative permittivity eps_r of the cells of cbox of member `member` (a grid; every excitation
         of it sees the change), recomputed on the GPU as Session.set_design_permittivity does."""
-        from .session import set_design_permittivity
+        from .design import set_design_permittivity
 
         if not hasattr(self, "_design_state"):
             self._design_state = {}

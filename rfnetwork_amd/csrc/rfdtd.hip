@@ -3598,6 +3598,12 @@ int rfdtd_set_history(rfdtd_session* s, int32_t n, const int32_t* instances, con
         if (n < 0 || (n > 0 && (!instances || !boxes || n_steps < 1)))
             RF_THROW(RFDTD_EINVAL, "history request arrays missing");
         if (n > 0 && S.step != 0) RF_THROW(RFDTD_ESTATE, "record the history from the first step (run after rfdtd_reset)");
+        // the same request again (a loop of runs on one session): keep the buffers
+        bool same = n == (int)S.hist.size();
+        for (int k = 0; same && k < n; ++k)
+            same = S.hist[k].inst == instances[k] && S.hist[k].capacity == n_steps &&
+                   std::equal(boxes + 6 * k, boxes + 6 * k + 6, S.hist[k].box);
+        if (same) return;
         HIP_OK(hipDeviceSynchronize());
         for (auto& h : S.hist)
             if (h.d) (void)hipFree(h.d);

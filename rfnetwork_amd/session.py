@@ -133,6 +133,8 @@ class Session:
         bit what the model's own assembly gives for that permittivity).  The box must hold plain
         dielectric edges only (no PML, conductor, lumped element or edge correction); later calls
         start from the permittivity set last."""
+        from .design import set_design_permittivity
+
         if not hasattr(self, "_design_state"):
             self._design_state = {}
         set_design_permittivity(self._h, self._L, 0, model, cbox, eps_r, self._design_state, self.dtype)
@@ -198,47 +200,3 @@ class Session:
     def __exit__(self, *exc):
         self.close()
 
-
-def design_permittivity_args(model, cbox, eps_now, sigma_now, eps_r, dtype):
-    """Arguments of rfdtd_set_design_permittivity for replacing the relative permittivity of the
-    cells of cbox = (x0, x1, y0, y1, z0, z1) by eps_r: the packed-cell box, the current and new
-    permittivity (F/m) and the conductivity over that box grown by one cell on each upper side,
-    the cell widths and the packing inverses.  eps_now, sigma_now: full-grid float64 arrays."""
-    from .adjoint import gpu_box_for_cells
-    from .units import conv, const
-
-    x0, x1, y0, y1, z0, z1 = (int(v) for v in cbox)
-    eps_r = np.asarray(eps_r, dtype=np.float64)
-    if eps_r.shape != (x1 - x0, y1 - y0, z1 - z0):
-        raise ValueError("eps_r must have the shape of the design box")
-    Nx, Ny, Nz = model.n_cells
-    gbox = gpu_box_for_cells(model, cbox)
-    gx0, gx1, gy0, gy1, gz0, gz1 = gbox
-    ext = (slice(gx0, min(gx1 + 1, Nx)), slice(gy0, min(gy1 + 1, Ny)), slice(gz0, min(gz1 + 1, Nz)))
-    now = np.ascontiguousarray(eps_now[ext], dtype=np.float64)
-    new = now.copy()
-    new[x0 - gx0:x1 - gx0, y0 - gy0:y1 - gy0, z0 - gz0:z1 - gz0] = eps_r * const.e0
-    sig = np.ascontiguousarray(sigma_now[ext], dtype=np.float64)
-    widths = [np.ascontiguousarray(conv.m_in(d), dtype=np.float64) for d in model.d_cells]
-    inverses = [np.ascontiguousarray(a) for a in model.packing_inverses(dtype)]
-    return gbox, now, new, sig, widths, inverses, float(model.dt), float(const.e0)
-
-
-def set_design_permittivity(handle, L, member, model, cbox, eps_r, state, dtype):
-    """Replace eps_r of the cells of cbox in member `member` of a session on the GPU; state holds
-    the session's current full-grid permittivity and conductivity per member (taken from the model
-    the first time) and is updated on success."""
-    if np.dtype(model.coef_dtype) != np.dtype(dtype):
-        raise ValueError("the model's coefficient dtype must be the session's element type")
-    if member not in state:
-        state[member] = (np.array(model.eps, dtype=np.float64), np.array(model.sigma, dtype=np.float64))
-    eps_now, sigma_now = state[member]
-    gbox, now, new, sig, (dx, dy, dz), (ix, iy, iz), dt, e0 = design_permittivity_args(
-        model, cbox, eps_now, sigma_now, eps_r, dtype)
-    b = (ctypes.c_int32 * 6)(*gbox)
-    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    check(L.rfdtd_set_design_permittivity(handle, int(member), ctypes.cast(b, ctypes.c_void_p), ptr(now), ptr(new),
-                                          ptr(sig), ptr(dx), ptr(dy), ptr(dz), ptr(ix), ptr(iy), ptr(iz),
-                                          ctypes.c_double(dt), ctypes.c_double(e0)), "rfdtd_set_design_permittivity")
-    x0, x1, y0, y1, z0, z1 = (int(v) for v in cbox)
-    eps_now[x0:x1, y0:y1, z0:z1] = np.asarray(eps_r, dtype=np.float64) * e0

@@ -123,3 +123,26 @@ def test_design_update_refuses_non_plain_edges(hip_ready):
             s.set_design_permittivity(m0, (x0 - 2, x0, y0, y1, z0, z1), np.full((2, 8, 6), 2.0))
         got = _run(s, m0, Nt)
     _assert_same(got, _reference(m0, np.float32, 0, Nt))
+
+
+@pytest.mark.parametrize("mode", ["history", "checkpoint"])
+def test_design_loop_equals_rebuilt_models(hip_ready, mode):
+    """Successive evaluations on one resident session give J, dJ/deps_r and S equal, bit for bit,
+    to eps_r_gradient on models rebuilt with each permittivity."""
+    print(f"[case] design loop {mode}", flush=True)
+    from rfnetwork_amd import adjoint
+    from rfnetwork_amd.design import DesignLoop
+
+    Nt = 300
+    freq = (16e9, 20e9, 24e9)
+    cbox = workloads.design_box(48, 32, DESIGN)
+    m0 = _model(_eps(1), Nt=Nt, frequency=freq)
+    with DesignLoop(m0, cbox, freq, mode=mode, checkpoint_every=70) as loop:
+        for seed in (2, 3):
+            J, g, S = loop.evaluate(_eps(seed))
+            mr = _model(_eps(seed), Nt=Nt, frequency=freq)
+            Jr, gr, Sr = adjoint.eps_r_gradient(mr, cbox, freq, mode=mode, checkpoint_every=70)
+            assert J == Jr
+            np.testing.assert_array_equal(g, gr)
+            np.testing.assert_array_equal(S, Sr)
+            assert loop.timing["update_ms"] > 0
