@@ -205,6 +205,16 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
  * workgroup).  block_threads 0 (the default) times the candidates on the zeroed adjoint state at
  * the next gradient and keeps the fastest for the session.  Results are identical either way.
  * rfdtd_get_adjoint_launch returns {variant, block_threads, x_chunks} in force. */
+/* Streamed time-domain monitors (the reference keeps the whole series in host memory it writes to,
+ * solver.cpp:1168-1191): with ring_samples > 0, time-domain monitors set afterwards keep only
+ * ring_samples sample planes on the device; the run is cut into segments producing at most half a
+ * ring each, and after every segment the new planes are copied, on a copy stream of the session,
+ * into the monitor's host array (rfdtd_monitor.values), overlapping the next segment.  For the
+ * copies to be asynchronous that array should be page-locked (allocated with hipHostMalloc or as a
+ * pinned tensor); any host array works.  rfdtd_read_monitors waits for the copies.  0 (default):
+ * the whole series stays on the device until rfdtd_read_monitors. */
+int rfdtd_set_monitor_streaming(rfdtd_session* s, int32_t ring_samples);
+
 /* Design-region coefficient assembly on the GPU (the on-device counterpart of the host assembly
  * rfnetwork/solver.py:892-961 and the packing :1463-1523, for edges of plain dielectric): the 12 E
  * coefficients of every packed cell of gbox (x0, x1, y0, y1, z0, z1, packed cells) of member

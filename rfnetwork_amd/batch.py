@@ -52,8 +52,16 @@ class Batch:
             self.instances += [(g, k) for k in range(K)]
         self._keep = {}
 
+    def set_monitor_streaming(self, ring_samples):
+        """As Session.set_monitor_streaming, for the instances' monitors installed afterwards."""
+        check(self._L.rfdtd_set_monitor_streaming(self._h, int(ring_samples)), "rfdtd_set_monitor_streaming")
+        self._stream_ring = int(ring_samples)
+
     def set_sources(self, instance, probes, monitors, Nt):
-        from .hip_func import check_probe_cells, pack_monitors, pack_probes
+        from .hip_func import check_probe_cells, pack_monitors, pack_probes, pin_time_monitors
+
+        if getattr(self, "_stream_ring", 0):
+            pin_time_monitors(monitors)
 
         Nx, Ny, Nz, _ = self.grids[self.instances[instance][0]]
         check_probe_cells(probes, Nx, Ny, Nz)

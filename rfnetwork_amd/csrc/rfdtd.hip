@@ -1562,7 +1562,7 @@ __global__ void __launch_bounds__(256) k_design_coefficients(DesignDev D, int co
 }
 
 struct MonitorDev {
-    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id, fblk0, pad;
+    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id, fblk0, ring;  // ring: samples kept (0: all)
     int64_t N1N2;
     void* values;
     const void* phase;
@@ -1611,10 +1611,12 @@ __global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDe
     else
         idx = ((int64_t)a * Ny + b) * Nz + M.pos;
     const T v = total_at(I, M.field, idx);
-    if (M.nf == 0)
-        ((T*)M.values)[(int64_t)(n / M.n_step) * M.N1N2 + q] = v;
-    else
+    if (M.nf == 0) {
+        const int64_t m = n / M.n_step;
+        ((T*)M.values)[(M.ring ? m % M.ring : m) * M.N1N2 + q] = v;
+    } else {
         ((T*)M.stage)[(int64_t)slot * M.N1N2 + q] = v;
+    }
 }
 
 // Accumulate staged slots [0, nslots): values[f] += v_s * phase[row_s, f] for s in time order (a real
@@ -1794,6 +1796,7 @@ struct MonitorHost {
     rfdtd_monitor desc;
     bool active = false;
     int N1 = 0, N2 = 0, Nm = 0, pos = 0;
+    int ring = 0;  // streamed time-domain monitor: sample planes kept on the device (0: all Nm)
     void* d_values = nullptr;
     void* d_phase = nullptr;
 };
@@ -1858,6 +1861,13 @@ struct Session {
     int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
     bool fused_deep = false;   // fused step with E one plane ahead of H (one barrier per plane)
+    // streamed time-domain monitors: a device ring of stream_ring sample planes per monitor, copied
+    // to the caller's host arrays on copy_stream after every run segment
+    int stream_ring = 0;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t seg_ev = nullptr, cpy_ev[2] = {nullptr, nullptr};
+    bool cpy_pending[2] = {false, false};
+    int64_t seg_count = 0;
     // reverse-pass launch shape: pipelined (1) or plain (0) columns, threads per workgroup and x
     // chunks per grid (0: the forward's planes per workgroup); adj_block 0 = timed at the first
     // gradient and kept
@@ -1953,6 +1963,9 @@ Session::~Session() {
     for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base, d_stage, (void*)d_rows,
                     d_insts2, (void*)d_flat0, d_hist_boxes, (void*)d_hist_h0})
         if (p) (void)hipFree(p);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    for (hipEvent_t e : {seg_ev, cpy_ev[0], cpy_ev[1]})
+        if (e) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (auto e : kev) (void)hipEventDestroy(e);
@@ -2342,6 +2355,7 @@ void refresh(Session& s) {
             d.id = (int32_t)md.size();
             d.axis = h.desc.axis; d.pos = h.pos; d.field = h.desc.field; d.n_step = h.desc.n_step;
             d.nf = h.desc.n_frequencies; d.N1 = h.N1; d.N2 = h.N2;
+            d.ring = h.ring;
             d.N1N2 = (int64_t)h.N1 * h.N2;
             d.blk0 = blocks;
             d.values = h.d_values;
@@ -2509,6 +2523,39 @@ void run_chunk(Session& s, int len, int offset, hipStream_t st, bool graphs, boo
     }
 }
 
+// Streamed monitors: the sample planes that steps [a, b) produced go from their ring slots to the
+// caller's host arrays on the copy stream, after the run segment on `st`; the segment two ahead
+// waits for these copies before it overwrites the slots.
+void stream_monitors(Session& s, hipStream_t st, int64_t a, int64_t b) {
+    HIP_OK(hipEventRecord(s.seg_ev, st));
+    HIP_OK(hipStreamWaitEvent(s.copy_stream, s.seg_ev, 0));
+    auto samples = [](int64_t N, int p) { return N <= 0 ? (int64_t)0 : (N - 1) / p + 1; };
+    for (Instance& I : s.insts)
+        for (auto& h : I.mons) {
+            if (!h.ring) continue;
+            const size_t plane = (size_t)h.N1 * h.N2 * s.esz;
+            int64_t m = std::min<int64_t>(samples(a, h.desc.n_step), h.Nm);
+            const int64_t m1 = std::min<int64_t>(samples(b, h.desc.n_step), h.Nm);
+            while (m < m1) {
+                const int64_t slot = m % h.ring, cnt = std::min<int64_t>(m1 - m, h.ring - slot);
+                HIP_OK(hipMemcpyAsync((char*)h.desc.values + (size_t)m * plane, (const char*)h.d_values + (size_t)slot * plane,
+                                      (size_t)cnt * plane, hipMemcpyDeviceToHost, s.copy_stream));
+                m += cnt;
+            }
+        }
+    const int e = (int)(s.seg_count & 1);
+    HIP_OK(hipEventRecord(s.cpy_ev[e], s.copy_stream));
+    s.cpy_pending[e] = true;
+    ++s.seg_count;
+}
+
+bool has_streamed_monitors(const Session& s) {
+    for (const Instance& I : s.insts)
+        for (const auto& h : I.mons)
+            if (h.ring) return true;
+    return false;
+}
+
 template <typename T>
 void run_steps(Session& s, int n_steps, hipStream_t st) {
     if (n_steps < 0) RF_THROW(RFDTD_EINVAL, "negative step count");
@@ -2531,24 +2578,41 @@ void run_steps(Session& s, int n_steps, hipStream_t st) {
     // kernel timing: events around the E and H updates of up to kTimedSteps evenly spread steps
     s.n_timed = 0;
     const int stride = timing ? std::max(1, n_steps / Session::kTimedSteps) : 0;
-    int done = 0;
-    if (s.ckpt_every > 0) {
-        // checkpointed run: save the split fields before every multiple of ckpt_every, and replay
-        // graph chunks only between checkpoints
-        while (done < n_steps) {
-            const int64_t n = s.step + done;
-            if (n % s.ckpt_every == 0) save_checkpoints(s, (int)(n / s.ckpt_every), st);
-            const int64_t next = (n / s.ckpt_every + 1) * s.ckpt_every;
-            const int len = (int)std::min<int64_t>(n_steps - done, next - n);
-            run_chunk<T>(s, len, done, st, graphs, timing, stride);
-            done += len;
+    // streamed monitors: segments short enough that each produces at most half a ring of samples
+    // (whole graph chunks where the ring allows)
+    const bool streaming = has_streamed_monitors(s);
+    int64_t seg_len = INT64_MAX;
+    if (streaming) {
+        for (const Instance& I : s.insts)
+            for (const auto& h : I.mons)
+                if (h.ring) seg_len = std::min<int64_t>(seg_len, (int64_t)std::max(1, h.ring / 2) * h.desc.n_step);
+        const int gs = chunk_steps(s);
+        if (graphs && seg_len >= gs) seg_len = seg_len / gs * gs;
+        if (!s.copy_stream) {
+            HIP_OK(hipStreamCreateWithFlags(&s.copy_stream, hipStreamNonBlocking));
+            HIP_OK(hipEventCreateWithFlags(&s.seg_ev, hipEventDisableTiming));
+            for (auto& e : s.cpy_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         }
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipEventRecord(s.ev1, st));
-        s.step += n_steps;
-        return;
     }
-    run_chunk<T>(s, n_steps, 0, st, graphs, timing, stride);
+    // checkpointed runs save the split fields before every multiple of ckpt_every and replay graph
+    // chunks only between checkpoints
+    int done = 0;
+    while (done < n_steps) {
+        const int64_t n = s.step + done;
+        int64_t len = n_steps - done;
+        if (s.ckpt_every > 0) {
+            if (n % s.ckpt_every == 0) save_checkpoints(s, (int)(n / s.ckpt_every), st);
+            len = std::min<int64_t>(len, (n / s.ckpt_every + 1) * s.ckpt_every - n);
+        }
+        if (streaming) {
+            len = std::min(len, seg_len);
+            const int e = (int)(s.seg_count & 1);  // the copies of the segment two back
+            if (s.cpy_pending[e]) HIP_OK(hipStreamWaitEvent(st, s.cpy_ev[e], 0));
+        }
+        run_chunk<T>(s, (int)len, done, st, graphs, timing, stride);
+        if (streaming) stream_monitors(s, st, n, n + len);
+        done += (int)len;
+    }
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(s.ev1, st));
     s.step += n_steps;
@@ -2808,9 +2872,12 @@ void set_monitors(Session& s, int inst, const rfdtd_monitor* mons, int n, int Nt
         h.Nm = Nt / d.n_step + 1;
         h.active = pos >= 0 && pos < along;
         const size_t plane = (size_t)h.N1 * h.N2;
-        const size_t vbytes = d.n_frequencies ? (size_t)d.n_frequencies * plane * 2 * s.esz : (size_t)h.Nm * plane * s.esz;
+        if (s.stream_ring > 0 && !d.n_frequencies && h.active) h.ring = std::min(h.Nm, s.stream_ring);
+        const size_t vbytes = d.n_frequencies ? (size_t)d.n_frequencies * plane * 2 * s.esz
+                              : (size_t)(h.ring ? h.ring : h.Nm) * plane * s.esz;
         HIP_OK(hipMalloc(&h.d_values, vbytes));
-        HIP_OK(hipMemcpy(h.d_values, d.values, vbytes, hipMemcpyHostToDevice));
+        if (h.ring) HIP_OK(hipMemset(h.d_values, 0, vbytes));  // the host array keeps unsampled rows
+        else HIP_OK(hipMemcpy(h.d_values, d.values, vbytes, hipMemcpyHostToDevice));
         if (d.n_frequencies) {
             const size_t pbytes = (size_t)h.Nm * d.n_frequencies * 2 * s.esz;
             HIP_OK(hipMalloc(&h.d_phase, pbytes));
@@ -2836,13 +2903,16 @@ void read_probes(Session& s) {
 
 size_t monitor_bytes(const Session& s, const MonitorHost& h) {
     const size_t plane = (size_t)h.N1 * h.N2;
-    return h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * s.esz : (size_t)h.Nm * plane * s.esz;
+    return h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * s.esz
+                                : (size_t)(h.ring ? h.ring : h.Nm) * plane * s.esz;
 }
 
 void read_monitors(Session& s) {
     HIP_OK(hipDeviceSynchronize());
+    if (s.copy_stream) HIP_OK(hipStreamSynchronize(s.copy_stream));
     for (Instance& I : s.insts)
-        for (auto& h : I.mons) HIP_OK(hipMemcpy(h.desc.values, h.d_values, monitor_bytes(s, h), hipMemcpyDeviceToHost));
+        for (auto& h : I.mons)
+            if (!h.ring) HIP_OK(hipMemcpy(h.desc.values, h.d_values, monitor_bytes(s, h), hipMemcpyDeviceToHost));
 }
 
 void read_fields(Session& s, int inst, void* const out[18]) {
@@ -3374,6 +3444,9 @@ int rfdtd_reset(rfdtd_session* s) {
             if (I.d_fields2) HIP_OK(hipMemset(I.d_fields2, 0, I.fields_bytes));
             for (auto& h : I.mons) HIP_OK(hipMemset(h.d_values, 0, monitor_bytes(S, h)));
         }
+        if (S.copy_stream) HIP_OK(hipStreamSynchronize(S.copy_stream));
+        S.cpy_pending[0] = S.cpy_pending[1] = false;
+        S.seg_count = 0;
         S.step = 0;
         S.parity = 0;
         S.totals_stale = false;
@@ -3541,6 +3614,14 @@ int rfdtd_set_design_permittivity(rfdtd_session* s, int32_t member, const int32_
         }
         launch(d_new, 0, base);
         HIP_OK(hipDeviceSynchronize());
+    });
+}
+
+int rfdtd_set_monitor_streaming(rfdtd_session* s, int32_t ring_samples) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (ring_samples != 0 && ring_samples < 2) RF_THROW(RFDTD_EINVAL, "ring_samples must be 0 or at least 2");
+        S.stream_ring = ring_samples;
     });
 }
 

@@ -96,6 +96,22 @@ def pack_monitors(monitors, Nx, Ny, Nz, Nt, dtype=np.float32):
     return mons, keep
 
 
+def pin_time_monitors(monitors):
+    """Move the value arrays of the time-domain monitors into page-locked host memory (pinned torch
+    tensors, viewed as numpy arrays that keep them alive), so that streamed monitor copies run
+    asynchronously into them."""
+    import torch
+
+    for d in monitors:
+        if "dtft_phase" in d:
+            continue
+        a = np.asarray(d["values"])
+        t = torch.empty(a.shape, dtype=getattr(torch, a.dtype.name), pin_memory=True)
+        v = t.numpy()
+        v[...] = a
+        d["values"] = v
+
+
 def pack_probes(probes, Nt, dtype=np.float32):
     """Validate probe dictionaries (solver.py:1541-1575; solver.cpp:132-170) and build the C records."""
     dtype = _real_dtype(dtype)

@@ -41,9 +41,19 @@ class Session:
         self._keep = []
         self.Nt = None
 
+    def set_monitor_streaming(self, ring_samples):
+        """Time-domain monitors installed afterwards keep ring_samples sample planes on the GPU and
+        stream the rest to their (page-locked) host arrays during the run; 0 turns it off."""
+        check(self._L.rfdtd_set_monitor_streaming(self._h, int(ring_samples)), "rfdtd_set_monitor_streaming")
+        self._stream_ring = int(ring_samples)
+
     def set_sources(self, probes, monitors, Nt):
-        """Install probe/source records and monitors (dictionaries as in the reference's solve())."""
-        from .hip_func import pack_monitors, pack_probes
+        """Install probe/source records and monitors (dictionaries as in the reference's solve()).
+        With monitor streaming on, time-domain monitor arrays are moved into pinned host memory."""
+        from .hip_func import pack_monitors, pack_probes, pin_time_monitors
+
+        if getattr(self, "_stream_ring", 0):
+            pin_time_monitors(monitors)
 
         prb, keep_p = pack_probes(probes, int(Nt), self.dtype)
         mons, keep_m = pack_monitors(monitors, self.Nx, self.Ny, self.Nz, int(Nt), self.dtype)
