@@ -375,6 +375,54 @@ def c5_leg(n_steps, flags=0):
                 finite=bool(all(np.isfinite(r[1]).all() for r in res)))
 
 
+def monitors_leg(n_steps=1000, ring=64):
+    """A field-visualisation run: the 400x400x200 workload with three time-domain plane monitors
+    sampled every step (Ez on the z mid-plane, Hy on the x mid-plane, Ex on the y mid-plane;
+    1.28 GB of samples over 1000 steps).  Timed from the start of the run until every sample is in
+    host memory: the whole series kept on the GPU and copied back at the end, against the monitors
+    streamed through a ring of `ring` planes into pinned host arrays during the run.  The two
+    results must be identical."""
+    from rfnetwork_amd.session import Session
+
+    model = workloads_dipole(n_steps)
+    cells = model.Nx * model.Ny * model.Nz
+    out, results = {}, {}
+    for mode in ("resident", "streamed"):
+        c, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+        with Session(c, Nx, Ny, Nz) as s:
+            del c
+            s.autotune()
+            if mode == "streamed":
+                s.set_monitor_streaming(ring)
+            s.set_sources(probes, monitors, Nt)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run(Nt)
+            s.read()
+            wall = time.perf_counter() - t0
+        results[mode] = [np.asarray(m["values"]) for m in monitors]
+        mon_bytes = sum(m.nbytes for m in results[mode])
+        out[mode] = dict(s_run_and_read=round(wall, 3), mcells=round(cells * Nt / wall / 1e6, 1))
+        log(f"monitors ({mode}): {wall:.3f}s from run start to all {mon_bytes / 1e9:.2f} GB of samples on the host")
+    same = all(np.array_equal(a, b) for a, b in zip(results["resident"], results["streamed"]))
+    plane_bytes = sum(m[0].nbytes for m in results["streamed"])
+    return dict(workload=f"dipole_slab 400x400x200, 3 time-domain planes every step, {n_steps} steps",
+                sample_bytes=int(mon_bytes), ring=ring, device_ring_bytes=int(ring * plane_bytes),
+                resident=out["resident"], streamed=out["streamed"],
+                speedup=round(out["resident"]["s_run_and_read"] / out["streamed"]["s_run_and_read"], 3),
+                identical=bool(same))
+
+
+def workloads_dipole(n_steps):
+    from rfnetwork_amd import workloads
+
+    model = workloads.dipole_slab(Nt=n_steps, monitors=False)
+    model.add_field_monitor("t_ez", "ez", "z", index=model.Nz // 2, n_step=1)
+    model.add_field_monitor("t_hy", "hy", "x", index=model.Nx // 2, n_step=1)
+    model.add_field_monitor("t_ex", "ex", "y", index=model.Ny // 2, n_step=1)
+    return model
+
+
 def adjoint_fd_leg(n_steps, every, n_cells=32, seed=41, h=1e-3, oracle_cells=1):
     """Validation (not a throughput figure): the adjoint dJ/deps_r of the 200x160x80 design model in
     float64 and float32 against central differences (Delta eps_r = h) of float64 forward solves on
@@ -441,7 +489,8 @@ def main():
                     help="comma list: plain (the headline), c1 (the 120x48x32 microstrip, GPU and CPU oracle), "
                          "batched (12 excitations), adjoint (gradients/s), "
                          "c5 (8 design variants, batched forward+adjoint, minutes), "
-                         "adjoint_fd (finite-difference validation, minutes)")
+                         "adjoint_fd (finite-difference validation, minutes), "
+                         "monitors (streamed time-domain monitors against resident ones)")
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--adjoint-steps", type=int, default=6000)
     ap.add_argument("--checkpoint-every", type=int, default=200)
@@ -473,6 +522,8 @@ def main():
         out["c5"] = c5_leg(args.c5_steps, gflags)
     if "adjoint_fd" in legs:
         out["adjoint_fd"] = adjoint_fd_leg(args.adjoint_steps, args.checkpoint_every)
+    if "monitors" in legs:
+        out["monitors"] = monitors_leg()
     print(json.dumps(out), flush=True)
 
 

@@ -210,10 +210,15 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
  * ring_samples sample planes on the device; the run is cut into segments producing at most half a
  * ring each, and after every segment the new planes are copied, on a copy stream of the session,
  * into the monitor's host array (rfdtd_monitor.values), overlapping the next segment.  For the
- * copies to be asynchronous that array should be page-locked (allocated with hipHostMalloc or as a
- * pinned tensor); any host array works.  rfdtd_read_monitors waits for the copies.  0 (default):
+ * copies to be asynchronous that array should be page-locked (rfdtd_host_alloc); any host array
+ * works.  rfdtd_read_monitors waits for the copies.  0 (default):
  * the whole series stays on the device until rfdtd_read_monitors. */
 int rfdtd_set_monitor_streaming(rfdtd_session* s, int32_t ring_samples);
+
+/* Page-locked host memory from this library's runtime (hipHostMalloc / hipHostFree), for the host
+ * arrays of streamed monitors (no counterpart in the reference). */
+int rfdtd_host_alloc(uint64_t bytes, void** ptr);
+int rfdtd_host_free(void* ptr);
 
 /* Design-region coefficient assembly on the GPU (the on-device counterpart of the host assembly
  * rfnetwork/solver.py:892-961 and the packing :1463-1523, for edges of plain dielectric): the 12 E

@@ -52,7 +52,7 @@ EXPORTS = (
     "rfdtd_set_checkpointing", "rfdtd_gradient", "rfdtd_autotune", "rfdtd_gradient_batch",
     "rfdtd_plan_checkpoints", "rfdtd_nf2ff", "rfdtd_set_history", "rfdtd_device_memory",
     "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
-    "rfdtd_set_monitor_streaming",
+    "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
 )
 
 
@@ -239,3 +239,28 @@ def gpu_visible() -> bool:
 
 def as_c(a: np.ndarray, dtype) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=dtype)
+
+
+class _PinnedOwner:
+    """Frees a page-locked host block of this library when the last array over it goes away."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().rfdtd_host_free(ctypes.c_void_p(self.ptr))
+        except Exception:
+            pass
+
+
+def pinned_empty(shape, dtype):
+    """A numpy array in page-locked host memory allocated by the library's runtime."""
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    ptr = ctypes.c_void_p()
+    check(lib().rfdtd_host_alloc(ctypes.c_uint64(max(n, 1)), ctypes.byref(ptr)), "rfdtd_host_alloc")
+    buf = (ctypes.c_char * max(n, 1)).from_address(ptr.value)
+    buf._owner = _PinnedOwner(ptr.value)
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)

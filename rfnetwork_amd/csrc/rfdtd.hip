@@ -3258,6 +3258,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     HIP_OK(hipStreamSynchronize(st));
     float ms = 0;
     if (hipEventElapsedTime(&ms, s.ev0, s.ev1) == hipSuccess) s.last_total_ms = ms;
+    else (void)hipGetLastError();  // clear the error the failed query leaves
 }
 
 // Checkpoint interval for a gradient over `items` after an Nt-step forward run: the interval K that
@@ -3399,7 +3400,10 @@ int rfdtd_synchronize(rfdtd_session* s) {
         Session& S = S_of(s);
         HIP_OK(hipDeviceSynchronize());
         float ms = 0;
+        // before any run the events hold no record: the query fails and leaves HIP's last error set,
+        // which a later hipGetLastError would report as the failure of an unrelated launch
         if (hipEventElapsedTime(&ms, S.ev0, S.ev1) == hipSuccess) S.last_total_ms = ms;
+        else (void)hipGetLastError();
         double se = 0, sh = 0;
         for (int k = 0; k < S.n_timed; ++k) {
             float a = 0, b = 0;
@@ -3614,6 +3618,22 @@ int rfdtd_set_design_permittivity(rfdtd_session* s, int32_t member, const int32_
         }
         launch(d_new, 0, base);
         HIP_OK(hipDeviceSynchronize());
+    });
+}
+
+int rfdtd_host_alloc(uint64_t bytes, void** ptr) {
+    return guarded([&] {
+        if (!ptr) RF_THROW(RFDTD_EINVAL, "null argument");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess)
+            RF_THROW(RFDTD_ENOMEM, "page-locked host allocation of " + std::to_string(bytes) + " bytes failed");
+    });
+}
+
+int rfdtd_host_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) HIP_OK(hipHostFree(ptr));
     });
 }
 

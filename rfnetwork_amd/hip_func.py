@@ -97,17 +97,14 @@ def pack_monitors(monitors, Nx, Ny, Nz, Nt, dtype=np.float32):
 
 
 def pin_time_monitors(monitors):
-    """Move the value arrays of the time-domain monitors into page-locked host memory (pinned torch
-    tensors, viewed as numpy arrays that keep them alive), so that streamed monitor copies run
+    """Move the value arrays of the time-domain monitors into page-locked host memory of the
+    library's runtime (numpy arrays that free it when they go), so that streamed monitor copies run
     asynchronously into them."""
-    import torch
-
     for d in monitors:
         if "dtft_phase" in d:
             continue
         a = np.asarray(d["values"])
-        t = torch.empty(a.shape, dtype=getattr(torch, a.dtype.name), pin_memory=True)
-        v = t.numpy()
+        v = _lib.pinned_empty(a.shape, a.dtype)
         v[...] = a
         d["values"] = v
 

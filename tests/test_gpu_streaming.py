@@ -80,3 +80,21 @@ def test_streamed_monitors_in_a_batch(hip_ready):
             np.testing.assert_array_equal(p["values"], r)
         for m, r in zip(monitors, ref[1]):
             np.testing.assert_array_equal(m["values"], r)
+
+
+def test_synchronize_before_the_first_run(hip_ready):
+    """A synchronize before any run (timing events not yet recorded) leaves no error behind for the
+    run that follows, streamed or not."""
+    print("[case] synchronize before run", flush=True)
+    model = workloads.tiny_model(Nt=40)
+    c, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    for ring in (0, 4):
+        _, probes, monitors, _, *_ = model.pack(gpu=True, coefficients=False)
+        with Session(c, Nx, Ny, Nz) as s:
+            if ring:
+                s.set_monitor_streaming(ring)
+            s.set_sources(probes, monitors, Nt)
+            s.synchronize()
+            s.run(Nt)
+            s.synchronize()
+            s.read()
