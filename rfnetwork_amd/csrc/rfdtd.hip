@@ -342,10 +342,12 @@ struct ProbeFold {
     const uint32_t* bits;
     const ProbeGroup<T>* groups;
     int ngroups, n;
-    __device__ __forceinline__ void apply(uint32_t cell, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) const {
-        if (!ngroups) return;
-        if ((ldg((const GLOBAL_AS uint32_t*)bits, (cell >> 5) * 4u) >> (cell & 31)) & 1u)
-            inject_cell(groups, ngroups, (int64_t)cell, n, true, xy, xz, yz, yx, zx, zy);
+    // the bitmask word of a cell, loaded with the rest of its plane's inputs (0 when nothing is folded)
+    __device__ __forceinline__ uint32_t word(uint32_t cell) const {
+        return ngroups ? ldg((const GLOBAL_AS uint32_t*)bits, (cell >> 5) * 4u) : 0u;
+    }
+    __device__ __forceinline__ void apply(uint32_t cell, uint32_t w, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) const {
+        if ((w >> (cell & 31)) & 1u) inject_cell(groups, ngroups, (int64_t)cell, n, true, xy, xz, yz, yx, zx, zy);
     }
 };
 
@@ -357,9 +359,15 @@ struct AdjFold {
     const uint32_t* bits;
     const ProbeGroup<T>* groups;
     int ngroups, n;
+    __device__ __forceinline__ uint32_t word(uint32_t cell) const {
+        return ngroups ? ldg((const GLOBAL_AS uint32_t*)bits, (cell >> 5) * 4u) : 0u;
+    }
     __device__ __forceinline__ void apply(uint32_t cell, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) const {
-        if (!ngroups) return;
-        if (!((ldg((const GLOBAL_AS uint32_t*)bits, (cell >> 5) * 4u) >> (cell & 31)) & 1u)) return;
+        apply(cell, word(cell), xy, xz, yz, yx, zx, zy);
+    }
+    // w: the cell's bitmask word, loaded ahead with the plane's inputs
+    __device__ __forceinline__ void apply(uint32_t cell, uint32_t w, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) const {
+        if (!((w >> (cell & 31)) & 1u)) return;
         int lo = 0, hi = ngroups;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
@@ -380,6 +388,21 @@ struct AdjFold {
     }
 };
 
+struct MonitorDev {
+    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id, fblk0, ring;  // ring: samples kept (0: all)
+    int64_t N1N2;
+    void* values;
+    const void* phase;
+    void* stage;  // phasor monitors: kStageSlots plane samples awaiting accumulation
+};
+
+// Running DFTs are accumulated every kStageSlots steps instead of every step: each step copies the
+// monitor plane into a staging slot, and a flush adds the staged samples to the accumulators in time
+// order, so the sums are the same float operations in the same order while the accumulators (Nf
+// complex values per plane cell) are read and written once per kStageSlots steps.
+constexpr int kStageSlots = 20;
+constexpr int kFreqGroup = 4;  // frequencies per thread in the flush
+
 // Everything one x plane of the E update reads, so that the next plane's loads can be issued
 // before this plane's arithmetic and stores (PF = true: two planes of loads in flight per lane).
 template <typename T>
@@ -387,6 +410,7 @@ struct EIn {
     T hx_c, hy_xn, hz_xn, hz_yn, hx_yn, hy_zn, hx_zn;
     T exy, exz, eyz, eyx, ezx, ezy;
     ECoef<T> c;
+    uint32_t pw;  // probe bitmask word of the cell
 };
 
 template <typename T>
@@ -394,6 +418,7 @@ struct HIn {
     T ex_c, ey_c, ez_c, ez_ym, ex_ym, ey_zm, ex_zm;
     T hxy, hxz, hyz, hyx, hzx, hzy;
     HCoef<T> c;
+    uint32_t pw;  // probe bitmask word of the cell
 };
 
 // E half step over one column of x planes.  Offsets are 32-bit byte offsets (grids up to 2^29
@@ -419,6 +444,7 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
         v.hy_zn = zin ? ldg(hy, b + S) : T(0);
         v.hx_zn = zin ? ldg(hx, b + S) : T(0);
         v.c = C.e(b);
+        v.pw = P.word(b / S);
         v.exy = ldg(ex_y, b); v.exz = ldg(ex_z, b); v.eyz = ldg(ey_z, b);
         v.eyx = ldg(ey_x, b); v.ezx = ldg(ez_x, b); v.ezy = ldg(ez_y, b);
     };
@@ -439,7 +465,7 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
         if (xin) eyx = c.ca_yx * eyx + c.cb_yx * (cur.hz_xn - hz_c);
         if (xin) ezx = c.ca_zx * ezx + c.cb_zx * (cur.hy_xn - hy_c);
         if (yin) ezy = c.ca_zy * ezy + c.cb_zy * (cur.hx_yn - cur.hx_c);
-        P.apply(b / S, exy, exz, eyz, eyx, ezx, ezy);
+        P.apply(b / S, cur.pw, exy, exz, eyz, eyx, ezx, ezy);
         stf<NT>(ex_y, b, exy); stf<NT>(ex_z, b, exz); stf<NT>(ey_z, b, eyz);
         stf<NT>(ey_x, b, eyx); stf<NT>(ez_x, b, ezx); stf<NT>(ez_y, b, ezy);
         stf<NT>(ex, b, exy + exz);
@@ -475,6 +501,7 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
         v.ey_zm = zlo ? ldg(ey, b - S) : T(0);
         v.ex_zm = zlo ? ldg(ex, b - S) : T(0);
         v.c = C.h(b);
+        v.pw = P.word(b / S);
         v.hxy = ldg(hx_y, b); v.hxz = ldg(hx_z, b); v.hyz = ldg(hy_z, b);
         v.hyx = ldg(hy_x, b); v.hzx = ldg(hz_x, b); v.hzy = ldg(hz_y, b);
     };
@@ -495,7 +522,7 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
         T hyx = c.da_yx * cur.hyx + (c.db_yx2 * cur.ez_c - c.db_yx1 * ez_xm);
         T hzx = c.da_zx * cur.hzx + (c.db_zx2 * cur.ey_c - c.db_zx1 * ey_xm);
         T hzy = c.da_zy * cur.hzy + (c.db_zy2 * cur.ex_c - c.db_zy1 * cur.ex_ym);
-        P.apply(b / S, hxy, hxz, hyz, hyx, hzx, hzy);
+        P.apply(b / S, cur.pw, hxy, hxz, hyz, hyx, hzx, hzy);
         stf<NT>(hx_y, b, hxy); stf<NT>(hx_z, b, hxz); stf<NT>(hy_z, b, hyz);
         stf<NT>(hy_x, b, hyx); stf<NT>(hz_x, b, hzx); stf<NT>(hz_y, b, hzy);
         stf<NT>(hx, b, hxy + hxz);
@@ -1091,6 +1118,7 @@ struct AEIn {
     HCoef<T> hn;                           // H coefficients at x + 1
     T ca_xy, ca_xz, ca_yz, ca_yx, ca_zx, ca_zy;
     T v_xy, v_xz, v_yz, v_yx, v_zx, v_zy;  // nu at the cell
+    uint32_t pw;                           // adjoint-source bitmask word of the cell
 };
 struct AEKeys {
     uint32_t hx, hy, hz, e;  // H entry at x + 1, y + 1, z + 1; E entry at the cell
@@ -1136,6 +1164,7 @@ __device__ __forceinline__ void adj_e_column_pf(const Fields<T>& F, const CA& C,
         const ECoef<T> ec = C.e_at(k.e);
         v.ca_xy = ec.ca_xy; v.ca_xz = ec.ca_xz; v.ca_yz = ec.ca_yz;
         v.ca_yx = ec.ca_yx; v.ca_zx = ec.ca_zx; v.ca_zy = ec.ca_zy;
+        v.pw = P.word(b / S);
     };
     uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
     HCoef<T> hc = C.h(b);
@@ -1166,7 +1195,7 @@ __device__ __forceinline__ void adj_e_column_pf(const Fields<T>& F, const CA& C,
         T o_yx = (xin ? cur.ca_yx : T(1)) * cur.v_yx + ey;
         T o_zx = (xin ? cur.ca_zx : T(1)) * cur.v_zx + ez;
         T o_zy = (yin ? cur.ca_zy : T(1)) * cur.v_zy + ez;
-        P.apply(b / S, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
+        P.apply(b / S, cur.pw, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
         stg(nxy, b, o_xy); stg(nxz, b, o_xz); stg(nyz, b, o_yz);
         stg(nyx, b, o_yx); stg(nzx, b, o_zx); stg(nzy, b, o_zy);
         hc = cur.hn;
@@ -1183,6 +1212,7 @@ struct AHIn {
     T w_xy_y, w_zy_y, w_xz_z, w_yz_z;            // Cb * nu at y - 1 and z - 1
     T da_xy, da_xz, da_yz, da_yx, da_zx, da_zy;  // H coefficients at the cell
     T m_xy, m_xz, m_yz, m_yx, m_zx, m_zy;        // mu at the cell
+    uint32_t pw;                                 // adjoint-source bitmask word of the cell
 };
 struct AHKeys {
     uint32_t e, ey, ez, h;  // E entry at the cell, y - 1, z - 1; H entry at the cell
@@ -1231,6 +1261,7 @@ __device__ __forceinline__ void adj_h_column_pf(const Fields<T>& F, const CA& C,
         v.da_yx = hc.da_yx; v.da_zx = hc.da_zx; v.da_zy = hc.da_zy;
         v.w_xy_y = c_xy * a_xy; v.w_zy_y = c_zy * a_zy;
         v.w_xz_z = c_xz * a_xz; v.w_yz_z = c_yz * a_yz;
+        v.pw = P.word(b / S);
     };
     uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
     T p_yx = T(0), p_zx = T(0);
@@ -1268,7 +1299,7 @@ __device__ __forceinline__ void adj_h_column_pf(const Fields<T>& F, const CA& C,
         T o_yx = cur.da_yx * cur.m_yx + hy;
         T o_zx = cur.da_zx * cur.m_zx + hz;
         T o_zy = cur.da_zy * cur.m_zy + hz;
-        P.apply(b / S, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
+        P.apply(b / S, cur.pw, o_xy, o_xz, o_yz, o_yx, o_zx, o_zy);
         stg(mxy, b, o_xy); stg(mxz, b, o_xz); stg(myz, b, o_yz);
         stg(myx, b, o_yx); stg(mzx, b, o_zx); stg(mzy, b, o_zy);
         p_yx = cur.cb_yx * cur.v_yx;
@@ -1561,22 +1592,6 @@ __global__ void __launch_bounds__(256) k_design_coefficients(DesignDev D, int co
     }
 }
 
-struct MonitorDev {
-    int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id, fblk0, ring;  // ring: samples kept (0: all)
-    int64_t N1N2;
-    void* values;
-    const void* phase;
-    void* stage;  // phasor monitors: kStageSlots plane samples awaiting accumulation
-};
-
-// Running DFTs are accumulated every kStageSlots steps instead of every step: each step copies the
-// monitor plane into a staging slot, and a flush adds the staged samples to the accumulators in time
-// order, so the sums are the same float operations in the same order while the accumulators (Nf
-// complex values per plane cell) are read and written once per kStageSlots steps.
-constexpr int kStageSlots = 10;
-constexpr int kFreqGroup = 4;  // frequencies per thread in the flush
-
-
 // The monitor whose block range holds this workgroup (blk0: sampling launch, fblk0: flush launch).
 __device__ __forceinline__ int monitor_of(const MonitorDev* __restrict__ mons, int nmon, int* sm, bool flush = false) {
     if (threadIdx.x == 0) {
@@ -1643,18 +1658,37 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
     }
     const T* phase = (const T*)M.phase;
     T* acc = (T*)M.values;
-    for (int f = f0; f < f1; ++f) {
-        T* c = acc + 2 * ((int64_t)f * M.N1N2 + q);
-        T re = c[0], im = c[1];
+    // every accumulator of the group is loaded before any is updated (the loads stay in flight
+    // together); per frequency the staged samples are added in time order
+    T re[kFreqGroup], im[kFreqGroup];
+#pragma unroll
+    for (int g = 0; g < kFreqGroup; ++g) {
+        re[g] = im[g] = T(0);
+        if (f0 + g < f1) {
+            const T* c = acc + 2 * ((int64_t)(f0 + g) * M.N1N2 + q);
+            re[g] = c[0];
+            im[g] = c[1];
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < kFreqGroup; ++g) {
+        const int f = f0 + g;
+        if (f >= f1) continue;
 #pragma unroll
         for (int j = 0; j < kStageSlots; ++j) {
             if (r[j] < 0) continue;
             const T* ph = phase + ((size_t)r[j] * M.nf + f) * 2;
-            re = re + v[j] * ph[0];
-            im = im + v[j] * ph[1];
+            re[g] = re[g] + v[j] * ph[0];
+            im[g] = im[g] + v[j] * ph[1];
         }
-        c[0] = re;
-        c[1] = im;
+    }
+#pragma unroll
+    for (int g = 0; g < kFreqGroup; ++g) {
+        if (f0 + g < f1) {
+            T* c = acc + 2 * ((int64_t)(f0 + g) * M.N1N2 + q);
+            c[0] = re[g];
+            c[1] = im[g];
+        }
     }
 }
 
@@ -2636,9 +2670,23 @@ void release_second_buffers(Session& s) {
     s.graph_dirty = true;
 }
 
+// A deterministic pseudo-random pattern in [-1e-3, 1e-3) over n elements: the autotuner's trials run
+// on non-zero fields, whose stores cost what a real run's do (zero data has been measured faster).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_pattern(T* p, int64_t n, uint32_t seed) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t)i * 0x9e3779b1u ^ seed;
+        h ^= h >> 16;
+        h *= 0x85ebca6bu;
+        h ^= h >> 13;
+        p[i] = (T)((double)(h & 0xffffu) / 32768.0 - 1.0) * (T)1e-3;
+    }
+}
+
 // Launch autotuner: times every (variant, block size, x planes per workgroup) candidate over
 // trial_steps field-only steps of this session's own grids and keeps the fastest.  The session must
-// not have run yet (step 0); its fields are zero before and after.
+// not have run yet (step 0); the trials run on a pseudo-random field pattern and the fields are zero
+// afterwards.
 template <typename T>
 void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     if (s.step != 0) RF_THROW(RFDTD_ESTATE, "autotune a session before its first run (or after rfdtd_reset)");
@@ -2671,6 +2719,9 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     double best = 1e300;
     int bv = v0, bblk = blk0, bxp = 0;
     s.tuning = true;
+    for (Instance& I : s.insts)
+        k_fill_pattern<T><<<1024, 256, 0, st>>>((T*)I.d_fields, (int64_t)(I.fields_bytes / sizeof(T)), 12345u);
+    HIP_OK(hipGetLastError());
     try {
         for (int v : {1, 4, 0, 2, 3, 5, 6})
             for (int blk : {128, 256, 512})
@@ -2687,6 +2738,11 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
                     float ms = 0;
                     HIP_OK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
                     const double per = (double)ms / trial_steps;
+                    if (s.flags & RFDTD_VERBOSE) {
+                        std::fprintf(stderr, "[rfdtd] autotune: variant %d, %d threads, x planes %d: %.4f ms/step\n", v,
+                                     blk, xp, per);
+                        std::fflush(stderr);
+                    }
                     if (per < best) {
                         best = per;
                         bv = v;
@@ -3154,14 +3210,17 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     };
     if (s.adj_block <= 0) {
         HIP_OK(hipStreamSynchronize(st));  // the descriptor uploads above read di and mda
-        // time the candidate shapes on the zeroed adjoint state (a linear map keeps it zero; it is
-        // cleared again below) and keep the fastest for this session
+        // time the candidate shapes on a pseudo-random adjoint state (zero data runs faster than
+        // real data; the state is cleared again below) and keep the fastest for this session
         std::vector<int> chunk_opts;
         int maxNx = 1;
         for (const GradItem& it : items) maxNx = std::max(maxNx, s.members[s.insts[it.inst].member].Nx);
         for (int c : {1, 2, 4, 8, 16, 32, 64, 128, 256})
             if (c < maxNx) chunk_opts.push_back(c);
         chunk_opts.push_back(maxNx);  // one plane per workgroup
+        for (int k = 0; k < n; ++k)
+            k_fill_pattern<T><<<1024, 256, 0, st>>>((T*)di[n + k].F.ex_y,
+                                                     (int64_t)(18 * s.insts[items[k].inst].pitch / sizeof(T)), 777u);
         double best = 1e300;
         int bpf = 1, bblk = ablock, bch = 1;
         const int reps = 3;
