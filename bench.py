@@ -41,12 +41,17 @@ METRIC = "Mcell-updates/s + % HBM peak, batched vs plain; adjoint gradients/s, 1
 MEASURED_TRAFFIC = {
     ("dipole_slab", "table", k, shape): v
     for shape, (e, h) in {
-        (4, 256, 4): (2543200000, 2501400000),
+        (4, 512, 4): (2543600000, 2501600000),
+        (1, 512, 4): (2563200000, 2504900000),
+        (4, 256, 4): (2542900000, 2501400000),
         (1, 256, 4): (2572700000, 2501900000),
-        (1, 128, 200): (2497900000, 2455000000),
-        (4, 128, 200): (2490300000, 2448600000),
-        (1, 256, 200): (2500000000, 2456700000),
+        (4, 128, 4): (2543100000, 2501400000),
+        (1, 128, 4): (2571000000, 2502300000),
+        (4, 512, 200): (2552000000, 2526600000),
         (4, 256, 200): (2493700000, 2452600000),
+        (1, 256, 200): (2500000000, 2456700000),
+        (4, 128, 200): (2496800000, 2452400000),
+        (1, 128, 200): (2497900000, 2455000000),
     }.items()
     for k, v in (("k_update_e", e), ("k_update_h", h))
 }
