@@ -126,8 +126,8 @@ def c1_leg(flags=0):
     c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True)
     cells = Nx * Ny * Nz
     with Session(c, Nx, Ny, Nz, flags=flags) as s:
+        s.set_sources(probes, monitors, Nt)  # installed first: the tuner's trials then apply them too
         tuned = s.autotune()
-        s.set_sources(probes, monitors, Nt)
         t0 = time.perf_counter()
         s.run(Nt)
         s.synchronize()
@@ -173,12 +173,12 @@ def batched_leg(n_steps, flags=0):
         f"{exc_cells} excitation-cells, built in {time.perf_counter() - t0:.1f}s")
 
     b = rb.Batch(grids, flags=flags)
-    log(f"batched: autotune {b.autotune()}")
     k = 0
     for _, _, exc, Nt in inputs:
         for _, probes, monitors in exc:
             b.set_sources(k, probes, monitors, Nt)
             k += 1
+    log(f"batched: autotune {b.autotune()}")  # after the sources, as for the sequential sessions
     t0 = time.perf_counter()
     b.run(n_steps)
     b.synchronize()
@@ -194,8 +194,8 @@ def batched_leg(n_steps, flags=0):
             exc2, _ = rb.pack_excitations(m, [port + 1])
             _, probes, monitors = exc2[0]
             s = Session(coeff, m.Nx, m.Ny, m.Nz, flags=flags)
-            s.autotune()
             s.set_sources(probes, monitors, Nt)
+            s.autotune()
             sessions.append((s, probes))
     t0 = time.perf_counter()
     for s, _ in sessions:
@@ -260,9 +260,9 @@ def batched_leg(n_steps, flags=0):
     sexc, sNt = rb.pack_excitations(sm, [1, 2] * 6)
     scoef = sm.coefficient_arrays(gpu=True)
     with rb.Batch([(scoef, sm.Nx, sm.Ny, sm.Nz, len(sexc))], flags=flags) as sb:
-        sb.autotune()
         for k, (_, probes, monitors) in enumerate(sexc):
             sb.set_sources(k, probes, monitors, sNt)
+        sb.autotune()
         t0 = time.perf_counter()
         sb.run(sNt)
         sb.synchronize()
@@ -271,8 +271,8 @@ def batched_leg(n_steps, flags=0):
     for port in [1, 2] * 6:
         (_, probes, monitors), = rb.pack_excitations(sm, [port])[0]
         ss = Session(scoef, sm.Nx, sm.Ny, sm.Nz, flags=flags)
-        ss.autotune()
         ss.set_sources(probes, monitors, sNt)
+        ss.autotune()
         singles.append(ss)
     t0 = time.perf_counter()
     for ss in singles:
@@ -553,11 +553,12 @@ def plain_leg(args):
         if args.variant >= 0:
             s.set_variant(args.variant)
         tuned = dict(variant=args.variant if args.variant >= 0 else 1, block=args.block or 256, x_planes=args.xplanes, cached=None)
+        s.set_sources(probes, monitors, Nt)
     else:
+        s.set_sources(probes, monitors, Nt)  # installed first: the tuner's trials then apply them too
         t1 = time.perf_counter()
         tuned = s.autotune()
         log(f"autotune {tuned} in {time.perf_counter() - t1:.1f}s")
-    s.set_sources(probes, monitors, Nt)
     info = s.memory_info()
     log(f"session ready in {time.perf_counter() - t0:.1f}s: {info}")
 

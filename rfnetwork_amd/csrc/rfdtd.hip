@@ -2683,6 +2683,8 @@ __global__ void __launch_bounds__(256) k_fill_pattern(T* p, int64_t n, uint32_t 
     }
 }
 
+size_t monitor_bytes(const Session& s, const MonitorHost& h);
+
 // Launch autotuner: times every (variant, block size, x planes per workgroup) candidate over
 // trial_steps field-only steps of this session's own grids and keeps the fastest.  The session must
 // not have run yet (step 0); the trials run on a pseudo-random field pattern and the fields are zero
@@ -2718,7 +2720,18 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     hipStream_t st = s.own_stream;
     double best = 1e300;
     int bv = v0, bblk = blk0, bxp = 0;
-    s.tuning = true;
+    // With sources installed (and room for the trial steps) the trials apply them as a run does, so
+    // that their cost is timed too; the probe rows and monitor values are restored afterwards and
+    // the design-box history slots are rewritten by the run.  Otherwise field updates only.
+    bool with_io = false, room = true;
+    for (const Instance& I : s.insts) {
+        with_io |= I.P > 0 || !I.mons.empty();
+        if (I.P > 0 && s.Nt_probes < trial_steps) room = false;
+        if (!I.mons.empty() && s.Nt_monitors < trial_steps) room = false;
+    }
+    for (const auto& h : s.hist) room &= h.capacity >= trial_steps;
+    with_io &= room;
+    s.tuning = !with_io;
     for (Instance& I : s.insts)
         k_fill_pattern<T><<<1024, 256, 0, st>>>((T*)I.d_fields, (int64_t)(I.fields_bytes / sizeof(T)), 12345u);
     HIP_OK(hipGetLastError());
@@ -2761,6 +2774,11 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     for (Instance& I : s.insts) {
         HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
         if (I.d_fields2) HIP_OK(hipMemset(I.d_fields2, 0, I.fields_bytes));
+        if (with_io) {
+            if (I.P)
+                HIP_OK(hipMemcpy(I.d_series, I.d_series_in, (size_t)s.Nt_probes * I.P * s.esz, hipMemcpyDeviceToDevice));
+            for (auto& h : I.mons) HIP_OK(hipMemset(h.d_values, 0, monitor_bytes(s, h)));
+        }
     }
     s.parity = 0;
     s.totals_stale = false;

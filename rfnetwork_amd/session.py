@@ -113,8 +113,9 @@ class Session:
         check(self._L.rfdtd_set_launch(self._h, int(block_threads), int(x_planes), int(graph_steps)))
 
     def autotune(self, trial_steps=20, use_cache=True):
-        """Pick the fastest launch shape for this grid (before the first run); cached per shape.
-        Returns {variant, block, x_planes, ms_per_step, cached}."""
+        """Pick the fastest launch shape for this grid (before the first run); cached per shape.  With
+        sources already installed the trials apply them as a run does (then restored), so their
+        cost is part of the timing.  Returns {variant, block, x_planes, ms_per_step, cached}."""
         from . import autotune
 
         kind = self.memory_info()["coef_kind"]

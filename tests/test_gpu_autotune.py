@@ -106,3 +106,30 @@ def test_adjoint_shape_timed_once_and_cached(hip_ready, tmp_path, monkeypatch):
     assert shapes[0]["block"] == 0  # nothing cached yet: timed at the gradient
     assert shapes[2] == shapes[1] == shapes[3]  # cached, applied before the second gradient
     np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_autotune_with_sources_installed_leaves_them_intact(hip_ready, tmp_path, monkeypatch):
+    """Tuning after set_sources times the trials with probes and monitors applied, then restores
+    them: the run afterwards equals the oracle bit for bit."""
+    print("[case] autotune with sources", flush=True)
+    monkeypatch.setenv("RFNETWORK_AMD_TUNE_CACHE", str(tmp_path / "launch.json"))
+    autotune._memory.clear()
+    model = workloads.tiny_model(Nt=40)
+    c, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    with Session(c, Nx, Ny, Nz) as s:
+        s.set_sources(probes, monitors, Nt)
+        s.autotune(trial_steps=5)
+        s.run(Nt)
+        s.synchronize()
+        s.read()
+        f = s.fields()
+    p_c, m_c, f_c = run_oracle(model, np.float32, n_threads=2)
+    for a, b in zip(probes, p_c):
+        np.testing.assert_array_equal(a["values"], b["values"])
+    model.unpack(probes, monitors, gpu=True)
+    g = {k: np.array(m["values"]) for k, m in model.monitors.items()}
+    model.unpack(p_c, m_c, gpu=False)
+    for k, m in model.monitors.items():
+        np.testing.assert_array_equal(g[k], np.array(m["values"]), err_msg=k)
+    for k in f_c:
+        np.testing.assert_array_equal(f[k], f_c[k], err_msg=k)
