@@ -418,6 +418,36 @@ def monitors_leg(n_steps=1000, ring=64):
                 identical=bool(same))
 
 
+def dft64_leg(n_steps=1000):
+    """The 400x400x200 workload with its 6 running-DFT planes (51 frequencies) accumulated in
+    complex64, as the reference does, and in complex128 (float32 fields): ms per step of each, tuned,
+    and how far the complex64 sums lie from the complex128 ones."""
+    from rfnetwork_amd import workloads
+    from rfnetwork_amd.session import Session
+
+    out, sums = {}, {}
+    for name, cdt in (("complex64", None), ("complex128", np.complex128)):
+        model = workloads.dipole_slab(Nt=n_steps, dft_dtype=cdt)
+        c, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+        with Session(c, Nx, Ny, Nz) as s:
+            del c
+            s.set_sources(probes, monitors, Nt)
+            tuned = s.autotune()
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run(Nt)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            s.read()
+        sums[name] = [np.asarray(m["values"]) for m in monitors]
+        out[name] = dict(ms_per_step=round(wall / Nt * 1e3, 4), launch=tuned,
+                         accumulator_bytes=int(sum(v.nbytes for v in sums[name])))
+        log(f"dft64 ({name}): {wall / Nt * 1e3:.4f} ms/step")
+    rel = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(sums["complex64"], sums["complex128"]))
+    return dict(workload=f"dipole_slab 400x400x200, 6 DFT planes x 51 frequencies, {n_steps} steps", **out,
+                max_rel_diff_complex64_vs_complex128=rel)
+
+
 def workloads_dipole(n_steps):
     from rfnetwork_amd import workloads
 
@@ -485,7 +515,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dipole_slab")
     ap.add_argument("--coef", choices=("auto", "raw", "table"), default="auto")
-    ap.add_argument("--variant", type=int, default=-1, help="kernel variant 0..4 (default: autotuned)")
+    ap.add_argument("--variant", type=int, default=-1, help="kernel variant 0..6 (default: autotuned)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--xplanes", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
@@ -495,7 +525,8 @@ def main():
                          "batched (12 excitations), adjoint (gradients/s), "
                          "c5 (8 design variants, batched forward+adjoint, minutes), "
                          "adjoint_fd (finite-difference validation, minutes), "
-                         "monitors (streamed time-domain monitors against resident ones)")
+                         "monitors (streamed time-domain monitors against resident ones), "
+                         "dft64 (the running DFTs accumulated in complex128 against complex64)")
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--adjoint-steps", type=int, default=6000)
     ap.add_argument("--checkpoint-every", type=int, default=200)
@@ -529,6 +560,8 @@ def main():
         out["adjoint_fd"] = adjoint_fd_leg(args.adjoint_steps, args.checkpoint_every)
     if "monitors" in legs:
         out["monitors"] = monitors_leg()
+    if "dft64" in legs:
+        out["dft64"] = dft64_leg()
     print(json.dumps(out), flush=True)
 
 

@@ -66,14 +66,19 @@ typedef struct rfdtd_probe {
 /* One plane monitor (solver.py:1597-1620; solver.cpp:351-463 with gpu=1).  position is the global
  * field index along axis.  n_frequencies == 0: values is float[Nt/n_step+1][N1][N2] (time samples);
  * otherwise values is complex64[n_frequencies][N1][N2] (running DFT) and dtft_phase is
- * complex64[Nt/n_step+1][n_frequencies].  (N1,N2) = (Ny,Nz), (Nx,Nz), (Nx,Ny) for axis 0, 1, 2. */
+ * complex64[Nt/n_step+1][n_frequencies].  (N1,N2) = (Ny,Nz), (Nx,Nz), (Nx,Ny) for axis 0, 1, 2.
+ * In a float64 session every one of these is double / complex128.  flags: 0, or
+ * RFDTD_MONITOR_DFT64 for a running DFT accumulated in double in a float32 session (values and
+ * dtft_phase complex128; the field samples stay float32).  The reference accumulates in complex64
+ * only (solver.cpp:1194-1225), so flags = 0 is its behaviour. */
+#define RFDTD_MONITOR_DFT64 0x1u
 typedef struct rfdtd_monitor {
     int32_t axis;
     int32_t position;
     int32_t field;
     int32_t n_step;
     int32_t n_frequencies;
-    int32_t reserved;
+    int32_t flags;
     void* values;
     const void* dtft_phase;
 } rfdtd_monitor;

@@ -24,6 +24,7 @@ NO_GRAPHS = 0x8
 QUIET = 0x10
 KERNEL_TIMING = 0x20
 VERBOSE = 0x40
+MONITOR_DFT64 = 0x1  # rfdtd_monitor.flags: running DFT accumulated in complex128
 
 # Coefficient keys in the order of the C ABI (rfnetwork/solver.py:1473-1518).
 COEFF_NAMES = (
@@ -73,7 +74,7 @@ class Monitor(ctypes.Structure):
         ("field", ctypes.c_int32),
         ("n_step", ctypes.c_int32),
         ("n_frequencies", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
         ("values", ctypes.c_void_p),
         ("dtft_phase", ctypes.c_void_p),
     ]

@@ -71,8 +71,10 @@ enum { EX = 0, EY = 1, EZ = 2, HX = 3, HY = 4, HZ = 5 };
 // Global address space qualifier for device code (the host pass parses device functions too).
 #if defined(__HIP_DEVICE_COMPILE__)
 #define GLOBAL_AS __attribute__((address_space(1)))
+#define CONST_AS __attribute__((address_space(4)))  // read-only for the whole kernel: scalar loads
 #else
 #define GLOBAL_AS
+#define CONST_AS
 #endif
 
 // ------------------------------------------------------------------------------------------------
@@ -390,6 +392,7 @@ struct AdjFold {
 
 struct MonitorDev {
     int32_t inst, axis, pos, field, n_step, nf, N1, N2, blk0, id, fblk0, ring;  // ring: samples kept (0: all)
+    int32_t acc64;  // running DFT accumulated in double (values and phase double2) in a float session
     int64_t N1N2;
     void* values;
     const void* phase;
@@ -401,7 +404,7 @@ struct MonitorDev {
 // order, so the sums are the same float operations in the same order while the accumulators (Nf
 // complex values per plane cell) are read and written once per kStageSlots steps.
 constexpr int kStageSlots = 20;
-constexpr int kFreqGroup = 4;  // frequencies per thread in the flush
+constexpr int kFreqGroup = 8;  // frequencies whose accumulators the flush loads together
 
 // Everything one x plane of the E update reads, so that the next plane's loads can be issued
 // before this plane's arithmetic and stores (PF = true: two planes of loads in flight per lane).
@@ -1592,24 +1595,28 @@ __global__ void __launch_bounds__(256) k_design_coefficients(DesignDev D, int co
     }
 }
 
-// The monitor whose block range holds this workgroup (blk0: sampling launch, fblk0: flush launch).
-__device__ __forceinline__ int monitor_of(const MonitorDev* __restrict__ mons, int nmon, int* sm, bool flush = false) {
-    if (threadIdx.x == 0) {
-        int m = 0;
-        while (m + 1 < nmon && (flush ? mons[m + 1].fblk0 : mons[m + 1].blk0) <= (int)blockIdx.x) ++m;
-        *sm = m;
+// The monitor whose block range holds this workgroup (blk0: sampling launch, fblk0: flush launch):
+// the last one whose first block is at most blockIdx.x.  Every wavefront finds it on its own from 64
+// table entries per load, so the lookup costs one memory round trip (no serial walk, no barrier).
+__device__ __forceinline__ int monitor_of(const MonitorDev* __restrict__ mons, int nmon, int block, bool flush = false) {
+    const int lane = (int)(threadIdx.x & 63);
+    int count = 0;
+    for (int b = 0; b < nmon; b += 64) {
+        const int k = b + lane;
+        const int first = k < nmon ? (flush ? mons[k].fblk0 : mons[k].blk0) : INT_MAX;
+        count += __popcll(__ballot(first <= block));
     }
-    __syncthreads();
-    return __builtin_amdgcn_readfirstlane(*sm);
+    return __builtin_amdgcn_readfirstlane(count - 1);
 }
 
 // Sample step n: time-domain monitors write their plane row; phasor monitors copy the plane into
 // staging slot `slot` and record the phase row of the sample (-1: not sampled at this step).
+// totals = 1 when the step's kernels stored the six totals (the separate E and H kernels do): the
+// sample is then one load of the total instead of two of its splits (the same sum, formed once).
 template <typename T>
 __global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDev* __restrict__ mons, int nmon,
-                           const int* base, int local, int slot, int* __restrict__ rows) {
-    __shared__ int sm;
-    const MonitorDev M = mons[monitor_of(mons, nmon, &sm)];
+                           const int* base, int local, int slot, int* __restrict__ rows, int totals) {
+    const MonitorDev M = mons[monitor_of(mons, nmon, (int)blockIdx.x)];
     const int n = step_of(base, local);
     const bool sampled = n == 0 || (n % M.n_step) == 0;
     const int64_t q = (int64_t)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
@@ -1625,71 +1632,92 @@ __global__ void k_monitors(const InstDev<T>* __restrict__ insts, const MonitorDe
         idx = ((int64_t)a * Ny + M.pos) * Nz + b;
     else
         idx = ((int64_t)a * Ny + b) * Nz + M.pos;
-    const T v = total_at(I, M.field, idx);
+    const T* tot[6] = {I.F.ex, I.F.ey, I.F.ez, I.F.hx, I.F.hy, I.F.hz};
+    const T v = totals ? ((const GLOBAL_T(T)*)tot[M.field])[idx] : total_at(I, M.field, idx);
     if (M.nf == 0) {
         const int64_t m = n / M.n_step;
-        ((T*)M.values)[(M.ring ? m % M.ring : m) * M.N1N2 + q] = v;
+        ((GLOBAL_T(T)*)M.values)[(M.ring ? m % M.ring : m) * M.N1N2 + q] = v;
     } else {
-        ((T*)M.stage)[(int64_t)slot * M.N1N2 + q] = v;
+        ((GLOBAL_T(T)*)M.stage)[(int64_t)slot * M.N1N2 + q] = v;
     }
 }
 
+// Frequencies per flush thread: their accumulators are loaded together.
+template <typename A>
+constexpr int flush_group() { return sizeof(A) == sizeof(double) ? kFreqGroup / 2 : kFreqGroup; }
+
 // Accumulate staged slots [0, nslots): values[f] += v_s * phase[row_s, f] for s in time order (a real
 // field times a complex phase: one product and one sum per component, as each step would).  A thread
-// owns one plane cell and kFreqGroup frequencies; consecutive workgroups cover consecutive cells of
-// one frequency group.
-template <typename T>
-__global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, int nslots, const int* __restrict__ rows) {
-    __shared__ int sm;
-    const MonitorDev M = mons[monitor_of(mons, nmon, &sm, true)];
-    if (M.nf == 0) return;
-    const int64_t qblocks = (M.N1N2 + blockDim.x - 1) / blockDim.x;
-    const int64_t local = (int64_t)blockIdx.x - M.fblk0;
-    const int fg = (int)(local / qblocks);
-    const int64_t q = (local - fg * qblocks) * blockDim.x + threadIdx.x;
+// owns one plane cell and one group of frequencies; the groups of a block of cells are consecutive
+// logical workgroups, which xcd_logical keeps on one XCD, so the staged samples that every group
+// reads come from that XCD's L2 after the first.  Within a group the samples are taken in time
+// order, each against the group's contiguous run of phase entries.  A is the accumulator and phase
+// type: T, or double for a float session's monitor accumulated in double (each float sample is
+// widened exactly, then multiplied and added in double).
+template <typename T, typename A>
+__device__ __forceinline__ void flush_monitor(const MonitorDev& M, int local, int nslots, const int* __restrict__ rows) {
+    constexpr int G = flush_group<A>();
+    const int ngroups = (M.nf + G - 1) / G;
+    const int cblock = local / ngroups;
+    const int f0 = (local - cblock * ngroups) * G;
+    const int64_t q = (int64_t)cblock * blockDim.x + threadIdx.x;
     if (q >= M.N1N2) return;
-    const int f0 = fg * kFreqGroup, f1 = min(M.nf, f0 + kFreqGroup);
-    T v[kStageSlots];
+    const int nf = M.nf;
+    const int64_t fs = 2 * M.N1N2;  // between the accumulators of consecutive frequencies
+    GLOBAL_T(A)* acc = (GLOBAL_T(A)*)M.values + 2 * q + f0 * fs;
+    A re[G], im[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        re[g] = f0 + g < nf ? acc[g * fs] : A(0);
+        im[g] = f0 + g < nf ? acc[g * fs + 1] : A(0);
+    }
+    A v[kStageSlots];
     int r[kStageSlots];
+    // the sample rows and the phase table are read-only during the flush and the same for every
+    // lane: scalar loads
+    const CONST_AS int* srows = (const CONST_AS int*)rows + M.id * kStageSlots;
+    const GLOBAL_T(T)* stage = (const GLOBAL_T(T)*)M.stage + q;
 #pragma unroll
     for (int j = 0; j < kStageSlots; ++j) {
-        r[j] = j < nslots ? rows[M.id * kStageSlots + j] : -1;
-        v[j] = r[j] >= 0 ? ((const T*)M.stage)[(int64_t)j * M.N1N2 + q] : T(0);
+        r[j] = j < nslots ? srows[j] : -1;
+        v[j] = r[j] >= 0 ? (A)stage[(int64_t)j * M.N1N2] : A(0);
     }
-    const T* phase = (const T*)M.phase;
-    T* acc = (T*)M.values;
-    // every accumulator of the group is loaded before any is updated (the loads stay in flight
-    // together); per frequency the staged samples are added in time order
-    T re[kFreqGroup], im[kFreqGroup];
+    const CONST_AS A* phase = (const CONST_AS A*)M.phase;
 #pragma unroll
-    for (int g = 0; g < kFreqGroup; ++g) {
-        re[g] = im[g] = T(0);
-        if (f0 + g < f1) {
-            const T* c = acc + 2 * ((int64_t)(f0 + g) * M.N1N2 + q);
-            re[g] = c[0];
-            im[g] = c[1];
+    for (int j = 0; j < kStageSlots; ++j) {
+        if (r[j] < 0) continue;
+        const CONST_AS A* ph = phase + ((size_t)r[j] * nf + f0) * 2;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (f0 + g >= nf) break;
+            re[g] = re[g] + v[j] * ph[2 * g];
+            im[g] = im[g] + v[j] * ph[2 * g + 1];
         }
     }
 #pragma unroll
-    for (int g = 0; g < kFreqGroup; ++g) {
-        const int f = f0 + g;
-        if (f >= f1) continue;
-#pragma unroll
-        for (int j = 0; j < kStageSlots; ++j) {
-            if (r[j] < 0) continue;
-            const T* ph = phase + ((size_t)r[j] * M.nf + f) * 2;
-            re[g] = re[g] + v[j] * ph[0];
-            im[g] = im[g] + v[j] * ph[1];
+    for (int g = 0; g < G; ++g) {
+        if (f0 + g < nf) {
+            acc[g * fs] = re[g];
+            acc[g * fs + 1] = im[g];
         }
     }
-#pragma unroll
-    for (int g = 0; g < kFreqGroup; ++g) {
-        if (f0 + g < f1) {
-            T* c = acc + 2 * ((int64_t)(f0 + g) * M.N1N2 + q);
-            c[0] = re[g];
-            c[1] = im[g];
+}
+
+// Launched on a multiple of 8 workgroups (xcd_logical); logical blocks past `total` have no work.
+template <typename T>
+__global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, int total, int nslots,
+                                const int* __restrict__ rows) {
+    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    if (L >= total) return;
+    const MonitorDev M = mons[monitor_of(mons, nmon, L, true)];
+    if (M.nf == 0) return;
+    if constexpr (std::is_same<T, float>::value) {
+        if (M.acc64) {
+            flush_monitor<float, double>(M, L - M.fblk0, nslots, rows);
+            return;
         }
     }
+    flush_monitor<T, T>(M, L - M.fblk0, nslots, rows);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2390,6 +2418,7 @@ void refresh(Session& s) {
             d.axis = h.desc.axis; d.pos = h.pos; d.field = h.desc.field; d.n_step = h.desc.n_step;
             d.nf = h.desc.n_frequencies; d.N1 = h.N1; d.N2 = h.N2;
             d.ring = h.ring;
+            d.acc64 = (h.desc.flags & RFDTD_MONITOR_DFT64) && !s.f64;
             d.N1N2 = (int64_t)h.N1 * h.N2;
             d.blk0 = blocks;
             d.values = h.d_values;
@@ -2397,7 +2426,8 @@ void refresh(Session& s) {
             d.stage = (void*)stage_elems;  // offset for now
             if (d.nf) stage_elems += (size_t)kStageSlots * d.N1N2;
             d.fblk0 = fblocks;
-            fblocks += (int)((d.N1N2 + 255) / 256) * ((d.nf + kFreqGroup - 1) / kFreqGroup);
+            const int G = (d.acc64 || s.f64) ? flush_group<double>() : flush_group<float>();
+            fblocks += d.nf ? (int)((d.N1N2 + 255) / 256) * ((d.nf + G - 1) / G) : 0;
             blocks += (int)((d.N1N2 + 255) / 256);
             md.push_back(d);
         }
@@ -2478,9 +2508,11 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     }
 
     if (io && s.n_active) {
-        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local, slot, s.d_rows);
+        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local, slot, s.d_rows,
+                                                    s.fused ? 0 : 1);
         if (flush && s.flush_blocks)
-            k_monitor_flush<T><<<s.flush_blocks, 256, 0, st>>>(s.d_mons, s.n_active, flush, s.d_rows);
+            k_monitor_flush<T><<<(s.flush_blocks + 7) & ~7, 256, 0, st>>>(s.d_mons, s.n_active, s.flush_blocks, flush,
+                                                                       s.d_rows);
     }
 }
 
@@ -2684,6 +2716,10 @@ __global__ void __launch_bounds__(256) k_fill_pattern(T* p, int64_t n, uint32_t 
 }
 
 size_t monitor_bytes(const Session& s, const MonitorHost& h);
+// bytes of one real component of a running-DFT accumulator or phase entry
+inline size_t dft_element_size(const Session& s, const MonitorHost& h) {
+    return (h.desc.flags & RFDTD_MONITOR_DFT64) ? sizeof(double) : s.esz;
+}
 
 // Launch autotuner: times every (variant, block size, x planes per workgroup) candidate over
 // trial_steps field-only steps of this session's own grids and keeps the fastest.  The session must
@@ -2931,6 +2967,9 @@ void set_monitors(Session& s, int inst, const rfdtd_monitor* mons, int n, int Nt
         if (d.axis < 0 || d.axis > 2 || d.field < 0 || d.field > 5 || d.n_step < 1 || d.n_frequencies < 0)
             RF_THROW(RFDTD_EINVAL, "invalid monitor description");
         if (!d.values || (d.n_frequencies > 0 && !d.dtft_phase)) RF_THROW(RFDTD_EINVAL, "monitor arrays missing");
+        if (d.flags & ~(int32_t)RFDTD_MONITOR_DFT64) RF_THROW(RFDTD_EINVAL, "unknown monitor flags");
+        if ((d.flags & RFDTD_MONITOR_DFT64) && d.n_frequencies == 0)
+            RF_THROW(RFDTD_EINVAL, "RFDTD_MONITOR_DFT64 is for running-DFT monitors");
         MonitorHost h;
         h.desc = d;
         const int f = d.field;
@@ -2947,13 +2986,12 @@ void set_monitors(Session& s, int inst, const rfdtd_monitor* mons, int n, int Nt
         h.active = pos >= 0 && pos < along;
         const size_t plane = (size_t)h.N1 * h.N2;
         if (s.stream_ring > 0 && !d.n_frequencies && h.active) h.ring = std::min(h.Nm, s.stream_ring);
-        const size_t vbytes = d.n_frequencies ? (size_t)d.n_frequencies * plane * 2 * s.esz
-                              : (size_t)(h.ring ? h.ring : h.Nm) * plane * s.esz;
+        const size_t vbytes = monitor_bytes(s, h);
         HIP_OK(hipMalloc(&h.d_values, vbytes));
         if (h.ring) HIP_OK(hipMemset(h.d_values, 0, vbytes));  // the host array keeps unsampled rows
         else HIP_OK(hipMemcpy(h.d_values, d.values, vbytes, hipMemcpyHostToDevice));
         if (d.n_frequencies) {
-            const size_t pbytes = (size_t)h.Nm * d.n_frequencies * 2 * s.esz;
+            const size_t pbytes = (size_t)h.Nm * d.n_frequencies * 2 * dft_element_size(s, h);
             HIP_OK(hipMalloc(&h.d_phase, pbytes));
             HIP_OK(hipMemcpy(h.d_phase, d.dtft_phase, pbytes, hipMemcpyHostToDevice));
         }
@@ -2977,7 +3015,7 @@ void read_probes(Session& s) {
 
 size_t monitor_bytes(const Session& s, const MonitorHost& h) {
     const size_t plane = (size_t)h.N1 * h.N2;
-    return h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * s.esz
+    return h.desc.n_frequencies ? (size_t)h.desc.n_frequencies * plane * 2 * dft_element_size(s, h)
                                 : (size_t)(h.ring ? h.ring : h.Nm) * plane * s.esz;
 }
 

@@ -84,9 +84,13 @@ def pack_monitors(monitors, Nx, Ny, Nz, Nt, dtype=np.float32):
         rec.position = int(d["position"])
         if "dtft_phase" in d:
             nf = int(d["n_frequencies"])
-            _check_array(d["dtft_phase"], "dtft_phase", (Nm, nf), cdtype)
-            _check_array(d["values"], "values", (nf, N1, N2), cdtype)
+            # complex128 arrays in a float32 solve: the running DFT is accumulated in double
+            wide = dtype == np.float32 and getattr(d["values"], "dtype", None) == np.complex128
+            mdtype = np.dtype(np.complex128) if wide else cdtype
+            _check_array(d["dtft_phase"], "dtft_phase", (Nm, nf), mdtype)
+            _check_array(d["values"], "values", (nf, N1, N2), mdtype)
             rec.n_frequencies = nf
+            rec.flags = _lib.MONITOR_DFT64 if wide else 0
             rec.dtft_phase = d["dtft_phase"].ctypes.data
         else:
             _check_array(d["values"], "values", (Nm, N1, N2), dtype)

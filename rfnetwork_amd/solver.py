@@ -510,9 +510,13 @@ class FDTD_Solver:
 
     # ------------------------------------------------------------------ probes and monitors
     def add_field_monitor(self, name: str, field: str, axis: str, position: float = None, index: int = None,
-                          n_step: int = 1, frequency=None):
+                          n_step: int = 1, frequency=None, dft_dtype=None):
         """Plane monitor of one component (or "e_total") normal to axis, time samples every n_step
-        steps or running DFTs at ``frequency``."""
+        steps or running DFTs at ``frequency``.  dft_dtype=np.complex128 accumulates the running DFT
+        (and evaluates its phase table) in double also in a float32 solve; by default it is complex64
+        in a float32 solve, as in the reference (solver.py:1604-1617)."""
+        if dft_dtype is not None and np.dtype(dft_dtype) not in (np.dtype(np.complex64), np.dtype(np.complex128)):
+            raise ValueError("dft_dtype must be complex64 or complex128")
         supported = tuple(self.fshape.keys()) + ("e_total",)
         if field not in supported:
             raise ValueError(f"Unsupported field: {field}. Expecting one of: {supported}")
@@ -536,7 +540,8 @@ class FDTD_Solver:
             if idx < 0 or idx >= axis_len - 1:
                 raise ValueError("Field position out of bounds")
             self.monitors[nm] = dict(field=f, axis=ai, position=pos, index=idx, n_step=n_step, shape=tuple(shape),
-                                     frequency=np.atleast_1d(frequency) if frequency is not None else None)
+                                     frequency=np.atleast_1d(frequency) if frequency is not None else None,
+                                     dft_dtype=None if dft_dtype is None else np.dtype(dft_dtype))
 
     def add_farfield_monitor(self, frequency, padding: int = 2):
         """Phasor monitors on the faces of a box ``padding`` cells inside the PML, for the far field
@@ -892,9 +897,10 @@ class FDTD_Solver:
             if m["frequency"] is not None:
                 f = m["frequency"]
                 omega = 2 * np.pi * (f / (1 / (self.dt * m["n_step"])))
-                cfg["values"] = np.zeros((len(f),) + shape, dtype=cdt, order="C")
+                mcdt = cdt if m.get("dft_dtype") is None else np.promote_types(cdt, m["dft_dtype"])
+                cfg["values"] = np.zeros((len(f),) + shape, dtype=mcdt, order="C")
                 # DTFT phase per sample and frequency, evaluated in the monitor's complex dtype
-                cfg["dtft_phase"] = np.exp(-1j * omega[None] * np.arange(n_m)[:, None], dtype=cdt, order="C")
+                cfg["dtft_phase"] = np.exp(-1j * omega[None] * np.arange(n_m)[:, None], dtype=mcdt, order="C")
                 cfg["n_frequencies"] = int(len(f))
             else:
                 cfg["values"] = np.zeros((n_m,) + shape, dtype=dtype_, order="C")

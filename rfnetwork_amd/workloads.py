@@ -71,8 +71,9 @@ def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=1
 
 
 def dipole_slab(Nt=8000, nx=400, ny=400, nz=200, cell_m=1e-4, courant=0.99, n_pml=12, n_freq=51, seed=2,
-                monitors=True):
-    """Dipole over a random dielectric slab (the 400x400x200 bandwidth workload)."""
+                monitors=True, dft_dtype=None):
+    """Dipole over a random dielectric slab (the 400x400x200 bandwidth workload).  dft_dtype:
+    accumulator type of the 6 running-DFT planes (default complex64 in a float32 solve)."""
     s, d = _grid_solver(nx, ny, nz, cell_m)
     cx, cy, cz = nx // 2, ny // 2, nz // 2
     rng = np.random.default_rng(seed)
@@ -93,12 +94,12 @@ def dipole_slab(Nt=8000, nx=400, ny=400, nz=200, cell_m=1e-4, courant=0.99, n_pm
     s.frequency = freqs
     if monitors:
         k = n_pml + 2
-        s.add_field_monitor("ez_x0", "ez", "x", index=k, frequency=freqs)
-        s.add_field_monitor("ez_x1", "ez", "x", index=nx - k - 1, frequency=freqs)
-        s.add_field_monitor("ez_y0", "ez", "y", index=k, frequency=freqs)
-        s.add_field_monitor("ez_y1", "ez", "y", index=ny - k - 1, frequency=freqs)
-        s.add_field_monitor("ex_z0", "ex", "z", index=k, frequency=freqs)
-        s.add_field_monitor("ex_z1", "ex", "z", index=nz - k - 1, frequency=freqs)
+        s.add_field_monitor("ez_x0", "ez", "x", index=k, frequency=freqs, dft_dtype=dft_dtype)
+        s.add_field_monitor("ez_x1", "ez", "x", index=nx - k - 1, frequency=freqs, dft_dtype=dft_dtype)
+        s.add_field_monitor("ez_y0", "ez", "y", index=k, frequency=freqs, dft_dtype=dft_dtype)
+        s.add_field_monitor("ez_y1", "ez", "y", index=ny - k - 1, frequency=freqs, dft_dtype=dft_dtype)
+        s.add_field_monitor("ex_z0", "ex", "z", index=k, frequency=freqs, dft_dtype=dft_dtype)
+        s.add_field_monitor("ex_z1", "ex", "z", index=nz - k - 1, frequency=freqs, dft_dtype=dft_dtype)
     return s
 
 

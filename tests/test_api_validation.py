@@ -82,3 +82,28 @@ def test_pec_planes_zeroed_in_caller_arrays(packed):
     for k in ("Cb_ey_z", "Ca_ey_x", "Cb_ez_x", "Ca_ez_y", "Db_hx_y1", "Da_hx_z"):
         assert not np.any(c[k][-1]), k
     assert np.any(c["Ca_ex_y"][-1] != 0)
+
+
+def test_double_accumulated_dft_monitors():
+    """dft_dtype=complex128 in a float32 solve gives complex128 values and phase tables, packed with
+    the double-accumulation flag; mixed dtypes are rejected; complex64 stays the default."""
+    from rfnetwork_amd import _lib
+    m = workloads.tiny_model(Nt=8, time_monitors=False, phasor_monitors=False)
+    m.add_field_monitor("p32", "ez", "x", index=6, frequency=[1e10, 2e10])
+    m.add_field_monitor("p64", "ez", "x", index=6, frequency=[1e10, 2e10], dft_dtype=np.complex128)
+    with pytest.raises(ValueError, match="dft_dtype"):
+        m.add_field_monitor("bad", "ez", "x", index=6, frequency=[1e10], dft_dtype=np.float64)
+    c, probes, monitors, mem, Nx, Ny, Nz, Nt = m.pack(gpu=True)
+    assert monitors[0]["values"].dtype == np.complex64 and monitors[0]["dtft_phase"].dtype == np.complex64
+    assert monitors[1]["values"].dtype == np.complex128 and monitors[1]["dtft_phase"].dtype == np.complex128
+    # the double table is the complex64 one evaluated in double
+    np.testing.assert_allclose(monitors[1]["dtft_phase"], monitors[0]["dtft_phase"], rtol=0, atol=1e-6)
+    recs, _ = hip_func.pack_monitors(monitors, Nx, Ny, Nz, Nt)
+    assert recs[0].flags == 0 and recs[1].flags == _lib.MONITOR_DFT64
+    recs, _ = hip_func.pack_monitors(m.pack(gpu=True, dtype=np.float64, coefficients=False)[2], Nx, Ny, Nz, Nt,
+                                     dtype=np.float64)
+    assert recs[0].flags == 0 and recs[1].flags == 0  # a float64 solve accumulates in double anyway
+    mixed = copy.deepcopy(monitors)
+    mixed[1]["dtft_phase"] = mixed[1]["dtft_phase"].astype(np.complex64)
+    with pytest.raises(ValueError, match="dtft_phase"):
+        hip_func.pack_monitors(mixed, Nx, Ny, Nz, Nt)

@@ -210,3 +210,20 @@ def test_hip_reproduces_golden(hip_ready, dtype, name):
         np.testing.assert_array_equal(m["values"], want[f"monitor_{k}"], err_msg=k)
     for k, v in f_g.items():
         np.testing.assert_array_equal(v, want[f"field_{k}"], err_msg=k)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 4, 6])
+def test_boundary_plane_monitors(hip_ready, variant):
+    """Time-domain monitors of all six components on the first and last planes each axis allows
+    (36 monitors): identical to the oracle whether the sample is read from the stored totals (the
+    separate kernels) or summed from the splits (the fused pass)."""
+    print(f"[case] boundary monitors variant={variant}", flush=True)
+    model = workloads.tiny_model(Nt=30, time_monitors=False, phasor_monitors=False)
+    for f in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        for ai, axis in enumerate("xyz"):
+            n = model.fshape[f][ai]
+            for idx in (0, n - 2):
+                model.add_field_monitor(f"b_{f}_{axis}{idx}", f, axis, index=idx, n_step=1)
+    p_c, m_c, _ = run_oracle(model, np.float32, n_threads=2, fields=False)
+    p_g, m_g, _, _ = _hip_session(model, np.float32, variant=variant, chunks=[17, 13])
+    _assert_monitors_equal(model, p_g, m_g, p_c, m_c)
