@@ -1408,28 +1408,43 @@ struct BoxDev {
     int64_t slot_elems;
 };
 
-// Save the forward H totals of every box (plus one cell on each upper side) into slot `j`.
+// The item (a design box) whose block range [blk0[i], blk0[i+1]) holds block b: every wavefront finds
+// it on its own from 64 offsets per load (one memory round trip for up to 64 boxes).
+__device__ __forceinline__ int box_of(const int64_t* __restrict__ blk0, int n, int64_t b) {
+    const int lane = (int)(threadIdx.x & 63);
+    int count = 0;
+    for (int c = 0; c < n; c += 64) {
+        const int k = c + lane;
+        count += __popcll(__ballot(k < n && blk0[k] <= b));
+    }
+    return __builtin_amdgcn_readfirstlane(count - 1);
+}
+
+// Save the forward H totals of every box (plus one cell on each upper side) into slot `j`.  Box
+// and grid indices fit 32 bits (a session holds at most 2^29 cells).
 template <typename T>
 __global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
                                                 const int64_t* __restrict__ blk0, int n, int j, const int* base = nullptr) {
+    using G = GLOBAL_T(T);
     if (base) j += *base;  // history recording inside a captured chunk: slot = step number
-    const int k = item_of(blk0, n, blockIdx.x);
+    const int k = box_of(blk0, n, blockIdx.x);
     const BoxDev<T> D = boxes[k];
     const InstDev<T>& I = insts[D.inst];
-    const int hy_n = D.B.ny + 1, hz_n = D.B.nz + 1;
-    const int64_t total = (int64_t)(D.B.nx + 1) * hy_n * hz_n;
-    const int64_t q = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    const uint32_t hy_n = D.B.ny + 1, hz_n = D.B.nz + 1;
+    const uint32_t total = (uint32_t)(D.B.nx + 1) * hy_n * hz_n;
+    const uint32_t q = (uint32_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
     if (q >= total) return;
-    const int lz = (int)(q % hz_n), ly = (int)(q / hz_n % hy_n), lx = (int)(q / ((int64_t)hz_n * hy_n));
-    const int x = D.B.x0 + lx, y = D.B.y0 + ly, z = D.B.z0 + lz;
+    const uint32_t lz = q % hz_n, ly = q / hz_n % hy_n, lx = q / (hz_n * hy_n);
+    const int x = D.B.x0 + (int)lx, y = D.B.y0 + (int)ly, z = D.B.z0 + (int)lz;
     T a = T(0), b = T(0), c = T(0);
     if (x < I.Nx && y < I.Ny && z < I.Nz) {
-        const int64_t i = ((int64_t)x * I.Ny + y) * I.Nz + z;
-        a = total_at(I, HX, i);
-        b = total_at(I, HY, i);
-        c = total_at(I, HZ, i);
+        const uint32_t i = ((uint32_t)x * I.Ny + y) * I.Nz + z;
+        const Fields<T>& F = I.F;
+        a = ((const G*)F.hx_y)[i] + ((const G*)F.hx_z)[i];
+        b = ((const G*)F.hy_z)[i] + ((const G*)F.hy_x)[i];
+        c = ((const G*)F.hz_x)[i] + ((const G*)F.hz_y)[i];
     }
-    T* slot = D.slots + (size_t)j * D.slot_elems;
+    G* slot = (G*)D.slots + (size_t)j * D.slot_elems;
     slot[q] = a;
     slot[total + q] = b;
     slot[2 * total + q] = c;
@@ -1440,22 +1455,24 @@ __global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ i
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
                                                   const int64_t* __restrict__ blk0, int n, int j) {
-    const int k = item_of(blk0, n, blockIdx.x);
+    using G = GLOBAL_T(T);
+    using GD = GLOBAL_T(double);
+    const int k = box_of(blk0, n, blockIdx.x);
     const BoxDev<T> D = boxes[k];
     const InstDev<T>& A = insts[D.inst];
     const Box B = D.B;
-    const int hy_n = B.ny + 1, hz_n = B.nz + 1;
-    const int64_t total_h = (int64_t)(B.nx + 1) * hy_n * hz_n;
-    const int64_t nbox = (int64_t)B.nx * B.ny * B.nz;
-    const int64_t q = (int64_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    const uint32_t hy_n = B.ny + 1, hz_n = B.nz + 1;
+    const uint32_t total_h = (uint32_t)(B.nx + 1) * hy_n * hz_n;
+    const uint32_t nbox = (uint32_t)B.nx * B.ny * B.nz;
+    const uint32_t q = (uint32_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
     if (q >= nbox) return;
-    const T* hx = D.slots + (size_t)j * D.slot_elems;
-    const T* hy = hx + total_h;
-    const T* hz = hx + 2 * total_h;
-    const int lz = (int)(q % B.nz), ly = (int)(q / B.nz % B.ny), lx = (int)(q / ((int64_t)B.nz * B.ny));
-    const int x = B.x0 + lx, y = B.y0 + ly, z = B.z0 + lz;
-    const size_t i = ((size_t)x * A.Ny + y) * A.Nz + z;
-    const int64_t h = ((int64_t)lx * hy_n + ly) * hz_n + lz, hyp = h + hz_n, hxp = h + (int64_t)hy_n * hz_n;
+    const G* hx = (const G*)D.slots + (size_t)j * D.slot_elems;
+    const G* hy = hx + total_h;
+    const G* hz = hx + 2 * total_h;
+    const uint32_t lz = q % B.nz, ly = q / B.nz % B.ny, lx = q / ((uint32_t)B.nz * B.ny);
+    const int x = B.x0 + (int)lx, y = B.y0 + (int)ly, z = B.z0 + (int)lz;
+    const uint32_t i = ((uint32_t)x * A.Ny + y) * A.Nz + z;
+    const uint32_t h = (lx * hy_n + ly) * hz_n + lz, hyp = h + hz_n, hxp = h + hy_n * hz_n;
     const bool xin = x < A.Nx - 1, yin = y < A.Ny - 1, zin = z < A.Nz - 1;
     const T d_xy = yin ? hz[hyp] - hz[h] : T(0);
     const T d_xz = zin ? hy[h + 1] - hy[h] : T(0);
@@ -1463,13 +1480,14 @@ __global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__
     const T d_yx = xin ? hz[hxp] - hz[h] : T(0);
     const T d_zx = xin ? hy[hxp] - hy[h] : T(0);
     const T d_zy = yin ? hx[hyp] - hx[h] : T(0);
-    double* g = D.grad;
-    g[0 * nbox + q] += (double)A.F.ex_y[i] * (double)d_xy;
-    g[1 * nbox + q] += (double)A.F.ex_z[i] * (double)d_xz;
-    g[2 * nbox + q] += (double)A.F.ey_z[i] * (double)d_yz;
-    g[3 * nbox + q] += (double)A.F.ey_x[i] * (double)d_yx;
-    g[4 * nbox + q] += (double)A.F.ez_x[i] * (double)d_zx;
-    g[5 * nbox + q] += (double)A.F.ez_y[i] * (double)d_zy;
+    const Fields<T>& F = A.F;
+    GD* g = (GD*)D.grad + q;
+    g[0 * (size_t)nbox] += (double)((const G*)F.ex_y)[i] * (double)d_xy;
+    g[1 * (size_t)nbox] += (double)((const G*)F.ex_z)[i] * (double)d_xz;
+    g[2 * (size_t)nbox] += (double)((const G*)F.ey_z)[i] * (double)d_yz;
+    g[3 * (size_t)nbox] += (double)((const G*)F.ey_x)[i] * (double)d_yx;
+    g[4 * (size_t)nbox] += (double)((const G*)F.ez_x)[i] * (double)d_zx;
+    g[5 * (size_t)nbox] += (double)((const G*)F.ez_y)[i] * (double)d_zy;
 }
 
 // ------------------------------------------------------------------------------------------------
