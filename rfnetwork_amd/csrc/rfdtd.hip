@@ -763,8 +763,9 @@ __device__ __forceinline__ void fused_tile(const InstDev<T>& I, const CA& C, int
 // totals, written to LDS in the previous iteration) and E on plane i (from the H totals just
 // exchanged).  The two LDS exchanges alternate between two buffers by plane parity, so a wave
 // writing plane i + 1's totals never overwrites what a slower wave still reads.  H splits are
-// loaded two planes ahead, E splits and coefficient keys one plane ahead, and the coefficient values
-// of the iteration are issued before its other loads.  Every value is formed as in fused_tile.
+// loaded two planes ahead, E splits and coefficient values one plane ahead and coefficient keys two
+// planes ahead, so that a table entry's gather is in flight for a whole plane.  Every value is
+// formed as in fused_tile.
 template <typename T>
 struct Split6 {
     T xy, xz, yz, yx, zx, zy;
@@ -811,30 +812,33 @@ __device__ __forceinline__ void fused_tile_skew(const InstDev<T>& I, const CA& C
     };
     Split6<T> hA{}, hB = load_h(xs), hC = load_h(xs + 1);  // H splits of planes i - 1, i, i + 1
     Split6<T> e{};                                          // E splits of plane i
-    uint32_t ek = 0, hk = 0;                                // keys: E of plane i, H of plane i - 1
+    ECoef<T> c{};                                           // E coefficients of plane i
+    HCoef<T> ch{};                                          // H coefficients of plane i - 1
+    uint32_t ek = 0, hk = 0;                                // keys: E of plane i + 1, H of plane i
     if (ecomp) {
         e = load_e(xs);
-        ek = C.ekey(off(xs));
+        c = C.e_at(C.ekey(off(xs)));
+        if (xs + 1 < x1) ek = C.ekey(off(xs + 1));
     }
+    if (inner && xs >= x0 && xs < x1) hk = C.hkey(off(xs));
     T ex1 = T(0), ey1 = T(0), ez1 = T(0);  // E totals of plane i - 1 at this cell
     T ey2 = T(0), ez2 = T(0);              // E totals of plane i - 2
     for (int i = xs; i <= x1; ++i) {
         const bool eplane = i < x1;     // E updated on plane i
         const bool hplane = i - 1 >= x0;  // H updated on plane i - 1 (not the extra plane x0 - 1)
         const int par = i & 1;
-        ECoef<T> c{};
-        HCoef<T> ch{};
-        if (eplane && ecomp) c = C.e_at(ek);
-        if (hplane && inner) ch = C.h_at(hk);
-        // the next iteration's loads
+        // the next iteration's loads: coefficient values (E of plane i + 1, H of plane i) from the
+        // keys loaded an iteration earlier, then splits and the keys after them
+        ECoef<T> cN{};
+        HCoef<T> chN{};
+        if (ecomp && i + 1 < x1) cN = C.e_at(ek);
+        if (inner && i >= x0 && i < x1) chN = C.h_at(hk);
         const Split6<T> hN = load_h(i + 2);
         Split6<T> eN{};
         uint32_t ekN = 0, hkN = 0;
-        if (ecomp && i + 1 < x1) {
-            eN = load_e(i + 1);
-            ekN = C.ekey(off(i + 1));
-        }
-        if (inner && i >= x0 && i < x1) hkN = C.hkey(off(i));
+        if (ecomp && i + 1 < x1) eN = load_e(i + 1);
+        if (ecomp && i + 2 < x1) ekN = C.ekey(off(i + 2));
+        if (inner && i + 1 >= x0 && i + 1 < x1) hkN = C.hkey(off(i + 1));
         const T hx_c = hB.xy + hB.xz, hy_c = hB.yz + hB.yx, hz_c = hB.zx + hB.zy;
         if (eplane) {
             sH[(par * 3 + 0) * NT + tid] = hx_c;
@@ -910,6 +914,8 @@ __device__ __forceinline__ void fused_tile_skew(const InstDev<T>& I, const CA& C
         hB = hC;
         hC = hN;
         e = eN;
+        c = cN;
+        ch = chN;
         ek = ekN;
         hk = hkN;
     }
@@ -1496,7 +1502,8 @@ __global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__
 // The E coefficients of a box of packed cells from cell permittivities and conductivities, as the
 // host assembly forms them (rfnetwork/solver.py:892-961 averaging and Ca/Cb, :1463-1523 packing):
 // the edge value of each E component is the half-width-weighted average of its 4 adjacent cells in
-// float64 (first along the lower of the two transverse axes, then the other), rounded to the
+// float64 (the weights' sum in float32, the widths being float32; first along the lower of the two
+// transverse axes, then the other), rounded to the
 // element type; Ca = (2 eps - sig dt) / (2 eps + sig dt), Cb = 2 dt / (2 eps + sig dt) in that
 // type; edges on the grid boundary take eps0 and no loss; the packed Cb carries the inverse dual
 // spacing (zero past the last one) and the curl sign; the x = Nx-1 plane of the Ey and Ez entries
@@ -1521,8 +1528,10 @@ __device__ __forceinline__ void design_tuple(const DesignDev& D, int i, int j, i
     auto cell = [&](const double* v, int a, int b, int c) {
         return v[((int64_t)(li + a) * D.ey1 + (lj + b)) * D.ez1 + (lk + c)];
     };
+    // the widths are float32 values: their halves are exact in double, and the weights' sum is
+    // formed in float32 as the host assembly forms it
     auto avg = [](double v0, double v1, double d0, double d1) {
-        return (v0 * (d0 / 2) + v1 * (d1 / 2)) / (d0 / 2 + d1 / 2);
+        return (v0 * (d0 / 2) + v1 * (d1 / 2)) / (double)((float)(d0 / 2) + (float)(d1 / 2));
     };
     const T dt = (T)D.dt;
     auto coef = [&](bool interior, double eps64, double sig64, T& ca, T& cb) {

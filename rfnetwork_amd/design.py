@@ -19,7 +19,11 @@ from .units import conv, const
 
 
 def _avg(v0, v1, d0, d1):
-    return (v0 * (d0 / 2) + v1 * (d1 / 2)) / (d0 / 2 + d1 / 2)
+    """Half-width-weighted average of two cell values (float64).  The widths are float32 values
+    (held in float64 here): their halves are exact, and the sum of the halves is formed in float32,
+    as rfnetwork/solver.py:892-912 forms it from float32 width arrays."""
+    h0, h1 = np.asarray(d0 / 2), np.asarray(d1 / 2)
+    return (v0 * h0 + v1 * h1) / (h0.astype(np.float32) + h1.astype(np.float32))
 
 
 def edge_values(model, gbox, eps_cells, sig_cells, dtype):

@@ -55,3 +55,31 @@ def test_edge_values_at_the_grid_boundary():
         assert plain.sum() > 50
         assert plain[-1].any() and plain[:, -1].any() and plain[..., -1].any()  # the upper faces are covered
         np.testing.assert_array_equal(coef_g[key][plain], ref[plain], err_msg=key)
+
+
+def test_edge_values_on_a_graded_grid():
+    """Cell widths that differ from cell to cell: the averaging weights' sum is formed in float32
+    as the host assembly forms it (the widths are float32), so the packed coefficients still agree
+    bit for bit."""
+    from rfnetwork_amd.solver import FDTD_Solver
+    from rfnetwork_amd.geometry import Box
+
+    rng = np.random.default_rng(11)
+    d = 1e-4 * 39.3701
+    edges = [np.concatenate([[0.0], np.cumsum(d * rng.uniform(0.6, 1.7, size=n))]) for n in (12, 10, 9)]
+    s = FDTD_Solver(Box((0, 0, 0), tuple(float(e[-1]) for e in edges)))
+    s.set_grid(*edges)
+    s.add_permittivity_volume(rng.uniform(1.0, 5.0, size=(12, 10, 9)), (0, 0, 0),
+                              sigma=rng.uniform(0.0, 0.3, size=(12, 10, 9)))
+    s.generate_mesh(courant=0.9)
+    for dtype in (np.float32, np.float64):
+        s.coef_dtype = dtype
+        s.generate_mesh(courant=0.9)
+        gbox = (0, s.Nx, 0, s.Ny, 0, s.Nz)
+        _, coef_g = edge_values(s, gbox, s.eps, s.sigma, dtype)
+        coeff = s.coefficient_arrays(gpu=True, dtype=dtype)
+        for n, key in enumerate(coef_g):
+            ref = coeff[key].copy()
+            if n >= 4:
+                ref[-1] = 0
+            np.testing.assert_array_equal(coef_g[key], ref, err_msg=f"{key} {np.dtype(dtype)}")

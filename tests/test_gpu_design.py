@@ -146,3 +146,43 @@ def test_design_loop_equals_rebuilt_models(hip_ready, mode):
             np.testing.assert_array_equal(g, gr)
             np.testing.assert_array_equal(S, Sr)
             assert loop.timing["update_ms"] > 0
+
+
+def _graded_model(eps_box, Nt):
+    """A graded grid (every cell width different), lossy random dielectric, PML on two sides and a
+    port outside a 6x5x4 design box at cells [8, 14) x [6, 11) x [5, 9)."""
+    from rfnetwork_amd.geometry import Rectangle
+    from rfnetwork_amd.solver import FDTD_Solver
+    from rfnetwork_amd.geometry import Box
+
+    rng = np.random.default_rng(21)
+    d = 1e-4 * 39.3701
+    widths = [d * rng.uniform(0.7, 1.5, size=n) for n in (24, 18, 14)]
+    edges = [np.concatenate([[0.0], np.cumsum(w)]) for w in widths]
+    s = FDTD_Solver(Box((0, 0, 0), tuple(float(e[-1]) for e in edges)))
+    s.set_grid(*edges)
+    vol = rng.uniform(1.0, 4.0, size=(24, 18, 14))
+    vol[8:14, 6:11, 5:9] = eps_box
+    s.add_permittivity_volume(vol, (0, 0, 0), sigma=rng.uniform(0.0, 0.05, size=(24, 18, 14)))
+    ex, ey, ez = edges
+    s.add_lumped_port(1, Rectangle([(ex[4], ey[4], ez[2]), (ex[4], ey[7], ez[2]), (ex[4], ey[7], ez[5])]), "z+")
+    s.assign_PML_boundaries("x-", "x+", n_pml=3)
+    s.generate_mesh(courant=0.9)
+    src = s.gaussian_modulated_source(f0=20e9, width=60e-12, t0=30e-12, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(1e-2 * np.asarray(src), 1)
+    return s
+
+
+def test_design_update_on_a_graded_grid(hip_ready):
+    """Cell widths that differ from cell to cell: the device averaging forms the weights' sum in
+    float32 as the host assembly does, so the updated session runs as the rebuilt model's."""
+    print("[case] design graded grid", flush=True)
+    Nt = 80
+    rng = np.random.default_rng(5)
+    e0, e1 = rng.uniform(1.0, 4.0, size=(2, 6, 5, 4))
+    m0, m1 = _graded_model(e0, Nt), _graded_model(e1, Nt)
+    coeff, _, _, _, Nx, Ny, Nz, _ = m0.pack(gpu=True)
+    with Session(coeff, Nx, Ny, Nz) as s:
+        s.set_design_permittivity(m0, (8, 14, 6, 11, 5, 9), e1)
+        got = _run(s, m0, Nt)
+    _assert_same(got, _reference(m1, np.float32, 0, Nt))
