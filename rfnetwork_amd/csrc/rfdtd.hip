@@ -33,6 +33,7 @@
 //   * a chunk of steps is captured once into a hipGraph and replayed; the kernels read the step
 //     number from a device word set before each replay.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -2063,12 +2064,19 @@ Session::~Session() {
 
 // ---- bit-exact deduplication of coefficient tuples ----
 
-inline uint64_t mix64(uint64_t h) {
+__host__ __device__ inline uint64_t mix64(uint64_t h) {
     h ^= h >> 33;
     h *= 0xff51afd7ed558ccdULL;
     h ^= h >> 33;
     h *= 0xc4ceb9fe1a85ec53ULL;
     h ^= h >> 33;
+    return h;
+}
+
+template <typename U, int K>
+__host__ __device__ inline uint64_t tuple_hash(const U* k) {
+    uint64_t h = 0x9e3779b97f4a7c15ULL;
+    for (int j = 0; j < K; ++j) h = mix64(h ^ (uint64_t)k[j]) + (uint64_t)j;
     return h;
 }
 
@@ -2078,11 +2086,7 @@ struct TupleSet {
     std::vector<U> table;  // K values per entry
     std::vector<int32_t> slots = std::vector<int32_t>(1 << 12, -1);
     int64_t count = 0;
-    static uint64_t hash(const U* k) {
-        uint64_t h = 0x9e3779b97f4a7c15ULL;
-        for (int j = 0; j < K; ++j) h = mix64(h ^ (uint64_t)k[j]) + (uint64_t)j;
-        return h;
-    }
+    static uint64_t hash(const U* k) { return tuple_hash<U, K>(k); }
     // entry of key, inserted if new; -1 once more than max_entries would exist
     int32_t find_or_insert(const U* key, int64_t max_entries) {
         size_t cap = slots.size();
@@ -2176,6 +2180,161 @@ bool dedup(const U* const* arrays, const bool* zero_last_x, int64_t cells, int64
 // which of the 30 arrays read as zero on the x = Nx-1 plane (solver.cpp:294-328: Cy, Cz, Dx)
 bool zero_last_plane(int k) { return (k >= 4 && k <= 11) || (k >= 12 && k <= 17); }
 
+// ---- the same deduplication on the device ----
+//
+// Every cell's tuple is hashed; the (hash, cell) pairs are radix-sorted, so equal tuples sit together
+// with their cells ascending; a run of equal tuples (compared bit for bit, so a hash collision only
+// splits a run) is one group whose first cell is its first occurrence.  Sorting the groups by first
+// cell and numbering them in that order gives each tuple the number the host pass gives it (first
+// occurrence in cell order), which keeps neighbouring cells' entries close together in the table.
+
+template <typename U, int K>
+struct TuplePtrs {
+    const U* p[K];
+};
+
+template <typename U, int K>
+__device__ __forceinline__ void tuple_at(const TuplePtrs<U, K>& a, uint32_t c, U t[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] = a.p[k][c];
+}
+
+template <typename U, int K>
+__global__ void __launch_bounds__(256) k_tuple_hash(TuplePtrs<U, K> a, uint32_t n, uint64_t* keys, uint32_t* cells) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    U t[K];
+    tuple_at<U, K>(a, i, t);
+    keys[i] = tuple_hash<U, K>(t);
+    cells[i] = i;
+}
+
+// head[i] = 1 where sorted position i starts a run of equal tuples
+template <typename U, int K>
+__global__ void __launch_bounds__(256) k_tuple_heads(TuplePtrs<U, K> a, uint32_t n, const uint64_t* keys,
+                                                     const uint32_t* cells, uint32_t* head) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = 1;
+    if (i > 0 && keys[i] == keys[i - 1]) {
+        U t0[K], t1[K];
+        tuple_at<U, K>(a, cells[i], t0);
+        tuple_at<U, K>(a, cells[i - 1], t1);
+        bool eq = true;
+#pragma unroll
+        for (int k = 0; k < K; ++k) eq &= t0[k] == t1[k];
+        h = eq ? 0 : 1;
+    }
+    head[i] = h;
+}
+
+// first[g] = first cell of group g (grp: inclusive scan of the heads, 1-based)
+__global__ void __launch_bounds__(256) k_group_first(uint32_t n, const uint32_t* head, const uint32_t* grp,
+                                                     const uint32_t* cells, uint32_t* first, uint32_t* gid) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !head[i]) return;
+    first[grp[i] - 1] = cells[i];
+    gid[grp[i] - 1] = grp[i] - 1;
+}
+
+// groups sorted by first cell: rank r is the entry number of group gid_sorted[r]; its tuple is the
+// table entry (stride values per entry, the rest zero)
+template <typename U, int K>
+__global__ void __launch_bounds__(256) k_group_entries(TuplePtrs<U, K> a, uint32_t ng, const uint32_t* first_sorted,
+                                                       const uint32_t* gid_sorted, uint32_t* entry_of, U* table,
+                                                       int stride) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ng) return;
+    entry_of[gid_sorted[r]] = r;
+    U t[K];
+    tuple_at<U, K>(a, first_sorted[r], t);
+    for (int k = 0; k < stride; ++k) table[(size_t)r * stride + k] = k < K ? t[k] : U(0);
+}
+
+__global__ void __launch_bounds__(256) k_cell_entries(uint32_t n, const uint32_t* grp, const uint32_t* cells,
+                                                      const uint32_t* entry_of, uint32_t* idx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    idx[cells[i]] = entry_of[grp[i] - 1];
+}
+
+// Scratch of one device deduplication, freed on every exit path.
+struct DedupScratch {
+    std::vector<void*> ptrs;
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        HIP_OK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+        ptrs.push_back(p);
+        return p;
+    }
+    ~DedupScratch() {
+        (void)hipDeviceSynchronize();
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+// Distinct K-tuples of the device arrays a over n cells, numbered by first occurrence.  Returns the
+// number of entries; the per-cell entry numbers and the table (stride values per entry) are left in
+// *idx and *table (device allocations owned by the caller) unless there are more than max_entries,
+// in which case nothing is allocated.
+template <typename U, int K>
+int64_t device_dedup(const TuplePtrs<U, K>& a, uint32_t n, int64_t max_entries, int stride, uint32_t** idx,
+                     U** table) {
+    DedupScratch S;
+    const int blocks = (int)((n + 255) / 256);
+    uint64_t* keys = (uint64_t*)S.alloc((size_t)n * 8);
+    uint64_t* keys_s = (uint64_t*)S.alloc((size_t)n * 8);
+    uint32_t* cells = (uint32_t*)S.alloc((size_t)n * 4);
+    uint32_t* cells_s = (uint32_t*)S.alloc((size_t)n * 4);
+    uint32_t* head = (uint32_t*)S.alloc((size_t)n * 4);
+    uint32_t* grp = (uint32_t*)S.alloc((size_t)n * 4);
+    k_tuple_hash<U, K><<<blocks, 256>>>(a, n, keys, cells);
+    HIP_OK(hipGetLastError());
+    size_t tb = 0;
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, keys_s, cells, cells_s, (int)n));
+    void* tmp = S.alloc(tb);
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys_s, cells, cells_s, (int)n));
+    k_tuple_heads<U, K><<<blocks, 256>>>(a, n, keys_s, cells_s, head);
+    HIP_OK(hipGetLastError());
+    size_t tb2 = 0;
+    HIP_OK(hipcub::DeviceScan::InclusiveSum(nullptr, tb2, head, grp, (int)n));
+    void* tmp2 = S.alloc(tb2);
+    HIP_OK(hipcub::DeviceScan::InclusiveSum(tmp2, tb2, head, grp, (int)n));
+    uint32_t ng = 0;
+    HIP_OK(hipMemcpy(&ng, grp + (n - 1), 4, hipMemcpyDeviceToHost));
+    if ((int64_t)ng > max_entries) return ng;
+    // reuse: first / gid per group, then their sorted copies
+    uint32_t* first = (uint32_t*)keys;
+    uint32_t* gid = first + ng;
+    uint32_t* first_s = (uint32_t*)keys_s;
+    uint32_t* gid_s = first_s + ng;
+    k_group_first<<<blocks, 256>>>(n, head, grp, cells_s, first, gid);
+    HIP_OK(hipGetLastError());
+    size_t tb3 = 0;
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, first, first_s, gid, gid_s, (int)ng));
+    void* tmp3 = tb3 <= tb ? tmp : S.alloc(tb3);
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(tmp3, tb3, first, first_s, gid, gid_s, (int)ng));
+    uint32_t* entry_of = cells;  // ng <= n
+    U* tab = nullptr;
+    uint32_t* ix = nullptr;
+    HIP_OK(hipMalloc(&tab, std::max<size_t>((size_t)ng * stride * sizeof(U), 256)));
+    if (hipMalloc(&ix, (size_t)n * 4) != hipSuccess) {
+        (void)hipFree(tab);
+        RF_THROW(RFDTD_ENOMEM, "no device memory for the coefficient table");
+    }
+    k_group_entries<U, K><<<(int)((ng + 255) / 256), 256>>>(a, ng, first_s, gid_s, entry_of, tab, stride);
+    k_cell_entries<<<blocks, 256>>>(n, grp, cells_s, entry_of, ix);
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipFree(tab);
+        (void)hipFree(ix);
+        HIP_OK(e);
+    }
+    *idx = ix;
+    *table = tab;
+    return ng;
+}
+
 template <typename T>
 void upload_coefficients(Member& M, uint32_t flags, const void* const coeff[30]) {
     const int64_t cells = M.cells, plane = (int64_t)M.Ny * M.Nz;
@@ -2187,22 +2346,13 @@ void upload_coefficients(Member& M, uint32_t flags, const void* const coeff[30])
         arr[k] = (const U*)coeff[k];
         zl[k] = zero_last_plane(k);
     }
-    bool use_table = false;
-    std::vector<uint32_t> eidx, hidx;
-    std::vector<U> etab, htab;
-    if (!(flags & RFDTD_COEF_RAW)) {
-        // A table is worth it while its entries cost well under the raw arrays.
-        const int64_t limit = (flags & RFDTD_COEF_TABLE) ? cells : std::max<int64_t>(4096, cells / 8);
-        use_table = dedup<U, 12>(arr, zl, cells, plane, limit, eidx, etab) &&
-                    dedup<U, 18>(arr + 12, zl + 12, cells, plane, limit, hidx, htab);
-        if (!use_table && (flags & RFDTD_COEF_TABLE))
-            RF_THROW(RFDTD_EINVAL, "coefficient table exceeds its entry limit");
-    }
-    if (use_table) {
+    // A table is worth it while its entries cost well under the raw arrays.
+    const int64_t limit = (flags & RFDTD_COEF_TABLE) ? cells : std::max<int64_t>(4096, cells / 8);
+    const size_t idx_bytes = (size_t)cells * 4;
+    auto table_layout = [&](int64_t nE, int64_t nH) {
         M.coef_kind = 1;
-        M.nE = (int64_t)etab.size() / 12;
-        M.nH = (int64_t)htab.size() / 18;
-        const size_t idx_bytes = (size_t)cells * 4;
+        M.nE = nE;
+        M.nH = nH;
         const size_t etab_bytes = (size_t)M.nE * 12 * sizeof(T);
         const size_t htab_bytes = (size_t)M.nH * 20 * sizeof(T);
         M.coef_bytes = 2 * align256(idx_bytes) + align256(etab_bytes) + align256(htab_bytes);
@@ -2212,26 +2362,80 @@ void upload_coefficients(Member& M, uint32_t flags, const void* const coeff[30])
         M.hidx = (const uint32_t*)(b + align256(idx_bytes));
         M.etab = b + 2 * align256(idx_bytes);
         M.htab = b + 2 * align256(idx_bytes) + align256(etab_bytes);
-        HIP_OK(hipMemcpy((void*)M.eidx, eidx.data(), idx_bytes, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy((void*)M.hidx, hidx.data(), idx_bytes, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy((void*)M.etab, etab.data(), etab_bytes, hipMemcpyHostToDevice));
-        std::vector<U> hpad((size_t)M.nH * 20, U(0));
-        for (int64_t e = 0; e < M.nH; ++e)
-            std::memcpy(&hpad[(size_t)e * 20], &htab[(size_t)e * 18], 18 * sizeof(U));
-        HIP_OK(hipMemcpy((void*)M.htab, hpad.data(), htab_bytes, hipMemcpyHostToDevice));
-    } else {
-        M.coef_kind = 0;
-        const size_t arr_bytes = align256((size_t)cells * sizeof(T));
-        M.coef_bytes = 30 * arr_bytes;
-        HIP_OK(hipMalloc(&M.d_coef, M.coef_bytes));
-        for (int k = 0; k < 30; ++k) {
-            char* dst = (char*)M.d_coef + (size_t)k * arr_bytes;
-            M.raw[k] = dst;
-            HIP_OK(hipMemcpy(dst, coeff[k], (size_t)cells * sizeof(T), hipMemcpyHostToDevice));
-            if (zl[k])
-                HIP_OK(hipMemset(dst + (size_t)(cells - plane) * sizeof(T), 0, (size_t)plane * sizeof(T)));
+        return std::make_pair(etab_bytes, htab_bytes);
+    };
+    if ((flags & RFDTD_HOST_TABLE) && !(flags & RFDTD_COEF_RAW)) {
+        std::vector<uint32_t> eidx, hidx;
+        std::vector<U> etab, htab;
+        const bool use_table = dedup<U, 12>(arr, zl, cells, plane, limit, eidx, etab) &&
+                               dedup<U, 18>(arr + 12, zl + 12, cells, plane, limit, hidx, htab);
+        if (!use_table && (flags & RFDTD_COEF_TABLE))
+            RF_THROW(RFDTD_EINVAL, "coefficient table exceeds its entry limit");
+        if (use_table) {
+            const auto tb = table_layout((int64_t)etab.size() / 12, (int64_t)htab.size() / 18);
+            HIP_OK(hipMemcpy((void*)M.eidx, eidx.data(), idx_bytes, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy((void*)M.hidx, hidx.data(), idx_bytes, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy((void*)M.etab, etab.data(), tb.first, hipMemcpyHostToDevice));
+            std::vector<U> hpad((size_t)M.nH * 20, U(0));
+            for (int64_t e = 0; e < M.nH; ++e)
+                std::memcpy(&hpad[(size_t)e * 20], &htab[(size_t)e * 18], 18 * sizeof(U));
+            HIP_OK(hipMemcpy((void*)M.htab, hpad.data(), tb.second, hipMemcpyHostToDevice));
+            return;
         }
     }
+    // the raw arrays on the device (the last x plane zeroed where the rule applies)
+    M.coef_kind = 0;
+    const size_t arr_bytes = align256((size_t)cells * sizeof(T));
+    M.coef_bytes = 30 * arr_bytes;
+    HIP_OK(hipMalloc(&M.d_coef, M.coef_bytes));
+    for (int k = 0; k < 30; ++k) {
+        char* dst = (char*)M.d_coef + (size_t)k * arr_bytes;
+        M.raw[k] = dst;
+        HIP_OK(hipMemcpy(dst, coeff[k], (size_t)cells * sizeof(T), hipMemcpyHostToDevice));
+        if (zl[k]) HIP_OK(hipMemset(dst + (size_t)(cells - plane) * sizeof(T), 0, (size_t)plane * sizeof(T)));
+    }
+    if (flags & (RFDTD_COEF_RAW | RFDTD_HOST_TABLE)) return;
+    // the table, built on the device from the raw arrays (numbered as the host pass numbers it)
+    TuplePtrs<U, 12> ea;
+    TuplePtrs<U, 18> ha;
+    for (int k = 0; k < 12; ++k) ea.p[k] = (const U*)M.raw[k];
+    for (int k = 0; k < 18; ++k) ha.p[k] = (const U*)M.raw[12 + k];
+    uint32_t *eix = nullptr, *hix = nullptr;
+    U *etb = nullptr, *htb = nullptr;
+    auto release = [&] {
+        for (void* q : {(void*)eix, (void*)hix, (void*)etb, (void*)htb})
+            if (q) (void)hipFree(q);
+    };
+    const int64_t nE = device_dedup<U, 12>(ea, (uint32_t)cells, limit, 12, &eix, &etb);
+    int64_t nH = 0;
+    if (nE <= limit) nH = device_dedup<U, 18>(ha, (uint32_t)cells, limit, 20, &hix, &htb);
+    if (nE > limit || nH > limit) {
+        release();
+        if (flags & RFDTD_COEF_TABLE) RF_THROW(RFDTD_EINVAL, "coefficient table exceeds its entry limit");
+        return;  // the raw arrays stay
+    }
+    void* raw_buf = M.d_coef;
+    const size_t raw_bytes = M.coef_bytes;
+    try {
+        M.d_coef = nullptr;
+        const auto tb = table_layout(nE, nH);
+        HIP_OK(hipMemcpy((void*)M.eidx, eix, idx_bytes, hipMemcpyDeviceToDevice));
+        HIP_OK(hipMemcpy((void*)M.hidx, hix, idx_bytes, hipMemcpyDeviceToDevice));
+        HIP_OK(hipMemcpy((void*)M.etab, etb, tb.first, hipMemcpyDeviceToDevice));
+        HIP_OK(hipMemcpy((void*)M.htab, htb, tb.second, hipMemcpyDeviceToDevice));
+    } catch (...) {  // back to the raw form, which the member's destructor frees
+        release();
+        if (M.d_coef) (void)hipFree(M.d_coef);
+        M.d_coef = raw_buf;
+        M.coef_bytes = raw_bytes;
+        M.coef_kind = 0;
+        M.eidx = M.hidx = nullptr;
+        M.etab = M.htab = nullptr;
+        throw;
+    }
+    release();
+    HIP_OK(hipFree(raw_buf));
+    for (int k = 0; k < 30; ++k) M.raw[k] = nullptr;
 }
 
 // x planes per workgroup: enough workgroups to give every CU several waves (>= ~2048 groups in a
