@@ -267,6 +267,64 @@ typedef struct rfdtd_ff_face {
 int rfdtd_nf2ff(int32_t n_faces, const rfdtd_ff_face* faces, int32_t n_frequencies, const float* beta,
                 int32_t n_theta, const float* theta, int32_t n_phi, const float* phi, float eta0, void* out);
 
+/*
+ * On-device coefficient assembly of a whole grid (the counterpart of the host assembly
+ * FDTD_Solver._init_coefficients / _init_PML / _init_conductors / _init_lumped_elements /
+ * edge_correction and the packing, rfnetwork/solver.py:817-1325, :2534-2700, :1463-1523): a session
+ * whose 30 coefficient arrays are computed on the GPU from cell permittivity and conductivity, the
+ * PML profiles and the overrides below, equal bit for bit to the host assembly's, then kept in the
+ * deduplicated table form (or raw, as rfdtd_create decides).  The element type is the session's
+ * (RFDTD_F64 in flags).  Sizes: Nx, Ny, Nz cells.
+ */
+/* One PML side: n layers along axis (0, 1, 2) at its lower (upper = 0) or upper end;
+ * sigma_edge[i], sigma_half[i], i = 0 .. n-1: the graded conductivity at the E edges and at the H
+ * half cells, as solver.py:1255-1274 forms them (float64). */
+typedef struct rfdtd_pml_side {
+    int32_t axis, upper, n, reserved;
+    const double* sigma_edge;
+    const double* sigma_half;
+} rfdtd_pml_side;
+/* Ca and Cb of both splits of E edges replaced after the PML, in list order.  component 0..2
+ * (Ex, Ey, Ez); cells: packed cell indices of those edges (the GPU packing's cell holding the
+ * component).  kind 0, a conductor (solver.py:863-868): s = values[0] = sigma * dt as the host forms
+ * it, Ca = (2 eps - s) / (2 eps + s), Cb = 2 dt / (2 eps + s).  kind 1, a lumped element
+ * (solver.py:1190-1200): q = values[j] per edge, Ca = (eps / dt - q) / (eps / dt + q),
+ * Cb = 1 / (eps / dt + q).  eps is the edge's permittivity in the element type; wide = 1 when s or
+ * q is a float64 value, so that the arithmetic after eps / dt is float64 and rounded at the end,
+ * as numpy's type promotion makes it. */
+typedef struct rfdtd_edge_fix {
+    int32_t component, kind, wide, reserved;
+    int64_t count;
+    const int64_t* cells;
+    const double* values;
+} rfdtd_edge_fix;
+/* One factor of an edge correction (solver.py:2614-2655): the unpacked Db of coefficient `coef`
+ * (14..17, 20..23, 26..29 in the coefficient order above) at packed cell `cell` is multiplied by
+ * factor, in float64 (wide = 1) or in the element type, in list order. */
+typedef struct rfdtd_h_scale {
+    int32_t coef, wide;
+    int64_t cell;
+    double factor;
+} rfdtd_h_scale;
+typedef struct rfdtd_assembly {
+    int32_t Nx, Ny, Nz, n_pml;
+    const double* eps;              /* [Nx][Ny][Nz] cell permittivity, F/m */
+    const double* sigma;            /* [Nx][Ny][Nz] cell conductivity, S/m */
+    const double* width[3];         /* cell widths along x, y, z, metres (float32 values) */
+    const void* inv_dual[3];        /* the packing's inverse dual spacings (element type, last 0) */
+    const void* inv_width[3];       /* 1 / width in the element type */
+    double dt;                      /* time step (a float32 value) */
+    double e0, u0;                  /* vacuum permittivity and permeability */
+    const rfdtd_pml_side* pml;      /* n_pml sides, in the order the host applies them */
+    int32_t n_fix, n_scale;
+    const rfdtd_edge_fix* fix;      /* n_fix groups */
+    const rfdtd_h_scale* scale;     /* n_scale factors */
+} rfdtd_assembly;
+int rfdtd_create_assembled(rfdtd_session** out, const rfdtd_assembly* a, int32_t excitations, uint32_t flags);
+/* The member's 30 packed coefficient arrays (the x = Nx-1 plane of arrays 4..17 reads zero), from
+ * its table or raw arrays, into host arrays of the element type (Nx*Ny*Nz each; null skips one). */
+int rfdtd_read_coefficients(rfdtd_session* s, int32_t member, void* const out[30]);
+
 /* Bandwidth probe (no counterpart in the reference): copy `bytes` between two device buffers `iters`
  * times with a plain streaming kernel and report read+write GB/s and ms per copy -- the measured HBM
  * ceiling, and the known-size kernel that calibrates the profiler's byte counters. */

@@ -55,6 +55,7 @@ EXPORTS = (
     "rfdtd_plan_checkpoints", "rfdtd_nf2ff", "rfdtd_set_history", "rfdtd_device_memory",
     "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
     "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
+    "rfdtd_create_assembled", "rfdtd_read_coefficients",
 )
 
 
@@ -79,6 +80,31 @@ class Monitor(ctypes.Structure):
         ("values", ctypes.c_void_p),
         ("dtft_phase", ctypes.c_void_p),
     ]
+
+
+class PmlSide(ctypes.Structure):  # rfdtd_pml_side
+    _fields_ = [("axis", ctypes.c_int32), ("upper", ctypes.c_int32), ("n", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("sigma_edge", ctypes.c_void_p), ("sigma_half", ctypes.c_void_p)]
+
+
+class EdgeFix(ctypes.Structure):  # rfdtd_edge_fix
+    _fields_ = [("component", ctypes.c_int32), ("kind", ctypes.c_int32), ("wide", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("count", ctypes.c_int64), ("cells", ctypes.c_void_p),
+                ("values", ctypes.c_void_p)]
+
+
+class HScale(ctypes.Structure):  # rfdtd_h_scale
+    _fields_ = [("coef", ctypes.c_int32), ("wide", ctypes.c_int32), ("cell", ctypes.c_int64),
+                ("factor", ctypes.c_double)]
+
+
+class Assembly(ctypes.Structure):  # rfdtd_assembly
+    _fields_ = [("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32), ("Nz", ctypes.c_int32), ("n_pml", ctypes.c_int32),
+                ("eps", ctypes.c_void_p), ("sigma", ctypes.c_void_p), ("width", ctypes.c_void_p * 3),
+                ("inv_dual", ctypes.c_void_p * 3), ("inv_width", ctypes.c_void_p * 3), ("dt", ctypes.c_double),
+                ("e0", ctypes.c_double), ("u0", ctypes.c_double), ("pml", ctypes.c_void_p),
+                ("n_fix", ctypes.c_int32), ("n_scale", ctypes.c_int32), ("fix", ctypes.c_void_p),
+                ("scale", ctypes.c_void_p)]
 
 
 class RfdtdError(RuntimeError):
@@ -149,6 +175,8 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(ctypes.c_uint64)]
     L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]
+    L.rfdtd_create_assembled.argtypes = [ctypes.POINTER(vp), vp, i32, u32]
+    L.rfdtd_read_coefficients.argtypes = [vp, i32, vp]
     _lib = L
     return L
 

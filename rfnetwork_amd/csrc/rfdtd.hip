@@ -1623,6 +1623,257 @@ __global__ void __launch_bounds__(256) k_design_coefficients(DesignDev D, int co
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// whole-grid coefficient assembly
+//
+// The host assembly (rfnetwork/solver.py:871-1003 averaging and Ca/Cb, :1239-1325 PML, :817-868
+// conductors, :1086-1236 lumped elements, :2534-2700 edge corrections, :1463-1523 packing) restated
+// per packed cell with the host's floating-point types: cell values and averages in float64, the
+// edge permittivity rounded to the element type T, Ca/Cb of plain edges in T, the PML formulas in
+// float64 (the float64 profile times eps / e0 formed in T) rounded to T at the end, H defaults
+// Da = 1 and Db = dt / u0 formed in float32.  The passes: dense (unpacked values of every cell, PML
+// included), the E overrides group by group, the edge-correction factors one by one in list order,
+// then the packing (Cb and Db times the signed inverse spacing; the x = Nx-1 plane of arrays 4..17
+// zeroed).
+// ------------------------------------------------------------------------------------------------
+
+struct AsmPml {
+    int32_t axis, upper, n, pad;
+    const double* edge;
+    const double* half;
+};
+
+constexpr int kMaxPmlSides = 6;
+
+template <typename T>
+struct AsmDev {
+    int32_t Nx, Ny, Nz, npml;
+    const double* eps;
+    const double* sig;
+    const double* w[3];
+    const T* idual[3];
+    const T* iw[3];
+    double dt, e0, u0;
+    AsmPml pml[kMaxPmlSides];
+};
+
+__device__ __forceinline__ double asm_avg(double v0, double v1, double d0, double d1) {
+    return (v0 * (d0 / 2) + v1 * (d1 / 2)) / (double)((float)(d0 / 2) + (float)(d1 / 2));
+}
+
+// permittivity (and conductivity) of E component c at packed cell (i, j, k), in T; interior edges
+// average their 4 cells (Ex: y then z; Ey: x then z; Ez: x then y), boundary edges take e0 and 0
+template <typename T>
+__device__ __forceinline__ void asm_edge(const AsmDev<T>& A, int c, int i, int j, int k, T& eps, T& sig) {
+    const int Ny = A.Ny, Nz = A.Nz;
+    const bool xi = i <= A.Nx - 2, yi = j <= Ny - 2, zi = k <= Nz - 2;
+    auto cell = [&](const double* v, int a, int b, int d) { return v[((int64_t)(i + a) * Ny + (j + b)) * Nz + (k + d)]; };
+    bool interior;
+    double ve = 0, vs = 0;
+    if (c == 0) {
+        interior = yi && zi;
+        if (interior) {
+            const double dy0 = A.w[1][j], dy1 = A.w[1][j + 1], dz0 = A.w[2][k], dz1 = A.w[2][k + 1];
+            for (int h = 0; h < 2; ++h) {
+                const double* v = h ? A.sig : A.eps;
+                const double u0 = asm_avg(cell(v, 0, 0, 0), cell(v, 0, 1, 0), dy0, dy1);
+                const double u1 = asm_avg(cell(v, 0, 0, 1), cell(v, 0, 1, 1), dy0, dy1);
+                (h ? vs : ve) = asm_avg(u0, u1, dz0, dz1);
+            }
+        }
+    } else if (c == 1) {
+        interior = xi && zi;
+        if (interior) {
+            const double dx0 = A.w[0][i], dx1 = A.w[0][i + 1], dz0 = A.w[2][k], dz1 = A.w[2][k + 1];
+            for (int h = 0; h < 2; ++h) {
+                const double* v = h ? A.sig : A.eps;
+                const double u0 = asm_avg(cell(v, 0, 0, 0), cell(v, 1, 0, 0), dx0, dx1);
+                const double u1 = asm_avg(cell(v, 0, 0, 1), cell(v, 1, 0, 1), dx0, dx1);
+                (h ? vs : ve) = asm_avg(u0, u1, dz0, dz1);
+            }
+        }
+    } else {
+        interior = xi && yi;
+        if (interior) {
+            const double dx0 = A.w[0][i], dx1 = A.w[0][i + 1], dy0 = A.w[1][j], dy1 = A.w[1][j + 1];
+            for (int h = 0; h < 2; ++h) {
+                const double* v = h ? A.sig : A.eps;
+                const double u0 = asm_avg(cell(v, 0, 0, 0), cell(v, 1, 0, 0), dx0, dx1);
+                const double u1 = asm_avg(cell(v, 0, 1, 0), cell(v, 1, 1, 0), dx0, dx1);
+                (h ? vs : ve) = asm_avg(u0, u1, dy0, dy1);
+            }
+        }
+    }
+    eps = interior ? (T)ve : (T)A.e0;
+    sig = interior ? (T)vs : T(0);
+}
+
+// permittivity of H component c (0 hx, 1 hy, 2 hz) at packed cell (i, j, k): the average of the two
+// cells either side along its own axis, e0 on the boundary
+template <typename T>
+__device__ __forceinline__ T asm_heps(const AsmDev<T>& A, int c, int i, int j, int k) {
+    const int Ny = A.Ny, Nz = A.Nz;
+    const int64_t q = ((int64_t)i * Ny + j) * Nz + k;
+    if (c == 0 && i <= A.Nx - 2) return (T)asm_avg(A.eps[q], A.eps[q + (int64_t)Ny * Nz], A.w[0][i], A.w[0][i + 1]);
+    if (c == 1 && j <= Ny - 2) return (T)asm_avg(A.eps[q], A.eps[q + Nz], A.w[1][j], A.w[1][j + 1]);
+    if (c == 2 && k <= Nz - 2) return (T)asm_avg(A.eps[q], A.eps[q + 1], A.w[2][k], A.w[2][k + 1]);
+    return (T)A.e0;
+}
+
+// the PML layer (profile index) of global index g along axis a (shape L), E edges (e = true) or H
+// half cells; the last side that covers it wins, as the host applies the sides in order
+template <typename T>
+__device__ __forceinline__ const double* asm_pml(const AsmDev<T>& A, int a, int g, int L, bool e, int& layer) {
+    const double* prof = nullptr;
+    for (int s = 0; s < A.npml; ++s) {
+        const AsmPml& P = A.pml[s];
+        if (P.axis != a) continue;
+        const int n = P.n;
+        if (e) {
+            if (!P.upper && g >= 1 && g <= n) { layer = n - g; prof = P.edge; }
+            if (P.upper && g >= L - n - 1 && g <= L - 2) { layer = g - (L - n - 1); prof = P.edge; }
+        } else {
+            if (!P.upper && g >= 0 && g <= n - 1) { layer = n - 1 - g; prof = P.half; }
+            if (P.upper && g >= L - n && g <= L - 1) { layer = g - (L - n); prof = P.half; }
+        }
+    }
+    return prof;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_asm_dense(AsmDev<T> A, T* const* raw) {
+    const int64_t n = (int64_t)A.Nx * A.Ny * A.Nz;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int k = (int)(q % A.Nz), j = (int)(q / A.Nz % A.Ny), i = (int)(q / ((int64_t)A.Nz * A.Ny));
+    const int p[3] = {i, j, k};
+    const int N[3] = {A.Nx, A.Ny, A.Nz};
+    const T dt = (T)A.dt;
+    const float dtf = (float)A.dt;
+    // E: split (component c, axis a) -> Ca at raw[ca], Cb at raw[ca + 2]
+    const int esplit[6][3] = {{0, 1, 0}, {0, 2, 1}, {1, 2, 4}, {1, 0, 5}, {2, 0, 8}, {2, 1, 9}};
+    T eps[3], sig[3];
+    for (int c = 0; c < 3; ++c) asm_edge<T>(A, c, i, j, k, eps[c], sig[c]);
+    for (int s = 0; s < 6; ++s) {
+        const int c = esplit[s][0], a = esplit[s][1], ca = esplit[s][2];
+        const T e = eps[c], sg = sig[c];
+        T Ca = (T(2) * e - (sg * dt)) / (T(2) * e + (sg * dt));
+        T Cb = (T)(2.0f * dtf) / (T(2) * e + (sg * dt));
+        int layer = 0;
+        const double* prof = asm_pml<T>(A, a, p[a] + 1, N[a] + 1, true, layer);
+        if (prof) {
+            const double sp = prof[layer] * (double)(e / (T)A.e0);
+            const double sdt = sp * (double)dtf;
+            const double te = (double)(T(2) * e);
+            Ca = (T)((te - sdt) / (te + sdt));
+            Cb = (T)((double)(2.0f * dtf) / (te + sdt));
+        }
+        raw[ca][q] = Ca;
+        raw[ca + 2][q] = Cb;
+    }
+    // H: split (component c, axis a) -> Da at raw[da], Db1 and Db2 at raw[db], raw[db + 1]
+    const int hsplit[6][4] = {{0, 1, 12, 14}, {0, 2, 13, 16}, {1, 2, 18, 20}, {1, 0, 19, 22}, {2, 0, 24, 26}, {2, 1, 25, 28}};
+    const T db0 = (T)(dtf / (float)A.u0);
+    for (int s = 0; s < 6; ++s) {
+        const int c = hsplit[s][0], a = hsplit[s][1], da = hsplit[s][2], db = hsplit[s][3];
+        T Da = T(1), Db = db0;
+        int layer = 0;
+        const double* prof = asm_pml<T>(A, a, p[a], N[a], false, layer);
+        if (prof) {
+            const T e = asm_heps<T>(A, c, i, j, k);
+            const double sp = prof[layer] * (double)(e / (T)A.e0);
+            const double sm = sp * A.u0 / (double)e;
+            const double smdt = sm * (double)dtf;
+            Da = (T)((2 * A.u0 - smdt) / (2 * A.u0 + smdt));
+            Db = (T)((double)(2.0f * dtf) / (2 * A.u0 + smdt));
+        }
+        raw[da][q] = Da;
+        raw[db][q] = Db;
+        raw[db + 1][q] = Db;
+    }
+}
+
+// one override group: Ca and Cb of both splits of component c at the listed packed cells
+template <typename T>
+__global__ void __launch_bounds__(256) k_asm_fix(AsmDev<T> A, T* const* raw, int c, int kind, int wide,
+                                                 int64_t count, const int64_t* cells, const double* values) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t q = cells[t];
+    const int k = (int)(q % A.Nz), j = (int)(q / A.Nz % A.Ny), i = (int)(q / ((int64_t)A.Nz * A.Ny));
+    T e, sg;
+    asm_edge<T>(A, c, i, j, k, e, sg);
+    const T dt = (T)A.dt;
+    const float dtf = (float)A.dt;
+    T Ca, Cb;
+    if (kind == 0) {  // conductor: s = sigma dt
+        const double s = values[0];
+        if (wide) {
+            const double te = (double)(T(2) * e);
+            Ca = (T)((te - s) / (te + s));
+            Cb = (T)((double)(2.0f * dtf) / (te + s));
+        } else {
+            const T st = (T)(float)s;
+            Ca = (T(2) * e - st) / (T(2) * e + st);
+            Cb = (T)(2.0f * dtf) / (T(2) * e + st);
+        }
+    } else {  // lumped element: q = df / (2 r dn dw)
+        const double qv = values[t];
+        const T a = e / dt;
+        if (wide) {
+            const double den = (double)a + qv;
+            Ca = (T)(((double)a - qv) / den);
+            Cb = (T)(1.0 / den);
+        } else {
+            const T qt = (T)(float)qv;
+            const T den = a + qt;
+            Ca = (a - qt) / den;
+            Cb = T(1) / den;
+        }
+    }
+    const int ca = 4 * c;  // Ca of the two splits at 4c, 4c + 1; Cb at 4c + 2, 4c + 3
+    raw[ca][q] = Ca;
+    raw[ca + 1][q] = Ca;
+    raw[ca + 2][q] = Cb;
+    raw[ca + 3][q] = Cb;
+}
+
+// edge-correction factors, one thread in list order (an entry may be scaled more than once)
+template <typename T>
+__global__ void k_asm_scale(T* const* raw, int n, const rfdtd_h_scale* sc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int m = 0; m < n; ++m) {
+        T* v = raw[sc[m].coef] + sc[m].cell;
+        if (sc[m].wide || sizeof(T) == sizeof(double))
+            *v = (T)((double)*v * sc[m].factor);
+        else
+            *v = (T)((float)*v * (float)sc[m].factor);
+    }
+}
+
+// packing: Cb and Db times the signed inverse spacing; arrays 4..17 read zero on x = Nx-1
+template <typename T>
+__global__ void __launch_bounds__(256) k_asm_pack(AsmDev<T> A, T* const* raw) {
+    const int64_t n = (int64_t)A.Nx * A.Ny * A.Nz;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int k = (int)(q % A.Nz), j = (int)(q / A.Nz % A.Ny), i = (int)(q / ((int64_t)A.Nz * A.Ny));
+    // array, sign, axis, table (0: inverse dual spacing, 1: inverse width)
+    const int8_t pk[20][4] = {{2, 1, 1, 0},  {3, -1, 2, 0}, {6, 1, 2, 0},  {7, -1, 0, 0},  {10, 1, 0, 0},
+                              {11, -1, 1, 0}, {14, -1, 1, 1}, {15, -1, 1, 1}, {16, 1, 2, 1},  {17, 1, 2, 1},
+                              {20, -1, 2, 1}, {21, -1, 2, 1}, {22, 1, 0, 1},  {23, 1, 0, 1},  {26, -1, 0, 1},
+                              {27, -1, 0, 1}, {28, 1, 1, 1},  {29, 1, 1, 1},  {0, 0, 0, 0},   {0, 0, 0, 0}};
+    const int p[3] = {i, j, k};
+    for (int m = 0; m < 18; ++m) {
+        const int a = pk[m][2];
+        const T inv = pk[m][3] ? A.iw[a][p[a]] : A.idual[a][p[a]];
+        T* v = raw[pk[m][0]] + q;
+        *v = (pk[m][1] < 0 ? -*v : *v) * inv;
+    }
+    if (i == A.Nx - 1)
+        for (int m = 4; m <= 17; ++m) raw[m][q] = T(0);
+}
+
 // The monitor whose block range holds this workgroup (blk0: sampling launch, fblk0: flush launch):
 // the last one whose first block is at most blockIdx.x.  Every wavefront finds it on its own from 64
 // table entries per load, so the lookup costs one memory round trip (no serial walk, no barrier).
@@ -2335,6 +2586,71 @@ int64_t device_dedup(const TuplePtrs<U, K>& a, uint32_t n, int64_t max_entries, 
     return ng;
 }
 
+// Replace the member's raw device arrays by the table built from them on the device (numbered as
+// the host pass numbers it), when it stays within the entry limit; else the raw arrays stay.
+template <typename T>
+void table_from_raw(Member& M, uint32_t flags) {
+    using U = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+    const int64_t cells = M.cells;
+    const int64_t limit = (flags & RFDTD_COEF_TABLE) ? cells : std::max<int64_t>(4096, cells / 8);
+    const size_t idx_bytes = (size_t)cells * 4;
+    auto table_layout = [&](int64_t nE, int64_t nH) {
+        M.coef_kind = 1;
+        M.nE = nE;
+        M.nH = nH;
+        const size_t etab_bytes = (size_t)M.nE * 12 * sizeof(T);
+        const size_t htab_bytes = (size_t)M.nH * 20 * sizeof(T);
+        M.coef_bytes = 2 * align256(idx_bytes) + align256(etab_bytes) + align256(htab_bytes);
+        HIP_OK(hipMalloc(&M.d_coef, M.coef_bytes));
+        char* b = (char*)M.d_coef;
+        M.eidx = (const uint32_t*)b;
+        M.hidx = (const uint32_t*)(b + align256(idx_bytes));
+        M.etab = b + 2 * align256(idx_bytes);
+        M.htab = b + 2 * align256(idx_bytes) + align256(etab_bytes);
+        return std::make_pair(etab_bytes, htab_bytes);
+    };
+    TuplePtrs<U, 12> ea;
+    TuplePtrs<U, 18> ha;
+    for (int k = 0; k < 12; ++k) ea.p[k] = (const U*)M.raw[k];
+    for (int k = 0; k < 18; ++k) ha.p[k] = (const U*)M.raw[12 + k];
+    uint32_t *eix = nullptr, *hix = nullptr;
+    U *etb = nullptr, *htb = nullptr;
+    auto release = [&] {
+        for (void* q : {(void*)eix, (void*)hix, (void*)etb, (void*)htb})
+            if (q) (void)hipFree(q);
+    };
+    const int64_t nE = device_dedup<U, 12>(ea, (uint32_t)cells, limit, 12, &eix, &etb);
+    int64_t nH = 0;
+    if (nE <= limit) nH = device_dedup<U, 18>(ha, (uint32_t)cells, limit, 20, &hix, &htb);
+    if (nE > limit || nH > limit) {
+        release();
+        if (flags & RFDTD_COEF_TABLE) RF_THROW(RFDTD_EINVAL, "coefficient table exceeds its entry limit");
+        return;  // the raw arrays stay
+    }
+    void* raw_buf = M.d_coef;
+    const size_t raw_bytes = M.coef_bytes;
+    try {
+        M.d_coef = nullptr;
+        const auto tb = table_layout(nE, nH);
+        HIP_OK(hipMemcpy((void*)M.eidx, eix, idx_bytes, hipMemcpyDeviceToDevice));
+        HIP_OK(hipMemcpy((void*)M.hidx, hix, idx_bytes, hipMemcpyDeviceToDevice));
+        HIP_OK(hipMemcpy((void*)M.etab, etb, tb.first, hipMemcpyDeviceToDevice));
+        HIP_OK(hipMemcpy((void*)M.htab, htb, tb.second, hipMemcpyDeviceToDevice));
+    } catch (...) {  // back to the raw form, which the member's destructor frees
+        release();
+        if (M.d_coef) (void)hipFree(M.d_coef);
+        M.d_coef = raw_buf;
+        M.coef_bytes = raw_bytes;
+        M.coef_kind = 0;
+        M.eidx = M.hidx = nullptr;
+        M.etab = M.htab = nullptr;
+        throw;
+    }
+    release();
+    HIP_OK(hipFree(raw_buf));
+    for (int k = 0; k < 30; ++k) M.raw[k] = nullptr;
+}
+
 template <typename T>
 void upload_coefficients(Member& M, uint32_t flags, const void* const coeff[30]) {
     const int64_t cells = M.cells, plane = (int64_t)M.Ny * M.Nz;
@@ -2395,47 +2711,92 @@ void upload_coefficients(Member& M, uint32_t flags, const void* const coeff[30])
         if (zl[k]) HIP_OK(hipMemset(dst + (size_t)(cells - plane) * sizeof(T), 0, (size_t)plane * sizeof(T)));
     }
     if (flags & (RFDTD_COEF_RAW | RFDTD_HOST_TABLE)) return;
-    // the table, built on the device from the raw arrays (numbered as the host pass numbers it)
-    TuplePtrs<U, 12> ea;
-    TuplePtrs<U, 18> ha;
-    for (int k = 0; k < 12; ++k) ea.p[k] = (const U*)M.raw[k];
-    for (int k = 0; k < 18; ++k) ha.p[k] = (const U*)M.raw[12 + k];
-    uint32_t *eix = nullptr, *hix = nullptr;
-    U *etb = nullptr, *htb = nullptr;
-    auto release = [&] {
-        for (void* q : {(void*)eix, (void*)hix, (void*)etb, (void*)htb})
-            if (q) (void)hipFree(q);
+    table_from_raw<T>(M, flags);
+}
+
+// The member's 30 coefficient arrays computed on the device from the assembly description, then
+// deduplicated into the table unless RFDTD_COEF_RAW.
+template <typename T>
+void assemble_coefficients(Member& M, uint32_t flags, const rfdtd_assembly& a) {
+    const int64_t cells = M.cells;
+    const int N[3] = {M.Nx, M.Ny, M.Nz};
+    if (a.Nx != M.Nx || a.Ny != M.Ny || a.Nz != M.Nz) RF_THROW(RFDTD_EINVAL, "assembly sizes differ from the grid");
+    if (!a.eps || !a.sigma) RF_THROW(RFDTD_EINVAL, "cell permittivity and conductivity are required");
+    for (int d = 0; d < 3; ++d)
+        if (!a.width[d] || !a.inv_dual[d] || !a.inv_width[d]) RF_THROW(RFDTD_EINVAL, "cell widths and inverses are required");
+    if (a.n_pml < 0 || a.n_pml > kMaxPmlSides || (a.n_pml && !a.pml)) RF_THROW(RFDTD_EINVAL, "at most 6 PML sides");
+    for (int m = 0; m < a.n_pml; ++m) {
+        const rfdtd_pml_side& P = a.pml[m];
+        if (P.axis < 0 || P.axis > 2 || P.n < 1 || P.n > N[P.axis] - 1 || !P.sigma_edge || !P.sigma_half)
+            RF_THROW(RFDTD_EINVAL, "invalid PML side");
+    }
+    if (a.n_fix < 0 || (a.n_fix && !a.fix) || a.n_scale < 0 || (a.n_scale && !a.scale))
+        RF_THROW(RFDTD_EINVAL, "invalid override lists");
+    for (int m = 0; m < a.n_fix; ++m) {
+        const rfdtd_edge_fix& F = a.fix[m];
+        if (F.component < 0 || F.component > 2 || F.kind < 0 || F.kind > 1 || F.count < 0 ||
+            (F.count && (!F.cells || !F.values)))
+            RF_THROW(RFDTD_EINVAL, "invalid edge override group");
+        for (int64_t t = 0; t < F.count; ++t)
+            if (F.cells[t] < 0 || F.cells[t] >= cells) RF_THROW(RFDTD_EINVAL, "edge override outside the grid");
+    }
+    for (int m = 0; m < a.n_scale; ++m) {
+        const int c = a.scale[m].coef;
+        const bool db = (c >= 14 && c <= 17) || (c >= 20 && c <= 23) || (c >= 26 && c <= 29);
+        if (!db || a.scale[m].cell < 0 || a.scale[m].cell >= cells) RF_THROW(RFDTD_EINVAL, "invalid edge-correction factor");
+    }
+    DedupScratch S;
+    auto up = [&](const void* src, size_t bytes) {
+        void* d = S.alloc(bytes);
+        HIP_OK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+        return d;
     };
-    const int64_t nE = device_dedup<U, 12>(ea, (uint32_t)cells, limit, 12, &eix, &etb);
-    int64_t nH = 0;
-    if (nE <= limit) nH = device_dedup<U, 18>(ha, (uint32_t)cells, limit, 20, &hix, &htb);
-    if (nE > limit || nH > limit) {
-        release();
-        if (flags & RFDTD_COEF_TABLE) RF_THROW(RFDTD_EINVAL, "coefficient table exceeds its entry limit");
-        return;  // the raw arrays stay
+    AsmDev<T> A{};
+    A.Nx = M.Nx; A.Ny = M.Ny; A.Nz = M.Nz; A.npml = a.n_pml;
+    A.eps = (const double*)up(a.eps, (size_t)cells * 8);
+    A.sig = (const double*)up(a.sigma, (size_t)cells * 8);
+    for (int d = 0; d < 3; ++d) {
+        A.w[d] = (const double*)up(a.width[d], (size_t)N[d] * 8);
+        A.idual[d] = (const T*)up(a.inv_dual[d], (size_t)N[d] * sizeof(T));
+        A.iw[d] = (const T*)up(a.inv_width[d], (size_t)N[d] * sizeof(T));
     }
-    void* raw_buf = M.d_coef;
-    const size_t raw_bytes = M.coef_bytes;
-    try {
-        M.d_coef = nullptr;
-        const auto tb = table_layout(nE, nH);
-        HIP_OK(hipMemcpy((void*)M.eidx, eix, idx_bytes, hipMemcpyDeviceToDevice));
-        HIP_OK(hipMemcpy((void*)M.hidx, hix, idx_bytes, hipMemcpyDeviceToDevice));
-        HIP_OK(hipMemcpy((void*)M.etab, etb, tb.first, hipMemcpyDeviceToDevice));
-        HIP_OK(hipMemcpy((void*)M.htab, htb, tb.second, hipMemcpyDeviceToDevice));
-    } catch (...) {  // back to the raw form, which the member's destructor frees
-        release();
-        if (M.d_coef) (void)hipFree(M.d_coef);
-        M.d_coef = raw_buf;
-        M.coef_bytes = raw_bytes;
-        M.coef_kind = 0;
-        M.eidx = M.hidx = nullptr;
-        M.etab = M.htab = nullptr;
-        throw;
+    A.dt = a.dt; A.e0 = a.e0; A.u0 = a.u0;
+    for (int m = 0; m < a.n_pml; ++m) {
+        const rfdtd_pml_side& P = a.pml[m];
+        A.pml[m] = AsmPml{P.axis, P.upper ? 1 : 0, P.n, 0, (const double*)up(P.sigma_edge, (size_t)P.n * 8),
+                          (const double*)up(P.sigma_half, (size_t)P.n * 8)};
     }
-    release();
-    HIP_OK(hipFree(raw_buf));
-    for (int k = 0; k < 30; ++k) M.raw[k] = nullptr;
+    // the raw arrays, then the passes
+    M.coef_kind = 0;
+    const size_t arr_bytes = align256((size_t)cells * sizeof(T));
+    M.coef_bytes = 30 * arr_bytes;
+    HIP_OK(hipMalloc(&M.d_coef, M.coef_bytes));
+    T* rawp[30];
+    for (int k = 0; k < 30; ++k) {
+        rawp[k] = (T*)((char*)M.d_coef + (size_t)k * arr_bytes);
+        M.raw[k] = rawp[k];
+    }
+    T* const* d_raw = (T* const*)up(rawp, sizeof(rawp));
+    const int blocks = (int)((cells + 255) / 256);
+    k_asm_dense<T><<<blocks, 256>>>(A, d_raw);
+    HIP_OK(hipGetLastError());
+    for (int m = 0; m < a.n_fix; ++m) {
+        const rfdtd_edge_fix& F = a.fix[m];
+        if (!F.count) continue;
+        const int64_t* dc = (const int64_t*)up(F.cells, (size_t)F.count * 8);
+        const double* dv = (const double*)up(F.values, (size_t)(F.kind == 0 ? 1 : F.count) * 8);
+        k_asm_fix<T><<<(int)((F.count + 255) / 256), 256>>>(A, d_raw, F.component, F.kind, F.wide ? 1 : 0, F.count, dc, dv);
+        HIP_OK(hipGetLastError());
+    }
+    if (a.n_scale) {
+        const rfdtd_h_scale* ds = (const rfdtd_h_scale*)up(a.scale, (size_t)a.n_scale * sizeof(rfdtd_h_scale));
+        k_asm_scale<T><<<1, 64>>>(d_raw, a.n_scale, ds);
+        HIP_OK(hipGetLastError());
+    }
+    k_asm_pack<T><<<blocks, 256>>>(A, d_raw);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    if (!(flags & RFDTD_COEF_RAW)) table_from_raw<T>(M, flags);
 }
 
 // x planes per workgroup: enough workgroups to give every CU several waves (>= ~2048 groups in a
@@ -3059,6 +3420,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
 struct MemberSpec {
     int Nx, Ny, Nz, K;
     const void* const* coeff;
+    const rfdtd_assembly* assembly = nullptr;  // coefficients assembled on the device instead
 };
 
 Session* make_session(const std::vector<MemberSpec>& specs, uint32_t flags) {
@@ -3077,7 +3439,7 @@ Session* make_session(const std::vector<MemberSpec>& specs, uint32_t flags) {
         // 32-bit element offsets in the kernels; byte offsets of double arrays must fit too
         if ((int64_t)sp.Nx * sp.Ny * sp.Nz > ((int64_t)1 << 29)) RF_THROW(RFDTD_EINVAL, "grid too large (max 2^29 cells)");
         if (sp.K < 1 || sp.K > 64) RF_THROW(RFDTD_EINVAL, "excitations per grid must be in [1, 64]");
-        if (!sp.coeff) RF_THROW(RFDTD_EINVAL, "coefficients missing");
+        if (!sp.coeff && !sp.assembly) RF_THROW(RFDTD_EINVAL, "coefficients missing");
         Member M;
         M.Nx = sp.Nx; M.Ny = sp.Ny; M.Nz = sp.Nz;
         M.cells = (int64_t)sp.Nx * sp.Ny * sp.Nz;
@@ -3085,7 +3447,14 @@ Session* make_session(const std::vector<MemberSpec>& specs, uint32_t flags) {
         M.inst0 = inst0;
         s->members.push_back(M);
         Member& Mr = s->members.back();
-        if (s->f64) upload_coefficients<double>(Mr, flags, sp.coeff); else upload_coefficients<float>(Mr, flags, sp.coeff);
+        if (sp.assembly) {
+            if (s->f64) assemble_coefficients<double>(Mr, flags, *sp.assembly);
+            else assemble_coefficients<float>(Mr, flags, *sp.assembly);
+        } else if (s->f64) {
+            upload_coefficients<double>(Mr, flags, sp.coeff);
+        } else {
+            upload_coefficients<float>(Mr, flags, sp.coeff);
+        }
         for (int e = 0; e < sp.K; ++e) {
             Instance I;
             I.member = (int)s->members.size() - 1;
@@ -3698,6 +4067,50 @@ int rfdtd_create(rfdtd_session** out, int32_t Nx, int32_t Ny, int32_t Nz, const 
         if (!out) RF_THROW(RFDTD_EINVAL, "null session pointer");
         *out = nullptr;
         *out = (rfdtd_session*)make_session({MemberSpec{Nx, Ny, Nz, 1, coefficients}}, flags);
+    });
+}
+
+int rfdtd_create_assembled(rfdtd_session** out, const rfdtd_assembly* a, int32_t excitations, uint32_t flags) {
+    return guarded([&] {
+        if (!out || !a) RF_THROW(RFDTD_EINVAL, "null argument");
+        *out = nullptr;
+        *out = (rfdtd_session*)make_session({MemberSpec{a->Nx, a->Ny, a->Nz, excitations, nullptr, a}}, flags);
+    });
+}
+
+int rfdtd_read_coefficients(rfdtd_session* sp, int32_t member, void* const out[30]) {
+    return guarded([&] {
+        if (!sp || !out) RF_THROW(RFDTD_EINVAL, "null argument");
+        Session& s = *(Session*)sp;
+        if (member < 0 || member >= (int)s.members.size()) RF_THROW(RFDTD_EINVAL, "member index out of range");
+        const Member& M = s.members[member];
+        HIP_OK(hipDeviceSynchronize());
+        const size_t esz = s.esz;
+        const int64_t cells = M.cells, plane = (int64_t)M.Ny * M.Nz;
+        if (M.coef_kind == 0) {
+            for (int k = 0; k < 30; ++k)
+                if (out[k]) HIP_OK(hipMemcpy(out[k], M.raw[k], (size_t)cells * esz, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<uint32_t> ei(cells), hi(cells);
+            HIP_OK(hipMemcpy(ei.data(), M.eidx, (size_t)cells * 4, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(hi.data(), M.hidx, (size_t)cells * 4, hipMemcpyDeviceToHost));
+            const uint32_t ne = *std::max_element(ei.begin(), ei.end()) + 1;
+            const uint32_t nh = *std::max_element(hi.begin(), hi.end()) + 1;
+            std::vector<char> et((size_t)ne * 12 * esz), ht((size_t)nh * 20 * esz);
+            HIP_OK(hipMemcpy(et.data(), M.etab, et.size(), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(ht.data(), M.htab, ht.size(), hipMemcpyDeviceToHost));
+            for (int k = 0; k < 30; ++k) {
+                if (!out[k]) continue;
+                char* dst = (char*)out[k];
+                for (int64_t c = 0; c < cells; ++c) {
+                    const char* src = k < 12 ? &et[((size_t)ei[c] * 12 + k) * esz] : &ht[((size_t)hi[c] * 20 + (k - 12)) * esz];
+                    std::memcpy(dst + (size_t)c * esz, src, esz);
+                }
+            }
+        }
+        // the x = Nx-1 plane rule, as every kernel reads the arrays
+        for (int k = 0; k < 30; ++k)
+            if (out[k] && zero_last_plane(k)) std::memset((char*)out[k] + (size_t)(cells - plane) * esz, 0, (size_t)plane * esz);
     });
 }
 

@@ -41,6 +41,30 @@ class Session:
         self._keep = []
         self.Nt = None
 
+    @classmethod
+    def assembled(cls, model, dtype=np.float32, flags=0):
+        """A session of a meshed model whose 30 coefficient arrays are computed on the GPU from the
+        model's description (cell permittivity and conductivity, widths, PML profiles, conductor and
+        lumped-element edges, edge corrections) instead of being built on the host and copied: equal
+        to the host assembly bit for bit (rfdtd_create_assembled)."""
+        from . import assembly
+        from .hip_func import _real_dtype
+
+        self = cls.__new__(cls)
+        self.dtype = _real_dtype(dtype)
+        self.Nx, self.Ny, self.Nz = (int(n) for n in model.n_cells)
+        self._L, self._h = assembly.create(model, self.dtype, flags)
+        self._keep = []
+        self.Nt = None
+        return self
+
+    def coefficients(self):
+        """The session's 30 packed coefficient arrays read back from the GPU (from its table or raw
+        arrays; the x = Nx-1 plane of arrays 4..17 reads zero, as the kernels see it)."""
+        from . import assembly
+
+        return assembly.read_coefficients(self._L, self._h, (self.Nx, self.Ny, self.Nz), self.dtype)
+
     def set_monitor_streaming(self, ring_samples):
         """Time-domain monitors installed afterwards keep ring_samples sample planes on the GPU and
         stream the rest to their (page-locked) host arrays during the run; 0 turns it off."""

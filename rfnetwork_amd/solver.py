@@ -249,6 +249,7 @@ class FDTD_Solver:
         else:
             edges = self._edges
         self._meshed = True
+        self._edge_ops = []  # edge-correction factors applied since this mesh: (Db name, index, factor)
         self._init_grid(edges)
         self._init_dielectrics()
         self._init_coefficients()
@@ -378,23 +379,33 @@ class FDTD_Solver:
                 self.Db[f"{h}_{ax}1"] = np.ones(hshape[h], dtype=dtype_) * Db0
                 self.Db[f"{h}_{ax}2"] = np.ones(hshape[h], dtype=dtype_) * Db0
 
+    def _pml_profiles(self, side: str):
+        """(axis index, upper side, layers, sigma at the E edges, sigma at the H half cells) of one
+        PML side (solver.py:1255-1274): float64 arrays over the layers, counted from the inner
+        edge."""
+        n = self._n_pml
+        ai = _AXES.index(side[0])
+        order = 3
+        cells = self.d_cells[ai]
+        d_border = conv.m_in(cells[-1]) if side[1] == "+" else conv.m_in(cells[0])
+        eta0 = np.sqrt(u0 / e0)
+        sigma_max = 0.8 * (order + 1) / (eta0 * d_border)
+        i = np.arange(0, n)
+        sig_edge = sigma_max * (i / n) ** order          # sigma at the E edges of the layer
+        sig_half = sigma_max * ((i + 0.5) / n) ** order  # sigma at the H half-cell points
+        return ai, side[1] == "+", n, sig_edge, sig_half
+
     def _init_PML(self, side: str):
         n = self._n_pml
         if n is None:
             return
         axis = side[0]
-        ai = _AXES.index(axis)
-        order = 3
+        ai, _, _, sig_edge, sig_half = self._pml_profiles(side)
         dt = self.dt
-        cells = self.d_cells[ai]
-        d_border = conv.m_in(cells[-1]) if side[1] == "+" else conv.m_in(cells[0])
-        eta0 = np.sqrt(u0 / e0)
-        sigma_max = 0.8 * (order + 1) / (eta0 * d_border)
         shape = [None] * 3
         shape[ai] = slice(None)
-        i = np.arange(0, n)[tuple(shape)]
-        sig_edge = sigma_max * (i / n) ** order          # sigma at the E edges of the layer
-        sig_half = sigma_max * ((i + 0.5) / n) ** order  # sigma at the H half-cell points
+        sig_edge = sig_edge[tuple(shape)]
+        sig_half = sig_half[tuple(shape)]
         e_idx = [slice(None)] * 3
         h_idx = [slice(None)] * 3
         e_idx[ai] = slice(n, 0, -1) if side[1] == "-" else slice(-n - 1, -1)
@@ -418,9 +429,9 @@ class FDTD_Solver:
             self.Db[f"{h}_{axis}1"][h_idx] = (2 * dt) / ((2 * u0 + (sig_m * dt)))
             self.Db[f"{h}_{axis}2"][h_idx] = (2 * dt) / ((2 * u0 + (sig_m * dt)))
 
-    def _init_conductors(self):
-        feps = dict(ex=self.eps_ex, ey=self.eps_ey, ez=self.eps_ez)
-        splits = dict(ex=("ex_y", "ex_z"), ey=("ey_z", "ey_x"), ez=("ez_x", "ez_y"))
+    def _conductor_edges(self):
+        """Per conductor and E component, in the order the host applies them: (component, block of
+        global edge indices, inside mask over the block, sigma)."""
         for cond in self.conductors.values():
             obj, sig = cond["obj"], cond["sigma"]
             pts = np.asarray(obj.points, float)
@@ -431,14 +442,21 @@ class FDTD_Solver:
                 grid = np.meshgrid(*[self.floc[f][i][blk[i]] for i in range(3)], indexing="ij")
                 points = np.stack(grid, axis=-1)
                 inside = geometry.inside_axis_aligned(points, obj, self._tol)
-                eps = feps[f][blk]
-                ca_c = (2 * eps - (sig * self.dt)) / (2 * eps + (sig * self.dt))
-                cb_c = (2 * self.dt) / ((2 * eps + (sig * self.dt)))
-                for s in splits[f]:
-                    self.Ca[s][blk] = np.where(inside, ca_c, self.Ca[s][blk])
-                    self.Cb[s][blk] = np.where(inside, cb_c, self.Cb[s][blk])
+                yield f, blk, inside, sig
+
+    def _init_conductors(self):
+        feps = dict(ex=self.eps_ex, ey=self.eps_ey, ez=self.eps_ez)
+        splits = dict(ex=("ex_y", "ex_z"), ey=("ey_z", "ey_x"), ez=("ez_x", "ez_y"))
+        for f, blk, inside, sig in self._conductor_edges():
+            eps = feps[f][blk]
+            ca_c = (2 * eps - (sig * self.dt)) / (2 * eps + (sig * self.dt))
+            cb_c = (2 * self.dt) / ((2 * eps + (sig * self.dt)))
+            for s in splits[f]:
+                self.Ca[s][blk] = np.where(inside, ca_c, self.Ca[s][blk])
+                self.Cb[s][blk] = np.where(inside, cb_c, self.Cb[s][blk])
 
     def _init_lumped_elements(self):
+        self._lumped_fix = []  # (component axis, global edge block, df / (2 r dn dw)) per element
         ports = [el for el in self.lumped_elements.values() if el["port"] is not None]
         if ports:
             n_ports = max(abs(el["port"]) for el in ports)
@@ -488,9 +506,11 @@ class FDTD_Solver:
             nf = (p2 - p1)[f_axis]
             r_cell = r * nw / nf  # the parallel/series split of r over the face's components
             rterm = r_cell * dn * dw
-            denom = (eps_r / self.dt) + (df / (2 * rterm))
+            q = df / (2 * rterm)
+            self._lumped_fix.append((f_axis, eidx, q))
+            denom = (eps_r / self.dt) + q
             for s in (f"e{fa}_{wa}", f"e{fa}_{na}"):
-                self.Ca[s][eidx] = ((eps_r / self.dt) - (df / (2 * rterm))) / denom
+                self.Ca[s][eidx] = ((eps_r / self.dt) - q) / denom
                 self.Cb[s][eidx] = 1 / denom
             if port is None:
                 continue
@@ -1061,21 +1081,25 @@ class FDTD_Solver:
         fh = i1[f_axis] if f_dir else i1[f_axis] - 1
         ni = i1[n_axis]
         Db = self.Db
+        ops = []  # (Db name, index, factor), applied in this order
         idx = at(centers, fh, ni)
-        Db[f"h{na}_{fa}2"][idx] *= 1 / CFe
-        Db[f"h{na}_{fa}1"][idx] *= 1 / CFe
+        ops.append((f"h{na}_{fa}2", idx, 1 / CFe))
+        ops.append((f"h{na}_{fa}1", idx, 1 / CFe))
         if i1[e_axis] > 0:
-            Db[f"h{na}_{ea}2"][at(i1[e_axis] - 1, fh, ni)] *= CFe
+            ops.append((f"h{na}_{ea}2", at(i1[e_axis] - 1, fh, ni), CFe))
         if i2[e_axis] < Db[f"h{na}_{ea}1"].shape[e_axis]:
-            Db[f"h{na}_{ea}1"][at(i2[e_axis], fh, ni)] *= CFe
-        Db[f"h{ea}_{na}2"][at(edges, fh, ni - 1)] *= CFe
-        Db[f"h{ea}_{na}1"][at(edges, fh, ni)] *= CFe
+            ops.append((f"h{na}_{ea}1", at(i2[e_axis], fh, ni), CFe))
+        ops.append((f"h{ea}_{na}2", at(edges, fh, ni - 1), CFe))
+        ops.append((f"h{ea}_{na}1", at(edges, fh, ni), CFe))
         for k in (ni - 1, ni):
-            Db[f"h{ea}_{fa}2"][at(edges, i1[f_axis] - 1, k)] *= CFe
-            Db[f"h{ea}_{fa}1"][at(edges, i1[f_axis], k)] *= CFe
-            Db[f"h{fa}_{na}1"][at(centers, i1[f_axis], k)] *= 1 / CFe
-            Db[f"h{fa}_{na}2"][at(centers, i1[f_axis], k)] *= 1 / CFe
+            ops.append((f"h{ea}_{fa}2", at(edges, i1[f_axis] - 1, k), CFe))
+            ops.append((f"h{ea}_{fa}1", at(edges, i1[f_axis], k), CFe))
+            ops.append((f"h{fa}_{na}1", at(centers, i1[f_axis], k), 1 / CFe))
+            ops.append((f"h{fa}_{na}2", at(centers, i1[f_axis], k), 1 / CFe))
             if i1[e_axis] > 0:
-                Db[f"h{fa}_{ea}2"][at(i1[e_axis] - 1, i1[f_axis], k)] *= CFe
+                ops.append((f"h{fa}_{ea}2", at(i1[e_axis] - 1, i1[f_axis], k), CFe))
             if i2[e_axis] < Db[f"h{fa}_{ea}1"].shape[e_axis]:
-                Db[f"h{fa}_{ea}1"][at(i2[e_axis], i1[f_axis], k)] *= CFe
+                ops.append((f"h{fa}_{ea}1", at(i2[e_axis], i1[f_axis], k), CFe))
+        for name, ix, factor in ops:
+            Db[name][ix] *= factor
+        self._edge_ops.extend(ops)

@@ -52,9 +52,10 @@ def _grid_solver(nx, ny, nz, cell_m):
     return s, d
 
 
-def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=10, n_freq=101):
+def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=10, n_freq=101, coef_dtype=np.float32):
     """Two-port microstrip line (the 120x48x32 plain-FDTD workload)."""
     s, d = _grid_solver(nx, ny, nz, cell_m)
+    s.coef_dtype = coef_dtype
     h = 4  # substrate thickness, cells
     y0, y1 = ny // 2 - 3, ny // 2 + 3  # 6-cell strip centred in y
     xa, xb = n_pml + 1, nx - n_pml - 1  # ports 1 cell in from the x PML
@@ -104,10 +105,11 @@ def dipole_slab(Nt=8000, nx=400, ny=400, nz=200, cell_m=1e-4, courant=0.99, n_pm
 
 
 def tiny_model(Nt=60, nx=14, ny=11, nz=9, cell_m=1e-4, courant=0.99, n_pml=3, seed=7, all_sides=True,
-               time_monitors=True, phasor_monitors=True):
+               time_monitors=True, phasor_monitors=True, coef_dtype=np.float32):
     """Small model that touches every code path: PML, random dielectric, conductor, port, probes,
     time-domain and phasor monitors on every axis."""
     s, d = _grid_solver(nx, ny, nz, cell_m)
+    s.coef_dtype = coef_dtype
     rng = np.random.default_rng(seed)
     s.add_permittivity_volume(rng.uniform(1.0, 4.0, size=(nx - 6, ny - 6, 3)), (3, 3, 1))
     s.add_dielectric(Box((0, 0, 0), (nx * d, ny * d, 1 * d)), er=2.2)
