@@ -236,8 +236,16 @@ class FDTD_Solver:
         return edges
 
     def generate_mesh(self, d_max: float = None, d_min: float = None, surface_tolerance: float = 0.1,
-                      courant: float = None):
-        """Build the grid and the FDTD coefficients (order of solver.py:413-449)."""
+                      courant: float = None, coefficients: str = "host"):
+        """Build the grid and the FDTD coefficients (order of solver.py:413-449).
+
+        coefficients="device" skips the host's full-grid coefficient arrays: the model keeps the cell
+        permittivity and conductivity, the time step, the ports (their source gains from the edge
+        permittivity of the port edges alone) and the edge corrections as a list, and a session of it
+        assembles the arrays on the GPU (Session.assembled; solve() does so).  The arrays are the same
+        bit for bit; coefficient_arrays() is then unavailable."""
+        if coefficients not in ("host", "device"):
+            raise ValueError('coefficients must be "host" or "device"')
         if courant is not None:
             self.courant = float(courant)
         if self._edges is None:
@@ -249,13 +257,17 @@ class FDTD_Solver:
         else:
             edges = self._edges
         self._meshed = True
+        self._coef_mode = coefficients
         self._edge_ops = []  # edge-correction factors applied since this mesh: (Db name, index, factor)
         self._init_grid(edges)
         self._init_dielectrics()
-        self._init_coefficients()
-        for side in self.pml_boundaries:
-            self._init_PML(side)
-        self._init_conductors()
+        if coefficients == "host":
+            self._init_coefficients()
+            for side in self.pml_boundaries:
+                self._init_PML(side)
+            self._init_conductors()
+        else:
+            self._init_time_step()
         self._init_lumped_elements()
         self.invalidate_solution()
 
@@ -306,10 +318,9 @@ class FDTD_Solver:
             if vol["sigma"] is not None:
                 self.sigma[blk] = vol["sigma"]
 
-    def _init_coefficients(self):
+    def _init_time_step(self):
         dtype_ = self.coef_dtype
         dx, dy, dz = [conv.m_in(d) for d in self.d_cells]
-        Nx, Ny, Nz = self.n_cells
         # time step from the smallest cell on each axis (free-space Courant limit)
         lmin = np.array([np.min(dx), np.min(dy), np.min(dz)], dtype=dtype_)
         dmin = 1 / np.sqrt(((1 / lmin) ** 2).sum())
@@ -317,6 +328,54 @@ class FDTD_Solver:
         if isinstance(dt, float):  # a python float courant keeps dt in float32 (numpy promotion)
             dt = np.float32(dt)
         self.dt = dt
+
+    def _edge_eps(self, f, blk):
+        """Permittivity of E component f on a block of its global edges, in the coefficient dtype,
+        formed as _init_coefficients forms eps_ex / eps_ey / eps_ez (a float64 average of the 4
+        adjacent cells weighted by float32 half widths; e0 on the boundary), without the full-grid
+        arrays."""
+        T = np.dtype(self.coef_dtype).type
+        dx, dy, dz = [conv.m_in(d) for d in self.d_cells]
+        d = (dx, dy, dz)
+        Nx, Ny, Nz = self.n_cells
+        axes = []
+        for b, n in zip(blk, self.fshape[f]):
+            axes.append(np.arange(*b.indices(n)) if isinstance(b, slice) else np.array([int(b)]))
+        # the two axes averaged, in the host's order (Ex: y then z, Ey: x then z, Ez: x then y)
+        first, second = {"ex": (1, 2), "ey": (0, 2), "ez": (0, 1)}[f]
+        v = self.eps
+        out = np.empty([len(a) for a in axes], dtype=T)
+
+        def avg(v0, v1, w0, w1):
+            return (v0 * (w0 / 2) + v1 * (w1 / 2)) / (w0 / 2 + w1 / 2)
+
+        for n, g in enumerate(np.ndindex(*out.shape)):
+            gi = [int(axes[a][g[a]]) for a in range(3)]
+            lim = (Nx, Ny, Nz)
+            if not (1 <= gi[first] <= lim[first] - 1 and 1 <= gi[second] <= lim[second] - 1):
+                out[g] = T(e0)
+                continue
+            c = list(gi)
+
+            def cell(a_off, b_off):
+                cc = list(c)
+                cc[first] = gi[first] - 1 + a_off
+                cc[second] = gi[second] - 1 + b_off
+                return v[tuple(cc)]
+
+            w_a0, w_a1 = d[first][gi[first] - 1], d[first][gi[first]]
+            w_b0, w_b1 = d[second][gi[second] - 1], d[second][gi[second]]
+            u0 = avg(cell(0, 0), cell(1, 0), w_a0, w_a1)
+            u1 = avg(cell(0, 1), cell(1, 1), w_a0, w_a1)
+            out[g] = T(avg(u0, u1, w_b0, w_b1))
+        return out
+
+    def _init_coefficients(self):
+        dtype_ = self.coef_dtype
+        dx, dy, dz = [conv.m_in(d) for d in self.d_cells]
+        Nx, Ny, Nz = self.n_cells
+        self._init_time_step()
+        dt = self.dt
 
         def edge_average(v):
             # cells adjacent to an edge, weighted by their half widths along each axis
@@ -498,10 +557,12 @@ class FDTD_Solver:
             df = d_cells[f_axis][p1[f_axis]: p2[f_axis]]
             gx, gy, gz = np.meshgrid(*place(dn, dw, df), indexing="ij")
             dn, dw, df = [(gx, gy, gz)[a] for a in (n_axis, w_axis, f_axis)]
-            eps_field = (self.eps_ex, self.eps_ey, self.eps_ez)[f_axis]
             eidx = place(slice(p1[n_axis], p2[n_axis] + 1), slice(p1[w_axis], p2[w_axis] + 1),
                          slice(p1[f_axis], p2[f_axis]))
-            eps_r = eps_field[eidx]
+            if getattr(self, "_coef_mode", "host") == "host":
+                eps_r = (self.eps_ex, self.eps_ey, self.eps_ez)[f_axis][eidx]
+            else:
+                eps_r = self._edge_eps(f"e{fa}", eidx)
             nw = (p2 - p1 + 1)[w_axis]
             nf = (p2 - p1)[f_axis]
             r_cell = r * nw / nf  # the parallel/series split of r over the face's components
@@ -509,9 +570,10 @@ class FDTD_Solver:
             q = df / (2 * rterm)
             self._lumped_fix.append((f_axis, eidx, q))
             denom = (eps_r / self.dt) + q
-            for s in (f"e{fa}_{wa}", f"e{fa}_{na}"):
-                self.Ca[s][eidx] = ((eps_r / self.dt) - q) / denom
-                self.Cb[s][eidx] = 1 / denom
+            if getattr(self, "_coef_mode", "host") == "host":
+                for s in (f"e{fa}_{wa}", f"e{fa}_{na}"):
+                    self.Ca[s][eidx] = ((eps_r / self.dt) - q) / denom
+                    self.Cb[s][eidx] = 1 / denom
             if port is None:
                 continue
             group = self.ports if port > 0 else self.ports_inv
@@ -837,6 +899,9 @@ class FDTD_Solver:
         """The 30 solver coefficient arrays with cell spacings and curl signs folded in
         (solver.py:1439-1523): all (Nx, Ny, Nz) for gpu=True, the CPU solver's shapes otherwise."""
         self.check_mesh()
+        if getattr(self, "_coef_mode", "host") != "host":
+            raise ValueError('the model was meshed with coefficients="device": its arrays are assembled on the '
+                             "GPU (Session.assembled)")
         dtype_ = np.dtype(dtype)
         dx, dy, dz = [conv.m_in(d).astype(dtype_) for d in self.d_cells]
         dx_inv, dy_inv, dz_inv = 1 / dx[:, None, None], 1 / dy[None, :, None], 1 / dz[None, None, :]
@@ -969,6 +1034,8 @@ class FDTD_Solver:
         if backend is None:
             if not gpu:
                 raise NotImplementedError("rfnetwork_amd runs on the GPU only: call solve(gpu=True)")
+            if getattr(self, "_coef_mode", "host") == "device":
+                return self._solve_assembled(dtype, flags, show_progress)
             backend = hip_func.solver_run_hip
         coefficients, probes, monitors, mem, Nx, Ny, Nz, Nt = self.pack(gpu=gpu, dtype=dtype, n_threads=n_threads)
         n_cells = self.Nx * self.Ny * self.Nz
@@ -987,6 +1054,24 @@ class FDTD_Solver:
         if show_progress:
             sys.stdout.write(f"\rDone in {time.time() - t0:.3f}s" + " " * 20 + "\n")
         self.unpack(probes, monitors, gpu=gpu)
+
+    def _solve_assembled(self, dtype, flags, show_progress):
+        """solve() of a model meshed with coefficients="device": the session assembles the
+        coefficient arrays on the GPU; probes and monitors as in solve()."""
+        from .session import Session
+
+        _, probes, monitors, _, Nx, Ny, Nz, Nt = self.pack(gpu=True, dtype=dtype, coefficients=False)
+        t0 = time.time()
+        if show_progress:
+            print(f"Running solver with {Nx * Ny * Nz / 1e3:.1f}k cells, and {Nt} time steps...")
+        with Session.assembled(self, dtype, flags) as s:
+            s.set_sources(probes, monitors, Nt)
+            s.run(Nt)
+            s.synchronize()
+            s.read()
+        if show_progress:
+            sys.stdout.write(f"\rDone in {time.time() - t0:.3f}s" + " " * 20 + "\n")
+        self.unpack(probes, monitors, gpu=True)
 
     # ------------------------------------------------------------------ results
     def line_probe_values(self, name: str) -> np.ndarray:
@@ -1080,14 +1165,15 @@ class FDTD_Solver:
         edges = slice(i1[e_axis], i2[e_axis] + 1)
         fh = i1[f_axis] if f_dir else i1[f_axis] - 1
         ni = i1[n_axis]
-        Db = self.Db
+        host = getattr(self, "_coef_mode", "host") == "host"
+        fs = self.fshape
         ops = []  # (Db name, index, factor), applied in this order
         idx = at(centers, fh, ni)
         ops.append((f"h{na}_{fa}2", idx, 1 / CFe))
         ops.append((f"h{na}_{fa}1", idx, 1 / CFe))
         if i1[e_axis] > 0:
             ops.append((f"h{na}_{ea}2", at(i1[e_axis] - 1, fh, ni), CFe))
-        if i2[e_axis] < Db[f"h{na}_{ea}1"].shape[e_axis]:
+        if i2[e_axis] < fs[f"h{na}"][e_axis]:
             ops.append((f"h{na}_{ea}1", at(i2[e_axis], fh, ni), CFe))
         ops.append((f"h{ea}_{na}2", at(edges, fh, ni - 1), CFe))
         ops.append((f"h{ea}_{na}1", at(edges, fh, ni), CFe))
@@ -1098,8 +1184,9 @@ class FDTD_Solver:
             ops.append((f"h{fa}_{na}2", at(centers, i1[f_axis], k), 1 / CFe))
             if i1[e_axis] > 0:
                 ops.append((f"h{fa}_{ea}2", at(i1[e_axis] - 1, i1[f_axis], k), CFe))
-            if i2[e_axis] < Db[f"h{fa}_{ea}1"].shape[e_axis]:
+            if i2[e_axis] < fs[f"h{fa}"][e_axis]:
                 ops.append((f"h{fa}_{ea}1", at(i2[e_axis], i1[f_axis], k), CFe))
-        for name, ix, factor in ops:
-            Db[name][ix] *= factor
+        if host:
+            for name, ix, factor in ops:
+                self.Db[name][ix] *= factor
         self._edge_ops.extend(ops)

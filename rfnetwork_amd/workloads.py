@@ -52,7 +52,8 @@ def _grid_solver(nx, ny, nz, cell_m):
     return s, d
 
 
-def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=10, n_freq=101, coef_dtype=np.float32):
+def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=10, n_freq=101, coef_dtype=np.float32,
+               coefficients="host"):
     """Two-port microstrip line (the 120x48x32 plain-FDTD workload)."""
     s, d = _grid_solver(nx, ny, nz, cell_m)
     s.coef_dtype = coef_dtype
@@ -64,7 +65,7 @@ def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=1
     s.add_lumped_port(1, Rectangle([(xa * d, y0 * d, 0), (xa * d, y1 * d, 0), (xa * d, y1 * d, h * d)]), "z+")
     s.add_lumped_port(2, Rectangle([(xb * d, y0 * d, 0), (xb * d, y1 * d, 0), (xb * d, y1 * d, h * d)]), "z+")
     s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=n_pml)
-    s.generate_mesh(courant=courant)
+    s.generate_mesh(courant=courant, coefficients=coefficients)
     src = s.gaussian_modulated_source(f0=5.5e9, width=430e-12, t0=320e-12, t_len=(Nt + 0.5) * s.dt)
     s.assign_excitation(1e-2 * np.asarray(src), 1)
     s.frequency = np.linspace(1e9, 1e10, n_freq)
@@ -72,7 +73,7 @@ def microstrip(Nt=4000, nx=120, ny=48, nz=32, cell_m=1e-4, courant=0.99, n_pml=1
 
 
 def dipole_slab(Nt=8000, nx=400, ny=400, nz=200, cell_m=1e-4, courant=0.99, n_pml=12, n_freq=51, seed=2,
-                monitors=True, dft_dtype=None):
+                monitors=True, dft_dtype=None, coefficients="host"):
     """Dipole over a random dielectric slab (the 400x400x200 bandwidth workload).  dft_dtype:
     accumulator type of the 6 running-DFT planes (default complex64 in a float32 solve)."""
     s, d = _grid_solver(nx, ny, nz, cell_m)
@@ -88,7 +89,7 @@ def dipole_slab(Nt=8000, nx=400, ny=400, nz=200, cell_m=1e-4, courant=0.99, n_pm
     s.add_lumped_port(1, Rectangle([(cx * d, (cy - 1) * d, cz * d), (cx * d, (cy + 1) * d, cz * d),
                                     (cx * d, (cy + 1) * d, (cz + 1) * d)]), "z+")
     s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z-", "z+", n_pml=n_pml)
-    s.generate_mesh(courant=courant)
+    s.generate_mesh(courant=courant, coefficients=coefficients)
     src = s.gaussian_source(width=50e-12, t0=40e-12, t_len=(Nt + 0.5) * s.dt)
     s.assign_excitation(np.asarray(src), 1)
     freqs = np.linspace(5e9, 3e10, n_freq)
@@ -105,7 +106,7 @@ def dipole_slab(Nt=8000, nx=400, ny=400, nz=200, cell_m=1e-4, courant=0.99, n_pm
 
 
 def tiny_model(Nt=60, nx=14, ny=11, nz=9, cell_m=1e-4, courant=0.99, n_pml=3, seed=7, all_sides=True,
-               time_monitors=True, phasor_monitors=True, coef_dtype=np.float32):
+               time_monitors=True, phasor_monitors=True, coef_dtype=np.float32, coefficients="host"):
     """Small model that touches every code path: PML, random dielectric, conductor, port, probes,
     time-domain and phasor monitors on every axis."""
     s, d = _grid_solver(nx, ny, nz, cell_m)
@@ -118,7 +119,7 @@ def tiny_model(Nt=60, nx=14, ny=11, nz=9, cell_m=1e-4, courant=0.99, n_pml=3, se
     s.add_lumped_port(2, Rectangle([(10 * d, 4 * d, 0), (10 * d, 7 * d, 0), (10 * d, 7 * d, 4 * d)]), "z+")
     sides = ("x-", "x+", "y-", "y+", "z-", "z+") if all_sides else ("x+", "z+")
     s.assign_PML_boundaries(*sides, n_pml=n_pml)
-    s.generate_mesh(courant=courant)
+    s.generate_mesh(courant=courant, coefficients=coefficients)
     src = s.gaussian_modulated_source(f0=20e9, width=60e-12, t0=30e-12, t_len=(Nt + 0.5) * s.dt)
     s.assign_excitation(1e-2 * np.asarray(src), 1)
     s.add_voltage_probe("v1", Line((7 * d, 5 * d, 0), (7 * d, 5 * d, 4 * d)))
@@ -261,7 +262,7 @@ def design_variant(v, eps_design=None, Nt=8000, cell_m=1e-4, courant=0.99, n_pml
     return s
 
 
-def half_wave_dipole(d=0.0075):
+def half_wave_dipole(d=0.0075, coefficients="host"):
     """The strip dipole of rfnetwork's dipole test on a uniform grid of d inches (the strip width, gap
     and arm length snapped to it: 4, 2 and 37 cells)."""
     nx, ny, nz = 200, 148, 148
@@ -277,7 +278,7 @@ def half_wave_dipole(d=0.0075):
     s.add_conductor(Rectangle([(0, ms_y[0], z2[0]), (0, ms_y[1], z2[0]), (0, ms_y[1], z2[1])]))
     s.add_lumped_port(1, Rectangle([(0, ms_y[0], gap / 2), (0, ms_y[1], gap / 2), (0, ms_y[1], -gap / 2)]), "z-")
     s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", "z-", n_pml=5)
-    s.generate_mesh()
+    s.generate_mesh(coefficients=coefficients)
     s.add_farfield_monitor(frequency=[10e9, 20e9])
     for i, mz in enumerate((z1, z2)):
         s.edge_correction((0, ms_y[0], mz[0]), (0, ms_y[0], mz[1]), integration_line="y-")

@@ -28,3 +28,37 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     den = np.linalg.norm(b.ravel())
     return np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0)
+
+
+def graded_everything(Nt, coef_dtype=np.float32, seed=13, coefficients="host"):
+    """A graded grid (every cell width different) holding every coefficient kind of the assembly:
+    lossy random permittivity cells, a lossy dielectric box, a conductor strip with edge
+    corrections along its edges, a two-part port (a Line integration path), a resistor, PML on five
+    sides, voltage and current probes, time and phasor monitors."""
+    from rfnetwork_amd.geometry import Box, Line, Rectangle
+    from rfnetwork_amd.solver import FDTD_Solver
+
+    rng = np.random.default_rng(seed)
+    d = 1e-4 * 39.3701
+    widths = [d * rng.uniform(0.7, 1.5, size=n) for n in (26, 20, 16)]
+    ex, ey, ez = edges = [np.concatenate([[0.0], np.cumsum(w)]) for w in widths]
+    s = FDTD_Solver(Box((0, 0, 0), (float(ex[-1]), float(ey[-1]), float(ez[-1]))))
+    s.coef_dtype = coef_dtype
+    s.set_grid(*edges)
+    s.add_dielectric(Box((0, 0, 0), (float(ex[-1]), float(ey[-1]), float(ez[3]))), er=3.4, loss_tan=0.02, f0=10e9)
+    s.add_permittivity_volume(rng.uniform(1.0, 4.0, size=(8, 6, 3)), (10, 7, 5),
+                              sigma=rng.uniform(0.0, 0.1, size=(8, 6, 3)))
+    s.add_conductor(Rectangle([(ex[6], ey[8], ez[3]), (ex[6], ey[12], ez[3]), (ex[20], ey[12], ez[3])]))
+    s.add_lumped_port(1, Rectangle([(ex[6], ey[8], ez[0]), (ex[6], ey[12], ez[0]), (ex[6], ey[12], ez[3])]),
+                      Line((ex[6], ey[10], ez[0]), (ex[6], ey[10], ez[2])))
+    s.add_resistor(Rectangle([(ex[20], ey[8], ez[0]), (ex[20], ey[12], ez[0]), (ex[20], ey[12], ez[3])]), 75.0, "z+")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=3)
+    s.generate_mesh(courant=0.9, coefficients=coefficients)
+    s.edge_correction((ex[6], ey[8], ez[3]), (ex[20], ey[8], ez[3]), integration_line="y-")
+    s.edge_correction((ex[6], ey[12], ez[3]), (ex[20], ey[12], ez[3]), integration_line="y+")
+    src = s.gaussian_modulated_source(f0=15e9, width=60e-12, t0=30e-12, t_len=(Nt + 0.5) * s.dt)
+    s.assign_excitation(1e-2 * np.asarray(src), 1)
+    s.add_voltage_probe("v_mid", Line((ex[13], ey[10], ez[0]), (ex[13], ey[10], ez[3])))
+    s.add_field_monitor("t_ez", "ez", "x", index=13, n_step=2)
+    s.add_field_monitor("p_hy", "hy", "z", index=5, frequency=np.array([8e9, 16e9, 30e9]))
+    return s

@@ -11,8 +11,7 @@ import pytest
 from rfnetwork_amd import workloads
 from rfnetwork_amd.session import Session
 
-from helpers import run_oracle
-from test_gpu_parity import _graded_everything
+from helpers import graded_everything as _graded_everything, run_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +97,37 @@ def test_assembled_session_runs_as_the_oracle(hip_ready):
         np.testing.assert_array_equal(a["values"], b["values"])
     for k in f_c:
         np.testing.assert_array_equal(f[k], f_c[k], err_msg=k)
+
+
+DEVICE_MODE = {
+    "tiny": lambda c: workloads.tiny_model(Nt=30, coefficients=c),
+    "microstrip": lambda c: workloads.microstrip(Nt=30, coefficients=c),
+    "graded": lambda c: _graded_everything(30, coefficients=c),
+    "half_wave_dipole": lambda c: workloads.half_wave_dipole(coefficients=c),
+    "dipole_slab": lambda c: workloads.dipole_slab(Nt=20, coefficients=c),
+}
+
+
+@pytest.mark.parametrize("name", list(DEVICE_MODE))
+def test_device_mode_model_assembles_the_host_arrays(hip_ready, name):
+    """A model meshed with coefficients="device" (no full-grid arrays on the host) gives, on the GPU,
+    the arrays the host assembly builds for the same model."""
+    print(f"[case] device-mode model {name}", flush=True)
+    ref = _host_arrays(DEVICE_MODE[name]("host"), np.float32)
+    model = DEVICE_MODE[name]("device")
+    with Session.assembled(model) as s:
+        got = s.coefficients()
+    for k in ref:
+        np.testing.assert_array_equal(_bits(got[k]), _bits(ref[k]), err_msg=k)
+
+
+def test_device_mode_solve_equals_host_mode(hip_ready):
+    """solve() of the microstrip meshed either way: identical probe series and S-parameters."""
+    print("[case] device-mode solve", flush=True)
+    out = []
+    for c in ("host", "device"):
+        m = workloads.microstrip(Nt=400, coefficients=c)
+        m.solve(show_progress=False)
+        out.append((m.vi_probe_values("port_1"), np.asarray(m.get_sparameters(m.frequency[:5]))))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
