@@ -574,11 +574,15 @@ def plain_leg(args):
     t0 = time.perf_counter()
     model = build_model(args.workload, Nt)
     cells = model.Nx * model.Ny * model.Nz
-    log(f"model {args.workload} {model.Nx}x{model.Ny}x{model.Nz} built in {time.perf_counter() - t0:.1f}s")
+    t_model = time.perf_counter() - t0
+    log(f"model {args.workload} {model.Nx}x{model.Ny}x{model.Nz} built in {t_model:.1f}s")
     c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    t_pack = time.perf_counter() - t0 - t_model
     flags = _lib.KERNEL_TIMING | {"auto": 0, "raw": _lib.COEF_RAW, "table": _lib.COEF_TABLE}[args.coef]
     t0 = time.perf_counter()
     s = Session(c, Nx, Ny, Nz, flags=flags)
+    s.synchronize()
+    t_session = time.perf_counter() - t0
     del c
     tuned = None
     if args.block or args.xplanes or args.variant >= 0:
@@ -628,6 +632,9 @@ def plain_leg(args):
         copy = dict(gb_per_s=round(gbs, 1), bytes_per_copy=2 * (1 << 30), ms_per_copy=round(cms, 4))
         log(f"copy probe {gbs:.1f} GB/s")
 
+    setup = None
+    if args.coef == "auto":
+        setup = setup_times(args.workload, Nt, t_model + t_pack + t_session)
     base = None
     if args.cpu_budget > 0:
         log("cpu baseline ...")
@@ -647,8 +654,28 @@ def plain_leg(args):
         step=dict(hbm_frac_algorithmic=round(step_frac, 4), algorithmic_bytes=step_bytes,
                   e_kernel_ms=None if fused else round(e_ms, 4), e_kernel_gbs=e_gbs and round(e_gbs, 1),
                   h_kernel_ms=None if fused else round(h_ms, 4), dft_bytes=dft_b),
-        copy_probe=copy, cpu_baseline=base, finite=finite,
+        copy_probe=copy, cpu_baseline=base, finite=finite, setup=setup,
     )
+
+
+def setup_times(workload, Nt, host_s):
+    """Model to resident session, with the coefficients assembled on the host (mesh, 30 arrays,
+    packing, copy, table) against the model meshed with coefficients="device" and its session
+    assembling them on the GPU (the same arrays bit for bit)."""
+    from rfnetwork_amd import workloads
+    from rfnetwork_amd.session import Session
+
+    t0 = time.perf_counter()
+    build = dict(dipole_slab=workloads.dipole_slab, microstrip=workloads.microstrip)[workload]
+    model = build(Nt=Nt, coefficients="device")
+    t_model = time.perf_counter() - t0
+    with Session.assembled(model) as s:
+        s.synchronize()
+        info = s.memory_info()
+    dev_s = time.perf_counter() - t0
+    log(f"setup: host assembly {host_s:.2f}s, device assembly {dev_s:.2f}s (model {t_model:.2f}s), {info['coef_kind']}")
+    return dict(host_assembly_s=round(host_s, 3), device_assembly_s=round(dev_s, 3), device_model_s=round(t_model, 3),
+                speedup=round(host_s / dev_s, 2))
 
 
 if __name__ == "__main__":
