@@ -321,6 +321,10 @@ typedef struct rfdtd_assembly {
     const rfdtd_h_scale* scale;     /* n_scale factors */
 } rfdtd_assembly;
 int rfdtd_create_assembled(rfdtd_session** out, const rfdtd_assembly* a, int32_t excitations, uint32_t flags);
+/* A batch (as rfdtd_create_batch) of n_grids grids each assembled on the GPU from a[g], with
+ * excitations[g] instances each. */
+int rfdtd_create_batch_assembled(rfdtd_session** out, int32_t n_grids, const rfdtd_assembly* a,
+                                 const int32_t* excitations, uint32_t flags);
 /* The member's 30 packed coefficient arrays (the x = Nx-1 plane of arrays 4..17 reads zero), from
  * its table or raw arrays, into host arrays of the element type (Nx*Ny*Nz each; null skips one). */
 int rfdtd_read_coefficients(rfdtd_session* s, int32_t member, void* const out[30]);

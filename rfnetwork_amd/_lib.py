@@ -55,7 +55,7 @@ EXPORTS = (
     "rfdtd_plan_checkpoints", "rfdtd_nf2ff", "rfdtd_set_history", "rfdtd_device_memory",
     "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
     "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
-    "rfdtd_create_assembled", "rfdtd_read_coefficients",
+    "rfdtd_create_assembled", "rfdtd_read_coefficients", "rfdtd_create_batch_assembled",
 )
 
 
@@ -177,6 +177,7 @@ def lib() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.c_double)]
     L.rfdtd_create_assembled.argtypes = [ctypes.POINTER(vp), vp, i32, u32]
     L.rfdtd_read_coefficients.argtypes = [vp, i32, vp]
+    L.rfdtd_create_batch_assembled.argtypes = [ctypes.POINTER(vp), i32, vp, vp, u32]
     _lib = L
     return L
 

@@ -52,6 +52,41 @@ class Batch:
             self.instances += [(g, k) for k in range(K)]
         self._keep = {}
 
+    @classmethod
+    def assembled(cls, models, excitations=None, dtype=np.float32, flags=0):
+        """A batch of meshed models whose coefficient arrays are assembled on the GPU
+        (rfdtd_create_batch_assembled; equal to the host assembly bit for bit); excitations: the
+        instances per model (default 1 each)."""
+        from . import assembly
+        from .hip_func import _real_dtype
+
+        self = cls.__new__(cls)
+        self.dtype = _real_dtype(dtype)
+        excitations = list(excitations) if excitations is not None else [1] * len(models)
+        if len(excitations) != len(models):
+            raise ValueError("one excitation count per model")
+        descs = [assembly.Description(m, self.dtype) for m in models]
+        arr = (_lib.Assembly * len(descs))(*[D.struct for D in descs])
+        e = (ctypes.c_int32 * len(excitations))(*[int(k) for k in excitations])
+        self.grids = [tuple(int(n) for n in m.n_cells) + (int(k),) for m, k in zip(models, excitations)]
+        self._L = _lib.lib()
+        self._h = ctypes.c_void_p()
+        f = int(flags) | (_lib.F64 if self.dtype == np.float64 else 0)
+        check(self._L.rfdtd_create_batch_assembled(ctypes.byref(self._h), len(descs), ctypes.cast(arr, ctypes.c_void_p),
+                                                   ctypes.cast(e, ctypes.c_void_p), f), "rfdtd_create_batch_assembled")
+        self.instances = []
+        for g, (_, _, _, K) in enumerate(self.grids):
+            self.instances += [(g, k) for k in range(K)]
+        self._keep = {}
+        return self
+
+    def coefficients(self, member):
+        """Member `member`'s 30 packed coefficient arrays read back from the GPU."""
+        from . import assembly
+
+        Nx, Ny, Nz, _ = self.grids[member]
+        return assembly.read_coefficients(self._L, self._h, (Nx, Ny, Nz), self.dtype, member)
+
     def set_monitor_streaming(self, ring_samples):
         """As Session.set_monitor_streaming, for the instances' monitors installed afterwards."""
         check(self._L.rfdtd_set_monitor_streaming(self._h, int(ring_samples)), "rfdtd_set_monitor_streaming")

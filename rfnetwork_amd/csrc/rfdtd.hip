@@ -4078,6 +4078,18 @@ int rfdtd_create_assembled(rfdtd_session** out, const rfdtd_assembly* a, int32_t
     });
 }
 
+int rfdtd_create_batch_assembled(rfdtd_session** out, int32_t n_grids, const rfdtd_assembly* a,
+                                 const int32_t* excitations, uint32_t flags) {
+    return guarded([&] {
+        if (!out || !a || !excitations || n_grids < 1) RF_THROW(RFDTD_EINVAL, "invalid batch arguments");
+        *out = nullptr;
+        std::vector<MemberSpec> specs;
+        for (int g = 0; g < n_grids; ++g)
+            specs.push_back(MemberSpec{a[g].Nx, a[g].Ny, a[g].Nz, excitations[g], nullptr, &a[g]});
+        *out = (rfdtd_session*)make_session(specs, flags);
+    });
+}
+
 int rfdtd_read_coefficients(rfdtd_session* sp, int32_t member, void* const out[30]) {
     return guarded([&] {
         if (!sp || !out) RF_THROW(RFDTD_EINVAL, "null argument");
