@@ -328,6 +328,9 @@ int rfdtd_create_batch_assembled(rfdtd_session** out, int32_t n_grids, const rfd
 /* The member's 30 packed coefficient arrays (the x = Nx-1 plane of arrays 4..17 reads zero), from
  * its table or raw arrays, into host arrays of the element type (Nx*Ny*Nz each; null skips one). */
 int rfdtd_read_coefficients(rfdtd_session* s, int32_t member, void* const out[30]);
+/* The same over a box of packed cells (x0, x1, y0, y1, z0, z1): out[k] holds
+ * (x1-x0)*(y1-y0)*(z1-z0) values, C order. */
+int rfdtd_read_coefficient_box(rfdtd_session* s, int32_t member, const int32_t box[6], void* const out[30]);
 
 /* Bandwidth probe (no counterpart in the reference): copy `bytes` between two device buffers `iters`
  * times with a plain streaming kernel and report read+write GB/s and ms per copy -- the measured HBM

@@ -56,6 +56,7 @@ EXPORTS = (
     "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
     "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
     "rfdtd_create_assembled", "rfdtd_read_coefficients", "rfdtd_create_batch_assembled",
+    "rfdtd_read_coefficient_box",
 )
 
 
@@ -178,6 +179,7 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_create_assembled.argtypes = [ctypes.POINTER(vp), vp, i32, u32]
     L.rfdtd_read_coefficients.argtypes = [vp, i32, vp]
     L.rfdtd_create_batch_assembled.argtypes = [ctypes.POINTER(vp), i32, vp, vp, u32]
+    L.rfdtd_read_coefficient_box.argtypes = [vp, i32, vp, vp]
     _lib = L
     return L
 

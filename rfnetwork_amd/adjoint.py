@@ -195,6 +195,21 @@ def _use_history(mode, boxes, Nt, dtype):
     return history_bytes(boxes, Nt, dtype) < 0.5 * device_memory()[0]
 
 
+def _assemble_on_device(model, dtype):
+    """Sessions of the gradient drivers assemble the coefficients on the GPU (no host packing or
+    copy of the 30 arrays) when the model's coefficient dtype is the session's."""
+    return np.dtype(model.coef_dtype) == np.dtype(dtype)
+
+
+def _box_edges(model, gbox, box_coefficients, dtype):
+    """The chain rule's inputs over gbox: each E component's edge permittivity (as the host
+    assembly forms it) and the session's own packed coefficients there (read back from the GPU)."""
+    from .design import edge_values
+
+    eps, _ = edge_values(model, gbox, model.eps, model.sigma, dtype)
+    return eps, box_coefficients
+
+
 def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=200, dtype=np.float32,
                    flags=0, return_session_timing=False, tune=True, mode="auto"):
     """Forward solve of the model on the GPU, then the adjoint pass.
@@ -203,10 +218,16 @@ def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=
     pass), "checkpoint" saves the fields every checkpoint_every steps and recomputes between them,
     "auto" records the history when it takes less than half the free device memory.
     Returns (J, dJ/deps_r over cbox, S_ba).  The model must be meshed and excited."""
-    coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype)
     gbox = gpu_box_for_cells(model, cbox)
+    if _assemble_on_device(model, dtype):
+        coeff = None
+        _, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype, coefficients=False)
+        session = Session.assembled(model, dtype, flags)
+    else:
+        coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype)
+        session = Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags)
     history = _use_history(mode, [gbox], Nt, dtype)
-    with Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags) as s:
+    with session as s:
         if tune:
             s.autotune()
         if history:
@@ -223,7 +244,8 @@ def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=
         grad_cb = s.gradient(gbox, [row.astype(dtype) for row in g])
         t_adj = s.timing()[0]
         adj_launch = s.adjoint_launch()
-    grad = cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox)
+        edges = None if coeff is not None else _box_edges(model, gbox, s.coefficients(box=gbox), dtype)
+    grad = cb_to_eps_r_gradient(model, coeff, grad_cb, gbox, cbox, edges=edges)
     if return_session_timing:
         return J, grad, S, dict(forward_ms=t_fwd, adjoint_ms=t_adj, mode="history" if history else "checkpoint",
                                 adjoint_launch=adj_launch)
@@ -240,9 +262,10 @@ def eps_r_gradients_batch(jobs, b_port=2, a_port=1, checkpoint_every=0, dtype=np
     [(J, dJ/deps_r over cbox, S_ba)] per job (and a dict of timings and the interval)."""
     from .batch import Batch
 
+    device = all(_assemble_on_device(model, dtype) for model, _, _ in jobs)
     grids, packed, Nt_all = [], [], None
     for model, cbox, _ in jobs:
-        coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype)
+        coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype, coefficients=not device)
         if Nt_all is None:
             Nt_all = Nt
         elif Nt != Nt_all:
@@ -252,7 +275,9 @@ def eps_r_gradients_batch(jobs, b_port=2, a_port=1, checkpoint_every=0, dtype=np
     info = {}
     history = _use_history(mode, [p[2] for p in packed], Nt_all, dtype)
     info["mode"] = "history" if history else "checkpoint"
-    with Batch(grids, dtype=dtype, flags=flags) as b:
+    batch = Batch.assembled([j[0] for j in jobs], dtype=dtype, flags=flags) if device else \
+        Batch(grids, dtype=dtype, flags=flags)
+    with batch as b:
         if tune:
             info["launch"] = b.autotune()
         if history:
@@ -278,9 +303,11 @@ def eps_r_gradients_batch(jobs, b_port=2, a_port=1, checkpoint_every=0, dtype=np
             requests.append((k, gbox, [row.astype(dtype) for row in g]))
         grads = b.gradients(requests)
         info["adjoint_ms"] = b.timing()[0]
+        edges = [None if coeff is not None else _box_edges(model, gbox, b.coefficients(k, box=gbox), dtype)
+                 for k, ((model, _, _), (coeff, _, gbox)) in enumerate(zip(jobs, packed))]
     out = []
-    for (model, cbox, _), (coeff, _, gbox), (J, S), g in zip(jobs, packed, objectives, grads):
-        out.append((J, cb_to_eps_r_gradient(model, coeff, g, gbox, cbox), S))
+    for (model, cbox, _), (coeff, _, gbox), (J, S), g, ed in zip(jobs, packed, objectives, grads, edges):
+        out.append((J, cb_to_eps_r_gradient(model, coeff, g, gbox, cbox, edges=ed), S))
     return (out, info) if return_info else out
 
 

@@ -164,3 +164,15 @@ def read_coefficients(L, handle, n_cells, dtype, member=0):
     ptrs = _lib.pointer_array([out[k] for k in _lib.COEFF_NAMES])
     check(L.rfdtd_read_coefficients(handle, int(member), ctypes.cast(ptrs, ctypes.c_void_p)), "rfdtd_read_coefficients")
     return out
+
+
+def read_coefficient_box(L, handle, box, dtype, member=0):
+    """The member's 30 packed coefficient arrays over box (x0, x1, y0, y1, z0, z1, packed cells)."""
+    x0, x1, y0, y1, z0, z1 = (int(v) for v in box)
+    shape = (x1 - x0, y1 - y0, z1 - z0)
+    out = {k: np.empty(shape, dtype=dtype) for k in _lib.COEFF_NAMES}
+    ptrs = _lib.pointer_array([out[k] for k in _lib.COEFF_NAMES])
+    b = (ctypes.c_int32 * 6)(x0, x1, y0, y1, z0, z1)
+    check(L.rfdtd_read_coefficient_box(handle, int(member), ctypes.cast(b, ctypes.c_void_p), ctypes.cast(ptrs, ctypes.c_void_p)),
+          "rfdtd_read_coefficient_box")
+    return out

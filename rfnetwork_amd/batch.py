@@ -80,10 +80,13 @@ class Batch:
         self._keep = {}
         return self
 
-    def coefficients(self, member):
-        """Member `member`'s 30 packed coefficient arrays read back from the GPU."""
+    def coefficients(self, member, box=None):
+        """Member `member`'s 30 packed coefficient arrays read back from the GPU, whole or over box =
+        (x0, x1, y0, y1, z0, z1) packed cells."""
         from . import assembly
 
+        if box is not None:
+            return assembly.read_coefficient_box(self._L, self._h, box, self.dtype, member)
         Nx, Ny, Nz, _ = self.grids[member]
         return assembly.read_coefficients(self._L, self._h, (Nx, Ny, Nz), self.dtype, member)
 

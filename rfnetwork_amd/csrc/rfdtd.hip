@@ -4090,39 +4090,89 @@ int rfdtd_create_batch_assembled(rfdtd_session** out, int32_t n_grids, const rfd
     });
 }
 
+// The member's packed coefficients over box b (x0, x1, y0, y1, z0, z1) into out[k] (box-sized, C
+// order; null skips one), with the x = Nx-1 plane rule applied as the kernels read the arrays.
+void read_coefficient_box(Session& s, int member, const int32_t b[6], void* const out[30]) {
+    if (member < 0 || member >= (int)s.members.size()) RF_THROW(RFDTD_EINVAL, "member index out of range");
+    const Member& M = s.members[member];
+    if (b[0] < 0 || b[0] >= b[1] || b[1] > M.Nx || b[2] < 0 || b[2] >= b[3] || b[3] > M.Ny || b[4] < 0 ||
+        b[4] >= b[5] || b[5] > M.Nz)
+        RF_THROW(RFDTD_EINVAL, "box outside the grid");
+    HIP_OK(hipDeviceSynchronize());
+    const size_t esz = s.esz;
+    const int64_t cells = M.cells;
+    const int bx = b[1] - b[0], by = b[3] - b[2], bz = b[5] - b[4];
+    auto cell_of = [&](int i, int j, int k) { return ((int64_t)(b[0] + i) * M.Ny + (b[2] + j)) * M.Nz + (b[4] + k); };
+    auto put = [&](int k, int64_t q, const char* src) {
+        char* v = (char*)out[k] + (size_t)q * esz;
+        const int x = (int)(q / ((int64_t)by * bz)) + b[0];
+        if (zero_last_plane(k) && x == M.Nx - 1) std::memset(v, 0, esz);
+        else std::memcpy(v, src, esz);
+    };
+    if (M.coef_kind == 0) {
+        std::vector<char> row((size_t)bz * esz);
+        for (int k = 0; k < 30; ++k) {
+            if (!out[k]) continue;
+            for (int i = 0; i < bx; ++i)
+                for (int j = 0; j < by; ++j) {
+                    HIP_OK(hipMemcpy(row.data(), (const char*)M.raw[k] + (size_t)cell_of(i, j, 0) * esz, row.size(),
+                                     hipMemcpyDeviceToHost));
+                    for (int z = 0; z < bz; ++z) put(k, ((int64_t)i * by + j) * bz + z, &row[(size_t)z * esz]);
+                }
+        }
+        return;
+    }
+    std::vector<uint32_t> ei(cells), hi(cells);
+    HIP_OK(hipMemcpy(ei.data(), M.eidx, (size_t)cells * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(hi.data(), M.hidx, (size_t)cells * 4, hipMemcpyDeviceToHost));
+    uint32_t ne = 0, nh = 0;
+    for (int i = 0; i < bx; ++i)
+        for (int j = 0; j < by; ++j)
+            for (int z = 0; z < bz; ++z) {
+                const int64_t c = cell_of(i, j, z);
+                ne = std::max(ne, ei[c] + 1);
+                nh = std::max(nh, hi[c] + 1);
+            }
+    std::vector<char> et((size_t)ne * 12 * esz), ht((size_t)nh * 20 * esz);
+    HIP_OK(hipMemcpy(et.data(), M.etab, et.size(), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ht.data(), M.htab, ht.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < bx; ++i)
+        for (int j = 0; j < by; ++j)
+            for (int z = 0; z < bz; ++z) {
+                const int64_t c = cell_of(i, j, z), q = ((int64_t)i * by + j) * bz + z;
+                for (int k = 0; k < 30; ++k) {
+                    if (!out[k]) continue;
+                    put(k, q, k < 12 ? &et[((size_t)ei[c] * 12 + k) * esz] : &ht[((size_t)hi[c] * 20 + (k - 12)) * esz]);
+                }
+            }
+}
+
 int rfdtd_read_coefficients(rfdtd_session* sp, int32_t member, void* const out[30]) {
     return guarded([&] {
         if (!sp || !out) RF_THROW(RFDTD_EINVAL, "null argument");
         Session& s = *(Session*)sp;
         if (member < 0 || member >= (int)s.members.size()) RF_THROW(RFDTD_EINVAL, "member index out of range");
         const Member& M = s.members[member];
-        HIP_OK(hipDeviceSynchronize());
-        const size_t esz = s.esz;
-        const int64_t cells = M.cells, plane = (int64_t)M.Ny * M.Nz;
-        if (M.coef_kind == 0) {
-            for (int k = 0; k < 30; ++k)
-                if (out[k]) HIP_OK(hipMemcpy(out[k], M.raw[k], (size_t)cells * esz, hipMemcpyDeviceToHost));
-        } else {
-            std::vector<uint32_t> ei(cells), hi(cells);
-            HIP_OK(hipMemcpy(ei.data(), M.eidx, (size_t)cells * 4, hipMemcpyDeviceToHost));
-            HIP_OK(hipMemcpy(hi.data(), M.hidx, (size_t)cells * 4, hipMemcpyDeviceToHost));
-            const uint32_t ne = *std::max_element(ei.begin(), ei.end()) + 1;
-            const uint32_t nh = *std::max_element(hi.begin(), hi.end()) + 1;
-            std::vector<char> et((size_t)ne * 12 * esz), ht((size_t)nh * 20 * esz);
-            HIP_OK(hipMemcpy(et.data(), M.etab, et.size(), hipMemcpyDeviceToHost));
-            HIP_OK(hipMemcpy(ht.data(), M.htab, ht.size(), hipMemcpyDeviceToHost));
+        const int32_t b[6] = {0, M.Nx, 0, M.Ny, 0, M.Nz};
+        if (M.coef_kind == 0) {  // whole arrays: one copy each
+            HIP_OK(hipDeviceSynchronize());
+            const size_t esz = s.esz;
+            const int64_t cells = M.cells, plane = (int64_t)M.Ny * M.Nz;
             for (int k = 0; k < 30; ++k) {
                 if (!out[k]) continue;
-                char* dst = (char*)out[k];
-                for (int64_t c = 0; c < cells; ++c) {
-                    const char* src = k < 12 ? &et[((size_t)ei[c] * 12 + k) * esz] : &ht[((size_t)hi[c] * 20 + (k - 12)) * esz];
-                    std::memcpy(dst + (size_t)c * esz, src, esz);
-                }
+                HIP_OK(hipMemcpy(out[k], M.raw[k], (size_t)cells * esz, hipMemcpyDeviceToHost));
+                if (zero_last_plane(k)) std::memset((char*)out[k] + (size_t)(cells - plane) * esz, 0, (size_t)plane * esz);
             }
+            return;
         }
-        // the x = Nx-1 plane rule, as every kernel reads the arrays
-        for (int k = 0; k < 30; ++k)
-            if (out[k] && zero_last_plane(k)) std::memset((char*)out[k] + (size_t)(cells - plane) * esz, 0, (size_t)plane * esz);
+        read_coefficient_box(s, member, b, out);
+    });
+}
+
+int rfdtd_read_coefficient_box(rfdtd_session* sp, int32_t member, const int32_t box[6], void* const out[30]) {
+    return guarded([&] {
+        if (!sp || !out || !box) RF_THROW(RFDTD_EINVAL, "null argument");
+        read_coefficient_box(*(Session*)sp, member, box, out);
     });
 }
 

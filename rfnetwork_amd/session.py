@@ -58,11 +58,14 @@ class Session:
         self.Nt = None
         return self
 
-    def coefficients(self):
+    def coefficients(self, box=None):
         """The session's 30 packed coefficient arrays read back from the GPU (from its table or raw
-        arrays; the x = Nx-1 plane of arrays 4..17 reads zero, as the kernels see it)."""
+        arrays; the x = Nx-1 plane of arrays 4..17 reads zero, as the kernels see it), whole or over
+        box = (x0, x1, y0, y1, z0, z1) packed cells."""
         from . import assembly
 
+        if box is not None:
+            return assembly.read_coefficient_box(self._L, self._h, box, self.dtype)
         return assembly.read_coefficients(self._L, self._h, (self.Nx, self.Ny, self.Nz), self.dtype)
 
     def set_monitor_streaming(self, ring_samples):
