@@ -11,7 +11,6 @@ port is driven in its own excitation (the others terminated in their resistance)
 columns are returned.  The reference runs one excitation per ``solve()`` call
 (rfnetwork/solver.py:1410; tests/test_solver/test_coupled_stripline.py:96-109 re-solves per mode).
 """
-import copy
 import ctypes
 
 import numpy as np

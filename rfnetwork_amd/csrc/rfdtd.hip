@@ -3584,7 +3584,6 @@ void set_monitors(Session& s, int inst, const rfdtd_monitor* mons, int n, int Nt
         h.N2 = d.axis == 2 ? M.Ny : M.Nz;
         h.Nm = Nt / d.n_step + 1;
         h.active = pos >= 0 && pos < along;
-        const size_t plane = (size_t)h.N1 * h.N2;
         if (s.stream_ring > 0 && !d.n_frequencies && h.active) h.ring = std::min(h.Nm, s.stream_ring);
         const size_t vbytes = monitor_bytes(s, h);
         HIP_OK(hipMalloc(&h.d_values, vbytes));

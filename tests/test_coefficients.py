@@ -8,7 +8,7 @@ termination (:1086-1236), PEC conductors (:817-868) and the Gaussian sources (:1
 import numpy as np
 
 from rfnetwork_amd import FDTD_Solver, workloads
-from rfnetwork_amd.geometry import Box, Rectangle
+from rfnetwork_amd.geometry import Box
 from rfnetwork_amd.units import const, conv
 
 IN_PER_M = 39.3701

@@ -1,7 +1,5 @@
 """The engine's adjoint (rfdtd_gradient) against an unrolled torch autograd of the same time loop,
 and the eps_r gradient against central finite differences of the float64 oracle."""
-import copy
-
 import numpy as np
 import pytest
 import torch

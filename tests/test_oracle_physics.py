@@ -16,7 +16,6 @@ import copy
 from itertools import permutations
 
 import numpy as np
-import pytest
 
 from oracle import cpu_solver
 from rfnetwork_amd import FDTD_Solver, utils, workloads
