@@ -288,16 +288,21 @@ def solve_ports(jobs, frequency, dtype=np.float32, flags=0, return_batch=False):
     monitor values per excitation (list of dicts)."""
     grids, inputs = [], []
     Nt_all = None
+    # the coefficients are assembled on the GPU when every model's coefficient dtype is the batch's
+    device = all(np.dtype(model.coef_dtype) == np.dtype(dtype) for model, _ in jobs)
     for model, ports in jobs:
         exc, Nt = pack_excitations(model, ports, dtype)
         if Nt_all is None:
             Nt_all = Nt
         elif Nt != Nt_all:
             raise ValueError("all batched models must run the same number of steps")
-        coeff = model.coefficient_arrays(gpu=True, dtype=dtype)
+        coeff = None if device else model.coefficient_arrays(gpu=True, dtype=dtype)
         grids.append((coeff, model.Nx, model.Ny, model.Nz, len(ports)))
         inputs.append(exc)
-    batch = Batch(grids, dtype=dtype, flags=flags)
+    if device:
+        batch = Batch.assembled([m for m, _ in jobs], [len(p) for _, p in jobs], dtype=dtype, flags=flags)
+    else:
+        batch = Batch(grids, dtype=dtype, flags=flags)
     k = 0
     for exc in inputs:
         for _, probes, monitors in exc:

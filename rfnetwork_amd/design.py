@@ -137,12 +137,11 @@ class DesignLoop:
             raise ValueError("the model's coefficient dtype must be the session's element type")
         self.model, self.cbox, self.frequency = model, tuple(int(v) for v in cbox), frequency
         self.b_port, self.a_port, self.dtype = b_port, a_port, np.dtype(dtype)
-        coeff, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype)
+        _, probes, _, _, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype, coefficients=False)
         self.Nt = Nt
         self.gbox = adjoint.gpu_box_for_cells(model, cbox)
         self.history = adjoint._use_history(mode, [self.gbox], Nt, dtype)
-        self.session = Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags)
-        del coeff
+        self.session = Session.assembled(model, dtype, flags)  # coefficients assembled on the GPU
         if tune:
             self.session.autotune()
         self.checkpoint_every = checkpoint_every
