@@ -326,7 +326,9 @@ int rfdtd_create_assembled(rfdtd_session** out, const rfdtd_assembly* a, int32_t
 int rfdtd_create_batch_assembled(rfdtd_session** out, int32_t n_grids, const rfdtd_assembly* a,
                                  const int32_t* excitations, uint32_t flags);
 /* The member's 30 packed coefficient arrays (the x = Nx-1 plane of arrays 4..17 reads zero), from
- * its table or raw arrays, into host arrays of the element type (Nx*Ny*Nz each; null skips one). */
+ * its table or raw arrays, into host arrays of the element type (Nx*Ny*Nz each; null skips one).
+ * No counterpart in the reference (its coefficients never leave the host): the tests compare the
+ * device assembly with the host's through it, and the gradient's chain rule reads a box of them. */
 int rfdtd_read_coefficients(rfdtd_session* s, int32_t member, void* const out[30]);
 /* The same over a box of packed cells (x0, x1, y0, y1, z0, z1): out[k] holds
  * (x1-x0)*(y1-y0)*(z1-z0) values, C order. */
