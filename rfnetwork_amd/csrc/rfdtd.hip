@@ -974,7 +974,7 @@ __global__ void __launch_bounds__((TY + 2) * 64) k_step_fused(const InstDev<T>* 
 //   mu  = lambda_h + g_H (both splits of each probed H component)           k_adj_inject (H)
 //   nu  = Ca nu_old + S^T G^T mu                                             k_adj_e
 //   nu += g_E (both splits)                                                  k_adj_inject (E)
-//   dJ/dCb += nu * D(H^n)   (design box, forward H^n recomputed)             k_adj_grad
+//   dJ/dCb += nu * D(H^n)   (design box, forward H^n saved or recomputed)    k_adj_h's extra workgroups
 //   lambda_h = Da mu + S^T D^T (Cb nu)                                       k_adj_h
 // where S^T copies a total's adjoint into both of its splits.  Split updates that the forward step
 // skips (upper edges of the GPU packing) pass their adjoint through unchanged.
@@ -1338,24 +1338,6 @@ __global__ void __launch_bounds__(1024) k_adj_e(const InstDev<T>* __restrict__ i
     }
 }
 
-template <typename T, bool PF>
-__global__ void __launch_bounds__(1024) k_adj_h(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
-                                                int nmem, int total, int n) {
-    __shared__ int s_member;
-    Tile t;
-    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
-    if (!I) return;
-    const AdjFold<T> P{I->hbits, (const ProbeGroup<T>*)I->hgroups, n >= 0 ? I->nhgroups : 0, n};
-    if (I->coef_kind == 1) {
-        const CoefTable<T> C{I->eidx, I->hidx, I->etab, I->htab};
-        if (PF) adj_h_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
-        else adj_h_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
-    } else {
-        if (PF) adj_h_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
-        else adj_h_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
-    }
-}
-
 // Totals from splits after a checkpoint restore (the forward computes them the same way).
 template <typename T>
 __global__ void __launch_bounds__(256) k_totals(const InstDev<T>* __restrict__ insts, const int64_t* __restrict__ blk0,
@@ -1458,20 +1440,21 @@ __global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ i
 }
 
 // dJ/dCb of the six E splits in every box: += nu * (the curl difference the forward update used),
-// with the forward H of this step from slot `j`.
+// with the forward H of this step from slot `j`.  `block` is the workgroup's index among the
+// gradient's workgroups (blockDim.x threads each; blk0 counts them per box).
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
-                                                  const int64_t* __restrict__ blk0, int n, int j) {
+__device__ __forceinline__ void adj_grad_block(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
+                                               const int64_t* __restrict__ blk0, int n, int j, int block) {
     using G = GLOBAL_T(T);
     using GD = GLOBAL_T(double);
-    const int k = box_of(blk0, n, blockIdx.x);
+    const int k = box_of(blk0, n, block);
     const BoxDev<T> D = boxes[k];
     const InstDev<T>& A = insts[D.inst];
     const Box B = D.B;
     const uint32_t hy_n = B.ny + 1, hz_n = B.nz + 1;
     const uint32_t total_h = (uint32_t)(B.nx + 1) * hy_n * hz_n;
     const uint32_t nbox = (uint32_t)B.nx * B.ny * B.nz;
-    const uint32_t q = (uint32_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    const uint32_t q = (uint32_t)(block - blk0[k]) * blockDim.x + threadIdx.x;
     if (q >= nbox) return;
     const G* hx = (const G*)D.slots + (size_t)j * D.slot_elems;
     const G* hy = hx + total_h;
@@ -1495,6 +1478,35 @@ __global__ void __launch_bounds__(256) k_adj_grad(const InstDev<T>* __restrict__
     g[3 * (size_t)nbox] += (double)((const G*)F.ey_x)[i] * (double)d_yx;
     g[4 * (size_t)nbox] += (double)((const G*)F.ez_x)[i] * (double)d_zx;
     g[5 * (size_t)nbox] += (double)((const G*)F.ez_y)[i] * (double)d_zy;
+}
+
+template <typename T, bool PF>
+__global__ void __launch_bounds__(1024) k_adj_h(const InstDev<T>* __restrict__ insts, const MemberDev* __restrict__ mem,
+                                                int nmem, int total, int n, const BoxDev<T>* __restrict__ gboxes = nullptr,
+                                                const int64_t* __restrict__ gblk0 = nullptr, int ngb = 0, int gj = 0,
+                                                const InstDev<T>* __restrict__ ginsts = nullptr) {
+    // workgroups past the column tiles (gboxes given) accumulate the design-box gradient of this
+    // step in the same launch: they read the E adjoint and the saved H, which this pass does not write
+    if (gboxes) {
+        const int L = xcd_logical(blockIdx.x, gridDim.x);
+        if (L >= total) {
+            if (L - total < (int)gblk0[ngb]) adj_grad_block<T>(ginsts, gboxes, gblk0, ngb, gj, L - total);
+            return;
+        }
+    }
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
+    if (!I) return;
+    const AdjFold<T> P{I->hbits, (const ProbeGroup<T>*)I->hgroups, n >= 0 ? I->nhgroups : 0, n};
+    if (I->coef_kind == 1) {
+        const CoefTable<T> C{I->eidx, I->hidx, I->etab, I->htab};
+        if (PF) adj_h_column_pf<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
+        else adj_h_column<T, CoefTable<T>>(I->F, C, I->Nx, I->Ny, I->Nz, t, P);
+    } else {
+        if (PF) adj_h_column_pf<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+        else adj_h_column<T, CoefRaw<T>>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2823,9 +2835,9 @@ void plan_launch(Session& s) {
 }
 
 // Block offsets of a batched flat launch over items of the given sizes (256 threads per block).
-std::vector<int64_t> block_offsets(const std::vector<int64_t>& sizes) {
+std::vector<int64_t> block_offsets(const std::vector<int64_t>& sizes, int64_t block = 256) {
     std::vector<int64_t> off(sizes.size() + 1, 0);
-    for (size_t i = 0; i < sizes.size(); ++i) off[i + 1] = off[i] + (sizes[i] + 255) / 256;
+    for (size_t i = 0; i < sizes.size(); ++i) off[i + 1] = off[i] + (sizes[i] + block - 1) / block;
     if (off.back() > INT32_MAX) RF_THROW(RFDTD_EINVAL, "batched launch too large");
     return off;
 }
@@ -3801,7 +3813,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         hcells[k] = slot_elems / 3;
         gcells[k] = (int64_t)nbox;
     }
-    const std::vector<int64_t> flat0 = block_offsets(cells), h0 = block_offsets(hcells), g0 = block_offsets(gcells);
+    const std::vector<int64_t> flat0 = block_offsets(cells), h0 = block_offsets(hcells);
     auto upload = [&](const void* src, size_t bytes) {
         void* d = buf.alloc(bytes);
         HIP_OK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st));
@@ -3923,7 +3935,10 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     const BoxDev<T>* d_bd = (const BoxDev<T>*)upload(bd.data(), bd.size() * sizeof(BoxDev<T>));
     const int64_t* d_flat0 = (const int64_t*)upload(flat0.data(), flat0.size() * 8);
     const int64_t* d_h0 = (const int64_t*)upload(h0.data(), h0.size() * 8);
-    const int64_t* d_g0 = (const int64_t*)upload(g0.data(), g0.size() * 8);
+    // the gradient's workgroups in the H adjoint launch: offsets in workgroups of its size
+    const std::vector<int64_t> ga0 = block_offsets(gcells, ablock);
+    const int64_t* d_ga0 = (const int64_t*)upload(ga0.data(), ga0.size() * 8);
+    const int gab = (int)ga0.back();
     const int ngE = (int)gE.size(), ngH = (int)gH.size();
     gE.insert(gE.end(), gH.begin(), gH.end());
     aE.insert(aE.end(), aH.begin(), aH.end());
@@ -3931,7 +3946,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     const ProbeGroup<T>* d_ag = aE.empty() ? nullptr : (const ProbeGroup<T>*)upload(aE.data(), aE.size() * sizeof(ProbeGroup<T>));
     HIP_OK(hipStreamSynchronize(st));  // host-side descriptor vectors stay alive; uploads done
     const int grid = (tile0 + 7) & ~7;
-    const int flat = (int)flat0.back(), hblocks = (int)h0.back(), gblocks = (int)g0.back();
+    const int flat = (int)flat0.back(), hblocks = (int)h0.back();
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int64_t n0 = hist_mode ? 0 : (int64_t)seg * K, n1 = hist_mode ? Nt : std::min<int64_t>(Nt, n0 + K);
         if (!hist_mode)
@@ -3956,8 +3971,11 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
             // last step's here); those of E by adj_e itself
             if (ngH && t == Nt - 1) k_adj_inject<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di + n, d_ag + ngE, ngH, (int)t);
             adj_e(apf, ablock, atiles, (int)t);
-            k_adj_grad<T><<<gblocks, 256, 0, st>>>(d_di + n, d_bd, d_g0, n, j);
-            adj_h(apf, ablock, atiles, (int)t - 1);
+            // H adjoint and the box gradient of step t (which reads the E adjoint adj_e just wrote
+            // and the saved H^t, neither written by the H pass) in one launch
+            const int g = (atiles + gab + 7) & ~7;
+            if (apf) k_adj_h<T, true><<<g, ablock, 0, st>>>(d_di + n, d_mda, n, atiles, (int)t - 1, d_bd, d_ga0, n, j, d_di + n);
+            else k_adj_h<T, false><<<g, ablock, 0, st>>>(d_di + n, d_mda, n, atiles, (int)t - 1, d_bd, d_ga0, n, j, d_di + n);
         }
         HIP_OK(hipGetLastError());
         if ((s.flags & RFDTD_VERBOSE) && (seg % std::max(1, nseg / 10) == 0 || seg == 0)) {
