@@ -1410,18 +1410,18 @@ __device__ __forceinline__ int box_of(const int64_t* __restrict__ blk0, int n, i
 }
 
 // Save the forward H totals of every box (plus one cell on each upper side) into slot `j`.  Box
-// and grid indices fit 32 bits (a session holds at most 2^29 cells).
+// and grid indices fit 32 bits (a session holds at most 2^29 cells).  `block` is the workgroup's
+// index among the save's workgroups (blockDim.x threads each; blk0 counts them per box).
 template <typename T>
-__global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
-                                                const int64_t* __restrict__ blk0, int n, int j, const int* base = nullptr) {
+__device__ __forceinline__ void save_h_block(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
+                                             const int64_t* __restrict__ blk0, int n, int j, int block) {
     using G = GLOBAL_T(T);
-    if (base) j += *base;  // history recording inside a captured chunk: slot = step number
-    const int k = box_of(blk0, n, blockIdx.x);
+    const int k = box_of(blk0, n, block);
     const BoxDev<T> D = boxes[k];
     const InstDev<T>& I = insts[D.inst];
     const uint32_t hy_n = D.B.ny + 1, hz_n = D.B.nz + 1;
     const uint32_t total = (uint32_t)(D.B.nx + 1) * hy_n * hz_n;
-    const uint32_t q = (uint32_t)(blockIdx.x - blk0[k]) * blockDim.x + threadIdx.x;
+    const uint32_t q = (uint32_t)(block - blk0[k]) * blockDim.x + threadIdx.x;
     if (q >= total) return;
     const uint32_t lz = q % hz_n, ly = q / hz_n % hy_n, lx = q / (hz_n * hy_n);
     const int x = D.B.x0 + (int)lx, y = D.B.y0 + (int)ly, z = D.B.z0 + (int)lz;
@@ -1437,6 +1437,40 @@ __global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ i
     slot[q] = a;
     slot[total + q] = b;
     slot[2 * total + q] = c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_save_h(const InstDev<T>* __restrict__ insts, const BoxDev<T>* __restrict__ boxes,
+                                                const int64_t* __restrict__ blk0, int n, int j, const int* base = nullptr) {
+    if (base) j += *base;  // history recording inside a captured chunk: slot = step number
+    save_h_block<T>(insts, boxes, blk0, n, j, (int)blockIdx.x);
+}
+
+// The E half step with the design-box H save of the same step in its launch: workgroups past the
+// column tiles save H^n (the H splits, which the E pass only reads) into slot n.
+template <typename T, bool PF, bool NT>
+__global__ void __launch_bounds__(1024) k_update_e_save(const InstDev<T>* __restrict__ insts,
+                                                        const MemberDev* __restrict__ mem, int nmem, int total,
+                                                        const int* base, int local, int io,
+                                                        const BoxDev<T>* __restrict__ boxes,
+                                                        const int64_t* __restrict__ blk0, int nb) {
+    {
+        const int L = xcd_logical(blockIdx.x, gridDim.x);
+        if (L >= total) {
+            if (L - total < (int)blk0[nb]) save_h_block<T>(insts, boxes, blk0, nb, step_of(base, local), L - total);
+            return;
+        }
+    }
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t);
+    if (!I) return;
+    const int ng = io ? I->negroups : 0;
+    const ProbeFold<T> P{I->ebits, (const ProbeGroup<T>*)I->egroups, ng, ng ? step_of(base, local) : 0};
+    if (I->coef_kind == 1)
+        e_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t, P);
+    else
+        e_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
 }
 
 // dJ/dCb of the six E splits in every box: += nu * (the curl difference the forward update used),
@@ -2240,6 +2274,8 @@ struct Session {
     void* d_hist_boxes = nullptr;
     int64_t* d_hist_h0 = nullptr;
     int hist_blocks = 0;
+    int64_t* d_hist_hb = nullptr;  // the save's workgroup offsets in workgroups of the E launch's size
+    int hist_blocks_e = 0;
     std::vector<int64_t> flat0; // block offsets of a flat launch over all instances' cells
     int64_t* d_flat0 = nullptr;
 
@@ -2314,7 +2350,7 @@ Session::~Session() {
     for (auto& h : hist)
         if (h.d) (void)hipFree(h.d);
     for (void* p : {(void*)d_insts, (void*)d_members, d_groups, (void*)d_mons, (void*)d_step_base, d_stage, (void*)d_rows,
-                    d_insts2, (void*)d_flat0, d_hist_boxes, (void*)d_hist_h0})
+                    d_insts2, (void*)d_flat0, d_hist_boxes, (void*)d_hist_h0, (void*)d_hist_hb})
         if (p) (void)hipFree(p);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
     for (hipEvent_t e : {seg_ev, cpy_ev[0], cpy_ev[1]})
@@ -2955,11 +2991,12 @@ void refresh(Session& s) {
     s.flat0 = block_offsets(cells);
     HIP_OK(hipMalloc(&s.d_flat0, s.flat0.size() * sizeof(int64_t)));
     HIP_OK(hipMemcpy(s.d_flat0, s.flat0.data(), s.flat0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-    for (void** p : {&s.d_hist_boxes, (void**)&s.d_hist_h0}) {
+    for (void** p : {&s.d_hist_boxes, (void**)&s.d_hist_h0, (void**)&s.d_hist_hb}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
     s.hist_blocks = 0;
+    s.hist_blocks_e = 0;
     if (!s.hist.empty()) {
         std::vector<BoxDev<T>> hb(s.hist.size());
         std::vector<int64_t> hc;
@@ -2978,6 +3015,10 @@ void refresh(Session& s) {
         HIP_OK(hipMemcpy(s.d_hist_boxes, hb.data(), hb.size() * sizeof(BoxDev<T>), hipMemcpyHostToDevice));
         HIP_OK(hipMalloc(&s.d_hist_h0, h0.size() * sizeof(int64_t)));
         HIP_OK(hipMemcpy(s.d_hist_h0, h0.data(), h0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        const std::vector<int64_t> he = block_offsets(hc, s.block);
+        s.hist_blocks_e = (int)he.back();
+        HIP_OK(hipMalloc(&s.d_hist_hb, he.size() * sizeof(int64_t)));
+        HIP_OK(hipMemcpy(s.d_hist_hb, he.data(), he.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     }
 
     // probe groups of all instances: E groups, then H groups
@@ -3066,7 +3107,10 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     const InstDev<T>* di = desc[s.parity];
     const int nmem = (int)s.members.size();
     const bool io = !s.tuning;
-    if (io && s.hist_blocks)  // H^n of the design boxes, before this step's E half step
+    // H^n of the design boxes before this step's E half step: in the E launch for the separate
+    // kernels, a launch of its own before the fused pass
+    const bool save_in_e = io && s.hist_blocks && !s.fused;
+    if (io && s.hist_blocks && s.fused)
         k_save_h<T><<<s.hist_blocks, 256, 0, st>>>(di, (const BoxDev<T>*)s.d_hist_boxes, s.d_hist_h0, (int)s.hist.size(),
                                                   local, base);
     if (s.fused) {
@@ -3092,7 +3136,20 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
         // E and H probes (sources and records) are applied inside the update kernels
         const int pio = io ? 1 : 0;
         if (ev) HIP_OK(hipEventRecord(ev[0], st));
-        if (s.nontemporal)
+        if (save_in_e) {
+            const int gs = (s.total_tiles + s.hist_blocks_e + 7) & ~7;
+            const BoxDev<T>* hb = (const BoxDev<T>*)s.d_hist_boxes;
+            const int nb = (int)s.hist.size();
+            if (s.nontemporal)
+                k_update_e_save<T, true, true><<<gs, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local,
+                                                                      pio, hb, s.d_hist_hb, nb);
+            else if (s.pipelined)
+                k_update_e_save<T, true, false><<<gs, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local,
+                                                                       pio, hb, s.d_hist_hb, nb);
+            else
+                k_update_e_save<T, false, false><<<gs, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base,
+                                                                        local, pio, hb, s.d_hist_hb, nb);
+        } else if (s.nontemporal)
             k_update_e<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         else if (s.pipelined)
             k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
