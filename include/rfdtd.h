@@ -40,7 +40,8 @@ extern "C" {
 #define RFDTD_QUIET 0x10u         /* no progress output */
 #define RFDTD_KERNEL_TIMING 0x20u /* launch eagerly and time the E/H update kernels with HIP events */
 #define RFDTD_VERBOSE 0x40u       /* progress lines on stderr during long gradient passes */
-#define RFDTD_HOST_TABLE 0x80u    /* build the coefficient table on the host (default: on the device) */
+#define RFDTD_HOST_TABLE 0x80u    /* sessions from host arrays: build the coefficient table on the host
+                                     (default: on the device; assembled sessions always do) */
 
 /*
  * Coefficient order: the keys of the reference's coefficient dictionary (solver.py:1473-1518):
