@@ -62,3 +62,18 @@ def test_edge_eps_equals_host_arrays(dtype):
         np.testing.assert_array_equal(_bits(got), _bits(full), err_msg=f)
         blk = (slice(3, 9), 4, slice(2, 7))
         np.testing.assert_array_equal(_bits(model._edge_eps(f, blk)).ravel(), _bits(full[blk]).ravel())
+
+
+def test_description_checks():
+    """The description refuses a model whose coefficient dtype is not the session's (the arrays
+    would be formed in the wrong type), and Batch.assembled needs one excitation count per model;
+    both before any GPU call."""
+    from rfnetwork_amd.batch import Batch
+
+    m = workloads.tiny_model(Nt=10)
+    with pytest.raises(ValueError, match="coef_dtype"):
+        assembly.Description(m, np.float64)
+    with pytest.raises(ValueError, match="one excitation count per model"):
+        Batch.assembled([m, m], excitations=[1])
+    D = assembly.Description(m, np.float32)
+    assert D.struct.Nx == m.Nx and D.struct.n_pml == 6 and D.struct.dt == float(m.dt)
