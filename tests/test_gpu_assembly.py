@@ -169,3 +169,32 @@ def test_assembled_batch(hip_ready):
     for fa, fb in zip(runs[0][1], runs[1][1]):
         for name in fa:
             np.testing.assert_array_equal(fa[name], fb[name], err_msg=name)
+
+
+def test_assembly_promotion_paths(hip_ready):
+    """The arithmetic types the host's numpy picks for the overrides: a conductor whose sigma is a
+    float64 scalar (sigma dt in float64) besides one with a Python float (in float32), and edge
+    corrections with a Python-float factor (float32 products) besides the default float64 one."""
+    from rfnetwork_amd import assembly
+    from rfnetwork_amd.geometry import Box
+
+    print("[case] assembly promotion paths", flush=True)
+    for coefficients in ("host", "device"):
+        m = workloads.tiny_model(Nt=20, nx=20, ny=16, nz=12, coefficients="host")
+        d = 1e-4 * 39.3701
+        m.add_conductor(Box((12 * d, 9 * d, 2 * d), (15 * d, 12 * d, 5 * d)), sigma=np.float64(5.8e7))
+        m.add_conductor(Box((16 * d, 3 * d, 6 * d), (18 * d, 6 * d, 8 * d)), sigma=3.5e6)
+        m.generate_mesh(courant=0.99, coefficients=coefficients)
+        m.edge_correction((12 * d, 9 * d, 5 * d), (15 * d, 9 * d, 5 * d), "y-", CFe=1.25)
+        m.edge_correction((12 * d, 12 * d, 5 * d), (15 * d, 12 * d, 5 * d), "y+")
+        if coefficients == "host":
+            ref = _host_arrays(m, np.float32)
+            D = assembly.Description(m, np.float32)
+            fixes = [D._keep[2][k] for k in range(D.struct.n_fix)]
+            assert {f.wide for f in fixes if f.kind == 0} == {0, 1}
+            scales = [D._keep[3][k] for k in range(D.n_scales)]
+            assert {s.wide for s in scales} == {0, 1}
+        with Session.assembled(m) as s:
+            got = s.coefficients()
+        for k in ref:
+            np.testing.assert_array_equal(_bits(got[k]), _bits(ref[k]), err_msg=f"{coefficients} {k}")
