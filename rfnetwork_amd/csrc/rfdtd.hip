@@ -305,8 +305,10 @@ struct ProbeGroup {
 __device__ __forceinline__ int step_of(const int* base, int local) { return base ? (*base + local) : local; }
 
 // Sources and records of the probe groups at one cell (groups sorted by cell, stably, so groups
-// keep list order) applied to the six split values of E, or of H, just computed there: the
-// operations k_probes performs, in its order, with the source samples from the pristine series.
+// keep list order) applied to the six split values of E, or of H, just computed there, with the
+// source samples from the pristine series: per probe in list order, a source adds its sample to
+// both splits (s1 += v, s2 += v, total = s1 + s2) and every probe records the total
+// (rfnetwork/core/src/solver.cpp:963-976, :1104-1116).
 template <typename T>
 __device__ __forceinline__ void inject_cell(const ProbeGroup<T>* __restrict__ g, int ng, int64_t cell, int n,
                                             bool record, T& xy, T& xz, T& yz, T& yx, T& zx, T& zy) {
@@ -574,42 +576,6 @@ __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict_
         h_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
 }
 
-
-// recompute = true replays a step from the pristine source samples and records nothing.
-template <typename T>
-__global__ void k_probes(const InstDev<T>* __restrict__ insts, const ProbeGroup<T>* __restrict__ groups,
-                         int ngroups, const int* base, int local, int recompute) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    const int n = step_of(base, local);
-    const ProbeGroup<T> G = groups[g];
-    const Fields<T>& F = insts[G.inst].F;
-    T *tot, *s1, *s2;
-    switch (G.field) {
-        case EX: tot = F.ex; s1 = F.ex_y; s2 = F.ex_z; break;
-        case EY: tot = F.ey; s1 = F.ey_z; s2 = F.ey_x; break;
-        case EZ: tot = F.ez; s1 = F.ez_x; s2 = F.ez_y; break;
-        case HX: tot = F.hx; s1 = F.hx_y; s2 = F.hx_z; break;
-        case HY: tot = F.hy; s1 = F.hy_z; s2 = F.hy_x; break;
-        default: tot = F.hz; s1 = F.hz_x; s2 = F.hz_y; break;
-    }
-    T a = s1[G.cell], b = s2[G.cell], s = a + b;  // the total, as the update kernels form it
-    T* row = G.series + (size_t)n * G.nprobes;
-    const T* row_in = recompute ? G.series_in + (size_t)n * G.nprobes : row;
-    for (int k = 0; k < G.count; ++k) {
-        const int pid = G.order[k];
-        const T v = row_in[pid];
-        if (G.is_src[pid]) {
-            a = a + v;
-            b = b + v;
-            s = a + b;
-        }
-        if (!recompute) row[pid] = s;
-    }
-    s1[G.cell] = a;
-    s2[G.cell] = b;
-    tot[G.cell] = s;
-}
 
 // ------------------------------------------------------------------------------------------------
 // fused step: E and H half steps of one time step in one pass, from one field buffer into the other
@@ -3823,6 +3789,14 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
         const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
         di[k] = make_instdev<T>(M, d_re[k], I.pitch, npb);
+        // the recomputation applies the sources inside the update kernels, from the pristine series;
+        // the records it writes equal the forward run's (the same operations on the same state)
+        di[k].ebits = I.d_ebits;
+        di[k].egroups = I.d_egroups;
+        di[k].negroups = I.negroups;
+        di[k].hbits = I.d_hbits;
+        di[k].hgroups = I.d_hgroups;
+        di[k].nhgroups = I.nhgroups;
         di[n + k] = make_instdev<T>(M, d_adj, I.pitch, npb);
         md[k] = MemberDev{tile0, npb * chunks, 1, k};
         tile0 += npb * chunks;
@@ -3996,10 +3970,8 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     const std::vector<int64_t> ga0 = block_offsets(gcells, ablock);
     const int64_t* d_ga0 = (const int64_t*)upload(ga0.data(), ga0.size() * 8);
     const int gab = (int)ga0.back();
-    const int ngE = (int)gE.size(), ngH = (int)gH.size();
-    gE.insert(gE.end(), gH.begin(), gH.end());
+    const int ngE = (int)gE.size(), ngH = (int)gH.size();  // the adjoint groups' E / H split
     aE.insert(aE.end(), aH.begin(), aH.end());
-    const ProbeGroup<T>* d_gr = gE.empty() ? nullptr : (const ProbeGroup<T>*)upload(gE.data(), gE.size() * sizeof(ProbeGroup<T>));
     const ProbeGroup<T>* d_ag = aE.empty() ? nullptr : (const ProbeGroup<T>*)upload(aE.data(), aE.size() * sizeof(ProbeGroup<T>));
     HIP_OK(hipStreamSynchronize(st));  // host-side descriptor vectors stay alive; uploads done
     const int grid = (tile0 + 7) & ~7;
@@ -4015,12 +3987,10 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         for (int64_t t = n0; t < n1 && !hist_mode; ++t) {
             const int j = (int)(t - n0);
             k_save_h<T><<<hblocks, 256, 0, st>>>(d_di, d_bd, d_h0, n, j);
-            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
-            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
-            if (ngE) k_probes<T><<<(ngE + 63) / 64, 64, 0, st>>>(d_di, d_gr, ngE, nullptr, (int)t, 1);
-            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
-            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, 0, 0);
-            if (ngH) k_probes<T><<<(ngH + 63) / 64, 64, 0, st>>>(d_di, d_gr + ngE, ngH, nullptr, (int)t, 1);
+            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
+            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
+            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
+            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
         }
         for (int64_t t = n1 - 1; t >= n0; --t) {
             const int j = (int)(t - n0);
