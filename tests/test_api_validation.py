@@ -28,6 +28,17 @@ def test_type_errors(packed):
         hip_func.pack_call(c, tuple(probes), monitors, mem, Nx, Ny, Nz, Nt)
 
 
+@pytest.mark.parametrize("Nt,shrink", [(0, None), (-3, None), (8, 0), (8, 1), (8, 2)])
+def test_step_count_and_grid_size(packed, Nt, shrink):
+    """No time step, or an axis of fewer than 2 cells, is refused before any array is read."""
+    c, probes, monitors, mem, Nx, Ny, Nz, _ = _args(packed)
+    dims = [Nx, Ny, Nz]
+    if shrink is not None:
+        dims[shrink] = 1
+    with pytest.raises(ValueError, match="grid size or number of time steps"):
+        hip_func.pack_call(c, probes, monitors, mem, *dims, Nt)
+
+
 @pytest.mark.parametrize("breakage", ["dtype", "shape", "order", "missing", "ndim"])
 def test_coefficient_array_checks(packed, breakage):
     c, probes, monitors, mem, Nx, Ny, Nz, Nt = _args(packed)

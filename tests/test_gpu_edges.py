@@ -1,0 +1,107 @@
+"""Edge cases of the drop-in call against the oracle: one time step, runs without monitors,
+without any probe or monitor, and a time monitor whose sampling interval is longer than the run
+(one sample, the count `Nt / n_step + 1` of solver.cpp:383).  Results must be identical bit for
+bit, as in the main parity tests."""
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import cpu_solver
+from rfnetwork_amd import workloads
+from rfnetwork_amd.hip_func import solver_run_hip
+from rfnetwork_amd.session import Session
+
+from helpers import run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _drop_in(model, dtype):
+    c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True, dtype=dtype)
+    solver_run_hip(c, probes, monitors, mem, Nx, Ny, Nz, Nt, dtype=dtype)
+    return probes, monitors
+
+
+def _assert_same(model, gpu, cpu):
+    (p_g, m_g), (p_c, m_c) = gpu, cpu
+    assert len(p_g) == len(p_c)
+    for a, b in zip(p_g, p_c):
+        np.testing.assert_array_equal(a["values"], b["values"])
+    model.unpack(p_g, m_g, gpu=True)
+    g = {k: np.array(m["values"]) for k, m in model.monitors.items()}
+    model.unpack(p_c, m_c, gpu=False)
+    c = {k: np.array(m["values"]) for k, m in model.monitors.items()}
+    assert g.keys() == c.keys()
+    for k in g:
+        assert g[k].shape == c[k].shape, k
+        np.testing.assert_array_equal(g[k], c[k], err_msg=k)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_single_step(hip_ready, dtype):
+    print(f"[case] single step {np.dtype(dtype).name}", flush=True)
+    model = workloads.tiny_model(Nt=1)
+    # a one-step Gaussian pulse normalises 0 / 0; drive the port with a plain nonzero series
+    model.assign_excitation(np.array([3e-3, 7e-3]), 1)
+    cpu = run_oracle(model, dtype, n_threads=2, fields=False)[:2]
+    assert all(np.all(np.isfinite(p["values"])) for p in cpu[0])
+    assert any(np.any(p["values"]) for p in cpu[0][1:])
+    _assert_same(model, _drop_in(model, dtype), cpu)
+
+
+def test_no_monitors(hip_ready):
+    print("[case] no monitors", flush=True)
+    model = workloads.tiny_model(Nt=40, time_monitors=False, phasor_monitors=False)
+    cpu = run_oracle(model, np.float32, n_threads=2, fields=False)[:2]
+    gpu = _drop_in(model, np.float32)
+    assert gpu[1] == [] and cpu[1] == []
+    _assert_same(model, gpu, cpu)
+
+
+def test_monitor_interval_longer_than_run(hip_ready):
+    print("[case] monitor interval > Nt", flush=True)
+    Nt = 12
+    model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
+    model.add_field_monitor("t_once", "ez", "x", index=6, n_step=Nt + 5)
+    model.add_field_monitor("t_last", "hy", "z", index=4, n_step=Nt)
+    cpu = run_oracle(model, np.float32, n_threads=2, fields=False)[:2]
+    gpu = _drop_in(model, np.float32)
+    _assert_same(model, gpu, cpu)
+    assert np.array(model.monitors["t_once"]["values"]).shape[0] == 1
+    assert np.array(model.monitors["t_last"]["values"]).shape[0] == 2
+
+
+def test_no_probes_no_monitors(hip_ready):
+    """Nothing drives the grid: every field stays zero, on the GPU as in the oracle."""
+    print("[case] no probes, no monitors", flush=True)
+    model = workloads.tiny_model(Nt=20, time_monitors=False, phasor_monitors=False)
+    c, _, _, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    with Session(c, Nx, Ny, Nz) as s:
+        s.set_sources([], [], Nt)
+        s.run(Nt)
+        s.synchronize()
+        s.read()
+        f_g = s.fields()
+    c_cpu, _, _, mem_cpu, *_ = model.pack(gpu=False, n_threads=2)
+    f_c = {n: np.zeros(cpu_solver.global_field_shape(k, Nx, Ny, Nz), np.float32)
+           for n, k in zip(cpu_solver.FIELD_NAMES, cpu_solver.FIELD_COMPONENT)}
+    cpu_solver.solver_run(c_cpu, [], [], mem_cpu, Nx, Ny, Nz, Nt, 2, 0, dtype=np.float32, fields_out=f_c)
+    assert f_g.keys() == f_c.keys()
+    for k in f_g:
+        assert not np.any(f_g[k]), k
+        np.testing.assert_array_equal(f_g[k], f_c[k], err_msg=k)
+
+
+def test_repeated_drop_in_calls_are_independent(hip_ready):
+    """Two drop-in calls with the same arguments give the same outputs (no state carried over)."""
+    print("[case] repeated drop-in calls", flush=True)
+    model = workloads.tiny_model(Nt=30)
+    c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    c2, probes2, monitors2 = copy.deepcopy(c), copy.deepcopy(probes), copy.deepcopy(monitors)
+    solver_run_hip(c, probes, monitors, mem, Nx, Ny, Nz, Nt)
+    solver_run_hip(c2, probes2, monitors2, mem, Nx, Ny, Nz, Nt)
+    for a, b in zip(probes, probes2):
+        np.testing.assert_array_equal(a["values"], b["values"])
+    for a, b in zip(monitors, monitors2):
+        np.testing.assert_array_equal(np.asarray(a["values"]), np.asarray(b["values"]))
