@@ -72,6 +72,51 @@ def test_monitor_interval_longer_than_run(hip_ready):
     assert np.array(model.monitors["t_last"]["values"]).shape[0] == 2
 
 
+def _extra_probes(model, n, Nt, rng, dtype):
+    """n probe records at random interior cells and components, a third of them sources with
+    random series; cells repeat, so several records share (component, cell) groups."""
+    Nx, Ny, Nz = model.Nx, model.Ny, model.Nz
+    out = []
+    for _ in range(n):
+        src = rng.random() < 1 / 3
+        out.append({"values": (rng.uniform(-1e-3, 1e-3, Nt) if src else np.zeros(Nt)).astype(dtype),
+                    "field": int(rng.integers(0, 6)),
+                    "idx": [int(rng.integers(1, Nx)), int(rng.integers(1, Ny)), int(rng.integers(1, Nz))],
+                    "is_source": int(src)})
+    return out
+
+
+def test_maximum_probe_and_monitor_counts(hip_ready):
+    """Exactly the reference's limits (solver.h:15-16): 50 monitors and 5000 probe records."""
+    from rfnetwork_amd import _lib
+
+    print("[case] 5000 probes, 50 monitors", flush=True)
+    Nt = 24
+    model = workloads.tiny_model(Nt=Nt)
+    f = np.array([12e9, 30e9])
+    k = 0
+    while len(model.monitors) < _lib.MAX_MONITORS:
+        field = ("ex", "ey", "ez", "hx", "hy", "hz")[k % 6]
+        axis = "xyz"[(k // 6) % 3]
+        index = 2 + k % 5
+        if k % 2:
+            model.add_field_monitor(f"m{k}", field, axis, index=index, frequency=f, n_step=1 + k % 3)
+        else:
+            model.add_field_monitor(f"m{k}", field, axis, index=index, n_step=1 + k % 4)
+        k += 1
+    c, p_g, m_g, mem, Nx, Ny, Nz, _ = model.pack(gpu=True)
+    c_c, p_c, m_c, mem_c, *_ = model.pack(gpu=False, n_threads=2)
+    extra = _extra_probes(model, _lib.MAX_PROBES - len(p_g), Nt, np.random.default_rng(5), np.float32)
+    p_g = p_g + copy.deepcopy(extra)
+    p_c = p_c + copy.deepcopy(extra)
+    assert len(p_g) == _lib.MAX_PROBES and len(m_g) == _lib.MAX_MONITORS
+    cpu_solver.solver_run(c_c, p_c, m_c, mem_c, Nx, Ny, Nz, Nt, 2, 0, dtype=np.float32)
+    solver_run_hip(c, p_g, m_g, mem, Nx, Ny, Nz, Nt)
+    for i, (a, b) in enumerate(zip(p_g, p_c)):
+        np.testing.assert_array_equal(a["values"], b["values"], err_msg=f"probe {i}")
+    _assert_same(model, (p_g[:len(p_g) - len(extra)], m_g), (p_c[:len(p_c) - len(extra)], m_c))
+
+
 def test_no_probes_no_monitors(hip_ready):
     """Nothing drives the grid: every field stays zero, on the GPU as in the oracle."""
     print("[case] no probes, no monitors", flush=True)
