@@ -1,7 +1,8 @@
 // TEST INFRASTRUCTURE ONLY -- the CPU oracle for the FDTD hot path.
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and
-// only as the checker or the timed CPU baseline.  The product (rfnetwork_amd) never calls it.
+// only to check the engine's results against, or as the timed CPU baseline.  The product
+// (rfnetwork_amd) never calls it.
 //
 // It restates, as plain loops, the threaded C++ solver of rfnetwork:
 //   rfnetwork/core/src/solver.cpp:239-349   solver_init_fields  (shape checks, PEC zeroing of the
