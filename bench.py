@@ -23,6 +23,7 @@ import json
 import os
 import sys
 import time
+import traceback
 from pathlib import Path
 
 import numpy as np
@@ -31,6 +32,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# The best streaming rate measured on MI355X with this library's own probes (a nontemporal copy, and
+# 9 input and 9 output streams as the update kernels have): the practical ceiling beside the peak.
+BEST_STREAMING_GBS = 5720.0
 E_BYTES, H_BYTES = 120, 144  # algorithmic bytes per cell of the E and H half steps (sum 264)
 METRIC = "Mcell-updates/s + % HBM peak, batched vs plain; adjoint gradients/s, 1 GPU"
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X (KiB per
@@ -84,34 +88,94 @@ def dft_bytes_per_step(model):
     return total
 
 
-def cpu_baseline(model, budget_s):
-    """Time the CPU oracle on this host over a bounded number of steps of the same workload."""
+# The GPU box allots 16 host cores to a one-GPU job, whatever nproc reports there.
+BOX_CPU_SHARE = 16
+REF_MAX_THREADS = 20  # the reference's thread cap (rfnetwork/core/inc/solver.h:17)
+API_DEFAULT_THREADS = 4  # FDTD_Solver.solve(n_threads=4) (rfnetwork/solver.py:1410)
+
+
+def host_cores():
+    """Cores this process may run on: the affinity mask, capped by a cgroup CPU quota if one is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_thread_counts():
+    """The reference's thread count on this host, min(cores, 20) (held to the box's share), and the
+    API default of 4."""
+    cores = host_cores()
+    top = min(cores, REF_MAX_THREADS, BOX_CPU_SHARE)
+    return cores, top, sorted({top, min(API_DEFAULT_THREADS, top)}, reverse=True)
+
+
+def _oracle_inputs(model, threads, nsteps):
+    """The model packed for the CPU oracle, its probe and monitor series cut to nsteps (the model's
+    own series must be at least that long)."""
+    c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=False, n_threads=threads)
+    if Nt < nsteps:
+        raise ValueError(f"model has {Nt} steps of source samples, the window needs {nsteps}")
+    for p in probes:
+        p["values"] = np.ascontiguousarray(p["values"][:nsteps])
+    for m in monitors:
+        if "dtft_phase" in m:
+            m["dtft_phase"] = np.ascontiguousarray(m["dtft_phase"][: nsteps // m["n_step"] + 1])
+        else:
+            m["values"] = np.ascontiguousarray(m["values"][: nsteps // m["n_step"] + 1])
+    return c, probes, monitors, mem, Nx, Ny, Nz
+
+
+def _time_oracle(model, threads, nsteps):
+    """Wall time of one oracle run of nsteps from zero fields (fresh probe and monitor arrays)."""
     from oracle import cpu_solver
 
-    threads = max(1, min(16, os.cpu_count() or 1, 20))
+    c, probes, monitors, mem, Nx, Ny, Nz = _oracle_inputs(model, threads, nsteps)
+    t0 = time.perf_counter()
+    cpu_solver.solver_run(c, probes, monitors, mem, Nx, Ny, Nz, nsteps, threads, 0)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(workload, budget_s, max_window=200, runs=3):
+    """The CPU oracle (a restatement of the reference's threaded solver, rfnetwork/core/src/solver.cpp,
+    x slabs, a controller thread and two barriers per step) timed on this host over a window of the
+    same workload, at the reference's thread count and at the API default: the median of `runs`
+    runs each, the window sized so that each thread count takes about budget_s seconds.  The model
+    is built here with source samples for the longest window, whatever the GPU leg's step count."""
+    cores, top, counts = cpu_thread_counts()
+    model = build_model(workload, max_window)
     cells = model.Nx * model.Ny * model.Nz
-
-    def run(nsteps):
-        c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=False, n_threads=threads)
-        for p in probes:
-            p["values"] = np.ascontiguousarray(p["values"][:nsteps])
-        for m in monitors:
-            if "dtft_phase" in m:
-                m["dtft_phase"] = np.ascontiguousarray(m["dtft_phase"][: nsteps // m["n_step"] + 1])
-            else:
-                m["values"] = np.ascontiguousarray(m["values"][: nsteps // m["n_step"] + 1])
-        t0 = time.perf_counter()
-        cpu_solver.solver_run(c, probes, monitors, mem, Nx, Ny, Nz, nsteps, threads, 0)
-        return time.perf_counter() - t0
-
-    t_probe = run(2)
-    per_step = t_probe / 2
-    steps = int(max(4, min(200, budget_s / max(per_step, 1e-6))))
-    dt = run(steps)
-    return dict(value=round(cells * steps / dt / 1e6, 2), unit="Mcell-updates/s", cores=threads, kind="port",
-                sample=f"{steps} steps of the full {model.Nx}x{model.Ny}x{model.Nz} workload from zero fields "
-                       f"(oracle restating rfnetwork/core/src/solver.cpp, {threads} threads, x slabs, 2 barriers "
-                       f"per step, -O2 -ffp-contract=off), {dt:.1f} s wall")
+    per_count = {}
+    for threads in counts:
+        per_step = _time_oracle(model, threads, 2) / 2
+        window = int(max(4, min(max_window, budget_s / runs / max(per_step, 1e-6))))
+        walls = sorted(_time_oracle(model, threads, window) for _ in range(runs))
+        med = walls[len(walls) // 2]
+        per_count[threads] = dict(mcells=round(cells * window / med / 1e6, 2), window_steps=window,
+                                  median_s=round(med, 3), runs_s=[round(w, 3) for w in walls])
+        log(f"cpu baseline: {threads} threads, {window}-step window, median {med:.2f}s "
+            f"-> {per_count[threads]['mcells']} Mcell/s")
+    best = per_count[top]
+    return dict(value=best["mcells"], unit="Mcell-updates/s", cores=top, kind="port",
+                sample=f"{best['window_steps']} steps of the full {model.Nx}x{model.Ny}x{model.Nz} workload from zero "
+                       f"fields, median of {runs} runs (oracle restating rfnetwork/core/src/solver.cpp: {top} threads "
+                       f"= min(cores, 20) held to the box's {BOX_CPU_SHARE}-core share, x slabs, 2 barriers per step, "
+                       f"-O2 -ffp-contract=off)",
+                cpu_model=cpu_model(), host_cores=cores, by_threads={str(k): v for k, v in per_count.items()})
 
 
 def c1_leg(flags=0):
@@ -135,19 +199,30 @@ def c1_leg(flags=0):
         s.read()
     model.unpack(probes, monitors, gpu=True)
     S_gpu = np.asarray(model.get_sparameters(model.frequency))
-    threads = max(1, min(16, os.cpu_count() or 1))
-    m2 = workloads.microstrip()
-    c2, p2, mo2, mem2, *_ = m2.pack(gpu=False, n_threads=threads)
-    t0 = time.perf_counter()
-    cpu_solver.solver_run(c2, p2, mo2, mem2, Nx, Ny, Nz, Nt, threads, 0)
-    wall_cpu = time.perf_counter() - t0
-    m2.unpack(p2, mo2, gpu=False)
-    S_cpu = np.asarray(m2.get_sparameters(m2.frequency))
-    log(f"c1: GPU {wall * 1e3:.1f} ms, CPU oracle {wall_cpu:.2f} s on {threads} threads, "
-        f"max |dS| {np.abs(S_gpu - S_cpu).max():.2e}")
+    # CPU: the full run on the oracle, median of 3 at the reference's thread count and at 4 threads
+    _, top, counts = cpu_thread_counts()
+    cpu = {}
+    S_cpu = None
+    for threads in counts:
+        walls = []
+        for _ in range(3):
+            m2 = workloads.microstrip()
+            c2, p2, mo2, mem2, *_ = m2.pack(gpu=False, n_threads=threads)
+            t0 = time.perf_counter()
+            cpu_solver.solver_run(c2, p2, mo2, mem2, Nx, Ny, Nz, Nt, threads, 0)
+            walls.append(time.perf_counter() - t0)
+        walls.sort()
+        cpu[threads] = walls[1]
+        if S_cpu is None:
+            m2.unpack(p2, mo2, gpu=False)
+            S_cpu = np.asarray(m2.get_sparameters(m2.frequency))
+    wall_cpu = cpu[top]
+    log(f"c1: GPU {wall * 1e3:.1f} ms, CPU oracle {wall_cpu:.2f} s on {top} threads "
+        f"({', '.join(f'{k}: {v:.2f} s' for k, v in cpu.items())}), max |dS| {np.abs(S_gpu - S_cpu).max():.2e}")
     return dict(workload=f"microstrip {Nx}x{Ny}x{Nz}, {Nt} steps, 101 frequencies", gpu_ms=round(wall * 1e3, 2),
                 gpu_mcells=round(cells * Nt / wall / 1e6, 1), launch=tuned,
-                cpu_oracle_s=round(wall_cpu, 3), cpu_mcells=round(cells * Nt / wall_cpu / 1e6, 1), cpu_threads=threads,
+                cpu_oracle_s=round(wall_cpu, 3), cpu_mcells=round(cells * Nt / wall_cpu / 1e6, 1), cpu_threads=top,
+                cpu_oracle_s_by_threads={str(k): round(v, 3) for k, v in cpu.items()}, cpu_runs=3,
                 speedup=round(wall_cpu / wall, 1), s_params_identical=bool(np.array_equal(S_gpu, S_cpu)))
 
 
@@ -508,6 +583,16 @@ def adjoint_fd_leg(n_steps, every, n_cells=32, seed=41, h=1e-3, oracle_cells=1):
                 oracle_J_equal=oracle_equal, pass_1e3=bool(r64.max() <= 1e-3))
 
 
+def guarded(name, fn, *a, **k):
+    """Run one leg; a failure is logged and recorded as {"error": ...} so the JSON line still prints."""
+    try:
+        return fn(*a, **k)
+    except Exception as e:  # noqa: BLE001 -- every leg's failure is reported, none ends the bench
+        log(f"{name} failed: {type(e).__name__}: {e}")
+        traceback.print_exc(file=sys.stderr)
+        return dict(error=f"{type(e).__name__}: {e}")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -518,11 +603,13 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant 0..6 (default: autotuned)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--xplanes", type=int, default=0)
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work (0 = skip)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of CPU oracle work per thread count (0 = skip)")
     ap.add_argument("--no-copy-probe", action="store_true")
-    ap.add_argument("--legs", default="plain,c1,batched,adjoint",
-                    help="comma list: plain (the headline), c1 (the 120x48x32 microstrip, GPU and CPU oracle), "
-                         "batched (12 excitations), adjoint (gradients/s), "
+    ap.add_argument("--legs", default="plain,raw,c1,batched,adjoint",
+                    help="comma list: plain (the headline), raw (the same workload with the 30 raw coefficient "
+                         "arrays: the reference formulation's bytes), c1 (the 120x48x32 microstrip, GPU and CPU "
+                         "oracle), batched (12 excitations), adjoint (gradients/s), "
                          "c5 (8 design variants, batched forward+adjoint, minutes), "
                          "adjoint_fd (finite-difference validation, minutes), "
                          "monitors (streamed time-domain monitors against resident ones), "
@@ -546,116 +633,175 @@ def main():
                ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
                data="synthetic")
     if "plain" in legs:
-        out.update(plain_leg(args))
+        res = guarded("plain", plain_leg, args)
+        if "error" in res:
+            out["plain_error"] = res["error"]
+        else:
+            out.update(res)
+    if "raw" in legs:
+        res = guarded("raw", raw_leg, args)
+        if "roofline" in res:
+            out["roofline"] = res.pop("roofline")
+        out["raw"] = res
+    if "plain" in legs and args.cpu_budget > 0:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = guarded("cpu_baseline", cpu_baseline, args.workload, args.cpu_budget)
     gflags = _lib.NO_GRAPHS if args.no_graphs else 0
     if "c1" in legs:
-        out["c1"] = c1_leg(gflags)
+        out["c1"] = guarded("c1", c1_leg, gflags)
     if "batched" in legs:
-        out["batched"] = batched_leg(args.batch_steps, gflags)
+        out["batched"] = guarded("batched", batched_leg, args.batch_steps, gflags)
     if "adjoint" in legs:
-        out["adjoint"] = adjoint_leg(args.adjoint_steps, args.checkpoint_every, flags=gflags)
+        out["adjoint"] = guarded("adjoint", adjoint_leg, args.adjoint_steps, args.checkpoint_every, flags=gflags)
     if "c5" in legs:
-        out["c5"] = c5_leg(args.c5_steps, gflags)
+        out["c5"] = guarded("c5", c5_leg, args.c5_steps, gflags)
     if "adjoint_fd" in legs:
-        out["adjoint_fd"] = adjoint_fd_leg(args.adjoint_steps, args.checkpoint_every)
+        out["adjoint_fd"] = guarded("adjoint_fd", adjoint_fd_leg, args.adjoint_steps, args.checkpoint_every)
     if "monitors" in legs:
-        out["monitors"] = monitors_leg()
+        out["monitors"] = guarded("monitors", monitors_leg)
     if "dft64" in legs:
-        out["dft64"] = dft64_leg()
+        out["dft64"] = guarded("dft64", dft64_leg)
     print(json.dumps(out), flush=True)
 
 
-def plain_leg(args):
-    """The headline: K timed steps of the single-grid workload after W warm-up steps."""
+def timed_run(args, coef, kernel_timing=True):
+    """W warm-up steps then K timed steps of the single-grid workload on a resident, tuned session
+    (coefficients `coef`: "auto", "raw" or "table").  Per-launch E and H kernel times come from HIP
+    events on the session's stream around up to 64 evenly spread steps of the timed region."""
     from rfnetwork_amd import _lib
     from rfnetwork_amd.session import Session
 
     Nt = args.warmup + args.steps
     t0 = time.perf_counter()
     model = build_model(args.workload, Nt)
-    cells = model.Nx * model.Ny * model.Nz
     t_model = time.perf_counter() - t0
     log(f"model {args.workload} {model.Nx}x{model.Ny}x{model.Nz} built in {t_model:.1f}s")
     c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True)
     t_pack = time.perf_counter() - t0 - t_model
-    flags = _lib.KERNEL_TIMING | {"auto": 0, "raw": _lib.COEF_RAW, "table": _lib.COEF_TABLE}[args.coef]
+    flags = (_lib.KERNEL_TIMING if kernel_timing else 0) | {"auto": 0, "raw": _lib.COEF_RAW,
+                                                             "table": _lib.COEF_TABLE}[coef]
     t0 = time.perf_counter()
     s = Session(c, Nx, Ny, Nz, flags=flags)
     s.synchronize()
     t_session = time.perf_counter() - t0
     del c
-    tuned = None
     if args.block or args.xplanes or args.variant >= 0:
         s.set_launch(args.block, args.xplanes, 0)
         if args.variant >= 0:
             s.set_variant(args.variant)
-        tuned = dict(variant=args.variant if args.variant >= 0 else 1, block=args.block or 256, x_planes=args.xplanes, cached=None)
+        tuned = dict(variant=args.variant if args.variant >= 0 else 1, block=args.block or 256, x_planes=args.xplanes,
+                     cached=None)
         s.set_sources(probes, monitors, Nt)
     else:
         s.set_sources(probes, monitors, Nt)  # installed first: the tuner's trials then apply them too
         t1 = time.perf_counter()
         tuned = s.autotune()
-        log(f"autotune {tuned} in {time.perf_counter() - t1:.1f}s")
+        log(f"autotune ({coef}) {tuned} in {time.perf_counter() - t1:.1f}s")
     info = s.memory_info()
     log(f"session ready in {time.perf_counter() - t0:.1f}s: {info}")
-
     s.run(args.warmup)
     s.synchronize()
-    log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
     s.run(args.steps)
     s.synchronize()
     wall = time.perf_counter() - t0
     total_ms, e_ms, h_ms = s.timing()
-    log(f"timed {args.steps} steps: wall {wall:.3f}s, events {total_ms / 1e3:.3f}s, E (or fused) {e_ms:.4f} ms, "
-        f"H {h_ms:.4f} ms")
+    log(f"timed {args.steps} steps ({coef}): wall {wall:.3f}s, events {total_ms / 1e3:.3f}s, E (or fused) "
+        f"{e_ms:.4f} ms, H {h_ms:.4f} ms")
     s.read()
     finite = all(np.isfinite(p["values"][:Nt]).all() for p in probes)
     s.close()
+    return dict(model=model, cells=model.Nx * model.Ny * model.Nz, wall=wall, e_ms=e_ms, h_ms=h_ms, tuned=tuned,
+                info=info, finite=finite, host_s=t_model + t_pack + t_session)
 
+
+def kernel_roofline(run, kname, kbytes, kms, traffic, bytes_model):
+    """Roofline object of one kernel: algorithmic bytes per launch over its HIP-event launch time."""
+    cells = run["cells"]
+    k_gbs = cells * kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    return dict(bound="hbm", kernel=kname, coefficients=run["info"]["coef_kind"], achieved=round(k_gbs, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(k_gbs / HBM_PEAK_GBS, 4), traffic=traffic,
+                bytes_per_cell=kbytes, bytes_model=bytes_model, algorithmic_bytes_per_launch=cells * kbytes,
+                launch_ms=round(kms, 4),
+                measured_hbm_gbs=round(traffic / (kms * 1e-3) / 1e9, 1) if traffic and kms else None,
+                measured_hbm_frac=round(traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic and kms else None)
+
+
+# Table form: the deduplicated coefficient tuples live in L2, so a half step moves its fields (6 split
+# values read and written, 3 neighbour totals read, 3 totals written: 72 B), the 4-byte entry index of
+# the cell and 1/8 B of probe bitmask.
+TABLE_BYTES = 72 + 4 + 0.125
+
+
+def plain_leg(args):
+    """The headline: K timed steps of the single-grid workload after W warm-up steps, on the engine's
+    default coefficient form (the deduplicated table wherever it is small)."""
+    from rfnetwork_amd import _lib
+
+    run = timed_run(args, args.coef)
+    model, cells, wall, e_ms, h_ms, tuned, info = (run[k] for k in ("model", "cells", "wall", "e_ms", "h_ms",
+                                                                     "tuned", "info"))
     ms_step = wall * 1e3 / args.steps
     mcells = cells * args.steps / wall / 1e6
     dft_b = dft_bytes_per_step(model)
     step_bytes = cells * (E_BYTES + H_BYTES) + dft_b
     step_frac = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6))
-    # the dominant kernel: the fused E+H pass (264 B/cell) or the H update (144 B/cell)
-    kname, kbytes, kms = ("k_step_fused", E_BYTES + H_BYTES, e_ms) if fused else ("k_update_h", H_BYTES, h_ms)
-    k_gbs = cells * kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    e_gbs = cells * E_BYTES / (e_ms * 1e-3) / 1e9 if e_ms > 0 and not fused else None
+    table = info["coef_kind"] == "table"
+    # the dominant kernel: the fused E+H pass or the H update
+    if fused:
+        kname, kbytes, kms = "k_step_fused", (2 * TABLE_BYTES if table else E_BYTES + H_BYTES), e_ms
+    else:
+        kname, kbytes, kms = "k_update_h", (TABLE_BYTES if table else H_BYTES), h_ms
     shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
     traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname, shape))
-
+    model_text = ("table form: 72 B of fields + 4 B entry index + 1/8 B probe bitmask per cell" if table
+                  else "reference formulation: 144 B per cell (6 split H read and written, 18 coefficients, "
+                       "3 E totals read, 3 H totals written)")
     copy = None
     if not args.no_copy_probe:
         gbs, cms = _lib.copy_bandwidth(1 << 30, 20)
-        copy = dict(gb_per_s=round(gbs, 1), bytes_per_copy=2 * (1 << 30), ms_per_copy=round(cms, 4))
+        copy = dict(gb_per_s=round(gbs, 1), bytes_per_copy=2 * (1 << 30), ms_per_copy=round(cms, 4),
+                    best_streaming_gb_per_s=BEST_STREAMING_GBS)
         log(f"copy probe {gbs:.1f} GB/s")
-
-    setup = None
-    if args.coef == "auto":
-        setup = setup_times(args.workload, Nt, t_model + t_pack + t_session)
-    base = None
-    if args.cpu_budget > 0:
-        log("cpu baseline ...")
-        base = cpu_baseline(model, args.cpu_budget)
-        log(f"cpu baseline {base['value']} Mcell/s on {base['cores']} threads")
-
+    setup = guarded("setup", setup_times, args.workload, args.warmup + args.steps, run["host_s"]) \
+        if args.coef == "auto" else None
+    e_gbs = cells * (TABLE_BYTES if table else E_BYTES) / (e_ms * 1e-3) / 1e9 if e_ms > 0 and not fused else None
     return dict(
         value=round(mcells, 1), ms_per_step=round(ms_step, 4),
         config=dict(workload=f"{args.workload} {model.Nx}x{model.Ny}x{model.Nz}", cells=cells,
                     dft_planes=len(model.monitors), dft_frequencies=51 if model.monitors else 0,
                     coefficients=info["coef_kind"], coef_entries=[info["n_e_entries"], info["n_h_entries"]],
                     launch=tuned),
-        roofline=dict(bound="hbm", kernel=kname, achieved=round(k_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(k_gbs / HBM_PEAK_GBS, 4), traffic=traffic,
-                      algorithmic_bytes_per_launch=cells * kbytes, launch_ms=round(kms, 4),
-                      measured_hbm_gbs=round(traffic / (kms * 1e-3) / 1e9, 1) if traffic and kms else None),
-        step=dict(hbm_frac_algorithmic=round(step_frac, 4), algorithmic_bytes=step_bytes,
+        roofline_table=kernel_roofline(run, kname, kbytes, kms, traffic, model_text),
+        step=dict(hbm_frac_reference_bytes=round(step_frac, 4), reference_bytes=step_bytes,
                   e_kernel_ms=None if fused else round(e_ms, 4), e_kernel_gbs=e_gbs and round(e_gbs, 1),
                   h_kernel_ms=None if fused else round(h_ms, 4), dft_bytes=dft_b),
-        copy_probe=copy, cpu_baseline=base, finite=finite, setup=setup,
+        copy_probe=copy, finite=run["finite"], setup=setup,
     )
+
+
+def raw_leg(args):
+    """The same workload with the 30 coefficient arrays read per cell (the reference formulation,
+    RFDTD_COEF_RAW): its H update moves the 144 B per cell that the roofline's byte count assumes,
+    so this is the like-for-like fraction of HBM peak."""
+    run = timed_run(args, "raw")
+    tuned, h_ms, e_ms, cells = run["tuned"], run["h_ms"], run["e_ms"], run["cells"]
+    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6))
+    if fused:
+        kname, kbytes, kms = "k_step_fused", E_BYTES + H_BYTES, e_ms
+    else:
+        kname, kbytes, kms = "k_update_h", H_BYTES, h_ms
+    shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
+    traffic = MEASURED_TRAFFIC.get((args.workload, "raw", kname, shape))
+    roof = kernel_roofline(run, kname, kbytes, kms, traffic,
+                           "reference formulation: 144 B per cell (6 split H read and written, 18 coefficients, "
+                           "3 E totals read, 3 H totals written); E update 120 B")
+    e_frac = cells * E_BYTES / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if e_ms > 0 and not fused else None
+    return dict(roofline=roof, ms_per_step=round(run["wall"] * 1e3 / args.steps, 4),
+                mcells=round(cells * args.steps / run["wall"] / 1e6, 1), launch=tuned,
+                e_kernel_ms=None if fused else round(e_ms, 4), e_kernel_frac=e_frac and round(e_frac, 4),
+                h_kernel_ms=round(h_ms, 4), finite=run["finite"])
 
 
 def setup_times(workload, Nt, host_s):

@@ -8,6 +8,7 @@ machine.  The file is ``$RFNETWORK_AMD_TUNE_CACHE`` or ``~/.cache/rfnetwork_amd/
 failing to read or write it only costs a re-tune.
 """
 import ctypes
+import hashlib
 import json
 import os
 import pathlib
@@ -23,10 +24,32 @@ def cache_path():
     return pathlib.Path(p) if p else pathlib.Path.home() / ".cache" / "rfnetwork_amd" / "launch.json"
 
 
+_CFG_KEYS = ("variant", "block", "x_planes")
+_build_id = None
+
+
+def build_id():
+    """The library's version and a digest of the built library file: any rebuild with changed
+    kernels or a changed config format gives new keys, so no entry tuned for other kernels is
+    applied."""
+    global _build_id
+    if _build_id is None:
+        try:
+            digest = hashlib.sha1(pathlib.Path(_lib.lib()._name).read_bytes()).hexdigest()[:12]
+        except (OSError, AttributeError):
+            digest = "nofile"
+        _build_id = f"v{_lib.version()}-{digest}"
+    return _build_id
+
+
 def shape_key(grids, dtype, coef_kind):
-    """grids: [(Nx, Ny, Nz, K)]; the key also holds the library version (kernels change with it)."""
+    """grids: [(Nx, Ny, Nz, K)]; the key also holds the build id (kernels change with it)."""
     g = ";".join(f"{nx}x{ny}x{nz}x{k}" for nx, ny, nz, k in grids)
-    return f"{_lib.version()}|{dtype}|coef{coef_kind}|{g}"
+    return f"{build_id()}|{dtype}|coef{coef_kind}|{g}"
+
+
+def _valid(cfg):
+    return isinstance(cfg, dict) and all(isinstance(cfg.get(k), int) for k in _CFG_KEYS)
 
 
 def _load():
@@ -108,6 +131,8 @@ def tune(session, grids, dtype, coef_kind, trial_steps=20, use_cache=True):
     cfg = _memory.get(key)
     if cfg is None and use_cache:
         cfg = _load().get(key)
+    if not _valid(cfg):
+        cfg = None  # an entry of another format counts as a miss
     if cfg is not None and use_cache:
         apply(session, cfg)
         _memory[key] = cfg

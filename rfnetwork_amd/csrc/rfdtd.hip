@@ -27,9 +27,10 @@
 //   * workgroups are renumbered so that consecutive tiles land on the same XCD and share its L2:
 //     neighbouring (y,z) tiles share y-neighbour rows, and the instances of one member -- ordered
 //     innermost -- share the coefficient reads of the same cells;
-//   * the 30 coefficient arrays are deduplicated on the host, bit-exactly, into per-cell 32-bit
-//     entry indices plus tables of distinct E (12 values) and H (18 values) tuples: a step moves
-//     8 B of coefficient index per cell instead of 120 B wherever materials repeat;
+//   * the 30 coefficient arrays are deduplicated bit-exactly, on the device (hash, radix sort, runs
+//     of equal tuples; a host pass on request), into per-cell 32-bit entry indices plus tables of
+//     distinct E (12 values) and H (18 values) tuples: a step moves 8 B of coefficient index per
+//     cell instead of 120 B wherever materials repeat;
 //   * a chunk of steps is captured once into a hipGraph and replayed; the kernels read the step
 //     number from a device word set before each replay.
 #include <hip/hip_runtime.h>
@@ -2265,6 +2266,9 @@ struct Session {
 };
 
 void Session::drop_graph() {
+    // replays of these graphs may still be queued (a launch-shape or design change between runs
+    // does not synchronize): drain the device before their executables go
+    if (graph[0] || graph[1]) (void)hipDeviceSynchronize();
     for (int p = 0; p < 2; ++p) {
         if (graph[p]) (void)hipGraphExecDestroy(graph[p]);
         if (graph_def[p]) (void)hipGraphDestroy(graph_def[p]);
@@ -4103,7 +4107,7 @@ extern "C" {
 
 const char* rfdtd_last_error(void) { return g_last_error.c_str(); }
 
-int32_t rfdtd_version(void) { return 2; }
+int32_t rfdtd_version(void) { return 3; }
 
 int rfdtd_create(rfdtd_session** out, int32_t Nx, int32_t Ny, int32_t Nz, const void* const coefficients[30],
                  uint32_t flags) {

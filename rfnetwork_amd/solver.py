@@ -1123,8 +1123,13 @@ class FDTD_Solver:
         return S
 
     def get_monitor_data(self, name: str):
-        """Monitor values with time (or frequency) and the two in-plane coordinates (inches)."""
+        """Monitor values with time (or frequency) and the two in-plane coordinates (inches).
+
+        A name whose three monitors ``name_x``, ``name_y``, ``name_z`` exist (an "e_total" monitor)
+        returns the total field instead (rfnetwork/solver.py:2720-2722)."""
         self.check_solution()
+        if all(f"{name}_{a}" in self.monitors for a in _AXES):
+            return self.get_total_monitor_data(name)
         m = self.monitors[name]
         axes = [a for a in range(3) if a != m["axis"]]
         coords = {_AXES[a]: self.floc[m["field"]][a] for a in axes}
@@ -1132,6 +1137,40 @@ class FDTD_Solver:
             return LabeledArray(m["values"], coords=dict(frequency=m["frequency"], **coords))
         t = np.arange(len(m["values"])) * (self.dt * m["n_step"])
         return LabeledArray(m["values"], coords=dict(time=t, **coords))
+
+    def get_total_monitor_data(self, name: str):
+        """The three E components of an "e_total" monitor on the corners of the plane's cells,
+        stacked x, y, z: shape (3, samples, N1 - 1, N2 - 1) (rfnetwork/solver.py:2750-2802).
+
+        With (a1, a2) the in-plane axes in increasing order and a3 the normal, the a1 component is
+        averaged over its two neighbours along a1 (its edge rows along a2 dropped), the a2 component
+        over its two neighbours along a2 (edge rows along a1 dropped), and the normal component keeps
+        its interior rows, so all three sit on the interior corners.  The averages are formed in the
+        monitor's dtype, (u[1:] + u[:-1]) / 2.  The coordinates are those the x component carries
+        after its first slice, as the reference labels them.  Phasor monitors are refused, as in the
+        reference."""
+        names = [f"{name}_{a}" for a in _AXES]
+        if not all(n in self.monitors for n in names):
+            raise ValueError(f"Monitor {name} does not contain data for all three field components.")
+        m0 = self.monitors[names[0]]
+        if m0["frequency"] is not None:
+            raise NotImplementedError("Phasor monitors not implemented yet for total fields.")
+        axis = m0["axis"]
+        surf = [a for a in range(3) if a != axis]
+        order = surf + [axis]  # in-plane components first, then the normal one
+        parts = [self.get_monitor_data(names[a]) for a in order]
+        vals = [np.asarray(p) for p in parts]
+        v1 = (vals[0][:, 1:, 1:-1] + vals[0][:, :-1, 1:-1]) / 2
+        v2 = (vals[1][:, 1:-1, 1:] + vals[1][:, 1:-1, :-1]) / 2
+        v3 = vals[2][:, 1:-1, 1:-1]
+        # the slice each component's labels follow (the first operand of its average)
+        cuts = [(slice(1, None), slice(1, -1)), (slice(1, -1), slice(1, None)), (slice(1, -1), slice(1, -1))]
+        by_axis = {a: (v, parts[k], cuts[k]) for k, (a, v) in enumerate(zip(order, (v1, v2, v3)))}
+        vx, px, cx = by_axis[0]
+        d1, d2 = _AXES[surf[0]], _AXES[surf[1]]
+        coords = dict(component=("x", "y", "z"), time=px.coords["time"],
+                      **{d1: np.asarray(px.coords[d1])[cx[0]], d2: np.asarray(px.coords[d2])[cx[1]]})
+        return LabeledArray(np.array([by_axis[a][0] for a in range(3)]), coords=coords)
 
     # ------------------------------------------------------------------ edge correction
     def edge_correction(self, p1, p2, integration_line: str, CFe: float = None, CFh: float = 1):

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_version():
     L = _lib.lib()
-    assert L.rfdtd_version() == 2
+    assert L.rfdtd_version() == 3
     assert L.rfdtd_last_error() == b""
 
 

@@ -98,3 +98,32 @@ def test_synchronize_before_the_first_run(hip_ready):
             s.run(Nt)
             s.synchronize()
             s.read()
+
+
+def test_streamed_total_field_monitor(hip_ready):
+    """An "e_total" monitor (three component planes) streamed through a device ring and read back
+    with get_monitor_data: the corner-averaged x, y, z stack equals the oracle run's bit for bit."""
+    from helpers import run_oracle
+
+    print("[case] streaming e_total", flush=True)
+    ref = workloads.tiny_model(Nt=90)
+    ref.add_field_monitor("tot", "e_total", "z", index=4, n_step=2)
+    probes, monitors, _ = run_oracle(ref, fields=False)
+    ref.unpack(probes, monitors, gpu=False)
+    want = ref.get_monitor_data("tot")
+
+    model = workloads.tiny_model(Nt=90)
+    model.add_field_monitor("tot", "e_total", "z", index=4, n_step=2)
+    c, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    with Session(c, Nx, Ny, Nz) as s:
+        s.set_monitor_streaming(6)
+        s.set_sources(probes, monitors, Nt)
+        s.run(41)
+        s.run(Nt - 41)
+        s.synchronize()
+        s.read()
+    model.unpack(probes, monitors, gpu=True)
+    got = model.get_monitor_data("tot")
+    assert got.shape == want.shape and got.shape[0] == 3
+    assert np.abs(np.asarray(want)).max() > 0
+    np.testing.assert_array_equal(np.asarray(got), np.asarray(want))
