@@ -121,10 +121,14 @@ def test_design_variants_batched_gradient(hip_ready):
     res, info = adjoint.eps_r_gradients_batch(jobs, return_info=True, mode="checkpoint")
     print(f"[c5] {info}", flush=True)
     assert info["checkpoint_every"] >= 1
-    for (J, g, _), (m, cbox, _) in zip(res, jobs):
+    for v, ((J, g, _), (m, cbox, _)) in enumerate(zip(res, jobs)):
         b = cbox[1] - cbox[0]
         assert g.shape == (b, b, b) and np.isfinite(g).all() and np.isfinite(J)
-        assert np.abs(g).max() > 0
+        # in 300 steps the pulse reaches port 2 of the small variants only; where it has not, the
+        # port voltage, S21, J and so the gradient are exactly zero
+        assert (np.abs(g).max() > 0) == (J > 0), f"variant {v}: J {J}"
+        if v < 3:
+            assert J > 0, f"variant {v}"
     for v in range(3):
         m = workloads.design_variant(v, Nt=Nt)
         J, g, _ = adjoint.eps_r_gradient(m, workloads.design_variant_box(v), m.frequency, mode="history")
