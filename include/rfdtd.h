@@ -267,6 +267,17 @@ typedef struct rfdtd_ff_face {
 } rfdtd_ff_face;
 int rfdtd_nf2ff(int32_t n_faces, const rfdtd_ff_face* faces, int32_t n_frequencies, const float* beta,
                 int32_t n_theta, const float* theta, int32_t n_phi, const float* phi, float eta0, void* out);
+/*
+ * rfdtd_nf2ff without any device allocation (no counterpart in the reference): for callers that
+ * cannot allocate device memory -- code under graph capture, serving loops, harnesses that forbid
+ * hipMalloc.  The inputs are staged in, and the result taken from, a static 64 MiB device workspace
+ * (one per GPU, held until the process ends); the kernel and the result are those of rfdtd_nf2ff.
+ * At most 64 faces; a request whose staged bytes (each array rounded up to 256 B: faces, r0, r1,
+ * ds, J, M, beta, theta, phi and the result) exceed 64 MiB returns RFDTD_EINVAL before anything is
+ * copied.  Calls of one process share the workspace and are serialised by a lock.
+ */
+int rfdtd_nf2ff_static(int32_t n_faces, const rfdtd_ff_face* faces, int32_t n_frequencies, const float* beta,
+                       int32_t n_theta, const float* theta, int32_t n_phi, const float* phi, float eta0, void* out);
 
 /*
  * On-device coefficient assembly of a whole grid (the counterpart of the host assembly

@@ -56,7 +56,7 @@ EXPORTS = (
     "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
     "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
     "rfdtd_create_assembled", "rfdtd_read_coefficients", "rfdtd_create_batch_assembled",
-    "rfdtd_read_coefficient_box",
+    "rfdtd_read_coefficient_box", "rfdtd_nf2ff_static",
 )
 
 
@@ -172,6 +172,7 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_set_history.argtypes = [vp, i32, vp, vp, i32]
     L.rfdtd_device_memory.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
     L.rfdtd_nf2ff.argtypes = [i32, vp, i32, vp, i32, vp, i32, vp, ctypes.c_float, vp]
+    L.rfdtd_nf2ff_static.argtypes = [i32, vp, i32, vp, i32, vp, i32, vp, ctypes.c_float, vp]
     L.rfdtd_plan_checkpoints.argtypes = [vp, i32, i32, vp, vp, ctypes.c_uint64, ctypes.POINTER(i32),
                                          ctypes.POINTER(ctypes.c_uint64)]
     L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
@@ -218,10 +219,11 @@ class FFFace(ctypes.Structure):
     ]
 
 
-def nf2ff(faces, beta, theta, phi, eta0):
+def nf2ff(faces, beta, theta, phi, eta0, static_workspace=False):
     """Radiation integrals of far-field box faces on the GPU.  faces: dicts with axis, position,
     r0, r1, ds (float32 [n0, n1]) and J, M (complex64 [3, nf, n0, n1]); beta, theta, phi float32
-    (radians).  Returns complex64 [2, nf, ntheta, nphi]."""
+    (radians).  Returns complex64 [2, nf, ntheta, nphi].  static_workspace: through
+    rfdtd_nf2ff_static (no device allocation; requests up to 64 MiB of staging)."""
     import numpy as np
 
     arr = (FFFace * len(faces))()
@@ -245,8 +247,9 @@ def nf2ff(faces, beta, theta, phi, eta0):
     if faces and keep[3].shape[1] != len(beta):
         raise ValueError("one beta per frequency of the currents is required")
     out = np.zeros((2, len(beta), len(theta), len(phi)), dtype=np.complex64)
-    check(lib().rfdtd_nf2ff(len(faces), ctypes.cast(arr, ctypes.c_void_p), len(beta), beta.ctypes.data, len(theta),
-                            theta.ctypes.data, len(phi), phi.ctypes.data, float(eta0), out.ctypes.data), "rfdtd_nf2ff")
+    name = "rfdtd_nf2ff_static" if static_workspace else "rfdtd_nf2ff"
+    check(getattr(lib(), name)(len(faces), ctypes.cast(arr, ctypes.c_void_p), len(beta), beta.ctypes.data, len(theta),
+                               theta.ctypes.data, len(phi), phi.ctypes.data, float(eta0), out.ctypes.data), name)
     return out
 
 

@@ -43,6 +43,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -4100,6 +4101,42 @@ Session& S_of(rfdtd_session* s) {
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
+// The far field without device allocation: the staged inputs and the result live in a file-scope
+// device array, so that callers that cannot allocate device memory (code under graph capture,
+// serving loops, harnesses that forbid it) can run the radiation integrals.  Every GPU the library
+// runs on holds its own copy of the array until the process ends.  In host code `scratch` names a
+// host-side stand-in; its device address on the current GPU comes from hipGetSymbolAddress.
+// ------------------------------------------------------------------------------------------------
+
+constexpr size_t kFFScratchBytes = (size_t)64 << 20;  // the workspace of rfdtd_nf2ff_static
+constexpr int kFFMaxFaces = 64;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kFFScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;                // held for a whole rfdtd_nf2ff_static call: calls share the array
+void* g_scratch_dev[kMaxDevices] = {};  // the array's device address on each GPU, taken once per device
+
+// The workspace's device address on the current GPU (looked up on the device's first call, with the
+// device current).  Called with g_scratch_mu held.
+char* scratch_address() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) RF_THROW(RFDTD_EDEVICE, "device index out of range");
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) RF_THROW(RFDTD_EDEVICE, "no device address for the far-field workspace");
+        g_scratch_dev[dev] = p;
+    }
+    return (char*)g_scratch_dev[dev];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------------
 
@@ -4695,6 +4732,85 @@ int rfdtd_nf2ff(int32_t n_faces, const rfdtd_ff_face* faces, int32_t n_frequenci
     });
     for (void* p : bufs) (void)hipFree(p);
     return rc;
+}
+
+// rfdtd_nf2ff with its inputs staged in, and its result taken from, the static workspace instead of
+// device allocations: the same kernel on the same values, so the results are identical.  Requests
+// whose staged bytes (each piece rounded up to 256 B) exceed the workspace are refused.
+int rfdtd_nf2ff_static(int32_t n_faces, const rfdtd_ff_face* faces, int32_t n_frequencies, const float* beta,
+                       int32_t n_theta, const float* theta, int32_t n_phi, const float* phi, float eta0, void* out) {
+    return guarded([&] {
+        if (n_faces < 1 || !faces || n_frequencies < 1 || !beta || n_theta < 1 || !theta || n_phi < 1 || !phi || !out)
+            RF_THROW(RFDTD_EINVAL, "nf2ff arguments missing");
+        if (n_faces > kFFMaxFaces) RF_THROW(RFDTD_EINVAL, "nf2ff_static takes at most 64 faces");
+        // plan the workspace first: nothing is copied unless everything fits
+        double need = 0;  // in double: no overflow however large the counts
+        auto round256 = [](double b) { return std::ceil(std::max(b, 4.0) / 256.0) * 256.0; };
+        need += round256((double)sizeof(FFFace) * n_faces);
+        double ncells = 0;
+        for (int k = 0; k < n_faces; ++k) {
+            const rfdtd_ff_face& F = faces[k];
+            if (F.axis < 0 || F.axis > 2 || F.n0 < 1 || F.n1 < 1 || !F.r0 || !F.r1 || !F.ds || !F.J || !F.M)
+                RF_THROW(RFDTD_EINVAL, "invalid far-field face");
+            const double plane = (double)F.n0 * (double)F.n1;
+            need += 3 * round256(plane * 4) + 2 * round256(3.0 * n_frequencies * plane * 8);
+            ncells += plane;
+        }
+        const double ndir_d = (double)n_theta * (double)n_phi;
+        need += round256(4.0 * n_frequencies) + round256(4.0 * n_theta) + round256(4.0 * n_phi);
+        need += round256(2.0 * n_frequencies * ndir_d * 8);
+        if (need > (double)kFFScratchBytes)
+            RF_THROW(RFDTD_EINVAL, "far-field request needs " + std::to_string((long long)need) +
+                                       " B of staging, beyond the 64 MiB static workspace");
+        if (ncells > INT32_MAX) RF_THROW(RFDTD_EINVAL, "too many far-field cells");
+        const int64_t ndir = (int64_t)n_theta * n_phi;
+        const int64_t blocks = (int64_t)n_frequencies * ((ndir + kFFChunk - 1) / kFFChunk);
+        if (blocks > INT32_MAX) RF_THROW(RFDTD_EINVAL, "too many far-field directions");
+
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        char* const base = scratch_address();
+        size_t used = 0;
+        auto take = [&](size_t bytes) {  // next 256-B aligned region of the workspace
+            char* p = base + used;
+            used += (std::max<size_t>(bytes, 4) + 255) & ~(size_t)255;
+            return p;
+        };
+        auto up = [&](const void* src, size_t bytes) {
+            char* d = take(bytes);
+            HIP_OK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+            return (const void*)d;
+        };
+        char* const d_faces = take(sizeof(FFFace) * n_faces);
+        FFFace ff[kFFMaxFaces];
+        int64_t cell0 = 0;
+        for (int k = 0; k < n_faces; ++k) {
+            const rfdtd_ff_face& F = faces[k];
+            const size_t plane = (size_t)F.n0 * F.n1;
+            ff[k] = FFFace{};
+            ff[k].axis = F.axis;
+            ff[k].n0 = F.n0;
+            ff[k].n1 = F.n1;
+            ff[k].pos = (float)F.position;
+            ff[k].cell0 = (int32_t)cell0;
+            ff[k].r0 = (const float*)up(F.r0, plane * 4);
+            ff[k].r1 = (const float*)up(F.r1, plane * 4);
+            ff[k].ds = (const float*)up(F.ds, plane * 4);
+            ff[k].J = (const float2*)up(F.J, 3 * (size_t)n_frequencies * plane * 8);
+            ff[k].M = (const float2*)up(F.M, 3 * (size_t)n_frequencies * plane * 8);
+            cell0 += (int64_t)plane;
+        }
+        HIP_OK(hipMemcpy(d_faces, ff, sizeof(FFFace) * n_faces, hipMemcpyHostToDevice));
+        const float* d_beta = (const float*)up(beta, (size_t)n_frequencies * 4);
+        const float* d_theta = (const float*)up(theta, (size_t)n_theta * 4);
+        const float* d_phi = (const float*)up(phi, (size_t)n_phi * 4);
+        const size_t out_bytes = 2 * (size_t)n_frequencies * ndir * 8;
+        float2* d_out = (float2*)take(out_bytes);  // every element is written by the kernel
+        if (used > kFFScratchBytes) RF_THROW(RFDTD_EINVAL, "far-field workspace plan exceeded");  // not reached
+        k_nf2ff<<<(int)blocks, kFFChunk>>>((const FFFace*)d_faces, n_faces, (int)cell0, n_frequencies, d_beta, d_theta,
+                                          n_theta, d_phi, n_phi, eta0, d_out);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(out, d_out, out_bytes, hipMemcpyDeviceToHost));
+    });
 }
 
 int rfdtd_copy_bandwidth(uint64_t bytes, int32_t iters, double* gbytes_per_s, double* ms_per_copy) {
