@@ -59,6 +59,11 @@ MEASURED_TRAFFIC = {
     }.items()
     for k, v in (("k_update_e", e), ("k_update_h", h))
 }
+# The raw-coefficient form (the reference formulation's bytes) at the shape the tuner picks for it.
+MEASURED_TRAFFIC.update({
+    ("dipole_slab", "raw", "k_update_e", (4, 128, 4)): 3976700000,
+    ("dipole_slab", "raw", "k_update_h", (4, 128, 4)): 4683300000,
+})
 
 
 def log(msg):
@@ -664,7 +669,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def timed_run(args, coef, kernel_timing=True):
+def timed_run(args, coef, kernel_timing=True, separate=False):
     """W warm-up steps then K timed steps of the single-grid workload on a resident, tuned session
     (coefficients `coef`: "auto", "raw" or "table").  Per-launch E and H kernel times come from HIP
     events on the session's stream around up to 64 evenly spread steps of the timed region."""
@@ -680,6 +685,8 @@ def timed_run(args, coef, kernel_timing=True):
     t_pack = time.perf_counter() - t0 - t_model
     flags = (_lib.KERNEL_TIMING if kernel_timing else 0) | {"auto": 0, "raw": _lib.COEF_RAW,
                                                              "table": _lib.COEF_TABLE}[coef]
+    if separate:  # one step per pass: every field and coefficient array read once per step
+        flags |= _lib.TUNE_SEPARATE
     t0 = time.perf_counter()
     s = Session(c, Nx, Ny, Nz, flags=flags)
     s.synchronize()
@@ -731,6 +738,11 @@ def kernel_roofline(run, kname, kbytes, kms, traffic, bytes_model):
 # values read and written, 3 neighbour totals read, 3 totals written: 72 B), the 4-byte entry index of
 # the cell and 1/8 B of probe bitmask.
 TABLE_BYTES = 72 + 4 + 0.125
+# The two-step pass advances two steps per launch and reads and writes the 12 split values of a cell
+# once (96 B), with the two entry indices (8 B) and two bitmask words per 32 cells, or the 30 raw
+# coefficient arrays (120 B) instead of the indices.  Halo rows of its tiles are re-read on top.
+TWO_STEP_TABLE_BYTES = 96 + 8 + 0.25
+TWO_STEP_RAW_BYTES = 96 + 120
 
 
 def plain_leg(args):
@@ -746,18 +758,28 @@ def plain_leg(args):
     dft_b = dft_bytes_per_step(model)
     step_bytes = cells * (E_BYTES + H_BYTES) + dft_b
     step_frac = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
-    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6))
+    variant = tuned.get("variant", 1) if tuned else 1
+    fused = variant in (2, 3, 5, 6, 7)
     table = info["coef_kind"] == "table"
-    # the dominant kernel: the fused E+H pass or the H update
-    if fused:
+    # the dominant kernel: the two-step pass (two cell-updates per cell and launch), the fused E+H
+    # pass or the H update
+    if variant == 7:
+        kname, kbytes, kms = "k_step2", (TWO_STEP_TABLE_BYTES if table else TWO_STEP_RAW_BYTES), e_ms
+    elif fused:
         kname, kbytes, kms = "k_step_fused", (2 * TABLE_BYTES if table else E_BYTES + H_BYTES), e_ms
     else:
         kname, kbytes, kms = "k_update_h", (TABLE_BYTES if table else H_BYTES), h_ms
     shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
     traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname, shape))
-    model_text = ("table form: 72 B of fields + 4 B entry index + 1/8 B probe bitmask per cell" if table
-                  else "reference formulation: 144 B per cell (6 split H read and written, 18 coefficients, "
-                       "3 E totals read, 3 H totals written)")
+    if variant == 7:
+        model_text = ("two-step pass, per cell and launch (2 cell-updates): 12 splits read and written once "
+                      "(96 B) + " + ("E and H entry indices 8 B + 2 probe bitmask words per 32 cells" if table
+                                     else "30 coefficients 120 B"))
+    elif table:
+        model_text = "table form: 72 B of fields + 4 B entry index + 1/8 B probe bitmask per cell"
+    else:
+        model_text = ("reference formulation: 144 B per cell (6 split H read and written, 18 coefficients, "
+                      "3 E totals read, 3 H totals written)")
     copy = None
     if not args.no_copy_probe:
         gbs, cms = _lib.copy_bandwidth(1 << 30, 20)
@@ -783,11 +805,12 @@ def plain_leg(args):
 
 def raw_leg(args):
     """The same workload with the 30 coefficient arrays read per cell (the reference formulation,
-    RFDTD_COEF_RAW): its H update moves the 144 B per cell that the roofline's byte count assumes,
-    so this is the like-for-like fraction of HBM peak."""
-    run = timed_run(args, "raw")
+    RFDTD_COEF_RAW) on the separate E and H kernels (one step per pass, RFDTD_TUNE_SEPARATE): its H
+    update moves the 144 B per cell that the roofline's byte count assumes, so this is the
+    like-for-like fraction of HBM peak."""
+    run = timed_run(args, "raw", separate=True)
     tuned, h_ms, e_ms, cells = run["tuned"], run["h_ms"], run["e_ms"], run["cells"]
-    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6))
+    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6, 7))
     if fused:
         kname, kbytes, kms = "k_step_fused", E_BYTES + H_BYTES, e_ms
     else:

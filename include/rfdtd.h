@@ -42,6 +42,8 @@ extern "C" {
 #define RFDTD_VERBOSE 0x40u       /* progress lines on stderr during long gradient passes */
 #define RFDTD_HOST_TABLE 0x80u    /* sessions from host arrays: build the coefficient table on the host
                                      (default: on the device; assembled sessions always do) */
+#define RFDTD_TUNE_SEPARATE 0x100u /* rfdtd_autotune tries the separate E and H kernels only (variants 0,
+                                      1, 4: one step per pass, every field and coefficient read once) */
 
 /*
  * Coefficient order: the keys of the reference's coefficient dictionary (solver.py:1473-1518):
@@ -154,11 +156,12 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
  * the automatic choice that gives enough workgroups to fill the chip); 0 keeps the current value. */
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
 /* Launch autotuner (no counterpart in the reference, whose CUDA path uses one fixed launch shape):
- * times every candidate {variant 1, 4, 0} x {block 128, 256, 512} and {variant 2, 3} x {x planes:
- * automatic, Nx/2 .. Nx/32} over trial_steps field-only steps of this session's grids, keeps the
- * fastest and returns it as config_out = {variant, block_threads, x_planes (0 = automatic)} with its
- * ms per step.  Only on a
- * session that has not run yet (step 0); fields stay zero, probes and monitors are untouched. */
+ * times every candidate {variant 1, 4, 0} x {block 128, 256, 512} x {x planes: automatic, Nx/2 ..
+ * Nx/128} and {variant 2, 3, 5, 6, 7} x {x planes} (RFDTD_TUNE_SEPARATE: the first set only) over
+ * trial_steps steps of this session's grids (with its probes and monitors applied when they are
+ * installed, restored afterwards), keeps the fastest and returns it as config_out = {variant,
+ * block_threads, x_planes (0 = automatic)} with its ms per step.  Only on a session that has not run
+ * yet (step 0); the fields are left zero. */
 int rfdtd_autotune(rfdtd_session* s, int32_t trial_steps, int32_t config_out[3], double* ms_per_step);
 /*
  * Adjoint gradients (no counterpart in the reference, which has no differentiation).
@@ -201,8 +204,12 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * 6 or 14 rows x 62 cells, neighbours through LDS, halos recomputed) that reads one buffer of a
  * per-instance pair and writes the other (the session then holds two field buffers per
  * instance); 4 variant 1 with nontemporal stores of the fields; 5 and 6 variants 2 and 3 with the
- * next x plane's H splits loaded one plane earlier and the coefficient values issued first.  Results
- * are identical; the choice only changes speed (the autotuner picks it). */
+ * next x plane's H splits loaded one plane earlier and the coefficient values issued first; 7 the
+ * two-step pass: steps n and n + 1 of a tile of 12 rows x 60 cells in one launch (two-cell halos
+ * recomputed, the x planes skewed by one), reading one buffer of the pair and writing the other, so
+ * that the fields are read and written once per two steps (single steps, and every step while a
+ * design-box history is recorded, go through the same pass one step at a time).  Results are
+ * identical; the choice only changes speed (the autotuner picks it). */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 
 /* Launch shape of the reverse pass of rfdtd_gradient(_batch) (no counterpart in the reference):

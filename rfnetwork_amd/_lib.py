@@ -25,6 +25,7 @@ QUIET = 0x10
 KERNEL_TIMING = 0x20
 VERBOSE = 0x40
 HOST_TABLE = 0x80
+TUNE_SEPARATE = 0x100  # the autotuner tries the separate E and H kernels only
 MONITOR_DFT64 = 0x1  # rfdtd_monitor.flags: running DFT accumulated in complex128
 
 # Coefficient keys in the order of the C ABI (rfnetwork/solver.py:1473-1518).

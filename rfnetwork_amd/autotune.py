@@ -126,6 +126,8 @@ def tune(session, grids, dtype, coef_kind, trial_steps=20, use_cache=True):
     cached).  The shape is applied to the session either way, with the reverse-pass shape when one
     is cached.  Returns the config with cached=True/False."""
     key = shape_key(grids, dtype, coef_kind)
+    if getattr(session, "flags", 0) & _lib.TUNE_SEPARATE:  # a restricted candidate set tunes apart
+        key += "|separate"
     trial_steps = trial_steps_for(grids, trial_steps, getattr(session, "Nt", None))
     _adjoint_from_cache(session, key, use_cache)
     cfg = _memory.get(key)

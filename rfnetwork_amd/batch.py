@@ -50,6 +50,7 @@ class Batch:
         for g, (_, _, _, K) in enumerate(self.grids):
             self.instances += [(g, k) for k in range(K)]
         self._keep = {}
+        self.flags = int(flags)
 
     @classmethod
     def assembled(cls, models, excitations=None, dtype=np.float32, flags=0):
@@ -76,6 +77,7 @@ class Batch:
         self.instances = []
         for g, (_, _, _, K) in enumerate(self.grids):
             self.instances += [(g, k) for k in range(K)]
+        self.flags = int(flags)
         self._keep = {}
         return self
 

@@ -2014,6 +2014,324 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
 }
 
 // ------------------------------------------------------------------------------------------------
+// two-step pass: time steps n and n + 1 in one pass over the fields (temporal blocking)
+//
+// A workgroup of kP2Ty + 4 wavefronts owns a tile of kP2Ty rows (y) x kP2Fz cells (z) and a range
+// [x0, x1) of x planes; lane l of wave w sits at y = y0 - 2 + w, z = z0 - 2 + l.  Each step shrinks
+// the region whose values are right by one cell on every side:
+//   step n:      E on w <= kP2Ty + 2, l <= 62;        H on 1 <= w, l and the same upper limits;
+//   step n + 1:  E on 1 <= w <= kP2Ty + 1, 1 <= l <= 61;  H (owned, stored) on 2 <= w, l likewise.
+// In x the pass is skewed by one plane: iteration i takes plane i through step n and plane i - 1
+// through step n + 1, the x neighbours carried in registers, the y/z neighbours exchanged through
+// LDS (three barriers per plane).  Step n runs on planes x0 - 2 .. x1: the planes outside [x0, x1)
+// and the halo lanes compute values the tile needs but does not own.  Only step n + 1's splits are
+// stored, into the other buffer of the instance's pair (a tile's halo reads must see the old state);
+// step n's monitor samples are written here by the lanes that own the cells, step n + 1's by
+// k_monitors from the stored splits.  Sources are applied by every lane that computes a cell (halo
+// copies included; the samples come from the pristine series), records are written by the owner.
+// With two == 0 the pass advances step n alone and stores it (planes x0 - 1 .. x1 - 1).  Every value
+// is formed by the same operations in the same order as in the separate kernels.
+// ------------------------------------------------------------------------------------------------
+
+constexpr int kP2Ty = 12;   // rows owned by a two-step tile (its workgroup has kP2Ty + 4 wavefronts)
+constexpr int kP2Fz = 60;   // z cells owned by a two-step tile
+constexpr int kP2Mon = 64;  // monitors a two-step pass samples itself, at most
+
+// The monitor samples of one owned cell at step n: every monitor of this instance whose plane holds
+// the cell and whose field is component fbase + 0..2 takes v[component] (k_monitors' placement).
+template <typename T>
+__device__ __forceinline__ void p2_sample(const MonitorDev* __restrict__ mons, int nmon, int inst, int fbase, int x,
+                                          int y, int z, int n, int slot, T vx, T vy, T vz) {
+    for (int k = 0; k < nmon; ++k) {
+        const MonitorDev& M = mons[k];
+        const int f = M.field - fbase;
+        if (M.inst != inst || f < 0 || f > 2 || n % M.n_step != 0) continue;
+        int a, b;
+        if (M.axis == 0) {
+            if (x != M.pos) continue;
+            a = y; b = z;
+        } else if (M.axis == 1) {
+            if (y != M.pos) continue;
+            a = x; b = z;
+        } else {
+            if (z != M.pos) continue;
+            a = x; b = y;
+        }
+        const int64_t q = (int64_t)a * M.N2 + b;
+        const T v = f == 0 ? vx : f == 1 ? vy : vz;
+        if (M.nf == 0) {
+            const int64_t m = n / M.n_step;
+            ((GLOBAL_T(T)*)M.values)[(M.ring ? m % M.ring : m) * M.N1N2 + q] = v;
+        } else {
+            ((GLOBAL_T(T)*)M.stage)[(int64_t)slot * M.N1N2 + q] = v;
+        }
+    }
+}
+
+template <typename T, class CA>
+__device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, int inst, int x0, int x1, int y0,
+                                              int z0, int n, bool io, bool two, const MonitorDev* __restrict__ mons,
+                                              int nmon, int slot, T* sm) {
+    using G = GLOBAL_T(T);
+    constexpr int TY = kP2Ty;
+    constexpr int NT = (TY + 4) * 64;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const int Nx = I.Nx, Ny = I.Ny, Nz = I.Nz;
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int y = y0 - 2 + w, z = z0 - 2 + l;
+    const bool valid = y >= 0 && y < Ny && z >= 0 && z < Nz;
+    const bool mAe = valid && w <= TY + 2 && l <= 62;    // E of step n right here
+    const bool mAh = mAe && w >= 1 && l >= 1;            // H of step n
+    const bool mBe = mAh && w <= TY + 1 && l <= 61;      // E of step n + 1
+    const bool own = mBe && w >= 2 && l >= 2;            // H of step n + 1; stored, recorded
+    const bool yin = y < Ny - 1, zin = z < Nz - 1;
+    T* const s0 = sm;              // [3][NT] input H totals, plane i
+    T* const s1e = sm + 3 * NT;    // [3][NT] E totals after step n, plane i
+    T* const s1h = sm + 6 * NT;    // [2][3][NT] H totals after step n, by plane parity
+    T* const s2e = sm + 12 * NT;   // [3][NT] E totals after step n + 1, plane i - 1
+    const Fields<T>& A = I.F;
+    const Fields<T>& B = I.G;
+    const int a0 = two ? max(x0 - 2, 0) : max(x0 - 1, 0);  // first plane through step n
+    const int a1 = two ? min(x1 + 1, Nx) : x1;             // one past the last
+    const int b0 = max(x0 - 1, 0);                         // first plane through step n + 1 (E)
+    const int ie = two ? x1 + 1 : x1;                      // iterations i in [a0, ie)
+    const uint32_t p = valid ? ((uint32_t)y * (uint32_t)Nz + (uint32_t)z) : 0u;
+    const uint32_t base = ((uint32_t)a0 * (uint32_t)Ny * (uint32_t)Nz + p) * S;
+    auto off = [&](int x) { return base + (uint32_t)(x - a0) * sx; };
+    auto load_h = [&](int x) {
+        Split6<T> h{};
+        if (valid && x < Nx) {
+            const uint32_t b = off(x);
+            h.xy = ldg((const G*)A.hx_y, b); h.xz = ldg((const G*)A.hx_z, b); h.yz = ldg((const G*)A.hy_z, b);
+            h.yx = ldg((const G*)A.hy_x, b); h.zx = ldg((const G*)A.hz_x, b); h.zy = ldg((const G*)A.hz_y, b);
+        }
+        return h;
+    };
+    auto load_e = [&](int x) {
+        Split6<T> e{};
+        if (mAe && x < a1) {
+            const uint32_t b = off(x);
+            e.xy = ldg((const G*)A.ex_y, b); e.xz = ldg((const G*)A.ex_z, b); e.yz = ldg((const G*)A.ey_z, b);
+            e.yx = ldg((const G*)A.ey_x, b); e.zx = ldg((const G*)A.ez_x, b); e.zy = ldg((const G*)A.ez_y, b);
+        }
+        return e;
+    };
+    auto store = [&](const Fields<T>& F, uint32_t b, const Split6<T>& e, bool h) {
+        if (h) {
+            stg((G*)F.hx_y, b, e.xy); stg((G*)F.hx_z, b, e.xz); stg((G*)F.hy_z, b, e.yz);
+            stg((G*)F.hy_x, b, e.yx); stg((G*)F.hz_x, b, e.zx); stg((G*)F.hz_y, b, e.zy);
+        } else {
+            stg((G*)F.ex_y, b, e.xy); stg((G*)F.ex_z, b, e.xz); stg((G*)F.ey_z, b, e.yz);
+            stg((G*)F.ey_x, b, e.yx); stg((G*)F.ez_x, b, e.zx); stg((G*)F.ez_y, b, e.zy);
+        }
+    };
+    const bool eprobes = io && I.negroups, hprobes = io && I.nhgroups;
+    auto cell_of = [&](int x) { return ((int64_t)x * Ny + y) * Nz + z; };
+    auto ebit = [&](int64_t c) { return ((I.ebits[c >> 5] >> (c & 31)) & 1u) != 0; };
+    auto hbit = [&](int64_t c) { return ((I.hbits[c >> 5] >> (c & 31)) & 1u) != 0; };
+
+    Split6<T> h0 = load_h(a0), e0 = load_e(a0);  // input H and E splits of plane i
+    uint32_t ek = mAe ? C.ekey(off(a0)) : 0u, hk = mAh ? C.hkey(off(a0)) : 0u;
+    Split6<T> e1p{}, h1p{};                      // splits after step n, plane i - 1
+    uint32_t ekp = 0u, hkp = 0u;                 // their coefficient keys
+    T e1y_m = T(0), e1z_m = T(0);                // Ey, Ez after step n, plane i - 1
+    T e2y_m = T(0), e2z_m = T(0);                // Ey, Ez after step n + 1, plane i - 2
+    for (int i = a0; i < ie; ++i) {
+        const bool doA = i < a1;  // uniform over the workgroup, as is every branch around a barrier
+        const int j = i - 1;
+        const bool doB = two && j >= b0 && j < x1;
+        const bool recA = own && i >= x0 && i < x1;
+        // the next plane's inputs, in flight during this plane's work
+        const Split6<T> hN = load_h(i + 1);
+        Split6<T> eN{};
+        uint32_t ekN = 0u, hkN = 0u;
+        if (i + 1 < a1) {
+            eN = load_e(i + 1);
+            if (mAe) ekN = C.ekey(off(i + 1));
+            if (mAh) hkN = C.hkey(off(i + 1));
+        }
+        // ---- step n, plane i: E
+        const T h0x = h0.xy + h0.xz, h0y = h0.yz + h0.yx, h0z = h0.zx + h0.zy;
+        s0[tid] = h0x;
+        s0[NT + tid] = h0y;
+        s0[2 * NT + tid] = h0z;
+        __syncthreads();
+        HCoef<T> ch{};
+        if (doA && mAh) ch = C.h_at(hk);
+        Split6<T> e1 = e0;
+        T e1x = T(0), e1y = T(0), e1z = T(0);
+        if (doA && mAe) {
+            const ECoef<T> c = C.e_at(ek);
+            const bool xin = i < Nx - 1;
+            const T hz_yn = yin ? s0[2 * NT + tid + 64] : T(0), hx_yn = yin ? s0[tid + 64] : T(0);
+            const T hy_zn = zin ? s0[NT + tid + 1] : T(0), hx_zn = zin ? s0[tid + 1] : T(0);
+            const T hy_xn = xin ? hN.yz + hN.yx : T(0), hz_xn = xin ? hN.zx + hN.zy : T(0);
+            if (yin) e1.xy = c.ca_xy * e1.xy + c.cb_xy * (hz_yn - h0z);
+            if (zin) e1.xz = c.ca_xz * e1.xz + c.cb_xz * (hy_zn - h0y);
+            if (zin) e1.yz = c.ca_yz * e1.yz + c.cb_yz * (hx_zn - h0x);
+            if (xin) e1.yx = c.ca_yx * e1.yx + c.cb_yx * (hz_xn - h0z);
+            if (xin) e1.zx = c.ca_zx * e1.zx + c.cb_zx * (hy_xn - h0y);
+            if (yin) e1.zy = c.ca_zy * e1.zy + c.cb_zy * (hx_yn - h0x);
+            if (eprobes) {
+                const int64_t cell = cell_of(i);
+                if (ebit(cell)) inject_e(I, cell, n, recA, e1.xy, e1.xz, e1.yz, e1.yx, e1.zx, e1.zy);
+            }
+            e1x = e1.xy + e1.xz;
+            e1y = e1.yz + e1.yx;
+            e1z = e1.zx + e1.zy;
+            if (recA) {
+                if (!two) store(B, off(i), e1, false);
+                else if (nmon) p2_sample(mons, nmon, inst, EX, i, y, z, n, slot, e1x, e1y, e1z);
+            }
+        }
+        s1e[tid] = e1x;
+        s1e[NT + tid] = e1y;
+        s1e[2 * NT + tid] = e1z;
+        __syncthreads();
+        // ---- step n, plane i: H
+        Split6<T> h1 = h0;
+        T h1x = T(0), h1y = T(0), h1z = T(0);
+        if (doA && mAh) {
+            const T ez_ym = y > 0 ? s1e[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s1e[tid - 64] : T(0);
+            const T ey_zm = z > 0 ? s1e[NT + tid - 1] : T(0), ex_zm = z > 0 ? s1e[tid - 1] : T(0);
+            h1.xy = ch.da_xy * h0.xy + (ch.db_xy2 * e1z - ch.db_xy1 * ez_ym);
+            h1.xz = ch.da_xz * h0.xz + (ch.db_xz2 * e1y - ch.db_xz1 * ey_zm);
+            h1.yz = ch.da_yz * h0.yz + (ch.db_yz2 * e1x - ch.db_yz1 * ex_zm);
+            h1.yx = ch.da_yx * h0.yx + (ch.db_yx2 * e1z - ch.db_yx1 * e1z_m);
+            h1.zx = ch.da_zx * h0.zx + (ch.db_zx2 * e1y - ch.db_zx1 * e1y_m);
+            h1.zy = ch.da_zy * h0.zy + (ch.db_zy2 * e1x - ch.db_zy1 * ex_ym);
+            if (hprobes) {
+                const int64_t cell = cell_of(i);
+                if (hbit(cell))
+                    inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell, n, recA, h1.xy, h1.xz, h1.yz,
+                                h1.yx, h1.zx, h1.zy);
+            }
+            h1x = h1.xy + h1.xz;
+            h1y = h1.yz + h1.yx;
+            h1z = h1.zx + h1.zy;
+            if (recA) {
+                if (!two) store(B, off(i), h1, true);
+                else if (nmon) p2_sample(mons, nmon, inst, HX, i, y, z, n, slot, h1x, h1y, h1z);
+            }
+        }
+        if (two) {
+            T* const sh = s1h + (i & 1) * 3 * NT;
+            sh[tid] = h1x;
+            sh[NT + tid] = h1y;
+            sh[2 * NT + tid] = h1z;
+        }
+        e1y_m = e1y;
+        e1z_m = e1z;
+        // ---- step n + 1, plane j = i - 1 (its H-after-step-n totals were exchanged an iteration ago)
+        if (doB) {
+            const T* const shj = s1h + (j & 1) * 3 * NT;
+            const bool recB = own && j >= x0;
+            Split6<T> e2 = e1p;
+            T e2x = T(0), e2y = T(0), e2z = T(0);
+            if (mBe) {
+                const ECoef<T> c = C.e_at(ekp);
+                const bool xin = j < Nx - 1;
+                const T hx_c = h1p.xy + h1p.xz, hy_c = h1p.yz + h1p.yx, hz_c = h1p.zx + h1p.zy;
+                const T hz_yn = yin ? shj[2 * NT + tid + 64] : T(0), hx_yn = yin ? shj[tid + 64] : T(0);
+                const T hy_zn = zin ? shj[NT + tid + 1] : T(0), hx_zn = zin ? shj[tid + 1] : T(0);
+                const T hy_xn = xin ? h1y : T(0), hz_xn = xin ? h1z : T(0);  // plane i, this iteration's step n
+                if (yin) e2.xy = c.ca_xy * e2.xy + c.cb_xy * (hz_yn - hz_c);
+                if (zin) e2.xz = c.ca_xz * e2.xz + c.cb_xz * (hy_zn - hy_c);
+                if (zin) e2.yz = c.ca_yz * e2.yz + c.cb_yz * (hx_zn - hx_c);
+                if (xin) e2.yx = c.ca_yx * e2.yx + c.cb_yx * (hz_xn - hz_c);
+                if (xin) e2.zx = c.ca_zx * e2.zx + c.cb_zx * (hy_xn - hy_c);
+                if (yin) e2.zy = c.ca_zy * e2.zy + c.cb_zy * (hx_yn - hx_c);
+                if (eprobes) {
+                    const int64_t cell = cell_of(j);
+                    if (ebit(cell)) inject_e(I, cell, n + 1, recB, e2.xy, e2.xz, e2.yz, e2.yx, e2.zx, e2.zy);
+                }
+                e2x = e2.xy + e2.xz;
+                e2y = e2.yz + e2.yx;
+                e2z = e2.zx + e2.zy;
+                if (recB) store(B, off(j), e2, false);
+            }
+            s2e[tid] = e2x;
+            s2e[NT + tid] = e2y;
+            s2e[2 * NT + tid] = e2z;
+            __syncthreads();
+            if (recB) {
+                const HCoef<T> c = C.h_at(hkp);
+                const T ez_ym = y > 0 ? s2e[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s2e[tid - 64] : T(0);
+                const T ey_zm = z > 0 ? s2e[NT + tid - 1] : T(0), ex_zm = z > 0 ? s2e[tid - 1] : T(0);
+                Split6<T> h2;
+                h2.xy = c.da_xy * h1p.xy + (c.db_xy2 * e2z - c.db_xy1 * ez_ym);
+                h2.xz = c.da_xz * h1p.xz + (c.db_xz2 * e2y - c.db_xz1 * ey_zm);
+                h2.yz = c.da_yz * h1p.yz + (c.db_yz2 * e2x - c.db_yz1 * ex_zm);
+                h2.yx = c.da_yx * h1p.yx + (c.db_yx2 * e2z - c.db_yx1 * e2z_m);
+                h2.zx = c.da_zx * h1p.zx + (c.db_zx2 * e2y - c.db_zx1 * e2y_m);
+                h2.zy = c.da_zy * h1p.zy + (c.db_zy2 * e2x - c.db_zy1 * ex_ym);
+                if (hprobes) {
+                    const int64_t cell = cell_of(j);
+                    if (hbit(cell))
+                        inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell, n + 1, true, h2.xy, h2.xz,
+                                    h2.yz, h2.yx, h2.zx, h2.zy);
+                }
+                store(B, off(j), h2, true);
+            }
+            e2y_m = e2y;
+            e2z_m = e2z;
+        }
+        e1p = e1;
+        h1p = h1;
+        ekp = ek;
+        hkp = hk;
+        h0 = hN;
+        e0 = eN;
+        ek = ekN;
+        hk = hkN;
+    }
+}
+
+// two != 0: steps n and n + 1 (n from base/local); the phasor monitors' sample rows of step n's
+// staging slot are written by logical workgroup 0 (k_monitors writes step n + 1's).
+template <typename T>
+__global__ void __launch_bounds__((kP2Ty + 4) * 64) k_step2(const InstDev<T>* __restrict__ insts,
+                                                            const MemberDev* __restrict__ mem, int nmem, int total,
+                                                            const int* base, int local, int io, int two,
+                                                            const MonitorDev* __restrict__ mons, int nmon, int slot,
+                                                            int* __restrict__ rows) {
+    __shared__ int s_member;
+    __shared__ T sm[15 * (kP2Ty + 4) * 64];
+    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    if (threadIdx.x == 0) {
+        int m = 0;
+        if (L < total)
+            while (m + 1 < nmem && mem[m + 1].tile0 <= L) ++m;
+        s_member = m;
+    }
+    __syncthreads();
+    if (L >= total) return;  // uniform over the workgroup: no barrier is skipped by part of it
+    const int n = step_of(base, local);
+    const int nm = (two && io) ? nmon : 0;
+    if (nm && L == 0 && (int)threadIdx.x < nm) {
+        const MonitorDev& M = mons[threadIdx.x];
+        if (M.nf) rows[M.id * kStageSlots + slot] = n % M.n_step == 0 ? n / M.n_step : -1;
+    }
+    const int m = __builtin_amdgcn_readfirstlane(s_member);
+    const MemberDev M = mem[m];
+    const int local_t = L - M.tile0;
+    const int e = local_t % M.K, tile = local_t / M.K;
+    const int inst = __builtin_amdgcn_readfirstlane(M.inst0 + e);
+    const InstDev<T>* I = insts + inst;
+    const int tz_n = (I->Nz + kP2Fz - 1) / kP2Fz;
+    const int pb = tile % I->npb, xc = tile / I->npb;
+    const int y0 = (pb / tz_n) * kP2Ty, z0 = (pb % tz_n) * kP2Fz;
+    const int x0 = xc * I->xplanes, x1 = min(I->Nx, x0 + I->xplanes);
+    if (I->coef_kind == 1)
+        two_step_tile<T, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, inst, x0, x1, y0, z0, n,
+                                       io != 0, two != 0, mons, nm, slot, sm);
+    else
+        two_step_tile<T, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, inst, x0, x1, y0, z0, n, io != 0, two != 0, mons, nm,
+                                     slot, sm);
+}
+
+// ------------------------------------------------------------------------------------------------
 // far field (NF2FF): radiation integrals of the equivalent surface currents of a closed box
 // ------------------------------------------------------------------------------------------------
 
@@ -2216,6 +2534,7 @@ struct Session {
     int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
     bool fused_deep = false;   // fused step with E one plane ahead of H (one barrier per plane)
+    bool two_step = false;     // two-step pass (k_step2): pairs of steps per launch; fused = its rows
     // streamed time-domain monitors: a device ring of stream_ring sample planes per monitor, copied
     // to the caller's host arrays on copy_stream after every run segment
     int stream_ring = 0;
@@ -2822,6 +3141,7 @@ void assemble_coefficients(Member& M, uint32_t flags, const rfdtd_assembly& a) {
 // launch on 256 CUs), as many planes as possible for register reuse along x.
 // Workgroups per x plane of a member: (y,z) columns per workgroup, or fused tiles.
 int64_t tiles_per_plane(const Session& s, const Member& M) {
+    if (s.two_step) return (int64_t)((M.Ny + kP2Ty - 1) / kP2Ty) * ((M.Nz + kP2Fz - 1) / kP2Fz);
     if (s.fused) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kFz - 1) / kFz);
     return ((int64_t)M.Ny * M.Nz + s.block - 1) / s.block;
 }
@@ -3068,6 +3388,45 @@ void refresh(Session& s) {
     s.drop_graph();
 }
 
+// Pairs of steps go through the two-step pass when its variant is chosen, no design-box history is
+// being recorded (it needs H at every step) and the pass can sample every active monitor itself.
+bool pairs_ok(const Session& s) { return s.two_step && s.hist.empty() && s.n_active <= kP2Mon; }
+
+// The two-step pass over every instance (two: steps local and local + 1, else step local alone),
+// from the current buffer of each pair into the other.
+template <typename T>
+void launch_step2(Session& s, hipStream_t st, const int* base, int local, bool two, int slot, hipEvent_t* ev) {
+    const int grid = (s.total_tiles + 7) & ~7;
+    const InstDev<T>* di = (const InstDev<T>*)(s.parity ? s.d_insts2 : s.d_insts);
+    const int io = s.tuning ? 0 : 1;
+    if (ev) HIP_OK(hipEventRecord(ev[0], st));
+    k_step2<T><<<grid, (kP2Ty + 4) * 64, 0, st>>>(di, s.d_members, (int)s.members.size(), s.total_tiles, base, local,
+                                                  io, two ? 1 : 0, s.d_mons, s.n_active, slot, s.d_rows);
+    if (ev) {
+        HIP_OK(hipEventRecord(ev[1], st));
+        HIP_OK(hipEventRecord(ev[2], st));
+        HIP_OK(hipEventRecord(ev[3], st));
+    }
+    s.parity ^= 1;
+    s.totals_stale = true;
+}
+
+// Steps local and local + 1 in one two-step pass.  slot_a, slot_b: their staging slots; the pass
+// samples step local's monitors itself, k_monitors step local + 1's from the stored splits; flush > 0
+// accumulates staged slots [0, flush) afterwards.
+template <typename T>
+void launch_pair(Session& s, hipStream_t st, const int* base, int local, int slot_a, int slot_b, int flush,
+                 hipEvent_t* ev = nullptr) {
+    launch_step2<T>(s, st, base, local, true, slot_a, ev);
+    if (!s.tuning && s.n_active) {
+        const InstDev<T>* di = (const InstDev<T>*)(s.parity ? s.d_insts2 : s.d_insts);
+        k_monitors<T><<<s.mon_blocks, 256, 0, st>>>(di, s.d_mons, s.n_active, base, local + 1, slot_b, s.d_rows, 0);
+        if (flush && s.flush_blocks)
+            k_monitor_flush<T><<<(s.flush_blocks + 7) & ~7, 256, 0, st>>>(s.d_mons, s.n_active, s.flush_blocks, flush,
+                                                                       s.d_rows);
+    }
+}
+
 // One time step.  slot: the step's staging slot for running DFTs; flush > 0 accumulates staged
 // slots [0, flush) after this step's sample.
 template <typename T>
@@ -3084,7 +3443,10 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     if (io && s.hist_blocks && s.fused)
         k_save_h<T><<<s.hist_blocks, 256, 0, st>>>(di, (const BoxDev<T>*)s.d_hist_boxes, s.d_hist_h0, (int)s.hist.size(),
                                                   local, base);
-    if (s.fused) {
+    if (s.two_step) {
+        launch_step2<T>(s, st, base, local, false, slot, ev);  // one step through the two-step pass
+        di = desc[s.parity];
+    } else if (s.fused) {
         // one pass writes the other buffer of each pair; monitors then sample it
         if (ev) HIP_OK(hipEventRecord(ev[0], st));
         if (s.fused == 14 && s.fused_deep)
@@ -3167,7 +3529,11 @@ void save_checkpoints(Session& s, int slot, hipStream_t st) {
 }
 
 // Steps per captured chunk: even for the fused step, so that a replay ends on the buffer it began on.
-int chunk_steps(const Session& s) { return s.fused ? std::max(2, s.graph_steps & ~1) : s.graph_steps; }
+// (the two-step pass flips the buffers once per pair: a multiple of 4 steps ends where it began)
+int chunk_steps(const Session& s) {
+    if (s.two_step) return std::max(4, s.graph_steps & ~3);
+    return s.fused ? std::max(2, s.graph_steps & ~1) : s.graph_steps;
+}
 
 // Capture one chunk of steps starting from the current parity (the graph for that parity).
 template <typename T>
@@ -3178,10 +3544,19 @@ void capture_graph(Session& s) {
     hipStream_t cs;
     HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < n; ++k) {
+    const bool pairs = pairs_ok(s);
+    for (int k = 0; k < n;) {
         const int slot = k % kStageSlots;
-        const bool last = slot == kStageSlots - 1 || k == n - 1;
-        launch_step<T>(s, cs, s.d_step_base, k, slot, last ? slot + 1 : 0);
+        if (pairs && k + 1 < n) {
+            const int slot2 = (k + 1) % kStageSlots;
+            const bool last = slot2 == kStageSlots - 1 || k + 1 == n - 1;
+            launch_pair<T>(s, cs, s.d_step_base, k, slot, slot2, last ? slot2 + 1 : 0);
+            k += 2;
+        } else {
+            const bool last = slot == kStageSlots - 1 || k == n - 1;
+            launch_step<T>(s, cs, s.d_step_base, k, slot, last ? slot + 1 : 0);
+            k += 1;
+        }
     }
     hipError_t ce = hipStreamEndCapture(cs, &s.graph_def[p]);
     (void)hipStreamDestroy(cs);
@@ -3205,10 +3580,10 @@ void run_chunk(Session& s, int len, int offset, hipStream_t st, bool graphs, boo
         }
         if (s.fused) s.totals_stale = true;
     }
-    for (int d = 0; done < len; ++done, ++d) {
+    const bool pairs = pairs_ok(s);
+    for (int d = 0; done < len;) {
         const int64_t n = s.step + offset + done;
         const int slot = d % kStageSlots;
-        const bool last = slot == kStageSlots - 1 || done == len - 1;
         hipEvent_t* ev = nullptr;
         if (timing && (offset + done) % stride == 0 && s.n_timed < Session::kTimedSteps) {
             if (s.kev.empty()) {
@@ -3217,7 +3592,18 @@ void run_chunk(Session& s, int len, int offset, hipStream_t st, bool graphs, boo
             }
             ev = &s.kev[4 * s.n_timed++];
         }
-        launch_step<T>(s, st, nullptr, (int)n, slot, last ? slot + 1 : 0, ev);
+        if (pairs && done + 1 < len) {  // (d is even here: the staging slots of a pair never straddle a flush)
+            const int slot2 = (d + 1) % kStageSlots;
+            const bool last = slot2 == kStageSlots - 1 || done + 1 == len - 1;
+            launch_pair<T>(s, st, nullptr, (int)n, slot, slot2, last ? slot2 + 1 : 0, ev);
+            done += 2;
+            d += 2;
+        } else {
+            const bool last = slot == kStageSlots - 1 || done == len - 1;
+            launch_step<T>(s, st, nullptr, (int)n, slot, last ? slot + 1 : 0, ev);
+            done += 1;
+            d += 1;
+        }
     }
 }
 
@@ -3368,17 +3754,18 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const int v0 = s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
+    const int v0 = s.two_step ? 7 : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
                  : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
     const int blk0 = s.block;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
     // variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row tiles, 3 fused 14-row tiles,
-    // 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead of H
+    // 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead of H, 7 the two-step pass
     auto apply = [&](int v, int blk, const std::vector<int>& xr) {
         s.pipelined = v != 0;
-        s.fused = (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
-        s.fused_deep = v >= 5;
+        s.two_step = v == 7;
+        s.fused = v == 7 ? kP2Ty : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
+        s.fused_deep = v == 5 || v == 6;
         s.nontemporal = v == 4;
         s.block = blk;
         for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
@@ -3404,10 +3791,12 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         k_fill_pattern<T><<<1024, 256, 0, st>>>((T*)I.d_fields, (int64_t)(I.fields_bytes / sizeof(T)), 12345u);
     HIP_OK(hipGetLastError());
     try {
-        for (int v : {1, 4, 0, 2, 3, 5, 6})
+        const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
+        for (int v : {1, 4, 0, 2, 3, 5, 6, 7})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
                     const bool fused_v = v == 2 || v == 3 || v >= 5;
+                    if (fused_v && separate_only) continue;
                     if (fused_v && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
                     apply(v, blk, std::vector<int>(s.members.size(), xp));
                     refresh<T>(s);
@@ -4419,10 +4808,11 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 6) RF_THROW(RFDTD_EINVAL, "variant must be in 0..6");
+        if (variant < 0 || variant > 7) RF_THROW(RFDTD_EINVAL, "variant must be in 0..7");
         S.pipelined = variant != 0;
-        S.fused = (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
-        S.fused_deep = variant >= 5;
+        S.two_step = variant == 7;
+        S.fused = variant == 7 ? kP2Ty : (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
+        S.fused_deep = variant == 5 || variant == 6;
         S.nontemporal = variant == 4;
         S.dirty = true;
         S.graph_dirty = true;

@@ -40,6 +40,7 @@ class Session:
                                    ctypes.cast(ptrs, ctypes.c_void_p), f), "rfdtd_create")
         self._keep = []
         self.Nt = None
+        self.flags = int(flags)
 
     @classmethod
     def assembled(cls, model, dtype=np.float32, flags=0):
@@ -56,6 +57,7 @@ class Session:
         self._L, self._h = assembly.create(model, self.dtype, flags)
         self._keep = []
         self.Nt = None
+        self.flags = int(flags)
         return self
 
     def coefficients(self, box=None):
@@ -218,7 +220,8 @@ class Session:
         """Kernel variant: 1 (default) separate E and H kernels issuing the next x plane's loads early,
         0 the same one plane at a time, 2 and 3 the fused E+H pass with 6- or 14-row tiles (two field
         buffers per instance), 4 variant 1 with nontemporal field stores, 5 and 6 variants 2 and 3
-        with deeper load pipelining.  Results are identical."""
+        with deeper load pipelining, 7 the two-step pass (steps n and n + 1 of a 12-row x 60-cell
+        tile per launch, two field buffers per instance).  Results are identical."""
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def close(self):

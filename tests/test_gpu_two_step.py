@@ -1,0 +1,146 @@
+"""The two-step pass (variant 7): time steps n and n + 1 of a tile in one launch, with two-cell halos
+recomputed and the x planes skewed by one; step n's monitor samples are taken inside the pass.  Its
+results must equal the oracle's and the other variants' bit for bit, whatever the tiling, x planes,
+chunking (odd chunk lengths end on a single step), buffer parity, graph replay or batch, and a
+forward run recorded for a gradient (checkpoints between pairs; a design-box history, which runs
+the pass one step at a time) must give the same gradient."""
+import numpy as np
+import pytest
+
+from rfnetwork_amd import _lib, workloads
+from rfnetwork_amd.batch import Batch
+from rfnetwork_amd.session import Session
+
+from helpers import run_oracle
+from test_gpu_fused import _assert_equal_to_oracle, _run
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("flags", [0, _lib.COEF_RAW, _lib.NO_GRAPHS, _lib.COEF_RAW | _lib.NO_GRAPHS])
+def test_two_step_tiny_fp32_bit_exact(hip_ready, flags):
+    """PML, dielectric, conductor, two ports, a voltage probe, time and phasor monitors on every axis;
+    chunks of 37 and 23 steps (graph chunks of 8 steps, then pairs, then a single step)."""
+    print(f"[case] two-step tiny flags={flags}", flush=True)
+    model = workloads.tiny_model(Nt=60)
+    p, m, f = _run(model, np.float32, flags, 7, chunks=[37, 23], launch=(0, 3, 10))
+    _assert_equal_to_oracle(model, np.float32, p, m, f)
+
+
+def test_two_step_tiny_fp64_bit_exact(hip_ready):
+    print("[case] two-step tiny fp64", flush=True)
+    model = workloads.tiny_model(Nt=50)
+    p, m, f = _run(model, np.float64, 0, 7, chunks=[21, 29], launch=(0, 4, 6))
+    _assert_equal_to_oracle(model, np.float64, p, m, f)
+
+
+def test_two_step_interleaves_with_other_variants(hip_ready):
+    print("[case] two-step interleave", flush=True)
+    model = workloads.tiny_model(Nt=60)
+    p, m, f = _run(model, np.float32, 0, 7, chunks=[13, 20, 7, 11, 9], switch=[7, 1, 2, 7, 0, 6])
+    _assert_equal_to_oracle(model, np.float32, p, m, f)
+
+
+def test_two_step_many_tiles_and_planes(hip_ready):
+    """Several tiles along y (12 rows) and z (60 cells) and x chunks of 1 .. all planes: the probes,
+    monitors and fields equal the separate kernels'."""
+    print("[case] two-step tiles", flush=True)
+    model = workloads.tiny_model(Nt=30, nx=23, ny=31, nz=131)
+    p0, m0, ref = _run(model, variant=1)
+    ref_p = [np.array(p["values"]) for p in p0]
+    ref_m = [np.array(m["values"]) for m in m0]
+    for xp in (1, 2, 3, 5, 23):
+        p, m, f = _run(model, variant=7, launch=(0, xp, 8), chunks=[11, 19])
+        for k in ref:
+            np.testing.assert_array_equal(f[k], ref[k], err_msg=f"{k} x planes {xp}")
+        for a, b in zip(p, ref_p):
+            np.testing.assert_array_equal(a["values"], b, err_msg=f"probe, x planes {xp}")
+        for k, (a, b) in enumerate(zip(m, ref_m)):
+            np.testing.assert_array_equal(a["values"], b, err_msg=f"monitor {k}, x planes {xp}")
+
+
+def test_two_step_batch_matches_oracle(hip_ready):
+    print("[case] two-step batch", flush=True)
+    models = [workloads.tiny_model(Nt=40, nx=14, ny=11, nz=9, seed=7),
+              workloads.tiny_model(Nt=40, nx=19, ny=26, nz=70, seed=8)]
+    grids, inputs = [], []
+    for m in models:
+        grids.append((m.coefficient_arrays(gpu=True), m.Nx, m.Ny, m.Nz, 1))
+        _, probes, monitors, _, *_ = m.pack(gpu=True, coefficients=False)
+        inputs.append((probes, monitors))
+    with Batch(grids) as b:
+        b.set_variant(7)
+        for k, (probes, monitors) in enumerate(inputs):
+            b.set_sources(k, probes, monitors, 40)
+        b.run(25)
+        b.run(15)
+        b.synchronize()
+        b.read()
+        fields = [b.fields(k) for k in range(2)]
+    for m, (probes, monitors), f in zip(models, inputs, fields):
+        _assert_equal_to_oracle(m, np.float32, probes, monitors, f)
+
+
+def test_two_step_streamed_time_monitors(hip_ready):
+    """Time-domain monitors streamed through small device rings: the samples step n of each pair
+    writes inside the pass land in the right ring slots."""
+    print("[case] two-step streaming", flush=True)
+    model = workloads.tiny_model(Nt=90)
+    _, m_ref, _ = _run(model, variant=1)
+    want = [np.array(m["values"]) for m in m_ref]
+    c, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    with Session(c, Nx, Ny, Nz) as s:
+        s.set_variant(7)
+        s.set_monitor_streaming(6)
+        s.set_sources(probes, monitors, Nt)
+        for n in (33, 41, 16):
+            s.run(n)
+        s.synchronize()
+        s.read()
+    for k, (m, w) in enumerate(zip(monitors, want)):
+        np.testing.assert_array_equal(np.asarray(m["values"]), w, err_msg=f"monitor {k}")
+
+
+@pytest.mark.parametrize("mode", ["checkpoint", "history"])
+def test_two_step_forward_gradient_equals_separate(hip_ready, mode):
+    """A forward run on the two-step pass (checkpoints every 9 steps, or a design-box history that
+    makes the pass run single steps) gives the separate kernels' gradient bit for bit."""
+    print(f"[case] two-step gradient {mode}", flush=True)
+    Nt = 40
+    model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
+    coeff, probes, _, _, Nx, Ny, Nz, _ = model.pack(gpu=True)
+    g = np.random.default_rng(3).standard_normal((len(probes), Nt)).astype(np.float32)
+    box = (2, Nx - 2, 2, Ny - 2, 1, Nz - 1)
+    out = []
+    for variant in (1, 7):
+        _, pr, _, _, *_ = model.pack(gpu=True, coefficients=False)
+        with Session(coeff, Nx, Ny, Nz) as s:
+            s.set_variant(variant)
+            if mode == "history":
+                s.set_history(box, Nt)
+            else:
+                s.set_checkpointing(9)
+            s.set_sources(pr, [], Nt)
+            s.run(Nt)
+            s.synchronize()
+            out.append(s.gradient(box, list(g)))
+    assert np.abs(out[0]).max() > 0
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_two_step_large_grid_parity(hip_ready):
+    """The 400x400x200 workload with its 6 running-DFT planes, 60 steps on the two-step pass (graph
+    replay): probes, monitors and fields bit-exact against the oracle."""
+    print("[case] two-step 400x400x200", flush=True)
+    model = workloads.dipole_slab(Nt=60, monitors=True)
+    p_c, m_c, f_c = run_oracle(model, np.float32, n_threads=16)
+    p, m, f = _run(model, np.float32, 0, 7, chunks=[43, 17])
+    for a, b in zip(p, p_c):
+        np.testing.assert_array_equal(a["values"], b["values"])
+    model.unpack(p, m, gpu=True)
+    g = {k: np.array(v["values"]) for k, v in model.monitors.items()}
+    model.unpack(p_c, m_c, gpu=False)
+    for k, v in model.monitors.items():
+        np.testing.assert_array_equal(g[k], np.array(v["values"]), err_msg=k)
+    for k in f_c:
+        np.testing.assert_array_equal(f[k], f_c[k], err_msg=k)
