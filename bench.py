@@ -759,11 +759,11 @@ def plain_leg(args):
     step_bytes = cells * (E_BYTES + H_BYTES) + dft_b
     step_frac = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     variant = tuned.get("variant", 1) if tuned else 1
-    fused = variant in (2, 3, 5, 6, 7)
+    fused = variant in (2, 3, 5, 6, 7, 8)
     table = info["coef_kind"] == "table"
     # the dominant kernel: the two-step pass (two cell-updates per cell and launch), the fused E+H
     # pass or the H update
-    if variant == 7:
+    if variant in (7, 8):
         kname, kbytes, kms = "k_step2", (TWO_STEP_TABLE_BYTES if table else TWO_STEP_RAW_BYTES), e_ms
     elif fused:
         kname, kbytes, kms = "k_step_fused", (2 * TABLE_BYTES if table else E_BYTES + H_BYTES), e_ms
@@ -771,7 +771,7 @@ def plain_leg(args):
         kname, kbytes, kms = "k_update_h", (TABLE_BYTES if table else H_BYTES), h_ms
     shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
     traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname, shape))
-    if variant == 7:
+    if variant in (7, 8):
         model_text = ("two-step pass, per cell and launch (2 cell-updates): 12 splits read and written once "
                       "(96 B) + " + ("E and H entry indices 8 B + 2 probe bitmask words per 32 cells" if table
                                      else "30 coefficients 120 B"))
@@ -810,7 +810,7 @@ def raw_leg(args):
     like-for-like fraction of HBM peak."""
     run = timed_run(args, "raw", separate=True)
     tuned, h_ms, e_ms, cells = run["tuned"], run["h_ms"], run["e_ms"], run["cells"]
-    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6, 7))
+    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6, 7, 8))
     if fused:
         kname, kbytes, kms = "k_step_fused", E_BYTES + H_BYTES, e_ms
     else:

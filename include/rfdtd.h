@@ -157,7 +157,7 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
 /* Launch autotuner (no counterpart in the reference, whose CUDA path uses one fixed launch shape):
  * times every candidate {variant 1, 4, 0} x {block 128, 256, 512} x {x planes: automatic, Nx/2 ..
- * Nx/128} and {variant 2, 3, 5, 6, 7} x {x planes} (RFDTD_TUNE_SEPARATE: the first set only) over
+ * Nx/128} and {variant 2, 3, 5, 6, 7, 8} x {x planes} (RFDTD_TUNE_SEPARATE: the first set only) over
  * trial_steps steps of this session's grids (with its probes and monitors applied when they are
  * installed, restored afterwards), keeps the fastest and returns it as config_out = {variant,
  * block_threads, x_planes (0 = automatic)} with its ms per step.  Only on a session that has not run
@@ -204,8 +204,8 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * 6 or 14 rows x 62 cells, neighbours through LDS, halos recomputed) that reads one buffer of a
  * per-instance pair and writes the other (the session then holds two field buffers per
  * instance); 4 variant 1 with nontemporal stores of the fields; 5 and 6 variants 2 and 3 with the
- * next x plane's H splits loaded one plane earlier and the coefficient values issued first; 7 the
- * two-step pass: steps n and n + 1 of a tile of 12 rows x 60 cells in one launch (two-cell halos
+ * next x plane's H splits loaded one plane earlier and the coefficient values issued first; 7 and 8
+ * the two-step pass: steps n and n + 1 of a tile of 12 (8: 4) rows x 60 cells in one launch (two-cell halos
  * recomputed, the x planes skewed by one), reading one buffer of the pair and writing the other, so
  * that the fields are read and written once per two steps (single steps, and every step while a
  * design-box history is recorded, go through the same pass one step at a time).  Results are

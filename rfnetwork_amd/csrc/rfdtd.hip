@@ -2016,11 +2016,11 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
 // ------------------------------------------------------------------------------------------------
 // two-step pass: time steps n and n + 1 in one pass over the fields (temporal blocking)
 //
-// A workgroup of kP2Ty + 4 wavefronts owns a tile of kP2Ty rows (y) x kP2Fz cells (z) and a range
+// A workgroup of TY + 4 wavefronts owns a tile of TY rows (y) x kP2Fz cells (z) and a range
 // [x0, x1) of x planes; lane l of wave w sits at y = y0 - 2 + w, z = z0 - 2 + l.  Each step shrinks
 // the region whose values are right by one cell on every side:
-//   step n:      E on w <= kP2Ty + 2, l <= 62;        H on 1 <= w, l and the same upper limits;
-//   step n + 1:  E on 1 <= w <= kP2Ty + 1, 1 <= l <= 61;  H (owned, stored) on 2 <= w, l likewise.
+//   step n:      E on w <= TY + 2, l <= 62;        H on 1 <= w, l and the same upper limits;
+//   step n + 1:  E on 1 <= w <= TY + 1, 1 <= l <= 61;  H (owned, stored) on 2 <= w, l likewise.
 // In x the pass is skewed by one plane: iteration i takes plane i through step n and plane i - 1
 // through step n + 1, the x neighbours carried in registers, the y/z neighbours exchanged through
 // LDS (three barriers per plane).  Step n runs on planes x0 - 2 .. x1: the planes outside [x0, x1)
@@ -2033,7 +2033,10 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
 // is formed by the same operations in the same order as in the separate kernels.
 // ------------------------------------------------------------------------------------------------
 
-constexpr int kP2Ty = 12;   // rows owned by a two-step tile (its workgroup has kP2Ty + 4 wavefronts)
+// Rows owned by a two-step tile: 12 (16 wavefronts, one workgroup per CU at the pass's register
+// count) or 4 (8 wavefronts: two workgroups per CU, each hiding the other's barriers and loads, for
+// 50% more halo reads).
+constexpr int kP2TyTall = 12, kP2TyShort = 4;
 constexpr int kP2Fz = 60;   // z cells owned by a two-step tile
 constexpr int kP2Mon = 64;  // monitors a two-step pass samples itself, at most
 
@@ -2068,12 +2071,12 @@ __device__ __forceinline__ void p2_sample(const MonitorDev* __restrict__ mons, i
     }
 }
 
-template <typename T, class CA>
+template <typename T, int TY, class CA>
 __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, int inst, int x0, int x1, int y0,
                                               int z0, int n, bool io, bool two, const MonitorDev* __restrict__ mons,
                                               int nmon, int slot, T* sm) {
     using G = GLOBAL_T(T);
-    constexpr int TY = kP2Ty;
+    using G32 = GLOBAL_AS uint32_t;
     constexpr int NT = (TY + 4) * 64;
     constexpr uint32_t S = (uint32_t)sizeof(T);
     const int Nx = I.Nx, Ny = I.Ny, Nz = I.Nz;
@@ -2128,13 +2131,16 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
     };
     const bool eprobes = io && I.negroups, hprobes = io && I.nhgroups;
     auto cell_of = [&](int x) { return ((int64_t)x * Ny + y) * Nz + z; };
-    auto ebit = [&](int64_t c) { return ((I.ebits[c >> 5] >> (c & 31)) & 1u) != 0; };
-    auto hbit = [&](int64_t c) { return ((I.hbits[c >> 5] >> (c & 31)) & 1u) != 0; };
+    // probe bitmask words of a plane's cell, loaded with the plane's inputs (0 without probes)
+    auto ewd = [&](int x) { return eprobes && mAe && x < a1 ? ldg((const G32*)I.ebits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
+    auto hwd = [&](int x) { return hprobes && mAh && x < a1 ? ldg((const G32*)I.hbits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
+    auto bit = [&](uint32_t word, int x) { return ((word >> (uint32_t)(cell_of(x) & 31)) & 1u) != 0; };
 
     Split6<T> h0 = load_h(a0), e0 = load_e(a0);  // input H and E splits of plane i
     uint32_t ek = mAe ? C.ekey(off(a0)) : 0u, hk = mAh ? C.hkey(off(a0)) : 0u;
+    uint32_t ew = ewd(a0), hw = hwd(a0);         // bitmask words of plane i
     Split6<T> e1p{}, h1p{};                      // splits after step n, plane i - 1
-    uint32_t ekp = 0u, hkp = 0u;                 // their coefficient keys
+    uint32_t ekp = 0u, hkp = 0u, ewp = 0u, hwp = 0u;  // keys and bitmask words of plane i - 1
     T e1y_m = T(0), e1z_m = T(0);                // Ey, Ez after step n, plane i - 1
     T e2y_m = T(0), e2z_m = T(0);                // Ey, Ez after step n + 1, plane i - 2
     for (int i = a0; i < ie; ++i) {
@@ -2142,7 +2148,10 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         const int j = i - 1;
         const bool doB = two && j >= b0 && j < x1;
         const bool recA = own && i >= x0 && i < x1;
-        // the next plane's inputs, in flight during this plane's work
+        // this plane's E coefficients (the keys came a plane ago), then the next plane's inputs, in
+        // flight during this plane's work; each later phase's coefficients are issued a phase early
+        ECoef<T> cA{};
+        if (doA && mAe) cA = C.e_at(ek);
         const Split6<T> hN = load_h(i + 1);
         Split6<T> eN{};
         uint32_t ekN = 0u, hkN = 0u;
@@ -2151,18 +2160,19 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
             if (mAe) ekN = C.ekey(off(i + 1));
             if (mAh) hkN = C.hkey(off(i + 1));
         }
+        const uint32_t ewN = ewd(i + 1), hwN = hwd(i + 1);
         // ---- step n, plane i: E
         const T h0x = h0.xy + h0.xz, h0y = h0.yz + h0.yx, h0z = h0.zx + h0.zy;
         s0[tid] = h0x;
         s0[NT + tid] = h0y;
         s0[2 * NT + tid] = h0z;
         __syncthreads();
-        HCoef<T> ch{};
-        if (doA && mAh) ch = C.h_at(hk);
+        HCoef<T> chA{};
+        if (doA && mAh) chA = C.h_at(hk);
         Split6<T> e1 = e0;
         T e1x = T(0), e1y = T(0), e1z = T(0);
         if (doA && mAe) {
-            const ECoef<T> c = C.e_at(ek);
+            const ECoef<T>& c = cA;
             const bool xin = i < Nx - 1;
             const T hz_yn = yin ? s0[2 * NT + tid + 64] : T(0), hx_yn = yin ? s0[tid + 64] : T(0);
             const T hy_zn = zin ? s0[NT + tid + 1] : T(0), hx_zn = zin ? s0[tid + 1] : T(0);
@@ -2173,10 +2183,8 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
             if (xin) e1.yx = c.ca_yx * e1.yx + c.cb_yx * (hz_xn - h0z);
             if (xin) e1.zx = c.ca_zx * e1.zx + c.cb_zx * (hy_xn - h0y);
             if (yin) e1.zy = c.ca_zy * e1.zy + c.cb_zy * (hx_yn - h0x);
-            if (eprobes) {
-                const int64_t cell = cell_of(i);
-                if (ebit(cell)) inject_e(I, cell, n, recA, e1.xy, e1.xz, e1.yz, e1.yx, e1.zx, e1.zy);
-            }
+            if (eprobes && bit(ew, i))
+                inject_e(I, cell_of(i), n, recA, e1.xy, e1.xz, e1.yz, e1.yx, e1.zx, e1.zy);
             e1x = e1.xy + e1.xz;
             e1y = e1.yz + e1.yx;
             e1z = e1.zx + e1.zy;
@@ -2185,6 +2193,8 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
                 else if (nmon) p2_sample(mons, nmon, inst, EX, i, y, z, n, slot, e1x, e1y, e1z);
             }
         }
+        ECoef<T> cB{};  // step n + 1's E coefficients of plane i - 1, in flight during H of plane i
+        if (doB && mBe) cB = C.e_at(ekp);
         s1e[tid] = e1x;
         s1e[NT + tid] = e1y;
         s1e[2 * NT + tid] = e1z;
@@ -2193,6 +2203,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         Split6<T> h1 = h0;
         T h1x = T(0), h1y = T(0), h1z = T(0);
         if (doA && mAh) {
+            const HCoef<T>& ch = chA;
             const T ez_ym = y > 0 ? s1e[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s1e[tid - 64] : T(0);
             const T ey_zm = z > 0 ? s1e[NT + tid - 1] : T(0), ex_zm = z > 0 ? s1e[tid - 1] : T(0);
             h1.xy = ch.da_xy * h0.xy + (ch.db_xy2 * e1z - ch.db_xy1 * ez_ym);
@@ -2201,12 +2212,9 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
             h1.yx = ch.da_yx * h0.yx + (ch.db_yx2 * e1z - ch.db_yx1 * e1z_m);
             h1.zx = ch.da_zx * h0.zx + (ch.db_zx2 * e1y - ch.db_zx1 * e1y_m);
             h1.zy = ch.da_zy * h0.zy + (ch.db_zy2 * e1x - ch.db_zy1 * ex_ym);
-            if (hprobes) {
-                const int64_t cell = cell_of(i);
-                if (hbit(cell))
-                    inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell, n, recA, h1.xy, h1.xz, h1.yz,
-                                h1.yx, h1.zx, h1.zy);
-            }
+            if (hprobes && bit(hw, i))
+                inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell_of(i), n, recA, h1.xy, h1.xz, h1.yz,
+                            h1.yx, h1.zx, h1.zy);
             h1x = h1.xy + h1.xz;
             h1y = h1.yz + h1.yx;
             h1z = h1.zx + h1.zy;
@@ -2215,6 +2223,9 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
                 else if (nmon) p2_sample(mons, nmon, inst, HX, i, y, z, n, slot, h1x, h1y, h1z);
             }
         }
+        const bool recB = doB && own && j >= x0;
+        HCoef<T> chB{};  // step n + 1's H coefficients of plane i - 1, in flight during its E
+        if (recB) chB = C.h_at(hkp);
         if (two) {
             T* const sh = s1h + (i & 1) * 3 * NT;
             sh[tid] = h1x;
@@ -2226,11 +2237,10 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         // ---- step n + 1, plane j = i - 1 (its H-after-step-n totals were exchanged an iteration ago)
         if (doB) {
             const T* const shj = s1h + (j & 1) * 3 * NT;
-            const bool recB = own && j >= x0;
             Split6<T> e2 = e1p;
             T e2x = T(0), e2y = T(0), e2z = T(0);
             if (mBe) {
-                const ECoef<T> c = C.e_at(ekp);
+                const ECoef<T>& c = cB;
                 const bool xin = j < Nx - 1;
                 const T hx_c = h1p.xy + h1p.xz, hy_c = h1p.yz + h1p.yx, hz_c = h1p.zx + h1p.zy;
                 const T hz_yn = yin ? shj[2 * NT + tid + 64] : T(0), hx_yn = yin ? shj[tid + 64] : T(0);
@@ -2242,10 +2252,8 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
                 if (xin) e2.yx = c.ca_yx * e2.yx + c.cb_yx * (hz_xn - hz_c);
                 if (xin) e2.zx = c.ca_zx * e2.zx + c.cb_zx * (hy_xn - hy_c);
                 if (yin) e2.zy = c.ca_zy * e2.zy + c.cb_zy * (hx_yn - hx_c);
-                if (eprobes) {
-                    const int64_t cell = cell_of(j);
-                    if (ebit(cell)) inject_e(I, cell, n + 1, recB, e2.xy, e2.xz, e2.yz, e2.yx, e2.zx, e2.zy);
-                }
+                if (eprobes && bit(ewp, j))
+                    inject_e(I, cell_of(j), n + 1, recB, e2.xy, e2.xz, e2.yz, e2.yx, e2.zx, e2.zy);
                 e2x = e2.xy + e2.xz;
                 e2y = e2.yz + e2.yx;
                 e2z = e2.zx + e2.zy;
@@ -2256,7 +2264,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
             s2e[2 * NT + tid] = e2z;
             __syncthreads();
             if (recB) {
-                const HCoef<T> c = C.h_at(hkp);
+                const HCoef<T>& c = chB;
                 const T ez_ym = y > 0 ? s2e[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s2e[tid - 64] : T(0);
                 const T ey_zm = z > 0 ? s2e[NT + tid - 1] : T(0), ex_zm = z > 0 ? s2e[tid - 1] : T(0);
                 Split6<T> h2;
@@ -2266,12 +2274,9 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
                 h2.yx = c.da_yx * h1p.yx + (c.db_yx2 * e2z - c.db_yx1 * e2z_m);
                 h2.zx = c.da_zx * h1p.zx + (c.db_zx2 * e2y - c.db_zx1 * e2y_m);
                 h2.zy = c.da_zy * h1p.zy + (c.db_zy2 * e2x - c.db_zy1 * ex_ym);
-                if (hprobes) {
-                    const int64_t cell = cell_of(j);
-                    if (hbit(cell))
-                        inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell, n + 1, true, h2.xy, h2.xz,
-                                    h2.yz, h2.yx, h2.zx, h2.zy);
-                }
+                if (hprobes && bit(hwp, j))
+                    inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell_of(j), n + 1, true, h2.xy, h2.xz,
+                                h2.yz, h2.yx, h2.zx, h2.zy);
                 store(B, off(j), h2, true);
             }
             e2y_m = e2y;
@@ -2281,23 +2286,27 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         h1p = h1;
         ekp = ek;
         hkp = hk;
+        ewp = ew;
+        hwp = hw;
         h0 = hN;
         e0 = eN;
         ek = ekN;
         hk = hkN;
+        ew = ewN;
+        hw = hwN;
     }
 }
 
 // two != 0: steps n and n + 1 (n from base/local); the phasor monitors' sample rows of step n's
 // staging slot are written by logical workgroup 0 (k_monitors writes step n + 1's).
-template <typename T>
-__global__ void __launch_bounds__((kP2Ty + 4) * 64) k_step2(const InstDev<T>* __restrict__ insts,
+template <typename T, int TY>
+__global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __restrict__ insts,
                                                             const MemberDev* __restrict__ mem, int nmem, int total,
                                                             const int* base, int local, int io, int two,
                                                             const MonitorDev* __restrict__ mons, int nmon, int slot,
                                                             int* __restrict__ rows) {
     __shared__ int s_member;
-    __shared__ T sm[15 * (kP2Ty + 4) * 64];
+    __shared__ T sm[15 * (TY + 4) * 64];
     const int L = xcd_logical(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         int m = 0;
@@ -2321,13 +2330,13 @@ __global__ void __launch_bounds__((kP2Ty + 4) * 64) k_step2(const InstDev<T>* __
     const InstDev<T>* I = insts + inst;
     const int tz_n = (I->Nz + kP2Fz - 1) / kP2Fz;
     const int pb = tile % I->npb, xc = tile / I->npb;
-    const int y0 = (pb / tz_n) * kP2Ty, z0 = (pb % tz_n) * kP2Fz;
+    const int y0 = (pb / tz_n) * TY, z0 = (pb % tz_n) * kP2Fz;
     const int x0 = xc * I->xplanes, x1 = min(I->Nx, x0 + I->xplanes);
     if (I->coef_kind == 1)
-        two_step_tile<T, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, inst, x0, x1, y0, z0, n,
+        two_step_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, inst, x0, x1, y0, z0, n,
                                        io != 0, two != 0, mons, nm, slot, sm);
     else
-        two_step_tile<T, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, inst, x0, x1, y0, z0, n, io != 0, two != 0, mons, nm,
+        two_step_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, inst, x0, x1, y0, z0, n, io != 0, two != 0, mons, nm,
                                      slot, sm);
 }
 
@@ -3141,7 +3150,7 @@ void assemble_coefficients(Member& M, uint32_t flags, const rfdtd_assembly& a) {
 // launch on 256 CUs), as many planes as possible for register reuse along x.
 // Workgroups per x plane of a member: (y,z) columns per workgroup, or fused tiles.
 int64_t tiles_per_plane(const Session& s, const Member& M) {
-    if (s.two_step) return (int64_t)((M.Ny + kP2Ty - 1) / kP2Ty) * ((M.Nz + kP2Fz - 1) / kP2Fz);
+    if (s.two_step) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kP2Fz - 1) / kP2Fz);
     if (s.fused) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kFz - 1) / kFz);
     return ((int64_t)M.Ny * M.Nz + s.block - 1) / s.block;
 }
@@ -3400,8 +3409,14 @@ void launch_step2(Session& s, hipStream_t st, const int* base, int local, bool t
     const InstDev<T>* di = (const InstDev<T>*)(s.parity ? s.d_insts2 : s.d_insts);
     const int io = s.tuning ? 0 : 1;
     if (ev) HIP_OK(hipEventRecord(ev[0], st));
-    k_step2<T><<<grid, (kP2Ty + 4) * 64, 0, st>>>(di, s.d_members, (int)s.members.size(), s.total_tiles, base, local,
-                                                  io, two ? 1 : 0, s.d_mons, s.n_active, slot, s.d_rows);
+    if (s.fused == kP2TyShort)
+        k_step2<T, kP2TyShort><<<grid, (kP2TyShort + 4) * 64, 0, st>>>(
+            di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
+            slot, s.d_rows);
+    else
+        k_step2<T, kP2TyTall><<<grid, (kP2TyTall + 4) * 64, 0, st>>>(
+            di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
+            slot, s.d_rows);
     if (ev) {
         HIP_OK(hipEventRecord(ev[1], st));
         HIP_OK(hipEventRecord(ev[2], st));
@@ -3754,17 +3769,18 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const int v0 = s.two_step ? 7 : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
+    const int v0 = s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
                  : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
     const int blk0 = s.block;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
     // variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row tiles, 3 fused 14-row tiles,
-    // 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead of H, 7 the two-step pass
+    // 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead of H, 7 and 8 the two-step
+    // pass with 12- and 4-row tiles
     auto apply = [&](int v, int blk, const std::vector<int>& xr) {
         s.pipelined = v != 0;
-        s.two_step = v == 7;
-        s.fused = v == 7 ? kP2Ty : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
+        s.two_step = v == 7 || v == 8;
+        s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
         s.fused_deep = v == 5 || v == 6;
         s.nontemporal = v == 4;
         s.block = blk;
@@ -3792,7 +3808,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     HIP_OK(hipGetLastError());
     try {
         const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
-        for (int v : {1, 4, 0, 2, 3, 5, 6, 7})
+        for (int v : {1, 4, 0, 2, 3, 5, 6, 7, 8})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
                     const bool fused_v = v == 2 || v == 3 || v >= 5;
@@ -4808,10 +4824,11 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 7) RF_THROW(RFDTD_EINVAL, "variant must be in 0..7");
+        if (variant < 0 || variant > 8) RF_THROW(RFDTD_EINVAL, "variant must be in 0..8");
         S.pipelined = variant != 0;
-        S.two_step = variant == 7;
-        S.fused = variant == 7 ? kP2Ty : (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
+        S.two_step = variant == 7 || variant == 8;
+        S.fused = variant == 7 ? kP2TyTall : variant == 8 ? kP2TyShort
+                : (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
         S.fused_deep = variant == 5 || variant == 6;
         S.nontemporal = variant == 4;
         S.dirty = true;

@@ -220,8 +220,8 @@ class Session:
         """Kernel variant: 1 (default) separate E and H kernels issuing the next x plane's loads early,
         0 the same one plane at a time, 2 and 3 the fused E+H pass with 6- or 14-row tiles (two field
         buffers per instance), 4 variant 1 with nontemporal field stores, 5 and 6 variants 2 and 3
-        with deeper load pipelining, 7 the two-step pass (steps n and n + 1 of a 12-row x 60-cell
-        tile per launch, two field buffers per instance).  Results are identical."""
+        with deeper load pipelining, 7 and 8 the two-step pass (steps n and n + 1 of a 12- or 4-row x
+        60-cell tile per launch, two field buffers per instance).  Results are identical."""
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def close(self):

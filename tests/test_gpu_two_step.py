@@ -1,5 +1,6 @@
-"""The two-step pass (variant 7): time steps n and n + 1 of a tile in one launch, with two-cell halos
-recomputed and the x planes skewed by one; step n's monitor samples are taken inside the pass.  Its
+"""The two-step pass (variants 7 and 8: 12- and 4-row tiles): time steps n and n + 1 of a tile in one
+launch, with two-cell halos recomputed and the x planes skewed by one; step n's monitor samples are
+taken inside the pass.  Its
 results must equal the oracle's and the other variants' bit for bit, whatever the tiling, x planes,
 chunking (odd chunk lengths end on a single step), buffer parity, graph replay or batch, and a
 forward run recorded for a gradient (checkpoints between pairs; a design-box history, which runs
@@ -17,27 +18,29 @@ from test_gpu_fused import _assert_equal_to_oracle, _run
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("variant", [7, 8])
 @pytest.mark.parametrize("flags", [0, _lib.COEF_RAW, _lib.NO_GRAPHS, _lib.COEF_RAW | _lib.NO_GRAPHS])
-def test_two_step_tiny_fp32_bit_exact(hip_ready, flags):
+def test_two_step_tiny_fp32_bit_exact(hip_ready, flags, variant):
     """PML, dielectric, conductor, two ports, a voltage probe, time and phasor monitors on every axis;
     chunks of 37 and 23 steps (graph chunks of 8 steps, then pairs, then a single step)."""
-    print(f"[case] two-step tiny flags={flags}", flush=True)
+    print(f"[case] two-step tiny flags={flags} variant={variant}", flush=True)
     model = workloads.tiny_model(Nt=60)
-    p, m, f = _run(model, np.float32, flags, 7, chunks=[37, 23], launch=(0, 3, 10))
+    p, m, f = _run(model, np.float32, flags, variant, chunks=[37, 23], launch=(0, 3, 10))
     _assert_equal_to_oracle(model, np.float32, p, m, f)
 
 
-def test_two_step_tiny_fp64_bit_exact(hip_ready):
-    print("[case] two-step tiny fp64", flush=True)
+@pytest.mark.parametrize("variant", [7, 8])
+def test_two_step_tiny_fp64_bit_exact(hip_ready, variant):
+    print(f"[case] two-step tiny fp64 variant={variant}", flush=True)
     model = workloads.tiny_model(Nt=50)
-    p, m, f = _run(model, np.float64, 0, 7, chunks=[21, 29], launch=(0, 4, 6))
+    p, m, f = _run(model, np.float64, 0, variant, chunks=[21, 29], launch=(0, 4, 6))
     _assert_equal_to_oracle(model, np.float64, p, m, f)
 
 
 def test_two_step_interleaves_with_other_variants(hip_ready):
     print("[case] two-step interleave", flush=True)
     model = workloads.tiny_model(Nt=60)
-    p, m, f = _run(model, np.float32, 0, 7, chunks=[13, 20, 7, 11, 9], switch=[7, 1, 2, 7, 0, 6])
+    p, m, f = _run(model, np.float32, 0, 7, chunks=[13, 20, 7, 11, 4, 5], switch=[7, 1, 2, 8, 0, 6, 7])
     _assert_equal_to_oracle(model, np.float32, p, m, f)
 
 
@@ -49,18 +52,20 @@ def test_two_step_many_tiles_and_planes(hip_ready):
     p0, m0, ref = _run(model, variant=1)
     ref_p = [np.array(p["values"]) for p in p0]
     ref_m = [np.array(m["values"]) for m in m0]
-    for xp in (1, 2, 3, 5, 23):
-        p, m, f = _run(model, variant=7, launch=(0, xp, 8), chunks=[11, 19])
-        for k in ref:
-            np.testing.assert_array_equal(f[k], ref[k], err_msg=f"{k} x planes {xp}")
-        for a, b in zip(p, ref_p):
-            np.testing.assert_array_equal(a["values"], b, err_msg=f"probe, x planes {xp}")
-        for k, (a, b) in enumerate(zip(m, ref_m)):
-            np.testing.assert_array_equal(a["values"], b, err_msg=f"monitor {k}, x planes {xp}")
+    for variant in (7, 8):
+        for xp in (1, 2, 3, 5, 23):
+            p, m, f = _run(model, variant=variant, launch=(0, xp, 8), chunks=[11, 19])
+            for k in ref:
+                np.testing.assert_array_equal(f[k], ref[k], err_msg=f"{k} variant {variant} x planes {xp}")
+            for a, b in zip(p, ref_p):
+                np.testing.assert_array_equal(a["values"], b, err_msg=f"probe, variant {variant} x planes {xp}")
+            for k, (a, b) in enumerate(zip(m, ref_m)):
+                np.testing.assert_array_equal(a["values"], b, err_msg=f"monitor {k}, variant {variant} x planes {xp}")
 
 
-def test_two_step_batch_matches_oracle(hip_ready):
-    print("[case] two-step batch", flush=True)
+@pytest.mark.parametrize("variant", [7, 8])
+def test_two_step_batch_matches_oracle(hip_ready, variant):
+    print(f"[case] two-step batch variant={variant}", flush=True)
     models = [workloads.tiny_model(Nt=40, nx=14, ny=11, nz=9, seed=7),
               workloads.tiny_model(Nt=40, nx=19, ny=26, nz=70, seed=8)]
     grids, inputs = [], []
@@ -69,7 +74,7 @@ def test_two_step_batch_matches_oracle(hip_ready):
         _, probes, monitors, _, *_ = m.pack(gpu=True, coefficients=False)
         inputs.append((probes, monitors))
     with Batch(grids) as b:
-        b.set_variant(7)
+        b.set_variant(variant)
         for k, (probes, monitors) in enumerate(inputs):
             b.set_sources(k, probes, monitors, 40)
         b.run(25)
@@ -112,7 +117,7 @@ def test_two_step_forward_gradient_equals_separate(hip_ready, mode):
     g = np.random.default_rng(3).standard_normal((len(probes), Nt)).astype(np.float32)
     box = (2, Nx - 2, 2, Ny - 2, 1, Nz - 1)
     out = []
-    for variant in (1, 7):
+    for variant in (1, 7, 8):
         _, pr, _, _, *_ = model.pack(gpu=True, coefficients=False)
         with Session(coeff, Nx, Ny, Nz) as s:
             s.set_variant(variant)
@@ -126,15 +131,17 @@ def test_two_step_forward_gradient_equals_separate(hip_ready, mode):
             out.append(s.gradient(box, list(g)))
     assert np.abs(out[0]).max() > 0
     np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
 
 
-def test_two_step_large_grid_parity(hip_ready):
+@pytest.mark.parametrize("variant", [7, 8])
+def test_two_step_large_grid_parity(hip_ready, variant):
     """The 400x400x200 workload with its 6 running-DFT planes, 60 steps on the two-step pass (graph
     replay): probes, monitors and fields bit-exact against the oracle."""
-    print("[case] two-step 400x400x200", flush=True)
+    print(f"[case] two-step 400x400x200 variant={variant}", flush=True)
     model = workloads.dipole_slab(Nt=60, monitors=True)
     p_c, m_c, f_c = run_oracle(model, np.float32, n_threads=16)
-    p, m, f = _run(model, np.float32, 0, 7, chunks=[43, 17])
+    p, m, f = _run(model, np.float32, 0, variant, chunks=[43, 17])
     for a, b in zip(p, p_c):
         np.testing.assert_array_equal(a["values"], b["values"])
     model.unpack(p, m, gpu=True)
