@@ -330,22 +330,34 @@ try:  # torch is plumbing for autograd users; the engine itself does not need it
     import torch
 
     class EpsRObjective(torch.autograd.Function):
-        """J(eps_r of a design box) with a hand-written adjoint backward.
+        """J(eps_r of a design box) with a hand-written adjoint backward, on one resident GPU session.
 
-        forward(eps_r, build, cbox, frequency): build(eps_r_numpy) returns a meshed, excited model whose
-        cells in cbox have that permittivity; J is sum_f |S_21|^2 (ports b=2, a=1)."""
+        apply(eps_r, loop): loop is a design.DesignLoop; forward recomputes the box's coefficients on
+        the GPU and runs the forward solve, and the reverse pass runs only when backward is called.
+        apply(eps_r, build, cbox, frequency[, checkpoint_every]): build(eps_r_numpy) returns a meshed,
+        excited model whose cells in cbox have that permittivity; a loop is made for that model and
+        released after backward.  J is sum_f |S_21|^2 (ports b=2, a=1)."""
 
         @staticmethod
-        def forward(ctx, eps_r, build, cbox, frequency, checkpoint_every=200):
-            model = build(eps_r.detach().cpu().numpy().astype(np.float64))
-            J, grad, _ = eps_r_gradient(model, cbox, frequency, checkpoint_every=checkpoint_every)
-            ctx.save_for_backward(torch.as_tensor(grad, dtype=eps_r.dtype))
+        def forward(ctx, eps_r, loop, cbox=None, frequency=None, checkpoint_every=200):
+            from .design import DesignLoop
+
+            eps = eps_r.detach().to("cpu", torch.float64).numpy()
+            owned = not isinstance(loop, DesignLoop)
+            if owned:
+                model = loop(eps)
+                loop = DesignLoop(model, cbox, frequency, dtype=model.coef_dtype, checkpoint_every=checkpoint_every)
+            J, _ = loop.forward(eps)
+            ctx.loop, ctx.owned, ctx.generation = loop, owned, loop.generation
             return eps_r.new_tensor(J)
 
         @staticmethod
         def backward(ctx, grad_out):
-            (grad,) = ctx.saved_tensors
-            return grad_out * grad.to(grad_out.device), None, None, None, None
+            grad = ctx.loop.backward(ctx.generation)
+            if ctx.owned:
+                ctx.loop.close()
+            g = torch.as_tensor(grad, dtype=grad_out.dtype, device=grad_out.device)
+            return grad_out * g, None, None, None, None
 
 except ImportError:  # pragma: no cover
     EpsRObjective = None

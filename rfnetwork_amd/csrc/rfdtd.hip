@@ -2037,18 +2037,20 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
 // count) or 4 (8 wavefronts: two workgroups per CU, each hiding the other's barriers and loads, for
 // 50% more halo reads).
 constexpr int kP2TyTall = 12, kP2TyShort = 4;
+constexpr int kP2TySkew = 8;  // the one-barrier pass (more registers: 12 wavefronts, one workgroup per CU)
 constexpr int kP2Fz = 60;   // z cells owned by a two-step tile
 constexpr int kP2Mon = 64;  // monitors a two-step pass samples itself, at most
 
-// The monitor samples of one owned cell at step n: every monitor of this instance whose plane holds
-// the cell and whose field is component fbase + 0..2 takes v[component] (k_monitors' placement).
+// The monitor samples of one owned cell at step n: every monitor of the tile's list (those of this
+// instance that sample step n and whose plane crosses the tile) whose plane holds the cell and whose
+// field is component fbase + 0..2 takes v[component] (k_monitors' placement).
 template <typename T>
-__device__ __forceinline__ void p2_sample(const MonitorDev* __restrict__ mons, int nmon, int inst, int fbase, int x,
-                                          int y, int z, int n, int slot, T vx, T vy, T vz) {
-    for (int k = 0; k < nmon; ++k) {
-        const MonitorDev& M = mons[k];
+__device__ __forceinline__ void p2_sample(const MonitorDev* __restrict__ mons, const int* list, int nlist, int fbase,
+                                          int x, int y, int z, int n, int slot, T vx, T vy, T vz) {
+    for (int k = 0; k < nlist; ++k) {
+        const MonitorDev& M = mons[__builtin_amdgcn_readfirstlane(list[k])];
         const int f = M.field - fbase;
-        if (M.inst != inst || f < 0 || f > 2 || n % M.n_step != 0) continue;
+        if (f < 0 || f > 2) continue;
         int a, b;
         if (M.axis == 0) {
             if (x != M.pos) continue;
@@ -2072,9 +2074,9 @@ __device__ __forceinline__ void p2_sample(const MonitorDev* __restrict__ mons, i
 }
 
 template <typename T, int TY, class CA>
-__device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, int inst, int x0, int x1, int y0,
-                                              int z0, int n, bool io, bool two, const MonitorDev* __restrict__ mons,
-                                              int nmon, int slot, T* sm) {
+__device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, int x0, int x1, int y0, int z0, int n,
+                                              bool io, bool two, const MonitorDev* __restrict__ mons,
+                                              const int* mlist, int nmon, int slot, T* sm) {
     using G = GLOBAL_T(T);
     using G32 = GLOBAL_AS uint32_t;
     constexpr int NT = (TY + 4) * 64;
@@ -2190,7 +2192,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
             e1z = e1.zx + e1.zy;
             if (recA) {
                 if (!two) store(B, off(i), e1, false);
-                else if (nmon) p2_sample(mons, nmon, inst, EX, i, y, z, n, slot, e1x, e1y, e1z);
+                else if (nmon) p2_sample(mons, mlist, nmon, EX, i, y, z, n, slot, e1x, e1y, e1z);
             }
         }
         ECoef<T> cB{};  // step n + 1's E coefficients of plane i - 1, in flight during H of plane i
@@ -2220,7 +2222,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
             h1z = h1.zx + h1.zy;
             if (recA) {
                 if (!two) store(B, off(i), h1, true);
-                else if (nmon) p2_sample(mons, nmon, inst, HX, i, y, z, n, slot, h1x, h1y, h1z);
+                else if (nmon) p2_sample(mons, mlist, nmon, HX, i, y, z, n, slot, h1x, h1y, h1z);
             }
         }
         const bool recB = doB && own && j >= x0;
@@ -2297,16 +2299,232 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
     }
 }
 
+// The two-step pass with one barrier per plane (SKEW): iteration i advances E of step n on plane i,
+// H of step n on plane i - 1, E of step n + 1 on plane i - 2 and H of step n + 1 on plane i - 3.  Each
+// y/z neighbour a phase reads was put in LDS an iteration earlier (the input H totals: before this
+// iteration's barrier), every exchange alternating between two buffers by plane parity, so the one
+// barrier orders them all, and the four phases' coefficient gathers are in flight across it.  The
+// x neighbours and the states the later phases advance pass in registers (planes i - 1 .. i - 3);
+// the input H splits are loaded two planes ahead.  Tiles, halos, sources, records and monitor
+// samples as in two_step_tile, whose single-step mode serves the odd steps at the same tiling.
+template <typename T, int TY, class CA>
+__device__ __forceinline__ void two_step_skew_tile(const InstDev<T>& I, const CA& C, int x0, int x1, int y0, int z0,
+                                                   int n, bool io, const MonitorDev* __restrict__ mons,
+                                                   const int* mlist, int nmon, int slot, T* sm) {
+    using G = GLOBAL_T(T);
+    using G32 = GLOBAL_AS uint32_t;
+    constexpr int NT = (TY + 4) * 64;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const int Nx = I.Nx, Ny = I.Ny, Nz = I.Nz;
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int y = y0 - 2 + w, z = z0 - 2 + l;
+    const bool valid = y >= 0 && y < Ny && z >= 0 && z < Nz;
+    const bool mAe = valid && w <= TY + 2 && l <= 62;  // E of step n right here
+    const bool mAh = mAe && w >= 1 && l >= 1;          // H of step n
+    const bool mBe = mAh && w <= TY + 1 && l <= 61;    // E of step n + 1
+    const bool own = mBe && w >= 2 && l >= 2;          // H of step n + 1; stored, recorded
+    const bool yin = y < Ny - 1, zin = z < Nz - 1;
+    T* const sH0 = sm;            // [2][3][NT] input H totals, by plane parity
+    T* const sEA = sm + 6 * NT;   // [2][3][NT] E totals after step n
+    T* const sHA = sm + 12 * NT;  // [2][3][NT] H totals after step n
+    T* const sEB = sm + 18 * NT;  // [2][3][NT] E totals after step n + 1
+    const Fields<T>& A = I.F;
+    const Fields<T>& B = I.G;
+    const int lo = max(x0 - 2, 0), aHi = min(x1 + 1, Nx);  // step n's E on planes [lo, aHi)
+    const int hLo = max(x0 - 1, 0);                        // step n's H on [hLo, aHi), step n + 1's E on [hLo, x1)
+    const uint32_t p = valid ? ((uint32_t)y * (uint32_t)Nz + (uint32_t)z) : 0u;
+    const uint32_t base = ((uint32_t)lo * (uint32_t)Ny * (uint32_t)Nz + p) * S;
+    auto off = [&](int x) { return base + (uint32_t)(x - lo) * sx; };
+    auto load_h = [&](int x) {
+        Split6<T> h{};
+        if (valid && x < Nx) {
+            const uint32_t b = off(x);
+            h.xy = ldg((const G*)A.hx_y, b); h.xz = ldg((const G*)A.hx_z, b); h.yz = ldg((const G*)A.hy_z, b);
+            h.yx = ldg((const G*)A.hy_x, b); h.zx = ldg((const G*)A.hz_x, b); h.zy = ldg((const G*)A.hz_y, b);
+        }
+        return h;
+    };
+    auto load_e = [&](int x) {
+        Split6<T> e{};
+        if (mAe && x < aHi) {
+            const uint32_t b = off(x);
+            e.xy = ldg((const G*)A.ex_y, b); e.xz = ldg((const G*)A.ex_z, b); e.yz = ldg((const G*)A.ey_z, b);
+            e.yx = ldg((const G*)A.ey_x, b); e.zx = ldg((const G*)A.ez_x, b); e.zy = ldg((const G*)A.ez_y, b);
+        }
+        return e;
+    };
+    auto store = [&](uint32_t b, const Split6<T>& e, bool h) {
+        if (h) {
+            stg((G*)B.hx_y, b, e.xy); stg((G*)B.hx_z, b, e.xz); stg((G*)B.hy_z, b, e.yz);
+            stg((G*)B.hy_x, b, e.yx); stg((G*)B.hz_x, b, e.zx); stg((G*)B.hz_y, b, e.zy);
+        } else {
+            stg((G*)B.ex_y, b, e.xy); stg((G*)B.ex_z, b, e.xz); stg((G*)B.ey_z, b, e.yz);
+            stg((G*)B.ey_x, b, e.yx); stg((G*)B.ez_x, b, e.zx); stg((G*)B.ez_y, b, e.zy);
+        }
+    };
+    auto put3 = [&](T* buf, T a, T b, T c) {
+        buf[tid] = a;
+        buf[NT + tid] = b;
+        buf[2 * NT + tid] = c;
+    };
+    const bool eprobes = io && I.negroups, hprobes = io && I.nhgroups;
+    auto cell_of = [&](int x) { return ((int64_t)x * Ny + y) * Nz + z; };
+    auto ewd = [&](int x) { return eprobes && mAe && x < aHi ? ldg((const G32*)I.ebits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
+    auto hwd = [&](int x) { return hprobes && mAh && x < aHi ? ldg((const G32*)I.hbits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
+    auto bit = [&](uint32_t word, int x) { return ((word >> (uint32_t)(cell_of(x) & 31)) & 1u) != 0; };
+    auto ekey = [&](int x) { return mAe && x < aHi ? C.ekey(off(x)) : 0u; };
+    auto hkey = [&](int x) { return mAh && x < aHi ? C.hkey(off(x)) : 0u; };
+
+    // input H splits of planes i - 1, i, i + 1 and E splits of plane i
+    Split6<T> h0p{}, h0c = load_h(lo), hN = load_h(lo + 1), e0c = load_e(lo);
+    // the step-n states of planes i - 1, i - 2 (E) and i - 2, i - 3 (H) that the later phases advance
+    Split6<T> eA1{}, eA2{}, hA2{}, hA3{};
+    T eB3x = T(0), eB3y = T(0), eB3z = T(0), eB4y = T(0), eB4z = T(0);  // E totals after step n + 1, planes i - 3, i - 4
+    // coefficient keys and probe bitmask words by plane: E of planes i, i - 1, i - 2, H of i .. i - 3
+    uint32_t ek0 = ekey(lo), ek1 = 0u, ek2 = 0u, hk0 = hkey(lo), hk1 = 0u, hk2 = 0u, hk3 = 0u;
+    uint32_t ew0 = ewd(lo), ew1 = 0u, ew2 = 0u, hw0 = hwd(lo), hw1 = 0u, hw2 = 0u, hw3 = 0u;
+    for (int i = lo; i < x1 + 3; ++i) {
+        const int p2 = i - 1, p3 = i - 2, p4 = i - 3;
+        const bool d1 = i < aHi;                   // step n, E, plane i
+        const bool d2 = p2 >= hLo && p2 < aHi;     // step n, H, plane i - 1
+        const bool d3 = p3 >= hLo && p3 < x1;      // step n + 1, E, plane i - 2
+        const bool d4 = p4 >= x0 && p4 < x1;       // step n + 1, H, plane i - 3
+        const int par = i & 1, prv = par ^ 1;
+        ECoef<T> c1{}, c3{};
+        HCoef<T> c2{}, c4{};
+        if (d1 && mAe) c1 = C.e_at(ek0);
+        if (d2 && mAh) c2 = C.h_at(hk1);
+        if (d3 && mBe) c3 = C.e_at(ek2);
+        if (d4 && own) c4 = C.h_at(hk3);
+        const Split6<T> hNN = load_h(i + 2);
+        const Split6<T> eN = load_e(i + 1);
+        const uint32_t ekN = ekey(i + 1), hkN = hkey(i + 1), ewN = ewd(i + 1), hwN = hwd(i + 1);
+        const T h0x = h0c.xy + h0c.xz, h0y = h0c.yz + h0c.yx, h0z = h0c.zx + h0c.zy;
+        put3(sH0 + par * 3 * NT, h0x, h0y, h0z);
+        __syncthreads();
+        // ---- step n, E, plane i (input H totals of plane i exchanged just now)
+        Split6<T> eA = e0c;
+        T eAx = T(0), eAy = T(0), eAz = T(0);
+        if (d1 && mAe) {
+            const T* const s0 = sH0 + par * 3 * NT;
+            const bool xin = i < Nx - 1;
+            const T hz_yn = yin ? s0[2 * NT + tid + 64] : T(0), hx_yn = yin ? s0[tid + 64] : T(0);
+            const T hy_zn = zin ? s0[NT + tid + 1] : T(0), hx_zn = zin ? s0[tid + 1] : T(0);
+            const T hy_xn = xin ? hN.yz + hN.yx : T(0), hz_xn = xin ? hN.zx + hN.zy : T(0);
+            if (yin) eA.xy = c1.ca_xy * eA.xy + c1.cb_xy * (hz_yn - h0z);
+            if (zin) eA.xz = c1.ca_xz * eA.xz + c1.cb_xz * (hy_zn - h0y);
+            if (zin) eA.yz = c1.ca_yz * eA.yz + c1.cb_yz * (hx_zn - h0x);
+            if (xin) eA.yx = c1.ca_yx * eA.yx + c1.cb_yx * (hz_xn - h0z);
+            if (xin) eA.zx = c1.ca_zx * eA.zx + c1.cb_zx * (hy_xn - h0y);
+            if (yin) eA.zy = c1.ca_zy * eA.zy + c1.cb_zy * (hx_yn - h0x);
+            const bool rec = own && i >= x0 && i < x1;
+            if (eprobes && bit(ew0, i)) inject_e(I, cell_of(i), n, rec, eA.xy, eA.xz, eA.yz, eA.yx, eA.zx, eA.zy);
+            eAx = eA.xy + eA.xz;
+            eAy = eA.yz + eA.yx;
+            eAz = eA.zx + eA.zy;
+            if (rec && nmon) p2_sample(mons, mlist, nmon, EX, i, y, z, n, slot, eAx, eAy, eAz);
+        }
+        put3(sEA + par * 3 * NT, eAx, eAy, eAz);
+        // ---- step n, H, plane i - 1 (its E totals were exchanged an iteration ago)
+        Split6<T> hA = h0p;
+        T hAx = T(0), hAy = T(0), hAz = T(0);
+        if (d2 && mAh) {
+            const T* const s1 = sEA + prv * 3 * NT;
+            const T ex = eA1.xy + eA1.xz, ey = eA1.yz + eA1.yx, ez = eA1.zx + eA1.zy;
+            const T ey_xm = p2 > 0 ? eA2.yz + eA2.yx : T(0), ez_xm = p2 > 0 ? eA2.zx + eA2.zy : T(0);
+            const T ez_ym = y > 0 ? s1[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s1[tid - 64] : T(0);
+            const T ey_zm = z > 0 ? s1[NT + tid - 1] : T(0), ex_zm = z > 0 ? s1[tid - 1] : T(0);
+            hA.xy = c2.da_xy * h0p.xy + (c2.db_xy2 * ez - c2.db_xy1 * ez_ym);
+            hA.xz = c2.da_xz * h0p.xz + (c2.db_xz2 * ey - c2.db_xz1 * ey_zm);
+            hA.yz = c2.da_yz * h0p.yz + (c2.db_yz2 * ex - c2.db_yz1 * ex_zm);
+            hA.yx = c2.da_yx * h0p.yx + (c2.db_yx2 * ez - c2.db_yx1 * ez_xm);
+            hA.zx = c2.da_zx * h0p.zx + (c2.db_zx2 * ey - c2.db_zx1 * ey_xm);
+            hA.zy = c2.da_zy * h0p.zy + (c2.db_zy2 * ex - c2.db_zy1 * ex_ym);
+            const bool rec = own && p2 >= x0 && p2 < x1;
+            if (hprobes && bit(hw1, p2))
+                inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell_of(p2), n, rec, hA.xy, hA.xz, hA.yz,
+                            hA.yx, hA.zx, hA.zy);
+            hAx = hA.xy + hA.xz;
+            hAy = hA.yz + hA.yx;
+            hAz = hA.zx + hA.zy;
+            if (rec && nmon) p2_sample(mons, mlist, nmon, HX, p2, y, z, n, slot, hAx, hAy, hAz);
+        }
+        put3(sHA + par * 3 * NT, hAx, hAy, hAz);
+        // ---- step n + 1, E, plane i - 2 (H totals of plane i - 2 after step n: an iteration ago;
+        // those of plane i - 1 at this cell: just computed)
+        T eBx = T(0), eBy = T(0), eBz = T(0);
+        if (d3 && mBe) {
+            const T* const s2 = sHA + prv * 3 * NT;
+            Split6<T> eB = eA2;
+            const bool xin = p3 < Nx - 1;
+            const T hx_c = hA2.xy + hA2.xz, hy_c = hA2.yz + hA2.yx, hz_c = hA2.zx + hA2.zy;
+            const T hz_yn = yin ? s2[2 * NT + tid + 64] : T(0), hx_yn = yin ? s2[tid + 64] : T(0);
+            const T hy_zn = zin ? s2[NT + tid + 1] : T(0), hx_zn = zin ? s2[tid + 1] : T(0);
+            const T hy_xn = xin ? hAy : T(0), hz_xn = xin ? hAz : T(0);
+            if (yin) eB.xy = c3.ca_xy * eB.xy + c3.cb_xy * (hz_yn - hz_c);
+            if (zin) eB.xz = c3.ca_xz * eB.xz + c3.cb_xz * (hy_zn - hy_c);
+            if (zin) eB.yz = c3.ca_yz * eB.yz + c3.cb_yz * (hx_zn - hx_c);
+            if (xin) eB.yx = c3.ca_yx * eB.yx + c3.cb_yx * (hz_xn - hz_c);
+            if (xin) eB.zx = c3.ca_zx * eB.zx + c3.cb_zx * (hy_xn - hy_c);
+            if (yin) eB.zy = c3.ca_zy * eB.zy + c3.cb_zy * (hx_yn - hx_c);
+            const bool rec = own && p3 >= x0;
+            if (eprobes && bit(ew2, p3))
+                inject_e(I, cell_of(p3), n + 1, rec, eB.xy, eB.xz, eB.yz, eB.yx, eB.zx, eB.zy);
+            eBx = eB.xy + eB.xz;
+            eBy = eB.yz + eB.yx;
+            eBz = eB.zx + eB.zy;
+            if (rec) store(off(p3), eB, false);
+        }
+        put3(sEB + par * 3 * NT, eBx, eBy, eBz);
+        // ---- step n + 1, H, plane i - 3 (its E totals were exchanged an iteration ago)
+        if (d4 && own) {
+            const T* const s3 = sEB + prv * 3 * NT;
+            const T ey_xm = p4 > 0 ? eB4y : T(0), ez_xm = p4 > 0 ? eB4z : T(0);
+            const T ez_ym = y > 0 ? s3[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s3[tid - 64] : T(0);
+            const T ey_zm = z > 0 ? s3[NT + tid - 1] : T(0), ex_zm = z > 0 ? s3[tid - 1] : T(0);
+            Split6<T> hB;
+            hB.xy = c4.da_xy * hA3.xy + (c4.db_xy2 * eB3z - c4.db_xy1 * ez_ym);
+            hB.xz = c4.da_xz * hA3.xz + (c4.db_xz2 * eB3y - c4.db_xz1 * ey_zm);
+            hB.yz = c4.da_yz * hA3.yz + (c4.db_yz2 * eB3x - c4.db_yz1 * ex_zm);
+            hB.yx = c4.da_yx * hA3.yx + (c4.db_yx2 * eB3z - c4.db_yx1 * ez_xm);
+            hB.zx = c4.da_zx * hA3.zx + (c4.db_zx2 * eB3y - c4.db_zx1 * ey_xm);
+            hB.zy = c4.da_zy * hA3.zy + (c4.db_zy2 * eB3x - c4.db_zy1 * ex_ym);
+            if (hprobes && bit(hw3, p4))
+                inject_cell((const ProbeGroup<T>*)I.hgroups, I.nhgroups, cell_of(p4), n + 1, true, hB.xy, hB.xz,
+                            hB.yz, hB.yx, hB.zx, hB.zy);
+            store(off(p4), hB, true);
+        }
+        eB4y = eB3y;
+        eB4z = eB3z;
+        eB3x = eBx;
+        eB3y = eBy;
+        eB3z = eBz;
+        hA3 = hA2;
+        hA2 = hA;
+        eA2 = eA1;
+        eA1 = eA;
+        h0p = h0c;
+        h0c = hN;
+        hN = hNN;
+        e0c = eN;
+        ek2 = ek1; ek1 = ek0; ek0 = ekN;
+        hk3 = hk2; hk2 = hk1; hk1 = hk0; hk0 = hkN;
+        ew2 = ew1; ew1 = ew0; ew0 = ewN;
+        hw3 = hw2; hw2 = hw1; hw1 = hw0; hw0 = hwN;
+    }
+}
+
 // two != 0: steps n and n + 1 (n from base/local); the phasor monitors' sample rows of step n's
 // staging slot are written by logical workgroup 0 (k_monitors writes step n + 1's).
-template <typename T, int TY>
+template <typename T, int TY, bool SKEW>
 __global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __restrict__ insts,
                                                             const MemberDev* __restrict__ mem, int nmem, int total,
                                                             const int* base, int local, int io, int two,
                                                             const MonitorDev* __restrict__ mons, int nmon, int slot,
                                                             int* __restrict__ rows) {
     __shared__ int s_member;
-    __shared__ T sm[15 * (TY + 4) * 64];
+    __shared__ T sm[(SKEW ? 24 : 15) * (TY + 4) * 64];
     const int L = xcd_logical(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         int m = 0;
@@ -2332,12 +2550,43 @@ __global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __res
     const int pb = tile % I->npb, xc = tile / I->npb;
     const int y0 = (pb / tz_n) * TY, z0 = (pb % tz_n) * kP2Fz;
     const int x0 = xc * I->xplanes, x1 = min(I->Nx, x0 + I->xplanes);
-    if (I->coef_kind == 1)
-        two_step_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, inst, x0, x1, y0, z0, n,
-                                       io != 0, two != 0, mons, nm, slot, sm);
-    else
-        two_step_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, inst, x0, x1, y0, z0, n, io != 0, two != 0, mons, nm,
-                                     slot, sm);
+    // the monitors this tile samples at step n: of this instance, sampling at n, with their plane
+    // through the tile's owned cells (usually none); wavefront 0 compacts them into LDS
+    __shared__ int s_mlist[kP2Mon];
+    __shared__ int s_mcount;
+    int nmt = 0;
+    if (nm) {  // uniform over the workgroup
+        if (threadIdx.x < 64) {
+            const int k = (int)threadIdx.x;
+            bool hit = false;
+            if (k < nm) {
+                const MonitorDev& Mk = mons[k];
+                if (Mk.inst == inst && n % Mk.n_step == 0)
+                    hit = Mk.axis == 0 ? (Mk.pos >= x0 && Mk.pos < x1)
+                        : Mk.axis == 1 ? (Mk.pos >= y0 && Mk.pos < y0 + TY)
+                                       : (Mk.pos >= z0 && Mk.pos < z0 + kP2Fz);
+            }
+            const uint64_t ball = __ballot(hit);
+            if (hit) s_mlist[__popcll(ball & ((1ull << k) - 1ull))] = k;
+            if (k == 0) s_mcount = __popcll(ball);
+        }
+        __syncthreads();
+        nmt = __builtin_amdgcn_readfirstlane(s_mcount);
+    }
+    if (SKEW && two) {
+        if (I->coef_kind == 1)
+            two_step_skew_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0,
+                                                    z0, n, io != 0, mons, s_mlist, nmt, slot, sm);
+        else
+            two_step_skew_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, mons, s_mlist, nmt,
+                                                  slot, sm);
+    } else if (I->coef_kind == 1) {
+        two_step_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0, z0, n,
+                                           io != 0, two != 0, mons, s_mlist, nmt, slot, sm);
+    } else {
+        two_step_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, two != 0, mons, s_mlist, nmt,
+                                         slot, sm);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2544,6 +2793,7 @@ struct Session {
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
     bool fused_deep = false;   // fused step with E one plane ahead of H (one barrier per plane)
     bool two_step = false;     // two-step pass (k_step2): pairs of steps per launch; fused = its rows
+    bool two_skew = false;     // the two-step pass's one-barrier form (variant 9)
     // streamed time-domain monitors: a device ring of stream_ring sample planes per monitor, copied
     // to the caller's host arrays on copy_stream after every run segment
     int stream_ring = 0;
@@ -3409,12 +3659,16 @@ void launch_step2(Session& s, hipStream_t st, const int* base, int local, bool t
     const InstDev<T>* di = (const InstDev<T>*)(s.parity ? s.d_insts2 : s.d_insts);
     const int io = s.tuning ? 0 : 1;
     if (ev) HIP_OK(hipEventRecord(ev[0], st));
-    if (s.fused == kP2TyShort)
-        k_step2<T, kP2TyShort><<<grid, (kP2TyShort + 4) * 64, 0, st>>>(
+    if (s.two_skew)
+        k_step2<T, kP2TySkew, true><<<grid, (kP2TySkew + 4) * 64, 0, st>>>(
+            di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
+            slot, s.d_rows);
+    else if (s.fused == kP2TyShort)
+        k_step2<T, kP2TyShort, false><<<grid, (kP2TyShort + 4) * 64, 0, st>>>(
             di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
             slot, s.d_rows);
     else
-        k_step2<T, kP2TyTall><<<grid, (kP2TyTall + 4) * 64, 0, st>>>(
+        k_step2<T, kP2TyTall, false><<<grid, (kP2TyTall + 4) * 64, 0, st>>>(
             di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
             slot, s.d_rows);
     if (ev) {
@@ -3769,18 +4023,20 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const int v0 = s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
+    const int v0 = s.two_skew ? 9 : s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
                  : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
     const int blk0 = s.block;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
     // variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row tiles, 3 fused 14-row tiles,
     // 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead of H, 7 and 8 the two-step
-    // pass with 12- and 4-row tiles
+    // pass with 12- and 4-row tiles, 9 its one-barrier form with 8-row tiles
     auto apply = [&](int v, int blk, const std::vector<int>& xr) {
         s.pipelined = v != 0;
-        s.two_step = v == 7 || v == 8;
-        s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
+        s.two_step = v == 7 || v == 8 || v == 9;
+        s.two_skew = v == 9;
+        s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : v == 9 ? kP2TySkew
+                : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
         s.fused_deep = v == 5 || v == 6;
         s.nontemporal = v == 4;
         s.block = blk;
@@ -3808,7 +4064,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     HIP_OK(hipGetLastError());
     try {
         const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
-        for (int v : {1, 4, 0, 2, 3, 5, 6, 7, 8})
+        for (int v : {1, 4, 0, 2, 3, 5, 6, 7, 8, 9})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
                     const bool fused_v = v == 2 || v == 3 || v >= 5;
@@ -4824,10 +5080,11 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 8) RF_THROW(RFDTD_EINVAL, "variant must be in 0..8");
+        if (variant < 0 || variant > 9) RF_THROW(RFDTD_EINVAL, "variant must be in 0..9");
         S.pipelined = variant != 0;
-        S.two_step = variant == 7 || variant == 8;
-        S.fused = variant == 7 ? kP2TyTall : variant == 8 ? kP2TyShort
+        S.two_step = variant == 7 || variant == 8 || variant == 9;
+        S.two_skew = variant == 9;
+        S.fused = variant == 7 ? kP2TyTall : variant == 8 ? kP2TyShort : variant == 9 ? kP2TySkew
                 : (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
         S.fused_deep = variant == 5 || variant == 6;
         S.nontemporal = variant == 4;

@@ -148,12 +148,16 @@ class DesignLoop:
         self.eps = np.array(model.eps, dtype=np.float64)  # the session's current cell permittivity
         self.sigma = np.array(model.sigma, dtype=np.float64)
         self.timing = {}
+        self.generation = 0
+        self._pending = None  # (generation, dJ/dy of that forward) until its backward runs
 
-    def evaluate(self, eps_r):
-        """(J, dJ/deps_r over the box, S_ba) for relative permittivity eps_r of the box cells."""
+    def forward(self, eps_r):
+        """(J, S_ba) for relative permittivity eps_r of the box cells: the box's coefficients
+        recomputed on the GPU, then the forward solve (recording what a reverse pass needs).  The
+        reverse pass runs only if backward() is called before the next forward()."""
         import time
 
-        from .adjoint import cb_to_eps_r_gradient, sparam_objective
+        from .adjoint import sparam_objective
 
         s, model = self.session, self.model
         t0 = time.perf_counter()
@@ -173,13 +177,36 @@ class DesignLoop:
         s.read()
         series = np.array([p["values"][:self.Nt] for p in probes])
         J, S, g = sparam_objective(model, series, self.frequency, self.b_port, self.a_port)
+        self.generation += 1
+        self._pending = (self.generation, g)
+        self.timing = dict(update_ms=(t1 - t0) * 1e3, forward_ms=t_fwd, adjoint_ms=0.0,
+                           wall_ms=(time.perf_counter() - t0) * 1e3)
+        return J, S
+
+    def backward(self, generation=None):
+        """dJ/deps_r over the box for the last forward() (generation: the forward it must belong to,
+        as forward() numbers them; a later forward makes an earlier one's backward fail)."""
+        import time
+
+        from .adjoint import cb_to_eps_r_gradient
+
+        if self._pending is None or (generation is not None and generation != self._pending[0]):
+            raise RuntimeError("backward() must follow its own forward() on the design loop")
+        t0 = time.perf_counter()
+        _, g = self._pending
+        self._pending = None
+        s, model = self.session, self.model
         grad_cb = s.gradient(self.gbox, [row.astype(self.dtype) for row in g])
-        t_adj = s.timing()[0]
+        self.timing["adjoint_ms"] = s.timing()[0]
         edges = edge_values(model, self.gbox, self.eps, self.sigma, self.dtype)
         grad = cb_to_eps_r_gradient(model, None, grad_cb, self.gbox, self.cbox, edges=edges)
-        self.timing = dict(update_ms=(t1 - t0) * 1e3, forward_ms=t_fwd, adjoint_ms=t_adj,
-                           wall_ms=(time.perf_counter() - t0) * 1e3)
-        return J, grad, S
+        self.timing["wall_ms"] += (time.perf_counter() - t0) * 1e3
+        return grad
+
+    def evaluate(self, eps_r):
+        """(J, dJ/deps_r over the box, S_ba) for relative permittivity eps_r of the box cells."""
+        J, S = self.forward(eps_r)
+        return J, self.backward(), S
 
     def close(self):
         self.session.close()

@@ -140,6 +140,34 @@ def test_autograd_function_backward(hip_ready):
     np.testing.assert_allclose(eps.grad.numpy(), 2.0 * g_ref, rtol=1e-6, atol=1e-9 * np.abs(g_ref).max())
 
 
+def test_autograd_function_on_a_design_loop(hip_ready):
+    """EpsRObjective on a resident DesignLoop: forward runs the forward solve only (no reverse pass
+    until backward), a backward after a later forward is refused, and the gradient torch receives
+    equals eps_r_gradient on a model rebuilt with that permittivity, bit for bit."""
+    print("[case] autograd on design loop", flush=True)
+    from rfnetwork_amd.design import DesignLoop
+
+    rng = np.random.default_rng(11)
+    cbox = workloads.design_box(48, 32, (8, 8, 6))
+    f = np.array(SMALL_F)
+    e1, e2 = rng.uniform(1.0, 4.0, size=(2, 8, 8, 6))
+    base = _small_design(e1, coef_dtype=np.float32)
+    with DesignLoop(base, cbox, f, mode="checkpoint", checkpoint_every=100) as loop:
+        t1 = torch.tensor(e1, requires_grad=True)
+        J1 = adjoint.EpsRObjective.apply(t1, loop)
+        assert loop.timing["adjoint_ms"] == 0.0  # no reverse pass yet
+        t2 = torch.tensor(e2, requires_grad=True)
+        J2 = adjoint.EpsRObjective.apply(t2, loop)
+        with pytest.raises(RuntimeError, match="must follow its own forward"):
+            J1.backward()
+        J2.backward()
+        assert loop.timing["adjoint_ms"] > 0.0
+    J_ref, g_ref, _ = adjoint.eps_r_gradient(_small_design(e2, coef_dtype=np.float32), cbox, f, checkpoint_every=100)
+    assert float(J2.detach()) == J_ref
+    np.testing.assert_array_equal(t2.grad.numpy(), g_ref)
+    assert t1.grad is None
+
+
 def test_batched_gradients_equal_single_sessions(hip_ready):
     """Reverse passes of uneven grids sharing launches give each instance exactly what a single-grid
     session gives; the checkpoint planner's interval fits and is used."""
