@@ -63,6 +63,9 @@ MEASURED_TRAFFIC = {
 MEASURED_TRAFFIC.update({
     ("dipole_slab", "raw", "k_update_e", (4, 128, 4)): 3976700000,
     ("dipole_slab", "raw", "k_update_h", (4, 128, 4)): 4683300000,
+    # the two-step pass at its tuned shape (12-row tiles, x chunks of 25 planes; 0 = the same automatic)
+    ("dipole_slab", "table", "k_step2", (7, 256, 25)): 4042600000,
+    ("dipole_slab", "table", "k_step2", (7, 256, 0)): 4042600000,
 })
 
 

@@ -43,7 +43,7 @@ extern "C" {
 #define RFDTD_HOST_TABLE 0x80u    /* sessions from host arrays: build the coefficient table on the host
                                      (default: on the device; assembled sessions always do) */
 #define RFDTD_TUNE_SEPARATE 0x100u /* rfdtd_autotune tries the separate E and H kernels only (variants 0,
-                                      1, 4: one step per pass, every field and coefficient read once) */
+                                      1, 4, 10: one step per pass, every field and coefficient read once) */
 
 /*
  * Coefficient order: the keys of the reference's coefficient dictionary (solver.py:1473-1518):
@@ -156,7 +156,7 @@ int rfdtd_memory_info(const rfdtd_session* s, uint64_t* device_bytes, int32_t* c
  * the automatic choice that gives enough workgroups to fill the chip); 0 keeps the current value. */
 int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, int32_t graph_steps);
 /* Launch autotuner (no counterpart in the reference, whose CUDA path uses one fixed launch shape):
- * times every candidate {variant 1, 4, 0} x {block 128, 256, 512} x {x planes: automatic, Nx/2 ..
+ * times every candidate {variant 1, 4, 10, 0} x {block 128, 256, 512} x {x planes: automatic, Nx/2 ..
  * Nx/128} and {variant 2, 3, 5, 6, 7, 8} x {x planes} (RFDTD_TUNE_SEPARATE: the first set only) over
  * trial_steps steps of this session's grids (with its probes and monitors applied when they are
  * installed, restored afterwards), keeps the fastest and returns it as config_out = {variant,
@@ -208,8 +208,9 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * the two-step pass: steps n and n + 1 of a tile of 12 (8: 4) rows x 60 cells in one launch (two-cell halos
  * recomputed, the x planes skewed by one), reading one buffer of the pair and writing the other, so
  * that the fields are read and written once per two steps (single steps, and every step while a
- * design-box history is recorded, go through the same pass one step at a time).  Results are
- * identical; the choice only changes speed (the autotuner picks it). */
+ * design-box history is recorded, go through the same pass one step at a time); 10 variant 4 with
+ * nontemporal loads of the arrays streamed once per step (split fields, raw coefficient arrays, entry
+ * indices).  Results are identical; the choice only changes speed (the autotuner picks it). */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 
 /* Launch shape of the reverse pass of rfdtd_gradient(_batch) (no counterpart in the reference):

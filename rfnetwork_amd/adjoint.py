@@ -228,11 +228,11 @@ def eps_r_gradient(model, cbox, frequency, b_port=2, a_port=1, checkpoint_every=
         session = Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags)
     history = _use_history(mode, [gbox], Nt, dtype)
     with session as s:
+        if history:  # before tuning: a recorded history rules out the two-step pass
+            s.set_history(gbox, Nt)
         if tune:
             s.autotune()
-        if history:
-            s.set_history(gbox, Nt)
-        else:
+        if not history:
             s.set_checkpointing(checkpoint_every)
         s.set_sources(probes, [], Nt)
         s.run(Nt)
@@ -278,11 +278,11 @@ def eps_r_gradients_batch(jobs, b_port=2, a_port=1, checkpoint_every=0, dtype=np
     batch = Batch.assembled([j[0] for j in jobs], dtype=dtype, flags=flags) if device else \
         Batch(grids, dtype=dtype, flags=flags)
     with batch as b:
+        if history:  # before tuning: a recorded history rules out the two-step pass
+            b.set_history([(k, p[2]) for k, p in enumerate(packed)], Nt_all)
         if tune:
             info["launch"] = b.autotune()
-        if history:
-            b.set_history([(k, p[2]) for k, p in enumerate(packed)], Nt_all)
-        else:
+        if not history:
             every = int(checkpoint_every)
             if every <= 0:
                 every, info["planned_bytes"] = b.plan_checkpoints(Nt_all, [(k, p[2]) for k, p in enumerate(packed)],

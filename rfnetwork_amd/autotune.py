@@ -128,6 +128,8 @@ def tune(session, grids, dtype, coef_kind, trial_steps=20, use_cache=True):
     key = shape_key(grids, dtype, coef_kind)
     if getattr(session, "flags", 0) & _lib.TUNE_SEPARATE:  # a restricted candidate set tunes apart
         key += "|separate"
+    if getattr(session, "_history", False):  # so does a session recording a design-box history
+        key += "|history"
     trial_steps = trial_steps_for(grids, trial_steps, getattr(session, "Nt", None))
     _adjoint_from_cache(session, key, use_cache)
     cfg = _memory.get(key)

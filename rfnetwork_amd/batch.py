@@ -182,6 +182,7 @@ class Batch:
         boxes = (ctypes.c_int32 * max(6 * n, 1))(*[int(v) for r in requests for v in r[1]])
         check(self._L.rfdtd_set_history(self._h, n, ctypes.cast(inst, ctypes.c_void_p), ctypes.cast(boxes, ctypes.c_void_p),
                                         int(Nt)), "rfdtd_set_history")
+        self._history = bool(requests)
 
     def plan_checkpoints(self, Nt, requests, budget_bytes=0):
         """Checkpoint interval for gradients of requests [(instance, box), ...] after Nt steps that

@@ -116,6 +116,16 @@ __device__ __forceinline__ void stf(GLOBAL_AS T* base, uint32_t boff, T v) {
         stg(base, boff, v);
 }
 
+// A load with the nontemporal hint (NL = true): arrays streamed once per step (split fields, raw
+// coefficient arrays, entry indices), whose lines need not stay in the caches.
+template <bool NL, class T>
+__device__ __forceinline__ T ldn(const GLOBAL_AS T* base, uint32_t boff) {
+    if constexpr (NL)
+        return __builtin_nontemporal_load((const GLOBAL_AS T*)((const GLOBAL_AS char*)base + boff));
+    else
+        return ldg(base, boff);
+}
+
 // E coefficients of one cell, order of the reference dictionary entries 0..11.
 template <typename T>
 struct ECoef {
@@ -173,6 +183,25 @@ struct CoefRaw {
     __device__ __forceinline__ T at_key(uint32_t key, int k) const { return at(key, k); }
     __device__ __forceinline__ ECoef<T> e_at(uint32_t key) const { return e(key); }
     __device__ __forceinline__ HCoef<T> h_at(uint32_t key) const { return h(key); }
+    // the same loads with the nontemporal hint when NL
+    template <bool NL>
+    __device__ __forceinline__ ECoef<T> e_s(uint32_t b) const {
+        auto c = [&](int k) { return ldn<NL>((const GLOBAL_AS T*)cp[k], b); };
+        ECoef<T> r;
+        r.ca_xy = c(0); r.ca_xz = c(1); r.cb_xy = c(2); r.cb_xz = c(3);
+        r.ca_yz = c(4); r.ca_yx = c(5); r.cb_yz = c(6); r.cb_yx = c(7);
+        r.ca_zx = c(8); r.ca_zy = c(9); r.cb_zx = c(10); r.cb_zy = c(11);
+        return r;
+    }
+    template <bool NL>
+    __device__ __forceinline__ HCoef<T> h_s(uint32_t b) const {
+        auto c = [&](int k) { return ldn<NL>((const GLOBAL_AS T*)cp[k], b); };
+        HCoef<T> r;
+        r.da_xy = c(12); r.da_xz = c(13); r.db_xy1 = c(14); r.db_xy2 = c(15); r.db_xz1 = c(16); r.db_xz2 = c(17);
+        r.da_yz = c(18); r.da_yx = c(19); r.db_yz1 = c(20); r.db_yz2 = c(21); r.db_yx1 = c(22); r.db_yx2 = c(23);
+        r.da_zx = c(24); r.da_zy = c(25); r.db_zx1 = c(26); r.db_zx2 = c(27); r.db_zy1 = c(28); r.db_zy2 = c(29);
+        return r;
+    }
     __device__ __forceinline__ HCoef<T> h(uint32_t b) const {
         auto c = [&](int k) { return ldg((const GLOBAL_AS T*)cp[k], b); };
         HCoef<T> r;
@@ -214,6 +243,15 @@ struct CoefTable {
     }
     __device__ __forceinline__ ECoef<T> e(uint32_t bc) const { return e_at(ekey(bc)); }
     __device__ __forceinline__ HCoef<T> h(uint32_t bc) const { return h_at(hkey(bc)); }
+    // the entry index streamed (nontemporal hint when NL), the table entry cached
+    template <bool NL>
+    __device__ __forceinline__ ECoef<T> e_s(uint32_t bc) const {
+        return e_at(ldn<NL>((const GLOBAL_AS uint32_t*)eidx, bc / (uint32_t)sizeof(T) * 4u));
+    }
+    template <bool NL>
+    __device__ __forceinline__ HCoef<T> h_s(uint32_t bc) const {
+        return h_at(ldn<NL>((const GLOBAL_AS uint32_t*)hidx, bc / (uint32_t)sizeof(T) * 4u));
+    }
     __device__ __forceinline__ ECoef<T> e_at(uint32_t k) const {
         const GLOBAL_AS Vec4<T>* tb = (const GLOBAL_AS Vec4<T>*)etab;
         const uint32_t o = k * (uint32_t)(3 * sizeof(Vec4<T>));
@@ -431,7 +469,7 @@ struct HIn {
 
 // E half step over one column of x planes.  Offsets are 32-bit byte offsets (grids up to 2^29
 // cells).
-template <typename T, class CA, bool PF, bool NT>
+template <typename T, class CA, bool PF, bool NT, bool NL = false>
 __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
                                          const ProbeFold<T>& P) {
     using G = GLOBAL_T(T);
@@ -451,10 +489,10 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
         v.hx_yn = yin ? ldg(hx, b + sy) : T(0);
         v.hy_zn = zin ? ldg(hy, b + S) : T(0);
         v.hx_zn = zin ? ldg(hx, b + S) : T(0);
-        v.c = C.e(b);
+        v.c = C.template e_s<NL>(b);
         v.pw = P.word(b / S);
-        v.exy = ldg(ex_y, b); v.exz = ldg(ex_z, b); v.eyz = ldg(ey_z, b);
-        v.eyx = ldg(ey_x, b); v.ezx = ldg(ez_x, b); v.ezy = ldg(ez_y, b);
+        v.exy = ldn<NL>(ex_y, b); v.exz = ldn<NL>(ex_z, b); v.eyz = ldn<NL>(ey_z, b);
+        v.eyx = ldn<NL>(ey_x, b); v.ezx = ldn<NL>(ez_x, b); v.ezy = ldn<NL>(ez_y, b);
     };
     if (t.x0 >= t.x1) return;
     uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
@@ -489,7 +527,7 @@ __device__ __forceinline__ void e_column(const Fields<T>& F, const CA& C, int Nx
 }
 
 // H half step over one column of x planes; neighbours below index 0 are zero.
-template <typename T, class CA, bool PF, bool NT>
+template <typename T, class CA, bool PF, bool NT, bool NL = false>
 __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
                                          const ProbeFold<T>& P) {
     using G = GLOBAL_T(T);
@@ -508,10 +546,10 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
         v.ex_ym = ylo ? ldg(ex, b - sy) : T(0);
         v.ey_zm = zlo ? ldg(ey, b - S) : T(0);
         v.ex_zm = zlo ? ldg(ex, b - S) : T(0);
-        v.c = C.h(b);
+        v.c = C.template h_s<NL>(b);
         v.pw = P.word(b / S);
-        v.hxy = ldg(hx_y, b); v.hxz = ldg(hx_z, b); v.hyz = ldg(hy_z, b);
-        v.hyx = ldg(hy_x, b); v.hzx = ldg(hz_x, b); v.hzy = ldg(hz_y, b);
+        v.hxy = ldn<NL>(hx_y, b); v.hxz = ldn<NL>(hx_z, b); v.hyz = ldn<NL>(hy_z, b);
+        v.hyx = ldn<NL>(hy_x, b); v.hzx = ldn<NL>(hz_x, b); v.hzy = ldn<NL>(hz_y, b);
     };
     if (t.x0 >= t.x1) return;
     uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
@@ -546,7 +584,7 @@ __device__ __forceinline__ void h_column(const Fields<T>& F, const CA& C, int Nx
 }
 
 // io != 0: the instance's E (H) probes are applied inside the kernel at step base/local.
-template <typename T, bool PF, bool NT = false>
+template <typename T, bool PF, bool NT = false, bool NL = false>
 __global__ void __launch_bounds__(1024) k_update_e(const InstDev<T>* __restrict__ insts,
                                                    const MemberDev* __restrict__ mem, int nmem, int total,
                                                    const int* base, int local, int io) {
@@ -557,12 +595,13 @@ __global__ void __launch_bounds__(1024) k_update_e(const InstDev<T>* __restrict_
     const int ng = io ? I->negroups : 0;
     const ProbeFold<T> P{I->ebits, (const ProbeGroup<T>*)I->egroups, ng, ng ? step_of(base, local) : 0};
     if (I->coef_kind == 1)
-        e_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t, P);
+        e_column<T, CoefTable<T>, PF, NT, NL>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny,
+                                              I->Nz, t, P);
     else
-        e_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+        e_column<T, CoefRaw<T>, PF, NT, NL>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
 }
 
-template <typename T, bool PF, bool NT = false>
+template <typename T, bool PF, bool NT = false, bool NL = false>
 __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict__ insts,
                                                    const MemberDev* __restrict__ mem, int nmem, int total,
                                                    const int* base, int local, int io) {
@@ -573,9 +612,10 @@ __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict_
     const int ng = io ? I->nhgroups : 0;
     const ProbeFold<T> P{I->hbits, (const ProbeGroup<T>*)I->hgroups, ng, ng ? step_of(base, local) : 0};
     if (I->coef_kind == 1)
-        h_column<T, CoefTable<T>, PF, NT>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny, I->Nz, t, P);
+        h_column<T, CoefTable<T>, PF, NT, NL>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny,
+                                              I->Nz, t, P);
     else
-        h_column<T, CoefRaw<T>, PF, NT>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+        h_column<T, CoefRaw<T>, PF, NT, NL>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
 }
 
 
@@ -2132,11 +2172,12 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         }
     };
     const bool eprobes = io && I.negroups, hprobes = io && I.nhgroups;
-    auto cell_of = [&](int x) { return ((int64_t)x * Ny + y) * Nz + z; };
+    // cell index of a valid lane's cell on plane x (grids hold under 2^32 bytes per array: 32 bits)
+    auto cell_of = [&](int x) { return ((uint32_t)x * (uint32_t)Ny + (uint32_t)y) * (uint32_t)Nz + (uint32_t)z; };
     // probe bitmask words of a plane's cell, loaded with the plane's inputs (0 without probes)
-    auto ewd = [&](int x) { return eprobes && mAe && x < a1 ? ldg((const G32*)I.ebits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
-    auto hwd = [&](int x) { return hprobes && mAh && x < a1 ? ldg((const G32*)I.hbits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
-    auto bit = [&](uint32_t word, int x) { return ((word >> (uint32_t)(cell_of(x) & 31)) & 1u) != 0; };
+    auto ewd = [&](int x) { return eprobes && mAe && x < a1 ? ldg((const G32*)I.ebits, (cell_of(x) >> 5) * 4u) : 0u; };
+    auto hwd = [&](int x) { return hprobes && mAh && x < a1 ? ldg((const G32*)I.hbits, (cell_of(x) >> 5) * 4u) : 0u; };
+    auto bit = [&](uint32_t word, int x) { return ((word >> (cell_of(x) & 31u)) & 1u) != 0; };
 
     Split6<T> h0 = load_h(a0), e0 = load_e(a0);  // input H and E splits of plane i
     uint32_t ek = mAe ? C.ekey(off(a0)) : 0u, hk = mAh ? C.hkey(off(a0)) : 0u;
@@ -2369,10 +2410,10 @@ __device__ __forceinline__ void two_step_skew_tile(const InstDev<T>& I, const CA
         buf[2 * NT + tid] = c;
     };
     const bool eprobes = io && I.negroups, hprobes = io && I.nhgroups;
-    auto cell_of = [&](int x) { return ((int64_t)x * Ny + y) * Nz + z; };
-    auto ewd = [&](int x) { return eprobes && mAe && x < aHi ? ldg((const G32*)I.ebits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
-    auto hwd = [&](int x) { return hprobes && mAh && x < aHi ? ldg((const G32*)I.hbits, (uint32_t)(cell_of(x) >> 5) * 4u) : 0u; };
-    auto bit = [&](uint32_t word, int x) { return ((word >> (uint32_t)(cell_of(x) & 31)) & 1u) != 0; };
+    auto cell_of = [&](int x) { return ((uint32_t)x * (uint32_t)Ny + (uint32_t)y) * (uint32_t)Nz + (uint32_t)z; };
+    auto ewd = [&](int x) { return eprobes && mAe && x < aHi ? ldg((const G32*)I.ebits, (cell_of(x) >> 5) * 4u) : 0u; };
+    auto hwd = [&](int x) { return hprobes && mAh && x < aHi ? ldg((const G32*)I.hbits, (cell_of(x) >> 5) * 4u) : 0u; };
+    auto bit = [&](uint32_t word, int x) { return ((word >> (cell_of(x) & 31u)) & 1u) != 0; };
     auto ekey = [&](int x) { return mAe && x < aHi ? C.ekey(off(x)) : 0u; };
     auto hkey = [&](int x) { return mAh && x < aHi ? C.hkey(off(x)) : 0u; };
 
@@ -2791,6 +2832,11 @@ struct Session {
     bool tuning = false;    // autotune trials: field updates only, no probes or monitors
     int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
+    bool nt_loads = false;     // ... and nontemporal loads of the arrays streamed once (variant 10)
+    // the separate kernels' own launch shape (threads, x planes per workgroup, 0 = automatic), which
+    // a gradient's recomputation uses whatever the forward variant: the tuner's best of variants
+    // 0, 1, 4, 10
+    int sep_block = 256, sep_xplanes = 0;
     bool fused_deep = false;   // fused step with E one plane ahead of H (one barrier per plane)
     bool two_step = false;     // two-step pass (k_step2): pairs of steps per launch; fused = its rows
     bool two_skew = false;     // the two-step pass's one-barrier form (variant 9)
@@ -3751,7 +3797,10 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
             else
                 k_update_e_save<T, false, false><<<gs, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base,
                                                                         local, pio, hb, s.d_hist_hb, nb);
-        } else if (s.nontemporal)
+        } else if (s.nt_loads)
+            k_update_e<T, true, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local,
+                                                                     pio);
+        else if (s.nontemporal)
             k_update_e<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         else if (s.pipelined)
             k_update_e<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
@@ -3761,7 +3810,10 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
             HIP_OK(hipEventRecord(ev[1], st));
             HIP_OK(hipEventRecord(ev[2], st));
         }
-        if (s.nontemporal)
+        if (s.nt_loads)
+            k_update_h<T, true, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local,
+                                                                     pio);
+        else if (s.nontemporal)
             k_update_h<T, true, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
         else if (s.pipelined)
             k_update_h<T, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
@@ -4023,7 +4075,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const int v0 = s.two_skew ? 9 : s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
+    const int v0 = s.nt_loads ? 10 : s.two_skew ? 9 : s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
                  : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
     const int blk0 = s.block;
     std::vector<int> xr0;
@@ -4038,15 +4090,16 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : v == 9 ? kP2TySkew
                 : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
         s.fused_deep = v == 5 || v == 6;
-        s.nontemporal = v == 4;
+        s.nontemporal = v == 4 || v == 10;
+        s.nt_loads = v == 10;
         s.block = blk;
         for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
         s.dirty = true;
         s.graph_dirty = true;
     };
     hipStream_t st = s.own_stream;
-    double best = 1e300;
-    int bv = v0, bblk = blk0, bxp = 0;
+    double best = 1e300, best_sep = 1e300;
+    int bv = v0, bblk = blk0, bxp = 0, sblk = s.sep_block, sxp = s.sep_xplanes;
     // With sources installed (and room for the trial steps) the trials apply them as a run does, so
     // that their cost is timed too; the probe rows and monitor values are restored afterwards and
     // the design-box history slots are rewritten by the run.  Otherwise field updates only.
@@ -4064,11 +4117,14 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     HIP_OK(hipGetLastError());
     try {
         const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
-        for (int v : {1, 4, 0, 2, 3, 5, 6, 7, 8, 9})
+        for (int v : {1, 4, 10, 0, 2, 3, 5, 6, 7, 8, 9})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
-                    const bool fused_v = v == 2 || v == 3 || v >= 5;
+                    const bool fused_v = v == 2 || v == 3 || (v >= 5 && v <= 9);
                     if (fused_v && separate_only) continue;
+                    // a design-box history needs H after every step: the two-step pass would run
+                    // single steps
+                    if (v >= 7 && v <= 9 && !s.hist.empty()) continue;
                     if (fused_v && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
                     apply(v, blk, std::vector<int>(s.members.size(), xp));
                     refresh<T>(s);
@@ -4091,6 +4147,11 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
                         bblk = blk;
                         bxp = xp;
                     }
+                    if (!fused_v && per < best_sep) {
+                        best_sep = per;
+                        sblk = blk;
+                        sxp = xp;
+                    }
                 }
     } catch (...) {
         s.tuning = false;
@@ -4099,6 +4160,8 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     }
     s.tuning = false;
     apply(bv, bblk, std::vector<int>(s.members.size(), bxp));
+    s.sep_block = sblk;
+    s.sep_xplanes = sxp;
     HIP_OK(hipDeviceSynchronize());
     for (Instance& I : s.insts) {
         HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
@@ -4444,6 +4507,19 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     std::vector<BoxDev<T>> bd(n);
     std::vector<int64_t> cells(n), hcells(n), gcells(n);
     std::vector<ProbeGroup<T>> gE, gH, aE, aH;
+    // the recomputation runs the separate kernels at their own launch shape (sep_block threads; x
+    // planes as tuned, else enough workgroups to fill the chip), whatever variant ran the forward
+    const int rb = s.sep_block;
+    int64_t rbpp = 0;
+    for (const GradItem& it : items) {
+        const Member& M = s.members[s.insts[it.inst].member];
+        rbpp += ((int64_t)M.Ny * M.Nz + rb - 1) / rb;
+    }
+    auto rxplanes = [&](const Member& M) {
+        if (s.sep_xplanes > 0) return std::min(s.sep_xplanes, M.Nx);
+        const int64_t chunks = std::min<int64_t>(M.Nx, std::max<int64_t>(1, (2048 + rbpp - 1) / rbpp));
+        return (int)((M.Nx + chunks - 1) / chunks);
+    };
     int tile0 = 0;
     for (int k = 0; k < n; ++k) {
         const GradItem& it = items[k];
@@ -4452,9 +4528,11 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         void* d_adj = buf.alloc(18 * I.pitch);
         d_re[k] = hist_mode ? d_adj : buf.alloc(18 * I.pitch);  // no recomputation in history mode
         HIP_OK(hipMemsetAsync(d_adj, 0, 18 * I.pitch, st));
-        const int npb = (int)(((int64_t)M.Ny * M.Nz + s.block - 1) / s.block);
-        const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
+        const int npb = (int)(((int64_t)M.Ny * M.Nz + rb - 1) / rb);
+        const int rxp = rxplanes(M);
+        const int chunks = (M.Nx + rxp - 1) / rxp;
         di[k] = make_instdev<T>(M, d_re[k], I.pitch, npb);
+        di[k].xplanes = rxp;
         // the recomputation applies the sources inside the update kernels, from the pristine series;
         // the records it writes equal the forward run's (the same operations on the same state)
         di[k].ebits = I.d_ebits;
@@ -4523,7 +4601,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         int tiles = 0;
         for (int k = 0; k < n; ++k) {
             const Member& M = s.members[s.insts[items[k].inst].member];
-            const int xp = chunks > 0 ? (M.Nx + chunks - 1) / chunks : M.xplanes;
+            const int xp = chunks > 0 ? (M.Nx + chunks - 1) / chunks : rxplanes(M);
             const int npb = (int)(((int64_t)M.Ny * M.Nz + block - 1) / block);
             const int nch = (M.Nx + xp - 1) / xp;
             di[n + k].npb = npb;
@@ -4556,7 +4634,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
             }
         }
     }
-    int ablock = s.adj_block > 0 ? s.adj_block : s.block;
+    int ablock = s.adj_block > 0 ? s.adj_block : rb;
     int atiles = adj_tiling(ablock, s.adj_block > 0 ? s.adj_chunks : 0);
     const InstDev<T>* d_di = (const InstDev<T>*)upload(di.data(), di.size() * sizeof(InstDev<T>));
     const MemberDev* d_md = (const MemberDev*)upload(md.data(), md.size() * sizeof(MemberDev));
@@ -4653,10 +4731,8 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         for (int64_t t = n0; t < n1 && !hist_mode; ++t) {
             const int j = (int)(t - n0);
             k_save_h<T><<<hblocks, 256, 0, st>>>(d_di, d_bd, d_h0, n, j);
-            if (s.pipelined) k_update_e<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
-            else k_update_e<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
-            if (s.pipelined) k_update_h<T, true><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
-            else k_update_h<T, false><<<grid, s.block, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
+            k_update_e<T, true><<<grid, rb, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
+            k_update_h<T, true><<<grid, rb, 0, st>>>(d_di, d_md, n, tile0, nullptr, (int)t, 1);
         }
         for (int64_t t = n1 - 1; t >= n0; --t) {
             const int j = (int)(t - n0);
@@ -5080,14 +5156,15 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 9) RF_THROW(RFDTD_EINVAL, "variant must be in 0..9");
+        if (variant < 0 || variant > 10) RF_THROW(RFDTD_EINVAL, "variant must be in 0..10");
         S.pipelined = variant != 0;
         S.two_step = variant == 7 || variant == 8 || variant == 9;
         S.two_skew = variant == 9;
         S.fused = variant == 7 ? kP2TyTall : variant == 8 ? kP2TyShort : variant == 9 ? kP2TySkew
                 : (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
         S.fused_deep = variant == 5 || variant == 6;
-        S.nontemporal = variant == 4;
+        S.nontemporal = variant == 4 || variant == 10;
+        S.nt_loads = variant == 10;
         S.dirty = true;
         S.graph_dirty = true;
         if (!S.fused) release_second_buffers(S);

@@ -142,6 +142,8 @@ class DesignLoop:
         self.gbox = adjoint.gpu_box_for_cells(model, cbox)
         self.history = adjoint._use_history(mode, [self.gbox], Nt, dtype)
         self.session = Session.assembled(model, dtype, flags)  # coefficients assembled on the GPU
+        if self.history:  # recorded every evaluation; set before tuning, which it constrains
+            self.session.set_history(self.gbox, self.Nt)
         if tune:
             self.session.autotune()
         self.checkpoint_every = checkpoint_every

@@ -186,6 +186,7 @@ class Session:
         b = (ctypes.c_int32 * 6)(*[int(v) for v in box])
         check(self._L.rfdtd_set_history(self._h, 1, ctypes.cast(inst, ctypes.c_void_p), ctypes.cast(b, ctypes.c_void_p),
                                         int(Nt)), "rfdtd_set_history")
+        self._history = True
 
     def plan_checkpoints(self, Nt, box, budget_bytes=0):
         """Checkpoint interval for a gradient over box after an Nt-step forward run that fits device
@@ -221,7 +222,8 @@ class Session:
         0 the same one plane at a time, 2 and 3 the fused E+H pass with 6- or 14-row tiles (two field
         buffers per instance), 4 variant 1 with nontemporal field stores, 5 and 6 variants 2 and 3
         with deeper load pipelining, 7 and 8 the two-step pass (steps n and n + 1 of a 12- or 4-row x
-        60-cell tile per launch, two field buffers per instance).  Results are identical."""
+        60-cell tile per launch, two field buffers per instance), 10 variant 4 with nontemporal loads of
+        the arrays streamed once.  Results are identical."""
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def close(self):

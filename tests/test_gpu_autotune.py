@@ -22,7 +22,7 @@ def test_autotuned_session_matches_oracle(hip_ready, tmp_path, monkeypatch):
         s.set_sources(probes, monitors, Nt)
         cfg = s.autotune(trial_steps=5)
         assert cfg["cached"] is False
-        assert cfg["variant"] in (0, 1, 2, 3, 4, 5, 6) and cfg["block"] in (128, 256, 512) and cfg["x_planes"] >= 0
+        assert cfg["variant"] in range(10) and cfg["block"] in (128, 256, 512) and cfg["x_planes"] >= 0
         assert cfg["ms_per_step"] > 0
         s.run(Nt)
         s.synchronize()

@@ -227,7 +227,7 @@ def test_hip_reproduces_golden(hip_ready, dtype, name):
         np.testing.assert_array_equal(v, want[f"field_{k}"], err_msg=k)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 4, 6])
+@pytest.mark.parametrize("variant", [0, 1, 4, 6, 7, 9, 10])
 def test_boundary_plane_monitors(hip_ready, variant):
     """Time-domain monitors of all six components on the first and last planes each axis allows
     (36 monitors): identical to the oracle whether the sample is read from the stored totals (the
