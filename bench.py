@@ -381,10 +381,10 @@ def batched_leg(n_steps, flags=0):
 
 def adjoint_leg(n_steps, every, repeats=2, flags=0):
     """dJ/deps_r of the 64x64x32 design box of the 200x160x80 two-port model, J = sum |S21|^2 at
-    2..6 GHz: one gradient = forward run recording the design box's H every step + reverse pass (no
-    recomputation); the checkpointed form (checkpoints every `every` steps, segments recomputed) is
-    timed beside it.  Best of `repeats` end-to-end evaluations each (model assembly excluded, host
-    objective included); both gradients must agree bit for bit."""
+    2..6 GHz.  Headline: one gradient = forward run with checkpoints every `every` steps + reverse pass
+    recomputing each segment; beside it the form that records the design box's H every step during
+    the forward run (no recomputation).  Best of `repeats` end-to-end evaluations each (model assembly
+    excluded, host objective included); both gradients must agree bit for bit."""
     from rfnetwork_amd import adjoint, workloads
 
     model = workloads.design_strip(Nt=n_steps)
@@ -422,14 +422,18 @@ def adjoint_leg(n_steps, every, repeats=2, flags=0):
     log(f"adjoint (design loop): {min(walls):.3f}s per gradient (update {lt['update_ms']:.2f} ms, forward "
         f"{lt['forward_ms']:.1f} ms, reverse {lt['adjoint_ms']:.1f} ms), equal to the rebuilt model's: {loop_equal}")
     cells = model.Nx * model.Ny * model.Nz
+    # the headline is the configuration's own method: checkpoints every `every` steps, segments
+    # recomputed in the reverse pass; the design-box history (no recomputation) beside it
     return dict(workload=f"design_strip {model.Nx}x{model.Ny}x{model.Nz}, design box 64x64x32, "
-                         f"{n_steps} steps, design-box H history", gradients_per_s=round(1.0 / wall, 4),
-                s_per_gradient=round(wall, 3), forward_ms=round(t["forward_ms"], 1),
-                reverse_ms=round(t["adjoint_ms"], 1), design_cells=int(grad.size), J=J,
-                adjoint_launch=t["adjoint_launch"], history_bytes=int(adjoint.history_bytes([adjoint.gpu_box_for_cells(model, cbox)], n_steps)),
-                forward_mcells=round(cells * n_steps / (t["forward_ms"] * 1e-3) / 1e6, 1),
-                checkpointed=dict(every=every, gradients_per_s=round(1.0 / wc, 4), forward_ms=round(tc["forward_ms"], 1),
-                                  reverse_ms=round(tc["adjoint_ms"], 1)),
+                         f"{n_steps} steps, checkpoints every {every} steps", checkpoint_every=every,
+                gradients_per_s=round(1.0 / wc, 4), s_per_gradient=round(wc, 3),
+                forward_ms=round(tc["forward_ms"], 1), reverse_ms=round(tc["adjoint_ms"], 1),
+                design_cells=int(grad.size), J=J, adjoint_launch=t["adjoint_launch"],
+                forward_mcells=round(cells * n_steps / (tc["forward_ms"] * 1e-3) / 1e6, 1),
+                history=dict(gradients_per_s=round(1.0 / wall, 4), s_per_gradient=round(wall, 3),
+                             forward_ms=round(t["forward_ms"], 1), reverse_ms=round(t["adjoint_ms"], 1),
+                             history_bytes=int(adjoint.history_bytes([adjoint.gpu_box_for_cells(model, cbox)],
+                                                                     n_steps))),
                 modes_equal=bool(J == Jc and np.array_equal(grad, gc)),
                 design_loop=dict(gradients_per_s=round(1.0 / min(walls), 4), s_per_gradient=round(min(walls), 3),
                                  coefficient_update_ms=round(lt["update_ms"], 2), equal=loop_equal),
@@ -746,6 +750,10 @@ TABLE_BYTES = 72 + 4 + 0.125
 # coefficient arrays (120 B) instead of the indices.  Halo rows of its tiles are re-read on top.
 TWO_STEP_TABLE_BYTES = 96 + 8 + 0.25
 TWO_STEP_RAW_BYTES = 96 + 120
+# kernel variants (rfdtd_set_variant): 0, 1, 4, 10 separate E and H kernels; 2, 3, 5, 6 the fused
+# E+H pass; 7, 8, 9 the two-step pass
+TWO_STEP_VARIANTS = (7, 8, 9)
+FUSED_VARIANTS = (2, 3, 5, 6) + TWO_STEP_VARIANTS
 
 
 def plain_leg(args):
@@ -762,11 +770,11 @@ def plain_leg(args):
     step_bytes = cells * (E_BYTES + H_BYTES) + dft_b
     step_frac = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     variant = tuned.get("variant", 1) if tuned else 1
-    fused = variant in (2, 3, 5, 6, 7, 8)
+    fused = variant in FUSED_VARIANTS
     table = info["coef_kind"] == "table"
     # the dominant kernel: the two-step pass (two cell-updates per cell and launch), the fused E+H
     # pass or the H update
-    if variant in (7, 8):
+    if variant in TWO_STEP_VARIANTS:
         kname, kbytes, kms = "k_step2", (TWO_STEP_TABLE_BYTES if table else TWO_STEP_RAW_BYTES), e_ms
     elif fused:
         kname, kbytes, kms = "k_step_fused", (2 * TABLE_BYTES if table else E_BYTES + H_BYTES), e_ms
@@ -774,7 +782,7 @@ def plain_leg(args):
         kname, kbytes, kms = "k_update_h", (TABLE_BYTES if table else H_BYTES), h_ms
     shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
     traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname, shape))
-    if variant in (7, 8):
+    if variant in TWO_STEP_VARIANTS:
         model_text = ("two-step pass, per cell and launch (2 cell-updates): 12 splits read and written once "
                       "(96 B) + " + ("E and H entry indices 8 B + 2 probe bitmask words per 32 cells" if table
                                      else "30 coefficients 120 B"))
@@ -813,7 +821,7 @@ def raw_leg(args):
     like-for-like fraction of HBM peak."""
     run = timed_run(args, "raw", separate=True)
     tuned, h_ms, e_ms, cells = run["tuned"], run["h_ms"], run["e_ms"], run["cells"]
-    fused = bool(tuned and tuned.get("variant", 1) in (2, 3, 5, 6, 7, 8))
+    fused = bool(tuned and tuned.get("variant", 1) in FUSED_VARIANTS)
     if fused:
         kname, kbytes, kms = "k_step_fused", E_BYTES + H_BYTES, e_ms
     else:

@@ -2121,6 +2121,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
     using G32 = GLOBAL_AS uint32_t;
     constexpr int NT = (TY + 4) * 64;
     constexpr uint32_t S = (uint32_t)sizeof(T);
+    constexpr bool kEarly = std::is_same<CA, CoefTable<T>>::value;  // all gathers at the iteration's top
     const int Nx = I.Nx, Ny = I.Ny, Nz = I.Nz;
     const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
@@ -2180,6 +2181,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
     auto bit = [&](uint32_t word, int x) { return ((word >> (cell_of(x) & 31u)) & 1u) != 0; };
 
     Split6<T> h0 = load_h(a0), e0 = load_e(a0);  // input H and E splits of plane i
+    Split6<T> hN = load_h(a0 + 1);                // input H splits of plane i + 1
     uint32_t ek = mAe ? C.ekey(off(a0)) : 0u, hk = mAh ? C.hkey(off(a0)) : 0u;
     uint32_t ew = ewd(a0), hw = hwd(a0);         // bitmask words of plane i
     Split6<T> e1p{}, h1p{};                      // splits after step n, plane i - 1
@@ -2191,27 +2193,34 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         const int j = i - 1;
         const bool doB = two && j >= b0 && j < x1;
         const bool recA = own && i >= x0 && i < x1;
-        // this plane's E coefficients (the keys came a plane ago), then the next plane's inputs, in
-        // flight during this plane's work; each later phase's coefficients are issued a phase early
+        const bool recB = doB && own && j >= x0;
+        // Table form: every coefficient gather of this iteration first (the keys came an iteration
+        // ago), step n's E and H coefficients of plane i and step n + 1's of plane i - 1; the next
+        // planes' inputs are issued after them, once step n's E is done.  Loads complete in issue
+        // order, so a gather issued after those loads could only be waited for together with them;
+        // in this order they stay in flight into the next iteration while this plane's phases wait
+        // for the gathers.  Raw arrays (30 loads per cell instead of 8 table rows) would need more
+        // registers than the 16-wavefront workgroup allows: there each phase's coefficients are
+        // issued a phase early.
         ECoef<T> cA{};
         if (doA && mAe) cA = C.e_at(ek);
-        const Split6<T> hN = load_h(i + 1);
-        Split6<T> eN{};
-        uint32_t ekN = 0u, hkN = 0u;
-        if (i + 1 < a1) {
-            eN = load_e(i + 1);
-            if (mAe) ekN = C.ekey(off(i + 1));
-            if (mAh) hkN = C.hkey(off(i + 1));
+        HCoef<T> chA{};
+        ECoef<T> cB{};
+        HCoef<T> chB{};
+        if constexpr (kEarly) {
+            if (doA && mAh) chA = C.h_at(hk);
+            if (doB && mBe) cB = C.e_at(ekp);
+            if (recB) chB = C.h_at(hkp);
         }
-        const uint32_t ewN = ewd(i + 1), hwN = hwd(i + 1);
         // ---- step n, plane i: E
         const T h0x = h0.xy + h0.xz, h0y = h0.yz + h0.yx, h0z = h0.zx + h0.zy;
         s0[tid] = h0x;
         s0[NT + tid] = h0y;
         s0[2 * NT + tid] = h0z;
         __syncthreads();
-        HCoef<T> chA{};
-        if (doA && mAh) chA = C.h_at(hk);
+        if constexpr (!kEarly) {
+            if (doA && mAh) chA = C.h_at(hk);
+        }
         Split6<T> e1 = e0;
         T e1x = T(0), e1y = T(0), e1z = T(0);
         if (doA && mAe) {
@@ -2236,8 +2245,21 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
                 else if (nmon) p2_sample(mons, mlist, nmon, EX, i, y, z, n, slot, e1x, e1y, e1z);
             }
         }
-        ECoef<T> cB{};  // step n + 1's E coefficients of plane i - 1, in flight during H of plane i
-        if (doB && mBe) cB = C.e_at(ekp);
+        if constexpr (!kEarly) {
+            if (doB && mBe) cB = C.e_at(ekp);
+        }
+        // the inputs of the next planes, in flight until the next iteration: H splits two planes
+        // ahead (step n's E of plane i + 1 reads plane i + 2), E splits, keys and bitmask words one
+        Split6<T> hNN{};  // read by the next iteration's E of plane i + 1 while that plane is in [a0, a1)
+        if (i + 2 <= a1) hNN = load_h(i + 2);
+        Split6<T> eN{};
+        uint32_t ekN = 0u, hkN = 0u;
+        if (i + 1 < a1) {
+            eN = load_e(i + 1);
+            if (mAe) ekN = C.ekey(off(i + 1));
+            if (mAh) hkN = C.hkey(off(i + 1));
+        }
+        const uint32_t ewN = ewd(i + 1), hwN = hwd(i + 1);
         s1e[tid] = e1x;
         s1e[NT + tid] = e1y;
         s1e[2 * NT + tid] = e1z;
@@ -2266,9 +2288,9 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
                 else if (nmon) p2_sample(mons, mlist, nmon, HX, i, y, z, n, slot, h1x, h1y, h1z);
             }
         }
-        const bool recB = doB && own && j >= x0;
-        HCoef<T> chB{};  // step n + 1's H coefficients of plane i - 1, in flight during its E
-        if (recB) chB = C.h_at(hkp);
+        if constexpr (!kEarly) {
+            if (recB) chB = C.h_at(hkp);
+        }
         if (two) {
             T* const sh = s1h + (i & 1) * 3 * NT;
             sh[tid] = h1x;
@@ -2332,6 +2354,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         ewp = ew;
         hwp = hw;
         h0 = hN;
+        hN = hNN;
         e0 = eN;
         ek = ekN;
         hk = hkN;
