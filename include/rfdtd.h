@@ -251,6 +251,25 @@ int rfdtd_set_design_permittivity(rfdtd_session* s, int32_t member, const int32_
                                   const double* dz, const void* dx1, const void* dy1, const void* dz1, double dt,
                                   double e0);
 
+/* The same update from permittivity held in device memory (a ROCm torch tensor, say), with no host
+ * copy or synchronisation: the device-pointer extension of the reference's array boundary
+ * (rfnetwork/core/src/cuda_binding.cpp:20-68 takes host arrays only).  rfdtd_design_prepare takes
+ * rfdtd_set_design_permittivity's arguments without eps_new, checks the box as it does (RFDTD_EINVAL
+ * if eps_now's tuples are not the session's), and keeps the box's arrays on the device; a host update
+ * of the member (rfdtd_set_design_permittivity) discards them.
+ * rfdtd_set_design_permittivity_device then sets the relative permittivity of the cells of cbox
+ * (x0, x1, y0, y1, z0, z1, cells; inside the prepared box grown by one cell on each upper side) from
+ * eps_r (device pointer, C order, float64 if f64 else float32): eps_r * e0 (in float64) replaces the
+ * held permittivity there and the box's coefficients are recomputed from it, bit for bit as the host
+ * update would, on `stream` (NULL: the session's stream), ordered after the session's earlier work
+ * and before its later work; eps_r must stay valid until then.  Later updates start from the values
+ * set last. */
+int rfdtd_design_prepare(rfdtd_session* s, int32_t member, const int32_t gbox[6], const double* eps_now,
+                         const double* sigma, const double* dx, const double* dy, const double* dz, const void* dx1,
+                         const void* dy1, const void* dz1, double dt, double e0);
+int rfdtd_set_design_permittivity_device(rfdtd_session* s, int32_t member, const int32_t cbox[6], const void* eps_r,
+                                         int32_t f64, void* stream);
+
 int rfdtd_set_adjoint_launch(rfdtd_session* s, int32_t variant, int32_t block_threads, int32_t x_chunks);
 int rfdtd_get_adjoint_launch(rfdtd_session* s, int32_t config_out[3]);
 void rfdtd_destroy(rfdtd_session* s);

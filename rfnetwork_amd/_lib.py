@@ -57,7 +57,8 @@ EXPORTS = (
     "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
     "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
     "rfdtd_create_assembled", "rfdtd_read_coefficients", "rfdtd_create_batch_assembled",
-    "rfdtd_read_coefficient_box", "rfdtd_nf2ff_static",
+    "rfdtd_read_coefficient_box", "rfdtd_nf2ff_static", "rfdtd_design_prepare",
+    "rfdtd_set_design_permittivity_device",
 )
 
 
@@ -182,6 +183,8 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_read_coefficients.argtypes = [vp, i32, vp]
     L.rfdtd_create_batch_assembled.argtypes = [ctypes.POINTER(vp), i32, vp, vp, u32]
     L.rfdtd_read_coefficient_box.argtypes = [vp, i32, vp, vp]
+    L.rfdtd_design_prepare.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double]
+    L.rfdtd_set_design_permittivity_device.argtypes = [vp, i32, vp, vp, i32, vp]
     _lib = L
     return L
 

@@ -333,7 +333,8 @@ try:  # torch is plumbing for autograd users; the engine itself does not need it
         """J(eps_r of a design box) with a hand-written adjoint backward, on one resident GPU session.
 
         apply(eps_r, loop): loop is a design.DesignLoop; forward recomputes the box's coefficients on
-        the GPU and runs the forward solve, and the reverse pass runs only when backward is called.
+        the GPU (from eps_r's own device memory when it is a GPU tensor) and runs the forward solve,
+        and the reverse pass runs only when backward is called.
         apply(eps_r, build, cbox, frequency[, checkpoint_every]): build(eps_r_numpy) returns a meshed,
         excited model whose cells in cbox have that permittivity; a loop is made for that model and
         released after backward.  J is sum_f |S_21|^2 (ports b=2, a=1)."""
@@ -342,12 +343,13 @@ try:  # torch is plumbing for autograd users; the engine itself does not need it
         def forward(ctx, eps_r, loop, cbox=None, frequency=None, checkpoint_every=200):
             from .design import DesignLoop
 
-            eps = eps_r.detach().to("cpu", torch.float64).numpy()
             owned = not isinstance(loop, DesignLoop)
             if owned:
+                eps = eps_r.detach().to("cpu", torch.float64).numpy()
                 model = loop(eps)
                 loop = DesignLoop(model, cbox, frequency, dtype=model.coef_dtype, checkpoint_every=checkpoint_every)
-            J, _ = loop.forward(eps)
+            # a tensor in device memory goes to the GPU's coefficient update as it is (no host copy)
+            J, _ = loop.forward(eps_r.detach() if eps_r.is_cuda else eps_r.detach().to(torch.float64).numpy())
             ctx.loop, ctx.owned, ctx.generation = loop, owned, loop.generation
             return eps_r.new_tensor(J)
 

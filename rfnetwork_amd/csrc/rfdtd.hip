@@ -1677,6 +1677,19 @@ __global__ void __launch_bounds__(256) k_design_coefficients(DesignDev D, int co
     }
 }
 
+// The relative permittivity of the cells of a box (C order, float or double, in device memory) into
+// the held cell permittivity of a design box (F/m: eps_r * e0 in double, as the host forms it).
+// (bx, by, bz): the box's first cell relative to the held volume's; (nx, ny, nz): its extents.
+__global__ void __launch_bounds__(256) k_design_put_eps(double* __restrict__ eps, int ey1, int ez1, int bx, int by,
+                                                        int bz, int nx, int ny, int nz, const void* __restrict__ eps_r,
+                                                        int f64, double e0) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny * nz) return;
+    const int k = (int)(q % nz), j = (int)(q / nz % ny), i = (int)(q / ((int64_t)nz * ny));
+    const double v = f64 ? ((const double*)eps_r)[q] : (double)((const float*)eps_r)[q];
+    eps[((int64_t)(bx + i) * ey1 + (by + j)) * ez1 + (bz + k)] = v * e0;
+}
+
 // ------------------------------------------------------------------------------------------------
 // whole-grid coefficient assembly
 //
@@ -2785,6 +2798,17 @@ struct Member {
     void* d_etab_design = nullptr;
     int64_t etab_cap = 0, design_used = 0, design_base = -1;
     int32_t design_box[6] = {};
+    // a design box prepared for updates from device memory (rfdtd_design_prepare): its cell
+    // permittivity (updated in place), conductivity, widths, inverses and coefficient pointers, held
+    // in one device allocation
+    struct DesignHeld {
+        DesignDev D{};
+        double* eps = nullptr;
+        void* const* raw = nullptr;
+        void* buf = nullptr;
+        int64_t base = 0;
+        bool ready = false;
+    } held;
 };
 
 struct MonitorHost {
@@ -2964,6 +2988,7 @@ Session::~Session() {
     for (auto& M : members) {
         if (M.d_coef) (void)hipFree(M.d_coef);
         if (M.d_etab_design) (void)hipFree(M.d_etab_design);
+        if (M.held.buf) (void)hipFree(M.held.buf);
     }
     for (auto& h : hist)
         if (h.d) (void)hipFree(h.d);
@@ -4900,6 +4925,141 @@ char* scratch_address() {
 // C ABI
 // ------------------------------------------------------------------------------------------------
 
+namespace {
+
+using DesignHeld = Member::DesignHeld;
+
+Member& design_member(Session& S, int member) {
+    if (member < 0 || member >= (int)S.members.size()) RF_THROW(RFDTD_EINVAL, "member out of range");
+    return S.members[member];
+}
+
+// Validates a design box of member M and the permittivity it holds, and puts the box's arrays on the
+// device (one allocation, returned in .buf, which the caller owns): the tuples of eps_now must equal
+// the session's coefficients, else RFDTD_EINVAL (a box touching PML, conductor, lumped-element or
+// edge-corrected edges, or a permittivity that is not the session's).  In table form the box's cells
+// are given entries of their own: the ones they already have if this box was set before, else new
+// ones after every entry in use (cells of earlier boxes keep theirs).
+DesignHeld design_hold(Session& S, Member& M, const int32_t* gbox, const double* eps_now, const double* sigma,
+                       const double* dx, const double* dy, const double* dz, const void* dx1, const void* dy1,
+                       const void* dz1, double dt, double e0) {
+    if (!gbox || !eps_now || !sigma || !dx || !dy || !dz || !dx1 || !dy1 || !dz1)
+        RF_THROW(RFDTD_EINVAL, "design permittivity arrays missing");
+    const int gx0 = gbox[0], gx1 = gbox[1], gy0 = gbox[2], gy1 = gbox[3], gz0 = gbox[4], gz1 = gbox[5];
+    if (gx0 < 0 || gy0 < 0 || gz0 < 0 || gx1 <= gx0 || gy1 <= gy0 || gz1 <= gz0 || gx1 > M.Nx || gy1 > M.Ny ||
+        gz1 > M.Nz)
+        RF_THROW(RFDTD_EINVAL, "design box outside the grid");
+    const int64_t n = (int64_t)(gx1 - gx0) * (gy1 - gy0) * (gz1 - gz0);
+    if (n > (int64_t)UINT32_MAX - M.nE - M.design_used) RF_THROW(RFDTD_EINVAL, "design box too large");
+    DesignHeld H;
+    DesignDev& D = H.D;
+    D.gx0 = gx0; D.gy0 = gy0; D.gz0 = gz0;
+    D.gnx = gx1 - gx0; D.gny = gy1 - gy0; D.gnz = gz1 - gz0;
+    D.ex1 = std::min(gx1 + 1, M.Nx) - gx0;
+    D.ey1 = std::min(gy1 + 1, M.Ny) - gy0;
+    D.ez1 = std::min(gz1 + 1, M.Nz) - gz0;
+    D.Nx = M.Nx; D.Ny = M.Ny; D.Nz = M.Nz;
+    D.dt = dt;
+    D.e0 = e0;
+    const size_t esz = S.esz;
+    const size_t vol = (size_t)D.ex1 * D.ey1 * D.ez1 * sizeof(double);
+    const size_t parts[9] = {vol, vol, M.Nx * sizeof(double), M.Ny * sizeof(double), M.Nz * sizeof(double),
+                             M.Nx * esz, M.Ny * esz, M.Nz * esz, sizeof(M.raw)};
+    const void* src[9] = {eps_now, sigma, dx, dy, dz, dx1, dy1, dz1, M.raw};
+    size_t total = 0;
+    for (size_t b : parts) total += align256(b);
+    HIP_OK(hipMalloc(&H.buf, total));
+    char* d[9];
+    char* q = (char*)H.buf;
+    for (int k = 0; k < 9; ++k) {
+        d[k] = q;
+        if (hipMemcpy(q, src[k], parts[k], hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(H.buf);
+            RF_THROW(RFDTD_EDEVICE, "design box upload failed");
+        }
+        q += align256(parts[k]);
+    }
+    H.eps = (double*)d[0];
+    D.eps = H.eps;
+    D.sig = (const double*)d[1];
+    D.dx = (const double*)d[2];
+    D.dy = (const double*)d[3];
+    D.dz = (const double*)d[4];
+    D.dx1 = d[5];
+    D.dy1 = d[6];
+    D.dz1 = d[7];
+    H.raw = (void* const*)d[8];
+    // the tuples of the current permittivity must be what the session holds
+    const int blocks = (int)((n + 255) / 256);
+    int* d_bad = nullptr;
+    HIP_OK(hipMalloc(&d_bad, (size_t)blocks * sizeof(int)));
+    if (S.f64)
+        k_design_coefficients<double><<<blocks, 256>>>(D, M.coef_kind, (uint32_t*)M.eidx, (double*)M.etab,
+                                                       (double* const*)H.raw, 0, 1, d_bad);
+    else
+        k_design_coefficients<float><<<blocks, 256>>>(D, M.coef_kind, (uint32_t*)M.eidx, (float*)M.etab,
+                                                      (float* const*)H.raw, 0, 1, d_bad);
+    std::vector<int> bad(blocks);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipMemcpy(bad.data(), d_bad, blocks * sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d_bad);
+    bool any = false;
+    for (int b : bad) any |= b != 0;
+    if (e1 != hipSuccess || e2 != hipSuccess || any) {
+        (void)hipFree(H.buf);
+        if (e1 != hipSuccess || e2 != hipSuccess) RF_THROW(RFDTD_EDEVICE, "design box check failed to run");
+        RF_THROW(RFDTD_EINVAL, "the design box touches edges that are not plain dielectric (PML, conductor, "
+                               "lumped element or edge correction), or the current permittivity given does "
+                               "not match the session's coefficients");
+    }
+    if (M.coef_kind == 1) {
+        const bool same = M.design_base >= 0 && std::equal(gbox, gbox + 6, M.design_box);
+        if (same) {
+            H.base = M.design_base;  // the box's cells already own entries [base, base + n)
+        } else {
+            const int64_t used = M.nE + M.design_used;
+            if (M.etab_cap < M.nE) M.etab_cap = M.nE;
+            if (used + n > M.etab_cap) {
+                const int64_t cap = used + n;
+                void* t = nullptr;
+                if (hipMalloc(&t, (size_t)cap * 12 * esz) != hipSuccess ||
+                    hipMemcpy(t, M.etab, (size_t)used * 12 * esz, hipMemcpyDeviceToDevice) != hipSuccess) {
+                    if (t) (void)hipFree(t);
+                    (void)hipFree(H.buf);
+                    RF_THROW(RFDTD_ENOMEM, "design box entries do not fit in device memory");
+                }
+                if (M.d_etab_design) (void)hipFree(M.d_etab_design);
+                M.d_etab_design = t;
+                M.etab = t;
+                M.etab_cap = cap;
+                S.dirty = true;
+                S.graph_dirty = true;
+            }
+            H.base = used;
+            M.design_used += n;
+            M.design_base = H.base;
+            std::copy(gbox, gbox + 6, M.design_box);
+        }
+    }
+    return H;
+}
+
+// The box's 12 E coefficients rewritten from its held permittivity, on stream st.
+void design_write(Session& S, Member& M, const DesignHeld& H, hipStream_t st) {
+    const DesignDev& D = H.D;
+    const int64_t n = (int64_t)D.gnx * D.gny * D.gnz;
+    const int blocks = (int)((n + 255) / 256);
+    if (S.f64)
+        k_design_coefficients<double><<<blocks, 256, 0, st>>>(D, M.coef_kind, (uint32_t*)M.eidx, (double*)M.etab,
+                                                              (double* const*)H.raw, H.base, 0, nullptr);
+    else
+        k_design_coefficients<float><<<blocks, 256, 0, st>>>(D, M.coef_kind, (uint32_t*)M.eidx, (float*)M.etab,
+                                                             (float* const*)H.raw, H.base, 0, nullptr);
+    HIP_OK(hipGetLastError());
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* rfdtd_last_error(void) { return g_last_error.c_str(); }
@@ -5200,94 +5360,65 @@ int rfdtd_set_design_permittivity(rfdtd_session* s, int32_t member, const int32_
                                   double e0) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (member < 0 || member >= (int)S.members.size()) RF_THROW(RFDTD_EINVAL, "member out of range");
-        if (!gbox || !eps_now || !eps_new || !sigma || !dx || !dy || !dz || !dx1 || !dy1 || !dz1)
-            RF_THROW(RFDTD_EINVAL, "design permittivity arrays missing");
-        Member& M = S.members[member];
-        const int gx0 = gbox[0], gx1 = gbox[1], gy0 = gbox[2], gy1 = gbox[3], gz0 = gbox[4], gz1 = gbox[5];
-        if (gx0 < 0 || gy0 < 0 || gz0 < 0 || gx1 <= gx0 || gy1 <= gy0 || gz1 <= gz0 || gx1 > M.Nx || gy1 > M.Ny ||
-            gz1 > M.Nz)
-            RF_THROW(RFDTD_EINVAL, "design box outside the grid");
-        const int64_t n = (int64_t)(gx1 - gx0) * (gy1 - gy0) * (gz1 - gz0);
-        if (n > (int64_t)UINT32_MAX - M.nE - M.design_used) RF_THROW(RFDTD_EINVAL, "design box too large");
-        DesignDev D{};
-        D.gx0 = gx0; D.gy0 = gy0; D.gz0 = gz0;
-        D.gnx = gx1 - gx0; D.gny = gy1 - gy0; D.gnz = gz1 - gz0;
-        D.ex1 = std::min(gx1 + 1, M.Nx) - gx0;
-        D.ey1 = std::min(gy1 + 1, M.Ny) - gy0;
-        D.ez1 = std::min(gz1 + 1, M.Nz) - gz0;
-        D.Nx = M.Nx; D.Ny = M.Ny; D.Nz = M.Nz;
-        D.dt = dt;
-        D.e0 = e0;
-        const size_t esz = S.esz;
-        const size_t vol = (size_t)D.ex1 * D.ey1 * D.ez1 * sizeof(double);
+        if (!eps_new) RF_THROW(RFDTD_EINVAL, "design permittivity arrays missing");
+        Member& M = design_member(S, member);
         HIP_OK(hipDeviceSynchronize());
-        GradBuffers buf;  // scratch freed on return
-        auto upload = [&](const void* src, size_t bytes) {
-            void* d = buf.alloc(bytes);
-            HIP_OK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
-            return d;
-        };
-        const double* d_now = (const double*)upload(eps_now, vol);
-        const double* d_new = (const double*)upload(eps_new, vol);
-        D.sig = (const double*)upload(sigma, vol);
-        D.dx = (const double*)upload(dx, M.Nx * sizeof(double));
-        D.dy = (const double*)upload(dy, M.Ny * sizeof(double));
-        D.dz = (const double*)upload(dz, M.Nz * sizeof(double));
-        D.dx1 = upload(dx1, M.Nx * esz);
-        D.dy1 = upload(dy1, M.Ny * esz);
-        D.dz1 = upload(dz1, M.Nz * esz);
-        const int blocks = (int)((n + 255) / 256);
-        int* d_bad = (int*)buf.alloc((size_t)blocks * sizeof(int));
-        void** d_raw = (void**)upload(M.raw, sizeof(M.raw));
-        auto launch = [&](const double* eps, int check, int64_t base) {
-            D.eps = eps;
-            if (S.f64)
-                k_design_coefficients<double><<<blocks, 256>>>(D, M.coef_kind, (uint32_t*)M.eidx, (double*)M.etab,
-                                                               (double* const*)d_raw, base, check, d_bad);
-            else
-                k_design_coefficients<float><<<blocks, 256>>>(D, M.coef_kind, (uint32_t*)M.eidx, (float*)M.etab,
-                                                              (float* const*)d_raw, base, check, d_bad);
-            HIP_OK(hipGetLastError());
-        };
-        // the tuples of the current permittivity must be what the session holds
-        launch(d_now, 1, 0);
-        std::vector<int> bad(blocks);
-        HIP_OK(hipMemcpy(bad.data(), d_bad, blocks * sizeof(int), hipMemcpyDeviceToHost));
-        for (int b : bad)
-            if (b)
-                RF_THROW(RFDTD_EINVAL, "the design box touches edges that are not plain dielectric (PML, conductor, "
-                                       "lumped element or edge correction), or the current permittivity given does "
-                                       "not match the session's coefficients");
-        int64_t base = 0;
-        if (M.coef_kind == 1) {
-            const bool same = M.design_base >= 0 && std::equal(gbox, gbox + 6, M.design_box);
-            if (same) {
-                base = M.design_base;  // the box's cells already own entries [base, base + n)
-            } else {
-                // new entries after every entry in use (cells of earlier boxes keep theirs)
-                const int64_t used = M.nE + M.design_used;
-                if (M.etab_cap < M.nE) M.etab_cap = M.nE;
-                if (used + n > M.etab_cap) {
-                    const int64_t cap = used + n;
-                    void* t = nullptr;
-                    HIP_OK(hipMalloc(&t, (size_t)cap * 12 * esz));
-                    HIP_OK(hipMemcpy(t, M.etab, (size_t)used * 12 * esz, hipMemcpyDeviceToDevice));
-                    if (M.d_etab_design) (void)hipFree(M.d_etab_design);
-                    M.d_etab_design = t;
-                    M.etab = t;
-                    M.etab_cap = cap;
-                    S.dirty = true;
-                    S.graph_dirty = true;
-                }
-                base = used;
-                M.design_used += n;
-                M.design_base = base;
-                std::copy(gbox, gbox + 6, M.design_box);
-            }
-        }
-        launch(d_new, 0, base);
+        DesignHeld H = design_hold(S, M, gbox, eps_now, sigma, dx, dy, dz, dx1, dy1, dz1, dt, e0);
+        // the new permittivity replaces the held one; the box's coefficients are rewritten from it
+        const DesignDev& D = H.D;
+        HIP_OK(hipMemcpy(H.eps, eps_new, (size_t)D.ex1 * D.ey1 * D.ez1 * sizeof(double), hipMemcpyHostToDevice));
+        design_write(S, M, H, nullptr);
         HIP_OK(hipDeviceSynchronize());
+        // a box prepared for device updates now holds other values: it must be prepared again
+        if (M.held.buf) (void)hipFree(M.held.buf);
+        M.held = Member::DesignHeld{};
+        (void)hipFree(H.buf);
+    });
+}
+
+int rfdtd_design_prepare(rfdtd_session* s, int32_t member, const int32_t gbox[6], const double* eps_now,
+                         const double* sigma, const double* dx, const double* dy, const double* dz, const void* dx1,
+                         const void* dy1, const void* dz1, double dt, double e0) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        Member& M = design_member(S, member);
+        HIP_OK(hipDeviceSynchronize());
+        DesignHeld H = design_hold(S, M, gbox, eps_now, sigma, dx, dy, dz, dx1, dy1, dz1, dt, e0);
+        if (M.held.buf) (void)hipFree(M.held.buf);
+        M.held = H;
+        M.held.ready = true;
+    });
+}
+
+int rfdtd_set_design_permittivity_device(rfdtd_session* s, int32_t member, const int32_t cbox[6], const void* eps_r,
+                                         int32_t f64, void* stream) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        Member& M = design_member(S, member);
+        if (!M.held.ready) RF_THROW(RFDTD_EINVAL, "no design box prepared on this member (rfdtd_design_prepare)");
+        if (!cbox || !eps_r) RF_THROW(RFDTD_EINVAL, "design box or permittivity missing");
+        const DesignDev& D = M.held.D;
+        // the cells must lie in the held volume: the packed-cell box grown by one cell on each upper side
+        const int bx = cbox[0] - D.gx0, by = cbox[2] - D.gy0, bz = cbox[4] - D.gz0;
+        const int nx = cbox[1] - cbox[0], ny = cbox[3] - cbox[2], nz = cbox[5] - cbox[4];
+        if (nx < 1 || ny < 1 || nz < 1 || bx < 0 || by < 0 || bz < 0 || bx + nx > D.ex1 || by + ny > D.ey1 ||
+            bz + nz > D.ez1)
+            RF_THROW(RFDTD_EINVAL, "cells outside the prepared design box");
+        hipStream_t st = stream ? (hipStream_t)stream : S.own_stream;
+        auto order = [](hipStream_t before, hipStream_t after) {  // `after` waits for `before`'s work so far
+            hipEvent_t ev;
+            HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            HIP_OK(hipEventRecord(ev, before));
+            HIP_OK(hipStreamWaitEvent(after, ev, 0));
+            (void)hipEventDestroy(ev);
+        };
+        if (st != S.own_stream) order(S.own_stream, st);  // the session's earlier steps read the old values
+        const int64_t n = (int64_t)nx * ny * nz;
+        k_design_put_eps<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(M.held.eps, D.ey1, D.ez1, bx, by, bz, nx, ny, nz,
+                                                                     eps_r, f64 ? 1 : 0, D.e0);
+        HIP_OK(hipGetLastError());
+        design_write(S, M, M.held, st);
+        if (st != S.own_stream) order(st, S.own_stream);  // and the session's later steps the new ones
     });
 }
 

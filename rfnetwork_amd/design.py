@@ -78,20 +78,23 @@ def design_permittivity_args(model, cbox, eps_now, sigma_now, eps_r, dtype):
     """Arguments of rfdtd_set_design_permittivity for replacing the relative permittivity of the
     cells of cbox = (x0, x1, y0, y1, z0, z1) by eps_r: the packed-cell box, the current and new
     permittivity (F/m) and the conductivity over that box grown by one cell on each upper side,
-    the cell widths and the packing inverses.  eps_now, sigma_now: full-grid float64 arrays."""
+    the cell widths and the packing inverses.  eps_now, sigma_now: full-grid float64 arrays.
+    eps_r None: no new permittivity (new is None; rfdtd_design_prepare's arguments)."""
     from .adjoint import gpu_box_for_cells
 
     x0, x1, y0, y1, z0, z1 = (int(v) for v in cbox)
-    eps_r = np.asarray(eps_r, dtype=np.float64)
-    if eps_r.shape != (x1 - x0, y1 - y0, z1 - z0):
-        raise ValueError("eps_r must have the shape of the design box")
     Nx, Ny, Nz = model.n_cells
     gbox = gpu_box_for_cells(model, cbox)
     gx0, gx1, gy0, gy1, gz0, gz1 = gbox
     ext = (slice(gx0, min(gx1 + 1, Nx)), slice(gy0, min(gy1 + 1, Ny)), slice(gz0, min(gz1 + 1, Nz)))
     now = np.ascontiguousarray(eps_now[ext], dtype=np.float64)
-    new = now.copy()
-    new[x0 - gx0:x1 - gx0, y0 - gy0:y1 - gy0, z0 - gz0:z1 - gz0] = eps_r * const.e0
+    new = None
+    if eps_r is not None:
+        eps_r = np.asarray(eps_r, dtype=np.float64)
+        if eps_r.shape != (x1 - x0, y1 - y0, z1 - z0):
+            raise ValueError("eps_r must have the shape of the design box")
+        new = now.copy()
+        new[x0 - gx0:x1 - gx0, y0 - gy0:y1 - gy0, z0 - gz0:z1 - gz0] = eps_r * const.e0
     sig = np.ascontiguousarray(sigma_now[ext], dtype=np.float64)
     widths = [np.ascontiguousarray(conv.m_in(d), dtype=np.float64) for d in model.d_cells]
     inverses = [np.ascontiguousarray(a) for a in model.packing_inverses(dtype)]
@@ -116,6 +119,54 @@ def set_design_permittivity(handle, L, member, model, cbox, eps_r, state, dtype)
                                           ctypes.c_double(dt), ctypes.c_double(e0)), "rfdtd_set_design_permittivity")
     x0, x1, y0, y1, z0, z1 = (int(v) for v in cbox)
     eps_now[x0:x1, y0:y1, z0:z1] = np.asarray(eps_r, dtype=np.float64) * e0
+
+
+def prepare_design_box(handle, L, member, model, cbox, state, dtype):
+    """Keep the arrays of cbox's box of member `member` on the device for updates from device memory
+    (rfdtd_design_prepare), checked against the session's coefficients; state as for
+    set_design_permittivity (the permittivity the session holds now)."""
+    if np.dtype(model.coef_dtype) != np.dtype(dtype):
+        raise ValueError("the model's coefficient dtype must be the session's element type")
+    if member not in state:
+        state[member] = (np.array(model.eps, dtype=np.float64), np.array(model.sigma, dtype=np.float64))
+    eps_now, sigma_now = state[member]
+    gbox, now, _, sig, (dx, dy, dz), (ix, iy, iz), dt, e0 = design_permittivity_args(
+        model, cbox, eps_now, sigma_now, None, dtype)
+    b = (ctypes.c_int32 * 6)(*gbox)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    check(L.rfdtd_design_prepare(handle, int(member), ctypes.cast(b, ctypes.c_void_p), ptr(now), ptr(sig), ptr(dx),
+                                 ptr(dy), ptr(dz), ptr(ix), ptr(iy), ptr(iz), dt, e0), "rfdtd_design_prepare")
+
+
+def is_device_tensor(a):
+    """True for a torch tensor in GPU memory."""
+    return type(a).__module__.startswith("torch") and getattr(a, "is_cuda", False)
+
+
+def set_design_permittivity_device(handle, L, member, cbox, eps_r):
+    """Relative permittivity of the cells of cbox from a torch tensor in device memory (float32 or
+    float64, the box's shape; other types are converted to float64), in a box prepared with
+    prepare_design_box: no host copy.  torch's current stream (which produced eps_r) is waited for,
+    then the update runs on the session's stream, before its later steps.  Returns the tensor the
+    GPU reads: the caller keeps it alive until the session has synchronised.
+
+    The library may run on a HIP runtime other than torch's (torch wheels bundle their own), so no
+    torch stream handle is passed to it; device addresses are valid in either."""
+    import torch
+
+    x0, x1, y0, y1, z0, z1 = (int(v) for v in cbox)
+    t = eps_r.detach()
+    if t.dtype not in (torch.float32, torch.float64):
+        t = t.to(torch.float64)
+    t = t.contiguous()
+    if tuple(t.shape) != (x1 - x0, y1 - y0, z1 - z0):
+        raise ValueError("eps_r must have the shape of the design box")
+    b = (ctypes.c_int32 * 6)(x0, x1, y0, y1, z0, z1)
+    torch.cuda.current_stream(t.device).synchronize()
+    check(L.rfdtd_set_design_permittivity_device(handle, int(member), ctypes.cast(b, ctypes.c_void_p),
+                                                 ctypes.c_void_p(t.data_ptr()), int(t.dtype == torch.float64),
+                                                 None), "rfdtd_set_design_permittivity_device")
+    return t
 
 
 class DesignLoop:
@@ -152,11 +203,23 @@ class DesignLoop:
         self.timing = {}
         self.generation = 0
         self._pending = None  # (generation, dJ/dy of that forward) until its backward runs
+        self._prepared = False  # the box held on the device for updates from device memory
+        self._eps_device = None  # permittivity set from device memory, not yet in self.eps
+
+    def _host_eps(self):
+        """Bring self.eps up to date with a permittivity last set from device memory (one copy of the
+        box's values, made only when the host needs them)."""
+        if self._eps_device is not None:
+            x0, x1, y0, y1, z0, z1 = self.cbox
+            vals = self._eps_device.detach().to("cpu").numpy().astype(np.float64)
+            self.eps[x0:x1, y0:y1, z0:z1] = vals * const.e0
+            self._eps_device = None
 
     def forward(self, eps_r):
         """(J, S_ba) for relative permittivity eps_r of the box cells: the box's coefficients
         recomputed on the GPU, then the forward solve (recording what a reverse pass needs).  The
-        reverse pass runs only if backward() is called before the next forward()."""
+        reverse pass runs only if backward() is called before the next forward().  eps_r: a numpy
+        array, or a torch tensor in device memory, which the GPU reads directly (no host copy)."""
         import time
 
         from .adjoint import sparam_objective
@@ -164,8 +227,20 @@ class DesignLoop:
         s, model = self.session, self.model
         t0 = time.perf_counter()
         s.reset()
-        state = {0: (self.eps, self.sigma)}
-        set_design_permittivity(s._h, s._L, 0, model, self.cbox, eps_r, state, self.dtype)
+        if is_device_tensor(eps_r):
+            if not self._prepared:
+                self._host_eps()
+                prepare_design_box(s._h, s._L, 0, model, self.cbox, {0: (self.eps, self.sigma)}, self.dtype)
+                self._prepared = True
+            # t stays referenced until the solve below has synchronised; the device copy of its values
+            # serves backward() (the caller may change eps_r before then)
+            t = set_design_permittivity_device(s._h, s._L, 0, self.cbox, eps_r)
+            self._eps_device = t.clone()
+        else:
+            self._host_eps()
+            state = {0: (self.eps, self.sigma)}
+            set_design_permittivity(s._h, s._L, 0, model, self.cbox, eps_r, state, self.dtype)
+            self._prepared = False  # a host update discards the device-held box
         t1 = time.perf_counter()
         if self.history:
             s.set_history(self.gbox, self.Nt)
@@ -200,6 +275,7 @@ class DesignLoop:
         s, model = self.session, self.model
         grad_cb = s.gradient(self.gbox, [row.astype(self.dtype) for row in g])
         self.timing["adjoint_ms"] = s.timing()[0]
+        self._host_eps()  # the chain rule's edge values need the design permittivity on the host
         edges = edge_values(model, self.gbox, self.eps, self.sigma, self.dtype)
         grad = cb_to_eps_r_gradient(model, None, grad_cb, self.gbox, self.cbox, edges=edges)
         self.timing["wall_ms"] += (time.perf_counter() - t0) * 1e3

@@ -172,12 +172,32 @@ class Session:
         session was built from: the E coefficients around the box are recomputed on the GPU (bit for
         bit what the model's own assembly gives for that permittivity).  The box must hold plain
         dielectric edges only (no PML, conductor, lumped element or edge correction); later calls
-        start from the permittivity set last."""
-        from .design import set_design_permittivity
+        start from the permittivity set last.  eps_r: a numpy array, or a torch tensor in device
+        memory, which the GPU reads directly (the box is prepared on the device once; while device
+        updates follow one another the host's copy of the permittivity is not kept)."""
+        from . import design
 
         if not hasattr(self, "_design_state"):
             self._design_state = {}
-        set_design_permittivity(self._h, self._L, 0, model, cbox, eps_r, self._design_state, self.dtype)
+            self._design_prepared = None  # the box held on the device
+            self._design_stale = False  # the host's record of the permittivity is behind the device's
+        key = tuple(int(v) for v in cbox)
+        if design.is_device_tensor(eps_r):
+            if self._design_prepared != key:
+                if self._design_stale:
+                    raise ValueError("another box after device updates: use a DesignLoop, which keeps track of "
+                                     "the permittivity, to mix boxes and host and device updates")
+                design.prepare_design_box(self._h, self._L, 0, model, key, self._design_state, self.dtype)
+                self._design_prepared = key
+            # the tensor read stays referenced until the next update (the session's work is ordered)
+            self._design_keep = design.set_design_permittivity_device(self._h, self._L, 0, key, eps_r)
+            self._design_stale = True
+            return
+        if self._design_stale:
+            raise ValueError("the session's permittivity was last set from device memory: use a DesignLoop, "
+                             "which keeps track of it, to mix host and device updates")
+        design.set_design_permittivity(self._h, self._L, 0, model, cbox, eps_r, self._design_state, self.dtype)
+        self._design_prepared = None
 
     def set_history(self, box, Nt):
         """Record the H totals of box at every step of later runs (up to Nt steps): a gradient over

@@ -186,3 +186,101 @@ def test_design_update_on_a_graded_grid(hip_ready):
         s.set_design_permittivity(m0, (8, 14, 6, 11, 5, 9), e1)
         got = _run(s, m0, Nt)
     _assert_same(got, _reference(m1, np.float32, 0, Nt))
+
+
+def _device(a, dtype):
+    import torch
+
+    return torch.tensor(np.asarray(a), dtype=dtype, device="cuda")
+
+
+@pytest.mark.parametrize("dtype,flags,tdt", [(np.float32, 0, "float32"), (np.float32, _lib.COEF_RAW, "float64"),
+                                             (np.float64, 0, "float64")])
+def test_design_update_from_device_memory(hip_ready, dtype, flags, tdt):
+    """rfdtd_design_prepare + rfdtd_set_design_permittivity_device, the permittivity read from a torch
+    tensor in device memory (twice: the second update rewrites the box in place), against the host
+    update with the same values: identical probes and fields."""
+    import torch
+
+    print(f"[case] design update from device memory dtype={np.dtype(dtype).name} flags={flags} tensor={tdt}",
+          flush=True)
+    Nt = 120
+    tdtype = getattr(torch, tdt)
+    # values a float32 tensor holds exactly, so that the host update (float64) sees the same numbers
+    e2, e3 = (_eps(s).astype(np.float32).astype(np.float64) for s in (2, 3))
+    m0 = _model(_eps(1), coef_dtype=dtype, Nt=Nt)
+    cbox = workloads.design_box(48, 32, DESIGN)
+    coeff, _, _, _, Nx, Ny, Nz, _ = m0.pack(gpu=True, dtype=dtype)
+    with Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags) as s:
+        s.set_design_permittivity(m0, cbox, e2)
+        s.set_design_permittivity(m0, cbox, e3)
+        want = _run(s, m0, Nt)
+    with Session(coeff, Nx, Ny, Nz, dtype=dtype, flags=flags) as s:
+        s.set_design_permittivity(m0, cbox, _device(e2, tdtype))
+        t3 = _device(e3, tdtype)
+        s.set_design_permittivity(m0, cbox, t3)
+        got = _run(s, m0, Nt)
+    _assert_same(got, want)
+
+
+def test_design_update_from_device_memory_refusals(hip_ready):
+    """A device update needs a prepared box and cells inside it; a box on a conductor edge is refused
+    when it is prepared."""
+    import ctypes
+
+    print("[case] design device refusals", flush=True)
+    from rfnetwork_amd import design
+
+    m0 = _model(_eps(1), Nt=60)
+    cbox = workloads.design_box(48, 32, DESIGN)
+    x0, x1, y0, y1, z0, z1 = cbox
+    coeff, _, _, _, Nx, Ny, Nz, _ = m0.pack(gpu=True)
+    with Session(coeff, Nx, Ny, Nz) as s:
+        import torch
+
+        t = torch.ones(DESIGN, dtype=torch.float64, device="cuda") * 2.0
+        with pytest.raises(ValueError, match="no design box prepared"):
+            design.set_design_permittivity_device(s._h, s._L, 0, cbox, t)
+        with pytest.raises(ValueError, match="plain dielectric"):
+            design.prepare_design_box(s._h, s._L, 0, m0, (x0 - 2, x0, y0, y1, z0, z1), {}, np.float32)
+        design.prepare_design_box(s._h, s._L, 0, m0, cbox, {}, np.float32)
+        far = (x0 + 3, x1 + 3, y0, y1, z0, z1)
+        b = (ctypes.c_int32 * 6)(*far)
+        rc = s._L.rfdtd_set_design_permittivity_device(s._h, 0, ctypes.cast(b, ctypes.c_void_p),
+                                                       ctypes.c_void_p(t.data_ptr()), 1, None)
+        assert rc == 1 and b"outside the prepared" in s._L.rfdtd_last_error()
+
+
+def test_design_loop_device_tensors(hip_ready):
+    """A DesignLoop fed torch tensors in device memory (float32 and float64) gives J, S and dJ/deps_r
+    identical to the same loop fed numpy arrays; the autograd Function on it hands torch that
+    gradient on the tensor's device."""
+    import torch
+
+    from rfnetwork_amd.adjoint import EpsRObjective
+    from rfnetwork_amd.design import DesignLoop
+
+    print("[case] design loop from device tensors", flush=True)
+    Nt = 300
+    freq = (16e9, 20e9, 24e9)
+    cbox = workloads.design_box(48, 32, DESIGN)
+    vals = [_eps(s).astype(np.float32).astype(np.float64) for s in (2, 3, 4)]
+    m0 = _model(_eps(1), Nt=Nt, frequency=freq)
+    with DesignLoop(m0, cbox, freq, mode="history") as host_loop:
+        want = [host_loop.evaluate(v) for v in vals]
+    m0 = _model(_eps(1), Nt=Nt, frequency=freq)
+    with DesignLoop(m0, cbox, freq, mode="history") as loop:
+        for v, tdt, (Jw, gw, Sw) in zip(vals, (torch.float32, torch.float64, torch.float32), want):
+            J, g, S = loop.evaluate(_device(v, tdt))
+            assert J == Jw
+            np.testing.assert_array_equal(g, gw)
+            np.testing.assert_array_equal(S, Sw)
+        # a host array after device updates, then the autograd Function with a device tensor
+        J, g, _ = loop.evaluate(vals[0])
+        assert J == want[0][0]
+        np.testing.assert_array_equal(g, want[0][1])
+        t = _device(vals[1], torch.float64).requires_grad_(True)
+        Jt = EpsRObjective.apply(t, loop)
+        Jt.backward()
+        assert float(Jt) == want[1][0] and t.grad.is_cuda
+        np.testing.assert_array_equal(t.grad.cpu().numpy(), want[1][1])
