@@ -59,13 +59,16 @@ MEASURED_TRAFFIC = {
     }.items()
     for k, v in (("k_update_e", e), ("k_update_h", h))
 }
-# The raw-coefficient form (the reference formulation's bytes) at the shape the tuner picks for it.
+# The raw-coefficient form (the reference formulation's bytes) at the shapes the tuner picks for it
+# (variant 10: nontemporal loads of the streamed arrays; variant 4 before it existed).
 MEASURED_TRAFFIC.update({
     ("dipole_slab", "raw", "k_update_e", (4, 128, 4)): 3976700000,
     ("dipole_slab", "raw", "k_update_h", (4, 128, 4)): 4683300000,
+    ("dipole_slab", "raw", "k_update_e", (10, 128, 4)): 3873700000,
+    ("dipole_slab", "raw", "k_update_h", (10, 128, 4)): 4637100000,
     # the two-step pass at its tuned shape (12-row tiles, x chunks of 25 planes; 0 = the same automatic)
-    ("dipole_slab", "table", "k_step2", (7, 256, 25)): 4042600000,
-    ("dipole_slab", "table", "k_step2", (7, 256, 0)): 4042600000,
+    ("dipole_slab", "table", "k_step2", (7, 256, 25)): 4175900000,
+    ("dipole_slab", "table", "k_step2", (7, 256, 0)): 4175900000,
 })
 
 
