@@ -66,6 +66,10 @@ MEASURED_TRAFFIC.update({
     ("dipole_slab", "raw", "k_update_h", (4, 128, 4)): 4683300000,
     ("dipole_slab", "raw", "k_update_e", (10, 128, 4)): 3873700000,
     ("dipole_slab", "raw", "k_update_h", (10, 128, 4)): 4637100000,
+    # variant 12: four z cells per thread, one x plane per workgroup (measured at 512 threads; the
+    # block size moves the counters by under 0.1%, as the variant 4 rows above show)
+    **{("dipole_slab", "raw", k, (12, blk, 1)): v for blk in (128, 256, 512)
+       for k, v in (("k_update_e_w", 3925300000), ("k_update_h_w", 4648700000))},
     # the two-step pass at its tuned shape (12-row tiles, x chunks of 25 planes; 0 = the same automatic)
     ("dipole_slab", "table", "k_step2", (7, 256, 25)): 4175900000,
     ("dipole_slab", "table", "k_step2", (7, 256, 0)): 4175900000,
@@ -757,6 +761,7 @@ TWO_STEP_RAW_BYTES = 96 + 120
 # E+H pass; 7, 8, 9 the two-step pass
 TWO_STEP_VARIANTS = (7, 8, 9)
 FUSED_VARIANTS = (2, 3, 5, 6) + TWO_STEP_VARIANTS
+VECTOR_VARIANTS = (11, 12)  # separate kernels over 2 or 4 z cells per thread (k_update_e_w, k_update_h_w)
 
 
 def plain_leg(args):
@@ -782,7 +787,8 @@ def plain_leg(args):
     elif fused:
         kname, kbytes, kms = "k_step_fused", (2 * TABLE_BYTES if table else E_BYTES + H_BYTES), e_ms
     else:
-        kname, kbytes, kms = "k_update_h", (TABLE_BYTES if table else H_BYTES), h_ms
+        kname = "k_update_h_w" if variant in VECTOR_VARIANTS else "k_update_h"
+        kbytes, kms = (TABLE_BYTES if table else H_BYTES), h_ms
     shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
     traffic = MEASURED_TRAFFIC.get((args.workload, info["coef_kind"], kname, shape))
     if variant in TWO_STEP_VARIANTS:
@@ -828,7 +834,8 @@ def raw_leg(args):
     if fused:
         kname, kbytes, kms = "k_step_fused", E_BYTES + H_BYTES, e_ms
     else:
-        kname, kbytes, kms = "k_update_h", H_BYTES, h_ms
+        kname = "k_update_h_w" if tuned and tuned.get("variant") in VECTOR_VARIANTS else "k_update_h"
+        kbytes, kms = H_BYTES, h_ms
     shape = (tuned["variant"], tuned["block"], tuned["x_planes"]) if tuned else None
     traffic = MEASURED_TRAFFIC.get((args.workload, "raw", kname, shape))
     roof = kernel_roofline(run, kname, kbytes, kms, traffic,

@@ -297,7 +297,7 @@ struct Tile {
 template <typename T>
 __device__ __forceinline__ const InstDev<T>* locate(const InstDev<T>* __restrict__ insts,
                                                      const MemberDev* __restrict__ mem, int nmem, int total,
-                                                     int* s_member, Tile& t) {
+                                                     int* s_member, Tile& t, int W = 1) {
     const int L = xcd_logical(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         int m = 0;
@@ -316,7 +316,7 @@ __device__ __forceinline__ const InstDev<T>* locate(const InstDev<T>* __restrict
     const InstDev<T>* I = insts + __builtin_amdgcn_readfirstlane(M.inst0 + e);
     const int Ny = I->Ny, Nz = I->Nz, npb = I->npb;
     const int pb = tile % npb, xc = tile / npb;
-    t.p = (int64_t)pb * blockDim.x + threadIdx.x;
+    t.p = ((int64_t)pb * blockDim.x + threadIdx.x) * W;  // W consecutive cells of a row per thread
     t.y = (int)(t.p / Nz);
     t.z = (int)(t.p - (int64_t)t.y * Nz);
     t.x0 = xc * I->xplanes;
@@ -618,6 +618,297 @@ __global__ void __launch_bounds__(1024) k_update_h(const InstDev<T>* __restrict_
         h_column<T, CoefRaw<T>, PF, NT, NL>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
 }
 
+
+// ------------------------------------------------------------------------------------------------
+// separate E and H kernels over W consecutive z cells per thread (variants 11 and 12, W = 2 and 4)
+//
+// The same columns, arithmetic and order as e_column / h_column, with every array streamed once per
+// step read and written as one W-wide vector per thread (global_load_dwordx2 / x4): a wavefront
+// moves W x 256 bytes per load instruction, so each lane keeps W times the bytes in flight for the
+// same number of instructions.  The z neighbours inside a group come from the group's own vector;
+// the one past its end (E: z + W) or before its start (H: z - 1) is one more scalar load, an L1 hit
+// since the neighbouring lane loads that line.  Requires Nz % W == 0, so that a group stays in one
+// row and every vector is aligned; the groups' probe cells share one 32-cell bitmask word.
+// ------------------------------------------------------------------------------------------------
+
+template <typename T, int W>
+struct VecOf {
+    typedef T type __attribute__((ext_vector_type(W)));
+};
+
+// W consecutive values at a byte offset from a wave-uniform base (nontemporal hint when NL)
+template <bool NL, int W, class T>
+__device__ __forceinline__ void ldw(const GLOBAL_AS T* base, uint32_t boff, T (&out)[W]) {
+    using V = typename VecOf<T, W>::type;
+    const GLOBAL_AS V* q = (const GLOBAL_AS V*)((const GLOBAL_AS char*)base + boff);
+    V v;
+    if constexpr (NL)
+        v = __builtin_nontemporal_load(q);
+    else
+        v = *q;
+#pragma unroll
+    for (int k = 0; k < W; ++k) out[k] = v[k];
+}
+template <int W, class T>
+__device__ __forceinline__ void stw(GLOBAL_AS T* base, uint32_t boff, const T (&in)[W]) {
+    using V = typename VecOf<T, W>::type;
+    V v;
+#pragma unroll
+    for (int k = 0; k < W; ++k) v[k] = in[k];
+    __builtin_nontemporal_store(v, (GLOBAL_AS V*)((GLOBAL_AS char*)base + boff));
+}
+
+// The 12 E (18 H) coefficients of the W cells of a group: coefficient k of cell i in c[k][i].
+template <typename T, int W>
+__device__ __forceinline__ void coef_e_w(const CoefRaw<T>& C, uint32_t b, T (&c)[12][W]) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) ldw<true, W>((const GLOBAL_AS T*)C.cp[k], b, c[k]);
+}
+template <typename T, int W>
+__device__ __forceinline__ void coef_e_w(const CoefTable<T>& C, uint32_t b, T (&c)[12][W]) {
+    uint32_t key[W];
+    ldw<true, W>((const GLOBAL_AS uint32_t*)C.eidx, b / (uint32_t)sizeof(T) * 4u, key);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const ECoef<T> e = C.e_at(key[i]);
+        c[0][i] = e.ca_xy; c[1][i] = e.ca_xz; c[2][i] = e.cb_xy; c[3][i] = e.cb_xz;
+        c[4][i] = e.ca_yz; c[5][i] = e.ca_yx; c[6][i] = e.cb_yz; c[7][i] = e.cb_yx;
+        c[8][i] = e.ca_zx; c[9][i] = e.ca_zy; c[10][i] = e.cb_zx; c[11][i] = e.cb_zy;
+    }
+}
+template <typename T, int W>
+__device__ __forceinline__ void coef_h_w(const CoefRaw<T>& C, uint32_t b, T (&c)[18][W]) {
+#pragma unroll
+    for (int k = 0; k < 18; ++k) ldw<true, W>((const GLOBAL_AS T*)C.cp[12 + k], b, c[k]);
+}
+template <typename T, int W>
+__device__ __forceinline__ void coef_h_w(const CoefTable<T>& C, uint32_t b, T (&c)[18][W]) {
+    uint32_t key[W];
+    ldw<true, W>((const GLOBAL_AS uint32_t*)C.hidx, b / (uint32_t)sizeof(T) * 4u, key);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const HCoef<T> h = C.h_at(key[i]);
+        c[0][i] = h.da_xy; c[1][i] = h.da_xz; c[2][i] = h.db_xy1; c[3][i] = h.db_xy2; c[4][i] = h.db_xz1;
+        c[5][i] = h.db_xz2; c[6][i] = h.da_yz; c[7][i] = h.da_yx; c[8][i] = h.db_yz1; c[9][i] = h.db_yz2;
+        c[10][i] = h.db_yx1; c[11][i] = h.db_yx2; c[12][i] = h.da_zx; c[13][i] = h.da_zy; c[14][i] = h.db_zx1;
+        c[15][i] = h.db_zx2; c[16][i] = h.db_zy1; c[17][i] = h.db_zy2;
+    }
+}
+
+template <typename T, int W>
+struct EInW {
+    T hx_c[W], hy_xn[W], hz_xn[W], hz_yn[W], hx_yn[W];
+    T hy_zw, hx_zw;  // hy, hx at z + W (the next group's first cell)
+    T e[6][W];       // ex_y, ex_z, ey_z, ey_x, ez_x, ez_y
+    T c[12][W];
+    uint32_t pw;
+};
+
+template <typename T, int W>
+struct HInW {
+    T ex_c[W], ey_c[W], ez_c[W], ez_ym[W], ex_ym[W];
+    T ey_z1, ex_z1;  // ey, ex at z - 1 (the previous group's last cell)
+    T h[6][W];       // hx_y, hx_z, hy_z, hy_x, hz_x, hz_y
+    T c[18][W];
+    uint32_t pw;
+};
+
+template <typename T, class CA, int W, bool PF>
+__device__ __forceinline__ void e_column_w(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                           const ProbeFold<T>& P) {
+    using G = GLOBAL_T(T);
+    G* const es[6] = {(G*)F.ex_y, (G*)F.ex_z, (G*)F.ey_z, (G*)F.ey_x, (G*)F.ez_x, (G*)F.ez_y};
+    G* const ex = (G*)F.ex; G* const ey = (G*)F.ey; G* const ez = (G*)F.ez;
+    const G* const hx = (const G*)F.hx; const G* const hy = (const G*)F.hy; const G* const hz = (const G*)F.hz;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
+    const bool yin = t.y < Ny - 1;
+    const bool next = t.z + W < Nz;  // the next group is in this row
+    auto load = [&](EInW<T, W>& v, uint32_t b, bool xin) {
+        ldw<false, W>(hx, b, v.hx_c);
+        if (xin) {
+            ldw<false, W>(hy, b + sx, v.hy_xn);
+            ldw<false, W>(hz, b + sx, v.hz_xn);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; ++i) v.hy_xn[i] = v.hz_xn[i] = T(0);
+        }
+        if (yin) {
+            ldw<false, W>(hz, b + sy, v.hz_yn);
+            ldw<false, W>(hx, b + sy, v.hx_yn);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; ++i) v.hz_yn[i] = v.hx_yn[i] = T(0);
+        }
+        v.hy_zw = next ? ldg(hy, b + W * S) : T(0);
+        v.hx_zw = next ? ldg(hx, b + W * S) : T(0);
+        coef_e_w<T, W>(C, b, v.c);
+        v.pw = P.word(b / S);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ldw<true, W>(es[k], b, v.e[k]);
+    };
+    if (t.x0 >= t.x1) return;
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    T hy_c[W], hz_c[W];
+    ldw<false, W>(hy, b, hy_c);
+    ldw<false, W>(hz, b, hz_c);
+    EInW<T, W> cur;
+    load(cur, b, t.x0 < Nx - 1);
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool xin = x < Nx - 1, more = x + 1 < t.x1;
+        EInW<T, W> nxt;
+        if (PF && more) load(nxt, b + sx, x + 1 < Nx - 1);
+        T o[6][W], tx[W], ty[W], tz[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const bool zin = t.z + i < Nz - 1;
+            const T hy_zn = i + 1 < W ? hy_c[i + 1] : cur.hy_zw;
+            const T hx_zn = i + 1 < W ? cur.hx_c[i + 1] : cur.hx_zw;
+            const T(&c)[12][W] = cur.c;
+            T exy = cur.e[0][i], exz = cur.e[1][i], eyz = cur.e[2][i], eyx = cur.e[3][i], ezx = cur.e[4][i],
+              ezy = cur.e[5][i];
+            if (yin) exy = c[0][i] * exy + c[2][i] * (cur.hz_yn[i] - hz_c[i]);
+            if (zin) exz = c[1][i] * exz + c[3][i] * (hy_zn - hy_c[i]);
+            if (zin) eyz = c[4][i] * eyz + c[6][i] * (hx_zn - cur.hx_c[i]);
+            if (xin) eyx = c[5][i] * eyx + c[7][i] * (cur.hz_xn[i] - hz_c[i]);
+            if (xin) ezx = c[8][i] * ezx + c[10][i] * (cur.hy_xn[i] - hy_c[i]);
+            if (yin) ezy = c[9][i] * ezy + c[11][i] * (cur.hx_yn[i] - cur.hx_c[i]);
+            P.apply(b / S + i, cur.pw, exy, exz, eyz, eyx, ezx, ezy);
+            o[0][i] = exy; o[1][i] = exz; o[2][i] = eyz; o[3][i] = eyx; o[4][i] = ezx; o[5][i] = ezy;
+            tx[i] = exy + exz;
+            ty[i] = eyz + eyx;
+            tz[i] = ezx + ezy;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stw<W>(es[k], b, o[k]);
+        stw<W>(ex, b, tx);
+        stw<W>(ey, b, ty);
+        stw<W>(ez, b, tz);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            hy_c[i] = cur.hy_xn[i];
+            hz_c[i] = cur.hz_xn[i];
+        }
+        if (PF)
+            cur = nxt;
+        else if (more)
+            load(cur, b + sx, x + 1 < Nx - 1);
+    }
+}
+
+template <typename T, class CA, int W, bool PF>
+__device__ __forceinline__ void h_column_w(const Fields<T>& F, const CA& C, int Nx, int Ny, int Nz, const Tile& t,
+                                           const ProbeFold<T>& P) {
+    using G = GLOBAL_T(T);
+    G* const hs[6] = {(G*)F.hx_y, (G*)F.hx_z, (G*)F.hy_z, (G*)F.hy_x, (G*)F.hz_x, (G*)F.hz_y};
+    G* const hx = (G*)F.hx; G* const hy = (G*)F.hy; G* const hz = (G*)F.hz;
+    const G* const ex = (const G*)F.ex; const G* const ey = (const G*)F.ey; const G* const ez = (const G*)F.ez;
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S, sy = (uint32_t)Nz * S;
+    const bool ylo = t.y > 0, zlo = t.z > 0;  // zlo: the previous group is in this row
+    (void)Nx;
+    auto load = [&](HInW<T, W>& v, uint32_t b) {
+        ldw<false, W>(ex, b, v.ex_c);
+        ldw<false, W>(ey, b, v.ey_c);
+        ldw<false, W>(ez, b, v.ez_c);
+        if (ylo) {
+            ldw<false, W>(ez, b - sy, v.ez_ym);
+            ldw<false, W>(ex, b - sy, v.ex_ym);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; ++i) v.ez_ym[i] = v.ex_ym[i] = T(0);
+        }
+        v.ey_z1 = zlo ? ldg(ey, b - S) : T(0);
+        v.ex_z1 = zlo ? ldg(ex, b - S) : T(0);
+        coef_h_w<T, W>(C, b, v.c);
+        v.pw = P.word(b / S);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ldw<true, W>(hs[k], b, v.h[k]);
+    };
+    if (t.x0 >= t.x1) return;
+    uint32_t b = ((uint32_t)t.x0 * (uint32_t)Ny * (uint32_t)Nz + (uint32_t)t.p) * S;
+    T ey_xm[W], ez_xm[W];
+    if (t.x0 > 0) {
+        ldw<false, W>(ey, b - sx, ey_xm);
+        ldw<false, W>(ez, b - sx, ez_xm);
+    } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) ey_xm[i] = ez_xm[i] = T(0);
+    }
+    HInW<T, W> cur;
+    load(cur, b);
+    for (int x = t.x0; x < t.x1; ++x, b += sx) {
+        const bool more = x + 1 < t.x1;
+        HInW<T, W> nxt;
+        if (PF && more) load(nxt, b + sx);
+        T o[6][W], tx[W], ty[W], tz[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const T ey_zm = i > 0 ? cur.ey_c[i - 1] : cur.ey_z1;
+            const T ex_zm = i > 0 ? cur.ex_c[i - 1] : cur.ex_z1;
+            const T(&c)[18][W] = cur.c;
+            T hxy = c[0][i] * cur.h[0][i] + (c[3][i] * cur.ez_c[i] - c[2][i] * cur.ez_ym[i]);
+            T hxz = c[1][i] * cur.h[1][i] + (c[5][i] * cur.ey_c[i] - c[4][i] * ey_zm);
+            T hyz = c[6][i] * cur.h[2][i] + (c[9][i] * cur.ex_c[i] - c[8][i] * ex_zm);
+            T hyx = c[7][i] * cur.h[3][i] + (c[11][i] * cur.ez_c[i] - c[10][i] * ez_xm[i]);
+            T hzx = c[12][i] * cur.h[4][i] + (c[15][i] * cur.ey_c[i] - c[14][i] * ey_xm[i]);
+            T hzy = c[13][i] * cur.h[5][i] + (c[17][i] * cur.ex_c[i] - c[16][i] * cur.ex_ym[i]);
+            P.apply(b / S + i, cur.pw, hxy, hxz, hyz, hyx, hzx, hzy);
+            o[0][i] = hxy; o[1][i] = hxz; o[2][i] = hyz; o[3][i] = hyx; o[4][i] = hzx; o[5][i] = hzy;
+            tx[i] = hxy + hxz;
+            ty[i] = hyz + hyx;
+            tz[i] = hzx + hzy;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stw<W>(hs[k], b, o[k]);
+        stw<W>(hx, b, tx);
+        stw<W>(hy, b, ty);
+        stw<W>(hz, b, tz);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            ey_xm[i] = cur.ey_c[i];
+            ez_xm[i] = cur.ez_c[i];
+        }
+        if (PF)
+            cur = nxt;
+        else if (more)
+            load(cur, b + sx);
+    }
+}
+
+template <typename T, int W, bool PF>
+__global__ void __launch_bounds__(512) k_update_e_w(const InstDev<T>* __restrict__ insts,
+                                                     const MemberDev* __restrict__ mem, int nmem, int total,
+                                                     const int* base, int local, int io) {
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t, W);
+    if (!I) return;
+    const int ng = io ? I->negroups : 0;
+    const ProbeFold<T> P{I->ebits, (const ProbeGroup<T>*)I->egroups, ng, ng ? step_of(base, local) : 0};
+    if (I->coef_kind == 1)
+        e_column_w<T, CoefTable<T>, W, PF>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny,
+                                           I->Nz, t, P);
+    else
+        e_column_w<T, CoefRaw<T>, W, PF>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+}
+
+template <typename T, int W, bool PF>
+__global__ void __launch_bounds__(512) k_update_h_w(const InstDev<T>* __restrict__ insts,
+                                                     const MemberDev* __restrict__ mem, int nmem, int total,
+                                                     const int* base, int local, int io) {
+    __shared__ int s_member;
+    Tile t;
+    const InstDev<T>* I = locate(insts, mem, nmem, total, &s_member, t, W);
+    if (!I) return;
+    const int ng = io ? I->nhgroups : 0;
+    const ProbeFold<T> P{I->hbits, (const ProbeGroup<T>*)I->hgroups, ng, ng ? step_of(base, local) : 0};
+    if (I->coef_kind == 1)
+        h_column_w<T, CoefTable<T>, W, PF>(I->F, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, I->Nx, I->Ny,
+                                           I->Nz, t, P);
+    else
+        h_column_w<T, CoefRaw<T>, W, PF>(I->F, CoefRaw<T>{I->raw}, I->Nx, I->Ny, I->Nz, t, P);
+}
 
 // ------------------------------------------------------------------------------------------------
 // fused step: E and H half steps of one time step in one pass, from one field buffer into the other
@@ -2880,6 +3171,7 @@ struct Session {
     int fused = 0;          // 0: separate E and H kernels; else the fused step with this many rows per tile
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
     bool nt_loads = false;     // ... and nontemporal loads of the arrays streamed once (variant 10)
+    int vec = 1;               // separate kernels over vec consecutive z cells per thread (variants 11, 12)
     // the separate kernels' own launch shape (threads, x planes per workgroup, 0 = automatic), which
     // a gradient's recomputation uses whatever the forward variant: the tuner's best of variants
     // 0, 1, 4, 10
@@ -3496,7 +3788,40 @@ void assemble_coefficients(Member& M, uint32_t flags, const rfdtd_assembly& a) {
 int64_t tiles_per_plane(const Session& s, const Member& M) {
     if (s.two_step) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kP2Fz - 1) / kP2Fz);
     if (s.fused) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kFz - 1) / kFz);
-    return ((int64_t)M.Ny * M.Nz + s.block - 1) / s.block;
+    return ((int64_t)M.Ny * M.Nz / s.vec + s.block - 1) / s.block;
+}
+
+// Cells per thread of the separate kernels for a variant (1, or 2 / 4 for the vector columns), and
+// whether every grid of the session allows it (Nz a multiple of it).
+int variant_vec(int v) { return v == 11 ? 2 : v == 12 ? 4 : 1; }
+bool vec_fits(const Session& s, int vec) {
+    for (const auto& M : s.members)
+        if (M.Nz % vec) return false;
+    return true;
+}
+
+// The session's kernel flags for a variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row
+// tiles, 3 fused 14-row tiles, 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead
+// of H, 7 and 8 the two-step pass with 12- and 4-row tiles, 9 its one-barrier form with 8-row tiles,
+// 10 = 4 with nontemporal loads, 11 and 12 = 10 over 2 and 4 consecutive z cells per thread.
+void apply_variant(Session& s, int v) {
+    s.pipelined = v != 0;
+    s.two_step = v == 7 || v == 8 || v == 9;
+    s.two_skew = v == 9;
+    s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : v == 9 ? kP2TySkew
+            : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
+    s.fused_deep = v == 5 || v == 6;
+    s.nontemporal = v == 4 || v >= 10;
+    s.nt_loads = v >= 10;
+    s.vec = variant_vec(v);
+    s.dirty = true;
+    s.graph_dirty = true;
+}
+
+int variant_of(const Session& s) {
+    return s.vec == 4 ? 12 : s.vec == 2 ? 11 : s.nt_loads ? 10 : s.two_skew ? 9
+         : s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3)
+         : s.fused == 6 ? (s.fused_deep ? 5 : 2) : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
 }
 
 void plan_launch(Session& s) {
@@ -3802,8 +4127,8 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
     const bool io = !s.tuning;
     // H^n of the design boxes before this step's E half step: in the E launch for the separate
     // kernels, a launch of its own before the fused pass
-    const bool save_in_e = io && s.hist_blocks && !s.fused;
-    if (io && s.hist_blocks && s.fused)
+    const bool save_in_e = io && s.hist_blocks && !s.fused && s.vec == 1;
+    if (io && s.hist_blocks && (s.fused || s.vec > 1))
         k_save_h<T><<<s.hist_blocks, 256, 0, st>>>(di, (const BoxDev<T>*)s.d_hist_boxes, s.d_hist_h0, (int)s.hist.size(),
                                                   local, base);
     if (s.two_step) {
@@ -3828,6 +4153,23 @@ void launch_step(Session& s, hipStream_t st, const int* base, int local, int slo
         s.parity ^= 1;
         s.totals_stale = true;
         di = desc[s.parity];
+    } else if (s.vec > 1) {
+        // the separate kernels over 2 or 4 z cells per thread, probes applied inside
+        const int pio = io ? 1 : 0;
+        if (ev) HIP_OK(hipEventRecord(ev[0], st));
+        if (s.vec == 2)
+            k_update_e_w<T, 2, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
+        else
+            k_update_e_w<T, 4, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
+        if (ev) {
+            HIP_OK(hipEventRecord(ev[1], st));
+            HIP_OK(hipEventRecord(ev[2], st));
+        }
+        if (s.vec == 2)
+            k_update_h_w<T, 2, true><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
+        else
+            k_update_h_w<T, 4, false><<<grid, s.block, 0, st>>>(di, s.d_members, nmem, s.total_tiles, base, local, pio);
+        if (ev) HIP_OK(hipEventRecord(ev[3], st));
     } else {
         // E and H probes (sources and records) are applied inside the update kernels
         const int pio = io ? 1 : 0;
@@ -4123,23 +4465,16 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const int xp = (maxNx + chunks - 1) / chunks;
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     }
-    const int v0 = s.nt_loads ? 10 : s.two_skew ? 9 : s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3) : s.fused == 6 ? (s.fused_deep ? 5 : 2)
-                 : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
+    // one or two planes per workgroup: no marching, every plane's loads of a thread in flight at once
+    // (the separate kernels on large grids run fastest so)
+    for (int xp : {1, 2})
+        if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
+    const int v0 = variant_of(s);
     const int blk0 = s.block;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
-    // variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row tiles, 3 fused 14-row tiles,
-    // 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead of H, 7 and 8 the two-step
-    // pass with 12- and 4-row tiles, 9 its one-barrier form with 8-row tiles
     auto apply = [&](int v, int blk, const std::vector<int>& xr) {
-        s.pipelined = v != 0;
-        s.two_step = v == 7 || v == 8 || v == 9;
-        s.two_skew = v == 9;
-        s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : v == 9 ? kP2TySkew
-                : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
-        s.fused_deep = v == 5 || v == 6;
-        s.nontemporal = v == 4 || v == 10;
-        s.nt_loads = v == 10;
+        apply_variant(s, v);
         s.block = blk;
         for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
         s.dirty = true;
@@ -4165,11 +4500,12 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     HIP_OK(hipGetLastError());
     try {
         const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
-        for (int v : {1, 4, 10, 0, 2, 3, 5, 6, 7, 8, 9})
+        for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
                     const bool fused_v = v == 2 || v == 3 || (v >= 5 && v <= 9);
                     if (fused_v && separate_only) continue;
+                    if (!vec_fits(s, variant_vec(v))) continue;
                     // a design-box history needs H after every step: the two-step pass would run
                     // single steps
                     if (v >= 7 && v <= 9 && !s.hist.empty()) continue;
@@ -4195,7 +4531,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
                         bblk = blk;
                         bxp = xp;
                     }
-                    if (!fused_v && per < best_sep) {
+                    if (!fused_v && variant_vec(v) == 1 && per < best_sep) {  // the recomputation's shape
                         best_sep = per;
                         sblk = blk;
                         sxp = xp;
@@ -5339,17 +5675,11 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 10) RF_THROW(RFDTD_EINVAL, "variant must be in 0..10");
-        S.pipelined = variant != 0;
-        S.two_step = variant == 7 || variant == 8 || variant == 9;
-        S.two_skew = variant == 9;
-        S.fused = variant == 7 ? kP2TyTall : variant == 8 ? kP2TyShort : variant == 9 ? kP2TySkew
-                : (variant == 2 || variant == 5) ? 6 : (variant == 3 || variant == 6) ? 14 : 0;
-        S.fused_deep = variant == 5 || variant == 6;
-        S.nontemporal = variant == 4 || variant == 10;
-        S.nt_loads = variant == 10;
-        S.dirty = true;
-        S.graph_dirty = true;
+        if (variant < 0 || variant > 12) RF_THROW(RFDTD_EINVAL, "variant must be in 0..12");
+        if (!vec_fits(S, variant_vec(variant)))
+            RF_THROW(RFDTD_EINVAL, "variant " + std::to_string(variant) + " needs Nz a multiple of " +
+                                       std::to_string(variant_vec(variant)) + " in every grid");
+        apply_variant(S, variant);
         if (!S.fused) release_second_buffers(S);
     });
 }

@@ -88,6 +88,36 @@ def test_launch_shapes_agree(hip_ready):
                     np.testing.assert_array_equal(f[k], ref[k], err_msg=f"{k} {launch} {variant}")
 
 
+@pytest.mark.parametrize("dtype,flags", [(np.float32, _lib.COEF_RAW), (np.float32, _lib.COEF_TABLE),
+                                         (np.float64, _lib.COEF_RAW)])
+def test_vector_columns_match_oracle(hip_ready, dtype, flags):
+    """The separate kernels over 2 and 4 consecutive z cells per thread (variants 11 and 12, Nz a
+    multiple of 4): probes, monitors and every field bit-exact against the oracle, across launch
+    shapes and graph chunk lengths (a group reaching the row's end, one starting a row, x chunks of
+    1 .. all planes)."""
+    model = workloads.tiny_model(Nt=60, nx=15, ny=11, nz=20)
+    print(f"[case] vector columns dtype={np.dtype(dtype).name} flags={flags}", flush=True)
+    p_c, m_c, f_c = run_oracle(model, dtype, n_threads=3)
+    for variant in (11, 12):
+        for launch in [(256, 0, 100), (64, 1, 7), (128, 15, 1), (512, 4, 60)]:
+            p_g, m_g, f_g, _ = _hip_session(model, dtype, flags, chunks=[29, 31], launch=launch, variant=variant)
+            for a, b in zip(p_g, p_c):
+                np.testing.assert_array_equal(a["values"], b["values"], err_msg=f"{variant} {launch}")
+            _assert_monitors_equal(model, p_g, m_g, p_c, m_c)
+            for k in f_c:
+                np.testing.assert_array_equal(f_g[k], f_c[k], err_msg=f"{k} {variant} {launch}")
+
+
+def test_vector_columns_need_nz_multiple(hip_ready):
+    """Variants 11 and 12 are refused on a grid whose Nz is not a multiple of 2 or 4."""
+    model = workloads.tiny_model(Nt=10, nz=9)
+    c, probes, monitors, mem, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    with Session(c, Nx, Ny, Nz) as s:
+        for v in (11, 12):
+            with pytest.raises(ValueError, match="multiple"):
+                s.set_variant(v)
+
+
 def test_solve_api_matches_oracle(hip_ready):
     """FDTD_Solver.solve(gpu=True) end to end: port series, probes, monitors and S-parameters."""
     from oracle import cpu_solver
