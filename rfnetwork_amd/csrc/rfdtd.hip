@@ -2382,7 +2382,11 @@ __global__ void k_monitor_flush(const MonitorDev* __restrict__ mons, int nmon, i
 // 50% more halo reads).
 constexpr int kP2TyTall = 12, kP2TyShort = 4;
 constexpr int kP2TySkew = 8;  // the one-barrier pass (more registers: 12 wavefronts, one workgroup per CU)
-constexpr int kP2Fz = 60;   // z cells owned by a two-step tile
+constexpr int kP2Fz = 60;   // z cells owned by a two-step tile of 64-lane rows
+// Tiles of 32-lane rows (two rows per wavefront; 28 z cells owned per row): 28 rows in 1024 threads
+// (76.5% of the lanes own a cell, against 70.3% for 12 x 60 in 64-lane rows), or 12 rows in 512
+// threads (two workgroups per CU); narrow grids (Nz = 64) waste less of their second z tile.
+constexpr int kP2Ty32Tall = 28, kP2Ty32Short = 12;
 constexpr int kP2Mon = 64;  // monitors a two-step pass samples itself, at most
 
 // The monitor samples of one owned cell at step n: every monitor of the tile's list (those of this
@@ -2417,23 +2421,23 @@ __device__ __forceinline__ void p2_sample(const MonitorDev* __restrict__ mons, c
     }
 }
 
-template <typename T, int TY, class CA>
+template <typename T, int TY, class CA, int RS = 64>
 __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, int x0, int x1, int y0, int z0, int n,
                                               bool io, bool two, const MonitorDev* __restrict__ mons,
                                               const int* mlist, int nmon, int slot, T* sm) {
     using G = GLOBAL_T(T);
     using G32 = GLOBAL_AS uint32_t;
-    constexpr int NT = (TY + 4) * 64;
+    constexpr int NT = (TY + 4) * RS;  // lanes: TY + 4 rows of RS lanes (64, or 32: two rows per wavefront)
     constexpr uint32_t S = (uint32_t)sizeof(T);
     constexpr bool kEarly = std::is_same<CA, CoefTable<T>>::value;  // all gathers at the iteration's top
     const int Nx = I.Nx, Ny = I.Ny, Nz = I.Nz;
     const uint32_t sx = (uint32_t)Ny * (uint32_t)Nz * S;
-    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int tid = threadIdx.x, w = tid / RS, l = tid % RS;  // row and lane in the row
     const int y = y0 - 2 + w, z = z0 - 2 + l;
     const bool valid = y >= 0 && y < Ny && z >= 0 && z < Nz;
-    const bool mAe = valid && w <= TY + 2 && l <= 62;    // E of step n right here
-    const bool mAh = mAe && w >= 1 && l >= 1;            // H of step n
-    const bool mBe = mAh && w <= TY + 1 && l <= 61;      // E of step n + 1
+    const bool mAe = valid && w <= TY + 2 && l <= RS - 2;  // E of step n right here
+    const bool mAh = mAe && w >= 1 && l >= 1;              // H of step n
+    const bool mBe = mAh && w <= TY + 1 && l <= RS - 3;    // E of step n + 1
     const bool own = mBe && w >= 2 && l >= 2;            // H of step n + 1; stored, recorded
     const bool yin = y < Ny - 1, zin = z < Nz - 1;
     T* const s0 = sm;              // [3][NT] input H totals, plane i
@@ -2530,7 +2534,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         if (doA && mAe) {
             const ECoef<T>& c = cA;
             const bool xin = i < Nx - 1;
-            const T hz_yn = yin ? s0[2 * NT + tid + 64] : T(0), hx_yn = yin ? s0[tid + 64] : T(0);
+            const T hz_yn = yin ? s0[2 * NT + tid + RS] : T(0), hx_yn = yin ? s0[tid + RS] : T(0);
             const T hy_zn = zin ? s0[NT + tid + 1] : T(0), hx_zn = zin ? s0[tid + 1] : T(0);
             const T hy_xn = xin ? hN.yz + hN.yx : T(0), hz_xn = xin ? hN.zx + hN.zy : T(0);
             if (yin) e1.xy = c.ca_xy * e1.xy + c.cb_xy * (hz_yn - h0z);
@@ -2573,7 +2577,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
         T h1x = T(0), h1y = T(0), h1z = T(0);
         if (doA && mAh) {
             const HCoef<T>& ch = chA;
-            const T ez_ym = y > 0 ? s1e[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s1e[tid - 64] : T(0);
+            const T ez_ym = y > 0 ? s1e[2 * NT + tid - RS] : T(0), ex_ym = y > 0 ? s1e[tid - RS] : T(0);
             const T ey_zm = z > 0 ? s1e[NT + tid - 1] : T(0), ex_zm = z > 0 ? s1e[tid - 1] : T(0);
             h1.xy = ch.da_xy * h0.xy + (ch.db_xy2 * e1z - ch.db_xy1 * ez_ym);
             h1.xz = ch.da_xz * h0.xz + (ch.db_xz2 * e1y - ch.db_xz1 * ey_zm);
@@ -2612,7 +2616,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
                 const ECoef<T>& c = cB;
                 const bool xin = j < Nx - 1;
                 const T hx_c = h1p.xy + h1p.xz, hy_c = h1p.yz + h1p.yx, hz_c = h1p.zx + h1p.zy;
-                const T hz_yn = yin ? shj[2 * NT + tid + 64] : T(0), hx_yn = yin ? shj[tid + 64] : T(0);
+                const T hz_yn = yin ? shj[2 * NT + tid + RS] : T(0), hx_yn = yin ? shj[tid + RS] : T(0);
                 const T hy_zn = zin ? shj[NT + tid + 1] : T(0), hx_zn = zin ? shj[tid + 1] : T(0);
                 const T hy_xn = xin ? h1y : T(0), hz_xn = xin ? h1z : T(0);  // plane i, this iteration's step n
                 if (yin) e2.xy = c.ca_xy * e2.xy + c.cb_xy * (hz_yn - hz_c);
@@ -2634,7 +2638,7 @@ __device__ __forceinline__ void two_step_tile(const InstDev<T>& I, const CA& C, 
             __syncthreads();
             if (recB) {
                 const HCoef<T>& c = chB;
-                const T ez_ym = y > 0 ? s2e[2 * NT + tid - 64] : T(0), ex_ym = y > 0 ? s2e[tid - 64] : T(0);
+                const T ez_ym = y > 0 ? s2e[2 * NT + tid - RS] : T(0), ex_ym = y > 0 ? s2e[tid - RS] : T(0);
                 const T ey_zm = z > 0 ? s2e[NT + tid - 1] : T(0), ex_zm = z > 0 ? s2e[tid - 1] : T(0);
                 Split6<T> h2;
                 h2.xy = c.da_xy * h1p.xy + (c.db_xy2 * e2z - c.db_xy1 * ez_ym);
@@ -2885,14 +2889,15 @@ __device__ __forceinline__ void two_step_skew_tile(const InstDev<T>& I, const CA
 
 // two != 0: steps n and n + 1 (n from base/local); the phasor monitors' sample rows of step n's
 // staging slot are written by logical workgroup 0 (k_monitors writes step n + 1's).
-template <typename T, int TY, bool SKEW>
-__global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __restrict__ insts,
+template <typename T, int TY, bool SKEW, int RS = 64>
+__global__ void __launch_bounds__((TY + 4) * RS) k_step2(const InstDev<T>* __restrict__ insts,
                                                             const MemberDev* __restrict__ mem, int nmem, int total,
                                                             const int* base, int local, int io, int two,
                                                             const MonitorDev* __restrict__ mons, int nmon, int slot,
                                                             int* __restrict__ rows) {
     __shared__ int s_member;
-    __shared__ T sm[(SKEW ? 24 : 15) * (TY + 4) * 64];
+    __shared__ T sm[(SKEW ? 24 : 15) * (TY + 4) * RS];
+    constexpr int FZ = RS - 4;  // z cells owned per row
     const int L = xcd_logical(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         int m = 0;
@@ -2914,9 +2919,9 @@ __global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __res
     const int e = local_t % M.K, tile = local_t / M.K;
     const int inst = __builtin_amdgcn_readfirstlane(M.inst0 + e);
     const InstDev<T>* I = insts + inst;
-    const int tz_n = (I->Nz + kP2Fz - 1) / kP2Fz;
+    const int tz_n = (I->Nz + FZ - 1) / FZ;
     const int pb = tile % I->npb, xc = tile / I->npb;
-    const int y0 = (pb / tz_n) * TY, z0 = (pb % tz_n) * kP2Fz;
+    const int y0 = (pb / tz_n) * TY, z0 = (pb % tz_n) * FZ;
     const int x0 = xc * I->xplanes, x1 = min(I->Nx, x0 + I->xplanes);
     // the monitors this tile samples at step n: of this instance, sampling at n, with their plane
     // through the tile's owned cells (usually none); wavefront 0 compacts them into LDS
@@ -2932,7 +2937,7 @@ __global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __res
                 if (Mk.inst == inst && n % Mk.n_step == 0)
                     hit = Mk.axis == 0 ? (Mk.pos >= x0 && Mk.pos < x1)
                         : Mk.axis == 1 ? (Mk.pos >= y0 && Mk.pos < y0 + TY)
-                                       : (Mk.pos >= z0 && Mk.pos < z0 + kP2Fz);
+                                       : (Mk.pos >= z0 && Mk.pos < z0 + FZ);
             }
             const uint64_t ball = __ballot(hit);
             if (hit) s_mlist[__popcll(ball & ((1ull << k) - 1ull))] = k;
@@ -2941,6 +2946,7 @@ __global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __res
         __syncthreads();
         nmt = __builtin_amdgcn_readfirstlane(s_mcount);
     }
+    static_assert(!SKEW || RS == 64, "the skewed pass has 64-lane rows");
     if (SKEW && two) {
         if (I->coef_kind == 1)
             two_step_skew_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0,
@@ -2949,11 +2955,11 @@ __global__ void __launch_bounds__((TY + 4) * 64) k_step2(const InstDev<T>* __res
             two_step_skew_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, mons, s_mlist, nmt,
                                                   slot, sm);
     } else if (I->coef_kind == 1) {
-        two_step_tile<T, TY, CoefTable<T>>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0, z0, n,
-                                           io != 0, two != 0, mons, s_mlist, nmt, slot, sm);
+        two_step_tile<T, TY, CoefTable<T>, RS>(*I, CoefTable<T>{I->eidx, I->hidx, I->etab, I->htab}, x0, x1, y0, z0,
+                                               n, io != 0, two != 0, mons, s_mlist, nmt, slot, sm);
     } else {
-        two_step_tile<T, TY, CoefRaw<T>>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, two != 0, mons, s_mlist, nmt,
-                                         slot, sm);
+        two_step_tile<T, TY, CoefRaw<T>, RS>(*I, CoefRaw<T>{I->raw}, x0, x1, y0, z0, n, io != 0, two != 0, mons,
+                                             s_mlist, nmt, slot, sm);
     }
 }
 
@@ -3172,6 +3178,7 @@ struct Session {
     bool nontemporal = false;  // separate pipelined kernels with nontemporal field stores
     bool nt_loads = false;     // ... and nontemporal loads of the arrays streamed once (variant 10)
     int vec = 1;               // separate kernels over vec consecutive z cells per thread (variants 11, 12)
+    int p2_rs = 64;            // two-step tiles: lanes per row (64, or 32 for variants 13, 14)
     // the separate kernels' own launch shape (threads, x planes per workgroup, 0 = automatic), which
     // a gradient's recomputation uses whatever the forward variant: the tuner's best of variants
     // 0, 1, 4, 10
@@ -3786,7 +3793,7 @@ void assemble_coefficients(Member& M, uint32_t flags, const rfdtd_assembly& a) {
 // launch on 256 CUs), as many planes as possible for register reuse along x.
 // Workgroups per x plane of a member: (y,z) columns per workgroup, or fused tiles.
 int64_t tiles_per_plane(const Session& s, const Member& M) {
-    if (s.two_step) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kP2Fz - 1) / kP2Fz);
+    if (s.two_step) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + s.p2_rs - 5) / (s.p2_rs - 4));
     if (s.fused) return (int64_t)((M.Ny + s.fused - 1) / s.fused) * ((M.Nz + kFz - 1) / kFz);
     return ((int64_t)M.Ny * M.Nz / s.vec + s.block - 1) / s.block;
 }
@@ -3803,13 +3810,16 @@ bool vec_fits(const Session& s, int vec) {
 // The session's kernel flags for a variant: 0 separate kernels, 1 separate pipelined, 2 fused 6-row
 // tiles, 3 fused 14-row tiles, 4 = 1 with nontemporal stores, 5 and 6 = 2 and 3 with E one plane ahead
 // of H, 7 and 8 the two-step pass with 12- and 4-row tiles, 9 its one-barrier form with 8-row tiles,
-// 10 = 4 with nontemporal loads, 11 and 12 = 10 over 2 and 4 consecutive z cells per thread.
+// 10 = 4 with nontemporal loads, 11 and 12 = 10 over 2 and 4 consecutive z cells per thread, 13 and
+// 14 the two-step pass with 32-lane rows (28 x 28 and 12 x 28 cell tiles).
+bool is_two_step(int v) { return v == 7 || v == 8 || v == 9 || v == 13 || v == 14; }
 void apply_variant(Session& s, int v) {
     s.pipelined = v != 0;
-    s.two_step = v == 7 || v == 8 || v == 9;
+    s.two_step = is_two_step(v);
     s.two_skew = v == 9;
-    s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : v == 9 ? kP2TySkew
-            : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
+    s.p2_rs = (v == 13 || v == 14) ? 32 : 64;
+    s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : v == 9 ? kP2TySkew : v == 13 ? kP2Ty32Tall
+            : v == 14 ? kP2Ty32Short : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
     s.fused_deep = v == 5 || v == 6;
     s.nontemporal = v == 4 || v >= 10;
     s.nt_loads = v >= 10;
@@ -3820,6 +3830,7 @@ void apply_variant(Session& s, int v) {
 
 int variant_of(const Session& s) {
     return s.vec == 4 ? 12 : s.vec == 2 ? 11 : s.nt_loads ? 10 : s.two_skew ? 9
+         : (s.two_step && s.p2_rs == 32) ? (s.fused == kP2Ty32Tall ? 13 : 14)
          : s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3)
          : s.fused == 6 ? (s.fused_deep ? 5 : 2) : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
 }
@@ -4080,6 +4091,14 @@ void launch_step2(Session& s, hipStream_t st, const int* base, int local, bool t
     if (ev) HIP_OK(hipEventRecord(ev[0], st));
     if (s.two_skew)
         k_step2<T, kP2TySkew, true><<<grid, (kP2TySkew + 4) * 64, 0, st>>>(
+            di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
+            slot, s.d_rows);
+    else if (s.p2_rs == 32 && s.fused == kP2Ty32Tall)
+        k_step2<T, kP2Ty32Tall, false, 32><<<grid, (kP2Ty32Tall + 4) * 32, 0, st>>>(
+            di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
+            slot, s.d_rows);
+    else if (s.p2_rs == 32)
+        k_step2<T, kP2Ty32Short, false, 32><<<grid, (kP2Ty32Short + 4) * 32, 0, st>>>(
             di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
             slot, s.d_rows);
     else if (s.fused == kP2TyShort)
@@ -4500,15 +4519,15 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     HIP_OK(hipGetLastError());
     try {
         const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
-        for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9})
+        for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9, 13, 14})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
-                    const bool fused_v = v == 2 || v == 3 || (v >= 5 && v <= 9);
+                    const bool fused_v = v == 2 || v == 3 || v == 5 || v == 6 || is_two_step(v);
                     if (fused_v && separate_only) continue;
                     if (!vec_fits(s, variant_vec(v))) continue;
                     // a design-box history needs H after every step: the two-step pass would run
                     // single steps
-                    if (v >= 7 && v <= 9 && !s.hist.empty()) continue;
+                    if (is_two_step(v) && !s.hist.empty()) continue;
                     if (fused_v && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
                     apply(v, blk, std::vector<int>(s.members.size(), xp));
                     refresh<T>(s);
@@ -5675,7 +5694,7 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 12) RF_THROW(RFDTD_EINVAL, "variant must be in 0..12");
+        if (variant < 0 || variant > 14) RF_THROW(RFDTD_EINVAL, "variant must be in 0..14");
         if (!vec_fits(S, variant_vec(variant)))
             RF_THROW(RFDTD_EINVAL, "variant " + std::to_string(variant) + " needs Nz a multiple of " +
                                        std::to_string(variant_vec(variant)) + " in every grid");
