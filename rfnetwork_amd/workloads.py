@@ -262,6 +262,59 @@ def design_variant(v, eps_design=None, Nt=8000, cell_m=1e-4, courant=0.99, n_pml
     return s
 
 
+def graded_edges(fixed, lo, hi, d_fine, d_max, ratio):
+    """Grid edges from lo to hi (inches) through every edge in `fixed`: cells of at most d_fine between
+    consecutive fixed edges (equal within each span), growing by `ratio` per cell up to d_max outside
+    them; a last cell shorter than half its neighbour is merged into it."""
+    fe = sorted(float(v) for v in fixed)
+    edges = [fe[0]]
+    for a, b in zip(fe[:-1], fe[1:]):
+        n = max(1, int(np.ceil((b - a) / d_fine - 1e-9)))
+        edges += list(a + (b - a) * np.arange(1, n + 1) / n)
+
+    def grow(start, stop, sign):
+        out, d = [start], d_fine
+        while sign * (stop - out[-1]) > 1e-12:
+            d = min(d * ratio, d_max)
+            out.append(out[-1] + sign * d if sign * (stop - out[-1] - sign * d) > 0 else stop)
+        if len(out) > 2 and abs(out[-1] - out[-2]) < 0.5 * abs(out[-2] - out[-3]):
+            out.pop(-2)
+        return out
+
+    up, down = grow(fe[-1], hi, 1), grow(fe[0], lo, -1)
+    return np.array(down[::-1][:-1] + edges + up[1:])
+
+
+def half_wave_dipole_exact(d_fine=0.0075, d_max=0.0075, ratio=1.0, coefficients="host"):
+    """The strip dipole of rfnetwork's dipole test (tests/test_solver/test_dipole.py:14-92) with its
+    exact dimensions (inches: strip 0.030 wide, gap 0.015, 0.546 end to end, solve box 1.5 x 1.1 x
+    1.1) on a grid whose lines pass through the strip edges, the gap and the arm ends: cells of at
+    most d_fine along the dipole (equal within each span), growing by `ratio` up to d_max towards the
+    box, as the reference's mesher grades between its d_min and d_max (the default: d_fine
+    everywhere).  PML of 5 cells on every side, a far-field box at 10 and 20 GHz, edge corrections
+    along the strip edges, the reference's Gaussian excitation."""
+    ms_w, gap, dl = 0.030, 0.015, 0.546
+    xs = graded_edges([-d_fine, 0.0, d_fine], -0.75, 0.75, d_fine, d_max, ratio)
+    ys = graded_edges([-ms_w / 2, ms_w / 2], -0.55, 0.55, d_fine, d_max, ratio)
+    zs = graded_edges([-dl / 2, -gap / 2, gap / 2, dl / 2], -0.55, 0.55, d_fine, d_max, ratio)
+    s = FDTD_Solver(Box((xs[0], ys[0], zs[0]), (xs[-1], ys[-1], zs[-1])))
+    s.set_grid(xs, ys, zs)
+    ms_y = (-ms_w / 2, ms_w / 2)
+    z1, z2 = (-dl / 2, -gap / 2), (gap / 2, dl / 2)
+    s.add_conductor(Rectangle([(0, ms_y[0], z1[0]), (0, ms_y[1], z1[0]), (0, ms_y[1], z1[1])]))
+    s.add_conductor(Rectangle([(0, ms_y[0], z2[0]), (0, ms_y[1], z2[0]), (0, ms_y[1], z2[1])]))
+    s.add_lumped_port(1, Rectangle([(0, ms_y[0], gap / 2), (0, ms_y[1], gap / 2), (0, ms_y[1], -gap / 2)]), "z-")
+    s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", "z-", n_pml=5)
+    s.generate_mesh(coefficients=coefficients)
+    s.add_farfield_monitor(frequency=[10e9, 20e9])
+    for i, mz in enumerate((z1, z2)):
+        s.edge_correction((0, ms_y[0], mz[0]), (0, ms_y[0], mz[1]), integration_line="y-")
+        s.edge_correction((0, ms_y[1], mz[0]), (0, ms_y[1], mz[1]), integration_line="y+")
+        s.edge_correction((0, ms_y[0], mz[i]), (0, ms_y[1], mz[i]), integration_line=("z-" if i == 0 else "z+"))
+    s.assign_excitation(s.gaussian_source(width=50e-12, t0=40e-12, t_len=600e-12), 1)
+    return s
+
+
 def half_wave_dipole(d=0.0075, coefficients="host"):
     """The strip dipole of rfnetwork's dipole test on a uniform grid of d inches (the strip width, gap
     and arm length snapped to it: 4, 2 and 37 cells)."""

@@ -54,16 +54,19 @@ def test_nf2ff_kernel_matches_direct_sum(hip_ready):
     assert err < 1e-4
 
 
-def test_half_wave_dipole_known_answers(hip_ready):
-    """rfnetwork's own dipole test (tests/test_solver/test_dipole.py:100-125) on the engine: realised
-    theta-polarised gain at theta = 90 deg within 0.35 dB of 2.1 dB and below 2.15 dB for every phi,
-    phi-polarised gain below -40 dB, nulls below -90 dB at the poles, S11 below -10 dB.  The input
-    resistance is checked to 10 ohm of 73 ohm rather than the reference's 3.5 ohm: the strip width,
-    gap and arm length are snapped to a uniform grid here (the arm is 1.6% longer than the
-    reference's), which the reference's graded mesher does not do (measured on the float32 oracle:
-    78.9 + 14.1j ohm, S11 -12.1 dB, gain 1.84-1.95 dB)."""
-    print("[case] half-wave dipole", flush=True)
-    s = workloads.half_wave_dipole()
+@pytest.mark.parametrize("mesh", [(0.0075, 0.0075, 1.0), (0.01, 0.03, 1.2)])
+def test_half_wave_dipole_known_answers(hip_ready, mesh):
+    """rfnetwork's own dipole test (tests/test_solver/test_dipole.py:100-125) on the engine, with the
+    reference's dimensions on a grid through the strip edges, gap and arm ends (0.0075 inch cells, and
+    the reference mesher's d_min 0.01 / d_max 0.03 graded by 1.2 per cell): realised theta-polarised
+    gain at theta = 90 deg within 0.35 dB of 2.1 dB and below 2.15 dB for every phi, phi-polarised gain
+    below -40 dB, nulls below -90 dB at the poles, S11 below -10 dB and the input resistance within
+    3.5 ohm of 73 ohm: the reference's bounds.  Measured: 74.65 + 5.42j ohm, S11 -13.9 dB, gain
+    1.876-2.082 dB, phi-pol -52 dB on the first grid; 75.96 + 8.82j ohm, gain 1.763-2.133 dB, phi-pol
+    -40.5 dB on the graded one (finer gradations put the gain up to 2.24 dB at some phi: the E and H
+    averages to the face cell centres are unweighted, as in the reference)."""
+    print(f"[case] half-wave dipole {mesh}", flush=True)
+    s = workloads.half_wave_dipole_exact(*mesh)
     s.solve(gpu=True, show_progress=False)
     gain = np.asarray(s.get_farfield_gain(theta=np.arange(0, 190, 10), phi=np.arange(-180, 184, 4)))
     gdb = 10 * np.log10(gain)
@@ -76,4 +79,5 @@ def test_half_wave_dipole_known_answers(hip_ready):
     s11 = np.asarray(s.get_sparameters([10e9]))[:, 0, 0]
     assert np.all(20 * np.log10(np.abs(s11)) < -10)
     z = 50 * (1 + s11) / (1 - s11)
-    assert np.all(np.abs(73 - z.real) < 10)
+    print(f"[farfield] S11 {20 * np.log10(np.abs(s11))} dB, Z {z} ohm", flush=True)
+    assert np.all(np.abs(73 - z.real) < 3.5)
