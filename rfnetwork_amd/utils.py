@@ -38,3 +38,18 @@ def dtft(xn: np.ndarray, frequency: np.ndarray, fs: float, downsample: bool = Fa
     if use_ds:
         Xw *= factor
     return Xw
+
+
+def coupled_sline_impedance(w: float, s: float, b: float, er: float, t: float = 0):
+    """Odd- and even-mode impedance (ohm) of edge-coupled stripline: strips of width w and thickness t,
+    gap s, ground planes b apart, relative permittivity er (any one length unit).  The closed form of
+    Pramanick and Bhartia, Modern RF and Microwave Filter Design, sec. 3.7.1, with the fringing factor
+    Cf of a strip of thickness t; the same helper rfnetwork's coupled-line tests check against
+    (rfnetwork/utils.py:199).  Returns (Z0_odd, Z0_even)."""
+    ln2 = np.log(2)
+    a_even = (ln2 + np.log(1 + np.tanh(np.pi * s / (2 * b)))) / (2 * np.pi * ln2)
+    a_odd = (ln2 + np.log(1 + 1 / np.tanh(np.pi * s / (2 * b)))) / (2 * np.pi * ln2)
+    cf = 2 * np.log((2 * b - t) / (b - t)) - (t / b) * (t * (2 * b - t) / (b - t) ** 2)
+    z_even = 30 * np.pi * (b - t) / (np.sqrt(er) * (w + a_even * b * cf))
+    z_odd = 30 * np.pi * (b - t) / (np.sqrt(er) * (w + a_odd * b * cf))
+    return z_odd, z_even

@@ -315,6 +315,40 @@ def half_wave_dipole_exact(d_fine=0.0075, d_max=0.0075, ratio=1.0, coefficients=
     return s
 
 
+def coupled_stripline(sl_w, sl_sp, d=0.0025, coefficients="host"):
+    """The edge-coupled stripline of rfnetwork's coupled-line test (tests/test_solver/
+    test_coupled_stripline.py:22-96), inches: strips sl_w wide, sl_sp apart, in a 0.06 thick
+    substrate of er 3.66 between PEC walls (the solve box, 0.25 x 0.4 x 0.06), running from 0.1 past
+    the x- wall into a 5-cell PML at x+, each fed by a lumped port across the substrate whose
+    integration line runs from the lower wall to its strip; edge corrections on the first strip's
+    edges applied twice, as that test applies them; uniform d cells, whose lines pass through every
+    strip edge for the test's three geometries.  Returns the model and the test's 10 GHz
+    Gaussian-modulated voltage (1e-2 amplitude)."""
+    b, er = 0.06, 3.66
+    sbox_w, sbox_len = 0.4, 0.25
+    y1, y2 = -(sl_w / 2) - (sl_sp / 2), (sl_w / 2) + (sl_sp / 2)
+    ms_x = (-sbox_len / 2 + 0.1, sbox_len / 2)
+    xs = -sbox_len / 2 + np.arange(int(round(sbox_len / d)) + 1) * d
+    ys = -sbox_w / 2 + np.arange(int(round(sbox_w / d)) + 1) * d
+    zs = -b / 2 + np.arange(int(round(b / d)) + 1) * d
+    s = FDTD_Solver(Box((xs[0], ys[0], zs[0]), (xs[-1], ys[-1], zs[-1])))
+    s.set_grid(xs, ys, zs)
+    s.add_dielectric(Box((xs[0], ys[0], zs[0]), (xs[-1], ys[-1], zs[-1])), er=er)
+    for i, y in enumerate((y1, y2)):
+        s.add_conductor(Rectangle([(ms_x[0], y - sl_w / 2, 0), (ms_x[0], y + sl_w / 2, 0),
+                                   (ms_x[1], y + sl_w / 2, 0)]))
+        s.add_lumped_port(i + 1, Rectangle([(ms_x[0], y - sl_w / 2, -b / 2), (ms_x[0], y + sl_w / 2, -b / 2),
+                                             (ms_x[0], y + sl_w / 2, b / 2)]),
+                          Line((ms_x[0], y, -b / 2), (ms_x[0], y, 0)))
+    s.assign_PML_boundaries("x+", n_pml=5)
+    s.generate_mesh(coefficients=coefficients)
+    for y in (y1, y1):
+        s.edge_correction((ms_x[0], y + sl_w / 2, 0), (ms_x[1], y + sl_w / 2, 0), integration_line="y+")
+        s.edge_correction((ms_x[0], y - sl_w / 2, 0), (ms_x[1], y - sl_w / 2, 0), integration_line="y-")
+    vsrc = 1e-2 * np.asarray(s.gaussian_modulated_source(f0=10e9, width=200e-12, t0=100e-12, t_len=500e-12))
+    return s, vsrc
+
+
 def half_wave_dipole(d=0.0075, coefficients="host"):
     """The strip dipole of rfnetwork's dipole test on a uniform grid of d inches (the strip width, gap
     and arm length snapped to it: 4, 2 and 37 cells)."""
