@@ -373,3 +373,48 @@ def half_wave_dipole(d=0.0075, coefficients="host"):
         s.edge_correction((0, ms_y[0], mz[i]), (0, ms_y[1], mz[i]), integration_line=("z-" if i == 0 else "z+"))
     s.assign_excitation(s.gaussian_source(width=50e-12, t0=40e-12, t_len=600e-12), 1)
     return s
+
+
+def microstrip_line(len_axis=0, width_axis=1, normal_axis=2, d_max=0.02, coefficients="host"):
+    """The microstrip of rfnetwork's microstrip test (tests/test_solver/test_microstrip.py:15-112),
+    inches, with its length, width and substrate normal along any permutation of the axes: a strip
+    0.04 wide on a substrate 0.02 thick of er 3.66, in a 1.3 x 0.6 x 0.5 solve box; the strip runs from
+    0.5 before the middle into a 10-cell PML, fed there by a lumped port across the substrate; edge
+    corrections on both strip edges; a voltage probe from the strip to the ground and an ampere loop
+    around the strip halfway along it; a monitor of the normal E component on the substrate top; the
+    mesher's cells at most d_max.  Returns the model and the test's Gaussian voltage (1e-2 amplitude,
+    80 ps wide, 300 ps long)."""
+    if sorted((len_axis, width_axis, normal_axis)) != [0, 1, 2]:
+        raise ValueError("the three axes must be a permutation of 0, 1, 2")
+    ms_w, ms_len, sub_h = 0.04, 1.0, 0.02
+    sbox_h, sbox_w, sbox_len = 0.5, 0.6, 1.3
+    ends = (-ms_len / 2, sbox_len / 2)
+
+    def dims(length, width, height):
+        out = [0.0] * 3
+        out[len_axis], out[width_axis], out[normal_axis] = length, width, height
+        return out
+
+    def cube(center, size):
+        return Box(np.subtract(center, np.divide(size, 2)), np.add(center, np.divide(size, 2)))
+
+    na = "xyz"[normal_axis]
+    s = FDTD_Solver(cube(dims(0, 0, sbox_h / 2), dims(sbox_len, sbox_w, sbox_h)))
+    s.add_dielectric(cube(dims(0, 0, sub_h / 2), dims(sbox_len, sbox_w, sub_h)), er=3.66)
+    s.add_conductor(Rectangle([dims(ends[0], -ms_w / 2, sub_h), dims(ends[0], ms_w / 2, sub_h),
+                               dims(ends[1], ms_w / 2, sub_h)]))
+    s.add_lumped_port(1, Rectangle([dims(ends[0], -ms_w / 2, sub_h), dims(ends[0], ms_w / 2, sub_h),
+                                    dims(ends[0], ms_w / 2, 0)]), integration_line=f"{na}+")
+    s.assign_PML_boundaries(f"{'xyz'[len_axis]}+", n_pml=10)
+    s.generate_mesh(d_max=d_max, coefficients=coefficients)
+    wa = "xyz"[width_axis]
+    for sgn in (1, -1):
+        s.edge_correction(dims(ends[0], sgn * ms_w / 2, sub_h), dims(ends[1], sgn * ms_w / 2, sub_h),
+                          f"{wa}{'+' if sgn > 0 else '-'}")
+    s.add_field_monitor("mon1", f"e{na}", na, sub_h, 5)
+    s.add_current_probe("c1", Rectangle([dims(0, -ms_w / 2 - 0.001, sub_h + 0.001),
+                                         dims(0, ms_w / 2 + 0.001, sub_h + 0.001),
+                                         dims(0, ms_w / 2 + 0.001, sub_h - 0.001)]))
+    s.add_voltage_probe("v1", Line(dims(0, 0, sub_h), dims(0, 0, 0)))
+    vsrc = 1e-2 * np.asarray(s.gaussian_source(width=80e-12, t0=80e-12, t_len=300e-12))
+    return s, vsrc

@@ -5,11 +5,20 @@ impedances from S11 at 10 GHz must lie within 2 ohm of the closed form (Pramanic
 the test's reference helper computes it).  On the float32 oracle the same models give 47.23 / 78.83,
 34.61 / 64.31 and 34.86 / 50.74 ohm against 46.46 / 78.63, 34.97 / 64.86 and 35.35 / 51.32.  The two
 excitations of one geometry also run as two instances of one batch, equal to the sequential solves
-bit for bit."""
+bit for bit.
+
+rfnetwork's microstrip test (tests/test_solver/test_microstrip.py:15-132) on its own model and mesh
+along every axis permutation: the line impedance V/I halfway along the strip and the input impedance
+from S11 over 8-12 GHz within its 1.5 ohm of the dispersive closed form (52.72 ohm at 10 GHz), with
+the probe and port series equal to the oracle's bit for bit."""
+import copy
+from itertools import permutations
+
 import numpy as np
 import pytest
 
-from rfnetwork_amd import workloads
+from oracle import cpu_solver
+from rfnetwork_amd import utils, workloads
 from rfnetwork_amd.batch import Batch
 from rfnetwork_amd.utils import coupled_sline_impedance
 
@@ -75,3 +84,27 @@ def test_coupled_stripline_modes_batched(hip_ready):
         for p, v in zip([p for p in model.ports if p is not None], seq):
             np.testing.assert_array_equal(np.array(p["values"]), v)
         np.testing.assert_array_equal(np.asarray(model.get_sparameters([F0])), S_seq)
+
+
+@pytest.mark.parametrize("perm", list(permutations([0, 1, 2], 3)))
+def test_microstrip_reference_model_impedance(hip_ready, perm):
+    print(f"[case] microstrip axes={perm}", flush=True)
+    model, vsrc = workloads.microstrip_line(*perm)
+    model.assign_excitation(vsrc, 1)
+    cpu = copy.deepcopy(model)
+    model.solve(gpu=True, show_progress=False)
+    cpu.solve(gpu=False, backend=cpu_solver.solver_run, n_threads=8, show_progress=False)
+    for name in ("v1", "c1"):
+        np.testing.assert_array_equal(model.vi_probe_values(name), cpu.vi_probe_values(name), err_msg=name)
+    np.testing.assert_array_equal(np.array(model.ports[0]["values"]), np.array(cpu.ports[0]["values"]))
+    f = np.arange(8e9, 12e9, 10e6)
+    z_ref = utils.ustrip_impedance(0.04, 0.02, 3.66, 10e9)[0]
+    V = utils.dtft(model.vi_probe_values("v1"), f, 1 / model.dt)
+    I = utils.dtft(model.vi_probe_values("c1"), f, 1 / model.dt)
+    zp = (V / I).real
+    s11 = np.asarray(model.get_sparameters(f))[:, 0, 0]
+    zs = (50 * (1 + s11) / (1 - s11)).real
+    print(f"[physics] V/I {zp.min():.2f}..{zp.max():.2f}  S11 {zs.min():.2f}..{zs.max():.2f}  "
+          f"ref {z_ref:.2f}", flush=True)
+    np.testing.assert_array_less(np.abs(zp - z_ref), 1.5)
+    np.testing.assert_array_less(np.abs(zs - z_ref), 1.5)

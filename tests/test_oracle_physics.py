@@ -10,12 +10,16 @@
   claim of the reference's tests/test_solver/test_microstrip.py:18-19); a transposed index or a
   wrong component shift on one axis fails it.
 * microstrip impedance against the Hammerstad-Jensen closed form, the known answer of
-  tests/test_solver/test_microstrip.py:131-132 (looser here: 4 cells across the substrate).
+  tests/test_solver/test_microstrip.py:131-132: on a hand-made grid of 4 cells across the substrate
+  (3 ohm against the static formula), and on that test's own model and mesh (one 0.02 inch cell
+  across the substrate, two across the strip) along every axis permutation, where V/I and the
+  impedance from S11 must lie within its 1.5 ohm of the dispersive closed form over 8-12 GHz.
 """
 import copy
 from itertools import permutations
 
 import numpy as np
+import pytest
 
 from oracle import cpu_solver
 from rfnetwork_amd import FDTD_Solver, utils, workloads
@@ -182,3 +186,20 @@ def test_microstrip_impedance_known_answer():
     z = (V / I).real
     z_ref = _hammerstad_z0(w, h, 3.66)
     assert np.all(np.abs(z - z_ref) < 3.0), (z, z_ref)
+
+
+@pytest.mark.parametrize("perm", list(permutations([0, 1, 2], 3)))
+def test_microstrip_reference_model_known_answer(perm):
+    """rfnetwork's microstrip test on its own geometry and mesh (d_max 0.02 inch): the line impedance
+    V/I halfway along the strip and the input impedance from S11, 8-12 GHz in 10 MHz steps, within
+    1.5 ohm of the closed form at 10 GHz (52.72 ohm; the oracle gives 52.69-53.56 and 51.91-52.14)."""
+    s, vsrc = workloads.microstrip_line(*perm)
+    s.assign_excitation(vsrc, 1)
+    _solve_cpu(s, 8)
+    f = np.arange(8e9, 12e9, 10e6)
+    z_ref = utils.ustrip_impedance(0.04, 0.02, 3.66, 10e9)[0]
+    zp = (utils.dtft(s.vi_probe_values("v1"), f, 1 / s.dt) / utils.dtft(s.vi_probe_values("c1"), f, 1 / s.dt)).real
+    s11 = np.asarray(s.get_sparameters(f))[:, 0, 0]
+    zs = (50 * (1 + s11) / (1 - s11)).real
+    np.testing.assert_array_less(np.abs(zp - z_ref), 1.5)
+    np.testing.assert_array_less(np.abs(zs - z_ref), 1.5)
