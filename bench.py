@@ -15,8 +15,10 @@ Further legs in the same line: "c1" -- the 120x48x32 microstrip (4000 steps) on 
 full, on the CPU oracle; "batched" -- 4 uneven multi-port grids x every port driven (12
 excitations, 5000 steps) in one batched session vs the same excitations one after another;
 "adjoint" -- gradients/s of dJ/deps_r over a 64x64x32 design box of a 200x160x80 model (6000 steps,
-checkpoints every 200).  On request: "c5" (8 design variants up to 512x512x256, batched forward
-and adjoint) and "adjoint_fd" (that gradient against central differences on 32 cells).
+checkpoints every 200); "barrier" -- a device-wide barrier inside one cooperative launch against
+the spacing of graph-replayed kernel boundaries (what a persistent launch would pay per step).  On
+request: "c5" (8 design variants up to 512x512x256, batched forward and adjoint) and "adjoint_fd"
+(that gradient against central differences on 32 cells).
 """
 import argparse
 import json
@@ -552,6 +554,26 @@ def workloads_dipole(n_steps):
     return model
 
 
+def barrier_leg(iters=2000):
+    """The cost of keeping time steps inside one persistent launch: workgroups of 256 threads (256,
+    512, 1024) meeting at a device-wide barrier `iters` times in a cooperative launch, with no fences
+    (the synchronisation alone, which does not make stores visible across the XCDs' L2 caches), with
+    one wave per workgroup writing back and invalidating its L2 (the least that carries a step's fields
+    to the next on other XCDs), and with every wave doing so; beside it, the spacing of dependent
+    graph-replayed launches of an empty kernel of the same grid (a kernel boundary, which makes the
+    stores visible by itself)."""
+    from rfnetwork_amd import _lib
+
+    rows = []
+    for wg in (256, 512, 1024):
+        for fences in (0, 1, 2):
+            b, l = _lib.grid_barrier_cost(wg, fences, iters)
+            rows.append(dict(workgroups=wg, fences=fences, barrier_us=round(b, 3), launch_us=round(l, 3)))
+            log(f"barrier: {wg} workgroups, fences {fences}: {b:.3f} us per barrier, {l:.3f} us per graph launch")
+    return dict(iters=iters, fences_meaning={"0": "none (no cross-XCD visibility)", "1": "one wave per workgroup",
+                                             "2": "every wave"}, rows=rows)
+
+
 def adjoint_fd_leg(n_steps, every, n_cells=32, seed=41, h=1e-3, oracle_cells=1):
     """Validation (not a throughput figure): the adjoint dJ/deps_r of the 200x160x80 design model in
     float64 and float32 against central differences (Delta eps_r = h) of float64 forward solves on
@@ -625,10 +647,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU oracle work per thread count (0 = skip)")
     ap.add_argument("--no-copy-probe", action="store_true")
-    ap.add_argument("--legs", default="plain,raw,c1,batched,adjoint",
+    ap.add_argument("--legs", default="plain,raw,c1,batched,adjoint,barrier",
                     help="comma list: plain (the headline), raw (the same workload with the 30 raw coefficient "
                          "arrays: the reference formulation's bytes), c1 (the 120x48x32 microstrip, GPU and CPU "
-                         "oracle), batched (12 excitations), adjoint (gradients/s), "
+                         "oracle), batched (12 excitations), adjoint (gradients/s), barrier (persistent-launch "
+                         "barrier against graph-replayed launches), "
                          "c5 (8 design variants, batched forward+adjoint, minutes), "
                          "adjoint_fd (finite-difference validation, minutes), "
                          "monitors (streamed time-domain monitors against resident ones), "
@@ -680,6 +703,8 @@ def main():
         out["monitors"] = guarded("monitors", monitors_leg)
     if "dft64" in legs:
         out["dft64"] = guarded("dft64", dft64_leg)
+    if "barrier" in legs:
+        out["barrier"] = guarded("barrier", barrier_leg)
     print(json.dumps(out), flush=True)
 
 

@@ -378,6 +378,17 @@ int rfdtd_read_coefficient_box(rfdtd_session* s, int32_t member, const int32_t b
  * ceiling, and the known-size kernel that calibrates the profiler's byte counters. */
 int rfdtd_copy_bandwidth(uint64_t bytes, int32_t iters, double* gbytes_per_s, double* ms_per_copy);
 
+/* Barrier probe (no counterpart in the reference; the measurement behind advancing time steps in
+ * graph-replayed launches rather than in one persistent launch): `workgroups` workgroups of 256
+ * threads in one cooperative launch meet `iters` times at a device-wide barrier; fences 0: none (the
+ * synchronisation alone), 1: one wave per workgroup writes back and invalidates its XCD's L2 (what
+ * makes one step's stores visible to the next across XCDs), 2: every wave does.  Reports the best of
+ * three runs in microseconds per barrier, and the spacing of dependent graph-replayed launches of an
+ * empty kernel of the same grid.  Every wait is bounded (one second of wall clock); a run that
+ * exceeds it returns RFDTD_EDEVICE.  RFDTD_EINVAL: more workgroups than can be resident at once. */
+int rfdtd_grid_barrier_cost(int32_t workgroups, int32_t fences, int32_t iters, double* us_per_barrier,
+                            double* us_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,7 +57,7 @@ EXPORTS = (
     "rfdtd_set_adjoint_launch", "rfdtd_get_adjoint_launch", "rfdtd_set_design_permittivity",
     "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
     "rfdtd_create_assembled", "rfdtd_read_coefficients", "rfdtd_create_batch_assembled",
-    "rfdtd_read_coefficient_box", "rfdtd_nf2ff_static", "rfdtd_design_prepare",
+    "rfdtd_read_coefficient_box", "rfdtd_nf2ff_static", "rfdtd_design_prepare", "rfdtd_grid_barrier_cost",
     "rfdtd_set_design_permittivity_device",
 )
 
@@ -179,6 +179,8 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(ctypes.c_uint64)]
     L.rfdtd_copy_bandwidth.argtypes = [ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]
+    L.rfdtd_grid_barrier_cost.argtypes = [i32, i32, i32, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
     L.rfdtd_create_assembled.argtypes = [ctypes.POINTER(vp), vp, i32, u32]
     L.rfdtd_read_coefficients.argtypes = [vp, i32, vp]
     L.rfdtd_create_batch_assembled.argtypes = [ctypes.POINTER(vp), i32, vp, vp, u32]
@@ -263,6 +265,16 @@ def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20):
     check(lib().rfdtd_copy_bandwidth(int(nbytes), int(iters), ctypes.byref(g), ctypes.byref(ms)),
           "rfdtd_copy_bandwidth")
     return g.value, ms.value
+
+
+def grid_barrier_cost(workgroups: int = 256, fences: int = 1, iters: int = 2000):
+    """(us per device-wide barrier in one cooperative launch, us between dependent graph-replayed
+    launches of an empty kernel) for `workgroups` workgroups of 256 threads; fences 0: none, 1: one
+    wave per workgroup writes back and invalidates its L2 around the barrier, 2: every wave."""
+    b, l = ctypes.c_double(), ctypes.c_double()
+    check(lib().rfdtd_grid_barrier_cost(int(workgroups), int(fences), int(iters), ctypes.byref(b), ctypes.byref(l)),
+          "rfdtd_grid_barrier_cost")
+    return b.value, l.value
 
 
 def gpu_visible() -> bool:

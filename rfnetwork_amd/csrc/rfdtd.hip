@@ -3068,6 +3068,61 @@ __global__ void __launch_bounds__(256) k_stream_copy(float4* __restrict__ dst, c
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Device-wide barrier probe: what one time step would pay to stay inside a single persistent launch
+// instead of ending at a kernel boundary.  The workgroups of a cooperative launch meet `iters` times
+// at a flag-gather barrier: each stores its epoch in a 128-byte slot of its own, workgroup 0 waits for
+// every slot and publishes the epoch, the others wait for that word.  FENCES 0: no fences (the
+// synchronisation alone; stores made before it are then not guaranteed visible across the XCDs'
+// L2 caches); 1: one wave per workgroup makes its XCD's L2 write back before arriving and
+// invalidates it after leaving (agent-scope release and acquire); 2: every wave does.  Each wait is
+// bounded by `limit` wall-clock ticks; one that runs out sets the abort word, and every wait also
+// ends when the abort word is set, so all workgroups leave.
+// words: [0] published epoch, [32] abort, [64 + 32 g] workgroup g's epoch.
+template <int FENCES>
+__global__ void __launch_bounds__(256) k_barrier_probe(uint32_t* words, int iters, long long limit) {
+    __shared__ int s_abort;
+    uint32_t* const pub = words;
+    uint32_t* const abort_word = words + 32;
+    const uint32_t G = gridDim.x;
+    auto wait_ge = [&](const uint32_t* w, uint32_t target) {
+        const long long t0 = wall_clock64();
+        while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+            if (wall_clock64() - t0 > limit) {
+                __hip_atomic_store(abort_word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        return true;
+    };
+    if (threadIdx.x == 0) s_abort = 0;
+    for (int k = 1; k <= iters; ++k) {
+        const uint32_t epoch = (uint32_t)k;
+        if (FENCES == 2) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (FENCES == 1 && threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (threadIdx.x == 0)
+            __hip_atomic_store(words + 64 + 32 * blockIdx.x, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool ok = true;
+        if (blockIdx.x == 0) {
+            for (uint32_t g = threadIdx.x; g < G && ok; g += blockDim.x) ok = wait_ge(words + 64 + 32 * g, epoch);
+            if (!ok) s_abort = 1;
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(pub, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (threadIdx.x == 0) {
+            if (!wait_ge(pub, epoch)) s_abort = 1;
+        }
+        if (FENCES == 1 && threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+        if (FENCES == 2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (s_abort) return;  // uniform over the workgroup (read after the barrier)
+    }
+}
+
+// An empty kernel: graph-replayed launches of it give the spacing of dependent kernel boundaries.
+__global__ void __launch_bounds__(256) k_boundary_probe() {}
+
 // ------------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------------
@@ -6087,6 +6142,75 @@ int rfdtd_copy_bandwidth(uint64_t bytes, int32_t iters, double* gbytes_per_s, do
     if (e1) (void)hipEventDestroy(e1);
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
+    return rc;
+}
+
+int rfdtd_grid_barrier_cost(int32_t workgroups, int32_t fences, int32_t iters, double* us_per_barrier,
+                            double* us_per_launch) {
+    uint32_t* words = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t cs = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    const int rc = guarded([&] {
+        if (workgroups < 1 || fences < 0 || fences > 2 || iters < 1 || iters > 100000)
+            RF_THROW(RFDTD_EINVAL, "workgroups >= 1, fences 0..2 and iters 1..100000 required");
+        int dev = 0, ncu = 0, per_cu = 0, khz = 0, coop = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        HIP_OK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+        const void* kern = fences == 0 ? (const void*)k_barrier_probe<0>
+                         : fences == 1 ? (const void*)k_barrier_probe<1> : (const void*)k_barrier_probe<2>;
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+        if (!coop) RF_THROW(RFDTD_EDEVICE, "the device does not support cooperative launches");
+        if ((int64_t)workgroups > (int64_t)per_cu * ncu)
+            RF_THROW(RFDTD_EINVAL, "more workgroups than can be resident at once (" + std::to_string(per_cu * ncu) + ")");
+        const size_t nwords = 64 + 32 * (size_t)workgroups;
+        HIP_OK(hipMalloc(&words, nwords * 4));
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        long long limit = (long long)khz * 1000LL;  // one second of wall clock per wait
+        int it = iters;
+        void* args[] = {&words, &it, &limit};
+        float best = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIP_OK(hipMemset(words, 0, nwords * 4));
+            HIP_OK(hipEventRecord(e0, nullptr));
+            HIP_OK(hipLaunchCooperativeKernel(kern, dim3(workgroups), dim3(256), args, 0, nullptr));
+            HIP_OK(hipEventRecord(e1, nullptr));
+            HIP_OK(hipEventSynchronize(e1));
+            uint32_t ab = 0;
+            HIP_OK(hipMemcpy(&ab, words + 32, 4, hipMemcpyDeviceToHost));
+            if (ab) RF_THROW(RFDTD_EDEVICE, "a barrier wait exceeded its time limit");
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep == 0 || ms < best) best = ms;
+        }
+        if (us_per_barrier) *us_per_barrier = 1000.0 * best / iters;
+        // the same number of dependent, graph-replayed launches of an empty kernel of the same grid
+        constexpr int kLaunches = 100;
+        HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < kLaunches; ++k) k_boundary_probe<<<workgroups, 256, 0, cs>>>();
+        HIP_OK(hipStreamEndCapture(cs, &g));
+        HIP_OK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        HIP_OK(hipGraphLaunch(ge, nullptr));
+        const int reps = std::max(1, iters / kLaunches);
+        HIP_OK(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps; ++r) HIP_OK(hipGraphLaunch(ge, nullptr));
+        HIP_OK(hipEventRecord(e1, nullptr));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        if (us_per_launch) *us_per_launch = 1000.0 * ms / (reps * kLaunches);
+    });
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (words) (void)hipFree(words);
     return rc;
 }
 
