@@ -418,3 +418,31 @@ def microstrip_line(len_axis=0, width_axis=1, normal_axis=2, d_max=0.02, coeffic
     s.add_voltage_probe("v1", Line(dims(0, 0, sub_h), dims(0, 0, 0)))
     vsrc = 1e-2 * np.asarray(s.gaussian_source(width=80e-12, t0=80e-12, t_len=300e-12))
     return s, vsrc
+
+
+def coupled_microstrip(ms_w, ms_sp, d_max=0.01, d_min=0.005, coefficients="host"):
+    """The edge-coupled microstrip of rfnetwork's coupled-microstrip test (tests/test_solver/
+    test_coupled_ustrip.py:22-96), inches: strips ms_w wide and ms_sp apart on a substrate 0.03 thick
+    of er 3.66 in a 0.3 x 0.3 x 0.25 solve box, from 0.1 past the x- wall into the PML at x+, each fed
+    by a lumped port across the substrate (integration line z+); PML of 5 cells on x-, x+, z+, y- and
+    y+; edge corrections along both edges of both strips; the mesher's cells at most d_max (and
+    d_min, which only bounds the graded cells of the reference's mesher).  Returns the model and the
+    test's Gaussian voltage (1e-2 amplitude, 80 ps wide, t0 50 ps, 300 ps long)."""
+    sub_h, er = 0.03, 3.66
+    sbox_h, sbox_w, sbox_len = 0.25, 0.3, 0.3
+    ys = (-(ms_w / 2) - (ms_sp / 2), (ms_w / 2) + (ms_sp / 2))
+    ms_x = (-sbox_len / 2 + 0.1, sbox_len / 2)
+    s = FDTD_Solver(Box((-sbox_len / 2, -sbox_w / 2, 0), (sbox_len / 2, sbox_w / 2, sbox_h)))
+    s.add_dielectric(Box((-sbox_len / 2, -sbox_w / 2, 0), (sbox_len / 2, sbox_w / 2, sub_h)), er=er)
+    for i, y in enumerate(ys):
+        s.add_conductor(Rectangle([(ms_x[0], y - ms_w / 2, sub_h), (ms_x[0], y + ms_w / 2, sub_h),
+                                   (ms_x[1], y + ms_w / 2, sub_h)]))
+        s.add_lumped_port(i + 1, Rectangle([(ms_x[0], y - ms_w / 2, sub_h), (ms_x[0], y + ms_w / 2, sub_h),
+                                            (ms_x[0], y + ms_w / 2, 0)]), integration_line="z+")
+    s.assign_PML_boundaries("x+", "x-", "z+", "y-", "y+", n_pml=5)
+    s.generate_mesh(d_max=d_max, d_min=d_min, coefficients=coefficients)
+    for y in ys:
+        s.edge_correction((ms_x[0], y + ms_w / 2, sub_h), (ms_x[1], y + ms_w / 2, sub_h), integration_line="y+")
+        s.edge_correction((ms_x[0], y - ms_w / 2, sub_h), (ms_x[1], y - ms_w / 2, sub_h), integration_line="y-")
+    vsrc = 1e-2 * np.asarray(s.gaussian_source(width=80e-12, t0=50e-12, t_len=300e-12))
+    return s, vsrc

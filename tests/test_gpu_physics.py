@@ -10,7 +10,13 @@ bit for bit.
 rfnetwork's microstrip test (tests/test_solver/test_microstrip.py:15-132) on its own model and mesh
 along every axis permutation: the line impedance V/I halfway along the strip and the input impedance
 from S11 over 8-12 GHz within its 1.5 ohm of the dispersive closed form (52.72 ohm at 10 GHz), with
-the probe and port series equal to the oracle's bit for bit."""
+the probe and port series equal to the oracle's bit for bit.
+
+rfnetwork's coupled-microstrip test (tests/test_solver/test_coupled_ustrip.py:14-127): even and odd
+solves of its three geometries on a uniform 0.0025 inch grid, within its 2 ohm of the published
+calculator values its parametrisation lists (wcalc; the converged engine gives 60.33 / 120.42,
+45.04 / 101.94 and 39.63 / 63.62 ohm against 60.34 / 120.39, 45.09 / 101.85 and 38.67 / 63.98); on
+a 0.005 inch grid the port series equal the oracle's bit for bit."""
 import copy
 from itertools import permutations
 
@@ -108,3 +114,42 @@ def test_microstrip_reference_model_impedance(hip_ready, perm):
           f"ref {z_ref:.2f}", flush=True)
     np.testing.assert_array_less(np.abs(zp - z_ref), 1.5)
     np.testing.assert_array_less(np.abs(zs - z_ref), 1.5)
+
+
+WCALC_COUPLED_MICROSTRIP = [(0.02, 0.01, 60.3447, 120.393), (0.03, 0.005, 45.0888, 101.847), (0.06, 0.01, 38.674, 63.9801)]
+
+
+def _modes(model, vsrc, solve):
+    z, series = [], []
+    for waves in (((vsrc, [1, 2]),), ((vsrc, 1), (-vsrc, 2))):
+        model.reset_excitations()
+        for w, p in waves:
+            model.assign_excitation(w, p)
+        solve(model)
+        s11 = np.asarray(model.get_sparameters([F0]))[0, 0, 0]
+        z.append((50 * (1 + s11) / (1 - s11)).real)
+        series.append([np.array(p["values"]) for p in model.ports if p is not None])
+    return z, series
+
+
+@pytest.mark.parametrize("ms_w,ms_sp,ref_odd,ref_even", WCALC_COUPLED_MICROSTRIP)
+def test_coupled_microstrip_even_odd_impedance(hip_ready, ms_w, ms_sp, ref_odd, ref_even):
+    print(f"[case] coupled microstrip w={ms_w} s={ms_sp}", flush=True)
+    model, vsrc = workloads.coupled_microstrip(ms_w, ms_sp, d_max=0.0025, d_min=0.00125)
+    (z_even, z_odd), _ = _modes(model, vsrc, lambda m: m.solve(gpu=True, show_progress=False))
+    print(f"[physics] odd {z_odd:.2f} (ref {ref_odd:.2f}) even {z_even:.2f} (ref {ref_even:.2f})", flush=True)
+    assert abs(z_odd - ref_odd) < 2
+    assert abs(z_even - ref_even) < 2
+
+
+def test_coupled_microstrip_matches_oracle(hip_ready):
+    print("[case] coupled microstrip against the oracle", flush=True)
+    model, vsrc = workloads.coupled_microstrip(0.03, 0.005, d_max=0.005, d_min=0.0025)
+    cpu = copy.deepcopy(model)
+    z_gpu, s_gpu = _modes(model, vsrc, lambda m: m.solve(gpu=True, show_progress=False))
+    z_cpu, s_cpu = _modes(cpu, vsrc, lambda m: m.solve(gpu=False, backend=cpu_solver.solver_run, n_threads=8,
+                                                       show_progress=False))
+    for a, b in zip(s_gpu, s_cpu):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    assert z_gpu == z_cpu

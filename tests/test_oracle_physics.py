@@ -14,6 +14,9 @@
   (3 ohm against the static formula), and on that test's own model and mesh (one 0.02 inch cell
   across the substrate, two across the strip) along every axis permutation, where V/I and the
   impedance from S11 must lie within its 1.5 ohm of the dispersive closed form over 8-12 GHz.
+* coupled microstrip even and odd modes (tests/test_solver/test_coupled_ustrip.py:14-127, the
+  even solve, reset_excitations, the odd solve) within that test's 2 ohm of the published
+  calculator values its parametrisation lists (wcalc), on a uniform 0.005 inch grid.
 """
 import copy
 from itertools import permutations
@@ -203,3 +206,29 @@ def test_microstrip_reference_model_known_answer(perm):
     zs = (50 * (1 + s11) / (1 - s11)).real
     np.testing.assert_array_less(np.abs(zp - z_ref), 1.5)
     np.testing.assert_array_less(np.abs(zs - z_ref), 1.5)
+
+
+WCALC_COUPLED_MICROSTRIP = [(0.02, 0.01, 60.3447, 120.393), (0.03, 0.005, 45.0888, 101.847), (0.06, 0.01, 38.674, 63.9801)]
+
+
+def coupled_microstrip_modes(model, vsrc, solve):
+    """Even mode (the same voltage at both ports), then after reset_excitations the odd mode
+    (opposite voltages): the input impedances from S11 at 10 GHz, (Z_even, Z_odd)."""
+    z = []
+    for waves in (((vsrc, [1, 2]),), ((vsrc, 1), (-vsrc, 2))):
+        model.reset_excitations()
+        for w, p in waves:
+            model.assign_excitation(w, p)
+        solve(model)
+        s11 = np.asarray(model.get_sparameters([10e9]))[0, 0, 0]
+        z.append((50 * (1 + s11) / (1 - s11)).real)
+    return z[0], z[1]
+
+
+@pytest.mark.parametrize("ms_w,ms_sp,ref_odd,ref_even", WCALC_COUPLED_MICROSTRIP)
+def test_coupled_microstrip_known_answer(ms_w, ms_sp, ref_odd, ref_even):
+    """Observed on the oracle: 59.52 / 120.40, 45.31 / 102.24 and 39.35 / 63.91 ohm (odd / even)."""
+    model, vsrc = workloads.coupled_microstrip(ms_w, ms_sp, d_max=0.005, d_min=0.0025)
+    z_even, z_odd = coupled_microstrip_modes(model, vsrc, lambda m: _solve_cpu(m, 8))
+    assert abs(z_odd - ref_odd) < 2, (z_odd, ref_odd)
+    assert abs(z_even - ref_even) < 2, (z_even, ref_even)
