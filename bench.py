@@ -359,21 +359,35 @@ def batched_leg(n_steps, flags=0):
         sb.run(sNt)
         sb.synchronize()
         small_b = time.perf_counter() - t0
-    singles = []
-    for port in [1, 2] * 6:
-        (_, probes, monitors), = rb.pack_excitations(sm, [port])[0]
-        ss = Session(scoef, sm.Nx, sm.Ny, sm.Nz, flags=flags)
-        ss.set_sources(probes, monitors, sNt)
-        ss.autotune()
-        singles.append(ss)
-    t0 = time.perf_counter()
-    for ss in singles:
-        ss.run(sNt)
-        ss.synchronize()
-    small_s = time.perf_counter() - t0
-    for ss in singles:
-        ss.close()
-    log(f"small grids: batched {small_b * 1e3:.1f} ms, sequential {small_s * 1e3:.1f} ms")
+        sb.read()
+    small_series = [[np.array(p["values"]) for p in probes] for _, probes, _ in sexc]
+
+    def small_sequential(extra_flags, tune):
+        runs = []
+        for port in [1, 2] * 6:
+            (_, probes, monitors), = rb.pack_excitations(sm, [port])[0]
+            ss = Session(scoef, sm.Nx, sm.Ny, sm.Nz, flags=flags | extra_flags)
+            ss.set_sources(probes, monitors, sNt)
+            if tune:
+                ss.autotune()
+            runs.append((ss, probes))
+        t0 = time.perf_counter()
+        for ss, _ in runs:
+            ss.run(sNt)
+            ss.synchronize()
+        wall = time.perf_counter() - t0
+        same = True
+        for (ss, probes), ref in zip(runs, small_series):
+            ss.read()
+            same &= all(np.array_equal(p["values"], r) for p, r in zip(probes, ref))
+            ss.close()
+        return wall, same
+
+    small_s, small_same = small_sequential(0, True)
+    # the reference formulation of the plain kernel, as for the port variants above
+    small_p, small_same_p = small_sequential(_lib.COEF_RAW | _lib.NO_GRAPHS, False)
+    log(f"small grids: batched {small_b * 1e3:.1f} ms, sequential {small_s * 1e3:.1f} ms, sequential plain "
+        f"formulation {small_p * 1e3:.1f} ms, identical {small_same and small_same_p}")
     b_rate = exc_cells * n_steps / wall_b / 1e6
     s_rate = exc_cells * n_steps / wall_s / 1e6
     return dict(workload="4 port variants 64^3..192x128x96, 12 excitations", steps=n_steps,
@@ -385,7 +399,10 @@ def batched_leg(n_steps, flags=0):
                                 speedup=round(e2e_s / e2e_b, 3), identical=bool(same)),
                 small_grids=dict(workload="12 excitations of the 120x48x32 microstrip, 4000 steps",
                                  batched_ms=round(small_b * 1e3, 2), sequential_ms=round(small_s * 1e3, 2),
-                                 speedup=round(small_s / small_b, 3)))
+                                 speedup=round(small_s / small_b, 3),
+                                 sequential_plain_formulation_ms=round(small_p * 1e3, 2),
+                                 speedup_vs_plain_formulation=round(small_p / small_b, 3),
+                                 identical=bool(small_same and small_same_p)))
 
 
 def adjoint_leg(n_steps, every, repeats=2, flags=0):
