@@ -20,9 +20,15 @@
 //   * a session holds one or more grids ("members"), each advanced for one or more excitations
 //     ("instances" sharing the member's coefficients); one launch per phase covers every instance
 //     of every member, so a batch of uneven grids costs the launches of a single grid;
-//   * one thread owns a (y,z) column of a block of x planes and marches up in x; the x+1 (E update)
-//     and x-1 (H update) neighbours are carried in registers, the y/z neighbours come from the
-//     lanes of the same or the next wavefront (L1/L2 hits), so each array is read from HBM once;
+//   * separate E and H kernels: one thread owns a (y,z) column of a block of x planes and marches
+//     up in x; the x+1 (E update) and x-1 (H update) neighbours are carried in registers, the y/z
+//     neighbours come from the lanes of the same or the next wavefront (L1/L2 hits), so each array
+//     is read from HBM once; where Nz allows, a thread owns 2 or 4 consecutive z cells and moves
+//     every array as one 8- or 16-byte vector per lane;
+//   * the two-step pass (k_step2) advances a tile through steps n and n + 1 in one launch: a
+//     workgroup computes its tile plus a two-cell halo, exchanges y/z neighbours through LDS and
+//     marches x with the second step one plane behind the first, so a cell's fields cross HBM once
+//     per two steps (the halo recomputed, not re-read from another workgroup's results);
 //   * lanes run along z, the contiguous axis: every load and store is a coalesced 256-byte row;
 //   * workgroups are renumbered so that consecutive tiles land on the same XCD and share its L2:
 //     neighbouring (y,z) tiles share y-neighbour rows, and the instances of one member -- ordered
