@@ -153,3 +153,35 @@ def test_two_step_large_grid_parity(hip_ready, variant):
         np.testing.assert_array_equal(g[k], np.array(v["values"]), err_msg=k)
     for k in f_c:
         np.testing.assert_array_equal(f[k], f_c[k], err_msg=k)
+
+
+@pytest.fixture(scope="module")
+def slab_440_oracle():
+    """The oracle's probes, DFT planes and fields of the 400x400x200 workload after 440 steps,
+    computed once for the variants below."""
+    model = workloads.dipole_slab(Nt=440, monitors=True)
+    p_c, m_c, f_c = run_oracle(model, np.float32, n_threads=16)
+    model.unpack(p_c, m_c, gpu=False)
+    return dict(probes=[np.array(p["values"]) for p in p_c],
+                monitors={k: np.array(v["values"]) for k, v in model.monitors.items()}, fields=f_c)
+
+
+@pytest.mark.parametrize("variant", [7, 13])
+def test_two_step_large_grid_440_steps(hip_ready, slab_440_oracle, variant):
+    """The two variants the tuner picks for the 400x400x200 workload, over 440 steps (graph chunks;
+    both run lengths odd, so each ends on a single step), by which time the pulse has reached every
+    PML slab and the z-normal DFT planes: probes, the 6 running-DFT planes and every field equal
+    to the oracle's bit for bit."""
+    print(f"[case] two-step 400x400x200 variant={variant}, 440 steps", flush=True)
+    ref = slab_440_oracle
+    model = workloads.dipole_slab(Nt=440, monitors=True)
+    p, m, f = _run(model, np.float32, 0, variant, chunks=[263, 177])
+    for a, b in zip(p, ref["probes"]):
+        np.testing.assert_array_equal(a["values"], b)
+    model.unpack(p, m, gpu=True)
+    for k, v in model.monitors.items():
+        got = np.array(v["values"])
+        np.testing.assert_array_equal(got, ref["monitors"][k], err_msg=k)
+        assert np.abs(got).max() > 0, k
+    for k in ref["fields"]:
+        np.testing.assert_array_equal(f[k], ref["fields"][k], err_msg=k)
