@@ -274,14 +274,18 @@ def test_boundary_plane_monitors(hip_ready, variant):
     _assert_monitors_equal(model, p_g, m_g, p_c, m_c)
 
 
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_graded_grid_every_coefficient_kind(hip_ready, dtype):
-    print(f"[case] graded grid {np.dtype(dtype).name}", flush=True)
+@pytest.mark.parametrize("dtype,flags", [(np.float32, 0), (np.float32, _lib.COEF_RAW), (np.float64, 0)])
+def test_graded_grid_every_coefficient_kind(hip_ready, dtype, flags):
+    """A graded grid holding every coefficient kind, through the session's default kernels and every
+    fused, two-step and vector-column variant (Nz = 16): probes, monitors and fields bit-exact
+    against the oracle."""
+    print(f"[case] graded grid {np.dtype(dtype).name} flags={flags}", flush=True)
     model = _graded_everything(Nt=70, coef_dtype=dtype)
     p_c, m_c, f_c = run_oracle(model, dtype, n_threads=3)
-    p_g, m_g, f_g, _ = _hip_session(model, dtype, chunks=[41, 29])
-    for a, b in zip(p_g, p_c):
-        np.testing.assert_array_equal(a["values"], b["values"])
-    _assert_monitors_equal(model, p_g, m_g, p_c, m_c)
-    for k in f_c:
-        np.testing.assert_array_equal(f_g[k], f_c[k], err_msg=k)
+    for variant in (None, 2, 3, 5, 6, 7, 8, 9, 13, 14, 10, 11, 12):
+        p_g, m_g, f_g, _ = _hip_session(model, dtype, flags, chunks=[41, 29], variant=variant)
+        for a, b in zip(p_g, p_c):
+            np.testing.assert_array_equal(a["values"], b["values"], err_msg=f"variant {variant}")
+        _assert_monitors_equal(model, p_g, m_g, p_c, m_c)
+        for k in f_c:
+            np.testing.assert_array_equal(f_g[k], f_c[k], err_msg=f"{k} variant {variant}")
