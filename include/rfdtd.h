@@ -261,9 +261,10 @@ int rfdtd_set_design_permittivity(rfdtd_session* s, int32_t member, const int32_
  * (x0, x1, y0, y1, z0, z1, cells; inside the prepared box grown by one cell on each upper side) from
  * eps_r (device pointer, C order, float64 if f64 else float32): eps_r * e0 (in float64) replaces the
  * held permittivity there and the box's coefficients are recomputed from it, bit for bit as the host
- * update would, on `stream` (NULL: the session's stream), ordered after the session's earlier work
- * and before its later work; eps_r must stay valid until then.  Later updates start from the values
- * set last. */
+ * update would, on `stream` (NULL: the session's stream).  It is ordered after the work on the
+ * session's own stream and after the session's last rfdtd_run, whatever stream that was given; the
+ * session's own stream and its next rfdtd_run, on any stream, wait for it.  eps_r must stay valid
+ * until the update has run.  Later updates start from the values set last. */
 int rfdtd_design_prepare(rfdtd_session* s, int32_t member, const int32_t gbox[6], const double* eps_now,
                          const double* sigma, const double* dx, const double* dy, const double* dz, const void* dx1,
                          const void* dy1, const void* dz1, double dt, double e0);

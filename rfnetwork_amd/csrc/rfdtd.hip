@@ -3283,7 +3283,10 @@ struct Session {
     hipGraph_t graph_def[2] = {nullptr, nullptr};
     bool graph_dirty = true;
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // ev1: the end of the last run, on the stream it ran on
+    // a design update from device memory on some stream: the next run, on any stream, waits for it
+    hipEvent_t design_ev = nullptr;
+    bool design_pending = false;
     double last_total_ms = 0, last_e_ms = 0, last_h_ms = 0;
     static constexpr int kTimedSteps = 64;
     std::vector<hipEvent_t> kev;  // 4 per timed step: E start/end, H start/end
@@ -3360,6 +3363,7 @@ Session::~Session() {
         if (e) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (design_ev) (void)hipEventDestroy(design_ev);
     for (auto e : kev) (void)hipEventDestroy(e);
     if (own_stream) (void)hipStreamDestroy(own_stream);
 }
@@ -4447,6 +4451,10 @@ void run_steps(Session& s, int n_steps, hipStream_t st) {
         if (s.step + n_steps > h.capacity)
             RF_THROW(RFDTD_ESTATE, "run past the design-box history (" + std::to_string(h.capacity) + " steps)");
     refresh<T>(s);
+    if (s.design_pending) {
+        HIP_OK(hipStreamWaitEvent(st, s.design_ev, 0));
+        s.design_pending = false;
+    }
     HIP_OK(hipEventRecord(s.ev0, st));
     const bool timing = (s.flags & RFDTD_KERNEL_TIMING) != 0;
     const bool graphs = !timing && !(s.flags & RFDTD_NO_GRAPHS) && s.graph_steps > 1;
@@ -5822,13 +5830,20 @@ int rfdtd_set_design_permittivity_device(rfdtd_session* s, int32_t member, const
             HIP_OK(hipStreamWaitEvent(after, ev, 0));
             (void)hipEventDestroy(ev);
         };
-        if (st != S.own_stream) order(S.own_stream, st);  // the session's earlier steps read the old values
+        // the session's earlier steps read the old values: those on its own stream, and the last run,
+        // whatever stream it was given (ev1 marks its end; an event never recorded is no wait)
+        if (st != S.own_stream) order(S.own_stream, st);
+        HIP_OK(hipStreamWaitEvent(st, S.ev1, 0));
         const int64_t n = (int64_t)nx * ny * nz;
         k_design_put_eps<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(M.held.eps, D.ey1, D.ez1, bx, by, bz, nx, ny, nz,
                                                                      eps_r, f64 ? 1 : 0, D.e0);
         HIP_OK(hipGetLastError());
         design_write(S, M, M.held, st);
-        if (st != S.own_stream) order(st, S.own_stream);  // and the session's later steps the new ones
+        // and the session's later steps the new ones: its own stream's work, and the next run on any stream
+        if (st != S.own_stream) order(st, S.own_stream);
+        if (!S.design_ev) HIP_OK(hipEventCreateWithFlags(&S.design_ev, hipEventDisableTiming));
+        HIP_OK(hipEventRecord(S.design_ev, st));
+        S.design_pending = true;
     });
 }
 
