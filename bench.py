@@ -75,6 +75,9 @@ MEASURED_TRAFFIC.update({
     # the two-step pass at its tuned shape (12-row tiles, x chunks of 25 planes; 0 = the same automatic)
     ("dipole_slab", "table", "k_step2", (7, 256, 25)): 4175900000,
     ("dipole_slab", "table", "k_step2", (7, 256, 0)): 4175900000,
+    # 28 x 28-cell tiles of 32-lane rows (variant 13), x chunks of 25 planes
+    ("dipole_slab", "table", "k_step2", (13, 256, 25)): 4097500000,
+    ("dipole_slab", "table", "k_step2", (13, 256, 0)): 4097500000,
 })
 
 
