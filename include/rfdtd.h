@@ -208,10 +208,24 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * the two-step pass: steps n and n + 1 of a tile of 12 (8: 4) rows x 60 cells in one launch (two-cell halos
  * recomputed, the x planes skewed by one), reading one buffer of the pair and writing the other, so
  * that the fields are read and written once per two steps (single steps, and every step while a
- * design-box history is recorded, go through the same pass one step at a time); 10 variant 4 with
+ * design-box history is recorded, go through the same pass one step at a time); 9 the two-step
+ * pass over 8-row tiles with the x planes skewed by three (one barrier per plane); 10 variant 4 with
  * nontemporal loads of the arrays streamed once per step (split fields, raw coefficient arrays, entry
- * indices).  Results are identical; the choice only changes speed (the autotuner picks it). */
+ * indices); 11 and 12 the separate kernels over 2 and 4 consecutive z cells per thread, every array
+ * moved as 8- or 16-byte vectors (Nz a multiple of 2 or 4, else RFDTD_EINVAL); 13 and 14 the
+ * two-step pass over tiles of 32-lane rows, 28 x 28 and 12 x 28 cells.  Results are identical; the
+ * choice only changes speed (the autotuner picks it). */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
+
+/* Workgroup numbering of the separate update kernels (variants 0, 1, 4, 10, 11, 12; no counterpart in
+ * the reference): 0 (the default) gives each XCD a contiguous range of tiles, so that neighbouring
+ * tiles share y-neighbour rows in its L2; 1 takes workgroups in launch order, so that every XCD
+ * streams through every region of the arrays.  Launch order is faster on grids of tens of millions of
+ * cells, the XCD ranges on small ones; the autotuner picks it with the launch shape (and apart for the
+ * checkpointed gradient's recomputation, which this call also sets).  Results are identical either
+ * way.  RFDTD_EINVAL for other values. */
+int rfdtd_set_column_order(rfdtd_session* s, int32_t order);
+int rfdtd_get_column_order(rfdtd_session* s, int32_t* order);
 
 /* Launch shape of the reverse pass of rfdtd_gradient(_batch) (no counterpart in the reference):
  * variant 1 x-marching adjoint columns that issue the next plane's loads (and the coefficient keys

@@ -58,6 +58,7 @@ EXPORTS = (
     "rfdtd_set_monitor_streaming", "rfdtd_host_alloc", "rfdtd_host_free",
     "rfdtd_create_assembled", "rfdtd_read_coefficients", "rfdtd_create_batch_assembled",
     "rfdtd_read_coefficient_box", "rfdtd_nf2ff_static", "rfdtd_design_prepare", "rfdtd_grid_barrier_cost",
+    "rfdtd_set_column_order", "rfdtd_get_column_order",
     "rfdtd_set_design_permittivity_device",
 )
 
@@ -167,6 +168,8 @@ def lib() -> ctypes.CDLL:
     L.rfdtd_set_instance_sources.argtypes = [vp, i32, vp, i32, vp, i32, i32]
     L.rfdtd_read_instance_fields.argtypes = [vp, i32, vp]
     L.rfdtd_set_variant.argtypes = [vp, i32]
+    L.rfdtd_set_column_order.argtypes = [vp, i32]
+    L.rfdtd_get_column_order.argtypes = [vp, ctypes.POINTER(i32)]
     L.rfdtd_set_checkpointing.argtypes = [vp, i32]
     L.rfdtd_gradient.argtypes = [vp, i32, vp, vp, i32, vp]
     L.rfdtd_autotune.argtypes = [vp, i32, vp, ctypes.POINTER(ctypes.c_double)]

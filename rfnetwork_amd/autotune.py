@@ -83,17 +83,20 @@ def trial_steps_for(grids, requested=20, limit=None):
 
 def run_tuner(session, trial_steps=20):
     """Run the engine's tuner on a Session or Batch that has not stepped yet; the best shape is
-    left applied.  Returns {variant, block, x_planes (0 = automatic), ms_per_step}."""
+    left applied.  Returns {variant, block, x_planes (0 = automatic), order (the separate kernels'
+    workgroup numbering), ms_per_step}."""
     cfg = (ctypes.c_int32 * 3)()
     ms = ctypes.c_double()
     check(session._L.rfdtd_autotune(session._h, int(trial_steps), ctypes.cast(cfg, ctypes.c_void_p),
                                     ctypes.byref(ms)), "rfdtd_autotune")
-    return dict(variant=int(cfg[0]), block=int(cfg[1]), x_planes=int(cfg[2]), ms_per_step=float(ms.value))
+    return dict(variant=int(cfg[0]), block=int(cfg[1]), x_planes=int(cfg[2]), order=session.column_order(),
+                ms_per_step=float(ms.value))
 
 
 def apply(session, cfg):
     session.set_variant(cfg["variant"])
     session.set_launch(cfg["block"], cfg["x_planes"] if cfg["x_planes"] > 0 else -1, 0)
+    session.set_column_order(cfg.get("order", 0))  # entries without it were tuned with order 0
 
 
 def _adjoint_from_cache(session, key, use_cache):

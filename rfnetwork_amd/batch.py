@@ -231,6 +231,15 @@ class Batch:
     def set_variant(self, variant=1):
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
+    def set_column_order(self, order):
+        """Workgroup numbering of the separate kernels (Session.set_column_order)."""
+        check(self._L.rfdtd_set_column_order(self._h, int(order)), "rfdtd_set_column_order")
+
+    def column_order(self):
+        o = ctypes.c_int32()
+        check(self._L.rfdtd_get_column_order(self._h, ctypes.byref(o)), "rfdtd_get_column_order")
+        return int(o.value)
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             self._L.rfdtd_destroy(self._h)

@@ -166,6 +166,7 @@ struct InstDev {
 // consecutive from inst0 and vary fastest in the logical numbering.
 struct MemberDev {
     int32_t tile0, tiles, K, inst0;
+    int32_t order;  // the column kernels' workgroup numbering: 0 grouped by XCD, 1 launch order
 };
 
 // The 30 arrays as given (one coalesced load per coefficient).
@@ -293,6 +294,17 @@ __device__ __forceinline__ int xcd_logical(int bid, int nblocks) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+// The column kernels (the separate E and H updates, their vector forms, the adjoint passes) number
+// their workgroups either way, as the launch's member table says (the same for every member):
+// grouped by XCD, or in launch order, where consecutive workgroups, which the hardware spreads over
+// the XCDs, stream neighbouring parts of the arrays together.  On a 32 M-cell grid launch order is
+// 3-10% faster (a plain streaming kernel too, by 3.5%); on grids of a few M cells and less the XCD
+// grouping is, as the y-neighbour rows its tiles share stay in one L2.  The tuner picks per session.
+// The tiled passes always group by XCD: their neighbouring tiles share halo rows through the L2.
+__device__ __forceinline__ int column_block(const MemberDev* __restrict__ mem) {
+    return mem[0].order ? (int)blockIdx.x : xcd_logical(blockIdx.x, gridDim.x);
+}
+
 struct Tile {
     int64_t p;  // in-plane offset y*Nz + z
     int y, z, x0, x1;
@@ -304,7 +316,7 @@ template <typename T>
 __device__ __forceinline__ const InstDev<T>* locate(const InstDev<T>* __restrict__ insts,
                                                      const MemberDev* __restrict__ mem, int nmem, int total,
                                                      int* s_member, Tile& t, int W = 1) {
-    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    const int L = column_block(mem);
     if (threadIdx.x == 0) {
         int m = 0;
         if (L < total)
@@ -1760,7 +1772,7 @@ __global__ void __launch_bounds__(1024) k_update_e_save(const InstDev<T>* __rest
                                                         const BoxDev<T>* __restrict__ boxes,
                                                         const int64_t* __restrict__ blk0, int nb) {
     {
-        const int L = xcd_logical(blockIdx.x, gridDim.x);
+        const int L = column_block(mem);  // the same numbering as locate's
         if (L >= total) {
             if (L - total < (int)blk0[nb]) save_h_block<T>(insts, boxes, blk0, nb, step_of(base, local), L - total);
             return;
@@ -1827,7 +1839,7 @@ __global__ void __launch_bounds__(1024) k_adj_h(const InstDev<T>* __restrict__ i
     // workgroups past the column tiles (gboxes given) accumulate the design-box gradient of this
     // step in the same launch: they read the E adjoint and the saved H, which this pass does not write
     if (gboxes) {
-        const int L = xcd_logical(blockIdx.x, gridDim.x);
+        const int L = column_block(mem);  // the same numbering as locate's
         if (L >= total) {
             if (L - total < (int)gblk0[ngb]) adj_grad_block<T>(ginsts, gboxes, gblk0, ngb, gj, L - total);
             return;
@@ -3244,6 +3256,8 @@ struct Session {
     // a gradient's recomputation uses whatever the forward variant: the tuner's best of variants
     // 0, 1, 4, 10
     int sep_block = 256, sep_xplanes = 0;
+    // workgroup numbering of the column kernels (MemberDev::order), forward steps and recomputation
+    int col_order = 0, sep_order = 0;
     bool fused_deep = false;   // fused step with E one plane ahead of H (one barrier per plane)
     bool two_step = false;     // two-step pass (k_step2): pairs of steps per launch; fused = its rows
     bool two_skew = false;     // the two-step pass's one-barrier form (variant 9)
@@ -4003,7 +4017,7 @@ void refresh(Session& s) {
         Member& M = s.members[m];
         const int npb = (int)tiles_per_plane(s, M);
         const int chunks = (M.Nx + M.xplanes - 1) / M.xplanes;
-        dm[m] = MemberDev{tile0, npb * chunks, M.K, M.inst0};
+        dm[m] = MemberDev{tile0, npb * chunks, M.K, M.inst0, s.col_order};
         tile0 += npb * chunks * M.K;
         for (int e = 0; e < M.K; ++e) {
             Instance& I = s.insts[M.inst0 + e];
@@ -4558,19 +4572,20 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     for (int xp : {1, 2})
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
     const int v0 = variant_of(s);
-    const int blk0 = s.block;
+    const int blk0 = s.block, ord0 = s.col_order;
     std::vector<int> xr0;
     for (const auto& M : s.members) xr0.push_back(M.xplanes_req);
-    auto apply = [&](int v, int blk, const std::vector<int>& xr) {
+    auto apply = [&](int v, int blk, const std::vector<int>& xr, int ord) {
         apply_variant(s, v);
         s.block = blk;
+        s.col_order = ord;
         for (size_t m = 0; m < s.members.size(); ++m) s.members[m].xplanes_req = xr[m];
         s.dirty = true;
         s.graph_dirty = true;
     };
     hipStream_t st = s.own_stream;
     double best = 1e300, best_sep = 1e300;
-    int bv = v0, bblk = blk0, bxp = 0, sblk = s.sep_block, sxp = s.sep_xplanes;
+    int bv = v0, bblk = blk0, bxp = 0, bord = ord0, sblk = s.sep_block, sxp = s.sep_xplanes, sord = s.sep_order;
     // With sources installed (and room for the trial steps) the trials apply them as a run does, so
     // that their cost is timed too; the probe rows and monitor values are restored afterwards and
     // the design-box history slots are rewritten by the run.  Otherwise field updates only.
@@ -4590,7 +4605,8 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
         for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9, 13, 14})
             for (int blk : {128, 256, 512})
-                for (int xp : xps) {
+                for (int xp : xps)
+                for (int ord : {0, 1}) {
                     const bool fused_v = v == 2 || v == 3 || v == 5 || v == 6 || is_two_step(v);
                     if (fused_v && separate_only) continue;
                     if (!vec_fits(s, variant_vec(v))) continue;
@@ -4598,7 +4614,10 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
                     // single steps
                     if (is_two_step(v) && !s.hist.empty()) continue;
                     if (fused_v && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
-                    apply(v, blk, std::vector<int>(s.members.size(), xp));
+                    // launch order is a choice of the column kernels, and pays on the shapes that
+                    // stream (few x planes per workgroup)
+                    if (ord == 1 && (fused_v || xp > 2)) continue;
+                    apply(v, blk, std::vector<int>(s.members.size(), xp), ord);
                     refresh<T>(s);
                     run_chunk<T>(s, std::min(trial_steps, 3), 0, st, false, false, 0);  // warm-up
                     HIP_OK(hipEventRecord(s.ev0, st));
@@ -4609,8 +4628,8 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
                     HIP_OK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
                     const double per = (double)ms / trial_steps;
                     if (s.flags & RFDTD_VERBOSE) {
-                        std::fprintf(stderr, "[rfdtd] autotune: variant %d, %d threads, x planes %d: %.4f ms/step\n", v,
-                                     blk, xp, per);
+                        std::fprintf(stderr, "[rfdtd] autotune: variant %d, %d threads, x planes %d, %s: %.4f ms/step\n",
+                                     v, blk, xp, ord ? "launch order" : "XCD ranges", per);
                         std::fflush(stderr);
                     }
                     if (per < best) {
@@ -4618,22 +4637,25 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
                         bv = v;
                         bblk = blk;
                         bxp = xp;
+                        bord = ord;
                     }
                     if (!fused_v && variant_vec(v) == 1 && per < best_sep) {  // the recomputation's shape
                         best_sep = per;
                         sblk = blk;
                         sxp = xp;
+                        sord = ord;
                     }
                 }
     } catch (...) {
         s.tuning = false;
-        apply(v0, blk0, xr0);
+        apply(v0, blk0, xr0, ord0);
         throw;
     }
     s.tuning = false;
-    apply(bv, bblk, std::vector<int>(s.members.size(), bxp));
+    apply(bv, bblk, std::vector<int>(s.members.size(), bxp), bord);
     s.sep_block = sblk;
     s.sep_xplanes = sxp;
+    s.sep_order = sord;
     HIP_OK(hipDeviceSynchronize());
     for (Instance& I : s.insts) {
         HIP_OK(hipMemset(I.d_fields, 0, I.fields_bytes));
@@ -5014,7 +5036,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         di[k].hgroups = I.d_hgroups;
         di[k].nhgroups = I.nhgroups;
         di[n + k] = make_instdev<T>(M, d_adj, I.pitch, npb);
-        md[k] = MemberDev{tile0, npb * chunks, 1, k};
+        md[k] = MemberDev{tile0, npb * chunks, 1, k, s.sep_order};
         tile0 += npb * chunks;
         cells[k] = M.cells;
         // adjoint sources, time-major [Nt][P]
@@ -5078,7 +5100,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
             const int nch = (M.Nx + xp - 1) / xp;
             di[n + k].npb = npb;
             di[n + k].xplanes = xp;
-            mda[k] = MemberDev{tiles, npb * nch, 1, k};
+            mda[k] = MemberDev{tiles, npb * nch, 1, k, 0};  // the adjoint passes keep the XCD grouping
             tiles += npb * nch;
         }
         return tiles;
@@ -5769,6 +5791,26 @@ int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
                                        std::to_string(variant_vec(variant)) + " in every grid");
         apply_variant(S, variant);
         if (!S.fused) release_second_buffers(S);
+    });
+}
+
+int rfdtd_set_column_order(rfdtd_session* s, int32_t order) {
+    return guarded([&] {
+        Session& S = S_of(s);
+        if (order != 0 && order != 1) RF_THROW(RFDTD_EINVAL, "column order must be 0 (XCD ranges) or 1 (launch order)");
+        S.sep_order = order;  // the checkpointed gradient's recomputation too
+        if (S.col_order != order) {
+            S.col_order = order;
+            S.dirty = true;
+            S.graph_dirty = true;
+        }
+    });
+}
+
+int rfdtd_get_column_order(rfdtd_session* s, int32_t* order) {
+    return guarded([&] {
+        if (!order) RF_THROW(RFDTD_EINVAL, "null argument");
+        *order = S_of(s).col_order;
     });
 }
 

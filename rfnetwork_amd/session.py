@@ -242,9 +242,22 @@ class Session:
         0 the same one plane at a time, 2 and 3 the fused E+H pass with 6- or 14-row tiles (two field
         buffers per instance), 4 variant 1 with nontemporal field stores, 5 and 6 variants 2 and 3
         with deeper load pipelining, 7 and 8 the two-step pass (steps n and n + 1 of a 12- or 4-row x
-        60-cell tile per launch, two field buffers per instance), 10 variant 4 with nontemporal loads of
-        the arrays streamed once.  Results are identical."""
+        60-cell tile per launch, two field buffers per instance), 9 the two-step pass over 8-row tiles
+        with the x planes skewed by three, 10 variant 4 with nontemporal loads of the arrays streamed
+        once, 11 and 12 the separate kernels over 2 and 4 z cells per thread (Nz a multiple of 2 or 4),
+        13 and 14 the two-step pass over 28 x 28 and 12 x 28-cell tiles of 32-lane rows.  Results are
+        identical."""
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
+
+    def set_column_order(self, order):
+        """Workgroup numbering of the separate kernels: 0 contiguous tile ranges per XCD, 1 launch
+        order (faster on large grids).  Results are identical."""
+        check(self._L.rfdtd_set_column_order(self._h, int(order)), "rfdtd_set_column_order")
+
+    def column_order(self):
+        o = ctypes.c_int32()
+        check(self._L.rfdtd_get_column_order(self._h, ctypes.byref(o)), "rfdtd_get_column_order")
+        return int(o.value)
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -111,7 +111,9 @@ def test_two_step_streamed_time_monitors(hip_ready):
 @pytest.mark.parametrize("mode", ["checkpoint", "history"])
 def test_two_step_forward_gradient_equals_separate(hip_ready, mode):
     """A forward run on the two-step pass (checkpoints every 9 steps, or a design-box history that
-    makes the pass run single steps) gives the separate kernels' gradient bit for bit."""
+    makes the pass run single steps) gives the separate kernels' gradient bit for bit, as do the
+    separate kernels with their workgroups in launch order (forward run, the history's save blocks,
+    the checkpoint recomputation)."""
     print(f"[case] two-step gradient {mode}", flush=True)
     Nt = 40
     model = workloads.tiny_model(Nt=Nt, time_monitors=False, phasor_monitors=False)
@@ -119,10 +121,11 @@ def test_two_step_forward_gradient_equals_separate(hip_ready, mode):
     g = np.random.default_rng(3).standard_normal((len(probes), Nt)).astype(np.float32)
     box = (2, Nx - 2, 2, Ny - 2, 1, Nz - 1)
     out = []
-    for variant in (1, 7, 8, 9, 13, 14):
+    for variant, order in ((1, 0), (1, 1), (4, 1), (7, 0), (8, 0), (9, 0), (13, 0), (14, 0)):
         _, pr, _, _, *_ = model.pack(gpu=True, coefficients=False)
         with Session(coeff, Nx, Ny, Nz) as s:
             s.set_variant(variant)
+            s.set_column_order(order)  # the separate kernels in launch order: the save blocks and the recomputation too
             if mode == "history":
                 s.set_history(box, Nt)
             else:
