@@ -34,9 +34,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# The best streaming rate measured on MI355X with this library's own probes (a nontemporal copy, and
-# 9 input and 9 output streams as the update kernels have): the practical ceiling beside the peak.
-BEST_STREAMING_GBS = 5720.0
+# The best streaming rate measured on MI355X with this library's own probes (the H update's mix of 27
+# arrays read and 9 written, nontemporal, 128-thread workgroups): the practical ceiling beside the peak.
+BEST_STREAMING_GBS = 5850.0
 E_BYTES, H_BYTES = 120, 144  # algorithmic bytes per cell of the E and H half steps (sum 264)
 METRIC = "Mcell-updates/s + % HBM peak, batched vs plain; adjoint gradients/s, 1 GPU"
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X (KiB per
