@@ -229,7 +229,8 @@ int rfdtd_get_column_order(rfdtd_session* s, int32_t* order);
 
 /* Launch shape of the reverse pass of rfdtd_gradient(_batch) (no counterpart in the reference):
  * variant 1 x-marching adjoint columns that issue the next plane's loads (and the coefficient keys
- * of the plane after) before the current plane's arithmetic, 0 one plane at a time; block_threads
+ * of the plane after) before the current plane's arithmetic, 0 one plane at a time, 2 and 3 variants
+ * 0 and 1 with the workgroups in launch order (rfdtd_set_column_order); block_threads
  * per workgroup; x_chunks workgroups along x per (y,z) tile (0: the forward's planes per
  * workgroup).  block_threads 0 (the default) times the candidates on the zeroed adjoint state at
  * the next gradient and keeps the fastest for the session.  Results are identical either way.

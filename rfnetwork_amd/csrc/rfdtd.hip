@@ -43,6 +43,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -4603,49 +4604,70 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     HIP_OK(hipGetLastError());
     try {
         const bool separate_only = (s.flags & RFDTD_TUNE_SEPARATE) != 0;
+        // every candidate with the workgroups in XCD ranges, and the separate kernels' streaming
+        // shapes (one or two x planes per workgroup) in launch order; then the four fastest other
+        // separate shapes again in launch order (it is faster on large grids, where they can win)
+        std::vector<std::array<double, 4>> sep_times;
+        struct Cand { int v, blk, xp, ord; };
+        std::vector<Cand> cands;
         for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9, 13, 14})
             for (int blk : {128, 256, 512})
-                for (int xp : xps)
-                for (int ord : {0, 1}) {
-                    const bool fused_v = v == 2 || v == 3 || v == 5 || v == 6 || is_two_step(v);
-                    if (fused_v && separate_only) continue;
-                    if (!vec_fits(s, variant_vec(v))) continue;
-                    // a design-box history needs H after every step: the two-step pass would run
-                    // single steps
-                    if (is_two_step(v) && !s.hist.empty()) continue;
-                    if (fused_v && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
-                    // launch order is a choice of the column kernels, and pays on the shapes that
-                    // stream (few x planes per workgroup)
-                    if (ord == 1 && (fused_v || xp > 2)) continue;
-                    apply(v, blk, std::vector<int>(s.members.size(), xp), ord);
-                    refresh<T>(s);
-                    run_chunk<T>(s, std::min(trial_steps, 3), 0, st, false, false, 0);  // warm-up
-                    HIP_OK(hipEventRecord(s.ev0, st));
-                    run_chunk<T>(s, trial_steps, 0, st, false, false, 0);
-                    HIP_OK(hipEventRecord(s.ev1, st));
-                    HIP_OK(hipEventSynchronize(s.ev1));
-                    float ms = 0;
-                    HIP_OK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
-                    const double per = (double)ms / trial_steps;
-                    if (s.flags & RFDTD_VERBOSE) {
-                        std::fprintf(stderr, "[rfdtd] autotune: variant %d, %d threads, x planes %d, %s: %.4f ms/step\n",
-                                     v, blk, xp, ord ? "launch order" : "XCD ranges", per);
-                        std::fflush(stderr);
-                    }
-                    if (per < best) {
-                        best = per;
-                        bv = v;
-                        bblk = blk;
-                        bxp = xp;
-                        bord = ord;
-                    }
-                    if (!fused_v && variant_vec(v) == 1 && per < best_sep) {  // the recomputation's shape
-                        best_sep = per;
-                        sblk = blk;
-                        sxp = xp;
-                        sord = ord;
-                    }
+                for (int xp : xps) {
+                    cands.push_back(Cand{v, blk, xp, 0});
+                    const bool sep_v = !(v == 2 || v == 3 || v == 5 || v == 6 || is_two_step(v));
+                    if (sep_v && (xp == 1 || xp == 2)) cands.push_back(Cand{v, blk, xp, 1});
                 }
+        const size_t n_main = cands.size();
+        for (size_t ci = 0;; ++ci) {
+            if (ci == n_main) {
+                std::sort(sep_times.begin(), sep_times.end());
+                int added = 0;
+                for (size_t k = 0; k < sep_times.size() && added < 4; ++k) {
+                    const int xp = (int)sep_times[k][3];
+                    if (xp == 1 || xp == 2) continue;  // timed in launch order already
+                    cands.push_back(Cand{(int)sep_times[k][1], (int)sep_times[k][2], xp, 1});
+                    ++added;
+                }
+            }
+            if (ci >= cands.size()) break;
+            const int v = cands[ci].v, blk = cands[ci].blk, xp = cands[ci].xp, ord = cands[ci].ord;
+            const bool fused_v = v == 2 || v == 3 || v == 5 || v == 6 || is_two_step(v);
+            if (fused_v && separate_only) continue;
+            if (!vec_fits(s, variant_vec(v))) continue;
+            // a design-box history needs H after every step: the two-step pass would run
+            // single steps
+            if (is_two_step(v) && !s.hist.empty()) continue;
+            if (fused_v && blk != 256) continue;  // the fused step's workgroup size is fixed by its tile
+            apply(v, blk, std::vector<int>(s.members.size(), xp), ord);
+            refresh<T>(s);
+            run_chunk<T>(s, std::min(trial_steps, 3), 0, st, false, false, 0);  // warm-up
+            HIP_OK(hipEventRecord(s.ev0, st));
+            run_chunk<T>(s, trial_steps, 0, st, false, false, 0);
+            HIP_OK(hipEventRecord(s.ev1, st));
+            HIP_OK(hipEventSynchronize(s.ev1));
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+            const double per = (double)ms / trial_steps;
+            if (s.flags & RFDTD_VERBOSE) {
+                std::fprintf(stderr, "[rfdtd] autotune: variant %d, %d threads, x planes %d, %s: %.4f ms/step\n",
+                             v, blk, xp, ord ? "launch order" : "XCD ranges", per);
+                std::fflush(stderr);
+            }
+            if (per < best) {
+                best = per;
+                bv = v;
+                bblk = blk;
+                bxp = xp;
+                bord = ord;
+            }
+            if (!fused_v && variant_vec(v) == 1 && per < best_sep) {  // the recomputation's shape
+                best_sep = per;
+                sblk = blk;
+                sxp = xp;
+                sord = ord;
+            }
+            if (!fused_v && ord == 0) sep_times.push_back({per, (double)v, (double)blk, (double)xp});
+        }
     } catch (...) {
         s.tuning = false;
         apply(v0, blk0, xr0, ord0);
@@ -5091,7 +5113,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
     // reverse-pass tiling: its own workgroup size and planes per workgroup in the adjoint
     // descriptors [n, 2n) and member table d_mda
     std::vector<MemberDev> mda(n);
-    auto adj_tiling = [&](int block, int chunks) {
+    auto adj_tiling = [&](int block, int chunks, int order) {
         int tiles = 0;
         for (int k = 0; k < n; ++k) {
             const Member& M = s.members[s.insts[items[k].inst].member];
@@ -5100,7 +5122,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
             const int nch = (M.Nx + xp - 1) / xp;
             di[n + k].npb = npb;
             di[n + k].xplanes = xp;
-            mda[k] = MemberDev{tiles, npb * nch, 1, k, 0};  // the adjoint passes keep the XCD grouping
+            mda[k] = MemberDev{tiles, npb * nch, 1, k, order};
             tiles += npb * nch;
         }
         return tiles;
@@ -5129,7 +5151,8 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         }
     }
     int ablock = s.adj_block > 0 ? s.adj_block : rb;
-    int atiles = adj_tiling(ablock, s.adj_block > 0 ? s.adj_chunks : 0);
+    // the reverse pass's variant: bit 0 pipelined columns, bit 1 workgroups in launch order
+    int atiles = adj_tiling(ablock, s.adj_block > 0 ? s.adj_chunks : 0, s.adj_pf >> 1);
     const InstDev<T>* d_di = (const InstDev<T>*)upload(di.data(), di.size() * sizeof(InstDev<T>));
     const MemberDev* d_md = (const MemberDev*)upload(md.data(), md.size() * sizeof(MemberDev));
     MemberDev* d_mda = (MemberDev*)upload(mda.data(), mda.size() * sizeof(MemberDev));
@@ -5160,10 +5183,12 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         double best = 1e300;
         int bpf = 1, bblk = ablock, bch = 1;
         const int reps = 3;
-        for (int pf : {1, 0})
+        for (int av : {1, 0, 3, 2})
             for (int blk : {128, 256, 512})
                 for (int ch : chunk_opts) {
-                    const int tiles = adj_tiling(blk, ch);
+                    const int pf = av & 1, ord = av >> 1;
+                    if (ord && 2 * ch < maxNx) continue;  // launch order pays on the streaming shapes
+                    const int tiles = adj_tiling(blk, ch, ord);
                     HIP_OK(hipMemcpyAsync((void*)(d_di + n), di.data() + n, n * sizeof(InstDev<T>), hipMemcpyHostToDevice, st));
                     HIP_OK(hipMemcpyAsync(d_mda, mda.data(), n * sizeof(MemberDev), hipMemcpyHostToDevice, st));
                     adj_e(pf, blk, tiles, -1);
@@ -5180,7 +5205,7 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
                     HIP_OK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
                     if (ms < best) {
                         best = ms;
-                        bpf = pf;
+                        bpf = av;
                         bblk = blk;
                         bch = ch;
                     }
@@ -5189,18 +5214,19 @@ void gradients(Session& s, const std::vector<GradItem>& items) {
         s.adj_block = bblk;
         s.adj_chunks = bch;
         if (s.flags & RFDTD_VERBOSE) {
-            std::fprintf(stderr, "[rfdtd] adjoint launch: %s columns, %d threads, %d x chunks (%.3f ms per step)\n",
-                         bpf ? "pipelined" : "plain", bblk, bch, best / reps);
+            std::fprintf(stderr, "[rfdtd] adjoint launch: %s columns, %s, %d threads, %d x chunks (%.3f ms per step)\n",
+                         (bpf & 1) ? "pipelined" : "plain", (bpf >> 1) ? "launch order" : "XCD ranges", bblk, bch,
+                         best / reps);
             std::fflush(stderr);
         }
         ablock = bblk;
-        atiles = adj_tiling(ablock, bch);
+        atiles = adj_tiling(ablock, bch, bpf >> 1);
         HIP_OK(hipMemcpyAsync((void*)(d_di + n), di.data() + n, n * sizeof(InstDev<T>), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync(d_mda, mda.data(), n * sizeof(MemberDev), hipMemcpyHostToDevice, st));
         for (int k = 0; k < n; ++k) HIP_OK(hipMemsetAsync(di[n + k].F.ex_y, 0, 18 * s.insts[items[k].inst].pitch, st));
         HIP_OK(hipEventRecord(s.ev0, st));
     }
-    const int apf = s.adj_pf;
+    const int apf = s.adj_pf & 1;
     const BoxDev<T>* d_bd = (const BoxDev<T>*)upload(bd.data(), bd.size() * sizeof(BoxDev<T>));
     const int64_t* d_flat0 = (const int64_t*)upload(flat0.data(), flat0.size() * 8);
     const int64_t* d_h0 = (const int64_t*)upload(h0.data(), h0.size() * 8);
@@ -5916,7 +5942,7 @@ int rfdtd_set_monitor_streaming(rfdtd_session* s, int32_t ring_samples) {
 int rfdtd_set_adjoint_launch(rfdtd_session* s, int32_t variant, int32_t block_threads, int32_t x_chunks) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 1) RF_THROW(RFDTD_EINVAL, "adjoint variant must be 0 or 1");
+        if (variant < 0 || variant > 3) RF_THROW(RFDTD_EINVAL, "adjoint variant must be 0, 1, 2 or 3");
         if (block_threads != 0 && (block_threads < 64 || block_threads > 1024 || block_threads % 64))
             RF_THROW(RFDTD_EINVAL, "block_threads must be 0 or a multiple of 64 in [64, 1024]");
         if (x_chunks < 0) RF_THROW(RFDTD_EINVAL, "x_chunks must be >= 0");

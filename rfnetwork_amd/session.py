@@ -156,8 +156,9 @@ class Session:
         check(self._L.rfdtd_set_checkpointing(self._h, int(every)))
 
     def set_adjoint_launch(self, variant=1, block=0, x_chunks=0):
-        """Reverse-pass launch shape: variant 1 pipelined adjoint columns, 0 plain; block threads per
-        workgroup (0: timed at the next gradient); x_chunks workgroups along x (0: the forward's)."""
+        """Reverse-pass launch shape: variant 1 pipelined adjoint columns, 0 plain, 2 and 3 the same with
+        the workgroups in launch order; block threads per workgroup (0: timed at the next gradient);
+        x_chunks workgroups along x (0: the forward's)."""
         check(self._L.rfdtd_set_adjoint_launch(self._h, int(variant), int(block), int(x_chunks)),
               "rfdtd_set_adjoint_launch")
 

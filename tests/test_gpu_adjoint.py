@@ -312,8 +312,9 @@ def test_history_batch_gradients(hip_ready):
 
 
 def test_adjoint_launch_shapes_give_identical_gradients(hip_ready):
-    """The reverse pass's launch shapes (pipelined or plain columns, workgroup sizes, x chunks down
-    to one plane per workgroup) and the shape its timing picks all give the same gradient bits."""
+    """The reverse pass's launch shapes (pipelined or plain columns, XCD ranges or launch order,
+    workgroup sizes, x chunks down to one plane per workgroup) and the shape its timing picks all give
+    the same gradient bits."""
     print("[case] adjoint launch shapes", flush=True)
     Nt = 40
     model = workloads.tiny_model(Nt=Nt, nx=17, ny=13, nz=70, time_monitors=False, phasor_monitors=False)
@@ -327,7 +328,8 @@ def test_adjoint_launch_shapes_give_identical_gradients(hip_ready):
         s.synchronize()
         s.set_adjoint_launch(0, 256, 0)
         ref = s.gradient(box, list(g))
-        for v, blk, ch in [(1, 256, 0), (1, 128, 1), (1, 512, 3), (0, 64, 7), (1, 64, Nx), (1, 1024, 2)]:
+        for v, blk, ch in [(1, 256, 0), (1, 128, 1), (1, 512, 3), (0, 64, 7), (1, 64, Nx), (1, 1024, 2),
+                           (3, 256, Nx), (2, 128, 1), (3, 512, 3)]:
             s.set_adjoint_launch(v, blk, ch)
             assert s.adjoint_launch() == dict(variant=v, block=blk, x_chunks=ch)
             np.testing.assert_array_equal(s.gradient(box, list(g)), ref, err_msg=f"{v} {blk} {ch}")
@@ -336,4 +338,6 @@ def test_adjoint_launch_shapes_give_identical_gradients(hip_ready):
         assert s.adjoint_launch()["block"] in (128, 256, 512)
         with pytest.raises(Exception, match="block_threads"):
             s.set_adjoint_launch(1, 100, 0)
+        with pytest.raises(Exception, match="adjoint variant"):
+            s.set_adjoint_launch(4, 256, 0)
     assert np.abs(ref).max() > 0
