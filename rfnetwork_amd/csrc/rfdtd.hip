@@ -32,7 +32,9 @@
 //   * lanes run along z, the contiguous axis: every load and store is a coalesced 256-byte row;
 //   * workgroups are renumbered so that consecutive tiles land on the same XCD and share its L2:
 //     neighbouring (y,z) tiles share y-neighbour rows, and the instances of one member -- ordered
-//     innermost -- share the coefficient reads of the same cells;
+//     innermost -- share the coefficient reads of the same cells.  The tiled passes always do so;
+//     the separate kernels can instead take workgroups in launch order, which streams faster on
+//     large grids (the autotuner chooses);
 //   * the 30 coefficient arrays are deduplicated bit-exactly, on the device (hash, radix sort, runs
 //     of equal tuples; a host pass on request), into per-cell 32-bit entry indices plus tables of
 //     distinct E (12 values) and H (18 values) tuples: a step moves 8 B of coefficient index per
