@@ -75,6 +75,9 @@ MEASURED_TRAFFIC.update({
     # the two-step pass at its tuned shape (12-row tiles, x chunks of 25 planes; 0 = the same automatic)
     ("dipole_slab", "table", "k_step2", (7, 256, 25)): 4175900000,
     ("dipole_slab", "table", "k_step2", (7, 256, 0)): 4175900000,
+    # the table form of variant 10 in launch order, 4 x planes per workgroup
+    ("dipole_slab", "table", "k_update_e", (10, 256, 4, 1)): 2766400000,
+    ("dipole_slab", "table", "k_update_h", (10, 256, 4, 1)): 2722500000,
     # variant 10 with one x plane per workgroup and the workgroups in launch order (order 1): the
     # counters read 8-10% above the algorithmic bytes, the neighbour rows no longer sharing an L2
     ("dipole_slab", "raw", "k_update_e", (10, 256, 1, 1)): 4225500000,
@@ -682,6 +685,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant 0..6 (default: autotuned)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--xplanes", type=int, default=0)
+    ap.add_argument("--order", type=int, choices=(0, 1), default=0,
+                    help="with a fixed launch shape: the separate kernels' workgroups in XCD ranges (0) or launch order (1)")
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU oracle work per thread count (0 = skip)")
     ap.add_argument("--no-copy-probe", action="store_true")
@@ -773,8 +778,9 @@ def timed_run(args, coef, kernel_timing=True, separate=False):
         s.set_launch(args.block, args.xplanes, 0)
         if args.variant >= 0:
             s.set_variant(args.variant)
+        s.set_column_order(args.order)
         tuned = dict(variant=args.variant if args.variant >= 0 else 1, block=args.block or 256, x_planes=args.xplanes,
-                     cached=None)
+                     order=args.order, cached=None)
         s.set_sources(probes, monitors, Nt)
     else:
         s.set_sources(probes, monitors, Nt)  # installed first: the tuner's trials then apply them too
