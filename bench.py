@@ -828,7 +828,7 @@ TWO_STEP_TABLE_BYTES = 96 + 8 + 0.25
 TWO_STEP_RAW_BYTES = 96 + 120
 # kernel variants (rfdtd_set_variant): 0, 1, 4, 10 separate E and H kernels; 2, 3, 5, 6 the fused
 # E+H pass; 7, 8, 9 the two-step pass
-TWO_STEP_VARIANTS = (7, 8, 9, 13, 14)
+TWO_STEP_VARIANTS = (7, 8, 9, 13, 14, 15)
 FUSED_VARIANTS = (2, 3, 5, 6) + TWO_STEP_VARIANTS
 VECTOR_VARIANTS = (11, 12)  # separate kernels over 2 or 4 z cells per thread (k_update_e_w, k_update_h_w)
 

@@ -212,9 +212,9 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * pass over 8-row tiles with the x planes skewed by three (one barrier per plane); 10 variant 4 with
  * nontemporal loads of the arrays streamed once per step (split fields, raw coefficient arrays, entry
  * indices); 11 and 12 the separate kernels over 2 and 4 consecutive z cells per thread, every array
- * moved as 8- or 16-byte vectors (Nz a multiple of 2 or 4, else RFDTD_EINVAL); 13 and 14 the
- * two-step pass over tiles of 32-lane rows, 28 x 28 and 12 x 28 cells.  Results are identical; the
- * choice only changes speed (the autotuner picks it). */
+ * moved as 8- or 16-byte vectors (Nz a multiple of 2 or 4, else RFDTD_EINVAL); 13, 14 and 15 the
+ * two-step pass over tiles of 32-lane rows, 28 x 28, 12 x 28 and 26 x 28 cells.  Results are
+ * identical; the choice only changes speed (the autotuner picks it). */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 
 /* Workgroup numbering of the separate update kernels (variants 0, 1, 4, 10, 11, 12; no counterpart in

@@ -2407,7 +2407,7 @@ constexpr int kP2Fz = 60;   // z cells owned by a two-step tile of 64-lane rows
 // Tiles of 32-lane rows (two rows per wavefront; 28 z cells owned per row): 28 rows in 1024 threads
 // (76.5% of the lanes own a cell, against 70.3% for 12 x 60 in 64-lane rows), or 12 rows in 512
 // threads (two workgroups per CU); narrow grids (Nz = 64) waste less of their second z tile.
-constexpr int kP2Ty32Tall = 28, kP2Ty32Short = 12;
+constexpr int kP2Ty32Tall = 28, kP2Ty32Short = 12, kP2Ty32Mid = 26;
 constexpr int kP2Mon = 64;  // monitors a two-step pass samples itself, at most
 
 // The monitor samples of one owned cell at step n: every monitor of the tile's list (those of this
@@ -3894,17 +3894,18 @@ bool vec_fits(const Session& s, int vec) {
 // of H, 7 and 8 the two-step pass with 12- and 4-row tiles, 9 its one-barrier form with 8-row tiles,
 // 10 = 4 with nontemporal loads, 11 and 12 = 10 over 2 and 4 consecutive z cells per thread, 13 and
 // 14 the two-step pass with 32-lane rows (28 x 28 and 12 x 28 cell tiles).
-bool is_two_step(int v) { return v == 7 || v == 8 || v == 9 || v == 13 || v == 14; }
+bool is_two_step(int v) { return v == 7 || v == 8 || v == 9 || v == 13 || v == 14 || v == 15; }
 void apply_variant(Session& s, int v) {
     s.pipelined = v != 0;
     s.two_step = is_two_step(v);
     s.two_skew = v == 9;
-    s.p2_rs = (v == 13 || v == 14) ? 32 : 64;
+    s.p2_rs = (v == 13 || v == 14 || v == 15) ? 32 : 64;
     s.fused = v == 7 ? kP2TyTall : v == 8 ? kP2TyShort : v == 9 ? kP2TySkew : v == 13 ? kP2Ty32Tall
-            : v == 14 ? kP2Ty32Short : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
+            : v == 14 ? kP2Ty32Short : v == 15 ? kP2Ty32Mid : (v == 2 || v == 5) ? 6 : (v == 3 || v == 6) ? 14 : 0;
     s.fused_deep = v == 5 || v == 6;
-    s.nontemporal = v == 4 || v >= 10;
-    s.nt_loads = v >= 10;
+    // the separate kernels' load and store hints (the tiled passes have their own)
+    s.nontemporal = v == 4 || (v >= 10 && v <= 12);
+    s.nt_loads = v >= 10 && v <= 12;
     s.vec = variant_vec(v);
     s.dirty = true;
     s.graph_dirty = true;
@@ -3912,7 +3913,7 @@ void apply_variant(Session& s, int v) {
 
 int variant_of(const Session& s) {
     return s.vec == 4 ? 12 : s.vec == 2 ? 11 : s.nt_loads ? 10 : s.two_skew ? 9
-         : (s.two_step && s.p2_rs == 32) ? (s.fused == kP2Ty32Tall ? 13 : 14)
+         : (s.two_step && s.p2_rs == 32) ? (s.fused == kP2Ty32Tall ? 13 : s.fused == kP2Ty32Mid ? 15 : 14)
          : s.two_step ? (s.fused == kP2TyShort ? 8 : 7) : s.fused == 14 ? (s.fused_deep ? 6 : 3)
          : s.fused == 6 ? (s.fused_deep ? 5 : 2) : s.nontemporal ? 4 : s.pipelined ? 1 : 0;
 }
@@ -4177,6 +4178,10 @@ void launch_step2(Session& s, hipStream_t st, const int* base, int local, bool t
             slot, s.d_rows);
     else if (s.p2_rs == 32 && s.fused == kP2Ty32Tall)
         k_step2<T, kP2Ty32Tall, false, 32><<<grid, (kP2Ty32Tall + 4) * 32, 0, st>>>(
+            di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
+            slot, s.d_rows);
+    else if (s.p2_rs == 32 && s.fused == kP2Ty32Mid)
+        k_step2<T, kP2Ty32Mid, false, 32><<<grid, (kP2Ty32Mid + 4) * 32, 0, st>>>(
             di, s.d_members, (int)s.members.size(), s.total_tiles, base, local, io, two ? 1 : 0, s.d_mons, s.n_active,
             slot, s.d_rows);
     else if (s.p2_rs == 32)
@@ -4612,7 +4617,7 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         std::vector<std::array<double, 4>> sep_times;
         struct Cand { int v, blk, xp, ord; };
         std::vector<Cand> cands;
-        for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9, 13, 14})
+        for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9, 13, 14, 15})
             for (int blk : {128, 256, 512})
                 for (int xp : xps) {
                     cands.push_back(Cand{v, blk, xp, 0});
@@ -5813,7 +5818,7 @@ int rfdtd_set_launch(rfdtd_session* s, int32_t block_threads, int32_t x_planes, 
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant) {
     return guarded([&] {
         Session& S = S_of(s);
-        if (variant < 0 || variant > 14) RF_THROW(RFDTD_EINVAL, "variant must be in 0..14");
+        if (variant < 0 || variant > 15) RF_THROW(RFDTD_EINVAL, "variant must be in 0..15");
         if (!vec_fits(S, variant_vec(variant)))
             RF_THROW(RFDTD_EINVAL, "variant " + std::to_string(variant) + " needs Nz a multiple of " +
                                        std::to_string(variant_vec(variant)) + " in every grid");

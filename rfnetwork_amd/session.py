@@ -246,8 +246,8 @@ class Session:
         60-cell tile per launch, two field buffers per instance), 9 the two-step pass over 8-row tiles
         with the x planes skewed by three, 10 variant 4 with nontemporal loads of the arrays streamed
         once, 11 and 12 the separate kernels over 2 and 4 z cells per thread (Nz a multiple of 2 or 4),
-        13 and 14 the two-step pass over 28 x 28 and 12 x 28-cell tiles of 32-lane rows.  Results are
-        identical."""
+        13, 14 and 15 the two-step pass over 28 x 28, 12 x 28 and 26 x 28-cell tiles of 32-lane rows.
+        Results are identical."""
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def set_column_order(self, order):

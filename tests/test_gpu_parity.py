@@ -315,7 +315,7 @@ def test_graded_grid_every_coefficient_kind(hip_ready, dtype, flags):
     print(f"[case] graded grid {np.dtype(dtype).name} flags={flags}", flush=True)
     model = _graded_everything(Nt=70, coef_dtype=dtype)
     p_c, m_c, f_c = run_oracle(model, dtype, n_threads=3)
-    for variant in (None, 2, 3, 5, 6, 7, 8, 9, 13, 14, 10, 11, 12):
+    for variant in (None, 2, 3, 5, 6, 7, 8, 9, 13, 14, 15, 10, 11, 12):
         p_g, m_g, f_g, _ = _hip_session(model, dtype, flags, chunks=[41, 29], variant=variant)
         for a, b in zip(p_g, p_c):
             np.testing.assert_array_equal(a["values"], b["values"], err_msg=f"variant {variant}")
