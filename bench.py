@@ -82,7 +82,8 @@ MEASURED_TRAFFIC.update({
     # counters read 8-10% above the algorithmic bytes, the neighbour rows no longer sharing an L2
     ("dipole_slab", "raw", "k_update_e", (10, 256, 1, 1)): 4225500000,
     ("dipole_slab", "raw", "k_update_h", (10, 256, 1, 1)): 4993600000,
-    # 26 x 28-cell tiles of 32-lane rows (variant 15), x chunks of 50 planes
+    # 26 x 28-cell tiles of 32-lane rows (variant 15), x chunks of 67 and 50 planes
+    ("dipole_slab", "table", "k_step2", (15, 256, 67)): 3946000000,
     ("dipole_slab", "table", "k_step2", (15, 256, 50)): 3911700000,
     # 28 x 28-cell tiles of 32-lane rows (variant 13), x chunks of 25 planes
     ("dipole_slab", "table", "k_step2", (13, 256, 25)): 4097500000,

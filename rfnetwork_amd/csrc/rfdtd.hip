@@ -4579,6 +4579,13 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
     // (the separate kernels on large grids run fastest so)
     for (int xp : {1, 2})
         if (std::find(xps.begin(), xps.end(), xp) == xps.end()) xps.push_back(xp);
+    // the tiled passes also try chunk counts between the powers of two: how their tiles fill the
+    // XCDs' rounds changes their time by several per cent (a trial each, not one per block size)
+    std::vector<int> xps_tiled = xps;
+    for (int chunks : {3, 5, 6, 10, 12}) {
+        const int xp = (maxNx + chunks - 1) / chunks;
+        if (std::find(xps_tiled.begin(), xps_tiled.end(), xp) == xps_tiled.end()) xps_tiled.push_back(xp);
+    }
     const int v0 = variant_of(s);
     const int blk0 = s.block, ord0 = s.col_order;
     std::vector<int> xr0;
@@ -4617,13 +4624,14 @@ void autotune(Session& s, int trial_steps, int32_t cfg[3], double* ms_out) {
         std::vector<std::array<double, 4>> sep_times;
         struct Cand { int v, blk, xp, ord; };
         std::vector<Cand> cands;
-        for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9, 13, 14, 15})
+        for (int v : {1, 4, 10, 11, 12, 0, 2, 3, 5, 6, 7, 8, 9, 13, 14, 15}) {
+            const bool sep_v = !(v == 2 || v == 3 || v == 5 || v == 6 || is_two_step(v));
             for (int blk : {128, 256, 512})
-                for (int xp : xps) {
+                for (int xp : sep_v ? xps : xps_tiled) {
                     cands.push_back(Cand{v, blk, xp, 0});
-                    const bool sep_v = !(v == 2 || v == 3 || v == 5 || v == 6 || is_two_step(v));
                     if (sep_v && (xp == 1 || xp == 2)) cands.push_back(Cand{v, blk, xp, 1});
                 }
+        }
         const size_t n_main = cands.size();
         for (size_t ci = 0;; ++ci) {
             if (ci == n_main) {
