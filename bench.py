@@ -496,14 +496,14 @@ def adjoint_leg(n_steps, every, repeats=2, flags=0):
 def c5_leg(n_steps, flags=0):
     """8 design variants from 96^3 to 512x512x256 (142 M cells), port 1 driven in each: the forward
     runs and the adjoint passes of all variants in batched launches, the checkpoint interval planned
-    to fit device memory.  Reported: wall time of the whole gradient of all 8 (model assembly
-    excluded), the planned interval and bytes."""
+    to fit device memory, the coefficients assembled on the GPU.  Reported: wall time of the whole
+    gradient of all 8 (meshing excluded), the planned interval and bytes."""
     from rfnetwork_amd import _lib, adjoint, workloads
 
     jobs, cells = [], 0
     for v in range(len(workloads.DESIGN_VARIANTS)):
         t0 = time.perf_counter()
-        m = workloads.design_variant(v, Nt=n_steps)
+        m = workloads.design_variant(v, Nt=n_steps, coefficients="device")  # assembled on the GPU
         jobs.append((m, workloads.design_variant_box(v), m.frequency))
         cells += m.Nx * m.Ny * m.Nz
         log(f"c5: variant {v} {m.Nx}x{m.Ny}x{m.Nz} built in {time.perf_counter() - t0:.1f}s")

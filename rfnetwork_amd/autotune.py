@@ -73,11 +73,13 @@ def _store(key, cfg):
 
 
 def trial_steps_for(grids, requested=20, limit=None):
-    """Steps per tuner trial: at least `requested`, and more for small grids, whose steps take a few
-    microseconds (about 3 ms of work per trial at 50 M cell-updates per ms), at most 200 and at
-    most `limit` (the installed series' length)."""
-    cells = sum(nx * ny * nz * k for nx, ny, nz, k in grids)
-    n = max(int(requested), min(200, int(1.5e8 // max(cells, 1))))
+    """Steps per tuner trial: `requested`, more for small grids, whose steps take a few microseconds
+    (about 3 ms of work per trial at 50 M cell-updates per ms, at most 200 steps), and fewer above
+    50 M cells, whose steps take milliseconds (the trials' cost grows with the grid, their timing
+    error does not; at least 6), at most `limit` (the installed series' length)."""
+    cells = max(sum(nx * ny * nz * k for nx, ny, nz, k in grids), 1)
+    base = int(requested) if cells <= 5e7 else max(6, int(requested * 5e7 / cells))
+    n = max(base, min(200, int(1.5e8 // cells)))
     return min(n, int(limit)) if limit else n
 
 

@@ -146,7 +146,7 @@ def design_box(nx=200, ny=160, design=(64, 64, 32)):
 
 def design_strip(eps_design=None, Nt=6000, nx=200, ny=160, nz=80, design=(64, 64, 32), cell_m=1e-4,
                  courant=0.99, n_pml=10, h=8, seed=4, coef_dtype=np.float32, pulse=(4e9, 600e-12, 420e-12),
-                 frequency=(2e9, 3e9, 4e9, 5e9, 6e9)):
+                 frequency=(2e9, 3e9, 4e9, 5e9, 6e9), coefficients="host"):
     """Two microstrip feeds coupled through a box of per-cell permittivity (the adjoint workload).
 
     eps_design: eps_r of the box cells, shape ``design``; default U[1, 4) from ``seed``.
@@ -169,7 +169,7 @@ def design_strip(eps_design=None, Nt=6000, nx=200, ny=160, nz=80, design=(64, 64
     s.add_lumped_port(1, Rectangle([(xa * d, w0 * d, 0), (xa * d, w1 * d, 0), (xa * d, w1 * d, h * d)]), "z+")
     s.add_lumped_port(2, Rectangle([(xb * d, w0 * d, 0), (xb * d, w1 * d, 0), (xb * d, w1 * d, h * d)]), "z+")
     s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=n_pml)
-    s.generate_mesh(courant=courant)
+    s.generate_mesh(courant=courant, coefficients=coefficients)
     f0, width, t0 = pulse
     src = s.gaussian_modulated_source(f0=f0, width=width, t0=t0, t_len=(Nt + 0.5) * s.dt)
     s.assign_excitation(1e-2 * np.asarray(src), 1)
@@ -227,8 +227,10 @@ def design_variant_box(v):
     return (x0, x0 + b, y0, y0 + b, 0, b)
 
 
-def design_variant(v, eps_design=None, Nt=8000, cell_m=1e-4, courant=0.99, n_pml=10, h=8, coef_dtype=np.float32):
-    """Variant v (0..7) of the batched forward+adjoint workload."""
+def design_variant(v, eps_design=None, Nt=8000, cell_m=1e-4, courant=0.99, n_pml=10, h=8, coef_dtype=np.float32,
+                   coefficients="host"):
+    """Variant v (0..7) of the batched forward+adjoint workload (coefficients="device": no full-grid
+    coefficient arrays on the host; a session assembles them on the GPU)."""
     nx, ny, nz, n_ports = DESIGN_VARIANTS[v]
     x0, x1, y0, y1, z0, z1 = design_variant_box(v)
     b = x1 - x0
@@ -255,7 +257,7 @@ def design_variant(v, eps_design=None, Nt=8000, cell_m=1e-4, courant=0.99, n_pml
             s.add_lumped_port(k + 1, Rectangle([((cx - 3) * d, p * d, 0), ((cx + 3) * d, p * d, 0),
                                                 ((cx + 3) * d, p * d, h * d)]), "z+")
     s.assign_PML_boundaries("x-", "x+", "y-", "y+", "z+", n_pml=n_pml)
-    s.generate_mesh(courant=courant)
+    s.generate_mesh(courant=courant, coefficients=coefficients)
     src = s.gaussian_modulated_source(f0=4e9, width=600e-12, t0=420e-12, t_len=(Nt + 0.5) * s.dt)
     s.assign_excitation(1e-2 * np.asarray(src), 1)
     s.frequency = np.array([2e9, 3e9, 4e9, 5e9, 6e9])

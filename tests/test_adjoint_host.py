@@ -163,3 +163,15 @@ def test_unrolled_torch_gradient_matches_finite_differences():
         ad = float(leaf[name].grad[idx])
         assert ad == pytest.approx(fd, rel=1e-6, abs=1e-9 * abs(ad)), name
         assert ad != 0
+
+
+def test_tuner_trial_lengths():
+    """Tuner trials: 20 steps on grids of up to 50 M cells, more on small ones (up to 200), fewer on
+    larger ones (at least 6), never past the installed series."""
+    from rfnetwork_amd.autotune import trial_steps_for
+
+    assert trial_steps_for([(400, 400, 200, 1)]) == 20
+    assert trial_steps_for([(120, 48, 32, 1)]) == 200
+    assert trial_steps_for([(512, 512, 256, 1), (384, 384, 224, 1)]) == 9  # 100.1 M cells
+    assert trial_steps_for([(1024, 1024, 512, 1)]) == 6
+    assert trial_steps_for([(120, 48, 32, 1)], limit=50) == 50

@@ -84,10 +84,11 @@ def test_adjoint_fp32_error_against_fp64(hip_ready):
 SMALL_F = (16e9, 20e9, 24e9)
 
 
-def _small_design(eps, coef_dtype=np.float64, Nt=900):
+def _small_design(eps, coef_dtype=np.float64, Nt=900, coefficients="host"):
     """The adjoint workload shrunk to 48x32x20 cells with an 8x8x6 design box and a 20 GHz pulse."""
     return workloads.design_strip(eps, Nt=Nt, nx=48, ny=32, nz=20, design=(8, 8, 6), n_pml=5, h=3,
-                                  coef_dtype=coef_dtype, pulse=(20e9, 80e-12, 60e-12), frequency=SMALL_F)
+                                  coef_dtype=coef_dtype, pulse=(20e9, 80e-12, 60e-12), frequency=SMALL_F,
+                                  coefficients=coefficients)
 
 
 def _fd_oracle(eps, cell, h, frequency):
@@ -226,6 +227,28 @@ def test_eps_gradients_batch_equal_single(hip_ready):
         assert info["mode"] == mode
         if mode == "checkpoint":
             assert info["checkpoint_every"] == 70
+
+
+def test_eps_gradients_batch_device_assembled_models(hip_ready):
+    """The batched gradient of models meshed without host coefficient arrays (coefficients="device":
+    the batch assembles them on the GPU, and the chain rule reads the design box's values back)
+    equals that of the same models assembled on the host, J and dJ/deps_r bit for bit."""
+    print("[case] eps gradients batch, device-assembled models", flush=True)
+    eps_a = np.random.default_rng(4).uniform(1.0, 4.0, size=(8, 8, 6))
+    eps_b = np.random.default_rng(5).uniform(1.0, 4.0, size=(6, 10, 5))
+    kw_b = dict(Nt=300, nx=52, ny=36, nz=18, design=(6, 10, 5), n_pml=5, h=3, pulse=(20e9, 80e-12, 60e-12),
+                frequency=SMALL_F)
+    ca, cb = workloads.design_box(48, 32, (8, 8, 6)), workloads.design_box(52, 36, (6, 10, 5))
+    out = []
+    for coefficients in ("host", "device"):
+        ma = _small_design(eps_a, coef_dtype=np.float32, Nt=300, coefficients=coefficients)
+        mb = workloads.design_strip(eps_b, coefficients=coefficients, **kw_b)
+        out.append(adjoint.eps_r_gradients_batch([(ma, ca, ma.frequency), (mb, cb, mb.frequency)],
+                                                 checkpoint_every=70, mode="checkpoint"))
+    for (Jh, gh, _), (Jd, gd, _) in zip(*out):
+        assert Jh == Jd
+        np.testing.assert_array_equal(gh, gd)
+        assert np.abs(gh).max() > 0
 
 
 def _session_gradient(model, g, box, variant, history, chunks, flags=0):
