@@ -132,7 +132,7 @@ def dft_bytes_per_step(model):
     return total
 
 
-# The GPU box allots 16 host cores to a one-GPU job, whatever nproc reports there.
+# A one-GPU job is allotted 16 host cores on the machines this runs on, whatever nproc reports.
 BOX_CPU_SHARE = 16
 REF_MAX_THREADS = 20  # the reference's thread cap (rfnetwork/core/inc/solver.h:17)
 API_DEFAULT_THREADS = 4  # FDTD_Solver.solve(n_threads=4) (rfnetwork/solver.py:1410)

@@ -1,6 +1,6 @@
 """Host-side pieces of bench.py: the CPU baseline leg and the per-leg guard.
 
-The driver runs `bench.py --steps 20 --warmup 5`, so the GPU leg's model holds 25 steps of source
+A run with `bench.py --steps 20 --warmup 5` builds the GPU leg's model with 25 steps of source
 samples while the CPU window can be longer: the baseline must build its own series for its window."""
 import json
 import subprocess
