@@ -133,7 +133,7 @@ def dft_bytes_per_step(model):
 
 
 # A one-GPU job is allotted 16 host cores on the machines this runs on, whatever nproc reports.
-BOX_CPU_SHARE = 16
+HOST_CPU_SHARE = 16
 REF_MAX_THREADS = 20  # the reference's thread cap (rfnetwork/core/inc/solver.h:17)
 API_DEFAULT_THREADS = 4  # FDTD_Solver.solve(n_threads=4) (rfnetwork/solver.py:1410)
 
@@ -164,7 +164,7 @@ def cpu_thread_counts():
     """The reference's thread count on this host, min(cores, 20) (held to the box's share), and the
     API default of 4."""
     cores = host_cores()
-    top = min(cores, REF_MAX_THREADS, BOX_CPU_SHARE)
+    top = min(cores, REF_MAX_THREADS, HOST_CPU_SHARE)
     return cores, top, sorted({top, min(API_DEFAULT_THREADS, top)}, reverse=True)
 
 
@@ -217,7 +217,7 @@ def cpu_baseline(workload, budget_s, max_window=200, runs=3):
     return dict(value=best["mcells"], unit="Mcell-updates/s", cores=top, kind="port",
                 sample=f"{best['window_steps']} steps of the full {model.Nx}x{model.Ny}x{model.Nz} workload from zero "
                        f"fields, median of {runs} runs (oracle restating rfnetwork/core/src/solver.cpp: {top} threads "
-                       f"= min(cores, 20) held to the box's {BOX_CPU_SHARE}-core share, x slabs, 2 barriers per step, "
+                       f"= min(cores, 20) held to the host's {HOST_CPU_SHARE}-core share, x slabs, 2 barriers per step, "
                        f"-O2 -ffp-contract=off)",
                 cpu_model=cpu_model(), host_cores=cores, by_threads={str(k): v for k, v in per_count.items()})
 
