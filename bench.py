@@ -161,7 +161,7 @@ def cpu_model():
 
 
 def cpu_thread_counts():
-    """The reference's thread count on this host, min(cores, 20) (held to the box's share), and the
+    """The reference's thread count on this host, min(cores, 20) (held to the host's share), and the
     API default of 4."""
     cores = host_cores()
     top = min(cores, REF_MAX_THREADS, HOST_CPU_SHARE)

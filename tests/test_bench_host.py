@@ -36,7 +36,7 @@ def test_oracle_inputs_refuse_a_short_series():
 
 def test_thread_counts_follow_the_reference_cap():
     cores, top, counts = bench.cpu_thread_counts()
-    assert top == min(cores, 20, bench.BOX_CPU_SHARE)
+    assert top == min(cores, 20, bench.HOST_CPU_SHARE)
     assert counts[0] == top and min(counts) == min(4, top)
 
 
