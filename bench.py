@@ -817,7 +817,10 @@ def kernel_roofline(run, kname, kbytes, kms, traffic, bytes_model):
                 bytes_per_cell=kbytes, bytes_model=bytes_model, algorithmic_bytes_per_launch=cells * kbytes,
                 launch_ms=round(kms, 4),
                 measured_hbm_gbs=round(traffic / (kms * 1e-3) / 1e9, 1) if traffic and kms else None,
-                measured_hbm_frac=round(traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic and kms else None)
+                measured_hbm_frac=round(traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic and kms else None,
+                traffic_note="rocprofv3 FETCH_SIZE and WRITE_SIZE for this launch shape, scaled by a 1 GiB copy's "
+                             "counts: the L2's fills from and write-backs to the data fabric, which the Infinity "
+                             "Cache serves in part, so measured_hbm_gbs can exceed what HBM alone delivers")
 
 
 # Table form: the deduplicated coefficient tuples live in L2, so a half step moves its fields (6 split
