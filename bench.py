@@ -685,7 +685,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dipole_slab")
     ap.add_argument("--coef", choices=("auto", "raw", "table"), default="auto")
-    ap.add_argument("--variant", type=int, default=-1, help="kernel variant 0..6 (default: autotuned)")
+    ap.add_argument("--variant", type=int, default=-1, help="kernel variant 0..15 (default: autotuned)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--xplanes", type=int, default=0)
     ap.add_argument("--order", type=int, choices=(0, 1), default=0,
