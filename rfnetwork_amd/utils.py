@@ -4,6 +4,8 @@
 optionally after a 20th-order Butterworth low-pass (second-order sections, forward-backward) and
 decimation to 20x the highest frequency.
 """
+import functools
+
 import numpy as np
 from scipy import signal
 
@@ -20,6 +22,22 @@ class LabeledArray(np.ndarray):
         self.coords = getattr(obj, "coords", {})
 
 
+_PHASE_CACHE_ENTRIES = 1 << 22  # tables up to 64 MiB (complex128) are cached, at most 8 of them
+
+
+@functools.lru_cache(maxsize=8)
+def _phases(n_samples: int, omega_bytes: bytes) -> np.ndarray:
+    """exp(-j omega n) for n = 0 .. n_samples - 1 (rows) and each omega (columns), read-only.  The
+    S-parameters of one solve transform every port and loop series over the same samples and
+    frequencies, so the table is built once for all of them; the values are those computed in
+    place, so the sums are unchanged."""
+    omega = np.frombuffer(omega_bytes, dtype=np.float64)
+    n = np.arange(n_samples)
+    e = np.exp(-1j * omega[None, :] * n[:, None])
+    e.flags.writeable = False
+    return e
+
+
 def dtft(xn: np.ndarray, frequency: np.ndarray, fs: float, downsample: bool = False) -> np.ndarray:
     """Discrete-time Fourier transform of x[n] sampled at fs, evaluated at ``frequency`` (Hz)."""
     xn = np.asarray(xn)
@@ -32,9 +50,12 @@ def dtft(xn: np.ndarray, frequency: np.ndarray, fs: float, downsample: bool = Fa
         xn = signal.sosfiltfilt(sos, xn)[::factor]
         fs = frs
     omega = 2 * np.pi * (frequency / fs)
-    n = np.arange(len(xn))
     # one row of phases per sample; float64 throughout
-    Xw = np.sum(xn[:, None] * np.exp(-1j * omega[None, :] * n[:, None]), axis=0)
+    if len(xn) * len(omega) <= _PHASE_CACHE_ENTRIES:
+        e = _phases(len(xn), omega.tobytes())
+    else:
+        e = np.exp(-1j * omega[None, :] * np.arange(len(xn))[:, None])
+    Xw = np.sum(xn[:, None] * e, axis=0)
     if use_ds:
         Xw *= factor
     return Xw

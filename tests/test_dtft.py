@@ -35,3 +35,21 @@ def test_downsampled_dtft_matches_full_for_band_limited_signal():
     # the decimation path is a 20th-order Butterworth low-pass applied forward and backward
     sos = signal.butter(20, np.max(f) * 20 / 2.5, btype="lowpass", output="sos", fs=fs)
     assert sos.shape == (10, 6)
+
+
+def test_cached_phase_table_gives_the_same_bits(monkeypatch):
+    """The phase table shared by the transforms of one solve changes no bit: a cached call, a
+    repeated call and a call with caching switched off agree exactly, and the cached table cannot be
+    written through."""
+    rng = np.random.default_rng(5)
+    f = np.linspace(1e9, 12e9, 57)
+    x = rng.standard_normal(1500).astype(np.float32)
+    utils._phases.cache_clear()
+    first = utils.dtft(x, f, 3e11)
+    again = utils.dtft(x, f, 3e11)
+    assert utils._phases.cache_info().hits >= 1
+    monkeypatch.setattr(utils, "_PHASE_CACHE_ENTRIES", 0)
+    direct = utils.dtft(x, f, 3e11)
+    assert np.array_equal(first, again) and np.array_equal(first, direct)
+    table = utils._phases(len(x), (2 * np.pi * (f / 3e11)).tobytes())
+    assert not table.flags.writeable
