@@ -198,7 +198,7 @@ int rfdtd_gradient_batch(rfdtd_session* s, int32_t n, const int32_t* instances, 
 int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_t* instances, const int32_t* boxes,
                            uint64_t budget_bytes, int32_t* every_out, uint64_t* bytes_out);
 
-/* Kernel variant (no counterpart in the reference): 1 (default) separate E and H kernels that issue
+/* Kernel variant (no counterpart in the reference): 1 separate E and H kernels that issue
  * the loads of the next x plane before the arithmetic and stores of the current one; 0 the same
  * kernels loading one plane at a time; 2 and 3 one fused pass per time step (E and H of a tile of
  * 6 or 14 rows x 62 cells, neighbours through LDS, halos recomputed) that reads one buffer of a
@@ -214,7 +214,9 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * indices); 11 and 12 the separate kernels over 2 and 4 consecutive z cells per thread, every array
  * moved as 8- or 16-byte vectors (Nz a multiple of 2 or 4, else RFDTD_EINVAL); 13, 14 and 15 the
  * two-step pass over tiles of 32-lane rows, 28 x 28, 12 x 28 and 26 x 28 cells.  Results are
- * identical; the choice only changes speed (the autotuner picks it). */
+ * identical; the choice only changes speed (the autotuner picks it).  A session starts on variant
+ * 15 with automatic x chunks when it is float, on the coefficient table, and advances at least 8 Mi
+ * cell-updates per step (cells x excitations over all grids); otherwise on variant 1. */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 
 /* Workgroup numbering of the separate update kernels (variants 0, 1, 4, 10, 11, 12; no counterpart in

@@ -2408,6 +2408,8 @@ constexpr int kP2Fz = 60;   // z cells owned by a two-step tile of 64-lane rows
 // (76.5% of the lanes own a cell, against 70.3% for 12 x 60 in 64-lane rows), or 12 rows in 512
 // threads (two workgroups per CU); narrow grids (Nz = 64) waste less of their second z tile.
 constexpr int kP2Ty32Tall = 28, kP2Ty32Short = 12, kP2Ty32Mid = 26;
+// cell-updates per step (cells x excitations, all grids) from which a session starts on the two-step pass
+constexpr int64_t kTwoStepDefaultWork = (int64_t)8 << 20;
 constexpr int kP2Mon = 64;  // monitors a two-step pass samples itself, at most
 
 // The monitor samples of one owned cell at step n: every monitor of the tile's list (those of this
@@ -4761,6 +4763,18 @@ Session* make_session(const std::vector<MemberSpec>& specs, uint32_t flags) {
         }
         inst0 += sp.K;
     }
+    // The launch shape a session starts with, before any set_variant or tuning: float sessions on the
+    // coefficient table with at least kTwoStepDefaultWork cell-updates per step take the two-step
+    // pass (26-row tiles, automatic x chunks), 1.5-1.7x faster than the separate kernels on the
+    // 400x400x200 grid and on 12 excitations of four port grids; below that the separate kernels
+    // stay, as fast on 200x160x80 and faster on 120x48x32.
+    int64_t work = 0;
+    bool table = true;
+    for (const Member& M : s->members) {
+        work += M.cells * M.K;
+        table &= M.coef_kind == 1;
+    }
+    if (!s->f64 && table && work >= kTwoStepDefaultWork) apply_variant(*s, 15);
     return s.release();
 }
 

@@ -239,7 +239,7 @@ class Session:
         return out
 
     def set_variant(self, variant=1):
-        """Kernel variant: 1 (default) separate E and H kernels issuing the next x plane's loads early,
+        """Kernel variant: 1 separate E and H kernels issuing the next x plane's loads early,
         0 the same one plane at a time, 2 and 3 the fused E+H pass with 6- or 14-row tiles (two field
         buffers per instance), 4 variant 1 with nontemporal field stores, 5 and 6 variants 2 and 3
         with deeper load pipelining, 7 and 8 the two-step pass (steps n and n + 1 of a 12- or 4-row x
@@ -247,7 +247,8 @@ class Session:
         with the x planes skewed by three, 10 variant 4 with nontemporal loads of the arrays streamed
         once, 11 and 12 the separate kernels over 2 and 4 z cells per thread (Nz a multiple of 2 or 4),
         13, 14 and 15 the two-step pass over 28 x 28, 12 x 28 and 26 x 28-cell tiles of 32-lane rows.
-        Results are identical."""
+        Results are identical.  A new session runs variant 15 (automatic x chunks) when it is float,
+        on the coefficient table, with at least 8 Mi cell-updates per step, else variant 1."""
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def set_column_order(self, order):

@@ -188,3 +188,60 @@ def test_two_step_large_grid_440_steps(hip_ready, slab_440_oracle, variant):
         assert np.abs(got).max() > 0, k
     for k in ref["fields"]:
         np.testing.assert_array_equal(f[k], ref["fields"][k], err_msg=k)
+
+
+def test_large_sessions_start_on_the_two_step_pass(hip_ready):
+    """A float session on the coefficient table that advances at least 8 Mi cell-updates per step
+    starts on the two-step pass: it holds the second field buffer of the pair, and its probes, DFT
+    planes and fields equal those of the separate kernels (variant 1) bit for bit.  Below that size,
+    and on the raw coefficient arrays, a session starts on the separate kernels (one buffer)."""
+    import copy
+
+    print("[case] starting shape of a 256x256x128 session", flush=True)
+    model = workloads.dipole_slab(Nt=30, nx=256, ny=256, nz=128, monitors=True)
+    c, probes, monitors, _, Nx, Ny, Nz, Nt = model.pack(gpu=True)
+    cells = Nx * Ny * Nz
+    assert cells >= 8 << 20
+    out = []
+    for variant in (None, 1):
+        pr, mo = copy.deepcopy(probes), copy.deepcopy(monitors)
+        with Session(c, Nx, Ny, Nz) as s:
+            if variant is not None:
+                s.set_variant(variant)
+            s.set_sources(pr, mo, Nt)
+            s.run(Nt)
+            s.synchronize()
+            s.read()
+            out.append((pr, mo, s.fields(("ez", "hy")), s.memory_info()["device_bytes"]))
+    (p0, m0, f0, b0), (p1, m1, f1, b1) = out
+    assert b0 >= b1 + 18 * cells * 4
+    for a, b in zip(p0, p1):
+        np.testing.assert_array_equal(a["values"], b["values"])
+    for a, b in zip(m0, m1):
+        np.testing.assert_array_equal(a["values"], b["values"])
+    for k in f0:
+        np.testing.assert_array_equal(f0[k], f1[k], err_msg=k)
+    assert np.abs(f0["ez"]).max() > 0
+    held = []
+    for flags, variant in ((_lib.COEF_RAW, None), (_lib.COEF_RAW, 1)):
+        with Session(c, Nx, Ny, Nz, flags=flags) as s:
+            if variant is not None:
+                s.set_variant(variant)
+            s.set_sources(copy.deepcopy(probes), [], Nt)
+            s.run(2)
+            s.synchronize()
+            held.append(s.memory_info()["device_bytes"])
+    assert held[0] == held[1]
+    print("[case] starting shape of a 200x160x80 session", flush=True)
+    small = workloads.dipole_slab(Nt=4, nx=200, ny=160, nz=80, monitors=False)
+    c, probes, _, _, Nx, Ny, Nz, Nt = small.pack(gpu=True)
+    held = []
+    for variant in (None, 1):
+        with Session(c, Nx, Ny, Nz) as s:
+            if variant is not None:
+                s.set_variant(variant)
+            s.set_sources(copy.deepcopy(probes), [], Nt)
+            s.run(2)
+            s.synchronize()
+            held.append(s.memory_info()["device_bytes"])
+    assert held[0] == held[1]
