@@ -85,6 +85,9 @@ MEASURED_TRAFFIC.update({
     # 26 x 28-cell tiles of 32-lane rows (variant 15), x chunks of 67 and 50 planes
     ("dipole_slab", "table", "k_step2", (15, 256, 67)): 3946000000,
     ("dipole_slab", "table", "k_step2", (15, 256, 50)): 3911700000,
+    # the automatic chunking of the two-step pass (768 workgroups aimed at: 6 chunks of 128 tiles)
+    # is the 67-plane shape on this grid
+    ("dipole_slab", "table", "k_step2", (15, 256, 0)): 3946000000,
     # 28 x 28-cell tiles of 32-lane rows (variant 13), x chunks of 25 planes
     ("dipole_slab", "table", "k_step2", (13, 256, 25)): 4097500000,
     ("dipole_slab", "table", "k_step2", (13, 256, 0)): 4097500000,

@@ -215,7 +215,7 @@ int rfdtd_plan_checkpoints(rfdtd_session* s, int32_t Nt, int32_t n, const int32_
  * moved as 8- or 16-byte vectors (Nz a multiple of 2 or 4, else RFDTD_EINVAL); 13, 14 and 15 the
  * two-step pass over tiles of 32-lane rows, 28 x 28, 12 x 28 and 26 x 28 cells.  Results are
  * identical; the choice only changes speed (the autotuner picks it).  A session starts on variant
- * 15 with automatic x chunks when it is float, on the coefficient table, and advances at least 2 Mi
+ * 15 with automatic x chunks when it is float, on the coefficient table, and advances at least 1.5 Mi
  * cell-updates per step (cells x excitations over all grids); otherwise on variant 1. */
 int rfdtd_set_variant(rfdtd_session* s, int32_t variant);
 

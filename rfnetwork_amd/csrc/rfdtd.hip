@@ -2409,7 +2409,7 @@ constexpr int kP2Fz = 60;   // z cells owned by a two-step tile of 64-lane rows
 // threads (two workgroups per CU); narrow grids (Nz = 64) waste less of their second z tile.
 constexpr int kP2Ty32Tall = 28, kP2Ty32Short = 12, kP2Ty32Mid = 26;
 // cell-updates per step (cells x excitations, all grids) from which a session starts on the two-step pass
-constexpr int64_t kTwoStepDefaultWork = (int64_t)2 << 20;
+constexpr int64_t kTwoStepDefaultWork = (int64_t)3 << 19;  // 1.5 Mi
 constexpr int kP2Mon = 64;  // monitors a two-step pass samples itself, at most
 
 // The monitor samples of one owned cell at step n: every monitor of the tile's list (those of this
@@ -4768,8 +4768,8 @@ Session* make_session(const std::vector<MemberSpec>& specs, uint32_t flags) {
     // The launch shape a session starts with, before any set_variant or tuning: float sessions on the
     // coefficient table with at least kTwoStepDefaultWork cell-updates per step take the two-step
     // pass (26-row tiles, automatic x chunks): 1.78x faster than the separate kernels on the
-    // 400x400x200 grid, 1.53x on 12 excitations of four port grids, 1.11x on 200x160x80; below that
-    // the separate kernels stay, faster on 120x48x32.
+    // 400x400x200 grid, 1.53x on 12 excitations of four port grids, 1.11x on 200x160x80 and on
+    // 150x120x100; below that the separate kernels stay (as fast on 128x96x96, faster on smaller grids).
     int64_t work = 0;
     bool table = true;
     for (const Member& M : s->members) {

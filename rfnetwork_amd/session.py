@@ -248,7 +248,7 @@ class Session:
         once, 11 and 12 the separate kernels over 2 and 4 z cells per thread (Nz a multiple of 2 or 4),
         13, 14 and 15 the two-step pass over 28 x 28, 12 x 28 and 26 x 28-cell tiles of 32-lane rows.
         Results are identical.  A new session runs variant 15 (automatic x chunks) when it is float,
-        on the coefficient table, with at least 2 Mi cell-updates per step, else variant 1."""
+        on the coefficient table, with at least 1.5 Mi cell-updates per step, else variant 1."""
         check(self._L.rfdtd_set_variant(self._h, int(variant)))
 
     def set_column_order(self, order):
